@@ -1,0 +1,157 @@
+/* libipc_warp_amd — MI355X (gfx950) IPC contact inner loop, C ABI.
+ *
+ * Drop-in for the contact kernels of Luke-Skycrawler/ipc_warp. Every entry
+ * point takes raw device pointers, sizes and scalars plus a hipStream_t
+ * (passed as void*, NULL = the default stream), enqueues its work on that
+ * stream and returns without synchronising, like a Warp launch.
+ *
+ * Data layout (the reference's own buffers):
+ *   x, xcs      float32 [V][3]       vertex positions, rest positions
+ *               (NewtonState.x, Soup.xcs — fem/params.py:43-50,
+ *                fem/geometry.py:165-171; x aliases Soup.x_transformed)
+ *   contacts    int32 [n][4]         XConstraint.a1a2b1b2 (contact.py:61-67);
+ *               EE (a0, a1, b0, b1), PT (p, t1, t0, t2) as point_triangle_
+ *               collision appends it (contact.py:336)
+ *   rows, cols  int32 [16 * total], vals float32 [16 * total][3][3]
+ *               Triplets (fem/fem.py:15-19), block (offset + i) * 16 + ii * 4 + jj
+ *   b           float32 [V][3]       force accumulator (the reference's `b`
+ *               stores -gradient; dynamic_contacts.py:193,231)
+ * The per-contact XConstraint.l0 is always 2 * thickness (contact.py:180,
+ * 195), so it is a scalar argument here. n_vertices is V: a contact or
+ * candidate naming a vertex outside [0, V) is treated as inactive (zero
+ * blocks, no force, no energy, TOI 1) instead of being read out of bounds.
+ *
+ * Status codes: 0 = success; IPC_EINVAL for a bad argument; a negative
+ * hipError_t value (-code) for a HIP failure; a positive value from a
+ * pair-producing entry point is the capacity it needed (grow and call again,
+ * as ContactSolverBase._grow_contact_list does, contact.py:523-537).
+ * ipc_last_error() describes the last failure on the calling thread.
+ */
+#ifndef IPC_WARP_AMD_H
+#define IPC_WARP_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IPC_OK 0
+#define IPC_EINVAL (-100000)
+
+/* ABI version of this header (bumped on any signature change). */
+int ipc_abi_version(void);
+/* Human-readable description of the last error on this thread. */
+const char* ipc_last_error(void);
+
+/* ------------------------------------------------- barrier Hessian (D8) ---
+ * Replaces the launch of contact_hessian_ee(states, soup, contacts, triplets,
+ * b, kappa), dim = n_contacts (dynamic_contacts.py:151-207, launched at
+ * dynamic_contacts.py:287 and affine_body_dynamics.py:470-476).
+ * For every contact whose fp32 edge-edge distance is < l0: the 16 blocks of
+ * the PSD-filtered 12x12 barrier Hessian go to triplet (offset + i) * 16 +
+ * ii * 4 + jj and the force is atomically added into b. Inactive contacts get
+ * zero blocks with row = col = 0. xcs supplies the rest edges of the
+ * mollifier threshold. */
+int ipc_contact_hessian_ee(const float* x, const float* xcs, int n_vertices, const int* contacts, int n_contacts, float l0,
+                           double kappa, int offset, int* rows, int* cols, float* vals, float* b, void* stream);
+
+/* ------------------------------------------------- barrier Hessian (D9) ---
+ * Replaces contact_hessian_pt(states, soup, contacts, triplets, b, offset,
+ * kappa), dim = n_contacts (dynamic_contacts.py:209-246, launched at
+ * dynamic_contacts.py:288 with offset = n_ee). */
+int ipc_contact_hessian_pt(const float* x, int n_vertices, const int* contacts, int n_contacts, float l0, double kappa, int offset,
+                           int* rows, int* cols, float* vals, float* b, void* stream);
+
+/* ------------------------------------------------- barrier energy (D10) ---
+ * Replace contact_energy_ee / contact_energy_pt(states, soup, contacts, e,
+ * kappa) (dynamic_contacts.py:374-398, launched at :304-305). The sum over
+ * active contacts is accumulated in FP64 and ADDED to energy[0] (a device
+ * double the caller zeroes, as the reference zeroes `e`). */
+int ipc_contact_energy_ee(const float* x, const float* xcs, int n_vertices, const int* contacts, int n_contacts, float l0,
+                          double kappa, double* energy, void* stream);
+int ipc_contact_energy_pt(const float* x, int n_vertices, const int* contacts, int n_contacts, float l0, double kappa,
+                          double* energy, void* stream);
+
+/* ----------------------------------------------- contact distance (C3d) ---
+ * Replace get_contact_points / get_contact_points_pt (contact.py:148-161)
+ * over fetch_dist_v0v1 / fetch_dist_v0v1_pt (contact.py:88-145): midpoint of
+ * the two closest points (points, float [n][3]) and the fp32 distance
+ * (dists, float [n]). features (int [n], optional) receives the feature the
+ * Hessian kernels dispatch on: PT 0..6 as closest_point_triangle's type
+ * (contact.py:230-302); EE kind * 16 + slot0 * 4 + slot1 with kind 0 PP,
+ * 1 PE, 3 EE (dynamic_contacts.py:48-108; xcs may be NULL, which disables
+ * the near-parallel fallback). Any output pointer may be NULL. */
+int ipc_contact_points_ee(const float* x, const float* xcs, int n_vertices, const int* contacts, int n_contacts, float* points,
+                          float* dists, int* features, void* stream);
+int ipc_contact_points_pt(const float* x, int n_vertices, const int* contacts, int n_contacts, float* points, float* dists,
+                          int* features, void* stream);
+
+/* ------------------------------------------------ CCD time of impact (E4) ---
+ * pt_collision_time / ee_collision_time (ccd/ccd.py:304-357) over a list of
+ * candidate vertex quadruples (int [n][4]: EE a0, a1, b0, b1; PT p, t0, t1,
+ * t2, the argument order of edge_edge_toi / point_triangle_toi,
+ * ccd/ccd.py:360-427) between positions x0 (t = 0) and x1 (t = 1).
+ * toi (double [n], optional) receives each candidate's TOI (1.0 = none).
+ * toi_min (optional) is a device uint64 holding the bit pattern of a
+ * non-negative double; every TOI < 1 is folded in with atomicMin, as the
+ * reference's wp.atomic_min(toi, 0, t). Initialise it to the bits of 1.0. */
+int ipc_collision_time_ee(const float* x0, const float* x1, int n_vertices, const int* quads, int n, double* toi,
+                          unsigned long long* toi_min, void* stream);
+int ipc_collision_time_pt(const float* x0, const float* x1, int n_vertices, const int* quads, int n, double* toi,
+                          unsigned long long* toi_min, void* stream);
+
+/* ===================================================== contact context ===
+ * The device state ContactSolverBase owns (contact.py:444-521): topology
+ * (copied at creation from device or host pointers: edges int32 [E][2],
+ * triangles int32 [F][3], body int32 [V]), the BVHs and the CCD buffers.
+ * One context per mesh soup; calls on one context must not overlap. */
+typedef struct ipc_ctx ipc_ctx;
+int ipc_ctx_create(ipc_ctx** ctx, int n_vertices, const int* edges, int n_edges, const int* triangles,
+                   int n_triangles, const int* body);
+int ipc_ctx_destroy(ipc_ctx* ctx);
+
+/* ------------------------------------------------------ DCD (B1-B5) ---
+ * Replace edge_edge_collision / point_triangle_collision and the BVH
+ * updates of detect_collision (contact.py:199-336,672-725). A BVH over the
+ * current positions x is built, every edge (EE) or every vertex (PT) queries
+ * it with the reference's inflation (edge boxes +- (thickness + buffer);
+ * vertex boxes +- 2 (thickness + buffer) against triangle bounds), and pairs
+ * passing the reference's filters (EE: i < j, no shared vertex; both: not
+ * the same body when exclude_same_body) and its fp32 distance test
+ * (< 2 thickness) are appended: contacts int32 [capacity][4] as
+ * XConstraint.a1a2b1b2 (EE (a1, a2, b1, b2), PT (p, t[1], t[0], t[2])),
+ * e0e1 int32 [capacity][2] (optional) as XConstraint.e0e1. *count (device
+ * int) receives the total number found; entries beyond capacity are not
+ * written (read the count, grow, call again — _grow_contact_list,
+ * contact.py:523-537). Order within the list is unspecified. */
+int ipc_detect_collision_ee(ipc_ctx* ctx, const float* x, float thickness, float buffer, int exclude_same_body,
+                            int* contacts, int* e0e1, int capacity, int* count, void* stream);
+int ipc_detect_collision_pt(ipc_ctx* ctx, const float* x, float thickness, float buffer, int exclude_same_body,
+                            int* contacts, int* e0e1, int capacity, int* count, void* stream);
+
+/* --------------------------------------------- CCD candidates (E1-E3) ---
+ * The candidate sets of edge_edge_toi / point_triangle_toi
+ * (ccd/ccd.py:166-210,360-427): swept boxes over x0 and x1 padded by
+ * `padding` (1e-7 in the reference), filters as above without a distance
+ * test. quads int32 [capacity][4] (EE a0, a1, b0, b1; PT p, t0, t1, t2),
+ * ids int32 [capacity][2] (optional, primitive ids), count as above. */
+int ipc_ccd_candidates_ee(ipc_ctx* ctx, const float* x0, const float* x1, float padding, int exclude_same_body,
+                          int* quads, int* ids, int capacity, int* count, void* stream);
+int ipc_ccd_candidates_pt(ipc_ctx* ctx, const float* x0, const float* x1, float padding, int exclude_same_body,
+                          int* quads, int* ids, int capacity, int* count, void* stream);
+
+/* ------------------------------------------- CCD step bound (E1-E8) ---
+ * Replaces collision_free_step(dx) (contact.py:539-585) up to its final
+ * `0.9 * toi if toi < 1 else 1`: x1 = x - dx, EE and PT candidates, FP64
+ * time of impact per candidate and their minimum. Synchronises `stream` once
+ * and writes the raw minimum TOI (1.0 = no impact) to *toi (host double).
+ * Candidate buffers belong to the context and grow as needed. */
+int ipc_collision_free_step(ipc_ctx* ctx, const float* x, const float* dx, float padding, int exclude_same_body,
+                            double* toi, void* stream);
+/* Candidate counts (EE, PT) of the context's last collision_free_step. */
+int ipc_ctx_last_ccd_counts(ipc_ctx* ctx, int* n_ee, int* n_pt);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IPC_WARP_AMD_H */

@@ -1,0 +1,67 @@
+"""ctypes binding of libipc_warp_amd.so (the C ABI in include/ipc_warp_amd.h).
+
+The library must have been built in-tree (``python -m ipc_warp_amd.build``);
+if it is missing this module raises instead of falling back to anything.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+from .build import LIB_PATH
+
+_lib = None
+
+_c_int, _c_float, _c_double, _vp = ctypes.c_int, ctypes.c_float, ctypes.c_double, ctypes.c_void_p
+
+_SIGNATURES = {
+    "ipc_abi_version": ([], _c_int),
+    "ipc_last_error": ([], ctypes.c_char_p),
+    "ipc_contact_hessian_ee": ([_vp, _vp, _c_int, _vp, _c_int, _c_float, _c_double, _c_int, _vp, _vp, _vp, _vp, _vp], _c_int),
+    "ipc_contact_hessian_pt": ([_vp, _c_int, _vp, _c_int, _c_float, _c_double, _c_int, _vp, _vp, _vp, _vp, _vp], _c_int),
+    "ipc_contact_energy_ee": ([_vp, _vp, _c_int, _vp, _c_int, _c_float, _c_double, _vp, _vp], _c_int),
+    "ipc_contact_energy_pt": ([_vp, _c_int, _vp, _c_int, _c_float, _c_double, _vp, _vp], _c_int),
+    "ipc_contact_points_ee": ([_vp, _vp, _c_int, _vp, _c_int, _vp, _vp, _vp, _vp], _c_int),
+    "ipc_contact_points_pt": ([_vp, _c_int, _vp, _c_int, _vp, _vp, _vp, _vp], _c_int),
+    "ipc_collision_time_ee": ([_vp, _vp, _c_int, _vp, _c_int, _vp, _vp, _vp], _c_int),
+    "ipc_collision_time_pt": ([_vp, _vp, _c_int, _vp, _c_int, _vp, _vp, _vp], _c_int),
+    "ipc_ctx_create": ([ctypes.POINTER(_vp), _c_int, _vp, _c_int, _vp, _c_int, _vp], _c_int),
+    "ipc_ctx_destroy": ([_vp], _c_int),
+    "ipc_detect_collision_ee": ([_vp, _vp, _c_float, _c_float, _c_int, _vp, _vp, _c_int, _vp, _vp], _c_int),
+    "ipc_detect_collision_pt": ([_vp, _vp, _c_float, _c_float, _c_int, _vp, _vp, _c_int, _vp, _vp], _c_int),
+    "ipc_ccd_candidates_ee": ([_vp, _vp, _vp, _c_float, _c_int, _vp, _vp, _c_int, _vp, _vp], _c_int),
+    "ipc_ccd_candidates_pt": ([_vp, _vp, _vp, _c_float, _c_int, _vp, _vp, _c_int, _vp, _vp], _c_int),
+    "ipc_collision_free_step": ([_vp, _vp, _vp, _c_float, _c_int, ctypes.POINTER(_c_double), _vp], _c_int),
+    "ipc_ctx_last_ccd_counts": ([_vp, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int)], _c_int),
+}
+
+EXPORTED = tuple(_SIGNATURES)
+
+
+class IPCError(RuntimeError):
+    """A libipc_warp_amd entry point returned a failure status."""
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise IPCError(
+                f"{LIB_PATH} is not built; run `python -m ipc_warp_amd.build` (hipcc, gfx950). "
+                "There is no fallback implementation."
+            )
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (argtypes, restype) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = argtypes
+            fn.restype = restype
+        _lib = L
+    return _lib
+
+
+def check(status: int, what: str) -> int:
+    """Raise on a negative status; return a positive one (a required capacity)."""
+    if status < 0:
+        msg = lib().ipc_last_error().decode(errors="replace")
+        raise IPCError(f"{what} failed with status {status}: {msg}")
+    return status

@@ -1,0 +1,241 @@
+"""Host-side mirror of the reference's contact solver over the HIP library.
+
+``ContactSolverBase`` keeps the reference's method names, argument meaning
+and error behaviour (contact.py:444-750, dynamic_contacts.py:249-308) so a
+time integrator calls it the way RodBCBase / RodComplexBC do:
+
+* ``detect_collision()`` fills ``contacts_new`` / ``contacts_pt`` and sets
+  ``n_contacts`` / ``n_contacts_pt``, growing the lists to the next power of
+  two on overflow (contact.py:523-537,687-725);
+* ``collision_free_step(dx)`` returns ``0.9 * toi`` if ``toi < 1`` else 1
+  (contact.py:539-585);
+* ``compute_collision_hessian(b)`` launches contact_hessian_ee then
+  contact_hessian_pt at offset n_ee into freshly zeroed triplets and adds
+  the force into ``b`` (dynamic_contacts.py:280-288);
+* ``compute_collision_energy()`` re-runs DCD and returns h^2 times the
+  barrier energy (dynamic_contacts.py:299-308).
+
+Every device buffer is a torch tensor on the GPU; every computation runs in
+libipc_warp_amd. Module globals mirror contact.py:14-22.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import torch
+
+from . import ops
+from ._lib import IPCError, check, lib
+
+_thickness = 0.01
+contact_volume = 10000
+buffer = _thickness
+disable_self_collision = False
+verbose = False
+CCD_PADDING = 1e-7
+
+
+@dataclass
+class Contacts:
+    """A contact list (contact.py:75-80) as SoA device tensors.
+
+    ``list`` holds XConstraint.a1a2b1b2 (int32 [capacity, 4]) and ``e0e1``
+    XConstraint.e0e1 (int32 [capacity, 2]); ``cnt`` is the device counter.
+    Every contact's l0 is 2 * thickness (contact.py:180).
+    """
+
+    list: torch.Tensor
+    e0e1: torch.Tensor
+    cnt: torch.Tensor
+    capacity: int
+
+
+@dataclass
+class Triplets:
+    """COO blocks (fem/fem.py:15-19): rows, cols int32 [n]; vals float32 [n, 3, 3]."""
+
+    rows: torch.Tensor
+    cols: torch.Tensor
+    vals: torch.Tensor
+
+
+def _new_contacts(capacity: int, device) -> Contacts:
+    return Contacts(
+        list=torch.zeros((capacity, 4), dtype=torch.int32, device=device),
+        e0e1=torch.zeros((capacity, 2), dtype=torch.int32, device=device),
+        cnt=torch.zeros(1, dtype=torch.int32, device=device),
+        capacity=capacity,
+    )
+
+
+class ContactSolverBase:
+    """Contact detection, barrier assembly and CCD over one mesh soup.
+
+    x: float32 [V,3] device tensor of current positions (the reference's
+    ``soup.x_transformed``, aliased to ``states.x``); xcs: rest positions;
+    edges int32 [E,2]; triangles int32 [F,3]; body int32 [V].
+    """
+
+    def __init__(self, x, xcs, edges, triangles, body, thickness=None, buffer_=None, kappa=1e9, h=1.0,
+                 disable_self_collision_=None, capacity=None):
+        dev = x.device
+        self.device = dev
+        self.x = x
+        self.xcs = xcs
+        self.edges = edges.to(dev, torch.int32).contiguous().reshape(-1, 2)
+        self.triangles = triangles.to(dev, torch.int32).contiguous().reshape(-1, 3)
+        self.body = body.to(dev, torch.int32).contiguous()
+        self.thickness = float(_thickness if thickness is None else thickness)
+        self.buffer = float(buffer if buffer_ is None else buffer_)
+        self.contact_stiffness = float(kappa)
+        self.h = float(h)
+        self.disable_self_collision = bool(disable_self_collision if disable_self_collision_ is None
+                                           else disable_self_collision_)
+        self.has_triangles = self.triangles.shape[0] > 0
+        cap = contact_volume if capacity is None else int(capacity)
+        self.contacts_new = _new_contacts(cap, dev)
+        self.contacts_pt = _new_contacts(cap, dev)
+        self.n_contacts = 0
+        self.n_contacts_pt = 0
+        self.collision_triplets = None
+        handle = ctypes.c_void_p()
+        check(lib().ipc_ctx_create(ctypes.byref(handle), int(x.shape[0]), self.edges.data_ptr(),
+                                   int(self.edges.shape[0]), self.triangles.data_ptr(), int(self.triangles.shape[0]),
+                                   self.body.data_ptr()), "ctx_create")
+        self._ctx = handle
+
+    def __del__(self):
+        ctx = getattr(self, "_ctx", None)
+        if ctx is not None and ctx.value:
+            try:
+                lib().ipc_ctx_destroy(ctx)
+            except Exception:
+                pass
+            self._ctx = None
+
+    # ------------------------------------------------------------- helpers ----
+    @property
+    def l0(self) -> float:
+        import numpy as np
+
+        return float(np.float32(self.thickness) * np.float32(2.0))
+
+    def _stream(self):
+        return ops.stream_handle(self.device)
+
+    def _grow_contact_list(self, contacts: Contacts, required: int) -> Contacts:
+        """contact.py:523-537: regrow to the next power of two >= required."""
+        capacity = 1 << (int(required) - 1).bit_length()
+        grown = _new_contacts(capacity, self.device)
+        if verbose:
+            print(f"contact buffer grown to {capacity}")
+        return grown
+
+    def _dcd(self, fn, contacts: Contacts, what: str):
+        while True:
+            check(fn(self._ctx, self.x.data_ptr(), float(self.thickness), float(self.buffer),
+                     int(self.disable_self_collision), contacts.list.data_ptr(), contacts.e0e1.data_ptr(),
+                     int(contacts.capacity), contacts.cnt.data_ptr(), self._stream()), what)
+            n = int(contacts.cnt.item())
+            if n <= contacts.capacity:
+                return contacts, n
+            contacts = self._grow_contact_list(contacts, n)
+
+    # ---------------------------------------------------------- reference API ----
+    def compute_V(self, ret=True):
+        # x_transformed is tied to the solver's x (contact.py:683-685).
+        return None
+
+    def detect_collision(self):
+        """contact.py:687-725."""
+        self.compute_V(ret=False)
+        self.contacts_new, self.n_contacts = self._dcd(lib().ipc_detect_collision_ee, self.contacts_new,
+                                                       "detect_collision_ee")
+        if verbose:
+            print(f"n ee contacts = {self.n_contacts}")
+        if self.has_triangles:
+            self.contacts_pt, self.n_contacts_pt = self._dcd(lib().ipc_detect_collision_pt, self.contacts_pt,
+                                                             "detect_collision_pt")
+        else:
+            self.n_contacts_pt = 0
+        if verbose:
+            print(f"n pt contacts = {self.n_contacts_pt}")
+
+    def collision_free_step(self, dx) -> float:
+        """Zero-thickness CCD upper bound for x -= alpha * dx (contact.py:539-585)."""
+        toi = self.raw_toi(dx)
+        return 0.9 * toi if toi < 1.0 else 1.0
+
+    def raw_toi(self, dx, x=None) -> float:
+        x = self.x if x is None else x
+        if dx.shape != x.shape or dx.dtype != torch.float32 or not dx.is_contiguous():
+            raise ValueError("dx must be a contiguous float32 tensor shaped like x")
+        out = ctypes.c_double(1.0)
+        check(lib().ipc_collision_free_step(self._ctx, x.data_ptr(), dx.data_ptr(), float(CCD_PADDING),
+                                            int(self.disable_self_collision), ctypes.byref(out), self._stream()),
+              "collision_free_step")
+        return float(out.value)
+
+    def ccd_candidates(self, kind: str, x0, x1, padding=CCD_PADDING, capacity=1 << 12):
+        """Candidate quadruples and primitive ids of edge_edge_toi (kind 'ee') or
+        point_triangle_toi ('pt') between x0 and x1 (ccd/ccd.py:360-427)."""
+        fn = lib().ipc_ccd_candidates_ee if kind == "ee" else lib().ipc_ccd_candidates_pt
+        for t, nm in ((x0, "x0"), (x1, "x1")):
+            if t.shape != self.x.shape or t.dtype != torch.float32 or not t.is_contiguous():
+                raise ValueError(f"{nm} must be a contiguous float32 tensor shaped like x")
+        while True:
+            quads = torch.zeros((capacity, 4), dtype=torch.int32, device=self.device)
+            ids = torch.zeros((capacity, 2), dtype=torch.int32, device=self.device)
+            cnt = torch.zeros(1, dtype=torch.int32, device=self.device)
+            check(fn(self._ctx, x0.data_ptr(), x1.data_ptr(), float(padding), int(self.disable_self_collision),
+                     quads.data_ptr(), ids.data_ptr(), int(capacity), cnt.data_ptr(), self._stream()),
+                  f"ccd_candidates_{kind}")
+            n = int(cnt.item())
+            if n <= capacity:
+                return quads[:n], ids[:n]
+            capacity = 1 << (n - 1).bit_length()
+
+    def last_ccd_counts(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        check(lib().ipc_ctx_last_ccd_counts(self._ctx, ctypes.byref(a), ctypes.byref(b)), "ctx_last_ccd_counts")
+        return a.value, b.value
+
+    def line_search_upper_bound(self, dx) -> float:
+        return self.collision_free_step(dx)
+
+    def compute_collision_hessian(self, b: torch.Tensor) -> Triplets:
+        """Contact part of RodComplexBC.compute_A (dynamic_contacts.py:280-288).
+
+        Allocates zeroed triplets for (n_ee + n_pt) * 16 blocks, launches the
+        EE kernel then the PT kernel at offset n_ee, and adds forces into b.
+        """
+        n_ee, n_pt = self.n_contacts, self.n_contacts_pt
+        nnz = (n_ee + n_pt) * 16
+        trip = Triplets(
+            rows=torch.zeros(nnz, dtype=torch.int32, device=self.device),
+            cols=torch.zeros(nnz, dtype=torch.int32, device=self.device),
+            vals=torch.zeros((nnz, 3, 3), dtype=torch.float32, device=self.device),
+        )
+        ops.contact_hessian_ee(self.x, self.xcs, self.contacts_new.list, n_ee, self.l0, self.contact_stiffness,
+                               trip.rows, trip.cols, trip.vals, b, 0)
+        ops.contact_hessian_pt(self.x, self.contacts_pt.list, n_pt, self.l0, self.contact_stiffness, trip.rows,
+                               trip.cols, trip.vals, b, n_ee)
+        self.collision_triplets = trip
+        return trip
+
+    def collision_energy_raw(self) -> float:
+        """Barrier energy of the current contact lists (no DCD, no h^2)."""
+        e = torch.zeros(1, dtype=torch.float64, device=self.device)
+        ops.contact_energy_ee(self.x, self.xcs, self.contacts_new.list, self.n_contacts, self.l0,
+                              self.contact_stiffness, e)
+        ops.contact_energy_pt(self.x, self.contacts_pt.list, self.n_contacts_pt, self.l0, self.contact_stiffness, e)
+        return float(e.item())
+
+    def compute_collision_energy(self) -> float:
+        """dynamic_contacts.py:299-308: DCD, then h^2 * sum of barrier energies."""
+        self.detect_collision()
+        return self.collision_energy_raw() * self.h * self.h
+
+
+__all__ = ["ContactSolverBase", "Contacts", "Triplets", "IPCError"]

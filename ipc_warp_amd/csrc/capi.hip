@@ -1,0 +1,143 @@
+// extern "C" entry points of libipc_warp_amd (declared in include/ipc_warp_amd.h).
+// Validation of sizes and pointers happens here on the host, before any
+// launch; nothing throws across the ABI.
+#include <cstdio>
+#include <cstring>
+
+#include "../../include/ipc_warp_amd.h"
+#include "ipc_kernels.hpp"
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* what) {
+  ipcd::set_error(what);
+  return code;
+}
+
+int hip_status(hipError_t e, const char* where) {
+  if (e == hipSuccess) return IPC_OK;
+  std::snprintf(g_err, sizeof(g_err), "%s: %s", where, hipGetErrorString(e));
+  return -int(e);
+}
+
+bool aligned16(const void* p) { return (reinterpret_cast<unsigned long long>(p) & 15ull) == 0; }
+
+}  // namespace
+
+void ipcd::set_error(const char* msg) { std::snprintf(g_err, sizeof(g_err), "%s", msg); }
+
+extern "C" {
+
+int ipc_abi_version(void) { return 1; }
+
+const char* ipc_last_error(void) { return g_err; }
+
+static int check_hessian_args(const float* x, int nv, const int* contacts, int n, float l0, int offset, int* rows,
+                              int* cols, float* vals, float* b) {
+  if (n < 0 || offset < 0 || nv < 0) return fail(IPC_EINVAL, "contact_hessian: negative n_vertices, n_contacts or offset");
+  if ((long long)(offset) + n > (1ll << 27)) return fail(IPC_EINVAL, "contact_hessian: offset + n_contacts exceeds 2^27");
+  if (n == 0) return IPC_OK;
+  if (!x || !contacts || !rows || !cols || !vals || !b) return fail(IPC_EINVAL, "contact_hessian: null pointer");
+  if (!(l0 > 0.0f)) return fail(IPC_EINVAL, "contact_hessian: l0 must be > 0");
+  if (!aligned16(contacts) || !aligned16(rows) || !aligned16(cols) || !aligned16(vals))
+    return fail(IPC_EINVAL, "contact_hessian: contacts/rows/cols/vals must be 16-byte aligned");
+  return IPC_OK;
+}
+
+int ipc_contact_hessian_ee(const float* x, const float* xcs, int n_vertices, const int* contacts, int n_contacts,
+                           float l0, double kappa, int offset, int* rows, int* cols, float* vals, float* b,
+                           void* stream) {
+  int st = check_hessian_args(x, n_vertices, contacts, n_contacts, l0, offset, rows, cols, vals, b);
+  if (st != IPC_OK || n_contacts == 0) return st;
+  if (!xcs) return fail(IPC_EINVAL, "contact_hessian_ee: xcs (rest positions) is required");
+  return hip_status(ipcd::launch_contact_hessian(true, x, xcs, n_vertices, contacts, n_contacts, l0, kappa, offset, rows, cols,
+                                                 vals, b, static_cast<hipStream_t>(stream)),
+                    "contact_hessian_ee");
+}
+
+int ipc_contact_hessian_pt(const float* x, int n_vertices, const int* contacts, int n_contacts, float l0,
+                           double kappa, int offset, int* rows, int* cols, float* vals, float* b, void* stream) {
+  int st = check_hessian_args(x, n_vertices, contacts, n_contacts, l0, offset, rows, cols, vals, b);
+  if (st != IPC_OK || n_contacts == 0) return st;
+  return hip_status(ipcd::launch_contact_hessian(false, x, nullptr, n_vertices, contacts, n_contacts, l0, kappa, offset, rows,
+                                                 cols, vals, b, static_cast<hipStream_t>(stream)),
+                    "contact_hessian_pt");
+}
+
+static int check_energy_args(const float* x, int nv, const int* contacts, int n, float l0, double* energy) {
+  if (n < 0 || nv < 0) return fail(IPC_EINVAL, "contact_energy: negative n_vertices or n_contacts");
+  if (n == 0) return IPC_OK;
+  if (!x || !contacts || !energy) return fail(IPC_EINVAL, "contact_energy: null pointer");
+  if (!(l0 > 0.0f)) return fail(IPC_EINVAL, "contact_energy: l0 must be > 0");
+  if (!aligned16(contacts)) return fail(IPC_EINVAL, "contact_energy: contacts must be 16-byte aligned");
+  return IPC_OK;
+}
+
+int ipc_contact_energy_ee(const float* x, const float* xcs, int n_vertices, const int* contacts, int n_contacts,
+                          float l0, double kappa, double* energy, void* stream) {
+  int st = check_energy_args(x, n_vertices, contacts, n_contacts, l0, energy);
+  if (st != IPC_OK || n_contacts == 0) return st;
+  if (!xcs) return fail(IPC_EINVAL, "contact_energy_ee: xcs (rest positions) is required");
+  return hip_status(ipcd::launch_contact_energy(true, x, xcs, n_vertices, contacts, n_contacts, l0, kappa, energy,
+                                                static_cast<hipStream_t>(stream)),
+                    "contact_energy_ee");
+}
+
+int ipc_contact_energy_pt(const float* x, int n_vertices, const int* contacts, int n_contacts, float l0,
+                          double kappa, double* energy, void* stream) {
+  int st = check_energy_args(x, n_vertices, contacts, n_contacts, l0, energy);
+  if (st != IPC_OK || n_contacts == 0) return st;
+  return hip_status(ipcd::launch_contact_energy(false, x, nullptr, n_vertices, contacts, n_contacts, l0, kappa, energy,
+                                                static_cast<hipStream_t>(stream)),
+                    "contact_energy_pt");
+}
+
+static int check_points_args(const float* x, int nv, const int* contacts, int n) {
+  if (n < 0 || nv < 0) return fail(IPC_EINVAL, "contact_points: negative n_vertices or n_contacts");
+  if (n == 0) return IPC_OK;
+  if (!x || !contacts) return fail(IPC_EINVAL, "contact_points: null pointer");
+  if (!aligned16(contacts)) return fail(IPC_EINVAL, "contact_points: contacts must be 16-byte aligned");
+  return IPC_OK;
+}
+
+int ipc_contact_points_ee(const float* x, const float* xcs, int n_vertices, const int* contacts, int n_contacts,
+                          float* points, float* dists, int* features, void* stream) {
+  int st = check_points_args(x, n_vertices, contacts, n_contacts);
+  if (st != IPC_OK || n_contacts == 0) return st;
+  return hip_status(ipcd::launch_contact_points(true, x, xcs, n_vertices, contacts, n_contacts, points, dists, features,
+                                                static_cast<hipStream_t>(stream)),
+                    "contact_points_ee");
+}
+
+int ipc_contact_points_pt(const float* x, int n_vertices, const int* contacts, int n_contacts, float* points,
+                          float* dists, int* features, void* stream) {
+  int st = check_points_args(x, n_vertices, contacts, n_contacts);
+  if (st != IPC_OK || n_contacts == 0) return st;
+  return hip_status(ipcd::launch_contact_points(false, x, nullptr, n_vertices, contacts, n_contacts, points, dists, features,
+                                                static_cast<hipStream_t>(stream)),
+                    "contact_points_pt");
+}
+
+static int collision_time(bool ee, const float* x0, const float* x1, int nv, const int* quads, int n, double* toi,
+                          unsigned long long* toi_min, void* stream) {
+  if (n < 0 || nv < 0) return fail(IPC_EINVAL, "collision_time: negative n_vertices or n");
+  if (n == 0) return IPC_OK;
+  if (!x0 || !x1 || !quads) return fail(IPC_EINVAL, "collision_time: null pointer");
+  if (!aligned16(quads)) return fail(IPC_EINVAL, "collision_time: quads must be 16-byte aligned");
+  return hip_status(ipcd::launch_collision_time(ee, x0, x1, nv, quads, n, toi, toi_min, static_cast<hipStream_t>(stream)),
+                    ee ? "collision_time_ee" : "collision_time_pt");
+}
+
+int ipc_collision_time_ee(const float* x0, const float* x1, int n_vertices, const int* quads, int n, double* toi,
+                          unsigned long long* toi_min, void* stream) {
+  return collision_time(true, x0, x1, n_vertices, quads, n, toi, toi_min, stream);
+}
+
+int ipc_collision_time_pt(const float* x0, const float* x1, int n_vertices, const int* quads, int n, double* toi,
+                          unsigned long long* toi_min, void* stream) {
+  return collision_time(false, x0, x1, n_vertices, quads, n, toi, toi_min, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,361 @@
+// Barrier gradient / PSD Hessian / energy kernels and CCD time-of-impact
+// kernels of libipc_warp_amd (gfx950, wave64).
+//
+// One lane owns one contact. Contacts are a stream of int4 vertex quadruples
+// (16 B, one 128-bit load per lane, coalesced across the wave). Vertex
+// positions are gathered from the float32 [V, 3] array (L2/MALL resident at
+// every configuration this library targets).
+#include "ipc_device.hpp"
+#include "ipc_kernels.hpp"
+
+namespace ipcd {
+
+// --------------------------------------------------------------- per pair ----
+// Indices outside [0, n_vertices) make a contact inactive instead of faulting.
+__device__ __forceinline__ bool valid4(int4 c, int nv) {
+  return (unsigned)c.x < (unsigned)nv && (unsigned)c.y < (unsigned)nv && (unsigned)c.z < (unsigned)nv &&
+         (unsigned)c.w < (unsigned)nv;
+}
+
+// contact_hessian_ee (dynamic_contacts.py:151-207). Returns false for an
+// inactive contact (fp32 distance >= l0). On an active contact, fills the
+// compressed Hessian and the force (-gradient) contribution of each slot.
+__device__ __forceinline__ bool pair_ee(const float* __restrict__ x, const float* __restrict__ xcs, int4 c, float l0,
+                                        double kappa, bool want_hessian, CompressedHessian& C, float force[12],
+                                        double* energy) {
+  V3<float> xf[4] = {ldx(x, c.x), ldx(x, c.y), ldx(x, c.z), ldx(x, c.w)};
+  EdgeClosest ec = closest_point_edge_edge(xf[0], xf[1], xf[2], xf[3], 1.0e-6f);
+  if (!(ec.dist < l0)) return false;
+  V3<float> ra = ldx(xcs, c.y) - ldx(xcs, c.x), rb = ldx(xcs, c.w) - ldx(xcs, c.z);
+  float eps_x = 1.0e-3f * dot(ra, ra) * dot(rb, rb);
+  V3<float> cr = cross(xf[1] - xf[0], xf[3] - xf[2]);
+  bool near_parallel = eps_x > 0.0f && dot(cr, cr) < eps_x;
+  Feature f = ee_feature(ec, cr, eps_x);
+  V3<double> xo = to_d(xf[0]);
+  V3<double> xr[4] = {mk(0.0, 0.0, 0.0), to_d(xf[1]) - xo, to_d(xf[2]) - xo, to_d(xf[3]) - xo};
+  FeatureGeom g = feature_geom(f, xr);
+  double d2 = want_hessian ? nsd_compressed(g, !near_parallel, C) : dot(g.r, g.r);
+  Barrier B = barrier(d2, double(l0) * double(l0), kappa);
+  double m = 1.0;
+  double gm[12];
+  if (near_parallel) {
+    // IPC edge-edge mollifier, m(c) = (c/eps_x)(2 - c/eps_x), c = |ea x eb|^2.
+    V3<double> ea = xr[1], eb = xr[3] - xr[2];
+    V3<double> n = cross(ea, eb);
+    double ex = double(eps_x);
+    double q = dot(n, n) / ex;
+    m = q * (2.0 - q);
+    double dm = 2.0 / ex * (1.0 - q);
+    V3<double> ga = cross(eb, n) * (2.0 * dm), gb = cross(n, ea) * (2.0 * dm);
+    gm[0] = -ga.x; gm[1] = -ga.y; gm[2] = -ga.z;
+    gm[3] = ga.x; gm[4] = ga.y; gm[5] = ga.z;
+    gm[6] = -gb.x; gm[7] = -gb.y; gm[8] = -gb.z;
+    gm[9] = gb.x; gm[10] = gb.y; gm[11] = gb.z;
+  }
+  if (energy) *energy = m * B.b;
+  if (!want_hessian) return true;
+  if (near_parallel) {
+    // Near-parallel edges keep only the Gauss-Newton term m b'' g g^T.
+    C.cn = 0.0f;
+    C.cg = float(4.0 * m * B.d2b * d2);
+  } else {
+    // Outside the mollified band m = 1 and grad m = 0, so the reference's
+    // b * H_m and PSD(sym(grad m, b' g)) terms are exactly zero.
+    C.cn = float(B.db);
+    C.cg = float(4.0 * B.d2b * d2);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    double s = -2.0 * m * B.db * g.w[i];
+    force[3 * i + 0] = float(s * g.r.x - (near_parallel ? B.b * gm[3 * i + 0] : 0.0));
+    force[3 * i + 1] = float(s * g.r.y - (near_parallel ? B.b * gm[3 * i + 1] : 0.0));
+    force[3 * i + 2] = float(s * g.r.z - (near_parallel ? B.b * gm[3 * i + 2] : 0.0));
+  }
+  return true;
+}
+
+// contact_hessian_pt (dynamic_contacts.py:209-246). The record is
+// (p, t1, t0, t2) and slots 1..3 are the triangle (a, b, c) of
+// fetch_dist_v0v1_pt (contact.py:116-145).
+__device__ __forceinline__ bool pair_pt(const float* __restrict__ x, int4 c, float l0, double kappa,
+                                        bool want_hessian, CompressedHessian& C, float force[12], double* energy) {
+  V3<float> xf[4] = {ldx(x, c.x), ldx(x, c.y), ldx(x, c.z), ldx(x, c.w)};
+  TriClosest tc = closest_point_triangle(xf[0], xf[1], xf[2], xf[3]);
+  if (!(tc.dist < l0)) return false;
+  Feature f = pt_feature(tc.type);
+  V3<double> xo = to_d(xf[0]);
+  V3<double> xr[4] = {mk(0.0, 0.0, 0.0), to_d(xf[1]) - xo, to_d(xf[2]) - xo, to_d(xf[3]) - xo};
+  FeatureGeom g = feature_geom(f, xr);
+  double d2 = want_hessian ? nsd_compressed(g, true, C) : dot(g.r, g.r);
+  Barrier B = barrier(d2, double(l0) * double(l0), kappa);
+  if (energy) *energy = B.b;
+  if (!want_hessian) return true;
+  C.cn = float(B.db);
+  C.cg = float(4.0 * B.d2b * d2);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    double s = -2.0 * B.db * g.w[i];
+    force[3 * i + 0] = float(s * g.r.x);
+    force[3 * i + 1] = float(s * g.r.y);
+    force[3 * i + 2] = float(s * g.r.z);
+  }
+  return true;
+}
+
+// ----------------------------------------------------------- hessian kernel ----
+// Writes the 16 triplets of contact i at (offset + i) * 16 + ii * 4 + jj
+// (dynamic_contacts.py:196-207,236-246); inactive contacts get zero blocks
+// with row = col = 0, which is what the reference's zero-initialised
+// triplet arrays hold for them. The force is atomically added into b.
+template <bool EE>
+__global__ void __launch_bounds__(256) contact_hessian_kernel(const float* __restrict__ x, const float* __restrict__ xcs,
+                                                              int nv, const int4* __restrict__ contacts, int n, float l0,
+                                                              double kappa, int offset, int* __restrict__ rows,
+                                                              int* __restrict__ cols, float* __restrict__ vals,
+                                                              float* __restrict__ b) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int4 c = contacts[i];
+  CompressedHessian C;
+  float force[12];
+  bool active = valid4(c, nv) && (EE ? pair_ee(x, xcs, c, l0, kappa, true, C, force, nullptr)
+                                     : pair_pt(x, c, l0, kappa, true, C, force, nullptr));
+  long long base = (long long)(offset + i) * 16;
+  int4* rows4 = reinterpret_cast<int4*>(rows + base);
+  int4* cols4 = reinterpret_cast<int4*>(cols + base);
+  const int cc[4] = {c.x, c.y, c.z, c.w};
+  if (!active) {
+    int4 z = make_int4(0, 0, 0, 0);
+#pragma unroll
+    for (int ii = 0; ii < 4; ++ii) { rows4[ii] = z; cols4[ii] = z; }
+    float4* v4 = reinterpret_cast<float4*>(vals + base * 9);
+    float4 zf = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < 36; ++k) v4[k] = zf;
+    return;
+  }
+#pragma unroll
+  for (int ii = 0; ii < 4; ++ii) {
+    rows4[ii] = make_int4(cc[ii], cc[ii], cc[ii], cc[ii]);
+    cols4[ii] = c;
+  }
+  float4* v4 = reinterpret_cast<float4*>(vals + base * 9);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    // Four consecutive 3x3 blocks = 36 floats = 9 float4 stores.
+    float blk[36];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) expand_block(C, (q * 4 + r) >> 2, (q * 4 + r) & 3, blk + 9 * r);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) v4[q * 9 + k] = make_float4(blk[4 * k], blk[4 * k + 1], blk[4 * k + 2], blk[4 * k + 3]);
+  }
+#pragma unroll
+  for (int ii = 0; ii < 4; ++ii) {
+    float* dst = b + 3 * (long long)cc[ii];
+    unsafeAtomicAdd(dst + 0, force[3 * ii + 0]);
+    unsafeAtomicAdd(dst + 1, force[3 * ii + 1]);
+    unsafeAtomicAdd(dst + 2, force[3 * ii + 2]);
+  }
+}
+
+// ------------------------------------------------------------ energy kernel ----
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// contact_energy_ee / contact_energy_pt (dynamic_contacts.py:374-398): the
+// per-contact energies are summed in FP64 per wave, per block through LDS,
+// then one atomicAdd(double) per block into energy[0].
+template <bool EE>
+__global__ void __launch_bounds__(256) contact_energy_kernel(const float* __restrict__ x,
+                                                             const float* __restrict__ xcs, int nv,
+                                                             const int4* __restrict__ contacts, int n, float l0,
+                                                             double kappa, double* __restrict__ energy) {
+  __shared__ double partial[4];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double e = 0.0;
+  if (i < n) {
+    CompressedHessian C;
+    float force[12];
+    double ei = 0.0;
+    int4 c = contacts[i];
+    bool active = valid4(c, nv) && (EE ? pair_ee(x, xcs, c, l0, kappa, false, C, force, &ei)
+                                       : pair_pt(x, c, l0, kappa, false, C, force, &ei));
+    if (active) e = ei;
+  }
+  e = wave_sum(e);
+  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) partial[wave] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < int(blockDim.x >> 6); ++w) s += partial[w];
+    if (s != 0.0) unsafeAtomicAdd(energy, s);
+  }
+}
+
+// ----------------------------------------------------- contact points kernel ----
+// get_contact_points / get_contact_points_pt (contact.py:148-161) over
+// fetch_dist_v0v1 / fetch_dist_v0v1_pt (contact.py:88-145): midpoint of the
+// two closest points, the fp32 distance, and (additionally) the feature the
+// Hessian kernels dispatch on.
+template <bool EE>
+__global__ void __launch_bounds__(256) contact_points_kernel(const float* __restrict__ x, const float* __restrict__ xcs,
+                                                             int nv, const int4* __restrict__ contacts, int n,
+                                                             float* __restrict__ points, float* __restrict__ dists,
+                                                             int* __restrict__ features) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int4 c = contacts[i];
+  if (!valid4(c, nv)) {
+    if (points) { points[3 * i] = points[3 * i + 1] = points[3 * i + 2] = __builtin_nanf(""); }
+    if (dists) dists[i] = __builtin_nanf("");
+    if (features) features[i] = -1;
+    return;
+  }
+  V3<float> x0 = ldx(x, c.x), x1 = ldx(x, c.y), x2 = ldx(x, c.z), x3 = ldx(x, c.w);
+  V3<float> v0, v1;
+  float dist;
+  int feat;
+  if (EE) {
+    EdgeClosest ec = closest_point_edge_edge(x0, x1, x2, x3, 1.0e-6f);
+    // wp.lerp(a, b, t) = a * (1 - t) + b * t
+    v0 = x0 * (1.0f - ec.s) + x1 * ec.s;
+    v1 = x2 * (1.0f - ec.t) + x3 * ec.t;
+    dist = ec.dist;
+    float eps_x = 0.0f;
+    if (xcs) {
+      V3<float> ra = ldx(xcs, c.y) - ldx(xcs, c.x), rb = ldx(xcs, c.w) - ldx(xcs, c.z);
+      eps_x = 1.0e-3f * dot(ra, ra) * dot(rb, rb);
+    }
+    Feature f = ee_feature(ec, cross(x1 - x0, x3 - x2), eps_x);
+    feat = f.kind * 16 + f.s0 * 4 + (f.s1 < 0 ? 0 : f.s1);
+  } else {
+    TriClosest tc = closest_point_triangle(x0, x1, x2, x3);
+    v0 = x0;
+    v1 = tc.wa * x1 + tc.wb * x2 + (1.0f - tc.wa - tc.wb) * x3;
+    dist = tc.dist;
+    feat = tc.type;
+  }
+  V3<float> mid = (v0 + v1) * 0.5f;
+  if (points) { points[3 * i] = mid.x; points[3 * i + 1] = mid.y; points[3 * i + 2] = mid.z; }
+  if (dists) dists[i] = dist;
+  if (features) features[i] = feat;
+}
+
+// -------------------------------------------------------------- CCD kernel ----
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// pt_collision_time / ee_collision_time (ccd/ccd.py:304-357) for a list of
+// candidate quadruples, in FP64. The number of candidates is n, or
+// min(*n_dev, n) when n_dev is given (a count produced on the device; n is
+// then the list's capacity), walked with a grid-stride loop. Writes
+// per-candidate TOI (optional) and folds TOI < 1 into *toi_min with one
+// 64-bit atomicMin per wave (TOI >= 0, so the IEEE bit pattern orders like
+// an unsigned integer).
+template <bool EE>
+__global__ void __launch_bounds__(256) collision_time_kernel(const float* __restrict__ x0, const float* __restrict__ x1,
+                                                             int nv, const int4* __restrict__ quads, int n,
+                                                             const int* __restrict__ n_dev,
+                                                             double* __restrict__ toi_out,
+                                                             unsigned long long* __restrict__ toi_min) {
+  int count = n;
+  if (n_dev) {
+    int d = *n_dev;
+    count = d < n ? d : n;
+  }
+  double tmin = 1.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+    int4 q = quads[i];
+    double t = 1.0;
+    if (valid4(q, nv)) {
+      const int id[4] = {q.x, q.y, q.z, q.w};
+      V3<double> p0[4], p1[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) { p0[k] = to_d(ldx(x0, id[k])); p1[k] = to_d(ldx(x1, id[k])); }
+      t = collision_time<EE, double>(p0, p1);
+    }
+    if (toi_out) toi_out[i] = t;
+    tmin = fmin(tmin, t);
+  }
+  if (toi_min) {
+    double m = wave_min(tmin);
+    if ((threadIdx.x & 63) == 0 && m < 1.0) atomicMin(toi_min, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// --------------------------------------------------------------- launchers ----
+static inline int blocks_for(long long n, int bs) { return int((n + bs - 1) / bs); }
+
+hipError_t launch_contact_hessian(bool ee, const float* x, const float* xcs, int nv, const int* contacts, int n, float l0,
+                                  double kappa, int offset, int* rows, int* cols, float* vals, float* b,
+                                  hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int4* c4 = reinterpret_cast<const int4*>(contacts);
+  if (ee)
+    hipLaunchKernelGGL(contact_hessian_kernel<true>, dim3(blocks_for(n, 256)), dim3(256), 0, s, x, xcs, nv, c4, n,
+                       l0, kappa, offset, rows, cols, vals, b);
+  else
+    hipLaunchKernelGGL(contact_hessian_kernel<false>, dim3(blocks_for(n, 256)), dim3(256), 0, s, x, xcs, nv, c4, n,
+                       l0, kappa, offset, rows, cols, vals, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_contact_energy(bool ee, const float* x, const float* xcs, int nv, const int* contacts, int n, float l0,
+                                 double kappa, double* energy, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int4* c4 = reinterpret_cast<const int4*>(contacts);
+  if (ee)
+    hipLaunchKernelGGL(contact_energy_kernel<true>, dim3(blocks_for(n, 256)), dim3(256), 0, s, x, xcs, nv, c4, n,
+                       l0, kappa, energy);
+  else
+    hipLaunchKernelGGL(contact_energy_kernel<false>, dim3(blocks_for(n, 256)), dim3(256), 0, s, x, xcs, nv, c4, n,
+                       l0, kappa, energy);
+  return hipGetLastError();
+}
+
+hipError_t launch_contact_points(bool ee, const float* x, const float* xcs, int nv, const int* contacts, int n, float* points,
+                                 float* dists, int* features, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int4* c4 = reinterpret_cast<const int4*>(contacts);
+  if (ee)
+    hipLaunchKernelGGL(contact_points_kernel<true>, dim3(blocks_for(n, 256)), dim3(256), 0, s, x, xcs, nv, c4, n,
+                       points, dists, features);
+  else
+    hipLaunchKernelGGL(contact_points_kernel<false>, dim3(blocks_for(n, 256)), dim3(256), 0, s, x, xcs, nv, c4, n,
+                       points, dists, features);
+  return hipGetLastError();
+}
+
+static hipError_t collision_time_grid(bool ee, const float* x0, const float* x1, int nv, const int* quads, int n,
+                                      const int* n_dev, double* toi_out, unsigned long long* toi_min, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int4* q4 = reinterpret_cast<const int4*>(quads);
+  int grid = blocks_for(n, 256);
+  if (grid > 8192) grid = 8192;
+  if (ee)
+    hipLaunchKernelGGL(collision_time_kernel<true>, dim3(grid), dim3(256), 0, s, x0, x1, nv, q4, n, n_dev, toi_out,
+                       toi_min);
+  else
+    hipLaunchKernelGGL(collision_time_kernel<false>, dim3(grid), dim3(256), 0, s, x0, x1, nv, q4, n, n_dev, toi_out,
+                       toi_min);
+  return hipGetLastError();
+}
+
+hipError_t launch_collision_time(bool ee, const float* x0, const float* x1, int nv, const int* quads, int n,
+                                 double* toi_out, unsigned long long* toi_min, hipStream_t s) {
+  return collision_time_grid(ee, x0, x1, nv, quads, n, nullptr, toi_out, toi_min, s);
+}
+
+hipError_t launch_collision_time_counted(bool ee, const float* x0, const float* x1, int nv, const int* quads,
+                                         const int* n_dev, int capacity, unsigned long long* toi_min, hipStream_t s) {
+  return collision_time_grid(ee, x0, x1, nv, quads, capacity, n_dev, nullptr, toi_min, s);
+}
+
+}  // namespace ipcd

@@ -1,0 +1,66 @@
+// Internal declarations of the culling layer (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+namespace ipcd {
+
+// Grow-only device allocation.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t b);
+  void release();
+  template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Linear BVH over n primitives (see cull.hip).
+struct Bvh {
+  int n = 0;
+  DevBuf lo, hi;              // float4 per primitive, original order
+  DevBuf keys, keys_sorted;   // uint64 Morton | id
+  DevBuf child;               // int2 per internal node
+  DevBuf parent;              // int per internal node
+  DevBuf leaf_parent;         // int per sorted leaf
+  DevBuf box;                 // 4 float4 per internal node
+  DevBuf flags;               // uint per internal node
+  DevBuf temp;                // radix sort workspace
+  DevBuf bounds;              // 6 ordered ints
+  void release();
+};
+
+enum BoxMode { BOX_EDGE = 0, BOX_TRI = 1, BOX_SWEPT_EDGE = 2, BOX_SWEPT_TRI = 3 };
+enum QueryMode { Q_DCD_EE = 0, Q_DCD_PT = 1, Q_CCD_EE = 2, Q_CCD_PT = 3 };
+
+hipError_t build_bvh(Bvh& B, int mode, int n, const int* prims, const float* x0, const float* x1, float pad,
+                     hipStream_t s);
+
+struct QueryLaunch {
+  int mode;
+  int n_query;
+  const int* edges;
+  const int* tris;
+  const int* body;
+  const float* x0;
+  const float* x1;
+  const float4* qlo;
+  const float4* qhi;
+  float pad;
+  float thickness;
+  int exclude_same_body;
+  const Bvh* bvh;
+  int4* rec;
+  int2* ids;
+  int capacity;
+  int* count;
+};
+
+hipError_t run_query(const QueryLaunch& q, hipStream_t s);
+hipError_t launch_sub(int n3, const float* x, const float* dx, float* out, hipStream_t s);
+hipError_t launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t s);
+hipError_t launch_zero_i32(int* p, int n, hipStream_t s);
+// Min and max of an int array into out_dev[0], out_dev[1] (device).
+hipError_t index_range(const int* a, long long n, int* out_dev, hipStream_t s);
+
+}  // namespace ipcd

@@ -1,0 +1,622 @@
+// Device math shared by the contact kernels of libipc_warp_amd (gfx950).
+//
+// Precision contract (the same for every kernel in this library):
+//  * the distance filter and the feature classification run in FP32, in the
+//    operation order of the reference's Warp functions, so pair sets and
+//    active sets agree bit for bit with the reference's FP32 mode;
+//  * the squared distance of the classified feature, its gradient, the
+//    negative spectral part of its Hessian and the barrier run in FP64
+//    (positions are fp32, so every difference of two of them is exact in
+//    fp64);
+//  * the 16 output blocks are expanded and stored in FP32.
+// The library is compiled with -ffp-contract=off, so no mul/add pair is fused
+// behind the source's back.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+// Every helper is host+device so the same source can be exercised on the CPU.
+#define IPCD_FN __host__ __device__ __forceinline__
+
+namespace ipcd {
+
+using std::fabs;
+using std::log;
+using std::sqrt;
+
+template <typename T> struct V3 { T x, y, z; };
+template <typename T> IPCD_FN V3<T> mk(T x, T y, T z) { return V3<T>{x, y, z}; }
+template <typename T> IPCD_FN V3<T> operator+(V3<T> a, V3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <typename T> IPCD_FN V3<T> operator-(V3<T> a, V3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <typename T> IPCD_FN V3<T> operator*(V3<T> a, T s) { return {a.x * s, a.y * s, a.z * s}; }
+template <typename T> IPCD_FN V3<T> operator*(T s, V3<T> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <typename T> IPCD_FN T dot(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <typename T> IPCD_FN V3<T> cross(V3<T> a, V3<T> b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <typename T> IPCD_FN T length(V3<T> a) { return sqrt(dot(a, a)); }
+template <typename T> IPCD_FN T smin(T a, T b) { return a < b ? a : b; }
+template <typename T> IPCD_FN T smax(T a, T b) { return a > b ? a : b; }
+template <typename T> IPCD_FN V3<T> vmin(V3<T> a, V3<T> b) { return {smin(a.x, b.x), smin(a.y, b.y), smin(a.z, b.z)}; }
+template <typename T> IPCD_FN V3<T> vmax(V3<T> a, V3<T> b) { return {smax(a.x, b.x), smax(a.y, b.y), smax(a.z, b.z)}; }
+IPCD_FN V3<double> to_d(V3<float> a) { return {double(a.x), double(a.y), double(a.z)}; }
+
+// Vertex positions are the reference's vec3 array: float32 [V, 3], AoS.
+IPCD_FN V3<float> ldx(const float* __restrict__ x, int i) {
+  const float* p = x + 3 * (long long)i;
+  return {p[0], p[1], p[2]};
+}
+
+// ------------------------------------------------- closest point, triangle ----
+// Voronoi-region closest point (contact.py:230-302). Feature: 0/1/2 vertex
+// a/b/c, 3 edge ab, 4 edge ac, 5 edge bc, 6 face. Returns the distance.
+struct TriClosest { float wa, wb, dist; int type; };
+IPCD_FN TriClosest closest_point_triangle(V3<float> p, V3<float> a, V3<float> b, V3<float> c) {
+  const float zero = 0.0f, one = 1.0f;
+  V3<float> ab = b - a, ac = c - a, ap = p - a;
+  float d1 = dot(ab, ap), d2 = dot(ac, ap);
+  if (d1 <= zero && d2 <= zero) return {one, zero, length(p - a), 0};
+  V3<float> bp = p - b;
+  float d3 = dot(ab, bp), d4 = dot(ac, bp);
+  if (d3 >= zero && d4 <= d3) return {zero, one, length(p - b), 1};
+  float vc = d1 * d4 - d3 * d2;
+  if (vc <= zero && d1 >= zero && d3 <= zero) {
+    float v = d1 / (d1 - d3);
+    V3<float> point = a + v * ab;
+    return {one - v, v, length(p - point), 3};
+  }
+  V3<float> cp = p - c;
+  float d5 = dot(ab, cp), d6 = dot(ac, cp);
+  if (d6 >= zero && d5 <= d6) return {zero, zero, length(p - c), 2};
+  float vb = d5 * d2 - d1 * d6;
+  if (vb <= zero && d2 >= zero && d6 <= zero) {
+    float w = d2 / (d2 - d6);
+    V3<float> point = a + w * ac;
+    return {one - w, zero, length(p - point), 4};
+  }
+  float va = d3 * d6 - d5 * d4;
+  if (va <= zero && (d4 - d3) >= zero && (d5 - d6) >= zero) {
+    float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    V3<float> point = b + w * (c - b);
+    return {zero, one - w, length(p - point), 5};
+  }
+  float denom = one / (va + vb + vc);
+  float v = vb * denom, w = vc * denom;
+  float u = one - v - w;
+  V3<float> point = a + ab * v + ac * w;
+  return {u, v, length(p - point), 6};
+}
+
+// ------------------------------------------------- closest points, segments ----
+// Segment-segment closest points (Ericson, Real-Time Collision Detection
+// 5.1.9), the role of Warp's closest_point_edge_edge(p1, q1, p2, q2, eps).
+struct EdgeClosest { float s, t, dist; };
+IPCD_FN float clamp01(float v) { return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v); }
+IPCD_FN EdgeClosest closest_point_edge_edge(V3<float> p1, V3<float> q1, V3<float> p2, V3<float> q2,
+                                                               float eps) {
+  V3<float> d1 = q1 - p1, d2 = q2 - p2, r = p1 - p2;
+  float a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
+  float s, t;
+  if (a <= eps && e <= eps) {
+    s = 0.0f; t = 0.0f;
+  } else if (a <= eps) {
+    s = 0.0f; t = clamp01(f / e);
+  } else {
+    float c = dot(d1, r);
+    if (e <= eps) {
+      t = 0.0f; s = clamp01(-c / a);
+    } else {
+      float b = dot(d1, d2);
+      float denom = a * e - b * b;
+      s = denom != 0.0f ? clamp01((b * f - c * e) / denom) : 0.0f;
+      t = (b * s + f) / e;
+      if (t < 0.0f) { t = 0.0f; s = clamp01(-c / a); }
+      else if (t > 1.0f) { t = 1.0f; s = clamp01((b - c) / a); }
+    }
+  }
+  V3<float> c1 = p1 + d1 * s, c2 = p2 + d2 * t;
+  return {s, t, length(c1 - c2)};
+}
+
+// ------------------------------------------------------------------ barrier ----
+// IPC log barrier in squared-distance space, x = d^2, D = dhat^2:
+// b(x) = -kappa (x - D)^2 ln(x / D) for 0 < x < D, and its x-derivatives.
+struct Barrier { double b, db, d2b; };
+IPCD_FN Barrier barrier(double x, double D, double kappa) {
+  if (!(x < D)) return {0.0, 0.0, 0.0};
+  double lg = log(x / D);
+  double dm = x - D;
+  double r = D / x;
+  Barrier B;
+  B.b = -kappa * dm * dm * lg;
+  B.db = kappa * (D - x) * (2.0 * lg - r + 1.0);
+  B.d2b = kappa * ((r + 2.0) * r - 2.0 * lg - 3.0);
+  return B;
+}
+
+// ------------------------------------------------------------ features ----
+enum { F_PP = 0, F_PE = 1, F_PT = 2, F_EE = 3 };
+struct Feature { int kind; int s0, s1, s2, s3; };
+
+// edge_edge_distance_gradient_hessian's dispatch (dynamic_contacts.py:48-108)
+// on the already computed fp32 closest points.
+IPCD_FN Feature ee_feature(EdgeClosest ec, V3<float> cr, float parallel_eps) {
+  const float fe = 1.0e-6f, one = 1.0f;
+  float s = ec.s, t = ec.t;
+  bool sb = s <= fe || s >= one - fe;
+  bool tb = t <= fe || t >= one - fe;
+  if (sb && tb) return {F_PP, (s >= one - fe) ? 1 : 0, (t >= one - fe) ? 3 : 2, -1, -1};
+  if (sb) return {F_PE, (s >= one - fe) ? 1 : 0, 2, 3, -1};
+  if (tb) return {F_PE, (t >= one - fe) ? 3 : 2, 0, 1, -1};
+  if (parallel_eps > 0.0f && dot(cr, cr) < parallel_eps) {
+    float ds0 = s, ds1 = one - s, dt0 = t, dt1 = one - t;
+    float m = smin(smin(ds0, ds1), smin(dt0, dt1));
+    if (m == ds0) return {F_PE, 0, 2, 3, -1};
+    if (m == ds1) return {F_PE, 1, 2, 3, -1};
+    if (m == dt0) return {F_PE, 2, 0, 1, -1};
+    return {F_PE, 3, 0, 1, -1};
+  }
+  return {F_EE, 0, 1, 2, 3};
+}
+
+// point_triangle_distance_gradient_hessian's dispatch (dynamic_contacts.py:110-125).
+IPCD_FN Feature pt_feature(int type) {
+  switch (type) {
+    case 0: return {F_PP, 0, 1, -1, -1};
+    case 1: return {F_PP, 0, 2, -1, -1};
+    case 2: return {F_PP, 0, 3, -1, -1};
+    case 3: return {F_PE, 0, 1, 2, -1};
+    case 4: return {F_PE, 0, 1, 3, -1};
+    case 5: return {F_PE, 0, 2, 3, -1};
+    default: return {F_PT, 0, 1, 2, 3};
+  }
+}
+
+// Squared distance of a feature in the minimisation form
+//     d^2(x) = min_u | sum_k w_k(u) x_k |^2,
+// w affine in u (k = 0 parameters for PP, 1 for PE, 2 for PT and EE). At the
+// minimiser r = sum w_k x_k is the closest-point offset, grad d^2 = 2 w (x) r,
+// and the Hessian is 2 (w w^T) (x) I3 - 2 P G^-1 P^T with
+// P_j = w (x) e_j + D_j (x) r, e_j = sum_k D_jk x_k, G = E^T E.
+struct FeatureGeom {
+  double w[4];      // weights over the 4 slots
+  double D[2][4];   // d w / d u_j
+  V3<double> e[2];  // e_j
+  V3<double> r;     // closest-point offset
+  int k;
+};
+
+// Positions are given relative to slot 0 (x[0] = 0) in double.
+IPCD_FN FeatureGeom feature_geom(const Feature& f, const V3<double> x[4]) {
+  FeatureGeom g;
+  for (int i = 0; i < 4; ++i) { g.w[i] = 0.0; g.D[0][i] = 0.0; g.D[1][i] = 0.0; }
+  g.e[0] = g.e[1] = mk(0.0, 0.0, 0.0);
+  if (f.kind == F_PP) {
+    g.k = 0;
+    g.w[f.s0] = 1.0; g.w[f.s1] = -1.0;
+    g.r = x[f.s0] - x[f.s1];
+  } else if (f.kind == F_PE) {
+    g.k = 1;
+    V3<double> e = x[f.s2] - x[f.s1];
+    V3<double> q = x[f.s0] - x[f.s1];
+    double lam = dot(q, e) / dot(e, e);
+    g.w[f.s0] = 1.0; g.w[f.s1] = -(1.0 - lam); g.w[f.s2] = -lam;
+    g.D[0][f.s1] = 1.0; g.D[0][f.s2] = -1.0;
+    g.e[0] = mk(-e.x, -e.y, -e.z);
+    g.r = q - e * lam;
+  } else if (f.kind == F_PT) {
+    g.k = 2;
+    V3<double> e1 = x[2] - x[1], e2 = x[3] - x[1], q = x[0] - x[1];
+    double a11 = dot(e1, e1), a12 = dot(e1, e2), a22 = dot(e2, e2);
+    double b1 = dot(e1, q), b2 = dot(e2, q);
+    double det = a11 * a22 - a12 * a12;
+    double be = (b1 * a22 - b2 * a12) / det;
+    double ga = (a11 * b2 - a12 * b1) / det;
+    g.w[0] = 1.0; g.w[1] = -(1.0 - be - ga); g.w[2] = -be; g.w[3] = -ga;
+    g.D[0][1] = 1.0; g.D[0][2] = -1.0;
+    g.D[1][1] = 1.0; g.D[1][3] = -1.0;
+    g.e[0] = mk(-e1.x, -e1.y, -e1.z);
+    g.e[1] = mk(-e2.x, -e2.y, -e2.z);
+    g.r = q - e1 * be - e2 * ga;
+  } else {
+    g.k = 2;
+    V3<double> ea = x[1] - x[0], eb = x[3] - x[2], q = x[0] - x[2];
+    double a11 = dot(ea, ea), a12 = -dot(ea, eb), a22 = dot(eb, eb);
+    double b1 = -dot(ea, q), b2 = dot(eb, q);
+    double det = a11 * a22 - a12 * a12;
+    double s = (b1 * a22 - b2 * a12) / det;
+    double t = (a11 * b2 - a12 * b1) / det;
+    g.w[0] = 1.0 - s; g.w[1] = s; g.w[2] = -(1.0 - t); g.w[3] = -t;
+    g.D[0][0] = -1.0; g.D[0][1] = 1.0;
+    g.D[1][2] = 1.0; g.D[1][3] = -1.0;
+    g.e[0] = ea;
+    g.e[1] = mk(-eb.x, -eb.y, -eb.z);
+    g.r = q + ea * s - eb * t;
+  }
+  return g;
+}
+
+// Cyclic Jacobi on a symmetric 5x5 (row-major), double. Returns eigenvalues
+// in ev and eigenvectors in the columns of V.
+IPCD_FN void jacobi5(double A[25], double ev[5], double V[25]) {
+#pragma unroll
+  for (int i = 0; i < 25; ++i) V[i] = (i % 6 == 0) ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 12; ++sweep) {
+    double off = 0.0, diag = 0.0;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+      diag += A[i * 5 + i] * A[i * 5 + i];
+#pragma unroll
+      for (int j = i + 1; j < 5; ++j) off += A[i * 5 + j] * A[i * 5 + j];
+    }
+    if (!(off > 1.0e-30 * diag)) break;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+#pragma unroll
+      for (int q = p + 1; q < 5; ++q) {
+        double apq = A[p * 5 + q];
+        if (apq != 0.0) {
+          double theta = (A[q * 5 + q] - A[p * 5 + p]) / (2.0 * apq);
+          double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+          double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+#pragma unroll
+          for (int k = 0; k < 5; ++k) {
+            double akp = A[k * 5 + p], akq = A[k * 5 + q];
+            A[k * 5 + p] = c * akp - s * akq;
+            A[k * 5 + q] = s * akp + c * akq;
+          }
+#pragma unroll
+          for (int k = 0; k < 5; ++k) {
+            double apk = A[p * 5 + k], aqk = A[q * 5 + k];
+            A[p * 5 + k] = c * apk - s * aqk;
+            A[q * 5 + k] = s * apk + c * aqk;
+          }
+#pragma unroll
+          for (int k = 0; k < 5; ++k) {
+            double vkp = V[k * 5 + p], vkq = V[k * 5 + q];
+            V[k * 5 + p] = c * vkp - s * vkq;
+            V[k * 5 + q] = s * vkp + c * vkq;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 5; ++i) ev[i] = A[i * 5 + i];
+}
+
+// Compressed form of the 16 blocks of a contact Hessian
+//     H_ij = c_n (w^_i w^_j Ntt + w^_i u_j r^T + w^_j r u_i^T) + (cg w_i w_j + c_n s_ij) r r^T
+// with r the unit closest-point direction (see feature_geom). Stored in fp32.
+struct CompressedHessian {
+  float what[4], w[4];
+  float rh[3];
+  float Ntt[6];     // xx, xy, xz, yy, yz, zz of the NSD part in the w^ (x) I3 block
+  float u[4][3];    // Ntz z_i
+  float s[10];      // z_i^T Nzz z_j, packed upper triangle (i <= j)
+  float cn, cg;
+};
+
+IPCD_FN int sidx(int i, int j) {
+  // packed index of (min, max) in a 4x4 upper triangle
+  int a = i < j ? i : j, b = i < j ? j : i;
+  return a * 4 - (a * (a - 1)) / 2 + (b - a);
+}
+
+// Negative spectral part of the Hessian of d^2 for a feature, reduced to the
+// (3 + k)-dimensional subspace span{w^ (x) R^3, Z (x) r^}, Z an orthonormal
+// basis of the D_j projected orthogonal to w. Fills Ntt, u, s and the
+// geometry fields of `C`; returns |r|^2 (the feature's squared distance).
+IPCD_FN double nsd_compressed(const FeatureGeom& g, bool want_nsd, CompressedHessian& C) {
+  double d2 = dot(g.r, g.r);
+  double rn = sqrt(d2);
+  V3<double> rh = rn > 0.0 ? g.r * (1.0 / rn) : mk(0.0, 0.0, 0.0);
+  double wn2 = g.w[0] * g.w[0] + g.w[1] * g.w[1] + g.w[2] * g.w[2] + g.w[3] * g.w[3];
+  double wn = sqrt(wn2);
+  double wh[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) { wh[i] = g.w[i] / wn; C.what[i] = float(wh[i]); C.w[i] = float(g.w[i]); }
+  C.rh[0] = float(rh.x); C.rh[1] = float(rh.y); C.rh[2] = float(rh.z);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) C.Ntt[i] = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) { C.u[i][0] = C.u[i][1] = C.u[i][2] = 0.0f; }
+#pragma unroll
+  for (int i = 0; i < 10; ++i) C.s[i] = 0.0f;
+  if (!want_nsd || g.k == 0) return d2;
+
+  // Z: Gram-Schmidt of the D_j orthogonal to w^.
+  double Z[4][2];
+  double wd[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    wd[j] = wh[0] * g.D[j][0] + wh[1] * g.D[j][1] + wh[2] * g.D[j][2] + wh[3] * g.D[j][3];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) Z[i][j] = g.D[j][i] - wd[j] * wh[i];
+  }
+  {
+    double n0 = sqrt(Z[0][0] * Z[0][0] + Z[1][0] * Z[1][0] + Z[2][0] * Z[2][0] + Z[3][0] * Z[3][0]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) Z[i][0] /= n0;
+    if (g.k == 2) {
+      double p = Z[0][0] * Z[0][1] + Z[1][0] * Z[1][1] + Z[2][0] * Z[2][1] + Z[3][0] * Z[3][1];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) Z[i][1] -= p * Z[i][0];
+      double n1 = sqrt(Z[0][1] * Z[0][1] + Z[1][1] * Z[1][1] + Z[2][1] * Z[2][1] + Z[3][1] * Z[3][1]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) Z[i][1] /= n1;
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) Z[i][1] = 0.0;
+    }
+  }
+  // M = Q^T P: rows 0..2 = |w| e_j + (w^.D_j) r, rows 3..4 = (z_l . D_j) |r|.
+  double M[5][2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    M[0][j] = wn * g.e[j].x + wd[j] * g.r.x;
+    M[1][j] = wn * g.e[j].y + wd[j] * g.r.y;
+    M[2][j] = wn * g.e[j].z + wd[j] * g.r.z;
+#pragma unroll
+    for (int l = 0; l < 2; ++l)
+      M[3 + l][j] = (Z[0][l] * g.D[j][0] + Z[1][l] * g.D[j][1] + Z[2][l] * g.D[j][2] + Z[3][l] * g.D[j][3]) * rn;
+  }
+  // G^-1 (k x k).
+  double Gi[2][2];
+  if (g.k == 2) {
+    double g11 = dot(g.e[0], g.e[0]), g12 = dot(g.e[0], g.e[1]), g22 = dot(g.e[1], g.e[1]);
+    double idet = 1.0 / (g11 * g22 - g12 * g12);
+    Gi[0][0] = g22 * idet; Gi[0][1] = -g12 * idet; Gi[1][0] = -g12 * idet; Gi[1][1] = g11 * idet;
+  } else {
+    Gi[0][0] = 1.0 / dot(g.e[0], g.e[0]); Gi[0][1] = Gi[1][0] = Gi[1][1] = 0.0;
+    for (int a = 0; a < 5; ++a) M[a][1] = 0.0;
+  }
+  // K = 2 diag(|w|^2 I3, 0) - 2 M G^-1 M^T.
+  double K[25];
+#pragma unroll
+  for (int a = 0; a < 5; ++a) {
+    double MG0 = M[a][0] * Gi[0][0] + M[a][1] * Gi[1][0];
+    double MG1 = M[a][0] * Gi[0][1] + M[a][1] * Gi[1][1];
+#pragma unroll
+    for (int b = 0; b < 5; ++b) {
+      double v = -2.0 * (MG0 * M[b][0] + MG1 * M[b][1]);
+      if (a == b && a < 3) v += 2.0 * wn2;
+      K[a * 5 + b] = v;
+    }
+  }
+  double ev[5], V[25];
+  jacobi5(K, ev, V);
+  // N = sum over negative eigenvalues of lambda v v^T.
+  double N[25];
+#pragma unroll
+  for (int i = 0; i < 25; ++i) N[i] = 0.0;
+#pragma unroll
+  for (int m = 0; m < 5; ++m) {
+    double lam = ev[m];
+    if (lam < 0.0) {
+#pragma unroll
+      for (int a = 0; a < 5; ++a)
+#pragma unroll
+        for (int b = 0; b < 5; ++b) N[a * 5 + b] += lam * V[a * 5 + m] * V[b * 5 + m];
+    }
+  }
+  C.Ntt[0] = float(N[0]); C.Ntt[1] = float(N[1]); C.Ntt[2] = float(N[2]);
+  C.Ntt[3] = float(N[6]); C.Ntt[4] = float(N[7]); C.Ntt[5] = float(N[12]);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) C.u[i][a] = float(N[a * 5 + 3] * Z[i][0] + N[a * 5 + 4] * Z[i][1]);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = i; j < 4; ++j) {
+      double v = Z[i][0] * (N[18] * Z[j][0] + N[19] * Z[j][1]) + Z[i][1] * (N[23] * Z[j][0] + N[24] * Z[j][1]);
+      C.s[sidx(i, j)] = float(v);
+    }
+  return d2;
+}
+
+// Block (i, j) of the compressed Hessian, row-major 3x3.
+IPCD_FN void expand_block(const CompressedHessian& C, int i, int j, float out[9]) {
+  float ww = C.what[i] * C.what[j];
+  float rr = C.cg * C.w[i] * C.w[j] + C.cn * C.s[sidx(i, j)];
+  const float* r = C.rh;
+  const float Nt[9] = {C.Ntt[0], C.Ntt[1], C.Ntt[2], C.Ntt[1], C.Ntt[3], C.Ntt[4], C.Ntt[2], C.Ntt[4], C.Ntt[5]};
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      // w^_i u_j r^T has entry w^_i u_j[a] r[b]; w^_j r u_i^T has w^_j r[a] u_i[b].
+      out[a * 3 + b] = C.cn * (ww * Nt[a * 3 + b] + C.what[i] * C.u[j][a] * r[b] + C.what[j] * r[a] * C.u[i][b]) +
+                       rr * r[a] * r[b];
+    }
+}
+
+}  // namespace ipcd
+
+// ================================================================ CCD math ====
+// Coplanarity cubic and root isolation (ccd/ccd.py:226-357,
+// ccd/cubic_roots.py:1-193) in the precision the caller instantiates (the
+// kernels use double). The cubic det[p1 - p0, p2 - p0, p3 - p0](t) is expanded
+// in coordinates relative to point 0, which is mathematically the reference's
+// sum of four absolute 3x3 determinants with far less cancellation.
+namespace ipcd {
+
+template <typename T> IPCD_FN T cevaluate(const T c[4], T x) { return c[0] + x * (c[1] + x * (c[2] + x * c[3])); }
+template <typename T> IPCD_FN T multi_sign(T a, T b) { return b < T(0) ? -a : a; }
+template <typename T> IPCD_FN bool diff_sign(T y0, T yr) { return y0 * yr < T(0); }
+template <typename T> IPCD_FN void deflate(const T coef[4], T root, T out[3]) {
+  out[0] = T(0); out[1] = T(0);
+  out[2] = coef[3];
+  out[1] = coef[2] + root * out[2];
+  out[0] = coef[1] + root * out[1];
+}
+// Safeguarded Newton / bisection of cubic_roots.py:29-62. The reference loop
+// is unbounded; this one stops after kFindClosedMaxIter rounds so every wave
+// terminates.
+constexpr int kFindClosedMaxIter = 200;
+template <typename T>
+IPCD_FN T find_closed(const T coef[4], const T deriv[4], T x0, T x1, T y0) {
+  const T x_error = T(1e-10);
+  T xr = (x0 + x1) / T(2);
+  T yr = cevaluate(coef, xr);
+  T xb0 = x0, xb1 = x1;
+  if (x1 - x0 <= x_error * T(2)) return xr;
+  for (int it = 0; it < kFindClosedMaxIter; ++it) {
+    bool side = diff_sign(y0, yr);
+    if (side) xb1 = xr; else xb0 = xr;
+    T dy = cevaluate(deriv, xr);
+    T dx = yr / dy;
+    T xn = xr - dx;
+    if (xn > xb0 && xn < xb1) {
+      T stepsize = fabs(xr - xn);
+      xr = xn;
+      if (stepsize > x_error) yr = cevaluate(coef, xr);
+      else break;
+    } else {
+      xr = (xb0 + xb1) / T(2);
+      yr = cevaluate(coef, xr);
+      if (xb0 == xr || xb1 == xr || xb1 - xb0 <= T(2) * x_error) break;
+    }
+  }
+  return xr;
+}
+template <typename T>
+IPCD_FN int quadratic_roots(const T coef[3], T x0, T x1, T roots[2]) {
+  roots[0] = T(0); roots[1] = T(0);
+  T c = coef[0], b = coef[1], a = coef[2];
+  T delta = b * b - T(4) * a * c;
+  int ret = 0;
+  if (delta > T(0)) {
+    T d = sqrt(delta);
+    T q = -(b + multi_sign(d, b)) * T(0.5);
+    T rv0 = q / a, rv1 = c / q;
+    T xa = smin(rv0, rv1), xb = smax(rv0, rv1);
+    if (xa >= x0 && xa <= x1) roots[ret++] = xa;
+    if (xb >= x0 && xb <= x1) roots[ret++] = xb;
+  } else if (!(delta < T(0))) {
+    T r0 = T(-0.5) * b / a;
+    roots[0] = r0;
+    if (r0 >= x0 && r0 <= x1) ret = 1;
+  }
+  return ret;
+}
+template <typename T>
+IPCD_FN int cubic_roots(const T coef[4], T x0, T x1, T roots[3]) {
+  roots[0] = roots[1] = roots[2] = T(0);
+  T y0 = cevaluate(coef, x0), y1 = cevaluate(coef, x1);
+  T a = coef[3] * T(3), b_2 = coef[2], c = coef[1];
+  T deriv[4] = {c, T(2) * b_2, a, T(0)};
+  T delta_4 = b_2 * b_2 - a * c;
+  T defpoly[3], qroots[2];
+  if (delta_4 > T(0)) {
+    T d_2 = sqrt(delta_4);
+    T q = -(b_2 + multi_sign(d_2, b_2));
+    T rv0 = q / a, rv1 = c / q;
+    T xa = smin(rv0, rv1), xb = smax(rv0, rv1);
+    bool outside = xa >= x1 || xb <= x0 || (xa <= x0 && xb >= x1);
+    if (diff_sign(y0, y1)) {
+      if (outside) { roots[0] = find_closed(coef, deriv, x0, x1, y0); return 1; }
+    } else if (outside) {
+      return 0;
+    }
+    if (xa > x0) {
+      T ya = cevaluate(coef, xa);
+      if (diff_sign(y0, ya)) {
+        roots[0] = find_closed(coef, deriv, x0, xa, y0);
+        if (diff_sign(ya, y1) || (xb < x1 && diff_sign(ya, cevaluate(coef, xb)))) {
+          deflate(coef, roots[0], defpoly);
+          int r = quadratic_roots(defpoly, xa, x1, qroots);
+          roots[1] = qroots[0]; roots[2] = qroots[1];
+          return r + 1;
+        }
+        return 1;
+      }
+      if (xb < x1) {
+        T yb = cevaluate(coef, xb);
+        if (diff_sign(ya, yb)) {
+          roots[0] = find_closed(coef, deriv, xa, xb, ya);
+          if (diff_sign(yb, y1)) {
+            deflate(coef, roots[0], defpoly);
+            int r = quadratic_roots(defpoly, xb, x1, qroots);
+            roots[1] = qroots[0]; roots[2] = qroots[1];
+            return r + 1;
+          }
+          return 1;
+        }
+        if (diff_sign(yb, y1)) { roots[0] = find_closed(coef, deriv, xb, x1, yb); return 1; }
+      } else if (diff_sign(ya, y1)) {
+        roots[0] = find_closed(coef, deriv, xa, x1, ya);
+        return 1;
+      }
+    } else {
+      T yb = cevaluate(coef, xb);
+      if (diff_sign(y0, yb)) {
+        roots[0] = find_closed(coef, deriv, x0, xb, y0);
+        if (diff_sign(yb, y1)) {
+          deflate(coef, roots[0], defpoly);
+          int r = quadratic_roots(defpoly, xb, x1, qroots);
+          roots[1] = qroots[0]; roots[2] = qroots[1];
+          return r + 1;
+        }
+        return 1;
+      }
+      if (diff_sign(yb, y1)) { roots[0] = find_closed(coef, deriv, xb, x1, yb); return 1; }
+    }
+    return 0;
+  }
+  if (diff_sign(y0, y1)) { roots[0] = find_closed(coef, deriv, x0, x1, y0); return 1; }
+  return 0;
+}
+
+template <typename T> IPCD_FN T det3(V3<T> a, V3<T> b, V3<T> c) {
+  return a.x * (b.y * c.z - b.z * c.y) + a.y * (b.z * c.x - b.x * c.z) + a.z * (b.x * c.y - b.y * c.x);
+}
+
+// Inside tests at a root (ccd/ccd.py:212-224) with the 2x2 least-squares
+// barycentric coordinates (beta_gamma_pt / beta_gamma_ee).
+template <typename T> IPCD_FN void ls2(V3<T> e1, V3<T> e2, V3<T> q, T* beta, T* gamma) {
+  T a11 = dot(e1, e1), a12 = dot(e1, e2), a22 = dot(e2, e2);
+  T b1 = dot(e1, q), b2 = dot(e2, q);
+  T det = a11 * a22 - a12 * a12;
+  *beta = (b1 * a22 - b2 * a12) / det;
+  *gamma = (a11 * b2 - a12 * b1) / det;
+}
+
+// pt_collision_time / ee_collision_time (ccd/ccd.py:304-357) on the 4 points
+// at t = 0 (p0) and t = 1 (p1). PT: point, t0, t1, t2; EE: a0, a1, b0, b1.
+template <bool EE, typename T>
+IPCD_FN T collision_time(const V3<T> p0[4], const V3<T> p1[4]) {
+  V3<T> u[3], v[3];
+  V3<T> d0 = p1[0] - p0[0];
+  for (int k = 0; k < 3; ++k) { u[k] = p0[k + 1] - p0[0]; v[k] = (p1[k + 1] - p0[k + 1]) - d0; }
+  T coef[4];
+  coef[0] = det3(u[0], u[1], u[2]);
+  coef[1] = det3(v[0], u[1], u[2]) + det3(u[0], v[1], u[2]) + det3(u[0], u[1], v[2]);
+  coef[2] = det3(u[0], v[1], v[2]) + det3(v[0], u[1], v[2]) + det3(v[0], v[1], u[2]);
+  coef[3] = det3(v[0], v[1], v[2]);
+  T roots[3];
+  int n = cubic_roots(coef, T(0), T(1), roots);
+  const T tol = T(1e-8);
+  for (int i = 0; i < n; ++i) {
+    T t = roots[i];
+    V3<T> q1 = u[0] + v[0] * t, q2 = u[1] + v[1] * t, q3 = u[2] + v[2] * t;
+    T beta, gamma;
+    bool ok;
+    if (EE) {
+      // point 0 sits at the origin: ea = q1, eb = q3 - q2, q = q2.
+      ls2(q1, q3 - q2, q2, &beta, &gamma);
+      ok = beta >= -tol && beta <= T(1) + tol && -gamma >= -tol && -gamma <= T(1) + tol;
+    } else {
+      // triangle (q1, q2, q3), point at the origin: q = -q1.
+      ls2(q2 - q1, q3 - q1, mk(T(0), T(0), T(0)) - q1, &beta, &gamma);
+      ok = beta >= -tol && gamma >= -tol && beta + gamma <= T(1) + tol;
+    }
+    if (ok) return t;
+  }
+  return T(1);
+}
+
+}  // namespace ipcd

@@ -1,0 +1,23 @@
+// Internal launcher declarations of libipc_warp_amd (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace ipcd {
+
+hipError_t launch_contact_hessian(bool ee, const float* x, const float* xcs, int nv, const int* contacts, int n, float l0,
+                                  double kappa, int offset, int* rows, int* cols, float* vals, float* b,
+                                  hipStream_t s);
+hipError_t launch_contact_energy(bool ee, const float* x, const float* xcs, int nv, const int* contacts, int n, float l0,
+                                 double kappa, double* energy, hipStream_t s);
+hipError_t launch_contact_points(bool ee, const float* x, const float* xcs, int nv, const int* contacts, int n, float* points,
+                                 float* dists, int* features, hipStream_t s);
+hipError_t launch_collision_time(bool ee, const float* x0, const float* x1, int nv, const int* quads, int n, double* toi_out,
+                                 unsigned long long* toi_min, hipStream_t s);
+
+hipError_t launch_collision_time_counted(bool ee, const float* x0, const float* x1, int nv, const int* quads,
+                                         const int* n_dev, int capacity, unsigned long long* toi_min, hipStream_t s);
+
+// Records the calling thread's last error message (read by ipc_last_error).
+void set_error(const char* msg);
+
+}  // namespace ipcd

@@ -1,0 +1,148 @@
+"""Tensor-level launches of the libipc_warp_amd kernels.
+
+PyTorch-ROCm tensors are only buffers here: each function checks device,
+dtype and contiguity, then hands raw device pointers and torch's current HIP
+stream to the C ABI. Names and argument meaning follow the reference's Warp
+kernels (file:line given per function).
+"""
+from __future__ import annotations
+
+import struct
+
+import torch
+
+from ._lib import check, lib
+
+
+def _need(t: torch.Tensor, dtype: torch.dtype, name: str, shape_tail=None) -> int:
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor")
+    if t.device.type != "cuda":
+        raise ValueError(f"{name} must live on the GPU (got {t.device})")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype} (got {t.dtype})")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if shape_tail is not None and tuple(t.shape[1:]) != tuple(shape_tail):
+        raise ValueError(f"{name} must have shape [n, {', '.join(map(str, shape_tail))}] (got {tuple(t.shape)})")
+    return t.data_ptr()
+
+
+def _opt(t, dtype, name, shape_tail=None):
+    return None if t is None else _need(t, dtype, name, shape_tail)
+
+
+def _rows(t: torch.Tensor, n: int, name: str) -> int:
+    """Pointer to an int32 [>=n, 4] record stream."""
+    p = _need(t, torch.int32, name, (4,))
+    if int(n) < 0 or int(n) > t.shape[0]:
+        raise ValueError(f"{name} holds {t.shape[0]} records, fewer than n = {n}")
+    return p
+
+
+def _vec_len(t, n, name):
+    if t is not None and t.shape[0] < int(n):
+        raise ValueError(f"{name} holds {t.shape[0]} entries, fewer than n = {n}")
+
+
+def stream_handle(device=None) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def contact_hessian_ee(x, xcs, contacts, n, l0, kappa, rows, cols, vals, b, offset=0):
+    """contact_hessian_ee (dynamic_contacts.py:151-207), dim = n.
+
+    x, xcs: float32 [V,3]; contacts: int32 [>=n,4]; rows/cols int32 [16*total];
+    vals float32 [16*total,3,3]; b float32 [V,3] (force accumulator).
+    """
+    _check_triplets(rows, cols, vals, offset, n, x, b)
+    if xcs.shape != x.shape:
+        raise ValueError("xcs must have the shape of x")
+    check(lib().ipc_contact_hessian_ee(_need(x, torch.float32, "x", (3,)), _need(xcs, torch.float32, "xcs", (3,)),
+                                       x.shape[0], _rows(contacts, n, "contacts"), int(n), float(l0), float(kappa),
+                                       int(offset), _need(rows, torch.int32, "rows"), _need(cols, torch.int32, "cols"),
+                                       _need(vals, torch.float32, "vals", (3, 3)), _need(b, torch.float32, "b", (3,)),
+                                       stream_handle(x.device)), "contact_hessian_ee")
+
+
+def contact_hessian_pt(x, contacts, n, l0, kappa, rows, cols, vals, b, offset=0):
+    """contact_hessian_pt (dynamic_contacts.py:209-246), dim = n, triplets at offset."""
+    _check_triplets(rows, cols, vals, offset, n, x, b)
+    check(lib().ipc_contact_hessian_pt(_need(x, torch.float32, "x", (3,)), x.shape[0],
+                                       _rows(contacts, n, "contacts"), int(n), float(l0), float(kappa),
+                                       int(offset), _need(rows, torch.int32, "rows"), _need(cols, torch.int32, "cols"),
+                                       _need(vals, torch.float32, "vals", (3, 3)), _need(b, torch.float32, "b", (3,)),
+                                       stream_handle(x.device)), "contact_hessian_pt")
+
+
+def _check_triplets(rows, cols, vals, offset, n, x=None, b=None):
+    if x is not None and b is not None and b.shape != x.shape:
+        raise ValueError("b must have the shape of x")
+    need = 16 * (int(offset) + int(n))
+    if int(offset) < 0:
+        raise ValueError("offset must be >= 0")
+    if rows.numel() < need or cols.numel() < need or vals.shape[0] < need:
+        raise ValueError(f"triplet buffers hold fewer than 16 * (offset + n) = {need} blocks")
+
+
+def contact_energy_ee(x, xcs, contacts, n, l0, kappa, e):
+    """contact_energy_ee (dynamic_contacts.py:374-388): adds the sum into e[0] (float64)."""
+    if xcs.shape != x.shape:
+        raise ValueError("xcs must have the shape of x")
+    check(lib().ipc_contact_energy_ee(_need(x, torch.float32, "x", (3,)), _need(xcs, torch.float32, "xcs", (3,)),
+                                      x.shape[0], _rows(contacts, n, "contacts"), int(n), float(l0), float(kappa),
+                                      _need(e, torch.float64, "e"), stream_handle(x.device)), "contact_energy_ee")
+
+
+def contact_energy_pt(x, contacts, n, l0, kappa, e):
+    """contact_energy_pt (dynamic_contacts.py:390-398): adds the sum into e[0] (float64)."""
+    check(lib().ipc_contact_energy_pt(_need(x, torch.float32, "x", (3,)), x.shape[0],
+                                      _rows(contacts, n, "contacts"), int(n), float(l0), float(kappa),
+                                      _need(e, torch.float64, "e"), stream_handle(x.device)), "contact_energy_pt")
+
+
+def contact_points_ee(x, contacts, n, points=None, dists=None, features=None, xcs=None):
+    """get_contact_points (contact.py:148-153) over fetch_dist_v0v1 (contact.py:88-110)."""
+    for t, nm in ((points, "points"), (dists, "dists"), (features, "features")):
+        _vec_len(t, n, nm)
+    if xcs is not None and xcs.shape != x.shape:
+        raise ValueError("xcs must have the shape of x")
+    check(lib().ipc_contact_points_ee(_need(x, torch.float32, "x", (3,)), _opt(xcs, torch.float32, "xcs", (3,)),
+                                      x.shape[0], _rows(contacts, n, "contacts"), int(n),
+                                      _opt(points, torch.float32, "points", (3,)), _opt(dists, torch.float32, "dists"),
+                                      _opt(features, torch.int32, "features"), stream_handle(x.device)),
+          "contact_points_ee")
+
+
+def contact_points_pt(x, contacts, n, points=None, dists=None, features=None):
+    """get_contact_points_pt (contact.py:155-161) over fetch_dist_v0v1_pt (contact.py:116-145)."""
+    for t, nm in ((points, "points"), (dists, "dists"), (features, "features")):
+        _vec_len(t, n, nm)
+    check(lib().ipc_contact_points_pt(_need(x, torch.float32, "x", (3,)), x.shape[0],
+                                      _rows(contacts, n, "contacts"), int(n),
+                                      _opt(points, torch.float32, "points", (3,)), _opt(dists, torch.float32, "dists"),
+                                      _opt(features, torch.int32, "features"), stream_handle(x.device)),
+          "contact_points_pt")
+
+
+ONE_BITS = struct.unpack("<q", struct.pack("<d", 1.0))[0]
+
+
+def new_toi_min(device) -> torch.Tensor:
+    """A device int64 holding the bit pattern of 1.0 (the reference's ccd_toi.fill_(1.0))."""
+    return torch.full((1,), ONE_BITS, dtype=torch.int64, device=device)
+
+
+def toi_value(toi_min: torch.Tensor) -> float:
+    return struct.unpack("<d", struct.pack("<q", int(toi_min.item())))[0]
+
+
+def collision_time(kind, x0, x1, quads, n, toi=None, toi_min=None):
+    """pt_collision_time / ee_collision_time (ccd/ccd.py:304-357) over candidate quadruples."""
+    fn = lib().ipc_collision_time_ee if kind == "ee" else lib().ipc_collision_time_pt
+    if x1.shape != x0.shape:
+        raise ValueError("x1 must have the shape of x0")
+    _vec_len(toi, n, "toi")
+    check(fn(_need(x0, torch.float32, "x0", (3,)), _need(x1, torch.float32, "x1", (3,)), x0.shape[0],
+             _rows(quads, n, "quads"), int(n), _opt(toi, torch.float64, "toi"),
+             _opt(toi_min, torch.int64, "toi_min"), stream_handle(x0.device)), f"collision_time_{kind}")

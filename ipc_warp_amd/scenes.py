@@ -1,0 +1,270 @@
+"""Procedural geometry for the contact workloads (host side, numpy).
+
+The reference loads meshes from files that do not ship with it; these
+generators rebuild the same kinds of input procedurally and deterministically
+(seeded):
+
+* tet-grid cubes split 6 tets per cell with the reference's split table
+  (fem/geometry.py:36-49) and their boundary surface;
+* subdivided icospheres (the wrecking-ball bodies);
+* the reference's ground quad (assets/plane.obj: 40 x 40, y = 0);
+* a soup assembly that concatenates bodies the way SimComplexBase does
+  (fem/geometry.py:174-229): vertices stacked, edges from boundary faces
+  (igl.edges of F), one body id per vertex;
+* synthetic contact pair streams with a controlled feature mix and
+  distance distribution, used by the kernel benchmarks.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# Corner order of a grid cell is J = i*4 + j*2 + k (fem/geometry.py:23-25,60-62);
+# the 6-tet split table is fem/geometry.py:44-49.
+_CELL_TETS = np.array([[4, 1, 5, 3], [5, 4, 3, 6], [3, 5, 6, 7], [0, 1, 4, 2], [1, 4, 2, 3], [4, 2, 3, 6]])
+
+
+@dataclass
+class Soup:
+    """Mesh soup handed to every contact kernel (fem/geometry.py:165-171)."""
+
+    x: np.ndarray  # float32 [V,3] current positions (x_transformed / states.x)
+    xcs: np.ndarray  # float32 [V,3] rest positions
+    edges: np.ndarray  # int32 [E,2]
+    triangles: np.ndarray  # int32 [F,3]
+    body: np.ndarray  # int32 [V]
+    tets: np.ndarray = field(default_factory=lambda: np.zeros((0, 4), np.int32))  # int32 [T,4]
+    n_bodies: int = 1
+    body_vertex_offsets: np.ndarray | None = None
+
+    @property
+    def n_vertices(self) -> int:
+        return int(self.x.shape[0])
+
+
+def tet_grid_cube(n: int = 8, size: float = 1.0, origin=(0.0, 0.0, 0.0)):
+    """(n+1)^3 nodes, 6 n^3 tets. Node index i*(n+1)^2 + j*(n+1) + k."""
+    h = size / n
+    g = np.arange(n + 1)
+    I, J, K = np.meshgrid(g, g, g, indexing="ij")
+    V = np.stack([I, J, K], -1).reshape(-1, 3).astype(np.float64) * h + np.asarray(origin, np.float64)
+    c = np.arange(n)
+    ci, cj, ck = (a.reshape(-1) for a in np.meshgrid(c, c, c, indexing="ij"))
+    corners = np.zeros((ci.size, 8), np.int64)
+    for J_ in range(8):
+        di, dj, dk = J_ // 4, (J_ % 4) // 2, J_ % 2
+        corners[:, J_] = (ci + di) * (n + 1) ** 2 + (cj + dj) * (n + 1) + (ck + dk)
+    T = corners[:, _CELL_TETS].reshape(-1, 4)
+    return V, T.astype(np.int32)
+
+
+def boundary_faces(V: np.ndarray, T: np.ndarray) -> np.ndarray:
+    """Faces of a tet mesh used by exactly one tet, oriented outward."""
+    faces = np.concatenate([T[:, [1, 2, 3]], T[:, [0, 3, 2]], T[:, [0, 1, 3]], T[:, [0, 2, 1]]])
+    opposite = np.concatenate([T[:, 0], T[:, 1], T[:, 2], T[:, 3]])
+    key = np.sort(faces, axis=1)
+    _, inv, cnt = np.unique(key, axis=0, return_inverse=True, return_counts=True)
+    once = cnt[inv.reshape(-1)] == 1
+    F, opp = faces[once], opposite[once]
+    n = np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]])
+    inward = np.einsum("ij,ij->i", n, V[opp] - V[F[:, 0]]) > 0
+    F[inward] = F[inward][:, [0, 2, 1]]
+    return F.astype(np.int32)
+
+
+def unique_edges(F: np.ndarray) -> np.ndarray:
+    """Undirected edges of a triangle list (igl.edges), sorted (min, max)."""
+    e = np.concatenate([F[:, [0, 1]], F[:, [1, 2]], F[:, [2, 0]]])
+    e = np.sort(e, axis=1)
+    return np.unique(e, axis=0).astype(np.int32)
+
+
+def icosphere(subdivisions: int = 3, radius: float = 0.5):
+    """Loop-free midpoint subdivision of the icosahedron, projected to the sphere."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    V = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    F = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+                  [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5],
+                  [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], np.int64)
+    V /= np.linalg.norm(V, axis=1, keepdims=True)
+    for _ in range(subdivisions):
+        verts = list(V)
+        cache = {}
+
+        def mid(a, b):
+            key = (a, b) if a < b else (b, a)
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in F:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [[a, ab, ca], [b, bc, ab], [c, ca, bc], [ab, bc, ca]]
+        V, F = np.array(verts), np.array(nf, np.int64)
+    return V * radius, F.astype(np.int32)
+
+
+def ground_quad(y: float = 0.0, half: float = 20.0):
+    """assets/plane.obj: a 40 x 40 quad at y = 0 as two triangles."""
+    V = np.array([[-half, y, half], [half, y, half], [-half, y, -half], [half, y, -half]], np.float64)
+    F = np.array([[1, 2, 0], [1, 3, 2]], np.int32)
+    return V, F
+
+
+def assemble(bodies) -> Soup:
+    """bodies: iterable of (V [n,3], F [m,3], T [k,4] or None)."""
+    Vs, Fs, Es, Ts, B, offsets = [], [], [], [], [], [0]
+    base = 0
+    for b, (V, F, T) in enumerate(bodies):
+        Vs.append(np.asarray(V, np.float64))
+        Fs.append(np.asarray(F, np.int64) + base)
+        Es.append(unique_edges(np.asarray(F)) + base)
+        if T is not None and len(T):
+            Ts.append(np.asarray(T, np.int64) + base)
+        B.append(np.full(len(V), b, np.int32))
+        base += len(V)
+        offsets.append(base)
+    x = np.concatenate(Vs).astype(np.float32)
+    return Soup(
+        x=x,
+        xcs=x.copy(),
+        edges=np.concatenate(Es).astype(np.int32),
+        triangles=np.concatenate(Fs).astype(np.int32),
+        body=np.concatenate(B),
+        tets=np.concatenate(Ts).astype(np.int32) if Ts else np.zeros((0, 4), np.int32),
+        n_bodies=len(B),
+        body_vertex_offsets=np.asarray(offsets, np.int64),
+    )
+
+
+# ------------------------------------------------------------- workloads ----
+def two_cubes_on_ground(seed: int = 0, n: int = 8, gap: float = 2.0e-3, jitter: float = 1.0e-4) -> Soup:
+    """Two n^3-cell tet cubes (edge 1.0) stacked `gap` apart above the ground quad.
+
+    The bottom cube sits `gap` above y = 0, the top cube `gap` above the
+    bottom one (origins 1 + gap apart); vertices get U(-jitter, jitter).
+    """
+    rng = np.random.default_rng(seed)
+    V1, T1 = tet_grid_cube(n, 1.0, (0.0, gap, 0.0))
+    V2, T2 = tet_grid_cube(n, 1.0, (0.0, 1.0 + 2.0 * gap, 0.0))
+    bodies = []
+    for V, T in ((V1, T1), (V2, T2)):
+        V = V + rng.uniform(-jitter, jitter, V.shape)
+        bodies.append((V, boundary_faces(V, T), T))
+    Vg, Fg = ground_quad(0.0)
+    bodies.append((Vg, Fg, None))
+    return assemble(bodies)
+
+
+def icosphere_grid(seed: int = 0, per_axis: int = 4, spacing: float = 1.001, subdivisions: int = 3,
+                   radius: float = 0.5):
+    """per_axis^3 icospheres on a grid `spacing` apart and seeded N(0,1) body velocities.
+
+    Returns (soup, velocity [n_bodies,3] float64).
+    """
+    rng = np.random.default_rng(seed)
+    Vs, Fs = icosphere(subdivisions, radius)
+    bodies = []
+    for i in range(per_axis):
+        for j in range(per_axis):
+            for k in range(per_axis):
+                bodies.append((Vs + spacing * np.array([i, j, k], np.float64), Fs, None))
+    soup = assemble(bodies)
+    vel = rng.standard_normal((soup.n_bodies, 3))
+    return soup, vel
+
+
+# --------------------------------------------------------- pair streams ----
+def _orthonormal_frames(rng, n):
+    a = rng.standard_normal((n, 3))
+    a /= np.linalg.norm(a, axis=1, keepdims=True)
+    t = np.where(np.abs(a[:, :1]) < 0.9, np.array([[1.0, 0, 0]]), np.array([[0, 1.0, 0]]))
+    b = np.cross(a, t)
+    b /= np.linalg.norm(b, axis=1, keepdims=True)
+    c = np.cross(a, b)
+    return a, b, c
+
+
+def pair_stream(kind: str, n: int, seed: int = 0, dhat: float = 1.0e-3, scale: float = 0.05,
+                center_spread: float = 1.0):
+    """Synthetic active contacts with 4 private vertices each.
+
+    Distances d/dhat are log-uniform on [1e-3, 1) with 15% forced below
+    1e-2 (the smallest distance, 1e-6 at dhat = 1e-3, stays well above the
+    fp32 spacing of coordinates in the unit cube). PT feature mix: 50% face, 30% edge, 20% vertex. EE mix: 70%
+    interior, 20% endpoint, 10% near-parallel. Primitives have size
+    ~`scale`, positions spread over a cube of side `center_spread`.
+    Returns (x float32 [4n,3], contacts int32 [n,4]) with contacts[i] =
+    4i .. 4i+3 in the reference's slot order (PT: p, t1, t0, t2).
+    """
+    rng = np.random.default_rng(seed)
+    u = rng.random(n)
+    ratio = np.exp(np.log(1e-3) + (np.log(1.0) - np.log(1e-3)) * rng.random(n))
+    small = rng.random(n) < 0.15
+    ratio[small] = np.exp(np.log(1e-3) + (np.log(1e-2) - np.log(1e-3)) * rng.random(small.sum()))
+    d = ratio * dhat * 0.999
+    nrm, t1, t2 = _orthonormal_frames(rng, n)
+    centre = rng.uniform(0, center_spread, (n, 3))
+    L = scale * (0.5 + rng.random((n, 1)))
+    x = np.zeros((n, 4, 3))
+    if kind == "pt":
+        # triangle in the plane spanned by t1, t2 through `centre`
+        ang = rng.uniform(0, 2 * np.pi, (n, 1)) + np.array([[0.0, 2.1, 4.2]]) + rng.uniform(-0.4, 0.4, (n, 3))
+        rad = L * (0.6 + 0.4 * rng.random((n, 3)))
+        tri = centre[:, None, :] + rad[..., None] * (np.cos(ang)[..., None] * t1[:, None, :] +
+                                                      np.sin(ang)[..., None] * t2[:, None, :])
+        a, b, c = tri[:, 0], tri[:, 1], tri[:, 2]
+        face = u < 0.5
+        edge = (u >= 0.5) & (u < 0.8)
+        p = np.zeros((n, 3))
+        # face: above a random interior point along the face normal
+        w = rng.dirichlet([2, 2, 2], n)
+        fn = np.cross(b - a, c - a)
+        fn /= np.linalg.norm(fn, axis=1, keepdims=True)
+        p[face] = (w[:, :1] * a + w[:, 1:2] * b + w[:, 2:] * c)[face] + d[face, None] * fn[face]
+        # edge: outside edge ab, in the plane spanned by the outward edge normal and the face normal
+        s = rng.uniform(0.1, 0.9, n)[:, None]
+        q = a + s * (b - a)
+        eo = np.cross(b - a, fn)
+        eo /= np.linalg.norm(eo, axis=1, keepdims=True)
+        eo *= np.sign(np.einsum("ij,ij->i", eo, q - c))[:, None]
+        phi = rng.uniform(0.2, 1.3, n)[:, None]
+        p[edge] = (q + d[:, None] * (np.cos(phi) * eo + np.sin(phi) * fn))[edge]
+        # vertex: outside vertex a, along the outward bisector-ish direction
+        vo = (a - (b + c) / 2)
+        vo /= np.linalg.norm(vo, axis=1, keepdims=True)
+        vdir = vo + 0.3 * fn
+        vdir /= np.linalg.norm(vdir, axis=1, keepdims=True)
+        vert = u >= 0.8
+        p[vert] = (a + d[:, None] * vdir)[vert]
+        # record order (p, t1, t0, t2): slots 1..3 are the triangle (a, b, c)
+        x[:, 0], x[:, 1], x[:, 2], x[:, 3] = p, a, b, c
+    else:
+        # edge a along t1 through centre, edge b along a direction rotated about nrm
+        interior = u < 0.7
+        endpoint = (u >= 0.7) & (u < 0.9)
+        ang = rng.uniform(0.3, np.pi - 0.3, n)[:, None]
+        ang[~(interior | endpoint)] = rng.uniform(1e-4, 5e-3, (~(interior | endpoint)).sum())[:, None]
+        db = np.cos(ang) * t1 + np.sin(ang) * t2
+        sa = rng.uniform(0.2, 0.8, n)[:, None]
+        sb = rng.uniform(0.2, 0.8, n)[:, None]
+        a0 = centre - sa * L * t1
+        a1 = centre + (1 - sa) * L * t1
+        pb = centre + d[:, None] * nrm
+        b0 = pb - sb * L * db
+        b1 = pb + (1 - sb) * L * db
+        # endpoint: edge b ends just above edge a's interior and rises away
+        # from it, so its end point b1 is the closest feature
+        e = endpoint
+        rise = 0.5 * db + nrm
+        rise /= np.linalg.norm(rise, axis=1, keepdims=True)
+        b1[e] = pb[e]
+        b0[e] = pb[e] + L[e] * rise[e]
+        x[:, 0], x[:, 1], x[:, 2], x[:, 3] = a0, a1, b0, b1
+    contacts = np.arange(4 * n, dtype=np.int32).reshape(n, 4)
+    return x.reshape(-1, 3).astype(np.float32), contacts

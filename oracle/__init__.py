@@ -1,0 +1,291 @@
+"""CPU oracle for the IPC contact inner loop — TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
+``cpu_baseline`` leg may import this package, and only as the checker (or the
+timed CPU baseline), never as the thing the product runs.
+
+It wraps ``libipc_oracle.so`` (``ipc_oracle.hpp``/``ipc_oracle.cpp``), a C++
+restatement of the reference's algorithms:
+
+* ``contact.py:230-302`` closest point on a triangle, ``contact.py:199-336``
+  DCD pair sets, ``ccd/ccd.py:166-427`` CCD candidates and times of impact,
+  ``ccd/cubic_roots.py:1-193`` root isolation,
+  ``dynamic_contacts.py:19-246,374-398`` barrier gradient/Hessian/energy.
+* Absent dependencies restated from their published algorithms:
+  Warp's ``closest_point_edge_edge`` (Ericson 5.1.9) and the ``ipctkwp``
+  barrier, mollifier, NSD distance Hessians and beta/gamma (IPC, Li et al.
+  2020). Parity of those against the originals is unpinned; the tests pin
+  them by closed forms, brute force and invariants instead.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libipc_oracle.so")
+_lib = None
+
+F32P = ctypes.POINTER(ctypes.c_float)
+F64P = ctypes.POINTER(ctypes.c_double)
+I32P = ctypes.POINTER(ctypes.c_int)
+U8P = ctypes.POINTER(ctypes.c_uint8)
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (g++, no FMA contraction)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        c_int, c_dbl, c_flt, c_i64 = ctypes.c_int, ctypes.c_double, ctypes.c_float, ctypes.c_int64
+        L.oracle_closest_point_triangle.argtypes = [c_int, c_int, F64P, F64P, F64P, F64P, F64P, I32P]
+        L.oracle_closest_point_edge_edge.argtypes = [c_int, c_int, F64P, F64P, F64P, F64P, c_dbl, F64P]
+        L.oracle_barrier.argtypes = [c_int, F64P, c_dbl, c_dbl, F64P]
+        L.oracle_feature_hessian.argtypes = [c_int, I32P, F64P, c_int, F64P, F64P]
+        L.oracle_feature_hessian.restype = c_dbl
+        L.oracle_feature_ee.argtypes = [c_int, F64P, c_dbl, I32P, I32P]
+        L.oracle_ee_mollifier.argtypes = [F64P, c_dbl, F64P, F64P]
+        L.oracle_ee_mollifier.restype = c_dbl
+        L.oracle_contact_hessian.argtypes = [c_int, c_int, c_int, F32P, F32P, I32P, c_int, c_flt, c_dbl, c_int,
+                                             I32P, I32P, F64P, F64P, U8P, c_int]
+        L.oracle_contact_energy.argtypes = [c_int, c_int, c_int, F32P, F32P, I32P, c_int, c_flt, c_dbl, c_int]
+        L.oracle_contact_energy.restype = c_dbl
+        L.oracle_cubic_roots.argtypes = [c_int, F64P, c_dbl, c_dbl, F64P, I32P]
+        L.oracle_cubic_roots.restype = c_int
+        L.oracle_coplanarity_cubic.argtypes = [c_int, F64P, F64P, F64P]
+        L.oracle_collision_time.argtypes = [c_int, c_int, c_int, c_int, F32P, F32P, I32P, F64P, c_int]
+        L.oracle_collision_time.restype = c_int
+        L.oracle_dcd_ee.argtypes = [F32P, I32P, c_int, I32P, c_flt, c_flt, c_int, I32P, I32P, c_i64]
+        L.oracle_dcd_ee.restype = c_i64
+        L.oracle_dcd_pt.argtypes = [F32P, c_int, I32P, c_int, I32P, c_flt, c_flt, c_int, I32P, I32P, c_i64]
+        L.oracle_dcd_pt.restype = c_i64
+        L.oracle_ccd_candidates_ee.argtypes = [F32P, F32P, I32P, c_int, I32P, c_flt, c_int, I32P, I32P, c_i64]
+        L.oracle_ccd_candidates_ee.restype = c_i64
+        L.oracle_ccd_candidates_pt.argtypes = [F32P, F32P, c_int, I32P, c_int, I32P, c_flt, c_int, I32P, I32P, c_i64]
+        L.oracle_ccd_candidates_pt.restype = c_i64
+        L.oracle_max_threads.restype = c_int
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def max_threads() -> int:
+    return int(lib().oracle_max_threads())
+
+
+# ------------------------------------------------------------- primitives ----
+def closest_point_triangle(p, a, b, c, double=False):
+    """contact.py:230-302 -> (weights_and_dist [n,3], feature [n])."""
+    p, a, b, c = (_f64(np.atleast_2d(v)) for v in (p, a, b, c))
+    n = p.shape[0]
+    out = np.zeros((n, 3))
+    typ = np.zeros(n, dtype=np.int32)
+    lib().oracle_closest_point_triangle(int(double), n, _p(p, F64P), _p(a, F64P), _p(b, F64P), _p(c, F64P),
+                                        _p(out, F64P), _p(typ, I32P))
+    return out, typ
+
+
+def closest_point_edge_edge(p1, q1, p2, q2, eps=1e-6, double=False):
+    """Warp closest_point_edge_edge stand-in -> [n,3] (s, t, dist)."""
+    p1, q1, p2, q2 = (_f64(np.atleast_2d(v)) for v in (p1, q1, p2, q2))
+    n = p1.shape[0]
+    out = np.zeros((n, 3))
+    lib().oracle_closest_point_edge_edge(int(double), n, _p(p1, F64P), _p(q1, F64P), _p(p2, F64P), _p(q2, F64P),
+                                         float(eps), _p(out, F64P))
+    return out
+
+
+def barrier(x, D, kappa):
+    """(b, db/dx, d2b/dx2) with x = d^2, D = dhat^2."""
+    x = _f64(np.atleast_1d(x))
+    out = np.zeros((x.shape[0], 3))
+    lib().oracle_barrier(x.shape[0], _p(x, F64P), float(D), float(kappa), _p(out, F64P))
+    return out
+
+
+FEATURE = {"PP": 0, "PE": 1, "PT": 2, "EE": 3}
+
+
+def feature_hessian(kind, slots, x12, nsd=False):
+    """Squared distance of a feature, its gradient and (NSD part of) Hessian."""
+    slots = _i32(list(slots) + [-1] * (4 - len(slots)))
+    x12 = _f64(np.asarray(x12).reshape(12))
+    g = np.zeros(12)
+    H = np.zeros((12, 12))
+    k = FEATURE[kind] if isinstance(kind, str) else int(kind)
+    d2 = lib().oracle_feature_hessian(k, _p(slots, I32P), _p(x12, F64P), int(nsd), _p(g, F64P), _p(H, F64P))
+    return d2, g, H
+
+
+def feature_ee(x12, parallel_eps, double=False):
+    kind = ctypes.c_int()
+    slots = np.zeros(4, dtype=np.int32)
+    x12 = _f64(np.asarray(x12).reshape(12))
+    lib().oracle_feature_ee(int(double), _p(x12, F64P), float(parallel_eps), ctypes.byref(kind), _p(slots, I32P))
+    return kind.value, slots
+
+
+def ee_mollifier(x12, eps_x):
+    x12 = _f64(np.asarray(x12).reshape(12))
+    g = np.zeros(12)
+    H = np.zeros((12, 12))
+    m = lib().oracle_ee_mollifier(_p(x12, F64P), float(eps_x), _p(g, F64P), _p(H, F64P))
+    return m, g, H
+
+
+# ---------------------------------------------------------- contact kernels ----
+# Modes: "mixed" classifies in float like the reference's FP32 mode and takes
+# derivatives of the classified feature in double (the HIP library's policy);
+# "ref32" also uses the float distance for the barrier argument, as the
+# reference does; "fp64" classifies in double too (the all-FP64 oracle).
+_MODES = {"mixed": (0, 1), "ref32": (0, 0), "fp64": (1, 1)}
+
+
+def contact_hessian(kind, x, contacts, l0, kappa, xcs=None, offset=0, n_total=None, mode="mixed", threads=1):
+    """contact_hessian_ee / _pt (dynamic_contacts.py:151-246).
+
+    Returns (rows, cols, vals [16*n_total,3,3] float64, b [V,3] force, active [n]).
+    """
+    x = _f32(x).reshape(-1, 3)
+    xcs = x if xcs is None else _f32(xcs).reshape(-1, 3)
+    contacts = _i32(contacts).reshape(-1, 4)
+    n = contacts.shape[0]
+    n_total = offset + n if n_total is None else n_total
+    rows = np.zeros(16 * n_total, dtype=np.int32)
+    cols = np.zeros(16 * n_total, dtype=np.int32)
+    vals = np.zeros((16 * n_total, 3, 3))
+    b = np.zeros((x.shape[0], 3))
+    active = np.zeros(n, dtype=np.uint8)
+    prec, pol = _MODES[mode]
+    lib().oracle_contact_hessian(int(kind == "ee"), prec, pol, _p(x, F32P), _p(xcs, F32P), _p(contacts, I32P), n,
+                                 float(l0), float(kappa), int(offset), _p(rows, I32P), _p(cols, I32P), _p(vals, F64P),
+                                 _p(b, F64P), _p(active, U8P), int(threads))
+    return rows, cols, vals, b, active.astype(bool)
+
+
+def contact_energy(kind, x, contacts, l0, kappa, xcs=None, mode="mixed", threads=1):
+    """contact_energy_ee / _pt (dynamic_contacts.py:374-398), double sum."""
+    x = _f32(x).reshape(-1, 3)
+    xcs = x if xcs is None else _f32(xcs).reshape(-1, 3)
+    contacts = _i32(contacts).reshape(-1, 4)
+    prec, pol = _MODES[mode]
+    return float(lib().oracle_contact_energy(int(kind == "ee"), prec, pol, _p(x, F32P), _p(xcs, F32P),
+                                             _p(contacts, I32P), contacts.shape[0], float(l0), float(kappa),
+                                             int(threads)))
+
+
+# --------------------------------------------------------------------- CCD ----
+def cubic_roots(coef, x0=0.0, x1=1.0, double=True):
+    """ccd/cubic_roots.py:95-193 -> (n_roots, roots[3], capped)."""
+    coef = _f64(coef)
+    roots = np.zeros(3)
+    capped = ctypes.c_int(0)
+    n = lib().oracle_cubic_roots(int(double), _p(coef, F64P), float(x0), float(x1), _p(roots, F64P),
+                                 ctypes.byref(capped))
+    return n, roots, capped.value
+
+
+def coplanarity_cubic(p0, p1, relative=False):
+    p0 = _f64(np.asarray(p0).reshape(12))
+    p1 = _f64(np.asarray(p1).reshape(12))
+    coef = np.zeros(4)
+    lib().oracle_coplanarity_cubic(int(relative), _p(p0, F64P), _p(p1, F64P), _p(coef, F64P))
+    return coef
+
+
+def collision_time(kind, x0, x1, quads, relative=False, double=True, threads=1):
+    """pt_collision_time / ee_collision_time per candidate -> (toi [n], n_capped)."""
+    x0 = _f32(x0).reshape(-1, 3)
+    x1 = _f32(x1).reshape(-1, 3)
+    quads = _i32(quads).reshape(-1, 4)
+    toi = np.zeros(quads.shape[0])
+    capped = lib().oracle_collision_time(int(kind == "ee"), int(relative), int(double), quads.shape[0],
+                                         _p(x0, F32P), _p(x1, F32P), _p(quads, I32P), _p(toi, F64P), int(threads))
+    return toi, capped
+
+
+# --------------------------------------------------------------- pair sets ----
+def _grow_call(fn, *args):
+    cap = 1 << 12
+    while True:
+        rec = np.zeros((cap, 4), dtype=np.int32)
+        e = np.zeros((cap, 2), dtype=np.int32)
+        n = fn(*args, _p(rec, I32P), _p(e, I32P), cap)
+        if n <= cap:
+            return rec[:n].copy(), e[:n].copy()
+        cap = 1 << int(n - 1).bit_length()
+
+
+def dcd_ee(x, edges, body, thickness, buffer, exclude_same_body=False):
+    """Exact EE contact set of edge_edge_collision (contact.py:199-229)."""
+    x = _f32(x).reshape(-1, 3)
+    edges = _i32(edges).reshape(-1)
+    body = _i32(body)
+    return _grow_call(lib().oracle_dcd_ee, _p(x, F32P), _p(edges, I32P), edges.size // 2, _p(body, I32P),
+                      float(thickness), float(buffer), int(exclude_same_body))
+
+
+def dcd_pt(x, triangles, body, thickness, buffer, exclude_same_body=False):
+    """Exact PT contact set of point_triangle_collision (contact.py:304-336)."""
+    x = _f32(x).reshape(-1, 3)
+    tris = _i32(triangles).reshape(-1)
+    body = _i32(body)
+    return _grow_call(lib().oracle_dcd_pt, _p(x, F32P), x.shape[0], _p(tris, I32P), tris.size // 3, _p(body, I32P),
+                      float(thickness), float(buffer), int(exclude_same_body))
+
+
+def ccd_candidates_ee(x0, x1, edges, body, padding=1e-7, exclude_same_body=False):
+    x0 = _f32(x0).reshape(-1, 3)
+    x1 = _f32(x1).reshape(-1, 3)
+    edges = _i32(edges).reshape(-1)
+    body = _i32(body)
+    return _grow_call(lib().oracle_ccd_candidates_ee, _p(x0, F32P), _p(x1, F32P), _p(edges, I32P), edges.size // 2,
+                      _p(body, I32P), float(padding), int(exclude_same_body))
+
+
+def ccd_candidates_pt(x0, x1, triangles, body, padding=1e-7, exclude_same_body=False):
+    x0 = _f32(x0).reshape(-1, 3)
+    x1 = _f32(x1).reshape(-1, 3)
+    tris = _i32(triangles).reshape(-1)
+    body = _i32(body)
+    return _grow_call(lib().oracle_ccd_candidates_pt, _p(x0, F32P), _p(x1, F32P), x0.shape[0], _p(tris, I32P),
+                      tris.size // 3, _p(body, I32P), float(padding), int(exclude_same_body))
+
+
+def ccd_toi(x0, x1, edges, triangles, body, padding=1e-7, exclude_same_body=False, relative=False, threads=1):
+    """Raw minimum TOI over EE and PT candidates (collision_free_step before the 0.9 factor)."""
+    ee, _ = ccd_candidates_ee(x0, x1, edges, body, padding, exclude_same_body)
+    pt, _ = ccd_candidates_pt(x0, x1, triangles, body, padding, exclude_same_body)
+    t = 1.0
+    if len(ee):
+        te, _ = collision_time("ee", x0, x1, ee, relative=relative, threads=threads)
+        t = min(t, float(te.min()))
+    if len(pt):
+        tp, _ = collision_time("pt", x0, x1, pt, relative=relative, threads=threads)
+        t = min(t, float(tp.min()))
+    return t

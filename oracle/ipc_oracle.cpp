@@ -1,0 +1,378 @@
+// TEST INFRASTRUCTURE ONLY: C entry points of the CPU oracle (see
+// ipc_oracle.hpp). Called through ctypes by tests/, __graft_entry__.smoke()
+// and bench.py's cpu_baseline leg; never by the product.
+#include "ipc_oracle.hpp"
+
+#include <omp.h>
+
+#include <cstdint>
+#include <utility>
+
+using namespace ipco;
+
+namespace {
+
+struct Box { float lo[3], hi[3]; int id; };
+
+inline bool overlap(const Box& a, const Box& b) {
+  // Inclusive AABB test (the reference's padded-overlap predicate,
+  // fixed_bvh_contacts.py:70-80 with padding already folded into the boxes).
+  for (int k = 0; k < 3; ++k)
+    if (a.lo[k] > b.hi[k] || b.lo[k] > a.hi[k]) return false;
+  return true;
+}
+
+// Sweep-and-prune on x. Calls fn(a_id, b_id) once for every overlapping pair
+// with one box from A and one from B (or every unordered pair when self=true).
+template <typename Fn>
+void sweep_pairs(std::vector<Box> A, std::vector<Box> B, bool self, Fn fn) {
+  auto by_lo = [](const Box& u, const Box& v) { return u.lo[0] < v.lo[0] || (u.lo[0] == v.lo[0] && u.id < v.id); };
+  std::sort(A.begin(), A.end(), by_lo);
+  if (self) {
+    std::vector<const Box*> active;
+    for (const Box& a : A) {
+      size_t w = 0;
+      for (size_t r = 0; r < active.size(); ++r)
+        if (!(active[r]->hi[0] < a.lo[0])) active[w++] = active[r];
+      active.resize(w);
+      for (const Box* b : active)
+        if (overlap(a, *b)) fn(a.id, b->id);
+      active.push_back(&a);
+    }
+    return;
+  }
+  std::sort(B.begin(), B.end(), by_lo);
+  std::vector<const Box*> actA, actB;
+  size_t ia = 0, ib = 0;
+  auto prune = [](std::vector<const Box*>& act, float x) {
+    size_t w = 0;
+    for (size_t r = 0; r < act.size(); ++r)
+      if (!(act[r]->hi[0] < x)) act[w++] = act[r];
+    act.resize(w);
+  };
+  while (ia < A.size() || ib < B.size()) {
+    bool takeA = ib >= B.size() || (ia < A.size() && A[ia].lo[0] <= B[ib].lo[0]);
+    if (takeA) {
+      const Box& a = A[ia++];
+      prune(actB, a.lo[0]);
+      for (const Box* b : actB) if (overlap(a, *b)) fn(a.id, b->id);
+      actA.push_back(&a);
+    } else {
+      const Box& b = B[ib++];
+      prune(actA, b.lo[0]);
+      for (const Box* a : actA) if (overlap(*a, b)) fn(a->id, b.id);
+      actB.push_back(&b);
+    }
+  }
+}
+
+inline V3<float> ld(const float* x, int i) { return load3<float>(x, i); }
+
+inline Box box_of(const V3<float>* pts, int n, float pad, int id) {
+  Box b;
+  V3<float> lo = pts[0], hi = pts[0];
+  for (int k = 1; k < n; ++k) { lo = vmin(lo, pts[k]); hi = vmax(hi, pts[k]); }
+  b.lo[0] = lo.x - pad; b.lo[1] = lo.y - pad; b.lo[2] = lo.z - pad;
+  b.hi[0] = hi.x + pad; b.hi[1] = hi.y + pad; b.hi[2] = hi.z + pad;
+  b.id = id;
+  return b;
+}
+
+struct Out {
+  int* rec; int* ee; int64_t cap; int64_t n;
+  void push(int a, int b, int c, int d, int e0, int e1) {
+    if (n < cap) {
+      rec[4 * n] = a; rec[4 * n + 1] = b; rec[4 * n + 2] = c; rec[4 * n + 3] = d;
+      if (ee) { ee[2 * n] = e0; ee[2 * n + 1] = e1; }
+    }
+    ++n;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+// ------------------------------------------------------------ primitives ----
+void oracle_closest_point_triangle(int use_double, int n, const double* p, const double* a, const double* b,
+                                   const double* c, double* out, int* type) {
+  for (int i = 0; i < n; ++i) {
+    if (use_double) {
+      auto r = closest_point_triangle(mk(p[3 * i], p[3 * i + 1], p[3 * i + 2]), mk(a[3 * i], a[3 * i + 1], a[3 * i + 2]),
+                                      mk(b[3 * i], b[3 * i + 1], b[3 * i + 2]), mk(c[3 * i], c[3 * i + 1], c[3 * i + 2]));
+      out[3 * i] = r.wa; out[3 * i + 1] = r.wb; out[3 * i + 2] = r.dist; type[i] = r.type;
+    } else {
+      auto f = [](const double* v, int i) { return mk(float(v[3 * i]), float(v[3 * i + 1]), float(v[3 * i + 2])); };
+      auto r = closest_point_triangle(f(p, i), f(a, i), f(b, i), f(c, i));
+      out[3 * i] = r.wa; out[3 * i + 1] = r.wb; out[3 * i + 2] = r.dist; type[i] = r.type;
+    }
+  }
+}
+
+void oracle_closest_point_edge_edge(int use_double, int n, const double* p1, const double* q1, const double* p2,
+                                    const double* q2, double eps, double* out) {
+  for (int i = 0; i < n; ++i) {
+    if (use_double) {
+      auto g = [](const double* v, int i) { return mk(v[3 * i], v[3 * i + 1], v[3 * i + 2]); };
+      auto r = closest_point_edge_edge(g(p1, i), g(q1, i), g(p2, i), g(q2, i), eps);
+      out[3 * i] = r.s; out[3 * i + 1] = r.t; out[3 * i + 2] = r.dist;
+    } else {
+      auto f = [](const double* v, int i) { return mk(float(v[3 * i]), float(v[3 * i + 1]), float(v[3 * i + 2])); };
+      auto r = closest_point_edge_edge(f(p1, i), f(q1, i), f(p2, i), f(q2, i), float(eps));
+      out[3 * i] = r.s; out[3 * i + 1] = r.t; out[3 * i + 2] = r.dist;
+    }
+  }
+}
+
+void oracle_barrier(int n, const double* x, double D, double kappa, double* out) {
+  for (int i = 0; i < n; ++i) {
+    Barrier b = barrier(x[i], D, kappa);
+    out[3 * i] = b.b; out[3 * i + 1] = b.db; out[3 * i + 2] = b.d2b;
+  }
+}
+
+// kind: 0 PP, 1 PE, 2 PT, 3 EE; nsd: 0 full Hessian of d^2, 1 its NSD part
+// (PP: zero). Returns the feature's squared distance.
+double oracle_feature_hessian(int kind, const int* slot, const double* x12, int nsd, double* g, double* H) {
+  Feature f{kind, {slot[0], slot[1], slot[2], slot[3]}};
+  V3<double> x[4];
+  for (int k = 0; k < 4; ++k) x[k] = mk(x12[3 * k], x12[3 * k + 1], x12[3 * k + 2]);
+  return nsd ? feature_grad_nsd(f, x, g, H) : feature_hessian(f, x, g, H);
+}
+
+// Feature chosen by the EE / PT dispatch, in float (prec=0) or double.
+// Writes kind and slots; returns nothing.
+void oracle_feature_ee(int use_double, const double* x12, double parallel_eps, int* kind, int* slot) {
+  Feature f;
+  if (use_double) {
+    auto g = [&](int k) { return mk(x12[3 * k], x12[3 * k + 1], x12[3 * k + 2]); };
+    f = ee_feature(g(0), g(1), g(2), g(3), parallel_eps);
+  } else {
+    auto g = [&](int k) { return mk(float(x12[3 * k]), float(x12[3 * k + 1]), float(x12[3 * k + 2])); };
+    f = ee_feature(g(0), g(1), g(2), g(3), float(parallel_eps));
+  }
+  *kind = f.kind;
+  for (int k = 0; k < 4; ++k) slot[k] = f.slot[k];
+}
+
+// Mollifier value, gradient and PSD Hessian with the branch decided in double.
+double oracle_ee_mollifier(const double* x12, double eps_x, double* g, double* H) {
+  V3<double> x[4];
+  for (int k = 0; k < 4; ++k) x[k] = mk(x12[3 * k], x12[3 * k + 1], x12[3 * k + 2]);
+  V3<double> cr = cross(x[1] - x[0], x[3] - x[2]);
+  bool active = eps_x > 0 && dot(cr, cr) < eps_x;
+  Mollifier m = ee_mollifier(x, eps_x, active);
+  std::memcpy(g, m.g, sizeof(m.g));
+  std::memcpy(H, m.H, sizeof(m.H));
+  return m.m;
+}
+
+// --------------------------------------------------------- contact kernels ----
+// contact_hessian_ee / contact_hessian_pt (dynamic_contacts.py:151-246).
+// Writes the 16 triplets of contact i at (offset + i) * 16 + ii * 4 + jj,
+// accumulates the force into b (double, V x 3) and sets active[i].
+// prec: 0 = classify in float (the reference's FP32 mode), 1 = double.
+void oracle_contact_hessian(int is_ee, int prec, int d2_policy, const float* x, const float* xcs, const int* contacts,
+                            int n, float l0, double kappa, int offset, int* rows, int* cols, double* vals, double* b,
+                            unsigned char* active, int threads) {
+  std::vector<PairResult> res(n > 0 ? n : 0);
+#pragma omp parallel for num_threads(threads > 0 ? threads : 1) schedule(dynamic, 64)
+  for (int i = 0; i < n; ++i) {
+    const int* c = contacts + 4 * i;
+    if (is_ee) res[i] = prec ? contact_ee<double>(x, xcs, c, l0, kappa, d2_policy) : contact_ee<float>(x, xcs, c, l0, kappa, d2_policy);
+    else res[i] = prec ? contact_pt<double>(x, c, l0, kappa, d2_policy) : contact_pt<float>(x, c, l0, kappa, d2_policy);
+  }
+  for (int i = 0; i < n; ++i) {
+    const int* c = contacts + 4 * i;
+    const PairResult& r = res[i];
+    if (active) active[i] = r.active ? 1 : 0;
+    if (!r.active) continue;
+    for (int ii = 0; ii < 4; ++ii) {
+      for (int k = 0; k < 3; ++k) b[3 * c[ii] + k] += r.grad_force[3 * ii + k];
+      for (int jj = 0; jj < 4; ++jj) {
+        int64_t idx = int64_t(offset + i) * 16 + ii * 4 + jj;
+        if (rows) rows[idx] = c[ii];
+        if (cols) cols[idx] = c[jj];
+        if (vals)
+          for (int r3 = 0; r3 < 3; ++r3)
+            for (int c3 = 0; c3 < 3; ++c3) vals[idx * 9 + r3 * 3 + c3] = r.H[(ii * 3 + r3) * 12 + jj * 3 + c3];
+      }
+    }
+  }
+}
+
+// contact_energy_ee / contact_energy_pt (dynamic_contacts.py:374-398):
+// sum over active contacts of m * b (EE) or b (PT), accumulated in double
+// with Kahan summation in contact order.
+double oracle_contact_energy(int is_ee, int prec, int d2_policy, const float* x, const float* xcs, const int* contacts,
+                             int n, float l0, double kappa, int threads) {
+  std::vector<double> e(n > 0 ? n : 0, 0.0);
+#pragma omp parallel for num_threads(threads > 0 ? threads : 1) schedule(dynamic, 64)
+  for (int i = 0; i < n; ++i) {
+    const int* c = contacts + 4 * i;
+    PairResult r;
+    if (is_ee) r = prec ? contact_ee<double>(x, xcs, c, l0, kappa, d2_policy) : contact_ee<float>(x, xcs, c, l0, kappa, d2_policy);
+    else r = prec ? contact_pt<double>(x, c, l0, kappa, d2_policy) : contact_pt<float>(x, c, l0, kappa, d2_policy);
+    e[i] = r.active ? r.energy : 0.0;
+  }
+  double s = 0.0, comp = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double y = e[i] - comp, t = s + y;
+    comp = (t - s) - y;
+    s = t;
+  }
+  return s;
+}
+
+// ------------------------------------------------------------------- CCD ----
+int oracle_cubic_roots(int use_double, const double* coef, double x0, double x1, double* roots, int* capped) {
+  if (use_double) return cubic_roots(coef, x0, x1, roots, capped);
+  float cf[4] = {float(coef[0]), float(coef[1]), float(coef[2]), float(coef[3])}, rf[3];
+  int n = cubic_roots(cf, float(x0), float(x1), rf, capped);
+  for (int k = 0; k < 3; ++k) roots[k] = rf[k];
+  return n;
+}
+
+void oracle_coplanarity_cubic(int relative, const double* p0_12, const double* p1_12, double* coef) {
+  V3<double> a[4], b[4], u[3], v[3];
+  for (int k = 0; k < 4; ++k) { a[k] = mk(p0_12[3 * k], p0_12[3 * k + 1], p0_12[3 * k + 2]); b[k] = mk(p1_12[3 * k], p1_12[3 * k + 1], p1_12[3 * k + 2]); }
+  if (relative) coplanarity_cubic_relative(a, b, coef, u, v);
+  else coplanarity_cubic_absolute(a, b, coef);
+}
+
+// pt_collision_time / ee_collision_time over a candidate list of vertex
+// quadruples (PT: point, t0, t1, t2; EE: a0, a1, b0, b1). Returns the number
+// of find_closed iteration caps hit.
+int oracle_collision_time(int is_ee, int relative, int use_double, int n, const float* x0, const float* x1,
+                          const int* quads, double* toi, int threads) {
+  int capped_total = 0;
+#pragma omp parallel for num_threads(threads > 0 ? threads : 1) schedule(dynamic, 64) reduction(+ : capped_total)
+  for (int i = 0; i < n; ++i) {
+    const int* q = quads + 4 * i;
+    int capped = 0;
+    if (use_double) {
+      V3<double> a[4], b[4];
+      for (int k = 0; k < 4; ++k) { a[k] = load3<double>(x0, q[k]); b[k] = load3<double>(x1, q[k]); }
+      toi[i] = collision_time<double>(is_ee != 0, relative != 0, a, b, &capped);
+    } else {
+      V3<float> a[4], b[4];
+      for (int k = 0; k < 4; ++k) { a[k] = load3<float>(x0, q[k]); b[k] = load3<float>(x1, q[k]); }
+      toi[i] = collision_time<float>(is_ee != 0, relative != 0, a, b, &capped);
+    }
+    capped_total += capped;
+  }
+  return capped_total;
+}
+
+// ------------------------------------------------------------------- DCD ----
+// edge_edge_collision (contact.py:199-229) with the BVH replaced by an exact
+// overlap enumeration of the same inflated boxes (edge_aabb, contact.py:30-37).
+// Records (edges[i], edges[j]) with i < j and e0e1 = (i, j). Returns the
+// number of contacts (may exceed cap; only cap are written).
+int64_t oracle_dcd_ee(const float* x, const int* edges, int n_edges, const int* body, float thickness, float buffer,
+                      int exclude_same_body, int* rec, int* e0e1, int64_t cap) {
+  float pad = thickness + buffer;
+  std::vector<Box> boxes(n_edges);
+  for (int i = 0; i < n_edges; ++i) {
+    V3<float> p[2] = {ld(x, edges[2 * i]), ld(x, edges[2 * i + 1])};
+    boxes[i] = box_of(p, 2, pad, i);
+  }
+  std::vector<std::pair<int, int>> hits;
+  sweep_pairs(boxes, {}, true, [&](int a, int b) { hits.emplace_back(std::min(a, b), std::max(a, b)); });
+  std::sort(hits.begin(), hits.end());
+  Out out{rec, e0e1, cap, 0};
+  for (auto& h : hits) {
+    int i = h.first, j = h.second;
+    int a1 = edges[2 * i], a2 = edges[2 * i + 1], b1 = edges[2 * j], b2 = edges[2 * j + 1];
+    bool connected = a1 == b1 || a1 == b2 || a2 == b1 || a2 == b2;
+    bool same = exclude_same_body && body[a1] == body[b1];
+    if (connected || same) continue;
+    auto r = closest_point_edge_edge(ld(x, a1), ld(x, a2), ld(x, b1), ld(x, b2), 1.0e-6f);
+    if (r.dist < thickness * 2.0f) out.push(a1, a2, b1, b2, i, j);
+  }
+  return out.n;
+}
+
+// point_triangle_collision (contact.py:304-336): every node against every
+// triangle whose bounds meet the node's box +-2(thickness+buffer). Records
+// (p, tri[1], tri[0], tri[2]) with e0e1 = (p, triangle), ordered by (p, triangle).
+int64_t oracle_dcd_pt(const float* x, int n_nodes, const int* tris, int n_tris, const int* body, float thickness,
+                      float buffer, int exclude_same_body, int* rec, int* e0e1, int64_t cap) {
+  float pad = (thickness + buffer) * 2.0f;
+  std::vector<Box> pb(n_nodes), tb(n_tris);
+  for (int i = 0; i < n_nodes; ++i) { V3<float> p = ld(x, i); pb[i] = box_of(&p, 1, pad, i); }
+  for (int j = 0; j < n_tris; ++j) {
+    V3<float> p[3] = {ld(x, tris[3 * j]), ld(x, tris[3 * j + 1]), ld(x, tris[3 * j + 2])};
+    tb[j] = box_of(p, 3, 0.0f, j);
+  }
+  std::vector<std::pair<int, int>> hits;
+  sweep_pairs(pb, tb, false, [&](int a, int b) { hits.emplace_back(a, b); });
+  std::sort(hits.begin(), hits.end());
+  Out out{rec, e0e1, cap, 0};
+  for (auto& h : hits) {
+    int i = h.first, j = h.second;
+    int t1 = tris[3 * j], t2 = tris[3 * j + 1], t3 = tris[3 * j + 2];
+    if (t1 == i || t2 == i || t3 == i) continue;
+    if (exclude_same_body && body[i] == body[t1]) continue;
+    auto r = closest_point_triangle(ld(x, i), ld(x, t1), ld(x, t2), ld(x, t3));
+    if (r.dist < thickness * 2.0f) out.push(i, t2, t1, t3, i, j);
+  }
+  return out.n;
+}
+
+// ------------------------------------------------------ CCD candidate sets ----
+// edge_edge_toi's candidates (ccd/ccd.py:166-186,397-427): swept edge boxes
+// padded by `padding`, i < j, not connected, not same body. Records
+// (a0, a1, b0, b1) ordered by (i, j).
+int64_t oracle_ccd_candidates_ee(const float* x0, const float* x1, const int* edges, int n_edges, const int* body,
+                                 float padding, int exclude_same_body, int* rec, int* e0e1, int64_t cap) {
+  std::vector<Box> boxes(n_edges);
+  for (int i = 0; i < n_edges; ++i) {
+    int a = edges[2 * i], b = edges[2 * i + 1];
+    V3<float> p[4] = {ld(x0, a), ld(x0, b), ld(x1, a), ld(x1, b)};
+    boxes[i] = box_of(p, 4, padding, i);
+  }
+  std::vector<std::pair<int, int>> hits;
+  sweep_pairs(boxes, {}, true, [&](int a, int b) { hits.emplace_back(std::min(a, b), std::max(a, b)); });
+  std::sort(hits.begin(), hits.end());
+  Out out{rec, e0e1, cap, 0};
+  for (auto& h : hits) {
+    int i = h.first, j = h.second;
+    int a0 = edges[2 * i], a1 = edges[2 * i + 1], b0 = edges[2 * j], b1 = edges[2 * j + 1];
+    if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) continue;
+    if (exclude_same_body && body[a0] == body[b0]) continue;
+    out.push(a0, a1, b0, b1, i, j);
+  }
+  return out.n;
+}
+
+// point_triangle_toi's candidates (ccd/ccd.py:188-210,360-394): swept point
+// box +-padding against padded swept triangle boxes, not incident, not same
+// body. Records (p, t0, t1, t2) ordered by (p, triangle).
+int64_t oracle_ccd_candidates_pt(const float* x0, const float* x1, int n_nodes, const int* tris, int n_tris,
+                                 const int* body, float padding, int exclude_same_body, int* rec, int* e0e1,
+                                 int64_t cap) {
+  std::vector<Box> pb(n_nodes), tb(n_tris);
+  for (int i = 0; i < n_nodes; ++i) { V3<float> p[2] = {ld(x0, i), ld(x1, i)}; pb[i] = box_of(p, 2, padding, i); }
+  for (int j = 0; j < n_tris; ++j) {
+    V3<float> p[6];
+    for (int k = 0; k < 3; ++k) { p[k] = ld(x0, tris[3 * j + k]); p[3 + k] = ld(x1, tris[3 * j + k]); }
+    tb[j] = box_of(p, 6, padding, j);
+  }
+  std::vector<std::pair<int, int>> hits;
+  sweep_pairs(pb, tb, false, [&](int a, int b) { hits.emplace_back(a, b); });
+  std::sort(hits.begin(), hits.end());
+  Out out{rec, e0e1, cap, 0};
+  for (auto& h : hits) {
+    int i = h.first, j = h.second;
+    int t0 = tris[3 * j], t1 = tris[3 * j + 1], t2 = tris[3 * j + 2];
+    if (t0 == i || t1 == i || t2 == i) continue;
+    if (exclude_same_body && body[i] == body[t0]) continue;
+    out.push(i, t0, t1, t2, i, j);
+  }
+  return out.n;
+}
+
+int oracle_max_threads(void) { return omp_get_max_threads(); }
+
+}  // extern "C"

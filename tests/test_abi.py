@@ -1,0 +1,46 @@
+"""The C-ABI library loads and exports exactly the entry points include/*.h
+declares; argument validation rejects bad calls before any launch."""
+import ctypes
+import glob
+import os
+import re
+
+import pytest
+
+import ipc_warp_amd
+from ipc_warp_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        src = open(h).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        names |= set(re.findall(r"\b(ipc_[a-z0-9_]+)\s*\(", src))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = ipc_warp_amd.lib()
+    names = declared()
+    assert names, "no declarations found"
+    for n in sorted(names):
+        assert hasattr(L, n), f"{n} declared in include/ but not exported"
+    assert names == set(_lib.EXPORTED), "ctypes signature table out of sync with include/"
+
+
+def test_abi_version_and_validation():
+    L = ipc_warp_amd.lib()
+    assert L.ipc_abi_version() == 1
+    # negative sizes are rejected on the host before any HIP call
+    st = L.ipc_contact_hessian_pt(None, 4, None, -1, ctypes.c_float(1e-3), 1e8, 0, None, None, None, None, None)
+    assert st == -100000
+    assert b"negative" in L.ipc_last_error()
+    st = L.ipc_collision_time_ee(None, None, 4, None, -5, None, None, None)
+    assert st == -100000
+    # zero-size calls are no-ops
+    assert L.ipc_contact_energy_pt(None, 4, None, 0, ctypes.c_float(1e-3), 1e8, None, None) == 0
+    with pytest.raises(_lib.IPCError):
+        _lib.check(-100000, "probe")

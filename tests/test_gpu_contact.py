@@ -1,0 +1,155 @@
+"""GPU parity of the barrier kernels (dynamic_contacts.py:151-246,374-398)
+and the contact distance kernels (contact.py:88-161) against the oracle.
+
+Bar: rows/cols bit-exact; every 3x3 block within 1e-5 of the pair's largest
+Hessian entry; forces within 1e-5 of the largest force; energy within 1e-9
+relative; fp32 distances and dispatched features bit-exact."""
+import numpy as np
+import pytest
+import torch
+
+from ipc_warp_amd import ops, scenes
+
+pytestmark = pytest.mark.gpu
+
+L0 = np.float32(1e-3)
+KAPPA = 1e8
+TOL = 1e-5
+
+
+def _run_hessian(dev, kind, x, contacts, xcs=None, offset=0, n_total=None):
+    n = contacts.shape[0]
+    n_total = offset + n if n_total is None else n_total
+    xt = torch.from_numpy(x).to(dev)
+    ct = torch.from_numpy(contacts).to(dev)
+    rows = torch.full((16 * n_total,), -7, dtype=torch.int32, device=dev)
+    cols = torch.full((16 * n_total,), -7, dtype=torch.int32, device=dev)
+    vals = torch.full((16 * n_total, 3, 3), float("nan"), dtype=torch.float32, device=dev)
+    b = torch.zeros_like(xt)
+    if kind == "ee":
+        xc = torch.from_numpy(x if xcs is None else xcs).to(dev)
+        ops.contact_hessian_ee(xt, xc, ct, n, L0, KAPPA, rows, cols, vals, b, offset)
+    else:
+        ops.contact_hessian_pt(xt, ct, n, L0, KAPPA, rows, cols, vals, b, offset)
+    torch.cuda.synchronize()
+    return rows.cpu().numpy(), cols.cpu().numpy(), vals.cpu().numpy().astype(np.float64), b.cpu().numpy()
+
+
+def _compare(rows, cols, vals, b, ref, lo, hi):
+    r_rows, r_cols, r_vals, r_b, act = ref
+    assert np.array_equal(rows[16 * lo:16 * hi], r_rows[16 * lo:16 * hi])
+    assert np.array_equal(cols[16 * lo:16 * hi], r_cols[16 * lo:16 * hi])
+    gv = vals[16 * lo:16 * hi].reshape(hi - lo, -1)
+    rv = r_vals[16 * lo:16 * hi].reshape(hi - lo, -1)
+    assert np.isfinite(gv).all()
+    scale = np.maximum(np.abs(rv).max(axis=1), 1e-30)
+    err = np.abs(gv - rv).max(axis=1) / scale
+    assert err.max() <= TOL, f"worst pair {err.argmax()} rel err {err.max():.3e}"
+    fb = np.abs(r_b).max()
+    assert np.abs(b - r_b).max() <= TOL * fb
+
+
+@pytest.mark.parametrize("kind", ["pt", "ee"])
+def test_hessian_stream_matches_oracle(dev, oracle, kind):
+    x, c = scenes.pair_stream(kind, 8192, seed=21)
+    x = x.copy()
+    # a quarter of the contacts are inactive: their first primitive (PT: the
+    # point, EE: edge a) moves a unit away, far beyond l0 and the primitive size
+    far = np.arange(8192) % 4 == 3
+    moved = c[far][:, :1] if kind == "pt" else c[far][:, :2]
+    x[moved.reshape(-1)] += np.float32(1.0)
+    rows, cols, vals, b = _run_hessian(dev, kind, x, c)
+    ref = oracle.contact_hessian(kind, x, c, L0, KAPPA, threads=8)
+    assert ref[4][~far].all() and not ref[4][far].any()
+    _compare(rows, cols, vals, b, ref, 0, 8192)
+    # per-pair force parity (vertices are private to a pair here)
+    gf = b[c].reshape(8192, 12)
+    rf = ref[3][c].reshape(8192, 12)
+    scale = np.maximum(np.abs(rf).max(axis=1), 1e-30)
+    act = ref[4]
+    assert (np.abs(gf - rf).max(axis=1)[act] / scale[act]).max() <= TOL
+    assert not gf[~act].any()
+
+
+def test_hessian_scene_with_offset_and_shared_vertices(dev, oracle):
+    soup = scenes.two_cubes_on_ground(seed=5, n=8, gap=4e-4, jitter=2e-4)
+    th = 5e-4
+    ee, _ = oracle.dcd_ee(soup.x, soup.edges, soup.body, th, th)
+    pt, _ = oracle.dcd_pt(soup.x, soup.triangles, soup.body, th, th)
+    assert len(ee) > 50 and len(pt) > 50
+    n_ee, n_pt = len(ee), len(pt)
+    total = n_ee + n_pt
+    xt = torch.from_numpy(soup.x).to(dev)
+    rows = torch.zeros(16 * total, dtype=torch.int32, device=dev)
+    cols = torch.zeros_like(rows)
+    vals = torch.zeros((16 * total, 3, 3), dtype=torch.float32, device=dev)
+    b = torch.zeros_like(xt)
+    ops.contact_hessian_ee(xt, xt, torch.from_numpy(ee).to(dev), n_ee, L0, KAPPA, rows, cols, vals, b, 0)
+    ops.contact_hessian_pt(xt, torch.from_numpy(pt).to(dev), n_pt, L0, KAPPA, rows, cols, vals, b, n_ee)
+    torch.cuda.synchronize()
+    r_ee = oracle.contact_hessian("ee", soup.x, ee, L0, KAPPA, n_total=total, threads=8)
+    r_pt = oracle.contact_hessian("pt", soup.x, pt, L0, KAPPA, offset=n_ee, n_total=total, threads=8)
+    ref = (r_ee[0] + r_pt[0], r_ee[1] + r_pt[1], r_ee[2] + r_pt[2], r_ee[3] + r_pt[3], None)
+    _compare(rows.cpu().numpy(), cols.cpu().numpy(), vals.cpu().numpy().astype(np.float64), b.cpu().numpy(), ref, 0,
+             total)
+
+
+@pytest.mark.parametrize("kind", ["pt", "ee"])
+def test_energy_matches_oracle(dev, oracle, kind):
+    x, c = scenes.pair_stream(kind, 20000, seed=22)
+    xt = torch.from_numpy(x).to(dev)
+    ct = torch.from_numpy(c).to(dev)
+    e = torch.zeros(1, dtype=torch.float64, device=dev)
+    if kind == "ee":
+        ops.contact_energy_ee(xt, xt, ct, len(c), L0, KAPPA, e)
+    else:
+        ops.contact_energy_pt(xt, ct, len(c), L0, KAPPA, e)
+    ref = oracle.contact_energy(kind, x, c, L0, KAPPA, threads=8)
+    assert e.item() == pytest.approx(ref, rel=1e-9)
+    # accumulation into a non-zero e[0] adds (the reference zeroes e itself)
+    before = e.item()
+    if kind == "ee":
+        ops.contact_energy_ee(xt, xt, ct, len(c), L0, KAPPA, e)
+    else:
+        ops.contact_energy_pt(xt, ct, len(c), L0, KAPPA, e)
+    assert e.item() == pytest.approx(2 * before, rel=1e-12)
+
+
+@pytest.mark.parametrize("kind", ["pt", "ee"])
+def test_contact_points_distance_and_feature_bit_exact(dev, oracle, kind):
+    x, c = scenes.pair_stream(kind, 4096, seed=23)
+    n = len(c)
+    xt = torch.from_numpy(x).to(dev)
+    ct = torch.from_numpy(c).to(dev)
+    pts = torch.zeros((n, 3), dtype=torch.float32, device=dev)
+    dist = torch.zeros(n, dtype=torch.float32, device=dev)
+    feat = torch.zeros(n, dtype=torch.int32, device=dev)
+    if kind == "ee":
+        ops.contact_points_ee(xt, ct, n, pts, dist, feat, xcs=xt)
+        ref = oracle.closest_point_edge_edge(*(x[c[:, k]] for k in range(4)))
+        eps = 1e-3 * np.sum((x[c[:, 1]] - x[c[:, 0]]) ** 2, 1, dtype=np.float32) * \
+            np.sum((x[c[:, 3]] - x[c[:, 2]]) ** 2, 1, dtype=np.float32)
+        kinds = []
+        for i in range(n):
+            k, sl = oracle.feature_ee(x[c[i]].astype(np.float64), float(np.float32(eps[i])))
+            kinds.append(k * 16 + sl[0] * 4 + max(sl[1], 0))
+        assert np.array_equal(feat.cpu().numpy(), np.array(kinds))
+    else:
+        ops.contact_points_pt(xt, ct, n, pts, dist, feat)
+        ref, typ = oracle.closest_point_triangle(*(x[c[:, k]] for k in range(4)))
+        assert np.array_equal(feat.cpu().numpy(), typ)
+    assert np.array_equal(dist.cpu().numpy(), ref[:, 2].astype(np.float32))
+    # midpoints are within fp32 rounding of the closest-point pair's midpoint
+    assert np.isfinite(pts.cpu().numpy()).all()
+
+
+def test_zero_contacts_is_a_noop(dev):
+    x = torch.zeros((4, 3), dtype=torch.float32, device=dev)
+    c = torch.zeros((0, 4), dtype=torch.int32, device=dev)
+    rows = torch.zeros(16, dtype=torch.int32, device=dev)
+    vals = torch.zeros((16, 3, 3), dtype=torch.float32, device=dev)
+    ops.contact_hessian_pt(x, c, 0, L0, KAPPA, rows, rows.clone(), vals, x.clone(), 0)
+    e = torch.zeros(1, dtype=torch.float64, device=dev)
+    ops.contact_energy_pt(x, c, 0, L0, KAPPA, e)
+    torch.cuda.synchronize()
+    assert e.item() == 0.0

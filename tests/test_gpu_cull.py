@@ -1,0 +1,121 @@
+"""GPU candidate culling (contact.py:199-336,539-725; ccd/ccd.py:166-210,
+360-427) against the oracle's exact pair enumeration.
+
+Bar: the DCD contact sets (records and ids) and the CCD candidate sets are
+bit-exact after sorting; the step bound equals the oracle's to 1e-12."""
+import numpy as np
+import pytest
+import torch
+
+from ipc_warp_amd import scenes
+from ipc_warp_amd.contact import ContactSolverBase
+
+pytestmark = pytest.mark.gpu
+
+TH = 5e-4
+
+
+def _solver(dev, soup, exclude=False, capacity=None):
+    x = torch.from_numpy(soup.x).to(dev)
+    return ContactSolverBase(x, x.clone(), torch.from_numpy(soup.edges), torch.from_numpy(soup.triangles),
+                             torch.from_numpy(soup.body), thickness=TH, buffer_=TH, kappa=1e8,
+                             disable_self_collision_=exclude, capacity=capacity)
+
+
+def _sorted(rec, ids):
+    rec = np.asarray(rec)
+    ids = np.asarray(ids)
+    order = np.lexsort((ids[:, 1], ids[:, 0]))
+    return rec[order], ids[order]
+
+
+def _check_dcd(S, soup, exclude):
+    S.detect_collision()
+    ee = S.contacts_new.list[:S.n_contacts].cpu().numpy()
+    ee_ids = S.contacts_new.e0e1[:S.n_contacts].cpu().numpy()
+    pt = S.contacts_pt.list[:S.n_contacts_pt].cpu().numpy()
+    pt_ids = S.contacts_pt.e0e1[:S.n_contacts_pt].cpu().numpy()
+    import oracle
+    r_ee, r_ee_ids = oracle.dcd_ee(soup.x, soup.edges, soup.body, TH, TH, exclude)
+    r_pt, r_pt_ids = oracle.dcd_pt(soup.x, soup.triangles, soup.body, TH, TH, exclude)
+    g_ee, g_ee_ids = _sorted(ee, ee_ids)
+    g_pt, g_pt_ids = _sorted(pt, pt_ids)
+    assert np.array_equal(g_ee_ids, r_ee_ids) and np.array_equal(g_ee, r_ee)
+    assert np.array_equal(g_pt_ids, r_pt_ids) and np.array_equal(g_pt, r_pt)
+    return len(r_ee), len(r_pt)
+
+
+def test_dcd_two_cubes_pressed(dev, oracle):
+    soup = scenes.two_cubes_on_ground(seed=11, n=8, gap=4e-4, jitter=2e-4)
+    n_ee, n_pt = _check_dcd(_solver(dev, soup), soup, False)
+    assert n_ee > 100 and n_pt > 100
+
+
+def test_dcd_overflow_grows_the_lists(dev, oracle):
+    soup = scenes.two_cubes_on_ground(seed=12, n=6, gap=4e-4, jitter=2e-4)
+    S = _solver(dev, soup, capacity=1)
+    n_ee, n_pt = _check_dcd(S, soup, False)
+    assert S.contacts_new.capacity >= n_ee and S.contacts_pt.capacity >= n_pt > 1
+
+
+def test_dcd_icospheres_exclude_same_body(dev, oracle):
+    soup, _ = scenes.icosphere_grid(seed=13, per_axis=2, spacing=1.0003, subdivisions=3)
+    rng = np.random.default_rng(13)
+    soup.x = (soup.x + rng.uniform(-2e-4, 2e-4, soup.x.shape)).astype(np.float32)
+    n_ee, n_pt = _check_dcd(_solver(dev, soup, exclude=True), soup, True)
+    assert n_ee + n_pt > 0
+
+
+def _ccd_setup(seed):
+    soup = scenes.two_cubes_on_ground(seed=seed, n=8, gap=2e-3, jitter=1e-4)
+    rng = np.random.default_rng(seed)
+    dx = np.zeros_like(soup.x)
+    dx[soup.body == 1, 1] = 6e-3
+    dx += (rng.normal(size=dx.shape) * 1e-3).astype(np.float32)
+    return soup, dx.astype(np.float32)
+
+
+@pytest.mark.parametrize("kind", ["ee", "pt"])
+def test_ccd_candidate_sets(dev, oracle, kind):
+    soup, dx = _ccd_setup(14)
+    S = _solver(dev, soup)
+    x0 = S.x
+    x1 = x0 - torch.from_numpy(dx).to(dev)
+    quads, ids = S.ccd_candidates(kind, x0, x1)
+    q, i = _sorted(quads.cpu().numpy(), ids.cpu().numpy())
+    fn = oracle.ccd_candidates_ee if kind == "ee" else oracle.ccd_candidates_pt
+    topo = soup.edges if kind == "ee" else soup.triangles
+    rq, ri = fn(soup.x, x1.cpu().numpy(), topo, soup.body)
+    assert np.array_equal(i, ri) and np.array_equal(q, rq)
+    assert len(rq) > 1000
+
+
+def test_collision_free_step_matches_oracle(dev, oracle):
+    soup, dx = _ccd_setup(15)
+    S = _solver(dev, soup)
+    dxt = torch.from_numpy(dx).to(dev)
+    toi = S.raw_toi(dxt)
+    x1 = soup.x - dx
+    ref = oracle.ccd_toi(soup.x, x1, soup.edges, soup.triangles, soup.body, relative=True, threads=8)
+    assert toi < 1.0
+    assert abs(toi - ref) <= 1e-12
+    assert S.collision_free_step(dxt) == pytest.approx(0.9 * toi)
+    n_ee, n_pt = S.last_ccd_counts()
+    r_ee, _ = oracle.ccd_candidates_ee(soup.x, x1, soup.edges, soup.body)
+    r_pt, _ = oracle.ccd_candidates_pt(soup.x, x1, soup.triangles, soup.body)
+    assert (n_ee, n_pt) == (len(r_ee), len(r_pt))
+    # no motion, no impact
+    assert S.collision_free_step(torch.zeros_like(dxt)) == 1.0
+
+
+def test_degenerate_soups(dev, oracle):
+    # one edge and no triangles; two crossing edges
+    x = np.array([[0, 0, 0], [1, 0, 0], [0.5, -0.5, 4e-4], [0.5, 0.5, 4e-4]], np.float32)
+    for edges in (np.array([[0, 1]], np.int32), np.array([[0, 1], [2, 3]], np.int32)):
+        soup = scenes.Soup(x=x, xcs=x.copy(), edges=edges, triangles=np.zeros((0, 3), np.int32),
+                           body=np.array([0, 0, 1, 1], np.int32))
+        S = _solver(dev, soup)
+        S.detect_collision()
+        assert S.n_contacts_pt == 0
+        assert S.n_contacts == (1 if len(edges) == 2 else 0)
+        assert S.collision_free_step(torch.zeros_like(S.x)) == 1.0
