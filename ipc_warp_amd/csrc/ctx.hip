@@ -113,11 +113,12 @@ int ipc_detect_collision_ee(ipc_ctx* c, const float* x, float thickness, float b
   int st = ctx_hip(launch_zero_i32(count, 1, s), "detect_collision_ee");
   if (st || c->ne == 0) return st;
   float pad = thickness + buffer;
-  st = ctx_hip(build_bvh(c->edge_bvh, BOX_EDGE, c->ne, c->edges.as<int>(), x, nullptr, pad, s), "edge BVH");
+  st = ctx_hip(build_bvh(c->edge_bvh, BOX_EDGE, c->ne, c->edges.as<int>(), c->body.as<int>(), x, nullptr, pad, s), "edge BVH");
   if (st) return st;
   QueryLaunch q{};
   q.mode = Q_DCD_EE;
   q.n_query = c->ne;
+  q.order = c->edge_bvh.leaf_id.as<int>();
   q.edges = c->edges.as<int>();
   q.body = c->body.as<int>();
   q.x0 = x;
@@ -142,7 +143,7 @@ int ipc_detect_collision_pt(ipc_ctx* c, const float* x, float thickness, float b
   hipStream_t s = static_cast<hipStream_t>(stream);
   int st = ctx_hip(launch_zero_i32(count, 1, s), "detect_collision_pt");
   if (st || c->nf == 0 || c->nv == 0) return st;
-  st = ctx_hip(build_bvh(c->tri_bvh, BOX_TRI, c->nf, c->tris.as<int>(), x, nullptr, 0.0f, s), "triangle BVH");
+  st = ctx_hip(build_bvh(c->tri_bvh, BOX_TRI, c->nf, c->tris.as<int>(), c->body.as<int>(), x, nullptr, 0.0f, s), "triangle BVH");
   if (st) return st;
   QueryLaunch q{};
   q.mode = Q_DCD_PT;
@@ -169,12 +170,13 @@ int ipc_ccd_candidates_ee(ipc_ctx* c, const float* x0, const float* x1, float pa
   hipStream_t s = static_cast<hipStream_t>(stream);
   int st = ctx_hip(launch_zero_i32(count, 1, s), "ccd_candidates_ee");
   if (st || c->ne == 0) return st;
-  st = ctx_hip(build_bvh(c->edge_bvh, BOX_SWEPT_EDGE, c->ne, c->edges.as<int>(), x0, x1, padding, s),
+  st = ctx_hip(build_bvh(c->edge_bvh, BOX_SWEPT_EDGE, c->ne, c->edges.as<int>(), c->body.as<int>(), x0, x1, padding, s),
                "swept edge BVH");
   if (st) return st;
   QueryLaunch q{};
   q.mode = Q_CCD_EE;
   q.n_query = c->ne;
+  q.order = c->edge_bvh.leaf_id.as<int>();
   q.edges = c->edges.as<int>();
   q.body = c->body.as<int>();
   q.x0 = x0;
@@ -197,7 +199,7 @@ int ipc_ccd_candidates_pt(ipc_ctx* c, const float* x0, const float* x1, float pa
   hipStream_t s = static_cast<hipStream_t>(stream);
   int st = ctx_hip(launch_zero_i32(count, 1, s), "ccd_candidates_pt");
   if (st || c->nf == 0 || c->nv == 0) return st;
-  st = ctx_hip(build_bvh(c->tri_bvh, BOX_SWEPT_TRI, c->nf, c->tris.as<int>(), x0, x1, padding, s),
+  st = ctx_hip(build_bvh(c->tri_bvh, BOX_SWEPT_TRI, c->nf, c->tris.as<int>(), c->body.as<int>(), x0, x1, padding, s),
                "swept triangle BVH");
   if (st) return st;
   QueryLaunch q{};

@@ -1,24 +1,26 @@
-// Candidate culling of libipc_warp_amd: linear BVH build and AABB queries for
-// DCD (edge_edge_collision / point_triangle_collision, contact.py:199-336)
-// and for the CCD broad phase (edge_edge_toi / point_triangle_toi,
+// Candidate culling of libipc_warp_amd: BVH build and AABB queries for DCD
+// (edge_edge_collision / point_triangle_collision, contact.py:199-336) and
+// for the CCD broad phase (edge_edge_toi / point_triangle_toi,
 // ccd/ccd.py:166-210,360-427).
 //
 // The BVH is rebuilt on every call from the current positions (the reference
 // refits a BVH built once; a rebuild keeps query cost independent of how far
-// the mesh has moved and costs a handful of small kernels):
+// the mesh has moved):
 //   1. one AABB per primitive (fp32, the reference's inflation),
 //   2. 30-bit Morton codes of box centres in the scene bounds, made unique by
 //      the primitive id in the low 32 bits of a 64-bit key,
 //   3. hipCUB radix sort of the keys,
-//   4. Karras (2012) hierarchy: internal node i spans a key range and splits
-//      at the highest differing bit,
-//   5. bottom-up boxes: each internal node stores the boxes of its two
-//      children (64 B per node), so a traversal step tests both children
-//      with one 64 B load.
-// Queries keep a 64-entry stack in LDS (a Karras tree over 64-bit unique keys
-// is at most 64 levels deep), run a wave-uniform loop (one node per lane per
-// iteration) and append accepted pairs with one atomicAdd per wave per
-// iteration (ballot + prefix count).
+//   4. an implicit 8-ary hierarchy over the sorted leaves: node k of level L
+//      bounds nodes 8k .. 8k+7 of level L-1 (level 0 = leaves). Levels 1-4
+//      are reduced by one kernel in LDS (a workgroup owns 4096 leaves), the
+//      levels above by a single workgroup. There is no bottom-up propagation
+//      across workgroups, hence no device-scope fence: on a multi-XCD part
+//      such a fence writes back and invalidates the L2.
+// Queries keep a 64-entry stack in LDS (7 pending siblings per level over at
+// most 9 levels), run a wave-uniform loop (one node per lane per iteration,
+// testing its 8 children from one contiguous 256 B run) and append accepted
+// pairs with one atomicAdd per wave per emission round (ballot + prefix
+// count).
 #include <hipcub/hipcub.hpp>
 
 #include "cull.hpp"
@@ -135,87 +137,109 @@ __global__ void __launch_bounds__(256) morton_kernel(int n, const float4* __rest
 }
 
 // ------------------------------------------------------------ hierarchy ----
-__device__ __forceinline__ int delta(const unsigned long long* keys, int n, int i, int j) {
-  if (j < 0 || j >= n) return -1;
-  return __clzll(keys[i] ^ keys[j]);
+constexpr int kReduceBlock = 512;                     // threads of the blocked reduction
+constexpr int kLeavesPerBlock = kFan * kReduceBlock;  // 4096 = 8^4 leaves per workgroup
+
+__device__ __forceinline__ void box_merge(float4& lo, float4& hi, float4 l, float4 h) {
+  lo = make_float4(fminf(lo.x, l.x), fminf(lo.y, l.y), fminf(lo.z, l.z), 0.f);
+  hi = make_float4(fmaxf(hi.x, h.x), fmaxf(hi.y, h.y), fmaxf(hi.z, h.z), 0.f);
 }
 
-// Internal node i of a Karras tree over n sorted unique keys. Children are
-// internal node ids (>= 0) or ~primitive id (< 0) for leaves.
-__global__ void __launch_bounds__(256) karras_kernel(int n, const unsigned long long* __restrict__ keys,
-                                                     int2* __restrict__ child, int* __restrict__ parent,
-                                                     int* __restrict__ leaf_parent) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n - 1) return;
-  int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
-  int dmin = delta(keys, n, i, i - d);
-  int lmax = 2;
-  while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
-  int l = 0;
-  for (int t = lmax >> 1; t >= 1; t >>= 1)
-    if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
-  int j = i + l * d;
-  int dnode = delta(keys, n, i, j);
-  int s = 0, t = l;
-  do {
-    t = (t + 1) >> 1;
-    if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
-  } while (t > 1);
-  int gamma = i + s * d + (d < 0 ? -1 : 0);
-  int lo = i < j ? i : j, hi = i < j ? j : i;
-  int left, right;
-  if (lo == gamma) {
-    left = ~static_cast<int>(keys[gamma] & 0xffffffffu);
-    leaf_parent[gamma] = i;
-  } else {
-    left = gamma;
-    parent[gamma] = i;
+__device__ __forceinline__ void box_empty(float4& lo, float4& hi) {
+  lo = make_float4(INFINITY, INFINITY, INFINITY, 0.f);
+  hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+}
+
+// Leaves in Morton order (boxes gathered through the sorted ids) and levels
+// 1..4 of the hierarchy. Workgroup b owns leaves [4096 b, 4096 (b + 1)).
+__global__ void __launch_bounds__(kReduceBlock) reduce_levels_kernel(
+    int n, int prim_stride, const int* __restrict__ prims, const int* __restrict__ body,
+    const unsigned long long* __restrict__ keys, const float4* __restrict__ lo4, const float4* __restrict__ hi4,
+    float4* __restrict__ leaf_lo, float4* __restrict__ leaf_hi, int* __restrict__ leaf_id, int* __restrict__ leaf_body,
+    float4* __restrict__ node_lo, float4* __restrict__ node_hi, int2* __restrict__ node_body, LevelInfo info) {
+  __shared__ float4 slo[kReduceBlock], shi[kReduceBlock];
+  __shared__ int2 sbody[kReduceBlock];
+  int t = threadIdx.x;
+  int g = blockIdx.x * kReduceBlock + t;  // level-1 node
+  float4 blo, bhi;
+  box_empty(blo, bhi);
+  int2 br = make_int2(0x7fffffff, (int)0x80000000);
+#pragma unroll
+  for (int c = 0; c < kFan; ++c) {
+    int k = g * kFan + c;
+    if (k < n) {
+      int prim = static_cast<int>(keys[k] & 0xffffffffu);
+      float4 l = lo4[prim], h = hi4[prim];
+      int bd = body[prims[prim_stride * prim]];
+      leaf_lo[k] = l;
+      leaf_hi[k] = h;
+      leaf_id[k] = prim;
+      leaf_body[k] = bd;
+      box_merge(blo, bhi, l, h);
+      br.x = min(br.x, bd);
+      br.y = max(br.y, bd);
+    }
   }
-  if (hi == gamma + 1) {
-    right = ~static_cast<int>(keys[gamma + 1] & 0xffffffffu);
-    leaf_parent[gamma + 1] = i;
-  } else {
-    right = gamma + 1;
-    parent[gamma + 1] = i;
+  if (1 <= info.top && g < info.count[1]) {
+    node_lo[info.offset[1] + g] = blo;
+    node_hi[info.offset[1] + g] = bhi;
+    node_body[info.offset[1] + g] = br;
   }
-  child[i] = make_int2(left, right);
-  if (i == 0) parent[0] = -1;
+  slo[t] = blo;
+  shi[t] = bhi;
+  sbody[t] = br;
+  int width = kReduceBlock;
+  for (int L = 2; L <= 4; ++L) {
+    __syncthreads();
+    width /= kFan;
+    float4 l, h;
+    box_empty(l, h);
+    int2 r2 = make_int2(0x7fffffff, (int)0x80000000);
+    if (t < width) {
+#pragma unroll
+      for (int c = 0; c < kFan; ++c) {
+        box_merge(l, h, slo[t * kFan + c], shi[t * kFan + c]);
+        r2.x = min(r2.x, sbody[t * kFan + c].x);
+        r2.y = max(r2.y, sbody[t * kFan + c].y);
+      }
+    }
+    __syncthreads();
+    if (t < width) {
+      slo[t] = l;
+      shi[t] = h;
+      sbody[t] = r2;
+      int gl = blockIdx.x * width + t;
+      if (L <= info.top && gl < info.count[L]) {
+        node_lo[info.offset[L] + gl] = l;
+        node_hi[info.offset[L] + gl] = h;
+        node_body[info.offset[L] + gl] = r2;
+      }
+    }
+  }
 }
 
-__global__ void __launch_bounds__(256) zero_u32_kernel(unsigned* p, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0u;
-}
-
-// Bottom-up boxes. Node q's 64 B record is (left lo, left hi, right lo,
-// right hi). The second child to arrive at a node (atomic counter) merges
-// the two slots and continues to the parent.
-__global__ void __launch_bounds__(256) refit_kernel(int n, const unsigned long long* __restrict__ keys,
-                                                    const float4* __restrict__ lo4, const float4* __restrict__ hi4,
-                                                    const int2* __restrict__ child, const int* __restrict__ parent,
-                                                    const int* __restrict__ leaf_parent, float4* box,
-                                                    unsigned* flags) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  int prim = static_cast<int>(keys[k] & 0xffffffffu);
-  float4 lo = lo4[prim], hi = hi4[prim];
-  int self = ~prim;
-  int q = leaf_parent[k];
-  while (q >= 0) {
-    int slot = (child[q].x == self) ? 0 : 1;
-    box[4 * q + 2 * slot] = lo;
-    box[4 * q + 2 * slot + 1] = hi;
-    __threadfence();
-    unsigned old = atomicAdd(flags + q, 1u);
-    if (old == 0u) return;
-    __threadfence();
-    const volatile float4* vb = box + 4 * q;
-    float4 l0 = make_float4(vb[0].x, vb[0].y, vb[0].z, 0.f), h0 = make_float4(vb[1].x, vb[1].y, vb[1].z, 0.f);
-    float4 l1 = make_float4(vb[2].x, vb[2].y, vb[2].z, 0.f), h1 = make_float4(vb[3].x, vb[3].y, vb[3].z, 0.f);
-    lo = make_float4(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z), 0.f);
-    hi = make_float4(fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z), 0.f);
-    self = q;
-    q = parent[q];
+// Levels 5..top in one workgroup (level 4 holds at most 2^27 / 8^4 nodes).
+__global__ void __launch_bounds__(1024) reduce_top_kernel(float4* __restrict__ node_lo, float4* __restrict__ node_hi,
+                                                         int2* __restrict__ node_body, LevelInfo info) {
+  for (int L = 5; L <= info.top; ++L) {
+    for (int g = threadIdx.x; g < info.count[L]; g += blockDim.x) {
+      float4 l, h;
+      box_empty(l, h);
+      int2 r2 = make_int2(0x7fffffff, (int)0x80000000);
+      for (int c = 0; c < kFan; ++c) {
+        int k = g * kFan + c;
+        if (k < info.count[L - 1]) {
+          box_merge(l, h, node_lo[info.offset[L - 1] + k], node_hi[info.offset[L - 1] + k]);
+          int2 cb = node_body[info.offset[L - 1] + k];
+          r2.x = min(r2.x, cb.x);
+          r2.y = max(r2.y, cb.y);
+        }
+      }
+      node_lo[info.offset[L] + g] = l;
+      node_hi[info.offset[L] + g] = h;
+      node_body[info.offset[L] + g] = r2;
+    }
+    __syncthreads();
   }
 }
 
@@ -233,8 +257,9 @@ __device__ __forceinline__ int wave_append(bool want, int* count) {
   return base + __popcll(m & ((1ull << lane) - 1ull));
 }
 
-constexpr int kQueryBlock = 128;
-constexpr int kStackDepth = 64;
+constexpr int kQueryBlock = 256;
+constexpr int kGroupsPerBlock = kQueryBlock / kFan;  // one query per 8 lanes
+constexpr int kMaskStack = 12;                       // > kMaxLevels - 1 entries per query
 
 // Query mode:
 //  0 DCD EE: query edge i's box against the edge BVH; accept i < j, not
@@ -255,17 +280,22 @@ struct QueryArgs {
   const int* body;
   const float* x0;
   const float* x1;
-  const float4* qlo;  // query boxes (modes 0, 2: the leaves' own boxes)
+  const float4* qlo;  // query boxes (modes 0, 2: the primitives' own boxes)
   const float4* qhi;
+  const int* order;   // optional query order (slot -> query primitive)
   float pad;          // mode 1: 2 (th + buf); mode 3: CCD padding
   float thickness;
   int exclude_same_body;
-  // BVH
+  // hierarchy
   int n_leaves;
-  const int2* child;
-  const float4* box;
-  const float4* leaf_lo;  // single-leaf trees
+  LevelInfo info;
+  const float4* node_lo;
+  const float4* node_hi;
+  const int2* node_body;
+  const float4* leaf_lo;
   const float4* leaf_hi;
+  const int* leaf_id;
+  const int* leaf_body;
   // output
   int4* rec;
   int2* ids;
@@ -296,17 +326,35 @@ __device__ __forceinline__ bool accept_pair(const QueryArgs& A, int i, int j, in
   return tc.dist < A.thickness * 2.0f;
 }
 
+// Stack entry: level (bits 40..43), child mask (bits 32..39), first child
+// index at that level (bits 0..31).
+__device__ __forceinline__ unsigned long long mask_entry(int level, unsigned mask, int first) {
+  return (static_cast<unsigned long long>(level) << 40) | (static_cast<unsigned long long>(mask & 0xffu) << 32) |
+         static_cast<unsigned>(first);
+}
+
+// Eight lanes share one query. Visiting a node, lane r tests the node's
+// child r; the group ballot of overlapping children becomes one stack entry
+// (level, first child, mask). Every lane of a group computes and writes the
+// same stack values, so each lane only ever reads back its own LDS writes.
 __global__ void __launch_bounds__(kQueryBlock) query_kernel(QueryArgs A) {
-  __shared__ int stack[kStackDepth * kQueryBlock];
+  __shared__ unsigned long long stack[kGroupsPerBlock][kMaskStack];
   int tid = threadIdx.x;
-  int i = blockIdx.x * blockDim.x + tid;
-  bool live = i < A.n_query;
+  int grp = tid / kFan, r = tid % kFan;
+  int lane = __lane_id();
+  int group_shift = lane & ~(kFan - 1);  // first lane of this group in the wave
+  int slot = blockIdx.x * kGroupsPerBlock + grp;
+  bool live = slot < A.n_query && A.n_leaves > 0;
+  int i = 0;
   float qlo[3] = {0.f, 0.f, 0.f}, qhi[3] = {-1.f, -1.f, -1.f};
+  int qbody = -1;
   if (live) {
+    i = A.order ? A.order[slot] : slot;
     if (A.mode == 0 || A.mode == 2) {
       float4 l = A.qlo[i], h = A.qhi[i];
       qlo[0] = l.x; qlo[1] = l.y; qlo[2] = l.z;
       qhi[0] = h.x; qhi[1] = h.y; qhi[2] = h.z;
+      if (A.exclude_same_body) qbody = A.body[A.edges[2 * i]];
     } else {
       V3<float> p = ldx(A.x0, i), lo = p, hi = p;
       if (A.mode == 3) {
@@ -319,51 +367,62 @@ __global__ void __launch_bounds__(kQueryBlock) query_kernel(QueryArgs A) {
       hi = hi + pv;
       qlo[0] = lo.x; qlo[1] = lo.y; qlo[2] = lo.z;
       qhi[0] = hi.x; qhi[1] = hi.y; qhi[2] = hi.z;
+      if (A.exclude_same_body) qbody = A.body[i];
     }
   }
   int sp = 0;
-  // hits of this iteration (primitive ids, -1 = none)
-  int h0 = -1, h1 = -1;
   if (live) {
-    if (A.n_leaves == 1) {
-      if (overlaps(qlo, qhi, A.leaf_lo[0], A.leaf_hi[0])) h0 = 0;
-    } else if (A.n_leaves > 1) {
-      stack[tid] = 0;
-      sp = 1;
-    }
+    stack[grp][0] = mask_entry(A.info.top, 1u, 0);  // the root as the only candidate of its level
+    sp = 1;
   }
   while (true) {
-    if (sp > 0) {
-      int node = stack[(--sp) * kQueryBlock + tid];
-      int2 ch = A.child[node];
-      const float4* bx = A.box + 4 * node;
-      float4 l0 = bx[0], u0 = bx[1], l1 = bx[2], u1 = bx[3];
-      // A Karras tree over 62-bit unique keys is at most 61 internal levels
-      // deep, so sp never reaches kStackDepth; the guard only keeps LDS
-      // writes in bounds.
-      if (overlaps(qlo, qhi, l0, u0)) {
-        if (ch.x < 0) h0 = ~ch.x;
-        else if (sp < kStackDepth) stack[(sp++) * kQueryBlock + tid] = ch.x;
+    bool want = false;
+    int j = -1;
+    int4 rec;
+    bool active = sp > 0;
+    int L = 0, node = 0;
+    if (active) {
+      unsigned long long e = stack[grp][sp - 1];
+      L = int(e >> 40);
+      unsigned mask = unsigned(e >> 32) & 0xffu;
+      int first = int(e & 0xffffffffu);
+      int c = __ffs(mask) - 1;
+      node = first + c;
+      mask &= mask - 1u;
+      if (mask == 0u) --sp;
+      else stack[grp][sp - 1] = mask_entry(L, mask, first);
+    }
+    // the node's children: internal nodes of level L-1, or leaves when L == 1
+    bool overlap_child = false;
+    int child = node * kFan + r;
+    if (active && L >= 2) {
+      int lv = L - 1;
+      if (child < A.info.count[lv]) {
+        int at = A.info.offset[lv] + child;
+        overlap_child = overlaps(qlo, qhi, A.node_lo[at], A.node_hi[at]);
+        if (overlap_child && qbody >= 0) {
+          int2 br = A.node_body[at];
+          overlap_child = !(br.x == qbody && br.y == qbody);
+        }
       }
-      if (overlaps(qlo, qhi, l1, u1)) {
-        if (ch.y < 0) h1 = ~ch.y;
-        else if (sp < kStackDepth) stack[(sp++) * kQueryBlock + tid] = ch.y;
+    } else if (active && L == 1) {
+      if (child < A.n_leaves && overlaps(qlo, qhi, A.leaf_lo[child], A.leaf_hi[child]) &&
+          !(qbody >= 0 && A.leaf_body[child] == qbody)) {
+        j = A.leaf_id[child];
+        want = accept_pair(A, i, j, rec);
       }
     }
-    int4 r0, r1;
-    bool w0 = h0 >= 0 && accept_pair(A, i, h0, r0);
-    bool w1 = h1 >= 0 && accept_pair(A, i, h1, r1);
-    int s0 = wave_append(w0, A.count);
-    if (s0 >= 0 && s0 < A.capacity) {
-      A.rec[s0] = r0;
-      if (A.ids) A.ids[s0] = make_int2(i, h0);
+    unsigned long long ball = __ballot(overlap_child);
+    unsigned gmask = unsigned(ball >> group_shift) & 0xffu;
+    if (active && L >= 2 && gmask != 0u && sp < kMaskStack) {
+      stack[grp][sp] = mask_entry(L - 1, gmask, node * kFan);
+      ++sp;
     }
-    int s1 = wave_append(w1, A.count);
-    if (s1 >= 0 && s1 < A.capacity) {
-      A.rec[s1] = r1;
-      if (A.ids) A.ids[s1] = make_int2(i, h1);
+    int out = wave_append(want, A.count);
+    if (out >= 0 && out < A.capacity) {
+      A.rec[out] = rec;
+      if (A.ids) A.ids[out] = make_int2(i, j);
     }
-    h0 = h1 = -1;
     if (!__any(sp > 0)) break;
   }
 }
@@ -429,9 +488,11 @@ void DevBuf::release() {
 }
 
 void Bvh::release() {
-  lo.release(); hi.release(); keys.release(); keys_sorted.release(); child.release(); parent.release();
-  leaf_parent.release(); box.release(); flags.release(); temp.release(); bounds.release();
+  lo.release(); hi.release(); keys.release(); keys_sorted.release(); leaf_lo.release(); leaf_hi.release();
+  leaf_id.release(); node_lo.release(); node_hi.release(); leaf_body.release(); node_body.release();
+  temp.release(); bounds.release();
   n = 0;
+  builds_since_sort = -1;
 }
 
 #define IPC_TRY(expr)                    \
@@ -440,47 +501,69 @@ void Bvh::release() {
     if (_e != hipSuccess) return _e;     \
   } while (0)
 
-hipError_t build_bvh(Bvh& B, int mode, int n, const int* prims, const float* x0, const float* x1, float pad,
-                     hipStream_t s) {
+hipError_t build_bvh(Bvh& B, int mode, int n, const int* prims, const int* body, const float* x0, const float* x1,
+                     float pad, hipStream_t s) {
   B.n = n;
   if (n <= 0) return hipSuccess;
+  if (n > (1 << 27)) return hipErrorInvalidValue;
+  // level sizes: count[0] = n leaves, count[L] = ceil(count[L-1] / 8) up to a single root
+  LevelInfo& I = B.info;
+  for (int L = 0; L < kMaxLevels; ++L) { I.count[L] = 0; I.offset[L] = 0; }
+  I.count[0] = n;
+  int L = 0, total = 0;
+  do {
+    ++L;
+    I.count[L] = (I.count[L - 1] + kFan - 1) / kFan;
+    I.offset[L] = total;
+    total += I.count[L];
+  } while (I.count[L] > 1);
+  I.top = L;
   IPC_TRY(B.lo.ensure(sizeof(float4) * n));
   IPC_TRY(B.hi.ensure(sizeof(float4) * n));
   IPC_TRY(B.bounds.ensure(sizeof(int) * 8));
+  IPC_TRY(B.keys.ensure(sizeof(unsigned long long) * n));
+  IPC_TRY(B.keys_sorted.ensure(sizeof(unsigned long long) * n));
+  IPC_TRY(B.leaf_lo.ensure(sizeof(float4) * n));
+  IPC_TRY(B.leaf_hi.ensure(sizeof(float4) * n));
+  IPC_TRY(B.leaf_id.ensure(sizeof(int) * n));
+  IPC_TRY(B.node_lo.ensure(sizeof(float4) * total));
+  IPC_TRY(B.node_hi.ensure(sizeof(float4) * total));
+  IPC_TRY(B.leaf_body.ensure(sizeof(int) * n));
+  IPC_TRY(B.node_body.ensure(sizeof(int2) * total));
+  bool resort = B.builds_since_sort < 0 || B.builds_since_sort >= kResortEvery || B.n_sorted != n;
   hipLaunchKernelGGL(init_bounds_kernel, dim3(1), dim3(64), 0, s, B.bounds.as<int>());
   hipLaunchKernelGGL(prim_boxes_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, mode, n, prims, x0, x1, pad,
                      B.lo.as<float4>(), B.hi.as<float4>(), B.bounds.as<int>());
   IPC_TRY(hipGetLastError());
-  if (n == 1) return hipSuccess;
-  IPC_TRY(B.keys.ensure(sizeof(unsigned long long) * n));
-  IPC_TRY(B.keys_sorted.ensure(sizeof(unsigned long long) * n));
-  hipLaunchKernelGGL(morton_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, n, B.lo.as<float4>(), B.hi.as<float4>(),
-                     B.bounds.as<int>(), B.keys.as<unsigned long long>());
-  IPC_TRY(hipGetLastError());
-  size_t temp_bytes = 0;
-  IPC_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, temp_bytes, B.keys.as<unsigned long long>(),
-                                            B.keys_sorted.as<unsigned long long>(), n, 0, 62, s));
-  IPC_TRY(B.temp.ensure(temp_bytes));
-  temp_bytes = B.temp.bytes;
-  IPC_TRY(hipcub::DeviceRadixSort::SortKeys(B.temp.p, temp_bytes, B.keys.as<unsigned long long>(),
-                                            B.keys_sorted.as<unsigned long long>(), n, 0, 62, s));
-  IPC_TRY(B.child.ensure(sizeof(int2) * (n - 1)));
-  IPC_TRY(B.parent.ensure(sizeof(int) * (n - 1)));
-  IPC_TRY(B.leaf_parent.ensure(sizeof(int) * n));
-  IPC_TRY(B.box.ensure(sizeof(float4) * 4 * (n - 1)));
-  IPC_TRY(B.flags.ensure(sizeof(unsigned) * (n - 1)));
-  hipLaunchKernelGGL(karras_kernel, dim3(nblocks(n - 1, 256)), dim3(256), 0, s, n,
-                     B.keys_sorted.as<unsigned long long>(), B.child.as<int2>(), B.parent.as<int>(),
-                     B.leaf_parent.as<int>());
-  hipLaunchKernelGGL(zero_u32_kernel, dim3(nblocks(n - 1, 256)), dim3(256), 0, s, B.flags.as<unsigned>(), n - 1);
-  hipLaunchKernelGGL(refit_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, n, B.keys_sorted.as<unsigned long long>(),
-                     B.lo.as<float4>(), B.hi.as<float4>(), B.child.as<int2>(), B.parent.as<int>(),
-                     B.leaf_parent.as<int>(), B.box.as<float4>(), B.flags.as<unsigned>());
+  if (resort) {
+    hipLaunchKernelGGL(morton_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, n, B.lo.as<float4>(), B.hi.as<float4>(),
+                       B.bounds.as<int>(), B.keys.as<unsigned long long>());
+    IPC_TRY(hipGetLastError());
+    size_t temp_bytes = 0;
+    IPC_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, temp_bytes, B.keys.as<unsigned long long>(),
+                                              B.keys_sorted.as<unsigned long long>(), n, 0, 62, s));
+    IPC_TRY(B.temp.ensure(temp_bytes));
+    temp_bytes = B.temp.bytes;
+    IPC_TRY(hipcub::DeviceRadixSort::SortKeys(B.temp.p, temp_bytes, B.keys.as<unsigned long long>(),
+                                              B.keys_sorted.as<unsigned long long>(), n, 0, 62, s));
+    B.builds_since_sort = 0;
+    B.n_sorted = n;
+  } else {
+    ++B.builds_since_sort;
+  }
+  int stride = (mode == BOX_EDGE || mode == BOX_SWEPT_EDGE) ? 2 : 3;
+  hipLaunchKernelGGL(reduce_levels_kernel, dim3(nblocks(n, kLeavesPerBlock)), dim3(kReduceBlock), 0, s, n, stride,
+                     prims, body, B.keys_sorted.as<unsigned long long>(), B.lo.as<float4>(), B.hi.as<float4>(),
+                     B.leaf_lo.as<float4>(), B.leaf_hi.as<float4>(), B.leaf_id.as<int>(), B.leaf_body.as<int>(),
+                     B.node_lo.as<float4>(), B.node_hi.as<float4>(), B.node_body.as<int2>(), I);
+  if (I.top >= 5)
+    hipLaunchKernelGGL(reduce_top_kernel, dim3(1), dim3(1024), 0, s, B.node_lo.as<float4>(), B.node_hi.as<float4>(),
+                       B.node_body.as<int2>(), I);
   return hipGetLastError();
 }
 
 hipError_t run_query(const QueryLaunch& q, hipStream_t s) {
-  if (q.n_query <= 0) return hipSuccess;
+  if (q.n_query <= 0 || q.bvh->n <= 0) return hipSuccess;
   QueryArgs A;
   A.mode = q.mode;
   A.n_query = q.n_query;
@@ -491,20 +574,24 @@ hipError_t run_query(const QueryLaunch& q, hipStream_t s) {
   A.x1 = q.x1;
   A.qlo = q.qlo;
   A.qhi = q.qhi;
+  A.order = q.order;
   A.pad = q.pad;
   A.thickness = q.thickness;
   A.exclude_same_body = q.exclude_same_body;
   A.n_leaves = q.bvh->n;
-  A.child = q.bvh->child.as<int2>();
-  A.box = q.bvh->box.as<float4>();
-  A.leaf_lo = q.bvh->lo.as<float4>();
-  A.leaf_hi = q.bvh->hi.as<float4>();
+  A.info = q.bvh->info;
+  A.node_lo = q.bvh->node_lo.as<float4>();
+  A.node_hi = q.bvh->node_hi.as<float4>();
+  A.node_body = q.bvh->node_body.as<int2>();
+  A.leaf_lo = q.bvh->leaf_lo.as<float4>();
+  A.leaf_hi = q.bvh->leaf_hi.as<float4>();
+  A.leaf_id = q.bvh->leaf_id.as<int>();
+  A.leaf_body = q.bvh->leaf_body.as<int>();
   A.rec = q.rec;
   A.ids = q.ids;
   A.capacity = q.capacity;
   A.count = q.count;
-  if (A.n_leaves <= 0) return hipSuccess;
-  hipLaunchKernelGGL(query_kernel, dim3(nblocks(q.n_query, kQueryBlock)), dim3(kQueryBlock), 0, s, A);
+  hipLaunchKernelGGL(query_kernel, dim3(nblocks(q.n_query, kGroupsPerBlock)), dim3(kQueryBlock), 0, s, A);
   return hipGetLastError();
 }
 

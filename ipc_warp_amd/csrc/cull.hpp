@@ -15,26 +15,45 @@ struct DevBuf {
   template <typename T> T* as() const { return static_cast<T*>(p); }
 };
 
-// Linear BVH over n primitives (see cull.hip).
+constexpr int kFan = 8;        // children per hierarchy node
+constexpr int kMaxLevels = 12;  // 8^10 > 2^27 primitives
+
+// Sizes of the implicit 8-ary hierarchy: level 0 holds the n leaves in
+// Morton order, level L holds ceil(count[L-1] / 8) nodes, the root is node 0
+// of level `top`. Levels >= 1 are stored back to back from offset[L].
+struct LevelInfo {
+  int top;
+  int count[kMaxLevels];
+  int offset[kMaxLevels];
+};
+
+// BVH over n primitives (see cull.hip).
 struct Bvh {
   int n = 0;
+  LevelInfo info{};
   DevBuf lo, hi;              // float4 per primitive, original order
   DevBuf keys, keys_sorted;   // uint64 Morton | id
-  DevBuf child;               // int2 per internal node
-  DevBuf parent;              // int per internal node
-  DevBuf leaf_parent;         // int per sorted leaf
-  DevBuf box;                 // 4 float4 per internal node
-  DevBuf flags;               // uint per internal node
+  DevBuf leaf_lo, leaf_hi;    // float4 per leaf, Morton order
+  DevBuf leaf_id;             // primitive id per leaf
+  DevBuf node_lo, node_hi;    // float4 per node of levels >= 1
+  DevBuf leaf_body;           // body id per leaf (first vertex's body)
+  DevBuf node_body;           // int2 (min, max) body id per node of levels >= 1
   DevBuf temp;                // radix sort workspace
   DevBuf bounds;              // 6 ordered ints
+  int builds_since_sort = -1; // -1: no leaf order yet
+  int n_sorted = 0;
   void release();
 };
+
+// Leaf order is recomputed (Morton sort) on the first build and then every
+// kResortEvery builds; in between, builds reuse it and only recompute boxes.
+constexpr int kResortEvery = 16;
 
 enum BoxMode { BOX_EDGE = 0, BOX_TRI = 1, BOX_SWEPT_EDGE = 2, BOX_SWEPT_TRI = 3 };
 enum QueryMode { Q_DCD_EE = 0, Q_DCD_PT = 1, Q_CCD_EE = 2, Q_CCD_PT = 3 };
 
-hipError_t build_bvh(Bvh& B, int mode, int n, const int* prims, const float* x0, const float* x1, float pad,
-                     hipStream_t s);
+hipError_t build_bvh(Bvh& B, int mode, int n, const int* prims, const int* body, const float* x0, const float* x1,
+                     float pad, hipStream_t s);
 
 struct QueryLaunch {
   int mode;
@@ -46,6 +65,7 @@ struct QueryLaunch {
   const float* x1;
   const float4* qlo;
   const float4* qhi;
+  const int* order;   // optional slot -> query primitive (Morton order of the queries)
   float pad;
   float thickness;
   int exclude_same_body;

@@ -6,6 +6,7 @@
 #include <omp.h>
 
 #include <cstdint>
+#include <unordered_map>
 #include <utility>
 
 using namespace ipco;
@@ -22,47 +23,76 @@ inline bool overlap(const Box& a, const Box& b) {
   return true;
 }
 
-// Sweep-and-prune on x. Calls fn(a_id, b_id) once for every overlapping pair
-// with one box from A and one from B (or every unordered pair when self=true).
+// Calls fn(a_id, b_id) once for every overlapping pair with one box from A and
+// one from B (every unordered pair when self = true), found through a uniform
+// hash grid; every candidate is confirmed with `overlap`, so the enumeration
+// is exact and independent of the grid's parameters. Cell size
+// is twice the median box extent; boxes spanning more than kBigCells cells on
+// an axis are tested against everything directly. A pair of grid boxes is
+// reported only from the cell holding the lower corner of their
+// intersection, so each pair is reported once.
 template <typename Fn>
-void sweep_pairs(std::vector<Box> A, std::vector<Box> B, bool self, Fn fn) {
-  auto by_lo = [](const Box& u, const Box& v) { return u.lo[0] < v.lo[0] || (u.lo[0] == v.lo[0] && u.id < v.id); };
-  std::sort(A.begin(), A.end(), by_lo);
-  if (self) {
-    std::vector<const Box*> active;
-    for (const Box& a : A) {
-      size_t w = 0;
-      for (size_t r = 0; r < active.size(); ++r)
-        if (!(active[r]->hi[0] < a.lo[0])) active[w++] = active[r];
-      active.resize(w);
-      for (const Box* b : active)
-        if (overlap(a, *b)) fn(a.id, b->id);
-      active.push_back(&a);
-    }
-    return;
-  }
-  std::sort(B.begin(), B.end(), by_lo);
-  std::vector<const Box*> actA, actB;
-  size_t ia = 0, ib = 0;
-  auto prune = [](std::vector<const Box*>& act, float x) {
-    size_t w = 0;
-    for (size_t r = 0; r < act.size(); ++r)
-      if (!(act[r]->hi[0] < x)) act[w++] = act[r];
-    act.resize(w);
+void grid_pairs(const std::vector<Box>& A, const std::vector<Box>& B, bool self, Fn fn) {
+  const std::vector<Box>& Bs = self ? A : B;
+  if (A.empty() || Bs.empty()) return;
+  std::vector<float> ext;
+  ext.reserve(A.size() + Bs.size());
+  for (const Box& b : A) ext.push_back(std::max(b.hi[0] - b.lo[0], std::max(b.hi[1] - b.lo[1], b.hi[2] - b.lo[2])));
+  if (!self)
+    for (const Box& b : Bs) ext.push_back(std::max(b.hi[0] - b.lo[0], std::max(b.hi[1] - b.lo[1], b.hi[2] - b.lo[2])));
+  std::nth_element(ext.begin(), ext.begin() + ext.size() / 2, ext.end());
+  double h = std::max(2.0 * double(ext[ext.size() / 2]), 1e-6);
+  const int kBigCells = 16;
+  auto cell = [h](float v) { return (long long)std::floor(double(v) / h); };
+  auto is_big = [&](const Box& b) {
+    for (int k = 0; k < 3; ++k)
+      if (cell(b.hi[k]) - cell(b.lo[k]) > kBigCells) return true;
+    return false;
   };
-  while (ia < A.size() || ib < B.size()) {
-    bool takeA = ib >= B.size() || (ia < A.size() && A[ia].lo[0] <= B[ib].lo[0]);
-    if (takeA) {
-      const Box& a = A[ia++];
-      prune(actB, a.lo[0]);
-      for (const Box* b : actB) if (overlap(a, *b)) fn(a.id, b->id);
-      actA.push_back(&a);
-    } else {
-      const Box& b = B[ib++];
-      prune(actA, b.lo[0]);
-      for (const Box* a : actA) if (overlap(*a, b)) fn(a->id, b.id);
-      actB.push_back(&b);
+  auto key = [](long long i, long long j, long long k) {
+    return (unsigned long long)((i & 0x1FFFFF) | ((j & 0x1FFFFF) << 21) | ((k & 0x1FFFFF) << 42));
+  };
+  std::vector<int> bigA, smallA, bigB, smallB;
+  for (size_t i = 0; i < A.size(); ++i) (is_big(A[i]) ? bigA : smallA).push_back(int(i));
+  if (self) { bigB = bigA; smallB = smallA; }
+  else for (size_t i = 0; i < Bs.size(); ++i) (is_big(Bs[i]) ? bigB : smallB).push_back(int(i));
+  // big A against all B; small A against big B (self: each unordered pair once)
+  for (int a : bigA)
+    for (size_t b = 0; b < Bs.size(); ++b) {
+      if (self && (int(b) == a || (is_big(Bs[b]) && int(b) < a))) continue;
+      if (overlap(A[a], Bs[b])) fn(A[a].id, Bs[b].id);
     }
+  for (int a : smallA)
+    for (int b : bigB) {
+      if (self) continue;  // covered above from the big side
+      if (overlap(A[a], Bs[b])) fn(A[a].id, Bs[b].id);
+    }
+  // small-small through the grid
+  std::unordered_map<unsigned long long, std::vector<int>> grid;
+  grid.reserve(smallB.size() * 2);
+  for (int b : smallB) {
+    const Box& x = Bs[b];
+    for (long long i = cell(x.lo[0]); i <= cell(x.hi[0]); ++i)
+      for (long long j = cell(x.lo[1]); j <= cell(x.hi[1]); ++j)
+        for (long long k = cell(x.lo[2]); k <= cell(x.hi[2]); ++k) grid[key(i, j, k)].push_back(b);
+  }
+  for (int a : smallA) {
+    const Box& x = A[a];
+    for (long long i = cell(x.lo[0]); i <= cell(x.hi[0]); ++i)
+      for (long long j = cell(x.lo[1]); j <= cell(x.hi[1]); ++j)
+        for (long long k = cell(x.lo[2]); k <= cell(x.hi[2]); ++k) {
+          auto it = grid.find(key(i, j, k));
+          if (it == grid.end()) continue;
+          for (int b : it->second) {
+            if (self && b <= a) continue;
+            const Box& y = Bs[b];
+            if (!overlap(x, y)) continue;
+            if (cell(std::max(x.lo[0], y.lo[0])) != i || cell(std::max(x.lo[1], y.lo[1])) != j ||
+                cell(std::max(x.lo[2], y.lo[2])) != k)
+              continue;
+            fn(x.id, y.id);
+          }
+        }
   }
 }
 
@@ -278,7 +308,7 @@ int64_t oracle_dcd_ee(const float* x, const int* edges, int n_edges, const int* 
     boxes[i] = box_of(p, 2, pad, i);
   }
   std::vector<std::pair<int, int>> hits;
-  sweep_pairs(boxes, {}, true, [&](int a, int b) { hits.emplace_back(std::min(a, b), std::max(a, b)); });
+  grid_pairs(boxes, {}, true, [&](int a, int b) { hits.emplace_back(std::min(a, b), std::max(a, b)); });
   std::sort(hits.begin(), hits.end());
   Out out{rec, e0e1, cap, 0};
   for (auto& h : hits) {
@@ -306,7 +336,7 @@ int64_t oracle_dcd_pt(const float* x, int n_nodes, const int* tris, int n_tris, 
     tb[j] = box_of(p, 3, 0.0f, j);
   }
   std::vector<std::pair<int, int>> hits;
-  sweep_pairs(pb, tb, false, [&](int a, int b) { hits.emplace_back(a, b); });
+  grid_pairs(pb, tb, false, [&](int a, int b) { hits.emplace_back(a, b); });
   std::sort(hits.begin(), hits.end());
   Out out{rec, e0e1, cap, 0};
   for (auto& h : hits) {
@@ -333,7 +363,7 @@ int64_t oracle_ccd_candidates_ee(const float* x0, const float* x1, const int* ed
     boxes[i] = box_of(p, 4, padding, i);
   }
   std::vector<std::pair<int, int>> hits;
-  sweep_pairs(boxes, {}, true, [&](int a, int b) { hits.emplace_back(std::min(a, b), std::max(a, b)); });
+  grid_pairs(boxes, {}, true, [&](int a, int b) { hits.emplace_back(std::min(a, b), std::max(a, b)); });
   std::sort(hits.begin(), hits.end());
   Out out{rec, e0e1, cap, 0};
   for (auto& h : hits) {
@@ -360,7 +390,7 @@ int64_t oracle_ccd_candidates_pt(const float* x0, const float* x1, int n_nodes, 
     tb[j] = box_of(p, 6, padding, j);
   }
   std::vector<std::pair<int, int>> hits;
-  sweep_pairs(pb, tb, false, [&](int a, int b) { hits.emplace_back(a, b); });
+  grid_pairs(pb, tb, false, [&](int a, int b) { hits.emplace_back(a, b); });
   std::sort(hits.begin(), hits.end());
   Out out{rec, e0e1, cap, 0};
   for (auto& h : hits) {
