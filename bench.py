@@ -122,37 +122,40 @@ def main() -> int:
                           disable_self_collision_=True)
     b = torch.zeros_like(x)
 
-    # per-kernel-stage timers on torch's current stream (the library's stream)
-    stage_names = ("dcd", "hessian", "energy", "ccd")
-    ev = {k: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for k in stage_names}
-    acc = {k: 0.0 for k in stage_names}
     counts = {}
+    # Single-kernel entry points are bracketed with events on torch's current
+    # stream (the stream the library launches on); builds, queries and TOI
+    # launches inside the context are bracketed by the library itself.
+    kev = {k: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for k in ("hessian_ee", "hessian_pt", "energy")}
+    kacc = {k: 0.0 for k in kev}
 
     def step(timed: bool):
         b.zero_()
-        if timed:
-            ev["dcd"][0].record()
         S.detect_collision()
+        n_ee, n_pt = S.n_contacts, S.n_contacts_pt
+        trip = S.new_triplets()
         if timed:
-            ev["dcd"][1].record()
-            ev["hessian"][0].record()
-        S.compute_collision_hessian(b)
+            kev["hessian_ee"][0].record()
+        S.contact_hessian_ee(trip, b)
         if timed:
-            ev["hessian"][1].record()
-            ev["energy"][0].record()
+            kev["hessian_ee"][1].record()
+            kev["hessian_pt"][0].record()
+        S.contact_hessian_pt(trip, b)
+        if timed:
+            kev["hessian_pt"][1].record()
+            kev["energy"][0].record()
         e = S.collision_energy_raw()
         if timed:
-            ev["energy"][1].record()
-            ev["ccd"][0].record()
+            kev["energy"][1].record()
         toi = S.raw_toi(dx)
         if timed:
-            ev["ccd"][1].record()
             torch.cuda.synchronize()
-            for k in stage_names:
-                acc[k] += ev[k][0].elapsed_time(ev[k][1])
+            for k in kev:
+                kacc[k] += kev[k][0].elapsed_time(kev[k][1])
         n_ccd = S.last_ccd_counts()
-        counts.update(ee=S.n_contacts, pt=S.n_contacts_pt, ccd_ee=n_ccd[0], ccd_pt=n_ccd[1], toi=toi, energy=e)
-        return S.n_contacts + S.n_contacts_pt + n_ccd[0] + n_ccd[1]
+        counts.update(ee=n_ee, pt=n_pt, ccd_ee=n_ccd[0], ccd_pt=n_ccd[1], toi=toi, energy=e)
+        return n_ee + n_pt + n_ccd[0] + n_ccd[1]
 
     for _ in range(args.warmup):
         step(False)
@@ -163,17 +166,65 @@ def main() -> int:
         pairs += step(False)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    # stage breakdown in a separate, event-instrumented pass
+
+    # instrumented pass: per-kernel durations
     n_prof = max(1, min(args.steps, 50))
+    S.set_kernel_timing(True)
+    S.kernel_times()
     for _ in range(n_prof):
         step(True)
-    stage_ms = {k: acc[k] / n_prof for k in stage_names}
+    lib_times = S.kernel_times()
+    S.set_kernel_timing(False)
+    kernel_ms = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in lib_times.items()}
+    per_step_ms = {k: v[0] / n_prof for k, v in lib_times.items()}
+    for k in kacc:
+        kernel_ms[k] = kacc[k] / n_prof
+        per_step_ms[k] = kacc[k] / n_prof
 
-    # Roofline of the barrier assembly kernel family at this workload's
-    # contact count: algorithmic bytes per contact (PT 816 B, EE 864 B).
-    alg_bytes = counts["pt"] * 816 + counts["ee"] * 864
-    hess_ms = stage_ms["hessian"]
-    achieved = alg_bytes / (hess_ms * 1e-3) / 1e9 if hess_ms > 0 else 0.0
+    # Algorithmic bytes per launch ("Kernels, their bound and
+    # algorithmic bytes"): barrier assembly 816 B (PT) / 864 B (EE) per
+    # contact; TOI 112 B per candidate; a query reads the tree once (leaf:
+    # 32 B box + 4 B id + 4 B body; node: 32 B box + 8 B body range), one
+    # 32 B (EE) or 12/24 B (PT DCD/CCD) query box per query and writes 16 B
+    # (+8 B ids in DCD) per emitted pair.
+    E, F, V = int(soup.edges.shape[0]), int(soup.triangles.shape[0]), int(soup.x.shape[0])
+    tree = lambda n: 40 * n + 40 * (n + 6) // 7
+    alg = {
+        "hessian_ee": 864 * counts["ee"],
+        "hessian_pt": 816 * counts["pt"],
+        "energy": 112 * counts["ee"] + 64 * counts["pt"],
+        "query_dcd_ee": tree(E) + 32 * E + 24 * counts["ee"],
+        "query_dcd_pt": tree(F) + 12 * V + 24 * counts["pt"],
+        "query_ccd_ee": tree(E) + 32 * E + 16 * counts["ccd_ee"],
+        "query_ccd_pt": tree(F) + 24 * V + 16 * counts["ccd_pt"],
+        "toi_ee": 112 * counts["ccd_ee"],
+        "toi_pt": 112 * counts["ccd_pt"],
+    }
+    dominant = max((k for k in alg if kernel_ms.get(k, 0) > 0), key=lambda k: per_step_ms[k])
+    ach = alg[dominant] / (kernel_ms[dominant] * 1e-3) / 1e9
+
+    # the barrier Hessian kernel on a large synthetic contact stream (2^22 PT contacts)
+    from ipc_warp_amd import ops, scenes
+    n_big = 1 << 22
+    xs, cs = scenes.pair_stream("pt", n_big, seed=args.seed)
+    xs_t = torch.from_numpy(xs).to(dev)
+    cs_t = torch.from_numpy(cs).to(dev)
+    rows = torch.empty(16 * n_big, dtype=torch.int32, device=dev)
+    cols = torch.empty_like(rows)
+    vals = torch.empty((16 * n_big, 3, 3), dtype=torch.float32, device=dev)
+    bb = torch.zeros_like(xs_t)
+    l0 = float(np.float32(THICKNESS) * np.float32(2))
+    for _ in range(3):
+        ops.contact_hessian_pt(xs_t, cs_t, n_big, l0, KAPPA, rows, cols, vals, bb, 0)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        ops.contact_hessian_pt(xs_t, cs_t, n_big, l0, KAPPA, rows, cols, vals, bb, 0)
+    e1.record()
+    torch.cuda.synchronize()
+    big_ms = e0.elapsed_time(e1) / 10
+    big_gbs = 816 * n_big / (big_ms * 1e-3) / 1e9
+    del rows, cols, vals
 
     result = {
         "metric": METRIC,
@@ -192,22 +243,34 @@ def main() -> int:
             "workload": "wrecking-ball ABD: 64 icospheres (642 v / 1280 t, r=0.5), 4x4x4 grid spaced 1.001, "
                         "N(0,1) velocities, h=1e-2, dhat=1e-3, kappa=1e8, same-body pairs excluded",
             "seed": args.seed,
-            "vertices": int(soup.x.shape[0]),
-            "edges": int(soup.edges.shape[0]),
-            "triangles": int(soup.triangles.shape[0]),
+            "vertices": V,
+            "edges": E,
+            "triangles": F,
         },
         "pairs_per_step": {"ee_contacts": counts["ee"], "pt_contacts": counts["pt"],
                            "ccd_ee_candidates": counts["ccd_ee"], "ccd_pt_candidates": counts["ccd_pt"]},
-        "stage_ms": stage_ms,
+        "kernel_ms_per_launch": kernel_ms,
+        "kernel_ms_per_step": per_step_ms,
         "toi": counts["toi"],
         "roofline": {
-            "kernel": "contact_hessian_ee + contact_hessian_pt",
+            "kernel": dominant,
             "bound": "hbm",
-            "achieved": achieved,
+            "achieved": ach,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "frac": ach / HBM_PEAK_GBS,
             "traffic": None,
+            "algorithmic_bytes_per_launch": alg[dominant],
+            "avg_launch_ms": kernel_ms[dominant],
+        },
+        "hessian_stream": {
+            "kernel": "contact_hessian_pt",
+            "contacts": n_big,
+            "ms": big_ms,
+            "contacts_per_s": n_big / (big_ms * 1e-3),
+            "achieved": big_gbs,
+            "unit": "GB/s",
+            "frac": big_gbs / HBM_PEAK_GBS,
         },
     }
     if not args.no_cpu_baseline:

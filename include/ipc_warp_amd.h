@@ -150,6 +150,55 @@ int ipc_collision_free_step(ipc_ctx* ctx, const float* x, const float* dx, float
 /* Candidate counts (EE, PT) of the context's last collision_free_step. */
 int ipc_ctx_last_ccd_counts(ipc_ctx* ctx, int* n_ee, int* n_pt);
 
+/* ------------------------------------------------------ stage timing ---
+ * The role of RodBCBase.profile_timer (time_integrator_base.py:245-252):
+ * while enabled, each BVH build, query and TOI launch of the context is
+ * bracketed by HIP events on its stream. ipc_ctx_kernel_times waits for
+ * them and returns, per kind, the accumulated milliseconds and number of
+ * launches since the last call (arrays of IPC_TIMER_KINDS entries). */
+#define IPC_TIMER_BUILD_EDGE 0
+#define IPC_TIMER_BUILD_TRI 1
+#define IPC_TIMER_QUERY_DCD_EE 2
+#define IPC_TIMER_QUERY_DCD_PT 3
+#define IPC_TIMER_QUERY_CCD_EE 4
+#define IPC_TIMER_QUERY_CCD_PT 5
+#define IPC_TIMER_TOI_EE 6
+#define IPC_TIMER_TOI_PT 7
+#define IPC_TIMER_KINDS 8
+int ipc_ctx_set_timing(ipc_ctx* ctx, int enable);
+int ipc_ctx_kernel_times(ipc_ctx* ctx, double* total_ms, long long* calls);
+
+/* ================================================= learned warm start ===
+ * No counterpart in the reference: the MLP 48 -> 256 -> 256 -> 256 -> 3
+ * (GELU, fp32) that predicts a vertex's converged Newton displacement from
+ * 48 features (the layout is documented in ipc_warp_amd/warmstart.py).
+ * params: IPC_MLP_PARAMS floats, W1 [256][48], b1, W2 [256][256], b2,
+ * W3 [256][256], b3, W4 [3][256], b4 (W row-major, out x in).
+ * x float [n][48], y float [n][3]. */
+#define IPC_MLP_IN 48
+#define IPC_MLP_HIDDEN 256
+#define IPC_MLP_OUT 3
+#define IPC_MLP_PARAMS 144899
+/* Forward pass. work (float [6 * 256 * n]) keeps Z1, H1, Z2, H2, Z3, H3 for
+ * a backward pass; with work = NULL, scratch (float [3 * 256 * n]) holds the
+ * activations. */
+int ipc_mlp_forward(const float* params, int n, const float* x, float* y, float* work, float* scratch, void* stream);
+/* Backward pass of the forward pass that filled `work`: ADDS the gradient of
+ * the loss with respect to the parameters into grad (IPC_MLP_PARAMS floats)
+ * given dy = dLoss/dy; scratch float [2 * 256 * n]. */
+int ipc_mlp_backward(const float* params, int n, const float* x, const float* work, const float* dy, float* grad,
+                     float* scratch, void* stream);
+/* Weighted mean squared error: loss[0] += sum_i w_i |y_i - t_i|^2 / weight_sum
+ * (double, device) and dy_i = 2 w_i (y_i - t_i) / weight_sum; weight may be
+ * NULL (all ones); dy or loss may be NULL. */
+int ipc_mlp_mse(const float* y, const float* target, const float* weight, int n, float weight_sum, float* dy,
+                double* loss, void* stream);
+/* One Adam step (bias-corrected, step >= 1) over n parameters. */
+int ipc_mlp_adam(float* params, const float* grad, float* m, float* v, long long n, float lr, float beta1, float beta2,
+                 float eps, int step, void* stream);
+/* p[0..n) = 0 (a fill kernel, safe inside captured graphs). */
+int ipc_zero_f32(float* p, long long n, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -33,7 +33,18 @@ _SIGNATURES = {
     "ipc_ccd_candidates_pt": ([_vp, _vp, _vp, _c_float, _c_int, _vp, _vp, _c_int, _vp, _vp], _c_int),
     "ipc_collision_free_step": ([_vp, _vp, _vp, _c_float, _c_int, ctypes.POINTER(_c_double), _vp], _c_int),
     "ipc_ctx_last_ccd_counts": ([_vp, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int)], _c_int),
+    "ipc_ctx_set_timing": ([_vp, _c_int], _c_int),
+    "ipc_ctx_kernel_times": ([_vp, ctypes.POINTER(_c_double), ctypes.POINTER(ctypes.c_longlong)], _c_int),
+    "ipc_mlp_forward": ([_vp, _c_int, _vp, _vp, _vp, _vp, _vp], _c_int),
+    "ipc_mlp_backward": ([_vp, _c_int, _vp, _vp, _vp, _vp, _vp, _vp], _c_int),
+    "ipc_mlp_mse": ([_vp, _vp, _vp, _c_int, _c_float, _vp, _vp, _vp], _c_int),
+    "ipc_mlp_adam": ([_vp, _vp, _vp, _vp, ctypes.c_longlong, _c_float, _c_float, _c_float, _c_float, _c_int, _vp],
+                     _c_int),
+    "ipc_zero_f32": ([_vp, ctypes.c_longlong, _vp], _c_int),
 }
+
+TIMER_KINDS = ("build_edge", "build_tri", "query_dcd_ee", "query_dcd_pt", "query_ccd_ee", "query_ccd_pt",
+               "toi_ee", "toi_pt")
 
 EXPORTED = tuple(_SIGNATURES)
 

@@ -201,26 +201,52 @@ class ContactSolverBase:
         check(lib().ipc_ctx_last_ccd_counts(self._ctx, ctypes.byref(a), ctypes.byref(b)), "ctx_last_ccd_counts")
         return a.value, b.value
 
+    def set_kernel_timing(self, enable: bool = True):
+        """Bracket the context's builds, queries and TOI launches with HIP events."""
+        check(lib().ipc_ctx_set_timing(self._ctx, int(enable)), "ctx_set_timing")
+
+    def kernel_times(self) -> dict:
+        """{kind: (total_ms, launches)} since the last call (waits for the events)."""
+        from ._lib import TIMER_KINDS
+
+        ms = (ctypes.c_double * len(TIMER_KINDS))()
+        n = (ctypes.c_longlong * len(TIMER_KINDS))()
+        check(lib().ipc_ctx_kernel_times(self._ctx, ms, n), "ctx_kernel_times")
+        return {k: (ms[i], n[i]) for i, k in enumerate(TIMER_KINDS)}
+
     def line_search_upper_bound(self, dx) -> float:
         return self.collision_free_step(dx)
 
-    def compute_collision_hessian(self, b: torch.Tensor) -> Triplets:
-        """Contact part of RodComplexBC.compute_A (dynamic_contacts.py:280-288).
+    def new_triplets(self) -> Triplets:
+        """Triplets for (n_ee + n_pt) * 16 blocks (dynamic_contacts.py:282-286).
 
-        Allocates zeroed triplets for (n_ee + n_pt) * 16 blocks, launches the
-        EE kernel then the PT kernel at offset n_ee, and adds forces into b.
+        The kernels write every block they own (inactive contacts get zero
+        blocks), so the buffers need no zeroing.
         """
-        n_ee, n_pt = self.n_contacts, self.n_contacts_pt
-        nnz = (n_ee + n_pt) * 16
-        trip = Triplets(
-            rows=torch.zeros(nnz, dtype=torch.int32, device=self.device),
-            cols=torch.zeros(nnz, dtype=torch.int32, device=self.device),
-            vals=torch.zeros((nnz, 3, 3), dtype=torch.float32, device=self.device),
+        nnz = (self.n_contacts + self.n_contacts_pt) * 16
+        return Triplets(
+            rows=torch.empty(nnz, dtype=torch.int32, device=self.device),
+            cols=torch.empty(nnz, dtype=torch.int32, device=self.device),
+            vals=torch.empty((nnz, 3, 3), dtype=torch.float32, device=self.device),
         )
-        ops.contact_hessian_ee(self.x, self.xcs, self.contacts_new.list, n_ee, self.l0, self.contact_stiffness,
-                               trip.rows, trip.cols, trip.vals, b, 0)
-        ops.contact_hessian_pt(self.x, self.contacts_pt.list, n_pt, self.l0, self.contact_stiffness, trip.rows,
-                               trip.cols, trip.vals, b, n_ee)
+
+    def contact_hessian_ee(self, trip: Triplets, b: torch.Tensor):
+        """wp.launch(contact_hessian_ee, dim=n_ee, ...) (dynamic_contacts.py:287)."""
+        ops.contact_hessian_ee(self.x, self.xcs, self.contacts_new.list, self.n_contacts, self.l0,
+                               self.contact_stiffness, trip.rows, trip.cols, trip.vals, b, 0)
+
+    def contact_hessian_pt(self, trip: Triplets, b: torch.Tensor):
+        """wp.launch(contact_hessian_pt, dim=n_pt, ..., offset=n_ee) (dynamic_contacts.py:288)."""
+        ops.contact_hessian_pt(self.x, self.contacts_pt.list, self.n_contacts_pt, self.l0, self.contact_stiffness,
+                               trip.rows, trip.cols, trip.vals, b, self.n_contacts)
+
+    def compute_collision_hessian(self, b: torch.Tensor) -> Triplets:
+        """Contact part of RodComplexBC.compute_A (dynamic_contacts.py:280-288):
+        triplets for (n_ee + n_pt) * 16 blocks, the EE kernel, then the PT
+        kernel at offset n_ee; forces are added into b."""
+        trip = self.new_triplets()
+        self.contact_hessian_ee(trip, b)
+        self.contact_hessian_pt(trip, b)
         self.collision_triplets = trip
         return trip
 

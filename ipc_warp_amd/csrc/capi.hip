@@ -140,4 +140,49 @@ int ipc_collision_time_pt(const float* x0, const float* x1, int n_vertices, cons
   return collision_time(false, x0, x1, n_vertices, quads, n, toi, toi_min, stream);
 }
 
+int ipc_mlp_forward(const float* params, int n, const float* x, float* y, float* work, float* scratch, void* stream) {
+  if (n < 0) return fail(IPC_EINVAL, "mlp_forward: negative n");
+  if (n == 0) return IPC_OK;
+  if (!params || !x || !y || (!work && !scratch)) return fail(IPC_EINVAL, "mlp_forward: null pointer");
+  if ((long long)n * IPC_MLP_HIDDEN * 6 > (1ll << 31)) return fail(IPC_EINVAL, "mlp_forward: batch too large");
+  return hip_status(ipcd::mlp_forward(params, n, x, y, work, scratch, static_cast<hipStream_t>(stream)),
+                    "mlp_forward");
+}
+
+int ipc_mlp_backward(const float* params, int n, const float* x, const float* work, const float* dy, float* grad,
+                     float* scratch, void* stream) {
+  if (n < 0) return fail(IPC_EINVAL, "mlp_backward: negative n");
+  if (n == 0) return IPC_OK;
+  if (!params || !x || !work || !dy || !grad || !scratch) return fail(IPC_EINVAL, "mlp_backward: null pointer");
+  if ((long long)n * IPC_MLP_HIDDEN * 6 > (1ll << 31)) return fail(IPC_EINVAL, "mlp_backward: batch too large");
+  return hip_status(ipcd::mlp_backward(params, n, x, work, dy, grad, scratch, static_cast<hipStream_t>(stream)),
+                    "mlp_backward");
+}
+
+int ipc_mlp_mse(const float* y, const float* target, const float* weight, int n, float weight_sum, float* dy,
+                double* loss, void* stream) {
+  if (n < 0) return fail(IPC_EINVAL, "mlp_mse: negative n");
+  if (n == 0) return IPC_OK;
+  if (!y || !target) return fail(IPC_EINVAL, "mlp_mse: null pointer");
+  if (!(weight_sum > 0.0f)) return fail(IPC_EINVAL, "mlp_mse: weight_sum must be > 0");
+  return hip_status(ipcd::mlp_mse(y, target, weight, n, weight_sum, dy, loss, static_cast<hipStream_t>(stream)),
+                    "mlp_mse");
+}
+
+int ipc_mlp_adam(float* params, const float* grad, float* m, float* v, long long n, float lr, float beta1, float beta2,
+                 float eps, int step, void* stream) {
+  if (n < 0 || step < 1) return fail(IPC_EINVAL, "mlp_adam: negative n or step < 1");
+  if (n == 0) return IPC_OK;
+  if (!params || !grad || !m || !v) return fail(IPC_EINVAL, "mlp_adam: null pointer");
+  return hip_status(ipcd::mlp_adam(params, grad, m, v, n, lr, beta1, beta2, eps, step, static_cast<hipStream_t>(stream)),
+                    "mlp_adam");
+}
+
+int ipc_zero_f32(float* p, long long n, void* stream) {
+  if (n < 0) return fail(IPC_EINVAL, "zero_f32: negative n");
+  if (n == 0) return IPC_OK;
+  if (!p) return fail(IPC_EINVAL, "zero_f32: null pointer");
+  return hip_status(ipcd::zero_f32(p, n, static_cast<hipStream_t>(stream)), "zero_f32");
+}
+
 }  // extern "C"

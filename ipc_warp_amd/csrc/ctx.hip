@@ -4,6 +4,7 @@
 // kernels with them.
 #include <cstdio>
 #include <cstring>
+#include <vector>
 
 #include "../../include/ipc_warp_amd.h"
 #include "cull.hpp"
@@ -21,6 +22,12 @@ struct ipc_ctx {
   DevBuf small;            // [0] uint64 toi bits, [8..] int counters
   void* host = nullptr;    // pinned readback area
   int last_counts[2] = {0, 0};
+  // optional per-kernel timing (HIP events recorded on the launch stream)
+  bool timing = false;
+  struct Pending { int kind; hipEvent_t a, b; };
+  std::vector<Pending> pending;
+  double total_ms[IPC_TIMER_KINDS] = {0};
+  long long calls[IPC_TIMER_KINDS] = {0};
 };
 
 namespace {
@@ -42,6 +49,22 @@ unsigned long long* toi_bits(ipc_ctx* c) { return c->small.as<unsigned long long
 int* counters(ipc_ctx* c) { return reinterpret_cast<int*>(c->small.as<char>() + 64); }
 
 constexpr unsigned long long kOneBits = 0x3FF0000000000000ull;
+
+// Runs `launch` between two events on `s` when the context is timing.
+template <typename F>
+hipError_t timed(ipc_ctx* c, int kind, hipStream_t s, F launch) {
+  if (!c->timing) return launch();
+  hipEvent_t a, b;
+  hipError_t e = hipEventCreate(&a);
+  if (e != hipSuccess) return e;
+  e = hipEventCreate(&b);
+  if (e != hipSuccess) { (void)hipEventDestroy(a); return e; }
+  (void)hipEventRecord(a, s);
+  e = launch();
+  (void)hipEventRecord(b, s);
+  c->pending.push_back({kind, a, b});
+  return e;
+}
 
 }  // namespace
 
@@ -94,6 +117,7 @@ int ipc_ctx_create(ipc_ctx** out, int n_vertices, const int* edges, int n_edges,
 int ipc_ctx_destroy(ipc_ctx* c) {
   if (!c) return IPC_OK;
   (void)hipDeviceSynchronize();
+  for (auto& p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
   c->edges.release(); c->tris.release(); c->body.release();
   c->edge_bvh.release(); c->tri_bvh.release();
   c->x1.release(); c->cand_ee.release(); c->cand_pt.release(); c->small.release();
@@ -113,7 +137,11 @@ int ipc_detect_collision_ee(ipc_ctx* c, const float* x, float thickness, float b
   int st = ctx_hip(launch_zero_i32(count, 1, s), "detect_collision_ee");
   if (st || c->ne == 0) return st;
   float pad = thickness + buffer;
-  st = ctx_hip(build_bvh(c->edge_bvh, BOX_EDGE, c->ne, c->edges.as<int>(), c->body.as<int>(), x, nullptr, pad, s), "edge BVH");
+  st = ctx_hip(timed(c, IPC_TIMER_BUILD_EDGE, s, [&] {
+                 return build_bvh(c->edge_bvh, BOX_EDGE, c->ne, c->edges.as<int>(), c->body.as<int>(), x, nullptr,
+                                  pad, s);
+               }),
+               "edge BVH");
   if (st) return st;
   QueryLaunch q{};
   q.mode = Q_DCD_EE;
@@ -122,8 +150,8 @@ int ipc_detect_collision_ee(ipc_ctx* c, const float* x, float thickness, float b
   q.edges = c->edges.as<int>();
   q.body = c->body.as<int>();
   q.x0 = x;
-  q.qlo = c->edge_bvh.lo.as<float4>();
-  q.qhi = c->edge_bvh.hi.as<float4>();
+  q.qlo = c->edge_bvh.leaf_lo.as<float4>();
+  q.qhi = c->edge_bvh.leaf_hi.as<float4>();
   q.thickness = thickness;
   q.exclude_same_body = exclude_same_body;
   q.bvh = &c->edge_bvh;
@@ -131,7 +159,7 @@ int ipc_detect_collision_ee(ipc_ctx* c, const float* x, float thickness, float b
   q.ids = reinterpret_cast<int2*>(e0e1);
   q.capacity = capacity;
   q.count = count;
-  return ctx_hip(run_query(q, s), "detect_collision_ee query");
+  return ctx_hip(timed(c, IPC_TIMER_QUERY_DCD_EE, s, [&] { return run_query(q, s); }), "detect_collision_ee query");
 }
 
 // point_triangle_collision over a freshly built triangle BVH
@@ -143,7 +171,11 @@ int ipc_detect_collision_pt(ipc_ctx* c, const float* x, float thickness, float b
   hipStream_t s = static_cast<hipStream_t>(stream);
   int st = ctx_hip(launch_zero_i32(count, 1, s), "detect_collision_pt");
   if (st || c->nf == 0 || c->nv == 0) return st;
-  st = ctx_hip(build_bvh(c->tri_bvh, BOX_TRI, c->nf, c->tris.as<int>(), c->body.as<int>(), x, nullptr, 0.0f, s), "triangle BVH");
+  st = ctx_hip(timed(c, IPC_TIMER_BUILD_TRI, s, [&] {
+                 return build_bvh(c->tri_bvh, BOX_TRI, c->nf, c->tris.as<int>(), c->body.as<int>(), x, nullptr, 0.0f,
+                                  s);
+               }),
+               "triangle BVH");
   if (st) return st;
   QueryLaunch q{};
   q.mode = Q_DCD_PT;
@@ -159,7 +191,7 @@ int ipc_detect_collision_pt(ipc_ctx* c, const float* x, float thickness, float b
   q.ids = reinterpret_cast<int2*>(e0e1);
   q.capacity = capacity;
   q.count = count;
-  return ctx_hip(run_query(q, s), "detect_collision_pt query");
+  return ctx_hip(timed(c, IPC_TIMER_QUERY_DCD_PT, s, [&] { return run_query(q, s); }), "detect_collision_pt query");
 }
 
 // Candidate sets of edge_edge_toi / point_triangle_toi between x0 and x1.
@@ -170,7 +202,10 @@ int ipc_ccd_candidates_ee(ipc_ctx* c, const float* x0, const float* x1, float pa
   hipStream_t s = static_cast<hipStream_t>(stream);
   int st = ctx_hip(launch_zero_i32(count, 1, s), "ccd_candidates_ee");
   if (st || c->ne == 0) return st;
-  st = ctx_hip(build_bvh(c->edge_bvh, BOX_SWEPT_EDGE, c->ne, c->edges.as<int>(), c->body.as<int>(), x0, x1, padding, s),
+  st = ctx_hip(timed(c, IPC_TIMER_BUILD_EDGE, s, [&] {
+                 return build_bvh(c->edge_bvh, BOX_SWEPT_EDGE, c->ne, c->edges.as<int>(), c->body.as<int>(), x0, x1,
+                                  padding, s);
+               }),
                "swept edge BVH");
   if (st) return st;
   QueryLaunch q{};
@@ -181,15 +216,15 @@ int ipc_ccd_candidates_ee(ipc_ctx* c, const float* x0, const float* x1, float pa
   q.body = c->body.as<int>();
   q.x0 = x0;
   q.x1 = x1;
-  q.qlo = c->edge_bvh.lo.as<float4>();
-  q.qhi = c->edge_bvh.hi.as<float4>();
+  q.qlo = c->edge_bvh.leaf_lo.as<float4>();
+  q.qhi = c->edge_bvh.leaf_hi.as<float4>();
   q.exclude_same_body = exclude_same_body;
   q.bvh = &c->edge_bvh;
   q.rec = reinterpret_cast<int4*>(quads);
   q.ids = reinterpret_cast<int2*>(ids);
   q.capacity = capacity;
   q.count = count;
-  return ctx_hip(run_query(q, s), "ccd_candidates_ee query");
+  return ctx_hip(timed(c, IPC_TIMER_QUERY_CCD_EE, s, [&] { return run_query(q, s); }), "ccd_candidates_ee query");
 }
 
 int ipc_ccd_candidates_pt(ipc_ctx* c, const float* x0, const float* x1, float padding, int exclude_same_body,
@@ -199,7 +234,10 @@ int ipc_ccd_candidates_pt(ipc_ctx* c, const float* x0, const float* x1, float pa
   hipStream_t s = static_cast<hipStream_t>(stream);
   int st = ctx_hip(launch_zero_i32(count, 1, s), "ccd_candidates_pt");
   if (st || c->nf == 0 || c->nv == 0) return st;
-  st = ctx_hip(build_bvh(c->tri_bvh, BOX_SWEPT_TRI, c->nf, c->tris.as<int>(), c->body.as<int>(), x0, x1, padding, s),
+  st = ctx_hip(timed(c, IPC_TIMER_BUILD_TRI, s, [&] {
+                 return build_bvh(c->tri_bvh, BOX_SWEPT_TRI, c->nf, c->tris.as<int>(), c->body.as<int>(), x0, x1,
+                                  padding, s);
+               }),
                "swept triangle BVH");
   if (st) return st;
   QueryLaunch q{};
@@ -216,7 +254,7 @@ int ipc_ccd_candidates_pt(ipc_ctx* c, const float* x0, const float* x1, float pa
   q.ids = reinterpret_cast<int2*>(ids);
   q.capacity = capacity;
   q.count = count;
-  return ctx_hip(run_query(q, s), "ccd_candidates_pt query");
+  return ctx_hip(timed(c, IPC_TIMER_QUERY_CCD_PT, s, [&] { return run_query(q, s); }), "ccd_candidates_pt query");
 }
 
 // collision_free_step (contact.py:539-585) before its 0.9 factor: x1 = x - dx,
@@ -248,12 +286,16 @@ int ipc_collision_free_step(ipc_ctx* c, const float* x, const float* dx, float p
     if (st) return st;
     // TOI over the first min(count, capacity) candidates: the kernels read the
     // device count, so no host round trip is needed between the phases.
-    st = ctx_hip(launch_collision_time_counted(true, x, c->x1.as<float>(), c->nv, c->cand_ee.as<int>(), cnt + 0,
-                                               c->cap_ee, toi_bits(c), s),
+    st = ctx_hip(timed(c, IPC_TIMER_TOI_EE, s, [&] {
+                   return launch_collision_time_counted(true, x, c->x1.as<float>(), c->nv, c->cand_ee.as<int>(),
+                                                        cnt + 0, c->cap_ee, toi_bits(c), s);
+                 }),
                  "collision_free_step EE TOI");
     if (!st)
-      st = ctx_hip(launch_collision_time_counted(false, x, c->x1.as<float>(), c->nv, c->cand_pt.as<int>(), cnt + 1,
-                                                 c->cap_pt, toi_bits(c), s),
+      st = ctx_hip(timed(c, IPC_TIMER_TOI_PT, s, [&] {
+                     return launch_collision_time_counted(false, x, c->x1.as<float>(), c->nv, c->cand_pt.as<int>(),
+                                                          cnt + 1, c->cap_pt, toi_bits(c), s);
+                   }),
                    "collision_free_step PT TOI");
     if (st) return st;
     st = ctx_hip(hipMemcpyAsync(c->host, c->small.p, 128, hipMemcpyDeviceToHost, s), "collision_free_step readback");
@@ -277,6 +319,41 @@ int ipc_collision_free_step(ipc_ctx* c, const float* x, const float* dx, float p
     }
   }
   return ctx_fail(IPC_EINVAL, "collision_free_step: candidate list kept overflowing");
+}
+
+// Per-kernel timing: while enabled, builds, queries and TOI launches are
+// bracketed by HIP events on their stream. ipc_ctx_kernel_times waits for
+// the recorded events and returns the accumulated milliseconds and launch
+// counts per kind (IPC_TIMER_*), then resets them.
+int ipc_ctx_set_timing(ipc_ctx* c, int enable) {
+  if (!c) return ctx_fail(IPC_EINVAL, "ctx_set_timing: null context");
+  c->timing = enable != 0;
+  return IPC_OK;
+}
+
+int ipc_ctx_kernel_times(ipc_ctx* c, double* total_ms, long long* calls) {
+  if (!c || !total_ms || !calls) return ctx_fail(IPC_EINVAL, "ctx_kernel_times: null pointer");
+  for (auto& p : c->pending) {
+    float ms = 0.f;
+    hipError_t e = hipEventSynchronize(p.b);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, p.a, p.b);
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+    if (e != hipSuccess) {
+      c->pending.clear();
+      return ctx_hip(e, "ctx_kernel_times");
+    }
+    c->total_ms[p.kind] += ms;
+    c->calls[p.kind] += 1;
+  }
+  c->pending.clear();
+  for (int k = 0; k < IPC_TIMER_KINDS; ++k) {
+    total_ms[k] = c->total_ms[k];
+    calls[k] = c->calls[k];
+    c->total_ms[k] = 0.0;
+    c->calls[k] = 0;
+  }
+  return IPC_OK;
 }
 
 // Candidate counts of the last collision_free_step (EE, PT).

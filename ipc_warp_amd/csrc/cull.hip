@@ -61,40 +61,45 @@ __device__ __forceinline__ bool overlaps(const float qlo[3], const float qhi[3],
 // mode 1: triangle bounds +- pad (wp.Mesh bounds when pad = 0)
 // mode 2: swept edge box over x0, x1, +- pad (swept_edge_aabbs, ccd/ccd.py:166-186)
 // mode 3: swept triangle box over x0, x1, +- pad (swept_triangle_aabbs, ccd/ccd.py:188-210)
-__global__ void __launch_bounds__(256) prim_boxes_kernel(int mode, int n, const int* __restrict__ prims,
-                                                         const float* __restrict__ x0, const float* __restrict__ x1,
-                                                         float pad, float4* __restrict__ lo4, float4* __restrict__ hi4,
-                                                         int* __restrict__ bounds) {
+__device__ __forceinline__ void prim_box(int mode, int p, const int* __restrict__ prims, const float* __restrict__ x0,
+                                         const float* __restrict__ x1, float pad, float4& lo4, float4& hi4) {
+  int k = (mode == 0 || mode == 2) ? 2 : 3;
+  V3<float> lo, hi;
+  lo = hi = ldx(x0, prims[k * p]);
+  for (int j = 1; j < k; ++j) {
+    V3<float> q = ldx(x0, prims[k * p + j]);
+    lo = vmin(lo, q);
+    hi = vmax(hi, q);
+  }
+  if (mode >= 2) {
+    for (int j = 0; j < k; ++j) {
+      V3<float> q = ldx(x1, prims[k * p + j]);
+      lo = vmin(lo, q);
+      hi = vmax(hi, q);
+    }
+  }
+  V3<float> pv = mk(pad, pad, pad);
+  lo = lo - pv;
+  hi = hi + pv;
+  lo4 = make_float4(lo.x, lo.y, lo.z, 0.f);
+  hi4 = make_float4(hi.x, hi.y, hi.z, 0.f);
+}
+
+// Scene bounds of the primitive box centres (ordered-int atomics), for the
+// Morton quantisation of a re-sort.
+__global__ void __launch_bounds__(256) centre_bounds_kernel(int mode, int n, const int* __restrict__ prims,
+                                                            const float* __restrict__ x0, const float* __restrict__ x1,
+                                                            float pad, int* __restrict__ bounds) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   float c[3] = {0.f, 0.f, 0.f};
   bool live = i < n;
   if (live) {
-    int k = (mode == 0 || mode == 2) ? 2 : 3;
-    V3<float> lo, hi;
-    int v0 = prims[k * i];
-    lo = hi = ldx(x0, v0);
-    for (int j = 1; j < k; ++j) {
-      V3<float> p = ldx(x0, prims[k * i + j]);
-      lo = vmin(lo, p);
-      hi = vmax(hi, p);
-    }
-    if (mode >= 2) {
-      for (int j = 0; j < k; ++j) {
-        V3<float> p = ldx(x1, prims[k * i + j]);
-        lo = vmin(lo, p);
-        hi = vmax(hi, p);
-      }
-    }
-    V3<float> pv = mk(pad, pad, pad);
-    lo = lo - pv;
-    hi = hi + pv;
-    lo4[i] = make_float4(lo.x, lo.y, lo.z, 0.f);
-    hi4[i] = make_float4(hi.x, hi.y, hi.z, 0.f);
+    float4 lo, hi;
+    prim_box(mode, i, prims, x0, x1, pad, lo, hi);
     c[0] = 0.5f * (lo.x + hi.x);
     c[1] = 0.5f * (lo.y + hi.y);
     c[2] = 0.5f * (lo.z + hi.z);
   }
-  // block-wide min / max of centres, then one atomic per block and axis
   __shared__ int smin_[3][4], smax_[3][4];
   int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int a = 0; a < 3; ++a) {
@@ -116,12 +121,14 @@ __global__ void __launch_bounds__(256) prim_boxes_kernel(int mode, int n, const 
   }
 }
 
-__global__ void __launch_bounds__(256) morton_kernel(int n, const float4* __restrict__ lo4,
-                                                     const float4* __restrict__ hi4, const int* __restrict__ bounds,
+__global__ void __launch_bounds__(256) morton_kernel(int mode, int n, const int* __restrict__ prims,
+                                                     const float* __restrict__ x0, const float* __restrict__ x1,
+                                                     float pad, const int* __restrict__ bounds,
                                                      unsigned long long* __restrict__ keys) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  float4 lo = lo4[i], hi = hi4[i];
+  float4 lo, hi;
+  prim_box(mode, i, prims, x0, x1, pad, lo, hi);
   float c[3] = {0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z)};
   unsigned q[3];
   for (int a = 0; a < 3; ++a) {
@@ -137,8 +144,9 @@ __global__ void __launch_bounds__(256) morton_kernel(int n, const float4* __rest
 }
 
 // ------------------------------------------------------------ hierarchy ----
-constexpr int kReduceBlock = 512;                     // threads of the blocked reduction
-constexpr int kLeavesPerBlock = kFan * kReduceBlock;  // 4096 = 8^4 leaves per workgroup
+constexpr int kReduceBlock = 64;                      // threads of the leaf/level kernel (one wave)
+constexpr int kLeavesPerBlock = kFan * kReduceBlock;  // 512 = 8^3 leaves per workgroup
+constexpr int kInBlockLevels = 3;                     // levels 1..3 reduced inside the workgroup
 
 __device__ __forceinline__ void box_merge(float4& lo, float4& hi, float4 l, float4 h) {
   lo = make_float4(fminf(lo.x, l.x), fminf(lo.y, l.y), fminf(lo.z, l.z), 0.f);
@@ -150,17 +158,19 @@ __device__ __forceinline__ void box_empty(float4& lo, float4& hi) {
   hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
 }
 
-// Leaves in Morton order (boxes gathered through the sorted ids) and levels
-// 1..4 of the hierarchy. Workgroup b owns leaves [4096 b, 4096 (b + 1)).
-__global__ void __launch_bounds__(kReduceBlock) reduce_levels_kernel(
-    int n, int prim_stride, const int* __restrict__ prims, const int* __restrict__ body,
-    const unsigned long long* __restrict__ keys, const float4* __restrict__ lo4, const float4* __restrict__ hi4,
+// Leaves in Morton order and levels 1..3 of the hierarchy. Workgroup b owns
+// leaves [512 b, 512 (b + 1)); each leaf's box is computed here from the
+// topology and positions of primitive keys[k] (low 32 bits).
+__global__ void __launch_bounds__(kReduceBlock) leaf_levels_kernel(
+    int mode, int n, const int* __restrict__ prims, const int* __restrict__ body, const float* __restrict__ x0,
+    const float* __restrict__ x1, float pad, const unsigned long long* __restrict__ keys,
     float4* __restrict__ leaf_lo, float4* __restrict__ leaf_hi, int* __restrict__ leaf_id, int* __restrict__ leaf_body,
     float4* __restrict__ node_lo, float4* __restrict__ node_hi, int2* __restrict__ node_body, LevelInfo info) {
   __shared__ float4 slo[kReduceBlock], shi[kReduceBlock];
   __shared__ int2 sbody[kReduceBlock];
   int t = threadIdx.x;
   int g = blockIdx.x * kReduceBlock + t;  // level-1 node
+  int stride = (mode == 0 || mode == 2) ? 2 : 3;
   float4 blo, bhi;
   box_empty(blo, bhi);
   int2 br = make_int2(0x7fffffff, (int)0x80000000);
@@ -169,8 +179,9 @@ __global__ void __launch_bounds__(kReduceBlock) reduce_levels_kernel(
     int k = g * kFan + c;
     if (k < n) {
       int prim = static_cast<int>(keys[k] & 0xffffffffu);
-      float4 l = lo4[prim], h = hi4[prim];
-      int bd = body[prims[prim_stride * prim]];
+      float4 l, h;
+      prim_box(mode, prim, prims, x0, x1, pad, l, h);
+      int bd = body[prims[stride * prim]];
       leaf_lo[k] = l;
       leaf_hi[k] = h;
       leaf_id[k] = prim;
@@ -180,7 +191,7 @@ __global__ void __launch_bounds__(kReduceBlock) reduce_levels_kernel(
       br.y = max(br.y, bd);
     }
   }
-  if (1 <= info.top && g < info.count[1]) {
+  if (g < info.count[1]) {
     node_lo[info.offset[1] + g] = blo;
     node_hi[info.offset[1] + g] = bhi;
     node_body[info.offset[1] + g] = br;
@@ -189,7 +200,7 @@ __global__ void __launch_bounds__(kReduceBlock) reduce_levels_kernel(
   shi[t] = bhi;
   sbody[t] = br;
   int width = kReduceBlock;
-  for (int L = 2; L <= 4; ++L) {
+  for (int L = 2; L <= kInBlockLevels; ++L) {
     __syncthreads();
     width /= kFan;
     float4 l, h;
@@ -218,10 +229,10 @@ __global__ void __launch_bounds__(kReduceBlock) reduce_levels_kernel(
   }
 }
 
-// Levels 5..top in one workgroup (level 4 holds at most 2^27 / 8^4 nodes).
+// Levels kInBlockLevels + 1 .. top in one workgroup.
 __global__ void __launch_bounds__(1024) reduce_top_kernel(float4* __restrict__ node_lo, float4* __restrict__ node_hi,
                                                          int2* __restrict__ node_body, LevelInfo info) {
-  for (int L = 5; L <= info.top; ++L) {
+  for (int L = kInBlockLevels + 1; L <= info.top; ++L) {
     for (int g = threadIdx.x; g < info.count[L]; g += blockDim.x) {
       float4 l, h;
       box_empty(l, h);
@@ -273,14 +284,13 @@ constexpr int kMaskStack = 12;                       // > kMaxLevels - 1 entries
 //  3 CCD PT: swept point box +- pad against swept triangle BVH; not
 //    incident, not same body. Record (i, t0, t1, t2)  ccd/ccd.py:360-394
 struct QueryArgs {
-  int mode;
   int n_query;
   const int* edges;
   const int* tris;
   const int* body;
   const float* x0;
   const float* x1;
-  const float4* qlo;  // query boxes (modes 0, 2: the primitives' own boxes)
+  const float4* qlo;  // modes 0, 2: query box of slot k (the tree's leaf k)
   const float4* qhi;
   const int* order;   // optional query order (slot -> query primitive)
   float pad;          // mode 1: 2 (th + buf); mode 3: CCD padding
@@ -303,21 +313,22 @@ struct QueryArgs {
   int* count;
 };
 
+template <int MODE>
 __device__ __forceinline__ bool accept_pair(const QueryArgs& A, int i, int j, int4& rec) {
-  if (A.mode == 0 || A.mode == 2) {
+  if (MODE == 0 || MODE == 2) {
     if (!(i < j)) return false;
     int a0 = A.edges[2 * i], a1 = A.edges[2 * i + 1], b0 = A.edges[2 * j], b1 = A.edges[2 * j + 1];
     if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return false;
     if (A.exclude_same_body && A.body[a0] == A.body[b0]) return false;
     rec = make_int4(a0, a1, b0, b1);
-    if (A.mode == 2) return true;
+    if (MODE == 2) return true;
     EdgeClosest ec = closest_point_edge_edge(ldx(A.x0, a0), ldx(A.x0, a1), ldx(A.x0, b0), ldx(A.x0, b1), 1.0e-6f);
     return ec.dist < A.thickness * 2.0f;
   }
   int t0 = A.tris[3 * j], t1 = A.tris[3 * j + 1], t2 = A.tris[3 * j + 2];
   if (t0 == i || t1 == i || t2 == i) return false;
   if (A.exclude_same_body && A.body[i] == A.body[t0]) return false;
-  if (A.mode == 3) {
+  if (MODE == 3) {
     rec = make_int4(i, t0, t1, t2);
     return true;
   }
@@ -337,6 +348,7 @@ __device__ __forceinline__ unsigned long long mask_entry(int level, unsigned mas
 // child r; the group ballot of overlapping children becomes one stack entry
 // (level, first child, mask). Every lane of a group computes and writes the
 // same stack values, so each lane only ever reads back its own LDS writes.
+template <int MODE>
 __global__ void __launch_bounds__(kQueryBlock) query_kernel(QueryArgs A) {
   __shared__ unsigned long long stack[kGroupsPerBlock][kMaskStack];
   int tid = threadIdx.x;
@@ -350,14 +362,14 @@ __global__ void __launch_bounds__(kQueryBlock) query_kernel(QueryArgs A) {
   int qbody = -1;
   if (live) {
     i = A.order ? A.order[slot] : slot;
-    if (A.mode == 0 || A.mode == 2) {
-      float4 l = A.qlo[i], h = A.qhi[i];
+    if (MODE == 0 || MODE == 2) {
+      float4 l = A.qlo[slot], h = A.qhi[slot];
       qlo[0] = l.x; qlo[1] = l.y; qlo[2] = l.z;
       qhi[0] = h.x; qhi[1] = h.y; qhi[2] = h.z;
       if (A.exclude_same_body) qbody = A.body[A.edges[2 * i]];
     } else {
       V3<float> p = ldx(A.x0, i), lo = p, hi = p;
-      if (A.mode == 3) {
+      if (MODE == 3) {
         V3<float> q = ldx(A.x1, i);
         lo = vmin(p, q);
         hi = vmax(p, q);
@@ -409,7 +421,7 @@ __global__ void __launch_bounds__(kQueryBlock) query_kernel(QueryArgs A) {
       if (child < A.n_leaves && overlaps(qlo, qhi, A.leaf_lo[child], A.leaf_hi[child]) &&
           !(qbody >= 0 && A.leaf_body[child] == qbody)) {
         j = A.leaf_id[child];
-        want = accept_pair(A, i, j, rec);
+        want = accept_pair<MODE>(A, i, j, rec);
       }
     }
     unsigned long long ball = __ballot(overlap_child);
@@ -488,7 +500,7 @@ void DevBuf::release() {
 }
 
 void Bvh::release() {
-  lo.release(); hi.release(); keys.release(); keys_sorted.release(); leaf_lo.release(); leaf_hi.release();
+  keys.release(); keys_sorted.release(); leaf_lo.release(); leaf_hi.release();
   leaf_id.release(); node_lo.release(); node_hi.release(); leaf_body.release(); node_body.release();
   temp.release(); bounds.release();
   n = 0;
@@ -518,8 +530,6 @@ hipError_t build_bvh(Bvh& B, int mode, int n, const int* prims, const int* body,
     total += I.count[L];
   } while (I.count[L] > 1);
   I.top = L;
-  IPC_TRY(B.lo.ensure(sizeof(float4) * n));
-  IPC_TRY(B.hi.ensure(sizeof(float4) * n));
   IPC_TRY(B.bounds.ensure(sizeof(int) * 8));
   IPC_TRY(B.keys.ensure(sizeof(unsigned long long) * n));
   IPC_TRY(B.keys_sorted.ensure(sizeof(unsigned long long) * n));
@@ -531,12 +541,11 @@ hipError_t build_bvh(Bvh& B, int mode, int n, const int* prims, const int* body,
   IPC_TRY(B.leaf_body.ensure(sizeof(int) * n));
   IPC_TRY(B.node_body.ensure(sizeof(int2) * total));
   bool resort = B.builds_since_sort < 0 || B.builds_since_sort >= kResortEvery || B.n_sorted != n;
-  hipLaunchKernelGGL(init_bounds_kernel, dim3(1), dim3(64), 0, s, B.bounds.as<int>());
-  hipLaunchKernelGGL(prim_boxes_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, mode, n, prims, x0, x1, pad,
-                     B.lo.as<float4>(), B.hi.as<float4>(), B.bounds.as<int>());
-  IPC_TRY(hipGetLastError());
   if (resort) {
-    hipLaunchKernelGGL(morton_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, n, B.lo.as<float4>(), B.hi.as<float4>(),
+    hipLaunchKernelGGL(init_bounds_kernel, dim3(1), dim3(64), 0, s, B.bounds.as<int>());
+    hipLaunchKernelGGL(centre_bounds_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, mode, n, prims, x0, x1, pad,
+                       B.bounds.as<int>());
+    hipLaunchKernelGGL(morton_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, mode, n, prims, x0, x1, pad,
                        B.bounds.as<int>(), B.keys.as<unsigned long long>());
     IPC_TRY(hipGetLastError());
     size_t temp_bytes = 0;
@@ -551,12 +560,11 @@ hipError_t build_bvh(Bvh& B, int mode, int n, const int* prims, const int* body,
   } else {
     ++B.builds_since_sort;
   }
-  int stride = (mode == BOX_EDGE || mode == BOX_SWEPT_EDGE) ? 2 : 3;
-  hipLaunchKernelGGL(reduce_levels_kernel, dim3(nblocks(n, kLeavesPerBlock)), dim3(kReduceBlock), 0, s, n, stride,
-                     prims, body, B.keys_sorted.as<unsigned long long>(), B.lo.as<float4>(), B.hi.as<float4>(),
-                     B.leaf_lo.as<float4>(), B.leaf_hi.as<float4>(), B.leaf_id.as<int>(), B.leaf_body.as<int>(),
-                     B.node_lo.as<float4>(), B.node_hi.as<float4>(), B.node_body.as<int2>(), I);
-  if (I.top >= 5)
+  hipLaunchKernelGGL(leaf_levels_kernel, dim3(nblocks(n, kLeavesPerBlock)), dim3(kReduceBlock), 0, s, mode, n, prims,
+                     body, x0, x1, pad, B.keys_sorted.as<unsigned long long>(), B.leaf_lo.as<float4>(),
+                     B.leaf_hi.as<float4>(), B.leaf_id.as<int>(), B.leaf_body.as<int>(), B.node_lo.as<float4>(),
+                     B.node_hi.as<float4>(), B.node_body.as<int2>(), I);
+  if (I.top > kInBlockLevels)
     hipLaunchKernelGGL(reduce_top_kernel, dim3(1), dim3(1024), 0, s, B.node_lo.as<float4>(), B.node_hi.as<float4>(),
                        B.node_body.as<int2>(), I);
   return hipGetLastError();
@@ -565,7 +573,6 @@ hipError_t build_bvh(Bvh& B, int mode, int n, const int* prims, const int* body,
 hipError_t run_query(const QueryLaunch& q, hipStream_t s) {
   if (q.n_query <= 0 || q.bvh->n <= 0) return hipSuccess;
   QueryArgs A;
-  A.mode = q.mode;
   A.n_query = q.n_query;
   A.edges = q.edges;
   A.tris = q.tris;
@@ -591,7 +598,14 @@ hipError_t run_query(const QueryLaunch& q, hipStream_t s) {
   A.ids = q.ids;
   A.capacity = q.capacity;
   A.count = q.count;
-  hipLaunchKernelGGL(query_kernel, dim3(nblocks(q.n_query, kGroupsPerBlock)), dim3(kQueryBlock), 0, s, A);
+  dim3 grid(nblocks(q.n_query, kGroupsPerBlock)), block(kQueryBlock);
+  switch (q.mode) {
+    case Q_DCD_EE: hipLaunchKernelGGL(query_kernel<Q_DCD_EE>, grid, block, 0, s, A); break;
+    case Q_DCD_PT: hipLaunchKernelGGL(query_kernel<Q_DCD_PT>, grid, block, 0, s, A); break;
+    case Q_CCD_EE: hipLaunchKernelGGL(query_kernel<Q_CCD_EE>, grid, block, 0, s, A); break;
+    case Q_CCD_PT: hipLaunchKernelGGL(query_kernel<Q_CCD_PT>, grid, block, 0, s, A); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -31,7 +31,6 @@ struct LevelInfo {
 struct Bvh {
   int n = 0;
   LevelInfo info{};
-  DevBuf lo, hi;              // float4 per primitive, original order
   DevBuf keys, keys_sorted;   // uint64 Morton | id
   DevBuf leaf_lo, leaf_hi;    // float4 per leaf, Morton order
   DevBuf leaf_id;             // primitive id per leaf
@@ -63,7 +62,7 @@ struct QueryLaunch {
   const int* body;
   const float* x0;
   const float* x1;
-  const float4* qlo;
+  const float4* qlo;  // modes 0, 2: query box per slot (leaf boxes of the same tree)
   const float4* qhi;
   const int* order;   // optional slot -> query primitive (Morton order of the queries)
   float pad;

@@ -17,6 +17,15 @@ hipError_t launch_collision_time(bool ee, const float* x0, const float* x1, int 
 hipError_t launch_collision_time_counted(bool ee, const float* x0, const float* x1, int nv, const int* quads,
                                          const int* n_dev, int capacity, unsigned long long* toi_min, hipStream_t s);
 
+hipError_t mlp_forward(const float* prm, int n, const float* x, float* y, float* work, float* scratch, hipStream_t s);
+hipError_t mlp_backward(const float* prm, int n, const float* x, const float* work, const float* dy, float* grad,
+                        float* scratch, hipStream_t s);
+hipError_t mlp_mse(const float* y, const float* t, const float* w, int n, float wsum, float* dy, double* loss,
+                   hipStream_t s);
+hipError_t mlp_adam(float* p, const float* g, float* m, float* v, long long n, float lr, float b1, float b2, float eps,
+                    int step, hipStream_t s);
+hipError_t zero_f32(float* p, long long n, hipStream_t s);
+
 // Records the calling thread's last error message (read by ipc_last_error).
 void set_error(const char* msg);
 

@@ -119,3 +119,20 @@ def test_degenerate_soups(dev, oracle):
         assert S.n_contacts_pt == 0
         assert S.n_contacts == (1 if len(edges) == 2 else 0)
         assert S.collision_free_step(torch.zeros_like(S.x)) == 1.0
+
+
+def test_kernel_timing_hooks(dev, oracle):
+    soup, dx = _ccd_setup(16)
+    S = _solver(dev, soup)
+    S.set_kernel_timing(True)
+    S.kernel_times()
+    S.detect_collision()
+    S.collision_free_step(torch.from_numpy(dx).to(dev))
+    t = S.kernel_times()
+    S.set_kernel_timing(False)
+    for k in ("build_edge", "build_tri", "query_dcd_ee", "query_dcd_pt", "query_ccd_ee", "query_ccd_pt",
+              "toi_ee", "toi_pt"):
+        ms, n = t[k]
+        assert n >= 1 and ms > 0.0, k
+    # reading resets the counters
+    assert all(n == 0 for _, n in S.kernel_times().values())
