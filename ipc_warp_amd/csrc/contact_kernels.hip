@@ -37,7 +37,8 @@ __device__ __forceinline__ bool pair_ee(const float* __restrict__ x, const float
   double d2 = want_hessian ? nsd_compressed(g, !near_parallel, C) : dot(g.r, g.r);
   Barrier B = barrier(d2, double(l0) * double(l0), kappa);
   double m = 1.0;
-  double gm[12];
+  // grad m over the slots is (-ga, ga, -gb, gb)
+  V3<double> ga = mk(0.0, 0.0, 0.0), gb = mk(0.0, 0.0, 0.0);
   if (near_parallel) {
     // IPC edge-edge mollifier, m(c) = (c/eps_x)(2 - c/eps_x), c = |ea x eb|^2.
     V3<double> ea = xr[1], eb = xr[3] - xr[2];
@@ -46,11 +47,8 @@ __device__ __forceinline__ bool pair_ee(const float* __restrict__ x, const float
     double q = dot(n, n) / ex;
     m = q * (2.0 - q);
     double dm = 2.0 / ex * (1.0 - q);
-    V3<double> ga = cross(eb, n) * (2.0 * dm), gb = cross(n, ea) * (2.0 * dm);
-    gm[0] = -ga.x; gm[1] = -ga.y; gm[2] = -ga.z;
-    gm[3] = ga.x; gm[4] = ga.y; gm[5] = ga.z;
-    gm[6] = -gb.x; gm[7] = -gb.y; gm[8] = -gb.z;
-    gm[9] = gb.x; gm[10] = gb.y; gm[11] = gb.z;
+    ga = cross(eb, n) * (2.0 * dm);
+    gb = cross(n, ea) * (2.0 * dm);
   }
   if (energy) *energy = m * B.b;
   if (!want_hessian) return true;
@@ -64,12 +62,15 @@ __device__ __forceinline__ bool pair_ee(const float* __restrict__ x, const float
     C.cn = float(B.db);
     C.cg = float(4.0 * B.d2b * d2);
   }
+  // force = -(m b' grad d^2 + b grad m); grad m is zero outside the band
+  double bm = near_parallel ? B.b : 0.0;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     double s = -2.0 * m * B.db * g.w[i];
-    force[3 * i + 0] = float(s * g.r.x - (near_parallel ? B.b * gm[3 * i + 0] : 0.0));
-    force[3 * i + 1] = float(s * g.r.y - (near_parallel ? B.b * gm[3 * i + 1] : 0.0));
-    force[3 * i + 2] = float(s * g.r.z - (near_parallel ? B.b * gm[3 * i + 2] : 0.0));
+    V3<double> gmi = i == 0 ? mk(-ga.x, -ga.y, -ga.z) : (i == 1 ? ga : (i == 2 ? mk(-gb.x, -gb.y, -gb.z) : gb));
+    force[3 * i + 0] = float(s * g.r.x - bm * gmi.x);
+    force[3 * i + 1] = float(s * g.r.y - bm * gmi.y);
+    force[3 * i + 2] = float(s * g.r.z - bm * gmi.z);
   }
   return true;
 }

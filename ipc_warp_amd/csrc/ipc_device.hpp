@@ -187,22 +187,40 @@ struct FeatureGeom {
   int k;
 };
 
+// Slot selection without a runtime-indexed local array (the compiler would
+// place it in scratch memory): a 0/1-weighted blend of the four slots, exact
+// because every product is either the value or zero.
+IPCD_FN V3<double> pick(const V3<double> x[4], int s) {
+  double c0 = s == 0 ? 1.0 : 0.0, c1 = s == 1 ? 1.0 : 0.0, c2 = s == 2 ? 1.0 : 0.0, c3 = s == 3 ? 1.0 : 0.0;
+  return mk(x[0].x * c0 + x[1].x * c1 + x[2].x * c2 + x[3].x * c3, x[0].y * c0 + x[1].y * c1 + x[2].y * c2 + x[3].y * c3,
+            x[0].z * c0 + x[1].z * c1 + x[2].z * c2 + x[3].z * c3);
+}
+
 // Positions are given relative to slot 0 (x[0] = 0) in double.
 IPCD_FN FeatureGeom feature_geom(const Feature& f, const V3<double> x[4]) {
   FeatureGeom g;
-  for (int i = 0; i < 4; ++i) { g.w[i] = 0.0; g.D[0][i] = 0.0; g.D[1][i] = 0.0; }
   g.e[0] = g.e[1] = mk(0.0, 0.0, 0.0);
   if (f.kind == F_PP) {
     g.k = 0;
-    g.w[f.s0] = 1.0; g.w[f.s1] = -1.0;
-    g.r = x[f.s0] - x[f.s1];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      g.w[i] = i == f.s0 ? 1.0 : (i == f.s1 ? -1.0 : 0.0);
+      g.D[0][i] = 0.0;
+      g.D[1][i] = 0.0;
+    }
+    g.r = pick(x, f.s0) - pick(x, f.s1);
   } else if (f.kind == F_PE) {
     g.k = 1;
-    V3<double> e = x[f.s2] - x[f.s1];
-    V3<double> q = x[f.s0] - x[f.s1];
+    V3<double> xp = pick(x, f.s0), x0 = pick(x, f.s1), x1 = pick(x, f.s2);
+    V3<double> e = x1 - x0;
+    V3<double> q = xp - x0;
     double lam = dot(q, e) / dot(e, e);
-    g.w[f.s0] = 1.0; g.w[f.s1] = -(1.0 - lam); g.w[f.s2] = -lam;
-    g.D[0][f.s1] = 1.0; g.D[0][f.s2] = -1.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      g.w[i] = i == f.s0 ? 1.0 : (i == f.s1 ? -(1.0 - lam) : (i == f.s2 ? -lam : 0.0));
+      g.D[0][i] = i == f.s1 ? 1.0 : (i == f.s2 ? -1.0 : 0.0);
+      g.D[1][i] = 0.0;
+    }
     g.e[0] = mk(-e.x, -e.y, -e.z);
     g.r = q - e * lam;
   } else if (f.kind == F_PT) {
@@ -214,8 +232,8 @@ IPCD_FN FeatureGeom feature_geom(const Feature& f, const V3<double> x[4]) {
     double be = (b1 * a22 - b2 * a12) / det;
     double ga = (a11 * b2 - a12 * b1) / det;
     g.w[0] = 1.0; g.w[1] = -(1.0 - be - ga); g.w[2] = -be; g.w[3] = -ga;
-    g.D[0][1] = 1.0; g.D[0][2] = -1.0;
-    g.D[1][1] = 1.0; g.D[1][3] = -1.0;
+    g.D[0][0] = 0.0; g.D[0][1] = 1.0; g.D[0][2] = -1.0; g.D[0][3] = 0.0;
+    g.D[1][0] = 0.0; g.D[1][1] = 1.0; g.D[1][2] = 0.0; g.D[1][3] = -1.0;
     g.e[0] = mk(-e1.x, -e1.y, -e1.z);
     g.e[1] = mk(-e2.x, -e2.y, -e2.z);
     g.r = q - e1 * be - e2 * ga;
@@ -228,8 +246,8 @@ IPCD_FN FeatureGeom feature_geom(const Feature& f, const V3<double> x[4]) {
     double s = (b1 * a22 - b2 * a12) / det;
     double t = (a11 * b2 - a12 * b1) / det;
     g.w[0] = 1.0 - s; g.w[1] = s; g.w[2] = -(1.0 - t); g.w[3] = -t;
-    g.D[0][0] = -1.0; g.D[0][1] = 1.0;
-    g.D[1][2] = 1.0; g.D[1][3] = -1.0;
+    g.D[0][0] = -1.0; g.D[0][1] = 1.0; g.D[0][2] = 0.0; g.D[0][3] = 0.0;
+    g.D[1][0] = 0.0; g.D[1][1] = 0.0; g.D[1][2] = 1.0; g.D[1][3] = -1.0;
     g.e[0] = ea;
     g.e[1] = mk(-eb.x, -eb.y, -eb.z);
     g.r = q + ea * s - eb * t;
