@@ -226,6 +226,21 @@ def main() -> int:
     big_gbs = 816 * n_big / (big_ms * 1e-3) / 1e9
     del rows, cols, vals
 
+    # STREAM-style copy of 256 MiB: the measured HBM ceiling, and the
+    # calibration kernel for the FETCH_SIZE / WRITE_SIZE counters.
+    copy_src = torch.empty(64 << 20, dtype=torch.float32, device=dev)
+    copy_dst = torch.empty_like(copy_src)
+    for _ in range(3):
+        copy_dst.copy_(copy_src)
+    e0.record()
+    for _ in range(10):
+        copy_dst.copy_(copy_src)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_ms = e0.elapsed_time(e1) / 10
+    copy_gbs = 2 * copy_src.numel() * 4 / (copy_ms * 1e-3) / 1e9
+    del copy_src, copy_dst
+
     result = {
         "metric": METRIC,
         "value": pairs / elapsed,
@@ -272,6 +287,7 @@ def main() -> int:
             "unit": "GB/s",
             "frac": big_gbs / HBM_PEAK_GBS,
         },
+        "hbm_copy_gbs": copy_gbs,
     }
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)

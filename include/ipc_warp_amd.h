@@ -199,6 +199,23 @@ int ipc_mlp_adam(float* params, const float* grad, float* m, float* v, long long
 /* p[0..n) = 0 (a fill kernel, safe inside captured graphs). */
 int ipc_zero_f32(float* p, long long n, void* stream);
 
+/* Per-vertex input features of the learned warm start (no counterpart in the
+ * reference, whose Newton loop starts from the previous state,
+ * time_integrator_base.py:375-401). features (float [n_vertices][48]) =
+ * base (float [n_vertices][12], NULL = zeros) followed by the vertex's
+ * IPC_WS_SLOTS nearest records of the contact lists ee / pt (int32 [n][4],
+ * the layouts ipc_detect_collision_* writes), nearest first by fp32 distance
+ * (ties by record: EE records first, then by index and vertex position), each
+ * as ((partner - self) / dhat [3], dist / dhat, w, is_pt): self is the
+ * closest point on the vertex's own primitive, partner the one on the other
+ * primitive, w the vertex's weight in self. Unused slots are zero. slots is
+ * workspace of 6 * n_vertices unsigned 64-bit words. */
+#define IPC_WS_BASE 12
+#define IPC_WS_SLOTS 6
+#define IPC_WS_SLOT_DIM 6
+int ipc_warmstart_features(const float* x, int n_vertices, const float* base, const int* ee, int n_ee, const int* pt,
+                           int n_pt, float dhat, unsigned long long* slots, float* features, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

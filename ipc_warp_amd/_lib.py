@@ -41,6 +41,7 @@ _SIGNATURES = {
     "ipc_mlp_adam": ([_vp, _vp, _vp, _vp, ctypes.c_longlong, _c_float, _c_float, _c_float, _c_float, _c_int, _vp],
                      _c_int),
     "ipc_zero_f32": ([_vp, ctypes.c_longlong, _vp], _c_int),
+    "ipc_warmstart_features": ([_vp, _c_int, _vp, _vp, _c_int, _vp, _c_int, _c_float, _vp, _vp, _vp], _c_int),
 }
 
 TIMER_KINDS = ("build_edge", "build_tri", "query_dcd_ee", "query_dcd_pt", "query_ccd_ee", "query_ccd_pt",

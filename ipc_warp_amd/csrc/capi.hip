@@ -185,4 +185,19 @@ int ipc_zero_f32(float* p, long long n, void* stream) {
   return hip_status(ipcd::zero_f32(p, n, static_cast<hipStream_t>(stream)), "zero_f32");
 }
 
+int ipc_warmstart_features(const float* x, int n_vertices, const float* base, const int* ee, int n_ee, const int* pt,
+                           int n_pt, float dhat, unsigned long long* slots, float* features, void* stream) {
+  if (n_vertices < 0 || n_ee < 0 || n_pt < 0) return fail(IPC_EINVAL, "warmstart_features: negative size");
+  if ((long long)n_ee + n_pt > (1ll << 27)) return fail(IPC_EINVAL, "warmstart_features: more than 2^27 records");
+  if (n_vertices == 0) return IPC_OK;
+  if (!x || !slots || !features) return fail(IPC_EINVAL, "warmstart_features: null pointer");
+  if ((n_ee > 0 && !ee) || (n_pt > 0 && !pt)) return fail(IPC_EINVAL, "warmstart_features: null contact list");
+  if (!(dhat > 0.0f)) return fail(IPC_EINVAL, "warmstart_features: dhat must be > 0");
+  if ((n_ee > 0 && !aligned16(ee)) || (n_pt > 0 && !aligned16(pt)))
+    return fail(IPC_EINVAL, "warmstart_features: contact lists must be 16-byte aligned");
+  return hip_status(ipcd::warmstart_features(x, n_vertices, base, ee, n_ee, pt, n_pt, dhat, slots, features,
+                                             static_cast<hipStream_t>(stream)),
+                    "warmstart_features");
+}
+
 }  // extern "C"

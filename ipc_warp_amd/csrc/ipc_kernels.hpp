@@ -25,6 +25,8 @@ hipError_t mlp_mse(const float* y, const float* t, const float* w, int n, float 
 hipError_t mlp_adam(float* p, const float* g, float* m, float* v, long long n, float lr, float b1, float b2, float eps,
                     int step, hipStream_t s);
 hipError_t zero_f32(float* p, long long n, hipStream_t s);
+hipError_t warmstart_features(const float* x, int nv, const float* base, const int* ee, int n_ee, const int* pt,
+                              int n_pt, float dhat, unsigned long long* slots, float* out, hipStream_t s);
 
 // Records the calling thread's last error message (read by ipc_last_error).
 void set_error(const char* msg);

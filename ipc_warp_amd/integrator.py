@@ -40,6 +40,7 @@ class StepStats:
     converged: bool = False
     max_dx_over_h: float = 0.0
     warm_start_accepted: bool = False
+    warm_start_alpha: float = 0.0
 
 
 class NeoHookeanTets:
@@ -124,6 +125,7 @@ class ContactIntegrator:
         self.xd = self.x.double()                    # float64 master copy of the iterate
         self.x0 = self.xd.clone()
         self.v = torch.zeros_like(self.xd)
+        self.v_prev = torch.zeros_like(self.xd)
         rest = torch.from_numpy(soup.xcs).to(device)
         self.tets = torch.from_numpy(soup.tets).to(device)
         mu, lam = lame(E, nu)
@@ -225,12 +227,14 @@ class ContactIntegrator:
             rz = rz_new
         return x
 
-    def line_search(self, dx, xt, E0):
-        """time_integrator_base.py:514-546 with the CCD bound of contact.py:539-585."""
+    def line_search(self, dx, xt, E0, max_halvings=64):
+        """time_integrator_base.py:514-546 with the CCD bound of contact.py:539-585:
+        x - alpha dx, alpha from the CCD bound, halved until the energy decreases;
+        on failure the iterate is restored and alpha = 0."""
         x_start = self.xd
         upper = self.contact.collision_free_step(dx.float().contiguous())
         alpha = upper
-        for _ in range(64):
+        for _ in range(max_halvings):
             E1 = self.energy(x_start - alpha * dx, xt)
             if math.isfinite(E1) and E1 < E0:
                 return alpha, E1
@@ -260,9 +264,12 @@ class ContactIntegrator:
                 break
         return stats
 
-    def step(self, initial=None) -> StepStats:
-        """One time step. `initial` (float64 [V,3]) replaces the previous state as
-        the first Newton iterate when it lowers the incremental potential."""
+    def step(self, initial=None, warm_start=None) -> StepStats:
+        """One time step. The first Newton iterate is the previous state, unless
+        `initial` (float64 [V,3]) lowers the incremental potential, or
+        `warm_start(self)` returns a displacement (float64 [V,3]) along which a
+        CCD-bounded backtracking search from the previous state finds a lower
+        potential. Neither counts as a Newton iteration (no linear solve)."""
         xt = self.x_tilde()
         stats = StepStats()
         self._set(self.x0.clone())
@@ -274,7 +281,14 @@ class ContactIntegrator:
                 stats.warm_start_accepted = True
             else:
                 self._set(self.x0.clone())
+        elif warm_start is not None:
+            delta = warm_start(self)
+            E_plain = self.energy(self.x0, xt)
+            alpha, _ = self.line_search(-delta, xt, E_plain, max_halvings=8)
+            stats.warm_start_accepted = alpha > 0.0
+            stats.warm_start_alpha = alpha
         self.newton(xt, stats)
+        self.v_prev = self.v
         self.v = (self.xd - self.x0) / self.h
         self.v[self.fixed] = 0.0
         self.x0 = self.xd.clone()

@@ -161,6 +161,32 @@ def two_cubes_on_ground(seed: int = 0, n: int = 8, gap: float = 2.0e-3, jitter: 
     return assemble(bodies)
 
 
+def cube_drop(seed: int = 0, n: int = 8):
+    """Seeded variants of the two-cube stack for the warm start's trajectories.
+
+    The bottom cube starts 2e-3 above the ground at rest; the top cube is
+    shifted U(-0.25, 0.25) in x and z, starts a gap U(2e-3, 3e-2) above the
+    bottom cube and moves with (N(0, 0.2), -U(0, 1.5), N(0, 0.2)) m/s.
+    Returns (soup, per-vertex velocity float64 [V, 3], fixed mask [V]).
+    """
+    rng = np.random.default_rng(seed)
+    gap = rng.uniform(2.0e-3, 3.0e-2)
+    shift = rng.uniform(-0.25, 0.25, 2)
+    vel = np.array([rng.normal(0, 0.2), -rng.uniform(0.0, 1.5), rng.normal(0, 0.2)])
+    V1, T1 = tet_grid_cube(n, 1.0, (0.0, 2.0e-3, 0.0))
+    V2, T2 = tet_grid_cube(n, 1.0, (shift[0], 1.0 + 2.0e-3 + gap, shift[1]))
+    bodies = []
+    for V, T in ((V1, T1), (V2, T2)):
+        V = V + rng.uniform(-1.0e-4, 1.0e-4, V.shape)
+        bodies.append((V, boundary_faces(V, T), T))
+    Vg, Fg = ground_quad(0.0)
+    bodies.append((Vg, Fg, None))
+    soup = assemble(bodies)
+    v = np.zeros((soup.x.shape[0], 3))
+    v[soup.body == 1] = vel
+    return soup, v, soup.body == 2
+
+
 def icosphere_grid(seed: int = 0, per_axis: int = 4, spacing: float = 1.001, subdivisions: int = 3,
                    radius: float = 0.5):
     """per_axis^3 icospheres on a grid `spacing` apart and seeded N(0,1) body velocities.

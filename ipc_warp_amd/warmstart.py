@@ -41,6 +41,34 @@ def init_params(seed: int = 0) -> np.ndarray:
     return np.concatenate(parts).astype(np.float32)
 
 
+BASE, SLOTS, SLOT_DIM = 12, 6, 6
+
+
+def vertex_features(x: torch.Tensor, base: torch.Tensor | None, ee: torch.Tensor, n_ee: int, pt: torch.Tensor,
+                    n_pt: int, dhat: float, out: torch.Tensor | None = None,
+                    slots: torch.Tensor | None = None) -> torch.Tensor:
+    """float32 [V, 48] network inputs: `base` [V, 12] followed by the vertex's 6 nearest
+    contact records of the lists ee / pt (ipc_warmstart_features)."""
+    V = x.shape[0]
+    if x.dtype != torch.float32 or x.dim() != 2 or x.shape[1] != 3:
+        raise ValueError("x must be float32 [V, 3]")
+    if base is not None and (base.shape != (V, BASE) or base.dtype != torch.float32):
+        raise ValueError(f"base must be float32 [V, {BASE}]")
+    for name, c, n in (("ee", ee, n_ee), ("pt", pt, n_pt)):
+        if n > 0 and (c.dtype != torch.int32 or c.dim() != 2 or c.shape[1] != 4 or c.shape[0] < n):
+            raise ValueError(f"{name} must be int32 [>= {n}, 4]")
+    if out is None:
+        out = torch.empty((V, IN), dtype=torch.float32, device=x.device)
+    elif out.shape != (V, IN) or out.dtype != torch.float32:
+        raise ValueError(f"out must be float32 [V, {IN}]")
+    if slots is None or slots.numel() < SLOTS * V:
+        slots = torch.empty(SLOTS * max(V, 1), dtype=torch.int64, device=x.device)
+    check(lib().ipc_warmstart_features(_p(x), V, None if base is None else _p(base), _p(ee) if n_ee else None, n_ee,
+                                       _p(pt) if n_pt else None, n_pt, float(dhat), _p(slots), _p(out),
+                                       stream_handle(x.device)), "warmstart_features")
+    return out
+
+
 def adam_step(p, g, m, v, lr=1e-3, step=1, beta1=0.9, beta2=0.999, eps=1e-8):
     check(lib().ipc_mlp_adam(_p(p), _p(g), _p(m), _p(v), p.numel(), float(lr), float(beta1), float(beta2), float(eps),
                              int(step), stream_handle(p.device)), "mlp_adam")
@@ -100,11 +128,12 @@ class MLP:
     def fit_normalisation(self, x: torch.Tensor, y: torch.Tensor):
         """Per-feature standardisation of inputs and per-axis scale of targets."""
         self.x_mean = x.mean(0)
-        self.x_scale = x.std(0).clamp_min(1e-12)
+        self.x_scale = x.std(0).clamp_min(1e-8)
         self.y_scale = y.abs().mean(0).clamp_min(1e-12)
 
     def normalise_x(self, x):
-        return ((x - self.x_mean) / self.x_scale).contiguous()
+        # clipped so that a feature that was (nearly) constant in training cannot blow up
+        return ((x - self.x_mean) / self.x_scale).clamp(-20.0, 20.0).contiguous()
 
     def train_epochs(self, x, t, w=None, epochs=1, batch=4096, lr=1e-3, seed=0) -> float:
         """Adam over shuffled mini-batches of already-normalised (x, t); returns the last epoch's mean loss."""
@@ -144,3 +173,136 @@ class MLP:
         self.x_scale = d["x_scale"].to(self.device)
         self.y_scale = d["y_scale"].to(self.device)
         self.step = int(d["step"])
+
+
+# ------------------------------------------------------------ the warm start ----
+FORCE_SCALE = 1.0e-4  # length scale of the asinh compression of the force features
+
+
+def base_features(sim) -> torch.Tensor:
+    """float32 [V, 12] at the start of a step of a ContactIntegrator `sim`:
+    h v, c(h^2/m f_elastic), c(h^2/m f_contact), h (v - v_prev), with
+    c(u) = s asinh(u / s), s = FORCE_SCALE; zero on fixed vertices. Runs
+    the DCD at the previous state (the lists then feed the contact slots)."""
+    h = sim.h
+    sim._set(sim.x0.clone())
+    grad_el, _ = sim.elastic.gradient_hessian(sim.x0)
+    sim.contact.detect_collision()
+    force = torch.zeros_like(sim.x)
+    sim.contact.compute_collision_hessian(force)
+    minv = (h * h / sim.m)[:, None]
+
+    def c(u):
+        return FORCE_SCALE * torch.asinh(u / FORCE_SCALE)
+
+    base = torch.cat([h * sim.v, c(-minv * grad_el), c(minv * force.double()), h * (sim.v - sim.v_prev)], dim=1)
+    base[sim.fixed] = 0.0
+    return base.float().contiguous()
+
+
+def step_features(sim) -> torch.Tensor:
+    """float32 [V, 48]: base_features and the 6 nearest contact records."""
+    base = base_features(sim)
+    c = sim.contact
+    return vertex_features(sim.x, base, c.contacts_new.list, c.n_contacts, c.contacts_pt.list, c.n_contacts_pt,
+                           float(c.l0))
+
+
+class WarmStart:
+    """Callable for ContactIntegrator.step(warm_start=...): the displacement
+    h v + MLP(features) (float64 [V, 3], zero on fixed vertices)."""
+
+    def __init__(self, mlp: MLP):
+        self.mlp = mlp
+
+    def __call__(self, sim) -> torch.Tensor:
+        y = self.mlp.predict(step_features(sim))
+        delta = sim.h * sim.v + y.double()
+        delta[sim.fixed] = 0.0
+        return delta
+
+    def save(self, path: str):
+        d = self.mlp.state_dict()
+        np.savez(path, params=d["params"].numpy(), x_mean=d["x_mean"].numpy(), x_scale=d["x_scale"].numpy(),
+                 y_scale=d["y_scale"].numpy(), step=np.int64(d["step"]), force_scale=np.float64(FORCE_SCALE))
+
+    @classmethod
+    def load(cls, path: str, device) -> "WarmStart":
+        z = np.load(path)  # plain arrays, no pickled objects
+        if float(z["force_scale"]) != FORCE_SCALE:
+            raise ValueError("model was trained with another feature scaling")
+        mlp = MLP(device, params=torch.from_numpy(z["params"].astype(np.float32)).to(device))
+        mlp.x_mean = torch.from_numpy(z["x_mean"]).float().to(device)
+        mlp.x_scale = torch.from_numpy(z["x_scale"]).float().to(device)
+        mlp.y_scale = torch.from_numpy(z["y_scale"]).float().to(device)
+        mlp.step = int(z["step"])
+        return cls(mlp)
+
+
+def record_trajectory(sim, steps: int, log=None):
+    """Plain-Newton trajectory: per step, the features at the previous state and
+    the target (converged displacement - h v) of every free vertex."""
+    X, T = [], []
+    free = ~sim.fixed
+    for s in range(steps):
+        f = step_features(sim)
+        x0 = sim.x0.clone()
+        hv = sim.h * sim.v
+        st = sim.step()
+        X.append(f[free])
+        T.append((sim.x0 - x0 - hv)[free].float())
+        if log:
+            log(s, st)
+    return torch.cat(X), torch.cat(T)
+
+
+def train(X: torch.Tensor, T: torch.Tensor, epochs: int = 60, batch: int = 4096, lr: float = 1e-3,
+          seed: int = 0, log=None) -> MLP:
+    """Adam with a cosine learning-rate decay over standardised features and
+    per-axis scaled targets."""
+    mlp = MLP(X.device, seed=seed)
+    mlp.fit_normalisation(X, T)
+    Xn = mlp.normalise_x(X)
+    Tn = (T / mlp.y_scale).contiguous()
+    for e in range(epochs):
+        lr_e = 1e-5 + 0.5 * (lr - 1e-5) * (1.0 + math.cos(math.pi * e / max(epochs, 1)))
+        loss = mlp.train_epochs(Xn, Tn, epochs=1, batch=batch, lr=lr_e, seed=seed + 7919 * e)
+        if log:
+            log(e, loss)
+    return mlp
+
+
+def drop_simulation(seed: int, device, n: int = 8):
+    """ContactIntegrator over scenes.cube_drop(seed) with its initial velocities."""
+    from . import scenes
+    from .integrator import ContactIntegrator
+
+    soup, v, fixed = scenes.cube_drop(seed, n=n)
+    sim = ContactIntegrator(soup, device, fixed=fixed)
+    sim.v = torch.from_numpy(v).to(device)
+    return sim
+
+
+def compare_newton(seed: int, steps: int, warm: WarmStart, device, n: int = 8, log=None) -> dict:
+    """Newton iterations per step from the previous state (plain) and from the
+    warm start, on the same seeded drop. Every step's motion is checked with
+    CCD: its time of impact must be 1 (no crossing, so no penetration, given
+    the intersection-free start)."""
+    out = {}
+    for mode in ("plain", "warm_start"):
+        sim = drop_simulation(seed, device, n=n)
+        rec = {"iterations": [], "converged": [], "accepted": [], "motion_toi": [], "max_dx_over_h": []}
+        for s in range(steps):
+            x_prev = sim.x0.clone()
+            st = sim.step(warm_start=warm if mode == "warm_start" else None)
+            toi = sim.contact.raw_toi((x_prev - sim.x0).float().contiguous(), x=x_prev.float().contiguous())
+            rec["iterations"].append(st.iterations)
+            rec["converged"].append(bool(st.converged))
+            rec["accepted"].append(bool(st.warm_start_accepted))
+            rec["motion_toi"].append(toi)
+            rec["max_dx_over_h"].append(st.max_dx_over_h)
+            if log:
+                log(mode, s, st, toi)
+        rec["mean_iterations"] = float(np.mean(rec["iterations"]))
+        out[mode] = rec
+    return out

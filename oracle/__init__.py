@@ -74,6 +74,7 @@ def lib():
         L.oracle_ccd_candidates_pt.argtypes = [F32P, F32P, c_int, I32P, c_int, I32P, c_flt, c_int, I32P, I32P, c_i64]
         L.oracle_ccd_candidates_pt.restype = c_i64
         L.oracle_max_threads.restype = c_int
+        L.oracle_warmstart_features.argtypes = [F32P, c_int, F32P, I32P, c_int, I32P, c_int, c_flt, F32P]
         _lib = L
     return _lib
 
@@ -289,3 +290,16 @@ def ccd_toi(x0, x1, edges, triangles, body, padding=1e-7, exclude_same_body=Fals
         tp, _ = collision_time("pt", x0, x1, pt, relative=relative, threads=threads)
         t = min(t, float(tp.min()))
     return t
+
+
+def warmstart_features(x, base, ee, pt, dhat):
+    """float32 [V, 48] warm-start features (layout of ipc_warmstart_features)."""
+    x = _f32(x)
+    V = x.shape[0]
+    base = None if base is None else _f32(base).reshape(V, 12)
+    ee = _i32(ee).reshape(-1, 4)
+    pt = _i32(pt).reshape(-1, 4)
+    out = np.zeros((V, 48), np.float32)
+    lib().oracle_warmstart_features(_p(x, F32P), V, None if base is None else _p(base, F32P), _p(ee, I32P), len(ee),
+                                    _p(pt, I32P), len(pt), float(dhat), _p(out, F32P))
+    return out

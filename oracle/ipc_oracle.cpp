@@ -403,6 +403,91 @@ int64_t oracle_ccd_candidates_pt(const float* x0, const float* x1, int n_nodes, 
   return out.n;
 }
 
+// Warm-start features (layout of ipc_warmstart_features; no reference
+// counterpart). Written independently of the GPU's atomic insertion: every
+// (record, vertex) pair is listed, the list is sorted by (vertex, fp32
+// distance, record id) and the first 6 entries of each vertex are kept.
+void oracle_warmstart_features(const float* x, int nv, const float* base, const int* ee, int n_ee, const int* pt,
+                               int n_pt, float dhat, float* out) {
+  const float inv = 1.0f / dhat;
+  struct Entry { int v; float d; uint32_t rid; };
+  std::vector<Entry> L;
+  auto ok = [&](const int* c) {
+    for (int k = 0; k < 4; ++k)
+      if (c[k] < 0 || c[k] >= nv) return false;
+    return true;
+  };
+  auto dist_of = [&](bool is_ee, const int* c) {
+    V3<float> a = load3<float>(x, c[0]), b = load3<float>(x, c[1]), cc = load3<float>(x, c[2]), d = load3<float>(x, c[3]);
+    return is_ee ? closest_point_edge_edge<float>(a, b, cc, d, 1.0e-6f).dist : closest_point_triangle<float>(a, b, cc, d).dist;
+  };
+  for (int pass = 0; pass < 2; ++pass) {
+    bool is_ee = pass == 0;
+    const int* C = is_ee ? ee : pt;
+    int n = is_ee ? n_ee : n_pt;
+    uint32_t base_id = is_ee ? 0u : 4u * uint32_t(n_ee);
+    for (int i = 0; i < n; ++i) {
+      const int* c = C + 4 * i;
+      if (!ok(c)) continue;
+      float d = dist_of(is_ee, c);
+      if (!(d >= 0.0f)) continue;
+      for (int k = 0; k < 4; ++k) L.push_back({c[k], d, base_id + 4u * uint32_t(i) + uint32_t(k)});
+    }
+  }
+  std::sort(L.begin(), L.end(), [](const Entry& a, const Entry& b) {
+    if (a.v != b.v) return a.v < b.v;
+    if (a.d != b.d) return a.d < b.d;
+    return a.rid < b.rid;
+  });
+  for (int v = 0; v < nv; ++v)
+    for (int q = 0; q < 48; ++q) out[48ll * v + q] = q < 12 && base ? base[12ll * v + q] : 0.0f;
+  size_t i = 0;
+  while (i < L.size()) {
+    int v = L[i].v;
+    size_t j = i;
+    while (j < L.size() && L[j].v == v) ++j;
+    for (size_t e = i; e < j && e < i + 6; ++e) {
+      uint32_t r = L[e].rid;
+      bool is_ee = r < 4u * uint32_t(n_ee);
+      uint32_t rr = is_ee ? r : r - 4u * uint32_t(n_ee);
+      int k = int(rr & 3u);
+      const int* c = (is_ee ? ee : pt) + 4 * (rr >> 2);
+      V3<float> x0 = load3<float>(x, c[0]), x1 = load3<float>(x, c[1]), x2 = load3<float>(x, c[2]),
+                x3 = load3<float>(x, c[3]);
+      V3<float> self, partner;
+      float w, dist;
+      if (is_ee) {
+        EdgeClosest<float> ec = closest_point_edge_edge<float>(x0, x1, x2, x3, 1.0e-6f);
+        V3<float> pa = x0 * (1.0f - ec.s) + x1 * ec.s;
+        V3<float> pb = x2 * (1.0f - ec.t) + x3 * ec.t;
+        self = k < 2 ? pa : pb;
+        partner = k < 2 ? pb : pa;
+        const float ws[4] = {1.0f - ec.s, ec.s, 1.0f - ec.t, ec.t};
+        w = ws[k];
+        dist = ec.dist;
+      } else {
+        TriClosest<float> tc = closest_point_triangle<float>(x0, x1, x2, x3);
+        float wc = 1.0f - tc.wa - tc.wb;
+        V3<float> q = tc.wa * x1 + tc.wb * x2 + wc * x3;
+        self = k == 0 ? x0 : q;
+        partner = k == 0 ? q : x0;
+        const float ws[4] = {1.0f, tc.wa, tc.wb, wc};
+        w = ws[k];
+        dist = tc.dist;
+      }
+      float* o = out + 48ll * v + 12 + 6 * (e - i);
+      V3<float> off = partner - self;
+      o[0] = off.x * inv;
+      o[1] = off.y * inv;
+      o[2] = off.z * inv;
+      o[3] = dist * inv;
+      o[4] = w;
+      o[5] = is_ee ? 0.0f : 1.0f;
+    }
+    i = j;
+  }
+}
+
 int oracle_max_threads(void) { return omp_get_max_threads(); }
 
 }  // extern "C"
