@@ -181,8 +181,7 @@ def main() -> int:
         kernel_ms[k] = kacc[k] / n_prof
         per_step_ms[k] = kacc[k] / n_prof
 
-    # Algorithmic bytes per launch ("Kernels, their bound and
-    # algorithmic bytes"): barrier assembly 816 B (PT) / 864 B (EE) per
+    # Algorithmic bytes per launch: barrier assembly 816 B (PT) / 864 B (EE) per
     # contact; TOI 112 B per candidate; a query reads the tree once (leaf:
     # 32 B box + 4 B id + 4 B body; node: 32 B box + 8 B body range), one
     # 32 B (EE) or 12/24 B (PT DCD/CCD) query box per query and writes 16 B

@@ -42,7 +42,7 @@ int ipc_abi_version(void);
 /* Human-readable description of the last error on this thread. */
 const char* ipc_last_error(void);
 
-/* ------------------------------------------------- barrier Hessian (D8) ---
+/* ------------------------------------------------- barrier Hessian ---
  * Replaces the launch of contact_hessian_ee(states, soup, contacts, triplets,
  * b, kappa), dim = n_contacts (dynamic_contacts.py:151-207, launched at
  * dynamic_contacts.py:287 and affine_body_dynamics.py:470-476).
@@ -54,14 +54,14 @@ const char* ipc_last_error(void);
 int ipc_contact_hessian_ee(const float* x, const float* xcs, int n_vertices, const int* contacts, int n_contacts, float l0,
                            double kappa, int offset, int* rows, int* cols, float* vals, float* b, void* stream);
 
-/* ------------------------------------------------- barrier Hessian (D9) ---
+/* ------------------------------------------------- barrier Hessian ---
  * Replaces contact_hessian_pt(states, soup, contacts, triplets, b, offset,
  * kappa), dim = n_contacts (dynamic_contacts.py:209-246, launched at
  * dynamic_contacts.py:288 with offset = n_ee). */
 int ipc_contact_hessian_pt(const float* x, int n_vertices, const int* contacts, int n_contacts, float l0, double kappa, int offset,
                            int* rows, int* cols, float* vals, float* b, void* stream);
 
-/* ------------------------------------------------- barrier energy (D10) ---
+/* ------------------------------------------------- barrier energy ---
  * Replace contact_energy_ee / contact_energy_pt(states, soup, contacts, e,
  * kappa) (dynamic_contacts.py:374-398, launched at :304-305). The sum over
  * active contacts is accumulated in FP64 and ADDED to energy[0] (a device
@@ -71,7 +71,7 @@ int ipc_contact_energy_ee(const float* x, const float* xcs, int n_vertices, cons
 int ipc_contact_energy_pt(const float* x, int n_vertices, const int* contacts, int n_contacts, float l0, double kappa,
                           double* energy, void* stream);
 
-/* ----------------------------------------------- contact distance (C3d) ---
+/* ----------------------------------------------- contact distance ---
  * Replace get_contact_points / get_contact_points_pt (contact.py:148-161)
  * over fetch_dist_v0v1 / fetch_dist_v0v1_pt (contact.py:88-145): midpoint of
  * the two closest points (points, float [n][3]) and the fp32 distance
@@ -85,7 +85,7 @@ int ipc_contact_points_ee(const float* x, const float* xcs, int n_vertices, cons
 int ipc_contact_points_pt(const float* x, int n_vertices, const int* contacts, int n_contacts, float* points, float* dists,
                           int* features, void* stream);
 
-/* ------------------------------------------------ CCD time of impact (E4) ---
+/* ------------------------------------------------ CCD time of impact ---
  * pt_collision_time / ee_collision_time (ccd/ccd.py:304-357) over a list of
  * candidate vertex quadruples (int [n][4]: EE a0, a1, b0, b1; PT p, t0, t1,
  * t2, the argument order of edge_edge_toi / point_triangle_toi,
@@ -109,7 +109,7 @@ int ipc_ctx_create(ipc_ctx** ctx, int n_vertices, const int* edges, int n_edges,
                    int n_triangles, const int* body);
 int ipc_ctx_destroy(ipc_ctx* ctx);
 
-/* ------------------------------------------------------ DCD (B1-B5) ---
+/* ------------------------------------------------------ DCD ---
  * Replace edge_edge_collision / point_triangle_collision and the BVH
  * updates of detect_collision (contact.py:199-336,672-725). A BVH over the
  * current positions x is built, every edge (EE) or every vertex (PT) queries
@@ -128,7 +128,7 @@ int ipc_detect_collision_ee(ipc_ctx* ctx, const float* x, float thickness, float
 int ipc_detect_collision_pt(ipc_ctx* ctx, const float* x, float thickness, float buffer, int exclude_same_body,
                             int* contacts, int* e0e1, int capacity, int* count, void* stream);
 
-/* --------------------------------------------- CCD candidates (E1-E3) ---
+/* --------------------------------------------- CCD candidates ---
  * The candidate sets of edge_edge_toi / point_triangle_toi
  * (ccd/ccd.py:166-210,360-427): swept boxes over x0 and x1 padded by
  * `padding` (1e-7 in the reference), filters as above without a distance
@@ -139,7 +139,7 @@ int ipc_ccd_candidates_ee(ipc_ctx* ctx, const float* x0, const float* x1, float 
 int ipc_ccd_candidates_pt(ipc_ctx* ctx, const float* x0, const float* x1, float padding, int exclude_same_body,
                           int* quads, int* ids, int capacity, int* count, void* stream);
 
-/* ------------------------------------------- CCD step bound (E1-E8) ---
+/* ------------------------------------------- CCD step bound ---
  * Replaces collision_free_step(dx) (contact.py:539-585) up to its final
  * `0.9 * toi if toi < 1 else 1`: x1 = x - dx, EE and PT candidates, FP64
  * time of impact per candidate and their minimum. Synchronises `stream` once
@@ -149,6 +149,62 @@ int ipc_collision_free_step(ipc_ctx* ctx, const float* x, const float* dx, float
                             double* toi, void* stream);
 /* Candidate counts (EE, PT) of the context's last collision_free_step. */
 int ipc_ctx_last_ccd_counts(ipc_ctx* ctx, int* n_ee, int* n_pt);
+
+/* Replaces new_intersections (contact.py:587-670), one primitive kind per
+ * call: the candidates of edge_edge_intersections / point_triangle_
+ * intersections (contact.py:370-442) between x0 and x1, FP64 TOI per
+ * candidate, and every candidate with TOI < 1 appended (append_ccd,
+ * contact.py:185-197) as records int32 [capacity][4] (EE (a0, a1, b0, b1);
+ * PT (p, t0, t1, t2)), ids int32 [capacity][2] (e0e1: edge i < j; point,
+ * triangle) and alpha double [capacity] (= TOI). *count (device int) is the
+ * number of such pairs and may exceed capacity: grow the lists and call
+ * again. Synchronises `stream` once and writes this kind's raw minimum TOI
+ * (1.0 = none) to *toi (host double); new_intersections returns
+ * 0.9 * min(toi_ee, toi_pt) if it is < 1, else 1. */
+int ipc_ccd_intersections_ee(ipc_ctx* ctx, const float* x0, const float* x1, float padding, int exclude_same_body,
+                             int* records, int* ids, double* alpha, int capacity, int* count, double* toi,
+                             void* stream);
+int ipc_ccd_intersections_pt(ipc_ctx* ctx, const float* x0, const float* x1, float padding, int exclude_same_body,
+                             int* records, int* ids, double* alpha, int capacity, int* count, double* toi,
+                             void* stream);
+
+/* ============================================== fixed per-body BVHs ===
+ * Affine bodies (ABD): vertex v sits at A_b xcs[v] + p_b for its body b, with
+ * q float [n_bodies][4][3] = (p, A[:,0], A[:,1], A[:,2]) per body (the
+ * reference's affine_q layout, fixed_bvh_contacts.py:32-36).
+ *
+ * Replaces FixedBodyContactSolver.initialize_fixed_body_bvhs
+ * (fixed_bvh_contacts.py:511-597): one tree per body over its edges and one
+ * over its triangles, built once from the body-local rest positions xcs
+ * (float [V][3], device or host), plus per-body vertex lists and rest boxes.
+ * Every vertex's body id must lie in [0, n_bodies). */
+int ipc_ctx_init_fixed_bodies(ipc_ctx* ctx, int n_bodies, const float* xcs);
+/* Replaces FixedBodyContactSolver.detect_collision (fixed_bvh_contacts.py:
+ * 758-827): body pairs whose world boxes overlap within 2 thickness, then
+ * the edges of each pair's first body against the second body's edge tree
+ * and the vertices of each body against the other's triangle tree, in the
+ * target body's rest frame with padding 2 thickness |A^-1|_F; accepted by
+ * the world-space fp32 distance < 2 thickness. Records as in
+ * ipc_detect_collision_ee / _pt: EE (source edge, target edge) vertices with
+ * e0e1 = (source, target), PT (p, t[1], t[0], t[2]) with (p, triangle).
+ * *ee_count / *pt_count (device) may exceed the capacities: grow and call
+ * again. Synchronises `stream` once. A body with |det A| < 1e-8 makes the
+ * call use the world-space DCD (same-body pairs excluded) instead. */
+int ipc_detect_collision_fixed(ipc_ctx* ctx, const float* x, const float* q, float thickness, int* ee, int* ee_ids,
+                               int ee_capacity, int* ee_count, int* pt, int* pt_ids, int pt_capacity, int* pt_count,
+                               void* stream);
+/* Replaces FixedBodyContactSolver.collision_free_step (fixed_bvh_contacts.py:
+ * 710-756) up to its 0.9 factor: x1 = x - dx with dq the matching per-body
+ * affine step (q1 = q - dq); swept body boxes and pairs (padding),
+ * candidates from the fixed trees with query boxes widened by the target's
+ * motion extent (a fixed source body of an EE pair is swapped to be the
+ * target when fixed_bodies, int [n_bodies] or NULL, marks it), FP64 TOI per
+ * candidate. Writes the raw minimum TOI (1.0 = none) to *toi (host). When a
+ * body's |det A| < 1e-8 at either end it runs ipc_collision_free_step with
+ * same-body pairs excluded instead and sets *used_fallback (may be NULL). */
+int ipc_collision_free_step_fixed(ipc_ctx* ctx, const float* x, const float* dx, const float* q, const float* dq,
+                                  const int* fixed_bodies, float padding, double* toi, int* used_fallback,
+                                  void* stream);
 
 /* ------------------------------------------------------ stage timing ---
  * The role of RodBCBase.profile_timer (time_integrator_base.py:245-252):
@@ -164,7 +220,10 @@ int ipc_ctx_last_ccd_counts(ipc_ctx* ctx, int* n_ee, int* n_pt);
 #define IPC_TIMER_QUERY_CCD_PT 5
 #define IPC_TIMER_TOI_EE 6
 #define IPC_TIMER_TOI_PT 7
-#define IPC_TIMER_KINDS 8
+#define IPC_TIMER_FIXED_CULL 8
+#define IPC_TIMER_FIXED_QUERY_DCD 9
+#define IPC_TIMER_FIXED_QUERY_CCD 10
+#define IPC_TIMER_KINDS 11
 int ipc_ctx_set_timing(ipc_ctx* ctx, int enable);
 int ipc_ctx_kernel_times(ipc_ctx* ctx, double* total_ms, long long* calls);
 

@@ -41,11 +41,17 @@ _SIGNATURES = {
     "ipc_mlp_adam": ([_vp, _vp, _vp, _vp, ctypes.c_longlong, _c_float, _c_float, _c_float, _c_float, _c_int, _vp],
                      _c_int),
     "ipc_zero_f32": ([_vp, ctypes.c_longlong, _vp], _c_int),
+    "ipc_ccd_intersections_ee": ([_vp, _vp, _vp, _c_float, _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp], _c_int),
+    "ipc_ccd_intersections_pt": ([_vp, _vp, _vp, _c_float, _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp], _c_int),
+    "ipc_ctx_init_fixed_bodies": ([_vp, _c_int, _vp], _c_int),
+    "ipc_detect_collision_fixed": ([_vp, _vp, _vp, _c_float, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _vp, _vp],
+                                   _c_int),
+    "ipc_collision_free_step_fixed": ([_vp, _vp, _vp, _vp, _vp, _vp, _c_float, _vp, _vp, _vp], _c_int),
     "ipc_warmstart_features": ([_vp, _c_int, _vp, _vp, _c_int, _vp, _c_int, _c_float, _vp, _vp, _vp], _c_int),
 }
 
 TIMER_KINDS = ("build_edge", "build_tri", "query_dcd_ee", "query_dcd_pt", "query_ccd_ee", "query_ccd_pt",
-               "toi_ee", "toi_pt")
+               "toi_ee", "toi_pt", "fixed_cull", "fixed_query_dcd", "fixed_query_ccd")
 
 EXPORTED = tuple(_SIGNATURES)
 

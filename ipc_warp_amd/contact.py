@@ -49,6 +49,7 @@ class Contacts:
     e0e1: torch.Tensor
     cnt: torch.Tensor
     capacity: int
+    alpha: torch.Tensor | None = None  # XConstraint.alpha (float64 [capacity]), set by new_intersections
 
 
 @dataclass
@@ -66,6 +67,7 @@ def _new_contacts(capacity: int, device) -> Contacts:
         e0e1=torch.zeros((capacity, 2), dtype=torch.int32, device=device),
         cnt=torch.zeros(1, dtype=torch.int32, device=device),
         capacity=capacity,
+        alpha=torch.zeros(capacity, dtype=torch.float64, device=device),
     )
 
 
@@ -177,6 +179,41 @@ class ContactSolverBase:
               "collision_free_step")
         return float(out.value)
 
+    def new_intersections(self, x0, x1) -> float:
+        """contact.py:587-670 (the augmented-Lagrangian active set): every EE pair
+        (edge_edge_intersections) and PT pair (point_triangle_intersections,
+        contact.py:370-442) whose time of impact between x0 and x1 is < 1
+        replaces contacts_new / contacts_pt, with alpha = that time
+        (append_ccd, contact.py:185-197); the lists grow to the next power of
+        two on overflow. Returns 0.9 * toi if toi < 1 else 1."""
+        for t in (x0, x1):
+            if t.shape != self.x.shape or t.dtype != torch.float32 or not t.is_contiguous():
+                raise ValueError("x0 and x1 must be contiguous float32 tensors shaped like x")
+        toi = 1.0
+        kinds = [("ee", lib().ipc_ccd_intersections_ee)]
+        if self.has_triangles:
+            kinds.append(("pt", lib().ipc_ccd_intersections_pt))
+        else:
+            self.n_contacts_pt = 0
+        for kind, fn in kinds:
+            contacts = self.contacts_new if kind == "ee" else self.contacts_pt
+            while True:
+                t = ctypes.c_double(1.0)
+                check(fn(self._ctx, x0.data_ptr(), x1.data_ptr(), float(CCD_PADDING), int(self.disable_self_collision),
+                         contacts.list.data_ptr(), contacts.e0e1.data_ptr(), contacts.alpha.data_ptr(),
+                         int(contacts.capacity), contacts.cnt.data_ptr(), ctypes.byref(t), self._stream()),
+                      f"ccd_intersections_{kind}")
+                n = int(contacts.cnt.item())
+                if n <= contacts.capacity:
+                    break
+                contacts = self._grow_contact_list(contacts, n)
+            toi = min(toi, t.value)
+            if kind == "ee":
+                self.contacts_new, self.n_contacts = contacts, n
+            else:
+                self.contacts_pt, self.n_contacts_pt = contacts, n
+        return 0.9 * toi if toi < 1.0 else 1.0
+
     def ccd_candidates(self, kind: str, x0, x1, padding=CCD_PADDING, capacity=1 << 12):
         """Candidate quadruples and primitive ids of edge_edge_toi (kind 'ee') or
         point_triangle_toi ('pt') between x0 and x1 (ccd/ccd.py:360-427)."""
@@ -265,3 +302,74 @@ class ContactSolverBase:
 
 
 __all__ = ["ContactSolverBase", "Contacts", "Triplets", "IPCError"]
+
+
+class FixedBodyContactSolver(ContactSolverBase):
+    """Contact solver whose broad phase uses per-body BVHs built once in the
+    bodies' rest frames (FixedBodyContactSolver, fixed_bvh_contacts.py:503-827).
+
+    ``xcs`` holds body-local rest coordinates and ``affine_q`` is a float32
+    device tensor of four vec3 blocks per body, ``[p, A[:,0], A[:,1], A[:,2]]``
+    ([n_bodies * 4, 3]); ``x`` must equal A xcs + p per body.
+    """
+
+    def initialize_fixed_body_bvhs(self, affine_q, n_bodies: int | None = None, fixed_bodies=None):
+        """fixed_bvh_contacts.py:511-597."""
+        n_bodies = int(self.body.max().item()) + 1 if n_bodies is None else int(n_bodies)
+        self._check_q(affine_q, n_bodies)
+        check(lib().ipc_ctx_init_fixed_bodies(self._ctx, n_bodies, self.xcs.data_ptr()), "ctx_init_fixed_bodies")
+        self.affine_q = affine_q
+        self.n_bodies = n_bodies
+        if fixed_bodies is None:
+            self.abd_fixed = None
+        else:
+            self.abd_fixed = torch.as_tensor(fixed_bodies, dtype=torch.int32).to(self.device).contiguous()
+            if self.abd_fixed.shape != (n_bodies,):
+                raise ValueError("fixed_bodies must hold one flag per body")
+
+    def _check_q(self, q, n_bodies):
+        if q.dtype != torch.float32 or not q.is_contiguous() or q.numel() != 12 * n_bodies or q.device != self.device:
+            raise ValueError(f"affine q must be a contiguous float32 device tensor of {n_bodies} x 4 vec3")
+
+    def detect_collision(self):
+        """fixed_bvh_contacts.py:758-827: both lists at once, regrown and rerun on overflow."""
+        while True:
+            c_ee, c_pt = self.contacts_new, self.contacts_pt
+            check(lib().ipc_detect_collision_fixed(self._ctx, self.x.data_ptr(), self.affine_q.data_ptr(),
+                                                   float(self.thickness), c_ee.list.data_ptr(), c_ee.e0e1.data_ptr(),
+                                                   int(c_ee.capacity), c_ee.cnt.data_ptr(), c_pt.list.data_ptr(),
+                                                   c_pt.e0e1.data_ptr(), int(c_pt.capacity), c_pt.cnt.data_ptr(),
+                                                   self._stream()), "detect_collision_fixed")
+            n_ee, n_pt = int(c_ee.cnt.item()), int(c_pt.cnt.item())
+            grown = False
+            if n_ee > c_ee.capacity:
+                self.contacts_new = self._grow_contact_list(c_ee, n_ee)
+                grown = True
+            if n_pt > c_pt.capacity:
+                self.contacts_pt = self._grow_contact_list(c_pt, n_pt)
+                grown = True
+            if not grown:
+                self.n_contacts, self.n_contacts_pt = n_ee, n_pt
+                return
+
+    def raw_toi_fixed(self, dx, affine_dx):
+        """Minimum TOI of the fixed-BVH CCD and whether it fell back to the world path."""
+        if dx.shape != self.x.shape or dx.dtype != torch.float32 or not dx.is_contiguous():
+            raise ValueError("dx must be a contiguous float32 tensor shaped like x")
+        self._check_q(affine_dx, self.n_bodies)
+        out = ctypes.c_double(1.0)
+        fb = ctypes.c_int(0)
+        check(lib().ipc_collision_free_step_fixed(self._ctx, self.x.data_ptr(), dx.data_ptr(), self.affine_q.data_ptr(),
+                                                  affine_dx.data_ptr(),
+                                                  None if self.abd_fixed is None else self.abd_fixed.data_ptr(),
+                                                  float(CCD_PADDING), ctypes.byref(out), ctypes.byref(fb),
+                                                  self._stream()), "collision_free_step_fixed")
+        return float(out.value), bool(fb.value)
+
+    def collision_free_step(self, dx, affine_dx=None) -> float:
+        """fixed_bvh_contacts.py:710-756; without the affine step (no ABD state)
+        it is the inherited world-space CCD."""
+        if affine_dx is None:
+            return ContactSolverBase.collision_free_step(self, dx)
+        toi, _ = self.raw_toi_fixed(dx, affine_dx)
+        return 0.9 * toi if toi < 1.0 else 1.0

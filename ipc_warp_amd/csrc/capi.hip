@@ -30,7 +30,7 @@ void ipcd::set_error(const char* msg) { std::snprintf(g_err, sizeof(g_err), "%s"
 
 extern "C" {
 
-int ipc_abi_version(void) { return 1; }
+int ipc_abi_version(void) { return 2; }
 
 const char* ipc_last_error(void) { return g_err; }
 

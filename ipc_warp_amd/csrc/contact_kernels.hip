@@ -291,6 +291,63 @@ __global__ void __launch_bounds__(256) collision_time_kernel(const float* __rest
   }
 }
 
+// edge_edge_intersections / point_triangle_intersections (contact.py:370-442)
+// after the candidate query: FP64 TOI of the first min(*n_dev, cap_in)
+// candidates; each candidate with t < 1 is appended (append_ccd,
+// contact.py:185-197) as its record, its primitive ids and alpha = t, and t
+// is folded into *toi_min. The loop runs per wave (every lane of a wave takes
+// the same trip count), so the append is one atomic per wave: ballot,
+// leader's atomicAdd, prefix popcount. Slots at or beyond cap_out are counted
+// but not written, so the caller can grow the list and run again.
+template <bool EE>
+__global__ void __launch_bounds__(256) record_intersections_kernel(
+    const float* __restrict__ x0, const float* __restrict__ x1, int nv, const int4* __restrict__ quads,
+    const int2* __restrict__ ids, const int* __restrict__ n_dev, int cap_in, int4* __restrict__ rec_out,
+    int2* __restrict__ ids_out, double* __restrict__ alpha_out, int cap_out, int* __restrict__ count_out,
+    unsigned long long* __restrict__ toi_min) {
+  int count = *n_dev;
+  count = count < cap_in ? count : cap_in;
+  const int lane = threadIdx.x & 63;
+  const int stride = gridDim.x * blockDim.x;
+  double tmin = 1.0;
+  for (int i0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63); i0 < count; i0 += stride) {
+    int i = i0 + lane;
+    double t = 1.0;
+    int4 q = make_int4(0, 0, 0, 0);
+    if (i < count) {
+      q = quads[i];
+      if (valid4(q, nv)) {
+        const int id[4] = {q.x, q.y, q.z, q.w};
+        V3<double> p0[4], p1[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) { p0[k] = to_d(ldx(x0, id[k])); p1[k] = to_d(ldx(x1, id[k])); }
+        t = collision_time<EE, double>(p0, p1);
+      }
+    }
+    bool hit = t < 1.0;
+    tmin = fmin(tmin, t);
+    unsigned long long m = __ballot(hit);
+    if (m) {
+      int leader = __builtin_ctzll(m);
+      int base = 0;
+      if (lane == leader) base = atomicAdd(count_out, __popcll(m));
+      base = __shfl(base, leader, 64);
+      if (hit) {
+        int slot = base + __popcll(m & ((1ull << lane) - 1ull));
+        if (slot < cap_out) {
+          rec_out[slot] = q;
+          ids_out[slot] = ids[i];
+          alpha_out[slot] = t;
+        }
+      }
+    }
+  }
+  if (toi_min) {
+    double mn = wave_min(tmin);
+    if (lane == 0 && mn < 1.0) atomicMin(toi_min, (unsigned long long)__double_as_longlong(mn));
+  }
+}
+
 // --------------------------------------------------------------- launchers ----
 static inline int blocks_for(long long n, int bs) { return int((n + bs - 1) / bs); }
 
@@ -357,6 +414,26 @@ hipError_t launch_collision_time(bool ee, const float* x0, const float* x1, int 
 hipError_t launch_collision_time_counted(bool ee, const float* x0, const float* x1, int nv, const int* quads,
                                          const int* n_dev, int capacity, unsigned long long* toi_min, hipStream_t s) {
   return collision_time_grid(ee, x0, x1, nv, quads, capacity, n_dev, nullptr, toi_min, s);
+}
+
+hipError_t launch_record_intersections(bool ee, const float* x0, const float* x1, int nv, const int* quads,
+                                      const int* ids, const int* n_dev, int cap_in, int* rec_out, int* ids_out,
+                                      double* alpha_out, int cap_out, int* count_out, unsigned long long* toi_min,
+                                      hipStream_t s) {
+  if (cap_in <= 0) return hipSuccess;
+  int grid = blocks_for(cap_in, 256);
+  if (grid > 8192) grid = 8192;
+  const int4* q4 = reinterpret_cast<const int4*>(quads);
+  const int2* i2 = reinterpret_cast<const int2*>(ids);
+  int4* r4 = reinterpret_cast<int4*>(rec_out);
+  int2* o2 = reinterpret_cast<int2*>(ids_out);
+  if (ee)
+    hipLaunchKernelGGL(record_intersections_kernel<true>, dim3(grid), dim3(256), 0, s, x0, x1, nv, q4, i2, n_dev,
+                       cap_in, r4, o2, alpha_out, cap_out, count_out, toi_min);
+  else
+    hipLaunchKernelGGL(record_intersections_kernel<false>, dim3(grid), dim3(256), 0, s, x0, x1, nv, q4, i2, n_dev,
+                       cap_in, r4, o2, alpha_out, cap_out, count_out, toi_min);
+  return hipGetLastError();
 }
 
 }  // namespace ipcd

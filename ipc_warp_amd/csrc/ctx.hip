@@ -8,6 +8,7 @@
 
 #include "../../include/ipc_warp_amd.h"
 #include "cull.hpp"
+#include "forest.hpp"
 #include "ipc_kernels.hpp"
 
 using namespace ipcd;
@@ -18,8 +19,14 @@ struct ipc_ctx {
   Bvh edge_bvh, tri_bvh;
   DevBuf x1;               // CCD end positions x - dx
   DevBuf cand_ee, cand_pt; // int4 candidate quadruples
+  DevBuf cand_ids_ee, cand_ids_pt;  // int2 candidate primitive ids (new_intersections)
   int cap_ee = 0, cap_pt = 0;
-  DevBuf small;            // [0] uint64 toi bits, [8..] int counters
+  // fixed per-body BVHs (ipc_ctx_init_fixed_bodies)
+  int nb = 0, max_verts = 0, cap_pairs = 0, cap_tasks_ee = 0, cap_tasks_pt = 0;
+  Forest fe, ft;
+  DevBuf vert_base, vert_ids, body_lo, body_hi;  // per-body vertex lists and rest boxes
+  DevBuf frames, world_lo, world_hi, pairs, tasks_ee, tasks_pt;
+  DevBuf small;            // [0] uint64 toi bits, [64..] int counters
   void* host = nullptr;    // pinned readback area
   int last_counts[2] = {0, 0};
   // optional per-kernel timing (HIP events recorded on the launch stream)
@@ -120,7 +127,12 @@ int ipc_ctx_destroy(ipc_ctx* c) {
   for (auto& p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
   c->edges.release(); c->tris.release(); c->body.release();
   c->edge_bvh.release(); c->tri_bvh.release();
-  c->x1.release(); c->cand_ee.release(); c->cand_pt.release(); c->small.release();
+  c->x1.release(); c->cand_ee.release(); c->cand_pt.release();
+  c->cand_ids_ee.release(); c->cand_ids_pt.release(); c->small.release();
+  c->fe.release(); c->ft.release();
+  c->vert_base.release(); c->vert_ids.release(); c->body_lo.release(); c->body_hi.release();
+  c->frames.release(); c->world_lo.release(); c->world_hi.release(); c->pairs.release();
+  c->tasks_ee.release(); c->tasks_pt.release();
   if (c->host) (void)hipHostFree(c->host);
   delete c;
   return IPC_OK;
@@ -319,6 +331,343 @@ int ipc_collision_free_step(ipc_ctx* c, const float* x, const float* dx, float p
     }
   }
   return ctx_fail(IPC_EINVAL, "collision_free_step: candidate list kept overflowing");
+}
+
+// new_intersections (contact.py:587-670) for one primitive kind: the swept
+// candidate query of edge_edge_toi / point_triangle_toi between x0 and x1,
+// FP64 TOI per candidate, and every candidate with TOI < 1 appended as
+// (record, ids, alpha = TOI) (edge_edge_intersections /
+// point_triangle_intersections, contact.py:370-442). The candidate list lives
+// in the context and grows on overflow; *count (device) may exceed capacity,
+// and then the caller grows its lists and calls again. Synchronises `stream`
+// once; writes the raw minimum TOI of this kind (1.0 = none) to *toi.
+static int ccd_intersections(ipc_ctx* c, bool ee, const float* x0, const float* x1, float padding,
+                             int exclude_same_body, int* records, int* ids, double* alpha, int capacity, int* count,
+                             double* toi, void* stream) {
+  const char* what = ee ? "ccd_intersections_ee" : "ccd_intersections_pt";
+  if (!c || !x0 || !x1 || !count || !toi) return ctx_fail(IPC_EINVAL, what);
+  if (capacity < 0 || (capacity > 0 && (!records || !ids || !alpha)))
+    return ctx_fail(IPC_EINVAL, ee ? "ccd_intersections_ee: bad capacity or null list"
+                                   : "ccd_intersections_pt: bad capacity or null list");
+  if ((reinterpret_cast<unsigned long long>(records) & 15ull) || (reinterpret_cast<unsigned long long>(ids) & 7ull) ||
+      (reinterpret_cast<unsigned long long>(alpha) & 7ull))
+    return ctx_fail(IPC_EINVAL, ee ? "ccd_intersections_ee: records must be 16-byte and ids/alpha 8-byte aligned"
+                                   : "ccd_intersections_pt: records must be 16-byte and ids/alpha 8-byte aligned");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int& cap = ee ? c->cap_ee : c->cap_pt;
+  if (cap == 0) cap = 1 << 16;
+  DevBuf& cq = ee ? c->cand_ee : c->cand_pt;
+  DevBuf& ci = ee ? c->cand_ids_ee : c->cand_ids_pt;
+  const int k = ee ? 0 : 1;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    int st = ctx_hip(cq.ensure(sizeof(int) * 4 * (size_t)cap), what);
+    if (!st) st = ctx_hip(ci.ensure(sizeof(int) * 2 * (size_t)cap), what);
+    if (!st) st = ctx_hip(launch_fill_u64(toi_bits(c), 1, kOneBits, s), what);
+    if (!st) st = ctx_hip(launch_zero_i32(count, 1, s), what);
+    if (st) return st;
+    int* cnt = counters(c) + k;
+    st = ee ? ipc_ccd_candidates_ee(c, x0, x1, padding, exclude_same_body, cq.as<int>(), ci.as<int>(), cap, cnt, stream)
+            : ipc_ccd_candidates_pt(c, x0, x1, padding, exclude_same_body, cq.as<int>(), ci.as<int>(), cap, cnt,
+                                    stream);
+    if (st) return st;
+    st = ctx_hip(timed(c, ee ? IPC_TIMER_TOI_EE : IPC_TIMER_TOI_PT, s, [&] {
+                   return launch_record_intersections(ee, x0, x1, c->nv, cq.as<int>(), ci.as<int>(), cnt, cap, records,
+                                                      ids, alpha, capacity, count, toi_bits(c), s);
+                 }),
+                 what);
+    if (!st) st = ctx_hip(hipMemcpyAsync(c->host, c->small.p, 128, hipMemcpyDeviceToHost, s), what);
+    if (!st) st = ctx_hip(hipStreamSynchronize(s), what);
+    if (st) return st;
+    unsigned long long bits;
+    std::memcpy(&bits, c->host, 8);
+    int n;
+    std::memcpy(&n, static_cast<char*>(c->host) + 64 + 4 * k, 4);
+    c->last_counts[k] = n;
+    if (n > cap) {
+      while (cap < n) cap *= 2;
+      continue;
+    }
+    double t;
+    std::memcpy(&t, &bits, 8);
+    *toi = t;
+    return IPC_OK;
+  }
+  return ctx_fail(IPC_EINVAL, ee ? "ccd_intersections_ee: candidate list kept overflowing"
+                                 : "ccd_intersections_pt: candidate list kept overflowing");
+}
+
+int ipc_ccd_intersections_ee(ipc_ctx* c, const float* x0, const float* x1, float padding, int exclude_same_body,
+                             int* records, int* ids, double* alpha, int capacity, int* count, double* toi,
+                             void* stream) {
+  return ccd_intersections(c, true, x0, x1, padding, exclude_same_body, records, ids, alpha, capacity, count, toi,
+                           stream);
+}
+
+int ipc_ccd_intersections_pt(ipc_ctx* c, const float* x0, const float* x1, float padding, int exclude_same_body,
+                             int* records, int* ids, double* alpha, int capacity, int* count, double* toi,
+                             void* stream) {
+  return ccd_intersections(c, false, x0, x1, padding, exclude_same_body, records, ids, alpha, capacity, count, toi,
+                           stream);
+}
+
+// ------------------------------------------------------ fixed body BVHs ----
+// Counters in the small buffer: [0], [1] CCD candidates (EE, PT), [2] body
+// pairs, [3], [4] tasks (EE, PT), [5] singular-frame flag.
+namespace {
+
+int grow_pow2(int cap, int need) {
+  long long c = cap > 0 ? cap : 1;
+  while (c < need) c *= 2;
+  return c > (1ll << 30) ? (1 << 30) : int(c);
+}
+
+int ensure_fixed_buffers(ipc_ctx* c) {
+  int st = ctx_hip(c->pairs.ensure(sizeof(int2) * (size_t)c->cap_pairs), "fixed buffers");
+  if (!st) st = ctx_hip(c->tasks_ee.ensure(sizeof(int2) * (size_t)c->cap_tasks_ee), "fixed buffers");
+  if (!st) st = ctx_hip(c->tasks_pt.ensure(sizeof(int2) * (size_t)c->cap_tasks_pt), "fixed buffers");
+  return st;
+}
+
+// Body frames, body pairs and tasks of one fixed-body step (DCD: dq = NULL).
+int fixed_cull(ipc_ctx* c, const float* x0, const float* x1, const float* q, const float* dq, const int* fixed,
+               float pair_pad, float task_pad, hipStream_t s) {
+  int* cnt = counters(c);
+  int st = ctx_hip(launch_zero_i32(cnt + 2, 4, s), "fixed cull");
+  if (st) return st;
+  return ctx_hip(timed(c, IPC_TIMER_FIXED_CULL, s, [&] {
+                   FixedStep f{c->nb, q, dq, c->body_lo.as<float4>(), c->body_hi.as<float4>(), c->world_lo.as<float4>(),
+                               c->world_hi.as<float4>(), c->frames.as<BodyFrame>(), cnt + 5};
+                   hipError_t e = launch_body_frames(f, s);
+                   if (e == hipSuccess)
+                     e = launch_body_pairs(c->nb, c->world_lo.as<float4>(), c->world_hi.as<float4>(), pair_pad,
+                                           c->pairs.as<int2>(), cnt + 2, c->cap_pairs, s);
+                   if (e != hipSuccess) return e;
+                   TaskLaunch t{};
+                   t.ccd = dq != nullptr;
+                   t.pairs = c->pairs.as<int2>();
+                   t.n_pairs = cnt + 2;
+                   t.pair_cap = c->cap_pairs;
+                   t.singular = cnt + 5;
+                   t.x0 = x0;
+                   t.x1 = x1;
+                   t.world_lo = c->world_lo.as<float4>();
+                   t.world_hi = c->world_hi.as<float4>();
+                   t.pad = task_pad;
+                   t.edges = c->edges.as<int>();
+                   t.edesc = c->fe.desc.as<TreeDesc>();
+                   t.eleaf_id = c->fe.leaf_id.as<int>();
+                   t.max_edges = c->fe.max_leaves;
+                   t.fixed = fixed;
+                   t.vert_base = c->vert_base.as<int>();
+                   t.vert_ids = c->vert_ids.as<int>();
+                   t.max_verts = c->ft.n_leaves > 0 ? c->max_verts : 0;
+                   t.tasks_ee = c->tasks_ee.as<int2>();
+                   t.count_ee = cnt + 3;
+                   t.cap_ee = c->cap_tasks_ee;
+                   t.tasks_pt = c->tasks_pt.as<int2>();
+                   t.count_pt = cnt + 4;
+                   t.cap_pt = c->cap_tasks_pt;
+                   return launch_fixed_tasks(t, s);
+                 }),
+                 "fixed cull");
+}
+
+FixedQuery fixed_query(ipc_ctx* c, int mode, const float* x0, const float* x1, float thickness, float pad, int4* rec,
+                       int2* ids, int capacity, int* count) {
+  bool ee = mode == Q_DCD_EE || mode == Q_CCD_EE;
+  FixedQuery fq{};
+  fq.mode = mode;
+  fq.tasks = ee ? c->tasks_ee.as<int2>() : c->tasks_pt.as<int2>();
+  fq.task_count = counters(c) + (ee ? 3 : 4);
+  fq.task_cap = ee ? c->cap_tasks_ee : c->cap_tasks_pt;
+  fq.edges = c->edges.as<int>();
+  fq.tris = c->tris.as<int>();
+  fq.x0 = x0;
+  fq.x1 = x1;
+  fq.frames = c->frames.as<BodyFrame>();
+  fq.forest = ee ? &c->fe : &c->ft;
+  fq.thickness = thickness;
+  fq.pad = pad;
+  fq.rec = rec;
+  fq.ids = ids;
+  fq.capacity = capacity;
+  fq.count = count;
+  return fq;
+}
+
+// Grows the pair and task lists after an overflow; true if anything grew.
+bool grow_fixed(ipc_ctx* c, const int* host_cnt) {
+  bool grow = false;
+  if (host_cnt[2] > c->cap_pairs) { c->cap_pairs = grow_pow2(c->cap_pairs, host_cnt[2]); grow = true; }
+  if (host_cnt[3] > c->cap_tasks_ee) { c->cap_tasks_ee = grow_pow2(c->cap_tasks_ee, host_cnt[3]); grow = true; }
+  if (host_cnt[4] > c->cap_tasks_pt) { c->cap_tasks_pt = grow_pow2(c->cap_tasks_pt, host_cnt[4]); grow = true; }
+  return grow;
+}
+
+}  // namespace
+
+int ipc_ctx_init_fixed_bodies(ipc_ctx* c, int n_bodies, const float* xcs) {
+  if (!c || !xcs) return ctx_fail(IPC_EINVAL, "ctx_init_fixed_bodies: null pointer");
+  if (n_bodies <= 0) return ctx_fail(IPC_EINVAL, "ctx_init_fixed_bodies: n_bodies must be > 0");
+  std::vector<int> body(c->nv), edges(2 * (size_t)c->ne), tris(3 * (size_t)c->nf);
+  std::vector<float> rest(3 * (size_t)c->nv);
+  hipError_t e = hipSuccess;
+  if (c->nv) e = hipMemcpy(body.data(), c->body.p, sizeof(int) * body.size(), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && c->ne) e = hipMemcpy(edges.data(), c->edges.p, sizeof(int) * edges.size(), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && c->nf) e = hipMemcpy(tris.data(), c->tris.p, sizeof(int) * tris.size(), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && c->nv) e = hipMemcpy(rest.data(), xcs, sizeof(float) * rest.size(), hipMemcpyDefault);
+  if (e != hipSuccess) return ctx_hip(e, "ctx_init_fixed_bodies");
+  for (int b : body)
+    if (b < 0 || b >= n_bodies) return ctx_fail(IPC_EINVAL, "ctx_init_fixed_bodies: a body id is outside [0, n_bodies)");
+  e = build_forest(c->fe, n_bodies, c->ne, 2, edges.data(), body.data(), rest.data());
+  if (e == hipSuccess) e = build_forest(c->ft, n_bodies, c->nf, 3, tris.data(), body.data(), rest.data());
+  if (e != hipSuccess) return ctx_hip(e, "ctx_init_fixed_bodies: forest");
+  std::vector<int> vbase(n_bodies + 1), vids(std::max(c->nv, 1));
+  std::vector<float> lo(4 * (size_t)n_bodies), hi(4 * (size_t)n_bodies);
+  body_vertex_lists(n_bodies, c->nv, body.data(), rest.data(), vbase.data(), vids.data(), lo.data(), hi.data());
+  int max_verts = 0;
+  for (int b = 0; b < n_bodies; ++b) max_verts = std::max(max_verts, vbase[b + 1] - vbase[b]);
+  e = c->vert_base.ensure(sizeof(int) * vbase.size());
+  if (e == hipSuccess) e = c->vert_ids.ensure(sizeof(int) * vids.size());
+  if (e == hipSuccess) e = c->body_lo.ensure(sizeof(float) * lo.size());
+  if (e == hipSuccess) e = c->body_hi.ensure(sizeof(float) * hi.size());
+  if (e == hipSuccess) e = c->frames.ensure(sizeof(BodyFrame) * (size_t)n_bodies);
+  if (e == hipSuccess) e = c->world_lo.ensure(sizeof(float4) * (size_t)n_bodies);
+  if (e == hipSuccess) e = c->world_hi.ensure(sizeof(float4) * (size_t)n_bodies);
+  if (e == hipSuccess) e = hipMemcpy(c->vert_base.p, vbase.data(), sizeof(int) * vbase.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->vert_ids.p, vids.data(), sizeof(int) * vids.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->body_lo.p, lo.data(), sizeof(float) * lo.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->body_hi.p, hi.data(), sizeof(float) * hi.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return ctx_hip(e, "ctx_init_fixed_bodies");
+  c->nb = n_bodies;
+  c->max_verts = max_verts;
+  long long all_pairs = (long long)n_bodies * (n_bodies - 1) / 2;
+  c->cap_pairs = int(std::max(1ll, std::min(all_pairs, 1ll << 20)));
+  c->cap_tasks_ee = c->cap_tasks_pt = 1 << 16;
+  return ensure_fixed_buffers(c);
+}
+
+// FixedBodyContactSolver.detect_collision (fixed_bvh_contacts.py:758-827).
+int ipc_detect_collision_fixed(ipc_ctx* c, const float* x, const float* q, float thickness, int* ee, int* ee_ids,
+                               int ee_capacity, int* ee_count, int* pt, int* pt_ids, int pt_capacity, int* pt_count,
+                               void* stream) {
+  if (!c || !x || !q || !ee_count || !pt_count) return ctx_fail(IPC_EINVAL, "detect_collision_fixed: null pointer");
+  if (c->nb == 0) return ctx_fail(IPC_EINVAL, "detect_collision_fixed: call ipc_ctx_init_fixed_bodies first");
+  if (ee_capacity < 0 || pt_capacity < 0 || (ee_capacity > 0 && !ee) || (pt_capacity > 0 && !pt))
+    return ctx_fail(IPC_EINVAL, "detect_collision_fixed: bad capacity or null list");
+  if ((reinterpret_cast<unsigned long long>(ee) & 15ull) || (reinterpret_cast<unsigned long long>(pt) & 15ull))
+    return ctx_fail(IPC_EINVAL, "detect_collision_fixed: contact lists must be 16-byte aligned");
+  if (!(thickness > 0.0f)) return ctx_fail(IPC_EINVAL, "detect_collision_fixed: thickness must be > 0");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const float pad = 2.0f * thickness;
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    int st = ensure_fixed_buffers(c);
+    if (!st) st = ctx_hip(launch_zero_i32(ee_count, 1, s), "detect_collision_fixed");
+    if (!st) st = ctx_hip(launch_zero_i32(pt_count, 1, s), "detect_collision_fixed");
+    if (!st) st = fixed_cull(c, x, nullptr, q, nullptr, nullptr, pad, pad, s);
+    if (st) return st;
+    st = ctx_hip(timed(c, IPC_TIMER_FIXED_QUERY_DCD, s, [&] {
+                   hipError_t e = launch_fixed_query(fixed_query(c, Q_DCD_EE, x, nullptr, thickness, 0.0f,
+                                                                 reinterpret_cast<int4*>(ee),
+                                                                 reinterpret_cast<int2*>(ee_ids), ee_capacity, ee_count),
+                                                     s);
+                   if (e == hipSuccess)
+                     e = launch_fixed_query(fixed_query(c, Q_DCD_PT, x, nullptr, thickness, 0.0f,
+                                                        reinterpret_cast<int4*>(pt), reinterpret_cast<int2*>(pt_ids),
+                                                        pt_capacity, pt_count),
+                                            s);
+                   return e;
+                 }),
+                 "detect_collision_fixed query");
+    if (!st) st = ctx_hip(hipMemcpyAsync(c->host, c->small.p, 128, hipMemcpyDeviceToHost, s), "detect_collision_fixed");
+    if (!st) st = ctx_hip(hipStreamSynchronize(s), "detect_collision_fixed sync");
+    if (st) return st;
+    int hc[6];
+    std::memcpy(hc, static_cast<char*>(c->host) + 64, sizeof(hc));
+    if (hc[5]) {
+      // A (near-)singular body frame: world-space DCD instead (same pair set).
+      st = ipc_detect_collision_ee(c, x, thickness, thickness, 1, ee, ee_ids, ee_capacity, ee_count, stream);
+      if (!st) st = ipc_detect_collision_pt(c, x, thickness, thickness, 1, pt, pt_ids, pt_capacity, pt_count, stream);
+      return st;
+    }
+    if (!grow_fixed(c, hc)) return IPC_OK;
+  }
+  return ctx_fail(IPC_EINVAL, "detect_collision_fixed: task lists kept overflowing");
+}
+
+// FixedBodyContactSolver.collision_free_step (fixed_bvh_contacts.py:710-756)
+// before its 0.9 factor.
+int ipc_collision_free_step_fixed(ipc_ctx* c, const float* x, const float* dx, const float* q, const float* dq,
+                                  const int* fixed_bodies, float padding, double* toi, int* used_fallback,
+                                  void* stream) {
+  if (!c || !x || !dx || !q || !dq || !toi) return ctx_fail(IPC_EINVAL, "collision_free_step_fixed: null pointer");
+  if (c->nb == 0) return ctx_fail(IPC_EINVAL, "collision_free_step_fixed: call ipc_ctx_init_fixed_bodies first");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (used_fallback) *used_fallback = 0;
+  int st = ctx_hip(c->x1.ensure(sizeof(float) * 3 * (size_t)c->nv + 16), "collision_free_step_fixed alloc");
+  if (st) return st;
+  if (c->cap_ee == 0) c->cap_ee = 1 << 16;
+  if (c->cap_pt == 0) c->cap_pt = 1 << 16;
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    st = ensure_fixed_buffers(c);
+    if (!st) st = ctx_hip(c->cand_ee.ensure(sizeof(int) * 4 * (size_t)c->cap_ee), "collision_free_step_fixed alloc");
+    if (!st) st = ctx_hip(c->cand_pt.ensure(sizeof(int) * 4 * (size_t)c->cap_pt), "collision_free_step_fixed alloc");
+    if (!st) st = ctx_hip(launch_sub(3 * c->nv, x, dx, c->x1.as<float>(), s), "collision_free_step_fixed");
+    if (!st) st = ctx_hip(launch_fill_u64(toi_bits(c), 1, kOneBits, s), "collision_free_step_fixed");
+    if (!st) st = ctx_hip(launch_zero_i32(counters(c), 2, s), "collision_free_step_fixed");
+    if (!st) st = fixed_cull(c, x, c->x1.as<float>(), q, dq, fixed_bodies, padding, padding, s);
+    if (st) return st;
+    int* cnt = counters(c);
+    const float* x1 = c->x1.as<float>();
+    st = ctx_hip(timed(c, IPC_TIMER_FIXED_QUERY_CCD, s, [&] {
+                   hipError_t e = launch_fixed_query(fixed_query(c, Q_CCD_EE, x, x1, 0.0f, padding,
+                                                                 c->cand_ee.as<int4>(), nullptr, c->cap_ee, cnt + 0),
+                                                     s);
+                   if (e == hipSuccess)
+                     e = launch_fixed_query(fixed_query(c, Q_CCD_PT, x, x1, 0.0f, padding, c->cand_pt.as<int4>(),
+                                                        nullptr, c->cap_pt, cnt + 1),
+                                            s);
+                   return e;
+                 }),
+                 "collision_free_step_fixed query");
+    if (!st)
+      st = ctx_hip(timed(c, IPC_TIMER_TOI_EE, s, [&] {
+                     return launch_collision_time_counted(true, x, x1, c->nv, c->cand_ee.as<int>(), cnt + 0, c->cap_ee,
+                                                          toi_bits(c), s);
+                   }),
+                   "collision_free_step_fixed EE TOI");
+    if (!st)
+      st = ctx_hip(timed(c, IPC_TIMER_TOI_PT, s, [&] {
+                     return launch_collision_time_counted(false, x, x1, c->nv, c->cand_pt.as<int>(), cnt + 1, c->cap_pt,
+                                                          toi_bits(c), s);
+                   }),
+                   "collision_free_step_fixed PT TOI");
+    if (!st) st = ctx_hip(hipMemcpyAsync(c->host, c->small.p, 128, hipMemcpyDeviceToHost, s), "collision_free_step_fixed");
+    if (!st) st = ctx_hip(hipStreamSynchronize(s), "collision_free_step_fixed sync");
+    if (st) return st;
+    int hc[6];
+    std::memcpy(hc, static_cast<char*>(c->host) + 64, sizeof(hc));
+    if (hc[5]) {
+      // min |det A| < 1e-8 at either end: the world-space swept CCD
+      // (fixed_bvh_contacts.py:733-735); the fixed path only pairs
+      // different bodies, so the fallback excludes same-body pairs too.
+      if (used_fallback) *used_fallback = 1;
+      return ipc_collision_free_step(c, x, dx, padding, 1, toi, stream);
+    }
+    bool grow = grow_fixed(c, hc);
+    if (hc[0] > c->cap_ee) { c->cap_ee = grow_pow2(c->cap_ee, hc[0]); grow = true; }
+    if (hc[1] > c->cap_pt) { c->cap_pt = grow_pow2(c->cap_pt, hc[1]); grow = true; }
+    if (grow) continue;
+    c->last_counts[0] = hc[0];
+    c->last_counts[1] = hc[1];
+    unsigned long long bits;
+    std::memcpy(&bits, c->host, 8);
+    double t;
+    std::memcpy(&t, &bits, 8);
+    *toi = t;
+    return IPC_OK;
+  }
+  return ctx_fail(IPC_EINVAL, "collision_free_step_fixed: candidate lists kept overflowing");
 }
 
 // Per-kernel timing: while enabled, builds, queries and TOI launches are

@@ -8,23 +8,26 @@
 // the mesh has moved):
 //   1. one AABB per primitive (fp32, the reference's inflation),
 //   2. 30-bit Morton codes of box centres in the scene bounds, made unique by
-//      the primitive id in the low 32 bits of a 64-bit key,
-//   3. hipCUB radix sort of the keys,
-//   4. an implicit 8-ary hierarchy over the sorted leaves: node k of level L
-//      bounds nodes 8k .. 8k+7 of level L-1 (level 0 = leaves). Levels 1-4
-//      are reduced by one kernel in LDS (a workgroup owns 4096 leaves), the
-//      levels above by a single workgroup. There is no bottom-up propagation
-//      across workgroups, hence no device-scope fence: on a multi-XCD part
-//      such a fence writes back and invalidates the L2.
-// Queries keep a 64-entry stack in LDS (7 pending siblings per level over at
-// most 9 levels), run a wave-uniform loop (one node per lane per iteration,
-// testing its 8 children from one contiguous 256 B run) and append accepted
-// pairs with one atomicAdd per wave per emission round (ballot + prefix
-// count).
+//      the primitive id in the low 32 bits of a 64-bit key, sorted with the
+//      hipCUB radix sort on the first build and every kResortEvery builds
+//      (in between, the leaf order is kept and only the boxes are refreshed),
+//   3. an implicit 8-ary hierarchy over the sorted leaves: node k of level L
+//      bounds nodes 8k .. 8k+7 of level L-1 (level 0 = leaves). Levels 1-3
+//      are reduced by one kernel in LDS (a 64-lane workgroup owns 512 leaves),
+//      the levels above by a single workgroup. There is no bottom-up
+//      propagation across workgroups, hence no device-scope fence: on a
+//      multi-XCD part such a fence writes back and invalidates the L2.
+// Nodes also carry the (min, max) body id below them, so a query that
+// excludes its own body skips single-body subtrees. Queries run eight lanes
+// per query: visiting a node, lane r tests the node's child r, and the
+// group's ballot of overlapping children becomes one stack entry (level,
+// first child, 8-bit mask) in LDS. Accepted pairs are appended with one
+// atomicAdd per wave per emission round (ballot + prefix count).
 #include <hipcub/hipcub.hpp>
 
 #include "cull.hpp"
 #include "ipc_device.hpp"
+#include "traverse.hpp"
 
 namespace ipcd {
 
@@ -51,10 +54,6 @@ __device__ __forceinline__ unsigned expand_bits10(unsigned v) {
 
 struct Box { float lo[3], hi[3]; };
 
-__device__ __forceinline__ bool overlaps(const float qlo[3], const float qhi[3], float4 lo, float4 hi) {
-  // Inclusive test (a lower bound above the other's upper bound separates).
-  return !(qlo[0] > hi.x || qlo[1] > hi.y || qlo[2] > hi.z || lo.x > qhi[0] || lo.y > qhi[1] || lo.z > qhi[2]);
-}
 
 // --------------------------------------------------------- primitive boxes ----
 // mode 0: edge box +- pad (edge_aabb, contact.py:30-37)
@@ -255,22 +254,7 @@ __global__ void __launch_bounds__(1024) reduce_top_kernel(float4* __restrict__ n
 }
 
 // ------------------------------------------------------------------ queries ----
-// Wave-aggregated append: every lane with `want` gets a distinct slot.
-__device__ __forceinline__ int wave_append(bool want, int* count) {
-  unsigned long long m = __ballot(want);
-  if (m == 0ull) return -1;
-  int lane = __lane_id();
-  int leader = __ffsll(static_cast<long long>(m)) - 1;
-  int base = 0;
-  if (lane == leader) base = atomicAdd(count, __popcll(m));
-  base = __shfl(base, leader, 64);
-  if (!want) return -1;
-  return base + __popcll(m & ((1ull << lane) - 1ull));
-}
 
-constexpr int kQueryBlock = 256;
-constexpr int kGroupsPerBlock = kQueryBlock / kFan;  // one query per 8 lanes
-constexpr int kMaskStack = 12;                       // > kMaxLevels - 1 entries per query
 
 // Query mode:
 //  0 DCD EE: query edge i's box against the edge BVH; accept i < j, not
@@ -337,12 +321,6 @@ __device__ __forceinline__ bool accept_pair(const QueryArgs& A, int i, int j, in
   return tc.dist < A.thickness * 2.0f;
 }
 
-// Stack entry: level (bits 40..43), child mask (bits 32..39), first child
-// index at that level (bits 0..31).
-__device__ __forceinline__ unsigned long long mask_entry(int level, unsigned mask, int first) {
-  return (static_cast<unsigned long long>(level) << 40) | (static_cast<unsigned long long>(mask & 0xffu) << 32) |
-         static_cast<unsigned>(first);
-}
 
 // Eight lanes share one query. Visiting a node, lane r tests the node's
 // child r; the group ballot of overlapping children becomes one stack entry

@@ -17,6 +17,11 @@ hipError_t launch_collision_time(bool ee, const float* x0, const float* x1, int 
 hipError_t launch_collision_time_counted(bool ee, const float* x0, const float* x1, int nv, const int* quads,
                                          const int* n_dev, int capacity, unsigned long long* toi_min, hipStream_t s);
 
+hipError_t launch_record_intersections(bool ee, const float* x0, const float* x1, int nv, const int* quads,
+                                      const int* ids, const int* n_dev, int cap_in, int* rec_out, int* ids_out,
+                                      double* alpha_out, int cap_out, int* count_out, unsigned long long* toi_min,
+                                      hipStream_t s);
+
 hipError_t mlp_forward(const float* prm, int n, const float* x, float* y, float* work, float* scratch, hipStream_t s);
 hipError_t mlp_backward(const float* prm, int n, const float* x, const float* work, const float* dy, float* grad,
                         float* scratch, hipStream_t s);

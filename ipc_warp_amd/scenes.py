@@ -205,6 +205,70 @@ def icosphere_grid(seed: int = 0, per_axis: int = 4, spacing: float = 1.001, sub
     return soup, vel
 
 
+def _random_rotations(rng, n):
+    q = rng.standard_normal((n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], -1),
+        np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], -1),
+        np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], -1),
+    ], 1)
+
+
+def affine_icospheres(seed: int = 0, per_axis: int = 3, spacing: float = 0.999, subdivisions: int = 2,
+                      radius: float = 0.5, shear: float = 0.0):
+    """Icospheres as affine bodies on a per_axis^3 grid `spacing` apart.
+
+    Body b has A_b = (I + shear N(0,1)) R_b with a seeded rotation R_b and
+    p_b at its grid point; xcs holds the body-local rest coordinates (the
+    sphere centred at the origin) and x = A_b xcs + p_b in float32.
+    Returns (soup, q float32 [n_bodies * 4, 3] as (p, A[:,0], A[:,1], A[:,2])).
+    """
+    rng = np.random.default_rng(seed)
+    Vs, Fs = icosphere(subdivisions, radius)
+    nb = per_axis ** 3
+    bodies = [(Vs, Fs, None)] * nb
+    soup = assemble(bodies)
+    R = _random_rotations(rng, nb)
+    A = (np.eye(3)[None] + shear * rng.standard_normal((nb, 3, 3))) @ R
+    grid = np.stack(np.meshgrid(*[np.arange(per_axis)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    p = spacing * grid.astype(np.float64)
+    q = np.concatenate([p[:, None, :], A.transpose(0, 2, 1)], axis=1).astype(np.float32)  # rows: p, A[:,0..2]
+    rest = soup.xcs.astype(np.float64)
+    b = soup.body
+    qd = q.astype(np.float64)
+    x = qd[b, 0] + np.einsum("vij,vj->vi", qd[b, 1:].transpose(0, 2, 1), rest)
+    soup.x = x.astype(np.float32)
+    return soup, q.reshape(-1, 3)
+
+
+def affine_step(q: np.ndarray, rest: np.ndarray, body: np.ndarray, seed: int = 0, move: float = 0.02,
+                turn: float = 0.05):
+    """A seeded affine step for the bodies of q: p moves N(0, move), A turns
+    by a rotation of angle ~ turn. Returns (dq, dx) with q1 = q - dq and
+    x1 = x - dx, x1 = A1 rest + p1 evaluated like affine_icospheres."""
+    rng = np.random.default_rng(seed)
+    nb = q.shape[0] // 4
+    qd = q.reshape(nb, 4, 3).astype(np.float64)
+    axis = rng.standard_normal((nb, 3))
+    axis /= np.linalg.norm(axis, axis=1, keepdims=True)
+    ang = turn * rng.standard_normal(nb)
+    K = np.zeros((nb, 3, 3))
+    K[:, 0, 1], K[:, 0, 2], K[:, 1, 2] = -axis[:, 2], axis[:, 1], -axis[:, 0]
+    K = K - K.transpose(0, 2, 1)
+    Rs = np.eye(3)[None] + np.sin(ang)[:, None, None] * K + (1 - np.cos(ang))[:, None, None] * (K @ K)
+    A0 = qd[:, 1:].transpose(0, 2, 1)
+    A1 = Rs @ A0
+    p1 = qd[:, 0] + move * rng.standard_normal((nb, 3))
+    q1 = np.concatenate([p1[:, None], A1.transpose(0, 2, 1)], axis=1).astype(np.float32)
+    x0 = qd[body, 0] + np.einsum("vij,vj->vi", A0[body], rest)
+    x1 = p1[body] + np.einsum("vij,vj->vi", A1[body], rest)
+    dq = (q.reshape(nb, 4, 3) - q1).reshape(-1, 3).astype(np.float32)
+    dx = (x0.astype(np.float32) - x1.astype(np.float32)).astype(np.float32)
+    return dq, dx
+
+
 # --------------------------------------------------------- pair streams ----
 def _orthonormal_frames(rng, n):
     a = rng.standard_normal((n, 3))

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_validation():
     L = ipc_warp_amd.lib()
-    assert L.ipc_abi_version() == 1
+    assert L.ipc_abi_version() == 2
     # negative sizes are rejected on the host before any HIP call
     st = L.ipc_contact_hessian_pt(None, 4, None, -1, ctypes.c_float(1e-3), 1e8, 0, None, None, None, None, None)
     assert st == -100000

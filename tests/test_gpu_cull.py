@@ -136,3 +136,59 @@ def test_kernel_timing_hooks(dev, oracle):
         assert n >= 1 and ms > 0.0, k
     # reading resets the counters
     assert all(n == 0 for _, n in S.kernel_times().values())
+
+
+def _intersections_ref(oracle, soup, x1, exclude):
+    out = {}
+    for kind, prims in (("ee", soup.edges), ("pt", soup.triangles)):
+        fn = oracle.ccd_candidates_ee if kind == "ee" else oracle.ccd_candidates_pt
+        q, ids = fn(soup.x, x1, prims, soup.body, 1e-7, exclude)
+        t, _ = oracle.collision_time(kind, soup.x, x1, q, relative=True, threads=8)
+        hit = t < 1.0
+        out[kind] = (q[hit], ids[hit], t[hit])
+    return out
+
+
+@pytest.mark.parametrize("capacity", [None, 4])
+def test_new_intersections_matches_oracle(dev, oracle, capacity):
+    """new_intersections (contact.py:587-670): the TOI < 1 pairs with their
+    alpha; the lists grow when they overflow."""
+    soup, dx = _ccd_setup(15)
+    S = _solver(dev, soup, capacity=capacity)
+    x0 = torch.from_numpy(soup.x).to(dev)
+    x1_np = (soup.x - dx).astype(np.float32)
+    x1 = torch.from_numpy(x1_np).to(dev)
+    ret = S.new_intersections(x0, x1)
+    ref = _intersections_ref(oracle, soup, x1_np, False)
+    assert len(ref["ee"][0]) > 0 and len(ref["pt"][0]) > 0
+    for kind, C, n in (("ee", S.contacts_new, S.n_contacts), ("pt", S.contacts_pt, S.n_contacts_pt)):
+        r_q, r_ids, r_t = ref[kind]
+        assert n == len(r_q)
+        g_q = C.list[:n].cpu().numpy()
+        g_ids = C.e0e1[:n].cpu().numpy()
+        g_t = C.alpha[:n].cpu().numpy()
+        go = np.lexsort((g_ids[:, 1], g_ids[:, 0]))
+        ro = np.lexsort((r_ids[:, 1], r_ids[:, 0]))
+        assert np.array_equal(g_ids[go], r_ids[ro]) and np.array_equal(g_q[go], r_q[ro])
+        assert np.abs(g_t[go] - r_t[ro]).max() <= 1e-12
+    tmin = min(ref["ee"][2].min(), ref["pt"][2].min())
+    assert ret == pytest.approx(0.9 * tmin, abs=1e-12)
+    assert S.raw_toi(torch.from_numpy(dx).to(dev)) == pytest.approx(tmin, abs=1e-12)
+    # no motion: nothing recorded
+    assert S.new_intersections(x0, x0.clone()) == 1.0
+    assert S.n_contacts == 0 and S.n_contacts_pt == 0
+
+
+def test_new_intersections_exclude_same_body(dev, oracle):
+    soup, vel = scenes.icosphere_grid(seed=3, per_axis=2, spacing=1.001)
+    dx = (-0.05 * vel[soup.body]).astype(np.float32)
+    S = _solver(dev, soup, exclude=True)
+    x0 = torch.from_numpy(soup.x).to(dev)
+    x1_np = (soup.x - dx).astype(np.float32)
+    S.new_intersections(x0, torch.from_numpy(x1_np).to(dev))
+    ref = _intersections_ref(oracle, soup, x1_np, True)
+    assert S.n_contacts == len(ref["ee"][0]) and S.n_contacts_pt == len(ref["pt"][0])
+    if S.n_contacts:
+        body = soup.body
+        q = S.contacts_new.list[:S.n_contacts].cpu().numpy()
+        assert (body[q[:, 0]] != body[q[:, 2]]).all()
