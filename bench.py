@@ -90,6 +90,85 @@ def cpu_baseline(soup, dx, threads: int, budget_s: float = 12.0):
     }
 
 
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "c2_contact_step_hbm_traffic.json")
+
+
+def measured_traffic(kernel: str, stream: bool = False):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None when it has no entry.
+    stream selects the 2^22-contact Hessian stream's launch over the C2 step's."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            entries = json.load(f)["kernels"].get(kernel, [])
+    except (OSError, ValueError, KeyError):
+        return None
+    pick = [e for e in entries if (e["grid"] == 1 << 22) == stream]
+    if not pick:
+        return None
+    return pick[0]["fetch_bytes"] + pick[0]["write_bytes"]
+
+
+def fixed_path_step(soup, dx_np, steps, warmup, world_counts, dev):
+    """ms per step of detect_collision + Hessian + energy + collision_free_step
+    through FixedBodyContactSolver on the same state and motion; checks that
+    the DCD sets have the world path's sizes and the TOI its value."""
+    import torch
+
+    from ipc_warp_amd.contact import FixedBodyContactSolver
+
+    nb = int(soup.body.max()) + 1
+    q = np.zeros((nb, 4, 3), np.float32)
+    q[:, 1:] = np.eye(3, dtype=np.float32)[None]
+    dq = np.zeros((nb, 4, 3), np.float32)
+    first = np.searchsorted(soup.body, np.arange(nb))
+    dq[:, 0] = dx_np[first]
+    x = torch.from_numpy(soup.x).to(dev)
+    S = FixedBodyContactSolver(x, x.clone(), torch.from_numpy(soup.edges), torch.from_numpy(soup.triangles),
+                               torch.from_numpy(soup.body), thickness=THICKNESS, buffer_=THICKNESS, kappa=KAPPA, h=H,
+                               disable_self_collision_=True)
+    S.initialize_fixed_body_bvhs(torch.from_numpy(q.reshape(-1, 3)).to(dev))
+    dx = torch.from_numpy(dx_np).to(dev)
+    dq_t = torch.from_numpy(dq.reshape(-1, 3)).to(dev)
+    b = torch.zeros_like(x)
+    out = {}
+
+    def step():
+        b.zero_()
+        S.detect_collision()
+        S.compute_collision_hessian(b)
+        e = S.collision_energy_raw()
+        toi, fb = S.raw_toi_fixed(dx, dq_t)
+        out.update(ee=S.n_contacts, pt=S.n_contacts_pt, toi=toi, fallback=fb, energy=e)
+        return S.n_contacts + S.n_contacts_pt + sum(S.last_ccd_counts())
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pairs = 0
+    for _ in range(steps):
+        pairs += step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    S.set_kernel_timing(True)
+    S.kernel_times()
+    n_prof = max(1, min(steps, 50))
+    for _ in range(n_prof):
+        step()
+    kt = {k: v[0] / n_prof for k, v in S.kernel_times().items() if v[1]}
+    S.set_kernel_timing(False)
+    return {
+        "ms_per_step": el / steps * 1e3,
+        "pairs_per_s": pairs / el,
+        "ccd_candidates": list(S.last_ccd_counts()),
+        "same_dcd_sizes_as_world": (out["ee"], out["pt"]) == (world_counts["ee"], world_counts["pt"]),
+        "same_toi_as_world": out["toi"] == world_counts["toi"],
+        "fallback": out["fallback"],
+        "kernel_ms_per_step": kt,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -225,6 +304,11 @@ def main() -> int:
     big_gbs = 816 * n_big / (big_ms * 1e-3) / 1e9
     del rows, cols, vals
 
+    # The same step through the fixed per-body BVHs (the reference's
+    # FixedBVHWreckingBalls option): identity frames, each body translated by
+    # its dx over the step.
+    fixed = fixed_path_step(soup, dx_np, args.steps, args.warmup, counts, dev)
+
     # STREAM-style copy of 256 MiB: the measured HBM ceiling, and the
     # calibration kernel for the FETCH_SIZE / WRITE_SIZE counters.
     copy_src = torch.empty(64 << 20, dtype=torch.float32, device=dev)
@@ -273,7 +357,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": ach / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(dominant),
+            "traffic_source": os.path.relpath(TRAFFIC_PROFILE, os.path.dirname(os.path.abspath(__file__))),
             "algorithmic_bytes_per_launch": alg[dominant],
             "avg_launch_ms": kernel_ms[dominant],
         },
@@ -285,8 +370,11 @@ def main() -> int:
             "achieved": big_gbs,
             "unit": "GB/s",
             "frac": big_gbs / HBM_PEAK_GBS,
+            "traffic": measured_traffic("hessian_pt", stream=True),
+            "algorithmic_bytes_per_launch": 816 * n_big,
         },
         "hbm_copy_gbs": copy_gbs,
+        "fixed_body_bvh_step": fixed,
     }
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)

@@ -16,6 +16,7 @@ import json
 import sys
 import time
 
+import numpy as np
 import torch
 
 from . import warmstart
@@ -33,6 +34,8 @@ def main(argv=None) -> int:
     ap.add_argument("--eval-seeds", type=int, nargs="*", default=[100, 101])
     ap.add_argument("--eval-steps", type=int, default=30)
     ap.add_argument("--cells", type=int, default=8)
+    ap.add_argument("--save-data", default=None, help="write the collected (features, targets) to this .npz")
+    ap.add_argument("--data", default=None, help="train on a saved .npz instead of collecting trajectories")
     args = ap.parse_args(argv)
     if not torch.cuda.is_available():
         print("train_warmstart needs a GPU", file=sys.stderr)
@@ -47,7 +50,11 @@ def main(argv=None) -> int:
     else:
         t0 = time.time()
         Xs, Ts = [], []
-        for seed in args.train_seeds:
+        if args.data:
+            z = np.load(args.data)
+            Xs.append(torch.from_numpy(z["X"]).to(dev))
+            Ts.append(torch.from_numpy(z["T"]).to(dev))
+        for seed in ([] if args.data else args.train_seeds):
             sim = warmstart.drop_simulation(seed, dev, n=args.cells)
 
             def log(s, st, seed=seed):
@@ -57,6 +64,8 @@ def main(argv=None) -> int:
             Xs.append(X)
             Ts.append(T)
         X, T = torch.cat(Xs), torch.cat(Ts)
+        if args.save_data:
+            np.savez_compressed(args.save_data, X=X.cpu().numpy(), T=T.cpu().numpy())
         t1 = time.time()
         print(f"samples {X.shape[0]} in {t1 - t0:.1f} s", flush=True)
 
