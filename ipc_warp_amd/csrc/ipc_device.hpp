@@ -198,6 +198,7 @@ IPCD_FN V3<double> pick(const V3<double> x[4], int s) {
 
 // Positions are given relative to slot 0 (x[0] = 0) in double.
 IPCD_FN FeatureGeom feature_geom(const Feature& f, const V3<double> x[4]) {
+#pragma clang fp contract(fast)
   FeatureGeom g;
   g.e[0] = g.e[1] = mk(0.0, 0.0, 0.0);
   if (f.kind == F_PP) {
@@ -228,9 +229,9 @@ IPCD_FN FeatureGeom feature_geom(const Feature& f, const V3<double> x[4]) {
     V3<double> e1 = x[2] - x[1], e2 = x[3] - x[1], q = x[0] - x[1];
     double a11 = dot(e1, e1), a12 = dot(e1, e2), a22 = dot(e2, e2);
     double b1 = dot(e1, q), b2 = dot(e2, q);
-    double det = a11 * a22 - a12 * a12;
-    double be = (b1 * a22 - b2 * a12) / det;
-    double ga = (a11 * b2 - a12 * b1) / det;
+    double idet = 1.0 / (a11 * a22 - a12 * a12);
+    double be = (b1 * a22 - b2 * a12) * idet;
+    double ga = (a11 * b2 - a12 * b1) * idet;
     g.w[0] = 1.0; g.w[1] = -(1.0 - be - ga); g.w[2] = -be; g.w[3] = -ga;
     g.D[0][0] = 0.0; g.D[0][1] = 1.0; g.D[0][2] = -1.0; g.D[0][3] = 0.0;
     g.D[1][0] = 0.0; g.D[1][1] = 1.0; g.D[1][2] = 0.0; g.D[1][3] = -1.0;
@@ -242,9 +243,9 @@ IPCD_FN FeatureGeom feature_geom(const Feature& f, const V3<double> x[4]) {
     V3<double> ea = x[1] - x[0], eb = x[3] - x[2], q = x[0] - x[2];
     double a11 = dot(ea, ea), a12 = -dot(ea, eb), a22 = dot(eb, eb);
     double b1 = -dot(ea, q), b2 = dot(eb, q);
-    double det = a11 * a22 - a12 * a12;
-    double s = (b1 * a22 - b2 * a12) / det;
-    double t = (a11 * b2 - a12 * b1) / det;
+    double idet = 1.0 / (a11 * a22 - a12 * a12);
+    double s = (b1 * a22 - b2 * a12) * idet;
+    double t = (a11 * b2 - a12 * b1) * idet;
     g.w[0] = 1.0 - s; g.w[1] = s; g.w[2] = -(1.0 - t); g.w[3] = -t;
     g.D[0][0] = -1.0; g.D[0][1] = 1.0; g.D[0][2] = 0.0; g.D[0][3] = 0.0;
     g.D[1][0] = 0.0; g.D[1][1] = 0.0; g.D[1][2] = 1.0; g.D[1][3] = -1.0;
@@ -255,53 +256,70 @@ IPCD_FN FeatureGeom feature_geom(const Feature& f, const V3<double> x[4]) {
   return g;
 }
 
-// Cyclic Jacobi on a symmetric 5x5 (row-major), double. Returns eigenvalues
-// in ev and eigenvectors in the columns of V.
-IPCD_FN void jacobi5(double A[25], double ev[5], double V[25]) {
+// 1 / sqrt(x): the device's reciprocal square root (the host pass, used
+// only to check these functions on a CPU, divides).
+IPCD_FN double inv_sqrt(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return rsqrt(x);
+#else
+  return 1.0 / sqrt(x);
+#endif
+}
+
+// Cyclic Jacobi on a symmetric n x n (row-major), double. Returns
+// eigenvalues in ev and eigenvectors in the columns of V. Sweeps stop once
+// the off-diagonal mass is below 1e-24 of the diagonal's (eigenvalues then
+// carry relative errors far below the 1e-5 the Hessians are held to).
+template <int n>
+IPCD_FN void jacobi_sym(double A[n * n], double ev[n], double V[n * n]) {
+#pragma clang fp contract(fast)
 #pragma unroll
-  for (int i = 0; i < 25; ++i) V[i] = (i % 6 == 0) ? 1.0 : 0.0;
+  for (int i = 0; i < n * n; ++i) V[i] = (i % (n + 1) == 0) ? 1.0 : 0.0;
   for (int sweep = 0; sweep < 12; ++sweep) {
     double off = 0.0, diag = 0.0;
 #pragma unroll
-    for (int i = 0; i < 5; ++i) {
-      diag += A[i * 5 + i] * A[i * 5 + i];
+    for (int i = 0; i < n; ++i) {
+      diag += A[i * n + i] * A[i * n + i];
 #pragma unroll
-      for (int j = i + 1; j < 5; ++j) off += A[i * 5 + j] * A[i * 5 + j];
+      for (int j = i + 1; j < n; ++j) off += A[i * n + j] * A[i * n + j];
     }
-    if (!(off > 1.0e-30 * diag)) break;
+    if (!(off > 1.0e-24 * diag)) break;
 #pragma unroll
-    for (int p = 0; p < 4; ++p) {
+    for (int p = 0; p < n - 1; ++p) {
 #pragma unroll
-      for (int q = p + 1; q < 5; ++q) {
-        double apq = A[p * 5 + q];
+      for (int q = p + 1; q < n; ++q) {
+        double apq = A[p * n + q];
         if (apq != 0.0) {
-          double theta = (A[q * 5 + q] - A[p * 5 + p]) / (2.0 * apq);
-          double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-          double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+          // tan of the rotation angle, t = sgn(d) e / (|d| + sqrt(d^2 + e^2))
+          // with d = a_qq - a_pp, e = 2 a_pq (the textbook theta = d / e form
+          // with its two divisions folded into one)
+          double d = A[q * n + q] - A[p * n + p], e = 2.0 * apq;
+          double t = (d >= 0.0 ? e : -e) / (fabs(d) + sqrt(d * d + e * e));
+          double c = inv_sqrt(t * t + 1.0), s = t * c;
 #pragma unroll
-          for (int k = 0; k < 5; ++k) {
-            double akp = A[k * 5 + p], akq = A[k * 5 + q];
-            A[k * 5 + p] = c * akp - s * akq;
-            A[k * 5 + q] = s * akp + c * akq;
+          for (int k = 0; k < n; ++k) {
+            double akp = A[k * n + p], akq = A[k * n + q];
+            A[k * n + p] = c * akp - s * akq;
+            A[k * n + q] = s * akp + c * akq;
           }
 #pragma unroll
-          for (int k = 0; k < 5; ++k) {
-            double apk = A[p * 5 + k], aqk = A[q * 5 + k];
-            A[p * 5 + k] = c * apk - s * aqk;
-            A[q * 5 + k] = s * apk + c * aqk;
+          for (int k = 0; k < n; ++k) {
+            double apk = A[p * n + k], aqk = A[q * n + k];
+            A[p * n + k] = c * apk - s * aqk;
+            A[q * n + k] = s * apk + c * aqk;
           }
 #pragma unroll
-          for (int k = 0; k < 5; ++k) {
-            double vkp = V[k * 5 + p], vkq = V[k * 5 + q];
-            V[k * 5 + p] = c * vkp - s * vkq;
-            V[k * 5 + q] = s * vkp + c * vkq;
+          for (int k = 0; k < n; ++k) {
+            double vkp = V[k * n + p], vkq = V[k * n + q];
+            V[k * n + p] = c * vkp - s * vkq;
+            V[k * n + q] = s * vkp + c * vkq;
           }
         }
       }
     }
   }
 #pragma unroll
-  for (int i = 0; i < 5; ++i) ev[i] = A[i * 5 + i];
+  for (int i = 0; i < n; ++i) ev[i] = A[i * n + i];
 }
 
 // Compressed form of the 16 blocks of a contact Hessian
@@ -327,14 +345,15 @@ IPCD_FN int sidx(int i, int j) {
 // basis of the D_j projected orthogonal to w. Fills Ntt, u, s and the
 // geometry fields of `C`; returns |r|^2 (the feature's squared distance).
 IPCD_FN double nsd_compressed(const FeatureGeom& g, bool want_nsd, CompressedHessian& C) {
+#pragma clang fp contract(fast)
   double d2 = dot(g.r, g.r);
   double rn = sqrt(d2);
   V3<double> rh = rn > 0.0 ? g.r * (1.0 / rn) : mk(0.0, 0.0, 0.0);
   double wn2 = g.w[0] * g.w[0] + g.w[1] * g.w[1] + g.w[2] * g.w[2] + g.w[3] * g.w[3];
-  double wn = sqrt(wn2);
+  double wn = sqrt(wn2), iwn = inv_sqrt(wn2);
   double wh[4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) { wh[i] = g.w[i] / wn; C.what[i] = float(wh[i]); C.w[i] = float(g.w[i]); }
+  for (int i = 0; i < 4; ++i) { wh[i] = g.w[i] * iwn; C.what[i] = float(wh[i]); C.w[i] = float(g.w[i]); }
   C.rh[0] = float(rh.x); C.rh[1] = float(rh.y); C.rh[2] = float(rh.z);
 #pragma unroll
   for (int i = 0; i < 6; ++i) C.Ntt[i] = 0.0f;
@@ -354,16 +373,16 @@ IPCD_FN double nsd_compressed(const FeatureGeom& g, bool want_nsd, CompressedHes
     for (int i = 0; i < 4; ++i) Z[i][j] = g.D[j][i] - wd[j] * wh[i];
   }
   {
-    double n0 = sqrt(Z[0][0] * Z[0][0] + Z[1][0] * Z[1][0] + Z[2][0] * Z[2][0] + Z[3][0] * Z[3][0]);
+    double in0 = inv_sqrt(Z[0][0] * Z[0][0] + Z[1][0] * Z[1][0] + Z[2][0] * Z[2][0] + Z[3][0] * Z[3][0]);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) Z[i][0] /= n0;
+    for (int i = 0; i < 4; ++i) Z[i][0] *= in0;
     if (g.k == 2) {
       double p = Z[0][0] * Z[0][1] + Z[1][0] * Z[1][1] + Z[2][0] * Z[2][1] + Z[3][0] * Z[3][1];
 #pragma unroll
       for (int i = 0; i < 4; ++i) Z[i][1] -= p * Z[i][0];
-      double n1 = sqrt(Z[0][1] * Z[0][1] + Z[1][1] * Z[1][1] + Z[2][1] * Z[2][1] + Z[3][1] * Z[3][1]);
+      double in1 = inv_sqrt(Z[0][1] * Z[0][1] + Z[1][1] * Z[1][1] + Z[2][1] * Z[2][1] + Z[3][1] * Z[3][1]);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) Z[i][1] /= n1;
+      for (int i = 0; i < 4; ++i) Z[i][1] *= in1;
     } else {
 #pragma unroll
       for (int i = 0; i < 4; ++i) Z[i][1] = 0.0;
@@ -390,34 +409,91 @@ IPCD_FN double nsd_compressed(const FeatureGeom& g, bool want_nsd, CompressedHes
     Gi[0][0] = 1.0 / dot(g.e[0], g.e[0]); Gi[0][1] = Gi[1][0] = Gi[1][1] = 0.0;
     for (int a = 0; a < 5; ++a) M[a][1] = 0.0;
   }
-  // K = 2 diag(|w|^2 I3, 0) - 2 M G^-1 M^T.
-  double K[25];
+  // K = 2 diag(|w|^2 I3, 0) - 2 M G^-1 M^T. The unit normal of the column
+  // span of M's rows 0..2 (padded with zeros) is an eigenvector of K with
+  // eigenvalue 2|w|^2 > 0, so the negative part lives in the 4-dimensional
+  // span of Q = [(q1, 0), (q2, 0), (0, e1), (0, e2)], q1, q2 an orthonormal
+  // basis of that column span: K is reduced to K4 = Q^T K Q.
+  double q1[3], q2[3];
+  {
+    double n0 = M[0][0] * M[0][0] + M[1][0] * M[1][0] + M[2][0] * M[2][0];
+    if (n0 > 0.0) {
+      double in0 = inv_sqrt(n0);
 #pragma unroll
-  for (int a = 0; a < 5; ++a) {
-    double MG0 = M[a][0] * Gi[0][0] + M[a][1] * Gi[1][0];
-    double MG1 = M[a][0] * Gi[0][1] + M[a][1] * Gi[1][1];
+      for (int a = 0; a < 3; ++a) q1[a] = M[a][0] * in0;
+    } else {
+      q1[0] = 1.0; q1[1] = 0.0; q1[2] = 0.0;
+    }
+    double p = q1[0] * M[0][1] + q1[1] * M[1][1] + q1[2] * M[2][1];
+    double b[3] = {M[0][1] - p * q1[0], M[1][1] - p * q1[1], M[2][1] - p * q1[2]};
+    double nb = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+    double m1 = M[0][1] * M[0][1] + M[1][1] * M[1][1] + M[2][1] * M[2][1];
+    if (!(nb > 1.0e-24 * m1) || !(nb > 0.0)) {
+      // second column absent or parallel: any unit vector normal to q1
+      int ax = fabs(q1[0]) < 0.577 ? 0 : (fabs(q1[1]) < 0.577 ? 1 : 2);
+      double ex[3] = {ax == 0 ? 1.0 : 0.0, ax == 1 ? 1.0 : 0.0, ax == 2 ? 1.0 : 0.0};
+      double pe = q1[ax];
 #pragma unroll
-    for (int b = 0; b < 5; ++b) {
-      double v = -2.0 * (MG0 * M[b][0] + MG1 * M[b][1]);
-      if (a == b && a < 3) v += 2.0 * wn2;
-      K[a * 5 + b] = v;
+      for (int a = 0; a < 3; ++a) b[a] = ex[a] - pe * q1[a];
+      nb = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+    }
+    double inb = inv_sqrt(nb);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) q2[a] = b[a] * inb;
+  }
+  double M4[4][2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    M4[0][j] = q1[0] * M[0][j] + q1[1] * M[1][j] + q1[2] * M[2][j];
+    M4[1][j] = q2[0] * M[0][j] + q2[1] * M[1][j] + q2[2] * M[2][j];
+    M4[2][j] = M[3][j];
+    M4[3][j] = M[4][j];
+  }
+  double K[16];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    double MG0 = M4[a][0] * Gi[0][0] + M4[a][1] * Gi[1][0];
+    double MG1 = M4[a][0] * Gi[0][1] + M4[a][1] * Gi[1][1];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      double v = -2.0 * (MG0 * M4[b][0] + MG1 * M4[b][1]);
+      if (a == b && a < 2) v += 2.0 * wn2;
+      K[a * 4 + b] = v;
     }
   }
-  double ev[5], V[25];
-  jacobi5(K, ev, V);
-  // N = sum over negative eigenvalues of lambda v v^T.
-  double N[25];
+  double ev[4], V[16];
+  jacobi_sym<4>(K, ev, V);
+  // N4 = sum over negative eigenvalues of lambda v v^T
+  double N4[16];
 #pragma unroll
-  for (int i = 0; i < 25; ++i) N[i] = 0.0;
+  for (int i = 0; i < 16; ++i) N4[i] = 0.0;
 #pragma unroll
-  for (int m = 0; m < 5; ++m) {
+  for (int m = 0; m < 4; ++m) {
     double lam = ev[m];
     if (lam < 0.0) {
 #pragma unroll
-      for (int a = 0; a < 5; ++a)
+      for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 5; ++b) N[a * 5 + b] += lam * V[a * 5 + m] * V[b * 5 + m];
+        for (int b = 0; b < 4; ++b) N4[a * 4 + b] += lam * V[a * 4 + m] * V[b * 4 + m];
     }
+  }
+  // back to the 5 x 5 layout: N = Q N4 Q^T
+  double N[25];
+  {
+    const double Qa[3][2] = {{q1[0], q2[0]}, {q1[1], q2[1]}, {q1[2], q2[2]}};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+        N[a * 5 + b] = Qa[a][0] * (N4[0] * Qa[b][0] + N4[1] * Qa[b][1]) + Qa[a][1] * (N4[4] * Qa[b][0] + N4[5] * Qa[b][1]);
+#pragma unroll
+      for (int l = 0; l < 2; ++l) {
+        double v = Qa[a][0] * N4[0 * 4 + 2 + l] + Qa[a][1] * N4[1 * 4 + 2 + l];
+        N[a * 5 + 3 + l] = v;
+        N[(3 + l) * 5 + a] = v;
+      }
+    }
+    N[18] = N4[10]; N[19] = N4[11]; N[23] = N4[14]; N[24] = N4[15];
   }
   C.Ntt[0] = float(N[0]); C.Ntt[1] = float(N[1]); C.Ntt[2] = float(N[2]);
   C.Ntt[3] = float(N[6]); C.Ntt[4] = float(N[7]); C.Ntt[5] = float(N[12]);
@@ -438,6 +514,7 @@ IPCD_FN double nsd_compressed(const FeatureGeom& g, bool want_nsd, CompressedHes
 
 // Block (i, j) of the compressed Hessian, row-major 3x3.
 IPCD_FN void expand_block(const CompressedHessian& C, int i, int j, float out[9]) {
+#pragma clang fp contract(fast)
   float ww = C.what[i] * C.what[j];
   float rr = C.cg * C.w[i] * C.w[j] + C.cn * C.s[sidx(i, j)];
   const float* r = C.rh;
