@@ -128,6 +128,14 @@ int ipc_detect_collision_ee(ipc_ctx* ctx, const float* x, float thickness, float
 int ipc_detect_collision_pt(ipc_ctx* ctx, const float* x, float thickness, float buffer, int exclude_same_body,
                             int* contacts, int* e0e1, int capacity, int* count, void* stream);
 
+/* Both kinds of detect_collision (contact.py:687-725) in one call: the EE
+ * half on `stream`, the PT half concurrently on the context's own side
+ * stream (default priority), joined back to `stream` before returning;
+ * arguments and results as the two calls above. No host synchronisation. */
+int ipc_detect_collision(ipc_ctx* ctx, const float* x, float thickness, float buffer, int exclude_same_body, int* ee,
+                         int* ee_ids, int ee_capacity, int* ee_count, int* pt, int* pt_ids, int pt_capacity,
+                         int* pt_count, void* stream);
+
 /* --------------------------------------------- CCD candidates ---
  * The candidate sets of edge_edge_toi / point_triangle_toi
  * (ccd/ccd.py:166-210,360-427): swept boxes over x0 and x1 padded by

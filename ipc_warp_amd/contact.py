@@ -61,11 +61,11 @@ class Triplets:
     vals: torch.Tensor
 
 
-def _new_contacts(capacity: int, device) -> Contacts:
+def _new_contacts(capacity: int, device, cnt: torch.Tensor | None = None) -> Contacts:
     return Contacts(
         list=torch.zeros((capacity, 4), dtype=torch.int32, device=device),
         e0e1=torch.zeros((capacity, 2), dtype=torch.int32, device=device),
-        cnt=torch.zeros(1, dtype=torch.int32, device=device),
+        cnt=torch.zeros(1, dtype=torch.int32, device=device) if cnt is None else cnt,
         capacity=capacity,
         alpha=torch.zeros(capacity, dtype=torch.float64, device=device),
     )
@@ -96,8 +96,10 @@ class ContactSolverBase:
                                            else disable_self_collision_)
         self.has_triangles = self.triangles.shape[0] > 0
         cap = contact_volume if capacity is None else int(capacity)
-        self.contacts_new = _new_contacts(cap, dev)
-        self.contacts_pt = _new_contacts(cap, dev)
+        # both lists' device counters live in one tensor: one readback per DCD
+        self._counts = torch.zeros(2, dtype=torch.int32, device=dev)
+        self.contacts_new = _new_contacts(cap, dev, self._counts[0:1])
+        self.contacts_pt = _new_contacts(cap, dev, self._counts[1:2])
         self.n_contacts = 0
         self.n_contacts_pt = 0
         self.collision_triplets = None
@@ -129,20 +131,10 @@ class ContactSolverBase:
     def _grow_contact_list(self, contacts: Contacts, required: int) -> Contacts:
         """contact.py:523-537: regrow to the next power of two >= required."""
         capacity = 1 << (int(required) - 1).bit_length()
-        grown = _new_contacts(capacity, self.device)
+        grown = _new_contacts(capacity, self.device, contacts.cnt)
         if verbose:
             print(f"contact buffer grown to {capacity}")
         return grown
-
-    def _dcd(self, fn, contacts: Contacts, what: str):
-        while True:
-            check(fn(self._ctx, self.x.data_ptr(), float(self.thickness), float(self.buffer),
-                     int(self.disable_self_collision), contacts.list.data_ptr(), contacts.e0e1.data_ptr(),
-                     int(contacts.capacity), contacts.cnt.data_ptr(), self._stream()), what)
-            n = int(contacts.cnt.item())
-            if n <= contacts.capacity:
-                return contacts, n
-            contacts = self._grow_contact_list(contacts, n)
 
     # ---------------------------------------------------------- reference API ----
     def compute_V(self, ret=True):
@@ -150,19 +142,30 @@ class ContactSolverBase:
         return None
 
     def detect_collision(self):
-        """contact.py:687-725."""
+        """contact.py:687-725: EE and PT lists in one library call (the two halves
+        run concurrently) and one readback of both counts; a list that overflowed
+        grows to the next power of two and the call runs again."""
         self.compute_V(ret=False)
-        self.contacts_new, self.n_contacts = self._dcd(lib().ipc_detect_collision_ee, self.contacts_new,
-                                                       "detect_collision_ee")
+        while True:
+            ee, pt = self.contacts_new, self.contacts_pt
+            check(lib().ipc_detect_collision(self._ctx, self.x.data_ptr(), float(self.thickness), float(self.buffer),
+                                             int(self.disable_self_collision), ee.list.data_ptr(), ee.e0e1.data_ptr(),
+                                             int(ee.capacity), ee.cnt.data_ptr(), pt.list.data_ptr(),
+                                             pt.e0e1.data_ptr(), int(pt.capacity) if self.has_triangles else 0,
+                                             pt.cnt.data_ptr(), self._stream()), "detect_collision")
+            n_ee, n_pt = self._counts.tolist()
+            grown = False
+            if n_ee > ee.capacity:
+                self.contacts_new = self._grow_contact_list(ee, n_ee)
+                grown = True
+            if n_pt > pt.capacity:
+                self.contacts_pt = self._grow_contact_list(pt, n_pt)
+                grown = True
+            if not grown:
+                break
+        self.n_contacts, self.n_contacts_pt = n_ee, n_pt
         if verbose:
-            print(f"n ee contacts = {self.n_contacts}")
-        if self.has_triangles:
-            self.contacts_pt, self.n_contacts_pt = self._dcd(lib().ipc_detect_collision_pt, self.contacts_pt,
-                                                             "detect_collision_pt")
-        else:
-            self.n_contacts_pt = 0
-        if verbose:
-            print(f"n pt contacts = {self.n_contacts_pt}")
+            print(f"n ee contacts = {self.n_contacts}, n pt contacts = {self.n_contacts_pt}")
 
     def collision_free_step(self, dx) -> float:
         """Zero-thickness CCD upper bound for x -= alpha * dx (contact.py:539-585)."""
@@ -340,7 +343,7 @@ class FixedBodyContactSolver(ContactSolverBase):
                                                    int(c_ee.capacity), c_ee.cnt.data_ptr(), c_pt.list.data_ptr(),
                                                    c_pt.e0e1.data_ptr(), int(c_pt.capacity), c_pt.cnt.data_ptr(),
                                                    self._stream()), "detect_collision_fixed")
-            n_ee, n_pt = int(c_ee.cnt.item()), int(c_pt.cnt.item())
+            n_ee, n_pt = self._counts.tolist()
             grown = False
             if n_ee > c_ee.capacity:
                 self.contacts_new = self._grow_contact_list(c_ee, n_ee)

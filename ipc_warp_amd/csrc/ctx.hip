@@ -26,6 +26,10 @@ struct ipc_ctx {
   Forest fe, ft;
   DevBuf vert_base, vert_ids, body_lo, body_hi;  // per-body vertex lists and rest boxes
   DevBuf frames, world_lo, world_hi, pairs, tasks_ee, tasks_pt;
+  // side stream (default priority) for the PT half of a call, joined back
+  // to the caller's stream with events
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   DevBuf small;            // [0] uint64 toi bits, [64..] int counters
   void* host = nullptr;    // pinned readback area
   int last_counts[2] = {0, 0};
@@ -70,6 +74,24 @@ hipError_t timed(ipc_ctx* c, int kind, hipStream_t s, F launch) {
   e = launch();
   (void)hipEventRecord(b, s);
   c->pending.push_back({kind, a, b});
+  return e;
+}
+
+// The EE and PT halves of a call are independent: the PT half runs on the
+// context's side stream, forked from and joined back to the caller's stream.
+hipError_t fork_side(ipc_ctx* c, hipStream_t s) {
+  hipError_t e = hipSuccess;
+  if (!c->side) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+  if (e == hipSuccess && !c->fork_ev) e = hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming);
+  if (e == hipSuccess && !c->join_ev) e = hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventRecord(c->fork_ev, s);
+  if (e == hipSuccess) e = hipStreamWaitEvent(c->side, c->fork_ev, 0);
+  return e;
+}
+
+hipError_t join_side(ipc_ctx* c, hipStream_t s) {
+  hipError_t e = hipEventRecord(c->join_ev, c->side);
+  if (e == hipSuccess) e = hipStreamWaitEvent(s, c->join_ev, 0);
   return e;
 }
 
@@ -134,6 +156,9 @@ int ipc_ctx_destroy(ipc_ctx* c) {
   c->frames.release(); c->world_lo.release(); c->world_hi.release(); c->pairs.release();
   c->tasks_ee.release(); c->tasks_pt.release();
   if (c->host) (void)hipHostFree(c->host);
+  if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+  if (c->join_ev) (void)hipEventDestroy(c->join_ev);
+  if (c->side) (void)hipStreamDestroy(c->side);
   delete c;
   return IPC_OK;
 }
@@ -204,6 +229,22 @@ int ipc_detect_collision_pt(ipc_ctx* c, const float* x, float thickness, float b
   q.capacity = capacity;
   q.count = count;
   return ctx_hip(timed(c, IPC_TIMER_QUERY_DCD_PT, s, [&] { return run_query(q, s); }), "detect_collision_pt query");
+}
+
+// detect_collision (contact.py:687-725): the EE half on `stream`, the PT half
+// on the context's side stream, joined back; both counts stay on the device.
+int ipc_detect_collision(ipc_ctx* c, const float* x, float thickness, float buffer, int exclude_same_body, int* ee,
+                         int* ee_ids, int ee_capacity, int* ee_count, int* pt, int* pt_ids, int pt_capacity,
+                         int* pt_count, void* stream) {
+  if (!c) return ctx_fail(IPC_EINVAL, "detect_collision: null context");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int st = ctx_hip(fork_side(c, s), "detect_collision fork");
+  if (st) return st;
+  st = ipc_detect_collision_ee(c, x, thickness, buffer, exclude_same_body, ee, ee_ids, ee_capacity, ee_count, stream);
+  int st2 = ipc_detect_collision_pt(c, x, thickness, buffer, exclude_same_body, pt, pt_ids, pt_capacity, pt_count,
+                                    c->side);
+  int st3 = ctx_hip(join_side(c, s), "detect_collision join");
+  return st ? st : (st2 ? st2 : st3);
 }
 
 // Candidate sets of edge_edge_toi / point_triangle_toi between x0 and x1.
@@ -290,25 +331,30 @@ int ipc_collision_free_step(ipc_ctx* c, const float* x, const float* dx, float p
     if (!st) st = ctx_hip(launch_fill_u64(toi_bits(c), 1, kOneBits, s), "collision_free_step");
     if (st) return st;
     int* cnt = counters(c);
-    st = ipc_ccd_candidates_ee(c, x, c->x1.as<float>(), padding, exclude_same_body, c->cand_ee.as<int>(), nullptr,
-                               c->cap_ee, cnt + 0, stream);
+    // EE half on `stream`, PT half on the side stream. The TOI kernels read
+    // the candidate counts from device memory, so no host round trip is
+    // needed between the phases.
+    st = ctx_hip(fork_side(c, s), "collision_free_step fork");
     if (st) return st;
-    st = ipc_ccd_candidates_pt(c, x, c->x1.as<float>(), padding, exclude_same_body, c->cand_pt.as<int>(), nullptr,
-                               c->cap_pt, cnt + 1, stream);
-    if (st) return st;
-    // TOI over the first min(count, capacity) candidates: the kernels read the
-    // device count, so no host round trip is needed between the phases.
-    st = ctx_hip(timed(c, IPC_TIMER_TOI_EE, s, [&] {
-                   return launch_collision_time_counted(true, x, c->x1.as<float>(), c->nv, c->cand_ee.as<int>(),
-                                                        cnt + 0, c->cap_ee, toi_bits(c), s);
-                 }),
-                 "collision_free_step EE TOI");
-    if (!st)
-      st = ctx_hip(timed(c, IPC_TIMER_TOI_PT, s, [&] {
-                     return launch_collision_time_counted(false, x, c->x1.as<float>(), c->nv, c->cand_pt.as<int>(),
-                                                          cnt + 1, c->cap_pt, toi_bits(c), s);
-                   }),
-                   "collision_free_step PT TOI");
+    int st_ee = ipc_ccd_candidates_ee(c, x, c->x1.as<float>(), padding, exclude_same_body, c->cand_ee.as<int>(), nullptr,
+                                      c->cap_ee, cnt + 0, stream);
+    if (!st_ee)
+      st_ee = ctx_hip(timed(c, IPC_TIMER_TOI_EE, s, [&] {
+                        return launch_collision_time_counted(true, x, c->x1.as<float>(), c->nv, c->cand_ee.as<int>(),
+                                                             cnt + 0, c->cap_ee, toi_bits(c), s);
+                      }),
+                      "collision_free_step EE TOI");
+    int st_pt = ipc_ccd_candidates_pt(c, x, c->x1.as<float>(), padding, exclude_same_body, c->cand_pt.as<int>(), nullptr,
+                                      c->cap_pt, cnt + 1, c->side);
+    if (!st_pt)
+      st_pt = ctx_hip(timed(c, IPC_TIMER_TOI_PT, c->side, [&] {
+                        return launch_collision_time_counted(false, x, c->x1.as<float>(), c->nv, c->cand_pt.as<int>(),
+                                                             cnt + 1, c->cap_pt, toi_bits(c), c->side);
+                      }),
+                      "collision_free_step PT TOI");
+    st = ctx_hip(join_side(c, s), "collision_free_step join");
+    if (st_ee) return st_ee;
+    if (st_pt) return st_pt;
     if (st) return st;
     st = ctx_hip(hipMemcpyAsync(c->host, c->small.p, 128, hipMemcpyDeviceToHost, s), "collision_free_step readback");
     if (!st) st = ctx_hip(hipStreamSynchronize(s), "collision_free_step sync");
