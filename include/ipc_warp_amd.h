@@ -136,6 +136,15 @@ int ipc_detect_collision(ipc_ctx* ctx, const float* x, float thickness, float bu
                          int* ee_ids, int ee_capacity, int* ee_count, int* pt, int* pt_ids, int pt_capacity,
                          int* pt_count, void* stream);
 
+/* Penetration check (no reference counterpart; it verifies the Newton
+ * loop's states): counts the edge-triangle pairs of the soup at x that
+ * cross (segment through the closed triangle, no shared vertex; FP64 signed
+ * volumes of the fp32 positions). (edge, triangle) ids go to ids int32
+ * [capacity][2] (may be NULL with capacity 0); *count (device int) is the
+ * total. A state reached by CCD-bounded moves from an intersection-free one
+ * has none. */
+int ipc_count_intersections(ipc_ctx* ctx, const float* x, int* ids, int capacity, int* count, void* stream);
+
 /* --------------------------------------------- CCD candidates ---
  * The candidate sets of edge_edge_toi / point_triangle_toi
  * (ccd/ccd.py:166-210,360-427): swept boxes over x0 and x1 padded by

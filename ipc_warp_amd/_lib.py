@@ -49,6 +49,7 @@ _SIGNATURES = {
     "ipc_collision_free_step_fixed": ([_vp, _vp, _vp, _vp, _vp, _vp, _c_float, _vp, _vp, _vp], _c_int),
     "ipc_detect_collision": ([_vp, _vp, _c_float, _c_float, _c_int, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _vp, _vp],
                              _c_int),
+    "ipc_count_intersections": ([_vp, _vp, _vp, _c_int, _vp, _vp], _c_int),
     "ipc_warmstart_features": ([_vp, _c_int, _vp, _vp, _c_int, _vp, _c_int, _c_float, _vp, _vp, _vp], _c_int),
 }
 

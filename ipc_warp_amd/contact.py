@@ -217,6 +217,19 @@ class ContactSolverBase:
                 self.contacts_pt, self.n_contacts_pt = contacts, n
         return 0.9 * toi if toi < 1.0 else 1.0
 
+    def count_intersections(self, x=None, capacity: int = 0):
+        """Edge-triangle crossings of the soup at x (default: the solver's x);
+        returns (count, int32 [min(count, capacity), 2] (edge, triangle) ids)."""
+        x = self.x if x is None else x
+        if x.shape != self.x.shape or x.dtype != torch.float32 or not x.is_contiguous():
+            raise ValueError("x must be a contiguous float32 tensor shaped like the solver's x")
+        ids = torch.zeros((max(capacity, 1), 2), dtype=torch.int32, device=self.device)
+        cnt = torch.zeros(1, dtype=torch.int32, device=self.device)
+        check(lib().ipc_count_intersections(self._ctx, x.data_ptr(), ids.data_ptr() if capacity else None,
+                                            int(capacity), cnt.data_ptr(), self._stream()), "count_intersections")
+        n = int(cnt.item())
+        return n, ids[:min(n, capacity)]
+
     def ccd_candidates(self, kind: str, x0, x1, padding=CCD_PADDING, capacity=1 << 12):
         """Candidate quadruples and primitive ids of edge_edge_toi (kind 'ee') or
         point_triangle_toi ('pt') between x0 and x1 (ccd/ccd.py:360-427)."""

@@ -247,6 +247,34 @@ int ipc_detect_collision(ipc_ctx* c, const float* x, float thickness, float buff
   return st ? st : (st2 ? st2 : st3);
 }
 
+// Edge-triangle crossings of the surface at x: every edge's bounds against
+// the triangle BVH, each hit that shares no vertex tested exactly
+// (segment_hits_triangle, FP64 signed volumes). A penetration-free state has
+// none. (edge, triangle) ids go to `ids` up to capacity; *count (device) is
+// the total.
+int ipc_count_intersections(ipc_ctx* c, const float* x, int* ids, int capacity, int* count, void* stream) {
+  if (!c || !x || !count) return ctx_fail(IPC_EINVAL, "count_intersections: null pointer");
+  if (capacity < 0 || (capacity > 0 && !ids)) return ctx_fail(IPC_EINVAL, "count_intersections: bad capacity");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int st = ctx_hip(launch_zero_i32(count, 1, s), "count_intersections");
+  if (st || c->ne == 0 || c->nf == 0) return st;
+  st = ctx_hip(build_bvh(c->tri_bvh, BOX_TRI, c->nf, c->tris.as<int>(), c->body.as<int>(), x, nullptr, 0.0f, s),
+               "count_intersections triangle BVH");
+  if (st) return st;
+  QueryLaunch q{};
+  q.mode = Q_ISECT;
+  q.n_query = c->ne;
+  q.edges = c->edges.as<int>();
+  q.tris = c->tris.as<int>();
+  q.body = c->body.as<int>();
+  q.x0 = x;
+  q.bvh = &c->tri_bvh;
+  q.ids = reinterpret_cast<int2*>(ids);
+  q.capacity = capacity;
+  q.count = count;
+  return ctx_hip(run_query(q, s), "count_intersections query");
+}
+
 // Candidate sets of edge_edge_toi / point_triangle_toi between x0 and x1.
 int ipc_ccd_candidates_ee(ipc_ctx* c, const float* x0, const float* x1, float padding, int exclude_same_body,
                           int* quads, int* ids, int capacity, int* count, void* stream) {

@@ -299,6 +299,15 @@ struct QueryArgs {
 
 template <int MODE>
 __device__ __forceinline__ bool accept_pair(const QueryArgs& A, int i, int j, int4& rec) {
+  if (MODE == Q_ISECT) {
+    // edge i against triangle j: a crossing, not sharing a vertex
+    int a0 = A.edges[2 * i], a1 = A.edges[2 * i + 1];
+    int t0 = A.tris[3 * j], t1 = A.tris[3 * j + 1], t2 = A.tris[3 * j + 2];
+    if (a0 == t0 || a0 == t1 || a0 == t2 || a1 == t0 || a1 == t1 || a1 == t2) return false;
+    rec = make_int4(a0, a1, t0, t1);
+    return segment_hits_triangle(to_d(ldx(A.x0, a0)), to_d(ldx(A.x0, a1)), to_d(ldx(A.x0, t0)), to_d(ldx(A.x0, t1)),
+                                 to_d(ldx(A.x0, t2)));
+  }
   if (MODE == 0 || MODE == 2) {
     if (!(i < j)) return false;
     int a0 = A.edges[2 * i], a1 = A.edges[2 * i + 1], b0 = A.edges[2 * j], b1 = A.edges[2 * j + 1];
@@ -340,7 +349,12 @@ __global__ void __launch_bounds__(kQueryBlock) query_kernel(QueryArgs A) {
   int qbody = -1;
   if (live) {
     i = A.order ? A.order[slot] : slot;
-    if (MODE == 0 || MODE == 2) {
+    if (MODE == Q_ISECT) {
+      V3<float> p = ldx(A.x0, A.edges[2 * i]), q = ldx(A.x0, A.edges[2 * i + 1]);
+      V3<float> lo = vmin(p, q), hi = vmax(p, q);
+      qlo[0] = lo.x; qlo[1] = lo.y; qlo[2] = lo.z;
+      qhi[0] = hi.x; qhi[1] = hi.y; qhi[2] = hi.z;
+    } else if (MODE == 0 || MODE == 2) {
       float4 l = A.qlo[slot], h = A.qhi[slot];
       qlo[0] = l.x; qlo[1] = l.y; qlo[2] = l.z;
       qhi[0] = h.x; qhi[1] = h.y; qhi[2] = h.z;
@@ -410,7 +424,7 @@ __global__ void __launch_bounds__(kQueryBlock) query_kernel(QueryArgs A) {
     }
     int out = wave_append(want, A.count);
     if (out >= 0 && out < A.capacity) {
-      A.rec[out] = rec;
+      if (A.rec) A.rec[out] = rec;
       if (A.ids) A.ids[out] = make_int2(i, j);
     }
     if (!__any(sp > 0)) break;
@@ -582,6 +596,7 @@ hipError_t run_query(const QueryLaunch& q, hipStream_t s) {
     case Q_DCD_PT: hipLaunchKernelGGL(query_kernel<Q_DCD_PT>, grid, block, 0, s, A); break;
     case Q_CCD_EE: hipLaunchKernelGGL(query_kernel<Q_CCD_EE>, grid, block, 0, s, A); break;
     case Q_CCD_PT: hipLaunchKernelGGL(query_kernel<Q_CCD_PT>, grid, block, 0, s, A); break;
+    case Q_ISECT: hipLaunchKernelGGL(query_kernel<Q_ISECT>, grid, block, 0, s, A); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -49,7 +49,7 @@ struct Bvh {
 constexpr int kResortEvery = 16;
 
 enum BoxMode { BOX_EDGE = 0, BOX_TRI = 1, BOX_SWEPT_EDGE = 2, BOX_SWEPT_TRI = 3 };
-enum QueryMode { Q_DCD_EE = 0, Q_DCD_PT = 1, Q_CCD_EE = 2, Q_CCD_PT = 3 };
+enum QueryMode { Q_DCD_EE = 0, Q_DCD_PT = 1, Q_CCD_EE = 2, Q_CCD_PT = 3, Q_ISECT = 4 };
 
 hipError_t build_bvh(Bvh& B, int mode, int n, const int* prims, const int* body, const float* x0, const float* x1,
                      float pad, hipStream_t s);

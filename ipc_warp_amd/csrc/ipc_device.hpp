@@ -256,6 +256,24 @@ IPCD_FN FeatureGeom feature_geom(const Feature& f, const V3<double> x[4]) {
   return g;
 }
 
+// Whether segment pq crosses triangle abc, by signed volumes in FP64 of the
+// fp32 inputs: p and q strictly on opposite sides of the plane or one of them
+// on it, and the line pq inside the closed triangle (the three volumes
+// det[q - p, a - p, b - p], det[q - p, b - p, c - p], det[q - p, c - p, a - p]
+// do not take both signs). A segment lying in the triangle's plane, or a
+// degenerate triangle, does not count: that is contact, not a crossing.
+IPCD_FN double orient3(V3<double> a, V3<double> b, V3<double> c, V3<double> d) {
+  return dot(cross(b - a, c - a), d - a);
+}
+IPCD_FN bool segment_hits_triangle(V3<double> p, V3<double> q, V3<double> a, V3<double> b, V3<double> c) {
+  double sp = orient3(a, b, c, p), sq = orient3(a, b, c, q);
+  if ((sp > 0.0 && sq > 0.0) || (sp < 0.0 && sq < 0.0) || (sp == 0.0 && sq == 0.0)) return false;
+  double u = orient3(p, q, a, b), v = orient3(p, q, b, c), w = orient3(p, q, c, a);
+  if (u == 0.0 && v == 0.0 && w == 0.0) return false;
+  bool neg = u < 0.0 || v < 0.0 || w < 0.0, pos = u > 0.0 || v > 0.0 || w > 0.0;
+  return !(neg && pos);
+}
+
 // 1 / sqrt(x): the device's reciprocal square root (the host pass, used
 // only to check these functions on a CPU, divides).
 IPCD_FN double inv_sqrt(double x) {

@@ -81,13 +81,13 @@ def main(argv=None) -> int:
             ws.save(args.out)
     evals = {}
     for seed in args.eval_seeds:
-        def log(mode, s, st, toi, seed=seed):
+        def log(mode, s, st, n_cross, seed=seed):
             print(f"eval seed {seed} {mode} step {s}: iters {st.iterations} conv {st.converged} "
-                  f"accepted {st.warm_start_accepted} toi {toi:.3f}", flush=True)
+                  f"accepted {st.warm_start_accepted} crossings {n_cross}", flush=True)
 
         r = warmstart.compare_newton(seed, args.eval_steps, ws, dev, n=args.cells, log=log)
         evals[seed] = {m: {"mean_iterations": r[m]["mean_iterations"], "all_converged": all(r[m]["converged"]),
-                           "min_motion_toi": min(r[m]["motion_toi"]), "iterations": r[m]["iterations"]}
+                           "max_intersections": max(r[m]["intersections"]), "iterations": r[m]["iterations"]}
                        for m in r}
     summary["eval"] = evals
     print(json.dumps(summary), flush=True)

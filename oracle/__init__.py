@@ -74,6 +74,8 @@ def lib():
         L.oracle_ccd_candidates_pt.argtypes = [F32P, F32P, c_int, I32P, c_int, I32P, c_flt, c_int, I32P, I32P, c_i64]
         L.oracle_ccd_candidates_pt.restype = c_i64
         L.oracle_max_threads.restype = c_int
+        L.oracle_count_intersections.argtypes = [F32P, I32P, c_int, I32P, c_int, I32P, c_i64]
+        L.oracle_count_intersections.restype = c_i64
         L.oracle_warmstart_features.argtypes = [F32P, c_int, F32P, I32P, c_int, I32P, c_int, c_flt, F32P]
         _lib = L
     return _lib
@@ -303,3 +305,18 @@ def warmstart_features(x, base, ee, pt, dhat):
     lib().oracle_warmstart_features(_p(x, F32P), V, None if base is None else _p(base, F32P), _p(ee, I32P), len(ee),
                                     _p(pt, I32P), len(pt), float(dhat), _p(out, F32P))
     return out
+
+
+def count_intersections(x, edges, triangles):
+    """(count, sorted (edge, triangle) ids) of the crossing edge-triangle pairs."""
+    x = _f32(x).reshape(-1, 3)
+    edges = _i32(edges).reshape(-1)
+    tris = _i32(triangles).reshape(-1)
+    cap = 1 << 10
+    while True:
+        ids = np.zeros((cap, 2), np.int32)
+        n = lib().oracle_count_intersections(_p(x, F32P), _p(edges, I32P), edges.size // 2, _p(tris, I32P),
+                                             tris.size // 3, _p(ids, I32P), cap)
+        if n <= cap:
+            return int(n), ids[:n].copy()
+        cap = 1 << int(n - 1).bit_length()

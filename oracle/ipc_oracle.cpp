@@ -403,6 +403,56 @@ int64_t oracle_ccd_candidates_pt(const float* x0, const float* x1, int n_nodes, 
   return out.n;
 }
 
+// Edge-triangle crossings (layout of ipc_count_intersections; no reference
+// counterpart). Written independently of the GPU's signed-volume test:
+// Moller-Trumbore ray-triangle intersection in double, the segment
+// parameter in [0, 1] and the barycentrics in the closed triangle; a segment
+// parallel to the triangle's plane (|det| <= 1e-300) or sharing a vertex
+// does not count. Pairs come from the hash-grid box enumeration. Writes the
+// (edge, triangle) ids, sorted, up to cap; returns the count.
+int64_t oracle_count_intersections(const float* x, const int* edges, int n_edges, const int* tris, int n_tris,
+                                   int* ids, int64_t cap) {
+  std::vector<Box> eb(n_edges), tb(n_tris);
+  for (int i = 0; i < n_edges; ++i) {
+    V3<float> p[2] = {ld(x, edges[2 * i]), ld(x, edges[2 * i + 1])};
+    eb[i] = box_of(p, 2, 0.0f, i);
+  }
+  for (int j = 0; j < n_tris; ++j) {
+    V3<float> p[3] = {ld(x, tris[3 * j]), ld(x, tris[3 * j + 1]), ld(x, tris[3 * j + 2])};
+    tb[j] = box_of(p, 3, 0.0f, j);
+  }
+  std::vector<std::pair<int, int>> hits;
+  grid_pairs(eb, tb, false, [&](int a, int b) { hits.emplace_back(a, b); });
+  std::sort(hits.begin(), hits.end());
+  int64_t n = 0;
+  for (auto& h : hits) {
+    int i = h.first, j = h.second;
+    int a0 = edges[2 * i], a1 = edges[2 * i + 1];
+    int t0 = tris[3 * j], t1 = tris[3 * j + 1], t2 = tris[3 * j + 2];
+    if (a0 == t0 || a0 == t1 || a0 == t2 || a1 == t0 || a1 == t1 || a1 == t2) continue;
+    V3<double> o = load3<double>(x, a0), d = load3<double>(x, a1) - o;
+    V3<double> v0 = load3<double>(x, t0), e1 = load3<double>(x, t1) - v0, e2 = load3<double>(x, t2) - v0;
+    V3<double> pv = cross(d, e2);
+    double det = dot(e1, pv);
+    if (std::fabs(det) <= 1.0e-300) continue;
+    double inv = 1.0 / det;
+    V3<double> tv = o - v0;
+    double u = dot(tv, pv) * inv;
+    if (u < 0.0 || u > 1.0) continue;
+    V3<double> qv = cross(tv, e1);
+    double v = dot(d, qv) * inv;
+    if (v < 0.0 || u + v > 1.0) continue;
+    double t = dot(e2, qv) * inv;
+    if (t < 0.0 || t > 1.0) continue;
+    if (n < cap) {
+      ids[2 * n] = i;
+      ids[2 * n + 1] = j;
+    }
+    ++n;
+  }
+  return n;
+}
+
 // Warm-start features (layout of ipc_warmstart_features; no reference
 // counterpart). Written independently of the GPU's atomic insertion: every
 // (record, vertex) pair is listed, the list is sorted by (vertex, fp32

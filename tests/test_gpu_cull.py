@@ -192,3 +192,20 @@ def test_new_intersections_exclude_same_body(dev, oracle):
         body = soup.body
         q = S.contacts_new.list[:S.n_contacts].cpu().numpy()
         assert (body[q[:, 0]] != body[q[:, 2]]).all()
+
+
+@pytest.mark.parametrize("shift", [0.0, -0.3])
+def test_count_intersections_matches_oracle(dev, oracle, shift):
+    """Edge-triangle crossing count (the Newton harness's penetration check)."""
+    soup = scenes.two_cubes_on_ground(seed=3, n=4, gap=2e-3, jitter=1e-4)
+    x = soup.x.copy()
+    x[soup.body == 1, 1] += np.float32(shift)
+    x[soup.body == 1, 0] += np.float32(0.137 if shift else 0.0)
+    S = _solver(dev, soup)
+    n, ids = S.count_intersections(torch.from_numpy(x).to(dev), capacity=1 << 14)
+    r_n, r_ids = oracle.count_intersections(x, soup.edges, soup.triangles)
+    assert n == r_n
+    assert (n > 0) == (shift != 0.0)
+    g = ids.cpu().numpy()
+    g = g[np.lexsort((g[:, 1], g[:, 0]))]
+    assert np.array_equal(g, r_ids)
