@@ -109,6 +109,44 @@ def measured_traffic(kernel: str, stream: bool = False):
     return pick[0]["fetch_bytes"] + pick[0]["write_bytes"]
 
 
+WARMSTART_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ipc_warp_amd", "data",
+                               "warmstart_cube_drop.npz")
+
+
+def newton_comparison(seeds, steps, dev):
+    """Newton iterations per time step on held-out seeded cube drops (two 8^3-cell
+    Neo-Hookean cubes over a fixed ground, implicit Euler h = 1e-2, kappa = 1e9,
+    dhat = 1e-3, 1e-6 relative energy tolerance) from the previous state
+    (plain, the reference's start), from the constant-acceleration
+    extrapolation, and from the learned warm start (committed network)."""
+    from ipc_warp_amd import warmstart
+
+    if not os.path.exists(WARMSTART_MODEL):
+        return {"error": f"no trained model at {os.path.relpath(WARMSTART_MODEL)}"}
+    ws = warmstart.WarmStart.load(WARMSTART_MODEL, dev)
+    its = {"plain": [], "extrapolation": [], "warm_start": []}
+    conv, cross, learned = True, 0, []
+    for seed in seeds:
+        r = warmstart.compare_newton(seed, steps, ws, dev)
+        for m in its:
+            its[m] += r[m]["iterations"]
+            conv = conv and all(r[m]["converged"])
+            cross = max(cross, max(r[m]["intersections"]))
+        learned += [c == 0 for c in r["warm_start"]["choice"]]
+    mean = {m: float(np.mean(v)) for m, v in its.items()}
+    return {
+        "workload": "cube drop: two 8^3-cell Neo-Hookean tet cubes (E=1e5, nu=0.4, rho=1000) over a fixed ground, "
+                    "h=1e-2, kappa=1e9, dhat=1e-3, Newton to 1e-6 relative energy decrease",
+        "seeds": list(seeds),
+        "steps_per_seed": steps,
+        "iters_per_step": mean,
+        "warm_start_vs_plain": mean["warm_start"] / mean["plain"],
+        "all_converged": conv,
+        "max_edge_triangle_crossings": cross,
+        "learned_candidate_share": float(np.mean(learned)) if learned else 0.0,
+    }
+
+
 def fixed_path_step(soup, dx_np, steps, warmup, world_counts, dev):
     """ms per step of detect_collision + Hessian + energy + collision_free_step
     through FixedBodyContactSolver on the same state and motion; checks that
@@ -177,6 +215,9 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0 = all host cores, max 16)")
+    ap.add_argument("--newton-seeds", type=int, nargs="*", default=[100, 101])
+    ap.add_argument("--newton-steps", type=int, default=20)
+    ap.add_argument("--no-newton", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -200,6 +241,8 @@ def main() -> int:
                           torch.from_numpy(soup.body), thickness=THICKNESS, buffer_=THICKNESS, kappa=KAPPA, h=H,
                           disable_self_collision_=True)
     b = torch.zeros_like(x)
+    e_dev = torch.zeros(1, dtype=torch.float64, device=dev)
+    from ipc_warp_amd import ops
 
     counts = {}
     # Single-kernel entry points are bracketed with events on torch's current
@@ -223,10 +266,15 @@ def main() -> int:
         S.contact_hessian_pt(trip, b)
         if timed:
             kev["hessian_pt"][1].record()
+        # the energy kernels alone are bracketed; the FP64 sum is read after
+        e_dev.zero_()
+        if timed:
             kev["energy"][0].record()
-        e = S.collision_energy_raw()
+        ops.contact_energy_ee(S.x, S.xcs, S.contacts_new.list, n_ee, S.l0, S.contact_stiffness, e_dev)
+        ops.contact_energy_pt(S.x, S.contacts_pt.list, n_pt, S.l0, S.contact_stiffness, e_dev)
         if timed:
             kev["energy"][1].record()
+        e = float(e_dev.item())
         toi = S.raw_toi(dx)
         if timed:
             torch.cuda.synchronize()
@@ -376,6 +424,8 @@ def main() -> int:
         "hbm_copy_gbs": copy_gbs,
         "fixed_body_bvh_step": fixed,
     }
+    if not args.no_newton:
+        result["newton"] = newton_comparison(args.newton_seeds, args.newton_steps, dev)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         result["cpu_baseline"] = cpu_baseline(soup, dx_np, threads)

@@ -41,6 +41,7 @@ class StepStats:
     max_dx_over_h: float = 0.0
     warm_start_accepted: bool = False
     warm_start_alpha: float = 0.0
+    warm_start_choice: int = -1
 
 
 class NeoHookeanTets:
@@ -267,9 +268,11 @@ class ContactIntegrator:
     def step(self, initial=None, warm_start=None) -> StepStats:
         """One time step. The first Newton iterate is the previous state, unless
         `initial` (float64 [V,3]) lowers the incremental potential, or
-        `warm_start(self)` returns a displacement (float64 [V,3]) along which a
-        CCD-bounded backtracking search from the previous state finds a lower
-        potential. Neither counts as a Newton iteration (no linear solve)."""
+        `warm_start(self)` returns a displacement (float64 [V,3]), or a list of
+        candidate displacements, along which a CCD-bounded backtracking search
+        from the previous state finds a lower potential (the lowest over the
+        candidates is kept). Neither counts as a Newton iteration (no linear
+        solve)."""
         xt = self.x_tilde()
         stats = StepStats()
         self._set(self.x0.clone())
@@ -282,11 +285,20 @@ class ContactIntegrator:
             else:
                 self._set(self.x0.clone())
         elif warm_start is not None:
-            delta = warm_start(self)
+            cands = warm_start(self)
+            if isinstance(cands, torch.Tensor):
+                cands = [cands]
             E_plain = self.energy(self.x0, xt)
-            alpha, _ = self.line_search(-delta, xt, E_plain, max_halvings=8)
-            stats.warm_start_accepted = alpha > 0.0
-            stats.warm_start_alpha = alpha
+            best_E, best_x, best_alpha, best_k = E_plain, None, 0.0, -1
+            for k, delta in enumerate(cands):
+                self._set(self.x0.clone())
+                alpha, E1 = self.line_search(-delta, xt, E_plain, max_halvings=8)
+                if alpha > 0.0 and E1 < best_E:
+                    best_E, best_x, best_alpha, best_k = E1, self.xd.clone(), alpha, k
+            self._set(self.x0.clone() if best_x is None else best_x)
+            stats.warm_start_accepted = best_x is not None
+            stats.warm_start_alpha = best_alpha
+            stats.warm_start_choice = best_k
         self.newton(xt, stats)
         self.v_prev = self.v
         self.v = (self.xd - self.x0) / self.h

@@ -240,25 +240,59 @@ def truncate_locally(sim, delta: torch.Tensor, rounds: int = 3, factor: float = 
     return delta
 
 
+def neighbour_average(sim, u: torch.Tensor, rounds: int) -> torch.Tensor:
+    """`rounds` passes of averaging u over each vertex and its tet neighbours
+    (removes vertex-to-vertex noise from a per-vertex prediction)."""
+    if rounds <= 0:
+        return u
+    A = getattr(sim, "_vertex_average", None)
+    if A is None:
+        T = sim.tets.long()
+        i = T[:, :, None].expand(-1, 4, 4).reshape(-1)
+        j = T[:, None, :].expand(-1, 4, 4).reshape(-1)
+        n = sim.V
+        ones = torch.ones(i.numel(), dtype=torch.float64, device=u.device)
+        A = torch.sparse_coo_tensor(torch.stack([i, j]), ones, (n, n)).coalesce()
+        idx = A.indices()
+        val = torch.ones_like(A.values())  # each neighbour once
+        deg = torch.zeros(n, dtype=torch.float64, device=u.device).index_add_(0, idx[0], val)
+        A = torch.sparse_coo_tensor(idx, val / deg[idx[0]], (n, n)).coalesce()
+        sim._vertex_average = A
+    for _ in range(rounds):
+        u = torch.sparse.mm(A, u) if u.dim() == 2 else u
+    return u
+
+
 class WarmStart:
     """Callable for ContactIntegrator.step(warm_start=...): the displacement
     extrapolation(sim) + MLP(features) (float64 [V, 3], zero on fixed
     vertices), optionally truncated per vertex (truncate_locally with
     `local_factor`) before the integrator's CCD-bounded line search. With
-    mlp = None it is the extrapolation alone."""
+    mlp = None it is the extrapolation alone. With `gate`, the extrapolation
+    alone is offered as a second candidate and the integrator keeps whichever
+    reaches the lower incremental potential; `smooth` averages the network's
+    correction over tet neighbourhoods that many times."""
 
-    def __init__(self, mlp: MLP | None, local_factor: float | None = None):
+    def __init__(self, mlp: MLP | None, local_factor: float | None = None, gate: bool = True, smooth: int = 2,
+                 scale: float = 1.0):
         self.mlp = mlp
         self.local_factor = local_factor
+        self.gate = gate
+        self.smooth = smooth
+        self.scale = scale
 
-    def __call__(self, sim) -> torch.Tensor:
-        delta = extrapolation(sim)
+    def __call__(self, sim):
+        base = extrapolation(sim)
+        delta = base
         if self.mlp is not None:
-            y = self.mlp.predict(step_features(sim))
-            delta = delta + y.double()
+            y = self.mlp.predict(step_features(sim)).double()
+            y = neighbour_average(sim, y, self.smooth) * self.scale
+            delta = base + y
             delta[sim.fixed] = 0.0
         if self.local_factor is not None:
             delta = truncate_locally(sim, delta, factor=self.local_factor)
+        if self.gate and self.mlp is not None:
+            return [delta, base]
         return delta
 
     def save(self, path: str):
@@ -336,13 +370,15 @@ def compare_newton(seed: int, steps: int, warm: WarmStart, device, n: int = 8, l
     out = {}
     for mode in modes:
         sim = drop_simulation(seed, device, n=n)
-        rec = {"iterations": [], "converged": [], "accepted": [], "intersections": [], "max_dx_over_h": []}
+        rec = {"iterations": [], "converged": [], "accepted": [], "choice": [], "intersections": [],
+               "max_dx_over_h": []}
         for s in range(steps):
             st = sim.step(warm_start=starts[mode])
             n_cross, _ = sim.contact.count_intersections(sim.x0.float().contiguous())
             rec["iterations"].append(st.iterations)
             rec["converged"].append(bool(st.converged))
             rec["accepted"].append(bool(st.warm_start_accepted))
+            rec["choice"].append(int(st.warm_start_choice))
             rec["intersections"].append(n_cross)
             rec["max_dx_over_h"].append(st.max_dx_over_h)
             if log:
