@@ -125,13 +125,14 @@ def newton_comparison(seeds, steps, dev):
         return {"error": f"no trained model at {os.path.relpath(WARMSTART_MODEL)}"}
     ws = warmstart.WarmStart.load(WARMSTART_MODEL, dev)
     its = {"plain": [], "extrapolation": [], "warm_start": []}
-    conv, cross, learned = True, 0, []
+    conv, cross, learned, rejected = True, 0, [], 0
     for seed in seeds:
         r = warmstart.compare_newton(seed, steps, ws, dev)
         for m in its:
             its[m] += r[m]["iterations"]
             conv = conv and all(r[m]["converged"])
             cross = max(cross, max(r[m]["intersections"]))
+            rejected += r[m]["crossing_rejections"]
         learned += [c == 0 for c in r["warm_start"]["choice"]]
     mean = {m: float(np.mean(v)) for m, v in its.items()}
     return {
@@ -143,6 +144,7 @@ def newton_comparison(seeds, steps, dev):
         "warm_start_vs_plain": mean["warm_start"] / mean["plain"],
         "all_converged": conv,
         "max_edge_triangle_crossings": cross,
+        "line_search_moves_rejected_for_crossing": rejected,
         "learned_candidate_share": float(np.mean(learned)) if learned else 0.0,
     }
 

@@ -10,7 +10,8 @@ counts with and without the learned warm start can be compared:
 * every Newton iteration: DCD + barrier gradient/PSD Hessian (HIP library),
   elastic gradient/PSD Hessian, a Jacobi-preconditioned CG solve, a CCD
   upper bound (HIP library, 0.9 toi) and backtracking until the energy
-  decreases (at most 64 halvings, alpha = 0 ends the solve);
+  decreases and the end state has no edge-triangle crossing (at most 64
+  halvings, alpha = 0 ends the solve);
 * convergence: the relative decrease of E between iterations falls below
   `tol` (1e-6). The reference's own test max|dx| <= tol_v h is reported too.
 
@@ -151,6 +152,8 @@ class ContactIntegrator:
         self._el_rows = ri.reshape(-1)
         self._el_cols = ci.reshape(-1)
         self.frame = 0
+        self.check_crossings = True
+        self.crossing_rejections = 0
 
     # ------------------------------------------------------------------ energy ----
     def x_tilde(self):
@@ -238,7 +241,14 @@ class ContactIntegrator:
         for _ in range(max_halvings):
             E1 = self.energy(x_start - alpha * dx, xt)
             if math.isfinite(E1) and E1 < E0:
-                return alpha, E1
+                # The zero-thickness CCD certifies the straight path, but a
+                # pair whose gap has collapsed below the fp32 resolution of
+                # the positions can still end up crossed once the iterate is
+                # rounded; such a move is rejected (exact crossing count).
+                if self.check_crossings and self.contact.count_intersections()[0] > 0:
+                    self.crossing_rejections += 1
+                else:
+                    return alpha, E1
             alpha *= 0.5
             if alpha <= 1.1920928955078125e-07 * max(1.0, upper):
                 break

@@ -384,5 +384,6 @@ def compare_newton(seed: int, steps: int, warm: WarmStart, device, n: int = 8, l
             if log:
                 log(mode, s, st, n_cross)
         rec["mean_iterations"] = float(np.mean(rec["iterations"]))
+        rec["crossing_rejections"] = int(sim.crossing_rejections)
         out[mode] = rec
     return out
