@@ -338,7 +338,14 @@ __device__ __forceinline__ bool accept_pair(const QueryArgs& A, int i, int j, in
 template <int MODE>
 __global__ void __launch_bounds__(kQueryBlock) query_kernel(QueryArgs A) {
   __shared__ unsigned long long stack[kGroupsPerBlock][kMaskStack];
+  // Accepted pairs are staged per block and written out with one global
+  // atomicAdd per block (one counter shared by every wave serialises).
+  __shared__ int4 out_rec[kBlockOut];
+  __shared__ int2 out_ids[kBlockOut];
+  __shared__ int out_n, out_base;
   int tid = threadIdx.x;
+  if (tid == 0) out_n = 0;
+  __syncthreads();
   int grp = tid / kFan, r = tid % kFan;
   int lane = __lane_id();
   int group_shift = lane & ~(kFan - 1);  // first lane of this group in the wave
@@ -422,12 +429,31 @@ __global__ void __launch_bounds__(kQueryBlock) query_kernel(QueryArgs A) {
       stack[grp][sp] = mask_entry(L - 1, gmask, node * kFan);
       ++sp;
     }
-    int out = wave_append(want, A.count);
-    if (out >= 0 && out < A.capacity) {
-      if (A.rec) A.rec[out] = rec;
-      if (A.ids) A.ids[out] = make_int2(i, j);
+    int slot_out = wave_append(want, &out_n);  // LDS counter
+    if (want) {
+      if (slot_out < kBlockOut) {
+        out_rec[slot_out] = rec;
+        out_ids[slot_out] = make_int2(i, j);
+      } else {
+        int out = atomicAdd(A.count, 1);  // staging full: straight to the output
+        if (out < A.capacity) {
+          if (A.rec) A.rec[out] = rec;
+          if (A.ids) A.ids[out] = make_int2(i, j);
+        }
+      }
     }
     if (!__any(sp > 0)) break;
+  }
+  __syncthreads();
+  int staged = out_n < kBlockOut ? out_n : kBlockOut;
+  if (tid == 0) out_base = staged ? atomicAdd(A.count, staged) : 0;
+  __syncthreads();
+  for (int k = tid; k < staged; k += kQueryBlock) {
+    int out = out_base + k;
+    if (out < A.capacity) {
+      if (A.rec) A.rec[out] = out_rec[k];
+      if (A.ids) A.ids[out] = out_ids[k];
+    }
   }
 }
 

@@ -10,6 +10,7 @@ namespace ipcd {
 constexpr int kQueryBlock = 256;
 constexpr int kGroupsPerBlock = kQueryBlock / kFan;  // one query per 8 lanes
 constexpr int kMaskStack = 12;                       // > kMaxLevels - 1 entries per query
+constexpr int kBlockOut = 512;                       // accepted pairs a query block stages in LDS
 
 // Inclusive box test (a lower bound above the other's upper bound separates).
 __device__ __forceinline__ bool overlaps(const float qlo[3], const float qhi[3], float4 lo, float4 hi) {
