@@ -187,8 +187,8 @@ int ipc_detect_collision_ee(ipc_ctx* c, const float* x, float thickness, float b
   q.edges = c->edges.as<int>();
   q.body = c->body.as<int>();
   q.x0 = x;
-  q.qlo = c->edge_bvh.leaf_lo.as<float4>();
-  q.qhi = c->edge_bvh.leaf_hi.as<float4>();
+  q.qlo = c->edge_bvh.node_lo.as<float4>();  // leaves are level 0, from slot 0
+  q.qhi = c->edge_bvh.node_hi.as<float4>();
   q.thickness = thickness;
   q.exclude_same_body = exclude_same_body;
   q.bvh = &c->edge_bvh;
@@ -297,8 +297,8 @@ int ipc_ccd_candidates_ee(ipc_ctx* c, const float* x0, const float* x1, float pa
   q.body = c->body.as<int>();
   q.x0 = x0;
   q.x1 = x1;
-  q.qlo = c->edge_bvh.leaf_lo.as<float4>();
-  q.qhi = c->edge_bvh.leaf_hi.as<float4>();
+  q.qlo = c->edge_bvh.node_lo.as<float4>();  // leaves are level 0, from slot 0
+  q.qhi = c->edge_bvh.node_hi.as<float4>();
   q.exclude_same_body = exclude_same_body;
   q.bvh = &c->edge_bvh;
   q.rec = reinterpret_cast<int4*>(quads);

@@ -13,16 +13,17 @@
 //      (in between, the leaf order is kept and only the boxes are refreshed),
 //   3. an implicit 8-ary hierarchy over the sorted leaves: node k of level L
 //      bounds nodes 8k .. 8k+7 of level L-1 (level 0 = leaves). Levels 1-3
-//      are reduced by one kernel in LDS (a 64-lane workgroup owns 512 leaves),
-//      the levels above by a single workgroup. There is no bottom-up
+//      are reduced by one kernel, one thread per leaf (a 512-thread
+//      workgroup owns 512 leaves: lane shuffles for levels 1 and 2, LDS for
+//      level 3), the levels above by a single workgroup. There is no bottom-up
 //      propagation across workgroups, hence no device-scope fence: on a
 //      multi-XCD part such a fence writes back and invalidates the L2.
 // Nodes also carry the (min, max) body id below them, so a query that
 // excludes its own body skips single-body subtrees. Queries run eight lanes
 // per query: visiting a node, lane r tests the node's child r, and the
 // group's ballot of overlapping children becomes one stack entry (level,
-// first child, 8-bit mask) in LDS. Accepted pairs are appended with one
-// atomicAdd per wave per emission round (ballot + prefix count).
+// first child, 8-bit mask) in LDS. Accepted pairs are staged in LDS and
+// each workgroup reserves its output range with one global atomicAdd.
 #include <hipcub/hipcub.hpp>
 
 #include "cull.hpp"
@@ -143,9 +144,8 @@ __global__ void __launch_bounds__(256) morton_kernel(int mode, int n, const int*
 }
 
 // ------------------------------------------------------------ hierarchy ----
-constexpr int kReduceBlock = 64;                      // threads of the leaf/level kernel (one wave)
-constexpr int kLeavesPerBlock = kFan * kReduceBlock;  // 512 = 8^3 leaves per workgroup
-constexpr int kInBlockLevels = 3;                     // levels 1..3 reduced inside the workgroup
+constexpr int kLeafBlock = kFan * kFan * kFan;  // 512 leaves (and threads) per workgroup of the leaf kernel
+constexpr int kInBlockLevels = 3;               // levels 1..3 reduced inside that workgroup
 
 __device__ __forceinline__ void box_merge(float4& lo, float4& hi, float4 l, float4 h) {
   lo = make_float4(fminf(lo.x, l.x), fminf(lo.y, l.y), fminf(lo.z, l.z), 0.f);
@@ -157,90 +157,117 @@ __device__ __forceinline__ void box_empty(float4& lo, float4& hi) {
   hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
 }
 
-// Leaves in Morton order and levels 1..3 of the hierarchy. Workgroup b owns
-// leaves [512 b, 512 (b + 1)); each leaf's box is computed here from the
-// topology and positions of primitive keys[k] (low 32 bits).
-__global__ void __launch_bounds__(kReduceBlock) leaf_levels_kernel(
+// Merge a box and body range with the lane `off` away (xor), so that after
+// offsets 1, 2, 4 every lane of an aligned group of 8 holds the group's merge.
+__device__ __forceinline__ void lane_merge(float4& lo, float4& hi, int2& br, int off) {
+  float4 l = make_float4(__shfl_xor(lo.x, off, 64), __shfl_xor(lo.y, off, 64), __shfl_xor(lo.z, off, 64), 0.f);
+  float4 h = make_float4(__shfl_xor(hi.x, off, 64), __shfl_xor(hi.y, off, 64), __shfl_xor(hi.z, off, 64), 0.f);
+  box_merge(lo, hi, l, h);
+  br.x = min(br.x, __shfl_xor(br.x, off, 64));
+  br.y = max(br.y, __shfl_xor(br.y, off, 64));
+}
+
+// Leaves in Morton order and levels 1..3 of the hierarchy, one thread per
+// leaf slot. Workgroup b owns leaf slots [512 b, 512 (b + 1)): level-1 nodes
+// are merged across 8-lane groups, level-2 nodes across a wave, the level-3
+// node across the workgroup's 8 waves in LDS. Leaf k's box is computed from
+// the topology and positions of primitive keys[k] (low 32 bits); slots past
+// n hold empty boxes. leaf_vid[k] holds the leaf's vertex ids (v2 = -1 for
+// an edge) and primitive id, so that a query reads one record per leaf
+// instead of chasing the topology.
+__global__ void __launch_bounds__(kLeafBlock) leaf_levels_kernel(
     int mode, int n, const int* __restrict__ prims, const int* __restrict__ body, const float* __restrict__ x0,
-    const float* __restrict__ x1, float pad, const unsigned long long* __restrict__ keys,
-    float4* __restrict__ leaf_lo, float4* __restrict__ leaf_hi, int* __restrict__ leaf_id, int* __restrict__ leaf_body,
-    float4* __restrict__ node_lo, float4* __restrict__ node_hi, int2* __restrict__ node_body, LevelInfo info) {
-  __shared__ float4 slo[kReduceBlock], shi[kReduceBlock];
-  __shared__ int2 sbody[kReduceBlock];
-  int t = threadIdx.x;
-  int g = blockIdx.x * kReduceBlock + t;  // level-1 node
-  int stride = (mode == 0 || mode == 2) ? 2 : 3;
-  float4 blo, bhi;
-  box_empty(blo, bhi);
+    const float* __restrict__ x1, float pad, const unsigned long long* __restrict__ keys, int* __restrict__ leaf_id,
+    int4* __restrict__ leaf_vid, float4* __restrict__ node_lo, float4* __restrict__ node_hi,
+    int2* __restrict__ node_body, LevelInfo info) {
+  __shared__ float4 wlo[kLeafBlock / 64], whi[kLeafBlock / 64];
+  __shared__ int2 wbody[kLeafBlock / 64];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int k = blockIdx.x * kLeafBlock + t;
+  const int stride = (mode == 0 || mode == 2) ? 2 : 3;
+  float4 lo, hi;
+  box_empty(lo, hi);
   int2 br = make_int2(0x7fffffff, (int)0x80000000);
-#pragma unroll
-  for (int c = 0; c < kFan; ++c) {
-    int k = g * kFan + c;
-    if (k < n) {
-      int prim = static_cast<int>(keys[k] & 0xffffffffu);
-      float4 l, h;
-      prim_box(mode, prim, prims, x0, x1, pad, l, h);
-      int bd = body[prims[stride * prim]];
-      leaf_lo[k] = l;
-      leaf_hi[k] = h;
-      leaf_id[k] = prim;
-      leaf_body[k] = bd;
-      box_merge(blo, bhi, l, h);
-      br.x = min(br.x, bd);
-      br.y = max(br.y, bd);
-    }
+  int4 vid = make_int4(-1, -1, -1, -1);
+  if (k < n) {
+    int prim = static_cast<int>(keys[k] & 0xffffffffu);
+    prim_box(mode, prim, prims, x0, x1, pad, lo, hi);
+    int v0 = prims[stride * prim], v1 = prims[stride * prim + 1];
+    int v2 = stride == 3 ? prims[stride * prim + 2] : -1;
+    int bd = body[v0];
+    leaf_id[k] = prim;
+    vid = make_int4(v0, v1, v2, prim);
+    br = make_int2(bd, bd);
   }
-  if (g < info.count[1]) {
-    node_lo[info.offset[1] + g] = blo;
-    node_hi[info.offset[1] + g] = bhi;
-    node_body[info.offset[1] + g] = br;
+  if (k < info.slots[0]) {  // offset[0] = 0
+    node_lo[k] = lo;
+    node_hi[k] = hi;
+    node_body[k] = br;
+    leaf_vid[k] = vid;
   }
-  slo[t] = blo;
-  shi[t] = bhi;
-  sbody[t] = br;
-  int width = kReduceBlock;
-  for (int L = 2; L <= kInBlockLevels; ++L) {
-    __syncthreads();
-    width /= kFan;
-    float4 l, h;
-    box_empty(l, h);
-    int2 r2 = make_int2(0x7fffffff, (int)0x80000000);
-    if (t < width) {
-#pragma unroll
-      for (int c = 0; c < kFan; ++c) {
-        box_merge(l, h, slo[t * kFan + c], shi[t * kFan + c]);
-        r2.x = min(r2.x, sbody[t * kFan + c].x);
-        r2.y = max(r2.y, sbody[t * kFan + c].y);
-      }
+  // level 1: groups of 8 lanes
+  lane_merge(lo, hi, br, 1);
+  lane_merge(lo, hi, br, 2);
+  lane_merge(lo, hi, br, 4);
+  int g1 = blockIdx.x * (kLeafBlock / kFan) + t / kFan;
+  if ((lane & (kFan - 1)) == 0 && g1 < info.slots[1]) {
+    node_lo[info.offset[1] + g1] = lo;
+    node_hi[info.offset[1] + g1] = hi;
+    node_body[info.offset[1] + g1] = br;
+  }
+  // level 2: the wave's 8 level-1 nodes
+  lane_merge(lo, hi, br, 8);
+  lane_merge(lo, hi, br, 16);
+  lane_merge(lo, hi, br, 32);
+  int g2 = blockIdx.x * (kLeafBlock / 64) + wave;
+  if (lane == 0) {
+    if (g2 < info.slots[2]) {
+      node_lo[info.offset[2] + g2] = lo;
+      node_hi[info.offset[2] + g2] = hi;
+      node_body[info.offset[2] + g2] = br;
     }
-    __syncthreads();
-    if (t < width) {
-      slo[t] = l;
-      shi[t] = h;
-      sbody[t] = r2;
-      int gl = blockIdx.x * width + t;
-      if (L <= info.top && gl < info.count[L]) {
-        node_lo[info.offset[L] + gl] = l;
-        node_hi[info.offset[L] + gl] = h;
-        node_body[info.offset[L] + gl] = r2;
-      }
+    wlo[wave] = lo;
+    whi[wave] = hi;
+    wbody[wave] = br;
+  }
+  __syncthreads();
+  // level 3: the workgroup's 8 level-2 nodes
+  if (t == 0 && int(blockIdx.x) < info.slots[3]) {
+    float4 l = wlo[0], h = whi[0];
+    int2 r2 = wbody[0];
+    for (int w = 1; w < kLeafBlock / 64; ++w) {
+      box_merge(l, h, wlo[w], whi[w]);
+      r2.x = min(r2.x, wbody[w].x);
+      r2.y = max(r2.y, wbody[w].y);
     }
+    node_lo[info.offset[3] + blockIdx.x] = l;
+    node_hi[info.offset[3] + blockIdx.x] = h;
+    node_body[info.offset[3] + blockIdx.x] = r2;
   }
 }
 
-// Levels kInBlockLevels + 1 .. top in one workgroup.
+// Levels kInBlockLevels + 1 .. top in one workgroup, and the padding slots
+// of level kInBlockLevels past the leaf kernel's one node per workgroup.
 __global__ void __launch_bounds__(1024) reduce_top_kernel(float4* __restrict__ node_lo, float4* __restrict__ node_hi,
                                                          int2* __restrict__ node_body, LevelInfo info) {
+  float4 el, eh;
+  box_empty(el, eh);
+  const int2 eb = make_int2(0x7fffffff, (int)0x80000000);
+  for (int g = info.count[kInBlockLevels] + threadIdx.x; g < info.slots[kInBlockLevels]; g += blockDim.x) {
+    node_lo[info.offset[kInBlockLevels] + g] = el;
+    node_hi[info.offset[kInBlockLevels] + g] = eh;
+    node_body[info.offset[kInBlockLevels] + g] = eb;
+  }
+  __syncthreads();
   for (int L = kInBlockLevels + 1; L <= info.top; ++L) {
-    for (int g = threadIdx.x; g < info.count[L]; g += blockDim.x) {
-      float4 l, h;
-      box_empty(l, h);
-      int2 r2 = make_int2(0x7fffffff, (int)0x80000000);
-      for (int c = 0; c < kFan; ++c) {
-        int k = g * kFan + c;
-        if (k < info.count[L - 1]) {
-          box_merge(l, h, node_lo[info.offset[L - 1] + k], node_hi[info.offset[L - 1] + k]);
-          int2 cb = node_body[info.offset[L - 1] + k];
+    for (int g = threadIdx.x; g < info.slots[L]; g += blockDim.x) {
+      float4 l = el, h = eh;
+      int2 r2 = eb;
+      if (g < info.count[L]) {
+        for (int c = 0; c < kFan; ++c) {
+          int k = info.offset[L - 1] + g * kFan + c;
+          box_merge(l, h, node_lo[k], node_hi[k]);
+          int2 cb = node_body[k];
           r2.x = min(r2.x, cb.x);
           r2.y = max(r2.y, cb.y);
         }
@@ -280,16 +307,13 @@ struct QueryArgs {
   float pad;          // mode 1: 2 (th + buf); mode 3: CCD padding
   float thickness;
   int exclude_same_body;
-  // hierarchy
+  // hierarchy (every level in the node arrays, leaves from slot 0)
   int n_leaves;
   LevelInfo info;
   const float4* node_lo;
   const float4* node_hi;
   const int2* node_body;
-  const float4* leaf_lo;
-  const float4* leaf_hi;
-  const int* leaf_id;
-  const int* leaf_body;
+  const int4* leaf_vid;
   // output
   int4* rec;
   int2* ids;
@@ -297,77 +321,102 @@ struct QueryArgs {
   int* count;
 };
 
+// The query's own vertices (q0, q1; q1 unused for a point) and positions
+// at x0 (p0, p1; DCD and crossing modes) are loaded once per query. The
+// same-body test was applied to the leaf before (leaf_body is the body of
+// the leaf's first vertex, qbody that of the query's first vertex).
+struct QueryPrim {
+  int i, q0, q1;
+  V3<float> p0, p1;
+};
+
 template <int MODE>
-__device__ __forceinline__ bool accept_pair(const QueryArgs& A, int i, int j, int4& rec) {
+__device__ __forceinline__ bool accept_pair(const QueryArgs& A, const QueryPrim& Q, int4 lv, int4& rec) {
   if (MODE == Q_ISECT) {
     // edge i against triangle j: a crossing, not sharing a vertex
-    int a0 = A.edges[2 * i], a1 = A.edges[2 * i + 1];
-    int t0 = A.tris[3 * j], t1 = A.tris[3 * j + 1], t2 = A.tris[3 * j + 2];
+    int a0 = Q.q0, a1 = Q.q1, t0 = lv.x, t1 = lv.y, t2 = lv.z;
     if (a0 == t0 || a0 == t1 || a0 == t2 || a1 == t0 || a1 == t1 || a1 == t2) return false;
     rec = make_int4(a0, a1, t0, t1);
-    return segment_hits_triangle(to_d(ldx(A.x0, a0)), to_d(ldx(A.x0, a1)), to_d(ldx(A.x0, t0)), to_d(ldx(A.x0, t1)),
+    return segment_hits_triangle(to_d(Q.p0), to_d(Q.p1), to_d(ldx(A.x0, t0)), to_d(ldx(A.x0, t1)),
                                  to_d(ldx(A.x0, t2)));
   }
   if (MODE == 0 || MODE == 2) {
-    if (!(i < j)) return false;
-    int a0 = A.edges[2 * i], a1 = A.edges[2 * i + 1], b0 = A.edges[2 * j], b1 = A.edges[2 * j + 1];
+    if (!(Q.i < lv.w)) return false;
+    int a0 = Q.q0, a1 = Q.q1, b0 = lv.x, b1 = lv.y;
     if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return false;
-    if (A.exclude_same_body && A.body[a0] == A.body[b0]) return false;
     rec = make_int4(a0, a1, b0, b1);
     if (MODE == 2) return true;
-    EdgeClosest ec = closest_point_edge_edge(ldx(A.x0, a0), ldx(A.x0, a1), ldx(A.x0, b0), ldx(A.x0, b1), 1.0e-6f);
+    EdgeClosest ec = closest_point_edge_edge(Q.p0, Q.p1, ldx(A.x0, b0), ldx(A.x0, b1), 1.0e-6f);
     return ec.dist < A.thickness * 2.0f;
   }
-  int t0 = A.tris[3 * j], t1 = A.tris[3 * j + 1], t2 = A.tris[3 * j + 2];
+  int i = Q.i, t0 = lv.x, t1 = lv.y, t2 = lv.z;
   if (t0 == i || t1 == i || t2 == i) return false;
-  if (A.exclude_same_body && A.body[i] == A.body[t0]) return false;
   if (MODE == 3) {
     rec = make_int4(i, t0, t1, t2);
     return true;
   }
   rec = make_int4(i, t1, t0, t2);
-  TriClosest tc = closest_point_triangle(ldx(A.x0, i), ldx(A.x0, t0), ldx(A.x0, t1), ldx(A.x0, t2));
+  TriClosest tc = closest_point_triangle(Q.p0, ldx(A.x0, t0), ldx(A.x0, t1), ldx(A.x0, t2));
   return tc.dist < A.thickness * 2.0f;
 }
 
-
-// Eight lanes share one query. Visiting a node, lane r tests the node's
-// child r; the group ballot of overlapping children becomes one stack entry
-// (level, first child, mask). Every lane of a group computes and writes the
-// same stack values, so each lane only ever reads back its own LDS writes.
+// Eight lanes share one query. A stack entry (level L, mask, cbase) stands
+// for the level-L nodes picked by the mask bits c of one parent; node c's
+// children sit at slots cbase + 8 c + 0..7 of level L - 1 (cbase is an
+// absolute slot index into the node arrays). A step takes up to kPopNodes
+// nodes of the top entry; lane r tests child r of each, with every load of
+// the step issued up front (padded levels keep them in bounds), and the
+// group's ballot of overlapping children of a node becomes one new entry.
+// Every lane of a group computes and writes the same stack values, so each
+// lane only ever reads back its own LDS writes.
 template <int MODE>
 __global__ void __launch_bounds__(kQueryBlock) query_kernel(QueryArgs A) {
   __shared__ unsigned long long stack[kGroupsPerBlock][kMaskStack];
-  // Accepted pairs are staged per block and written out with one global
-  // atomicAdd per block (one counter shared by every wave serialises).
-  __shared__ int4 out_rec[kBlockOut];
-  __shared__ int2 out_ids[kBlockOut];
-  __shared__ int out_n, out_base;
+  __shared__ int level_off[kMaxLevels], level_cnt[kMaxLevels];
+  __shared__ BlockStage stage;
+  const StageOut out{A.rec, A.ids, A.count, A.capacity};
   int tid = threadIdx.x;
-  if (tid == 0) out_n = 0;
-  __syncthreads();
+  if (tid < kMaxLevels) {
+    level_off[tid] = A.info.offset[tid];
+    level_cnt[tid] = A.info.count[tid];
+  }
+  stage_begin(stage);
   int grp = tid / kFan, r = tid % kFan;
   int lane = __lane_id();
   int group_shift = lane & ~(kFan - 1);  // first lane of this group in the wave
   int slot = blockIdx.x * kGroupsPerBlock + grp;
   bool live = slot < A.n_query && A.n_leaves > 0;
-  int i = 0;
+  QueryPrim Q;
+  Q.i = Q.q0 = Q.q1 = 0;
+  Q.p0 = Q.p1 = mk(0.f, 0.f, 0.f);
   float qlo[3] = {0.f, 0.f, 0.f}, qhi[3] = {-1.f, -1.f, -1.f};
   int qbody = -1;
   if (live) {
-    i = A.order ? A.order[slot] : slot;
+    int i = A.order ? A.order[slot] : slot;
+    Q.i = i;
     if (MODE == Q_ISECT) {
-      V3<float> p = ldx(A.x0, A.edges[2 * i]), q = ldx(A.x0, A.edges[2 * i + 1]);
-      V3<float> lo = vmin(p, q), hi = vmax(p, q);
+      Q.q0 = A.edges[2 * i];
+      Q.q1 = A.edges[2 * i + 1];
+      Q.p0 = ldx(A.x0, Q.q0);
+      Q.p1 = ldx(A.x0, Q.q1);
+      V3<float> lo = vmin(Q.p0, Q.p1), hi = vmax(Q.p0, Q.p1);
       qlo[0] = lo.x; qlo[1] = lo.y; qlo[2] = lo.z;
       qhi[0] = hi.x; qhi[1] = hi.y; qhi[2] = hi.z;
     } else if (MODE == 0 || MODE == 2) {
       float4 l = A.qlo[slot], h = A.qhi[slot];
       qlo[0] = l.x; qlo[1] = l.y; qlo[2] = l.z;
       qhi[0] = h.x; qhi[1] = h.y; qhi[2] = h.z;
-      if (A.exclude_same_body) qbody = A.body[A.edges[2 * i]];
+      Q.q0 = A.edges[2 * i];
+      Q.q1 = A.edges[2 * i + 1];
+      if (MODE == 0) {
+        Q.p0 = ldx(A.x0, Q.q0);
+        Q.p1 = ldx(A.x0, Q.q1);
+      }
+      if (A.exclude_same_body) qbody = A.body[Q.q0];
     } else {
+      Q.q0 = i;
       V3<float> p = ldx(A.x0, i), lo = p, hi = p;
+      Q.p0 = p;
       if (MODE == 3) {
         V3<float> q = ldx(A.x1, i);
         lo = vmin(p, q);
@@ -383,78 +432,78 @@ __global__ void __launch_bounds__(kQueryBlock) query_kernel(QueryArgs A) {
   }
   int sp = 0;
   if (live) {
-    stack[grp][0] = mask_entry(A.info.top, 1u, 0);  // the root as the only candidate of its level
+    stack[grp][0] = mask_entry(A.info.top, 1u, level_off[A.info.top - 1]);  // the root
     sp = 1;
   }
   while (true) {
-    bool want = false;
-    int j = -1;
-    int4 rec;
     bool active = sp > 0;
-    int L = 0, node = 0;
+    int L = 0, cbase = 0;
+    unsigned take = 0u;
     if (active) {
       unsigned long long e = stack[grp][sp - 1];
-      L = int(e >> 40);
+      L = int(e >> 40) & 0xf;
       unsigned mask = unsigned(e >> 32) & 0xffu;
-      int first = int(e & 0xffffffffu);
-      int c = __ffs(mask) - 1;
-      node = first + c;
-      mask &= mask - 1u;
-      if (mask == 0u) --sp;
-      else stack[grp][sp - 1] = mask_entry(L, mask, first);
+      cbase = int(e & 0xffffffffu);
+      unsigned rest = mask;
+#pragma unroll
+      for (int p = 0; p < kPopNodes; ++p) rest &= rest - 1u;
+      take = mask ^ rest;  // the kPopNodes lowest nodes
+      if (rest == 0u) --sp;
+      else stack[grp][sp - 1] = mask_entry(L, rest, cbase);
     }
-    // the node's children: internal nodes of level L-1, or leaves when L == 1
-    bool overlap_child = false;
-    int child = node * kFan + r;
-    if (active && L >= 2) {
-      int lv = L - 1;
-      if (child < A.info.count[lv]) {
-        int at = A.info.offset[lv] + child;
-        overlap_child = overlaps(qlo, qhi, A.node_lo[at], A.node_hi[at]);
-        if (overlap_child && qbody >= 0) {
-          int2 br = A.node_body[at];
-          overlap_child = !(br.x == qbody && br.y == qbody);
-        }
-      }
-    } else if (active && L == 1) {
-      if (child < A.n_leaves && overlaps(qlo, qhi, A.leaf_lo[child], A.leaf_hi[child]) &&
-          !(qbody >= 0 && A.leaf_body[child] == qbody)) {
-        j = A.leaf_id[child];
-        want = accept_pair<MODE>(A, i, j, rec);
+    int cpos[kPopNodes], at[kPopNodes];
+    {
+      unsigned tk = take;
+#pragma unroll
+      for (int p = 0; p < kPopNodes; ++p) {
+        cpos[p] = tk ? __ffs(tk) - 1 : -1;
+        at[p] = tk ? cbase + cpos[p] * kFan + r : 0;
+        tk &= tk - 1u;
       }
     }
-    unsigned long long ball = __ballot(overlap_child);
-    unsigned gmask = unsigned(ball >> group_shift) & 0xffu;
-    if (active && L >= 2 && gmask != 0u && sp < kMaskStack) {
-      stack[grp][sp] = mask_entry(L - 1, gmask, node * kFan);
-      ++sp;
+    float4 clo[kPopNodes], chi[kPopNodes];
+    int2 cbr[kPopNodes];
+    int4 cvid[kPopNodes];
+#pragma unroll
+    for (int p = 0; p < kPopNodes; ++p) {
+      clo[p] = A.node_lo[at[p]];
+      chi[p] = A.node_hi[at[p]];
+      cbr[p] = A.node_body[at[p]];
+      cvid[p] = A.leaf_vid[L == 1 ? at[p] : 0];
     }
-    int slot_out = wave_append(want, &out_n);  // LDS counter
-    if (want) {
-      if (slot_out < kBlockOut) {
-        out_rec[slot_out] = rec;
-        out_ids[slot_out] = make_int2(i, j);
-      } else {
-        int out = atomicAdd(A.count, 1);  // staging full: straight to the output
-        if (out < A.capacity) {
-          if (A.rec) A.rec[out] = rec;
-          if (A.ids) A.ids[out] = make_int2(i, j);
-        }
+    // Only real children count: a query box with NaN or infinite bounds
+    // also "overlaps" the empty boxes of padding slots.
+    const bool internal = L >= 2;
+    const int child_off = level_off[L >= 1 ? L - 1 : 0], child_cnt = level_cnt[L >= 1 ? L - 1 : 0];
+    const int grand_off = level_off[internal ? L - 2 : 0];
+    // (bitwise, so that no load is sunk behind a branch on another term)
+    bool hit[kPopNodes];
+#pragma unroll
+    for (int p = 0; p < kPopNodes; ++p)
+      hit[p] = (cpos[p] >= 0) & (at[p] - child_off < child_cnt) & overlaps(qlo, qhi, clo[p], chi[p]) &
+               !((qbody >= 0) & (cbr[p].x == qbody) & (cbr[p].y == qbody));
+    // internal level: one entry per node with overlapping children, the
+    // first node's pushed last so that it is visited first
+#pragma unroll
+    for (int p = kPopNodes - 1; p >= 0; --p) {
+      unsigned gmask = unsigned(__ballot(internal && hit[p]) >> group_shift) & 0xffu;
+      if (internal && gmask != 0u && sp < kMaskStack) {
+        int rel = cbase - child_off + cpos[p] * kFan;  // first child's index within level L - 1
+        stack[grp][sp] = mask_entry(L - 1, gmask, grand_off + rel * kFan);
+        ++sp;
       }
+    }
+    // leaf level: test and stage the pairs
+#pragma unroll
+    for (int p = 0; p < kPopNodes; ++p) {
+      bool want = false;
+      int4 rec = make_int4(0, 0, 0, 0);
+      if (L == 1 && hit[p]) want = accept_pair<MODE>(A, Q, cvid[p], rec);
+      stage_append(stage, out, want, rec, make_int2(Q.i, cvid[p].w));
     }
     if (!__any(sp > 0)) break;
   }
-  __syncthreads();
-  int staged = out_n < kBlockOut ? out_n : kBlockOut;
-  if (tid == 0) out_base = staged ? atomicAdd(A.count, staged) : 0;
-  __syncthreads();
-  for (int k = tid; k < staged; k += kQueryBlock) {
-    int out = out_base + k;
-    if (out < A.capacity) {
-      if (A.rec) A.rec[out] = out_rec[k];
-      if (A.ids) A.ids[out] = out_ids[k];
-    }
-  }
+  stage_flush(stage, out);
 }
 
 // x1 = x - dx (_add_dx, contact.py:49-52)
@@ -518,8 +567,8 @@ void DevBuf::release() {
 }
 
 void Bvh::release() {
-  keys.release(); keys_sorted.release(); leaf_lo.release(); leaf_hi.release();
-  leaf_id.release(); node_lo.release(); node_hi.release(); leaf_body.release(); node_body.release();
+  keys.release(); keys_sorted.release();
+  leaf_id.release(); node_lo.release(); node_hi.release(); leaf_vid.release(); node_body.release();
   temp.release(); bounds.release();
   n = 0;
   builds_since_sort = -1;
@@ -536,27 +585,31 @@ hipError_t build_bvh(Bvh& B, int mode, int n, const int* prims, const int* body,
   B.n = n;
   if (n <= 0) return hipSuccess;
   if (n > (1 << 27)) return hipErrorInvalidValue;
-  // level sizes: count[0] = n leaves, count[L] = ceil(count[L-1] / 8) up to a single root
+  // level sizes: count[0] = n leaves, count[L] = ceil(count[L-1] / 8) up to
+  // a single root; level L occupies slots[L] = 8 count[L+1] slots from
+  // offset[L] (the root level one slot)
   LevelInfo& I = B.info;
-  for (int L = 0; L < kMaxLevels; ++L) { I.count[L] = 0; I.offset[L] = 0; }
+  for (int L = 0; L < kMaxLevels; ++L) { I.count[L] = 0; I.slots[L] = 0; I.offset[L] = 0; }
   I.count[0] = n;
-  int L = 0, total = 0;
+  int L = 0;
   do {
     ++L;
     I.count[L] = (I.count[L - 1] + kFan - 1) / kFan;
-    I.offset[L] = total;
-    total += I.count[L];
   } while (I.count[L] > 1);
   I.top = L;
+  long long total = 0;
+  for (int l = 0; l <= I.top; ++l) {
+    I.slots[l] = l < I.top ? kFan * I.count[l + 1] : 1;
+    I.offset[l] = static_cast<int>(total);
+    total += I.slots[l];
+  }
   IPC_TRY(B.bounds.ensure(sizeof(int) * 8));
   IPC_TRY(B.keys.ensure(sizeof(unsigned long long) * n));
   IPC_TRY(B.keys_sorted.ensure(sizeof(unsigned long long) * n));
-  IPC_TRY(B.leaf_lo.ensure(sizeof(float4) * n));
-  IPC_TRY(B.leaf_hi.ensure(sizeof(float4) * n));
   IPC_TRY(B.leaf_id.ensure(sizeof(int) * n));
+  IPC_TRY(B.leaf_vid.ensure(sizeof(int4) * I.slots[0]));
   IPC_TRY(B.node_lo.ensure(sizeof(float4) * total));
   IPC_TRY(B.node_hi.ensure(sizeof(float4) * total));
-  IPC_TRY(B.leaf_body.ensure(sizeof(int) * n));
   IPC_TRY(B.node_body.ensure(sizeof(int2) * total));
   bool resort = B.builds_since_sort < 0 || B.builds_since_sort >= kResortEvery || B.n_sorted != n;
   if (resort) {
@@ -578,10 +631,10 @@ hipError_t build_bvh(Bvh& B, int mode, int n, const int* prims, const int* body,
   } else {
     ++B.builds_since_sort;
   }
-  hipLaunchKernelGGL(leaf_levels_kernel, dim3(nblocks(n, kLeavesPerBlock)), dim3(kReduceBlock), 0, s, mode, n, prims,
-                     body, x0, x1, pad, B.keys_sorted.as<unsigned long long>(), B.leaf_lo.as<float4>(),
-                     B.leaf_hi.as<float4>(), B.leaf_id.as<int>(), B.leaf_body.as<int>(), B.node_lo.as<float4>(),
-                     B.node_hi.as<float4>(), B.node_body.as<int2>(), I);
+  hipLaunchKernelGGL(leaf_levels_kernel, dim3(nblocks(n, kLeafBlock)), dim3(kLeafBlock), 0, s, mode, n, prims,
+                     body, x0, x1, pad, B.keys_sorted.as<unsigned long long>(), B.leaf_id.as<int>(),
+                     B.leaf_vid.as<int4>(), B.node_lo.as<float4>(), B.node_hi.as<float4>(), B.node_body.as<int2>(),
+                     I);
   if (I.top > kInBlockLevels)
     hipLaunchKernelGGL(reduce_top_kernel, dim3(1), dim3(1024), 0, s, B.node_lo.as<float4>(), B.node_hi.as<float4>(),
                        B.node_body.as<int2>(), I);
@@ -608,10 +661,7 @@ hipError_t run_query(const QueryLaunch& q, hipStream_t s) {
   A.node_lo = q.bvh->node_lo.as<float4>();
   A.node_hi = q.bvh->node_hi.as<float4>();
   A.node_body = q.bvh->node_body.as<int2>();
-  A.leaf_lo = q.bvh->leaf_lo.as<float4>();
-  A.leaf_hi = q.bvh->leaf_hi.as<float4>();
-  A.leaf_id = q.bvh->leaf_id.as<int>();
-  A.leaf_body = q.bvh->leaf_body.as<int>();
+  A.leaf_vid = q.bvh->leaf_vid.as<int4>();
   A.rec = q.rec;
   A.ids = q.ids;
   A.capacity = q.capacity;

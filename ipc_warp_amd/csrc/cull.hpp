@@ -19,11 +19,15 @@ constexpr int kFan = 8;        // children per hierarchy node
 constexpr int kMaxLevels = 12;  // 8^10 > 2^27 primitives
 
 // Sizes of the implicit 8-ary hierarchy: level 0 holds the n leaves in
-// Morton order, level L holds ceil(count[L-1] / 8) nodes, the root is node 0
-// of level `top`. Levels >= 1 are stored back to back from offset[L].
+// Morton order, level L holds count[L] = ceil(count[L-1] / 8) nodes, the
+// root is node 0 of level `top`. Every level, the leaves included, is stored
+// in the same node arrays from offset[L] (offset[0] = 0) and padded to
+// slots[L] = 8 count[L+1] entries (slots[top] = 1) with empty boxes, so the
+// eight children of any node are in bounds.
 struct LevelInfo {
   int top;
   int count[kMaxLevels];
+  int slots[kMaxLevels];
   int offset[kMaxLevels];
 };
 
@@ -32,11 +36,10 @@ struct Bvh {
   int n = 0;
   LevelInfo info{};
   DevBuf keys, keys_sorted;   // uint64 Morton | id
-  DevBuf leaf_lo, leaf_hi;    // float4 per leaf, Morton order
+  DevBuf node_lo, node_hi;    // float4 per slot of every level (leaves first)
+  DevBuf node_body;           // int2 (min, max) body id per slot; (b, b) for a leaf
   DevBuf leaf_id;             // primitive id per leaf
-  DevBuf node_lo, node_hi;    // float4 per node of levels >= 1
-  DevBuf leaf_body;           // body id per leaf (first vertex's body)
-  DevBuf node_body;           // int2 (min, max) body id per node of levels >= 1
+  DevBuf leaf_vid;            // int4 per leaf slot: vertex ids (v2 = -1 for an edge), primitive id
   DevBuf temp;                // radix sort workspace
   DevBuf bounds;              // 6 ordered ints
   int builds_since_sort = -1; // -1: no leaf order yet

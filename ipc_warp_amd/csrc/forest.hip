@@ -497,6 +497,9 @@ __device__ __forceinline__ bool fixed_accept(const FixedArgs& A, int src, int j,
 template <int MODE>
 __global__ void __launch_bounds__(kQueryBlock) fixed_query_kernel(FixedArgs A) {
   __shared__ unsigned long long stack[kGroupsPerBlock][kMaskStack];
+  __shared__ BlockStage stage;
+  const StageOut out{A.rec, A.ids, A.count, A.capacity};
+  stage_begin(stage);
   const int tid = threadIdx.x;
   const int grp = tid / kFan, r = tid % kFan;
   const int lane = __lane_id();
@@ -604,14 +607,11 @@ __global__ void __launch_bounds__(kQueryBlock) fixed_query_kernel(FixedArgs A) {
         stack[grp][sp] = mask_entry(L - 1, gmask, node * kFan);
         ++sp;
       }
-      int out = wave_append(want, A.count);
-      if (out >= 0 && out < A.capacity) {
-        A.rec[out] = rec;
-        if (A.ids) A.ids[out] = make_int2(src, j);
-      }
+      stage_append(stage, out, want, rec, make_int2(src, j));
       if (!__any(sp > 0)) break;
     }
   }
+  stage_flush(stage, out);
 }
 
 hipError_t launch_fixed_query(const FixedQuery& q, hipStream_t s) {

@@ -9,16 +9,24 @@ namespace ipcd {
 
 constexpr int kQueryBlock = 256;
 constexpr int kGroupsPerBlock = kQueryBlock / kFan;  // one query per 8 lanes
-constexpr int kMaskStack = 12;                       // > kMaxLevels - 1 entries per query
-constexpr int kBlockOut = 512;                       // accepted pairs a query block stages in LDS
+constexpr int kPopNodes = 1;  // sibling nodes a query group takes from its stack per step
+// A step pops from the top entry and pushes at most kPopNodes entries one
+// level down, so a level never holds more than kPopNodes entries; n <= 2^27
+// primitives gives at most 9 levels above the leaves.
+constexpr int kMaskStack = kPopNodes * 9 + 1;
+constexpr int kBlockOut = 512;  // accepted pairs a query block stages in LDS
 
-// Inclusive box test (a lower bound above the other's upper bound separates).
+// Inclusive box test (a lower bound above the other's upper bound
+// separates). The terms are combined bitwise: with short-circuit operators
+// the compiler splits the test into branches and defers each component's
+// load behind the previous compare, one memory round trip per component.
 __device__ __forceinline__ bool overlaps(const float qlo[3], const float qhi[3], float4 lo, float4 hi) {
-  return !(qlo[0] > hi.x || qlo[1] > hi.y || qlo[2] > hi.z || lo.x > qhi[0] || lo.y > qhi[1] || lo.z > qhi[2]);
+  return !((qlo[0] > hi.x) | (qlo[1] > hi.y) | (qlo[2] > hi.z) | (lo.x > qhi[0]) | (lo.y > qhi[1]) |
+           (lo.z > qhi[2]));
 }
 
-// Stack entry: level (bits 40..43), child mask (bits 32..39), first child
-// index at that level (bits 0..31).
+// Stack entry: level (bits 40..43), node mask (bits 32..39), and an index
+// (bits 0..31) whose meaning each query kernel states.
 __device__ __forceinline__ unsigned long long mask_entry(int level, unsigned mask, int first) {
   return (static_cast<unsigned long long>(level) << 40) | (static_cast<unsigned long long>(mask & 0xffu) << 32) |
          static_cast<unsigned>(first);
@@ -35,6 +43,59 @@ __device__ __forceinline__ int wave_append(bool want, int* count) {
   base = __shfl(base, leader, 64);
   if (!want) return -1;
   return base + __popcll(m & ((1ull << lane) - 1ull));
+}
+
+// Accepted pairs staged in LDS per workgroup. Appends from every wave go to
+// an LDS counter; the workgroup reserves its output range with one global
+// atomicAdd at the end (a single global counter hit by every wave
+// serialises). Pairs past kBlockOut go straight to the output, one global
+// atomicAdd per wave.
+struct BlockStage {
+  int4 rec[kBlockOut];
+  int2 ids[kBlockOut];
+  int n, base;
+};
+
+struct StageOut {
+  int4* rec;  // may be null (count only)
+  int2* ids;  // may be null
+  int* count;
+  int capacity;
+};
+
+// Called by the whole workgroup before the first stage_append.
+__device__ __forceinline__ void stage_begin(BlockStage& S) {
+  if (threadIdx.x == 0) S.n = 0;
+  __syncthreads();
+}
+
+// Called by every lane of a wave together (ballots inside).
+__device__ __forceinline__ void stage_append(BlockStage& S, const StageOut& O, bool want, int4 rec, int2 id) {
+  int s = wave_append(want, &S.n);
+  bool spill = want && s >= kBlockOut;
+  int g = wave_append(spill, O.count);
+  if (want && !spill) {
+    S.rec[s] = rec;
+    S.ids[s] = id;
+  } else if (spill && g < O.capacity) {
+    if (O.rec) O.rec[g] = rec;
+    if (O.ids) O.ids[g] = id;
+  }
+}
+
+// Called by the whole workgroup once every wave has finished appending.
+__device__ __forceinline__ void stage_flush(BlockStage& S, const StageOut& O) {
+  __syncthreads();
+  int staged = S.n < kBlockOut ? S.n : kBlockOut;
+  if (threadIdx.x == 0) S.base = staged ? atomicAdd(O.count, staged) : 0;
+  __syncthreads();
+  for (int k = threadIdx.x; k < staged; k += blockDim.x) {
+    int out = S.base + k;
+    if (out < O.capacity) {
+      if (O.rec) O.rec[out] = S.rec[k];
+      if (O.ids) O.ids[out] = S.ids[k];
+    }
+  }
 }
 
 }  // namespace ipcd
