@@ -90,11 +90,10 @@ hipError_t upload(DevBuf& b, const std::vector<T>& v) {
 
 void Forest::release() {
   desc.release();
-  leaf_lo.release();
-  leaf_hi.release();
   leaf_id.release();
   node_lo.release();
   node_hi.release();
+  slot_vid.release();
   n_trees = n_leaves = n_nodes = max_leaves = 0;
 }
 
@@ -106,13 +105,17 @@ hipError_t build_forest(Forest& F, int n_bodies, int n_prims, int k, const int* 
     if (b >= 0 && b < n_bodies) per[b].push_back(p);
   }
   std::vector<TreeDesc> D(n_bodies);
-  std::vector<float4> llo, lhi, nlo, nhi;
+  std::vector<float4> nlo, nhi;
+  std::vector<int4> vid;
   std::vector<int> lid;
+  const float4 empty_lo = make_float4(INFINITY, INFINITY, INFINITY, 0.f);
+  const float4 empty_hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+  const int4 no_vid = make_int4(-1, -1, -1, -1);
   int max_leaves = 0;
   for (int b = 0; b < n_bodies; ++b) {
     const std::vector<int>& L = per[b];
     TreeDesc d{};
-    d.leaf_base = int(llo.size());
+    d.leaf_base = int(lid.size());
     d.n_leaves = int(L.size());
     max_leaves = std::max(max_leaves, d.n_leaves);
     if (L.empty()) {
@@ -134,52 +137,57 @@ hipError_t build_forest(Forest& F, int n_bodies, int n_prims, int k, const int* 
       }
     }
     std::vector<int> ord = morton_order(L, cen);
+    // level sizes
+    d.count[0] = d.n_leaves;
+    int top = 0;
+    do {
+      ++top;
+      d.count[top] = (d.count[top - 1] + kFan - 1) / kFan;
+    } while (d.count[top] > 1 && top + 1 < kMaxLevels);
+    d.top = top;
+    // level 0: leaves in Morton order, then padding
     std::vector<float4> clo, chi;
     for (int i : ord) {
-      float4 l = make_float4(blo[3 * i], blo[3 * i + 1], blo[3 * i + 2], 0.f);
-      float4 h = make_float4(bhi[3 * i], bhi[3 * i + 1], bhi[3 * i + 2], 0.f);
-      llo.push_back(l);
-      lhi.push_back(h);
+      clo.push_back(make_float4(blo[3 * i], blo[3 * i + 1], blo[3 * i + 2], 0.f));
+      chi.push_back(make_float4(bhi[3 * i], bhi[3 * i + 1], bhi[3 * i + 2], 0.f));
       lid.push_back(L[i]);
-      clo.push_back(l);
-      chi.push_back(h);
     }
-    d.count[0] = d.n_leaves;
-    int lev = 0;
-    do {
-      ++lev;
-      int m = (d.count[lev - 1] + kFan - 1) / kFan;
-      d.count[lev] = m;
-      d.offset[lev] = int(nlo.size());
-      std::vector<float4> nxl(m), nxh(m);
-      for (int g = 0; g < m; ++g) {
-        float4 l = make_float4(INFINITY, INFINITY, INFINITY, 0.f), h = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
-        for (int c = 0; c < kFan; ++c) {
-          int ch = g * kFan + c;
-          if (ch >= d.count[lev - 1]) break;
-          l = make_float4(std::min(l.x, clo[ch].x), std::min(l.y, clo[ch].y), std::min(l.z, clo[ch].z), 0.f);
-          h = make_float4(std::max(h.x, chi[ch].x), std::max(h.y, chi[ch].y), std::max(h.z, chi[ch].z), 0.f);
-        }
-        nxl[g] = l;
-        nxh[g] = h;
-        nlo.push_back(l);
-        nhi.push_back(h);
+    for (int lev = 0; lev <= top; ++lev) {
+      if (lev > 0) {  // merge the level below
+        std::vector<float4> nxl(d.count[lev], empty_lo), nxh(d.count[lev], empty_hi);
+        for (int g = 0; g < d.count[lev]; ++g)
+          for (int c = 0; c < kFan && g * kFan + c < d.count[lev - 1]; ++c) {
+            const float4 &l = clo[g * kFan + c], &h = chi[g * kFan + c];
+            nxl[g] = make_float4(std::min(nxl[g].x, l.x), std::min(nxl[g].y, l.y), std::min(nxl[g].z, l.z), 0.f);
+            nxh[g] = make_float4(std::max(nxh[g].x, h.x), std::max(nxh[g].y, h.y), std::max(nxh[g].z, h.z), 0.f);
+          }
+        clo.swap(nxl);
+        chi.swap(nxh);
       }
-      clo.swap(nxl);
-      chi.swap(nxh);
-    } while (d.count[lev] > 1 && lev + 1 < kMaxLevels);
-    d.top = lev;
+      int slots = lev < top ? kFan * d.count[lev + 1] : 1;
+      d.offset[lev] = int(nlo.size());
+      for (int g = 0; g < slots; ++g) {
+        bool real = g < d.count[lev];
+        nlo.push_back(real ? clo[g] : empty_lo);
+        nhi.push_back(real ? chi[g] : empty_hi);
+        if (lev == 0 && real) {
+          int p = L[ord[g]];
+          vid.push_back(make_int4(prims[k * p], prims[k * p + 1], k == 3 ? prims[k * p + 2] : -1, p));
+        } else {
+          vid.push_back(no_vid);
+        }
+      }
+    }
     D[b] = d;
   }
   F.release();
   hipError_t e = upload(F.desc, D);
-  if (e == hipSuccess) e = upload(F.leaf_lo, llo);
-  if (e == hipSuccess) e = upload(F.leaf_hi, lhi);
   if (e == hipSuccess) e = upload(F.leaf_id, lid);
   if (e == hipSuccess) e = upload(F.node_lo, nlo);
   if (e == hipSuccess) e = upload(F.node_hi, nhi);
+  if (e == hipSuccess) e = upload(F.slot_vid, vid);
   F.n_trees = n_bodies;
-  F.n_leaves = int(llo.size());
+  F.n_leaves = int(lid.size());
   F.n_nodes = int(nlo.size());
   F.max_leaves = max_leaves;
   return e;
@@ -453,11 +461,9 @@ struct FixedArgs {
   const float* x1;
   const BodyFrame* frames;
   const TreeDesc* desc;
-  const float4* leaf_lo;
-  const float4* leaf_hi;
-  const int* leaf_id;
   const float4* node_lo;
   const float4* node_hi;
+  const int4* slot_vid;
   float thickness;
   float pad;
   int4* rec;
@@ -472,28 +478,39 @@ __device__ __forceinline__ V3<double> to_local(const BodyFrame& F, V3<float> w) 
             F.inv[6] * d0 + F.inv[7] * d1 + F.inv[8] * d2);
 }
 
+// The task's own primitive: vertex ids a0, a1 (edge) or the point, and for
+// DCD their positions at x0, loaded once per task; lv is the leaf's record.
+struct TaskPrim {
+  int src, a0, a1;
+  V3<float> p0, p1;
+};
+
 template <int MODE>
-__device__ __forceinline__ bool fixed_accept(const FixedArgs& A, int src, int j, int4& rec) {
+__device__ __forceinline__ bool fixed_accept(const FixedArgs& A, const TaskPrim& P, int4 lv, int4& rec) {
   if (MODE == Q_DCD_EE || MODE == Q_CCD_EE) {
-    int a0 = A.edges[2 * src], a1 = A.edges[2 * src + 1], b0 = A.edges[2 * j], b1 = A.edges[2 * j + 1];
-    rec = make_int4(a0, a1, b0, b1);
+    int b0 = lv.x, b1 = lv.y;
+    rec = make_int4(P.a0, P.a1, b0, b1);
     if (MODE == Q_CCD_EE) return true;
-    EdgeClosest ec = closest_point_edge_edge(ldx(A.x0, a0), ldx(A.x0, a1), ldx(A.x0, b0), ldx(A.x0, b1), 1.0e-6f);
+    EdgeClosest ec = closest_point_edge_edge(P.p0, P.p1, ldx(A.x0, b0), ldx(A.x0, b1), 1.0e-6f);
     return ec.dist < A.thickness * 2.0f;
   }
-  int t0 = A.tris[3 * j], t1 = A.tris[3 * j + 1], t2 = A.tris[3 * j + 2];
+  int t0 = lv.x, t1 = lv.y, t2 = lv.z;
   if (MODE == Q_CCD_PT) {
-    rec = make_int4(src, t0, t1, t2);
+    rec = make_int4(P.src, t0, t1, t2);
     return true;
   }
-  rec = make_int4(src, t1, t0, t2);
-  TriClosest tc = closest_point_triangle(ldx(A.x0, src), ldx(A.x0, t0), ldx(A.x0, t1), ldx(A.x0, t2));
+  rec = make_int4(P.src, t1, t0, t2);
+  TriClosest tc = closest_point_triangle(P.p0, ldx(A.x0, t0), ldx(A.x0, t1), ldx(A.x0, t2));
   return tc.dist < A.thickness * 2.0f;
 }
 
-// One task per eight lanes, as query_kernel in cull.hip; the grid is
-// persistent (tasks are counted on the device) and each wave strides over
-// groups of eight tasks, so every lane of a wave runs the same trip count.
+// One task per eight lanes, with the step of query_kernel in cull.hip: a
+// stack entry (level L, mask, cbase) stands for level-L nodes of the task's
+// tree whose children sit at slots cbase + 8 c + 0..7 of the forest's node
+// arrays; every load of a step is issued up front (padded levels keep them
+// in bounds). The grid is persistent (tasks are counted on the device) and
+// each wave strides over groups of eight tasks, so every lane of a wave runs
+// the same trip count.
 template <int MODE>
 __global__ void __launch_bounds__(kQueryBlock) fixed_query_kernel(FixedArgs A) {
   __shared__ unsigned long long stack[kGroupsPerBlock][kMaskStack];
@@ -513,28 +530,41 @@ __global__ void __launch_bounds__(kQueryBlock) fixed_query_kernel(FixedArgs A) {
        wbase += waves_total * groups_per_wave) {
     int slot = wbase + gw;
     bool live = slot < n;
-    int src = 0, tb = 0;
+    TaskPrim P;
+    P.src = P.a0 = P.a1 = 0;
+    P.p0 = P.p1 = mk(0.f, 0.f, 0.f);
+    int tb = 0;
     float qlo[3] = {0.f, 0.f, 0.f}, qhi[3] = {-1.f, -1.f, -1.f};
     if (live) {
       int2 task = A.tasks[slot];
-      src = task.x;
+      P.src = task.x;
       tb = task.y;
       live = A.desc[tb].n_leaves > 0;
     }
     if (live) {
+      const int src = P.src;
       const BodyFrame& F = A.frames[tb];
       V3<double> lo, hi;
       if (MODE == Q_DCD_EE || MODE == Q_CCD_EE) {
-        V3<double> a = to_local(F, ldx(A.x0, A.edges[2 * src])), b = to_local(F, ldx(A.x0, A.edges[2 * src + 1]));
+        P.a0 = A.edges[2 * src];
+        P.a1 = A.edges[2 * src + 1];
+        V3<float> w0 = ldx(A.x0, P.a0), w1 = ldx(A.x0, P.a1);
+        if (MODE == Q_DCD_EE) {
+          P.p0 = w0;
+          P.p1 = w1;
+        }
+        V3<double> a = to_local(F, w0), b = to_local(F, w1);
         lo = vmin(a, b);
         hi = vmax(a, b);
         if (MODE == Q_CCD_EE) {
-          V3<double> a1 = to_local(F, ldx(A.x1, A.edges[2 * src])), b1 = to_local(F, ldx(A.x1, A.edges[2 * src + 1]));
+          V3<double> a1 = to_local(F, ldx(A.x1, P.a0)), b1 = to_local(F, ldx(A.x1, P.a1));
           lo = vmin(lo, vmin(a1, b1));
           hi = vmax(hi, vmax(a1, b1));
         }
       } else {
-        lo = hi = to_local(F, ldx(A.x0, src));
+        V3<float> w = ldx(A.x0, src);
+        if (MODE == Q_DCD_PT) P.p0 = w;
+        lo = hi = to_local(F, w);
         if (MODE == Q_CCD_PT) {
           V3<double> p1 = to_local(F, ldx(A.x1, src));
           lo = vmin(lo, p1);
@@ -564,50 +594,40 @@ __global__ void __launch_bounds__(kQueryBlock) fixed_query_kernel(FixedArgs A) {
     const TreeDesc* D = A.desc + tb;
     int sp = 0;
     if (live) {
-      stack[grp][0] = mask_entry(D->top, 1u, 0);
+      stack[grp][0] = mask_entry(D->top, 1u, D->offset[D->top - 1]);  // the root
       sp = 1;
     }
     while (true) {
-      bool want = false;
-      int j = -1;
-      int4 rec;
       bool active = sp > 0;
-      int L = 0, node = 0;
+      int L = 0, cbase = 0, c = -1;
       if (active) {
         unsigned long long e = stack[grp][sp - 1];
-        L = int(e >> 40);
+        L = int(e >> 40) & 0xf;
         unsigned mask = unsigned(e >> 32) & 0xffu;
-        int first = int(e & 0xffffffffu);
-        int c = __ffs(mask) - 1;
-        node = first + c;
+        cbase = int(e & 0xffffffffu);
+        c = __ffs(mask) - 1;
         mask &= mask - 1u;
         if (mask == 0u) --sp;
-        else stack[grp][sp - 1] = mask_entry(L, mask, first);
+        else stack[grp][sp - 1] = mask_entry(L, mask, cbase);
       }
-      bool overlap_child = false;
-      int child = node * kFan + r;
-      if (active && L >= 2) {
-        int lv = L - 1;
-        if (child < D->count[lv]) {
-          int at = D->offset[lv] + child;
-          overlap_child = overlaps(qlo, qhi, A.node_lo[at], A.node_hi[at]);
-        }
-      } else if (active && L == 1) {
-        if (child < D->n_leaves) {
-          int gl = D->leaf_base + child;
-          if (overlaps(qlo, qhi, A.leaf_lo[gl], A.leaf_hi[gl])) {
-            j = A.leaf_id[gl];
-            want = fixed_accept<MODE>(A, src, j, rec);
-          }
-        }
-      }
-      unsigned long long ball = __ballot(overlap_child);
-      unsigned gmask = unsigned(ball >> group_shift) & 0xffu;
-      if (active && L >= 2 && gmask != 0u && sp < kMaskStack) {
-        stack[grp][sp] = mask_entry(L - 1, gmask, node * kFan);
+      const int at = c >= 0 ? cbase + c * kFan + r : 0;
+      const float4 clo = A.node_lo[at], chi = A.node_hi[at];
+      const int4 lv = A.slot_vid[L == 1 ? at : 0];
+      const bool internal = L >= 2;
+      const int child_off = D->offset[L >= 1 ? L - 1 : 0], child_cnt = D->count[L >= 1 ? L - 1 : 0];
+      const int grand_off = D->offset[internal ? L - 2 : 0];
+      // only real children (bitwise, so that no load is sunk behind a branch)
+      const bool hit = (c >= 0) & (at - child_off < child_cnt) & overlaps(qlo, qhi, clo, chi);
+      unsigned gmask = unsigned(__ballot(internal && hit) >> group_shift) & 0xffu;
+      if (internal && gmask != 0u && sp < kMaskStack) {
+        int rel = cbase - child_off + c * kFan;  // first child's index within level L - 1
+        stack[grp][sp] = mask_entry(L - 1, gmask, grand_off + rel * kFan);
         ++sp;
       }
-      stage_append(stage, out, want, rec, make_int2(src, j));
+      bool want = false;
+      int4 rec = make_int4(0, 0, 0, 0);
+      if (L == 1 && hit) want = fixed_accept<MODE>(A, P, lv, rec);
+      stage_append(stage, out, want, rec, make_int2(P.src, lv.w));
       if (!__any(sp > 0)) break;
     }
   }
@@ -626,11 +646,9 @@ hipError_t launch_fixed_query(const FixedQuery& q, hipStream_t s) {
   A.x1 = q.x1;
   A.frames = q.frames;
   A.desc = q.forest->desc.as<TreeDesc>();
-  A.leaf_lo = q.forest->leaf_lo.as<float4>();
-  A.leaf_hi = q.forest->leaf_hi.as<float4>();
-  A.leaf_id = q.forest->leaf_id.as<int>();
   A.node_lo = q.forest->node_lo.as<float4>();
   A.node_hi = q.forest->node_hi.as<float4>();
+  A.slot_vid = q.forest->slot_vid.as<int4>();
   A.thickness = q.thickness;
   A.pad = q.pad;
   A.rec = q.rec;

@@ -6,10 +6,11 @@
 
 namespace ipcd {
 
-// One body's tree inside a Forest: leaves [leaf_base, leaf_base + n_leaves)
-// of the forest's leaf arrays in Morton order of the body's rest frame; its
-// level L >= 1 holds count[L] nodes from offset[L] of the node arrays; the
-// root is node 0 of level `top` (top >= 1 when n_leaves > 0).
+// One body's tree inside a Forest: its primitives are leaf_id[leaf_base ..
+// leaf_base + n_leaves) in Morton order of the body's rest frame. Level L
+// (0 = leaves) holds count[L] entries in the node arrays from slot
+// offset[L], padded with empty boxes to 8 count[L+1] slots (one slot for the
+// root level `top`; top >= 1 when n_leaves > 0).
 struct TreeDesc {
   int leaf_base, n_leaves, top, pad;
   int count[kMaxLevels];
@@ -21,9 +22,9 @@ struct TreeDesc {
 struct Forest {
   int n_trees = 0, n_leaves = 0, n_nodes = 0, max_leaves = 0;
   DevBuf desc;              // TreeDesc per body
-  DevBuf leaf_lo, leaf_hi;  // float4 per leaf
-  DevBuf leaf_id;           // primitive id per leaf
-  DevBuf node_lo, node_hi;  // float4 per node
+  DevBuf leaf_id;           // primitive id per leaf, body by body
+  DevBuf node_lo, node_hi;  // float4 per slot of every level of every tree
+  DevBuf slot_vid;          // int4 per slot: a leaf's vertex ids (v2 = -1 for an edge) and primitive id, else -1
   void release();
 };
 
