@@ -276,8 +276,8 @@ def main() -> int:
         ops.contact_energy_pt(S.x, S.contacts_pt.list, n_pt, S.l0, S.contact_stiffness, e_dev)
         if timed:
             kev["energy"][1].record()
-        e = float(e_dev.item())
         toi = S.raw_toi(dx)
+        e = float(e_dev.item())  # after the CCD's own synchronisation
         if timed:
             torch.cuda.synchronize()
             for k in kev:

@@ -568,7 +568,12 @@ template <typename T> IPCD_FN void deflate(const T coef[4], T root, T out[3]) {
 }
 // Safeguarded Newton / bisection of cubic_roots.py:29-62. The reference loop
 // is unbounded; this one stops after kFindClosedMaxIter rounds so every wave
-// terminates.
+// terminates. A round takes the Newton step when it stays inside the
+// bracket and the bracket's midpoint otherwise; both are formed and one is
+// selected, so lanes of a wave taking different branches do not run the two
+// branches one after the other. The values are those of the branched loop:
+// the stop tests do not depend on the new y, which is only evaluated to
+// continue.
 constexpr int kFindClosedMaxIter = 200;
 template <typename T>
 IPCD_FN T find_closed(const T coef[4], const T deriv[4], T x0, T x1, T y0) {
@@ -583,16 +588,12 @@ IPCD_FN T find_closed(const T coef[4], const T deriv[4], T x0, T x1, T y0) {
     T dy = cevaluate(deriv, xr);
     T dx = yr / dy;
     T xn = xr - dx;
-    if (xn > xb0 && xn < xb1) {
-      T stepsize = fabs(xr - xn);
-      xr = xn;
-      if (stepsize > x_error) yr = cevaluate(coef, xr);
-      else break;
-    } else {
-      xr = (xb0 + xb1) / T(2);
-      yr = cevaluate(coef, xr);
-      if (xb0 == xr || xb1 == xr || xb1 - xb0 <= T(2) * x_error) break;
-    }
+    bool newton = xn > xb0 && xn < xb1;
+    T mid = (xb0 + xb1) / T(2);
+    bool done = newton ? !(fabs(xr - xn) > x_error) : (xb0 == mid || xb1 == mid || xb1 - xb0 <= T(2) * x_error);
+    xr = newton ? xn : mid;
+    if (done) break;
+    yr = cevaluate(coef, xr);
   }
   return xr;
 }
@@ -616,6 +617,12 @@ IPCD_FN int quadratic_roots(const T coef[3], T x0, T x1, T roots[2]) {
   }
   return ret;
 }
+// The case analysis of cubic_roots.py picks one bracket for the iterative
+// root and, in some cases, an interval for the deflated quadratic's roots;
+// neither choice depends on the root's value, so the cases are decided
+// first and find_closed runs at a single call site. Lanes of a wave that
+// take different cases then share one loop instead of running one inlined
+// loop per case after another.
 template <typename T>
 IPCD_FN int cubic_roots(const T coef[4], T x0, T x1, T roots[3]) {
   roots[0] = roots[1] = roots[2] = T(0);
@@ -623,65 +630,67 @@ IPCD_FN int cubic_roots(const T coef[4], T x0, T x1, T roots[3]) {
   T a = coef[3] * T(3), b_2 = coef[2], c = coef[1];
   T deriv[4] = {c, T(2) * b_2, a, T(0)};
   T delta_4 = b_2 * b_2 - a * c;
-  T defpoly[3], qroots[2];
+  bool search = false, deflated = false;
+  T lo = x0, hi = x1, ylo = y0, dlo = x0;  // bracket of the root; lower end of the quadratic's interval
   if (delta_4 > T(0)) {
     T d_2 = sqrt(delta_4);
     T q = -(b_2 + multi_sign(d_2, b_2));
     T rv0 = q / a, rv1 = c / q;
     T xa = smin(rv0, rv1), xb = smax(rv0, rv1);
     bool outside = xa >= x1 || xb <= x0 || (xa <= x0 && xb >= x1);
-    if (diff_sign(y0, y1)) {
-      if (outside) { roots[0] = find_closed(coef, deriv, x0, x1, y0); return 1; }
-    } else if (outside) {
-      return 0;
-    }
-    if (xa > x0) {
+    if (outside) {
+      search = diff_sign(y0, y1);  // bracket [x0, x1]
+    } else if (xa > x0) {
       T ya = cevaluate(coef, xa);
       if (diff_sign(y0, ya)) {
-        roots[0] = find_closed(coef, deriv, x0, xa, y0);
-        if (diff_sign(ya, y1) || (xb < x1 && diff_sign(ya, cevaluate(coef, xb)))) {
-          deflate(coef, roots[0], defpoly);
-          int r = quadratic_roots(defpoly, xa, x1, qroots);
-          roots[1] = qroots[0]; roots[2] = qroots[1];
-          return r + 1;
-        }
-        return 1;
-      }
-      if (xb < x1) {
+        search = true;
+        hi = xa;
+        deflated = diff_sign(ya, y1) || (xb < x1 && diff_sign(ya, cevaluate(coef, xb)));
+        dlo = xa;
+      } else if (xb < x1) {
         T yb = cevaluate(coef, xb);
         if (diff_sign(ya, yb)) {
-          roots[0] = find_closed(coef, deriv, xa, xb, ya);
-          if (diff_sign(yb, y1)) {
-            deflate(coef, roots[0], defpoly);
-            int r = quadratic_roots(defpoly, xb, x1, qroots);
-            roots[1] = qroots[0]; roots[2] = qroots[1];
-            return r + 1;
-          }
-          return 1;
+          search = true;
+          lo = xa;
+          hi = xb;
+          ylo = ya;
+          deflated = diff_sign(yb, y1);
+          dlo = xb;
+        } else if (diff_sign(yb, y1)) {
+          search = true;
+          lo = xb;
+          ylo = yb;
         }
-        if (diff_sign(yb, y1)) { roots[0] = find_closed(coef, deriv, xb, x1, yb); return 1; }
       } else if (diff_sign(ya, y1)) {
-        roots[0] = find_closed(coef, deriv, xa, x1, ya);
-        return 1;
+        search = true;
+        lo = xa;
+        ylo = ya;
       }
     } else {
       T yb = cevaluate(coef, xb);
       if (diff_sign(y0, yb)) {
-        roots[0] = find_closed(coef, deriv, x0, xb, y0);
-        if (diff_sign(yb, y1)) {
-          deflate(coef, roots[0], defpoly);
-          int r = quadratic_roots(defpoly, xb, x1, qroots);
-          roots[1] = qroots[0]; roots[2] = qroots[1];
-          return r + 1;
-        }
-        return 1;
+        search = true;
+        hi = xb;
+        deflated = diff_sign(yb, y1);
+        dlo = xb;
+      } else if (diff_sign(yb, y1)) {
+        search = true;
+        lo = xb;
+        ylo = yb;
       }
-      if (diff_sign(yb, y1)) { roots[0] = find_closed(coef, deriv, xb, x1, yb); return 1; }
     }
-    return 0;
+  } else {
+    search = diff_sign(y0, y1);
   }
-  if (diff_sign(y0, y1)) { roots[0] = find_closed(coef, deriv, x0, x1, y0); return 1; }
-  return 0;
+  if (!search) return 0;
+  roots[0] = find_closed(coef, deriv, lo, hi, ylo);
+  if (!deflated) return 1;
+  T defpoly[3], qroots[2];
+  deflate(coef, roots[0], defpoly);
+  int r = quadratic_roots(defpoly, dlo, x1, qroots);
+  roots[1] = qroots[0];
+  roots[2] = qroots[1];
+  return r + 1;
 }
 
 template <typename T> IPCD_FN T det3(V3<T> a, V3<T> b, V3<T> c) {
