@@ -128,10 +128,10 @@ int ipc_detect_collision_ee(ipc_ctx* ctx, const float* x, float thickness, float
 int ipc_detect_collision_pt(ipc_ctx* ctx, const float* x, float thickness, float buffer, int exclude_same_body,
                             int* contacts, int* e0e1, int capacity, int* count, void* stream);
 
-/* Both kinds of detect_collision (contact.py:687-725) in one call: the EE
- * half on `stream`, the PT half concurrently on the context's own side
- * stream (default priority), joined back to `stream` before returning;
- * arguments and results as the two calls above. No host synchronisation. */
+/* Both kinds of detect_collision (contact.py:687-725) in one call: both
+ * trees in one build launch and both queries in one query launch on
+ * `stream` (the halves share the device within each launch); arguments and
+ * results as the two calls above. No host synchronisation. */
 int ipc_detect_collision(ipc_ctx* ctx, const float* x, float thickness, float buffer, int exclude_same_body, int* ee,
                          int* ee_ids, int ee_capacity, int* ee_count, int* pt, int* pt_ids, int pt_capacity,
                          int* pt_count, void* stream);
@@ -228,7 +228,10 @@ int ipc_collision_free_step_fixed(ipc_ctx* ctx, const float* x, const float* dx,
  * while enabled, each BVH build, query and TOI launch of the context is
  * bracketed by HIP events on its stream. ipc_ctx_kernel_times waits for
  * them and returns, per kind, the accumulated milliseconds and number of
- * launches since the last call (arrays of IPC_TIMER_KINDS entries). */
+ * launches since the last call (arrays of IPC_TIMER_KINDS entries). The
+ * calls that serve both halves (ipc_detect_collision,
+ * ipc_collision_free_step and the fixed-body CCD's TOI) launch each stage
+ * once for EE and PT together and record it under the EE kind. */
 #define IPC_TIMER_BUILD_EDGE 0
 #define IPC_TIMER_BUILD_TRI 1
 #define IPC_TIMER_QUERY_DCD_EE 2

@@ -260,19 +260,19 @@ __device__ __forceinline__ double wave_min(double v) {
 // per-candidate TOI (optional) and folds TOI < 1 into *toi_min with one
 // 64-bit atomicMin per wave (TOI >= 0, so the IEEE bit pattern orders like
 // an unsigned integer).
+// Workgroup `bid` of `nblk` walking one candidate list.
 template <bool EE>
-__global__ void __launch_bounds__(256) collision_time_kernel(const float* __restrict__ x0, const float* __restrict__ x1,
-                                                             int nv, const int4* __restrict__ quads, int n,
-                                                             const int* __restrict__ n_dev,
-                                                             double* __restrict__ toi_out,
-                                                             unsigned long long* __restrict__ toi_min) {
+__device__ __forceinline__ void collision_time_block(const float* __restrict__ x0, const float* __restrict__ x1, int nv,
+                                                     const int4* __restrict__ quads, int n,
+                                                     const int* __restrict__ n_dev, double* __restrict__ toi_out,
+                                                     unsigned long long* __restrict__ toi_min, int bid, int nblk) {
   int count = n;
   if (n_dev) {
     int d = *n_dev;
     count = d < n ? d : n;
   }
   double tmin = 1.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+  for (int i = bid * blockDim.x + threadIdx.x; i < count; i += nblk * blockDim.x) {
     int4 q = quads[i];
     double t = 1.0;
     if (valid4(q, nv)) {
@@ -289,6 +289,29 @@ __global__ void __launch_bounds__(256) collision_time_kernel(const float* __rest
     double m = wave_min(tmin);
     if ((threadIdx.x & 63) == 0 && m < 1.0) atomicMin(toi_min, (unsigned long long)__double_as_longlong(m));
   }
+}
+
+template <bool EE>
+__global__ void __launch_bounds__(256) collision_time_kernel(const float* __restrict__ x0, const float* __restrict__ x1,
+                                                             int nv, const int4* __restrict__ quads, int n,
+                                                             const int* __restrict__ n_dev,
+                                                             double* __restrict__ toi_out,
+                                                             unsigned long long* __restrict__ toi_min) {
+  collision_time_block<EE>(x0, x1, nv, quads, n, n_dev, toi_out, toi_min, blockIdx.x, gridDim.x);
+}
+
+// The EE and PT candidate lists of one CCD call in one launch: workgroups
+// [0, blocks_ee) walk the EE list, the rest the PT list; both fold into the
+// same minimum.
+__global__ void __launch_bounds__(256) collision_time_pair_kernel(
+    const float* __restrict__ x0, const float* __restrict__ x1, int nv, const int4* __restrict__ quads_ee,
+    const int* __restrict__ n_ee, int cap_ee, int blocks_ee, const int4* __restrict__ quads_pt,
+    const int* __restrict__ n_pt, int cap_pt, unsigned long long* __restrict__ toi_min) {
+  if (int(blockIdx.x) < blocks_ee)
+    collision_time_block<true>(x0, x1, nv, quads_ee, cap_ee, n_ee, nullptr, toi_min, blockIdx.x, blocks_ee);
+  else
+    collision_time_block<false>(x0, x1, nv, quads_pt, cap_pt, n_pt, nullptr, toi_min, blockIdx.x - blocks_ee,
+                                gridDim.x - blocks_ee);
 }
 
 // edge_edge_intersections / point_triangle_intersections (contact.py:370-442)
@@ -414,6 +437,20 @@ hipError_t launch_collision_time(bool ee, const float* x0, const float* x1, int 
 hipError_t launch_collision_time_counted(bool ee, const float* x0, const float* x1, int nv, const int* quads,
                                          const int* n_dev, int capacity, unsigned long long* toi_min, hipStream_t s) {
   return collision_time_grid(ee, x0, x1, nv, quads, capacity, n_dev, nullptr, toi_min, s);
+}
+
+hipError_t launch_collision_time_pair(const float* x0, const float* x1, int nv, const int* quads_ee, const int* n_ee,
+                                      int cap_ee, const int* quads_pt, const int* n_pt, int cap_pt,
+                                      unsigned long long* toi_min, hipStream_t s) {
+  if (cap_ee <= 0) return launch_collision_time_counted(false, x0, x1, nv, quads_pt, n_pt, cap_pt, toi_min, s);
+  if (cap_pt <= 0) return launch_collision_time_counted(true, x0, x1, nv, quads_ee, n_ee, cap_ee, toi_min, s);
+  int blocks_ee = blocks_for(cap_ee, 256), blocks_pt = blocks_for(cap_pt, 256);
+  if (blocks_ee > 8192) blocks_ee = 8192;
+  if (blocks_pt > 8192) blocks_pt = 8192;
+  hipLaunchKernelGGL(collision_time_pair_kernel, dim3(blocks_ee + blocks_pt), dim3(256), 0, s, x0, x1, nv,
+                     reinterpret_cast<const int4*>(quads_ee), n_ee, cap_ee, blocks_ee,
+                     reinterpret_cast<const int4*>(quads_pt), n_pt, cap_pt, toi_min);
+  return hipGetLastError();
 }
 
 hipError_t launch_record_intersections(bool ee, const float* x0, const float* x1, int nv, const int* quads,

@@ -26,10 +26,7 @@ struct ipc_ctx {
   Forest fe, ft;
   DevBuf vert_base, vert_ids, body_lo, body_hi;  // per-body vertex lists and rest boxes
   DevBuf frames, world_lo, world_hi, pairs, tasks_ee, tasks_pt;
-  // side stream (default priority) for the PT half of a call, joined back
-  // to the caller's stream with events
-  hipStream_t side = nullptr;
-  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  int fixed_gen = 0;        // generation of the last fixed-body call (singular-frame flag)
   DevBuf small;            // [0] uint64 toi bits, [64..] int counters
   void* host = nullptr;    // pinned readback area
   int last_counts[2] = {0, 0};
@@ -77,24 +74,6 @@ hipError_t timed(ipc_ctx* c, int kind, hipStream_t s, F launch) {
   return e;
 }
 
-// The EE and PT halves of a call are independent: the PT half runs on the
-// context's side stream, forked from and joined back to the caller's stream.
-hipError_t fork_side(ipc_ctx* c, hipStream_t s) {
-  hipError_t e = hipSuccess;
-  if (!c->side) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
-  if (e == hipSuccess && !c->fork_ev) e = hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming);
-  if (e == hipSuccess && !c->join_ev) e = hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming);
-  if (e == hipSuccess) e = hipEventRecord(c->fork_ev, s);
-  if (e == hipSuccess) e = hipStreamWaitEvent(c->side, c->fork_ev, 0);
-  return e;
-}
-
-hipError_t join_side(ipc_ctx* c, hipStream_t s) {
-  hipError_t e = hipEventRecord(c->join_ev, c->side);
-  if (e == hipSuccess) e = hipStreamWaitEvent(s, c->join_ev, 0);
-  return e;
-}
-
 }  // namespace
 
 extern "C" {
@@ -116,6 +95,7 @@ int ipc_ctx_create(ipc_ctx** out, int n_vertices, const int* edges, int n_edges,
   if (e == hipSuccess) e = c->tris.ensure(sizeof(int) * 3 * (size_t)n_triangles + 16);
   if (e == hipSuccess) e = c->body.ensure(sizeof(int) * (size_t)n_vertices + 16);
   if (e == hipSuccess) e = c->small.ensure(256);
+  if (e == hipSuccess) e = hipMemset(c->small.p, 0, 256);
   if (e == hipSuccess && n_edges) e = hipMemcpy(c->edges.p, edges, sizeof(int) * 2 * (size_t)n_edges, hipMemcpyDefault);
   if (e == hipSuccess && n_triangles)
     e = hipMemcpy(c->tris.p, triangles, sizeof(int) * 3 * (size_t)n_triangles, hipMemcpyDefault);
@@ -156,30 +136,16 @@ int ipc_ctx_destroy(ipc_ctx* c) {
   c->frames.release(); c->world_lo.release(); c->world_hi.release(); c->pairs.release();
   c->tasks_ee.release(); c->tasks_pt.release();
   if (c->host) (void)hipHostFree(c->host);
-  if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
-  if (c->join_ev) (void)hipEventDestroy(c->join_ev);
-  if (c->side) (void)hipStreamDestroy(c->side);
   delete c;
   return IPC_OK;
 }
 
-// edge_edge_collision over a freshly built edge BVH (contact.py:672-704):
-// boxes +- (thickness + buffer), accepted pairs into contacts/e0e1 up to
-// capacity, total count into *count (device int).
-int ipc_detect_collision_ee(ipc_ctx* c, const float* x, float thickness, float buffer, int exclude_same_body,
-                            int* contacts, int* e0e1, int capacity, int* count, void* stream) {
-  if (!c || !x || !count) return ctx_fail(IPC_EINVAL, "detect_collision_ee: null pointer");
-  if (capacity < 0 || (capacity > 0 && !contacts)) return ctx_fail(IPC_EINVAL, "detect_collision_ee: bad capacity");
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  int st = ctx_hip(launch_zero_i32(count, 1, s), "detect_collision_ee");
-  if (st || c->ne == 0) return st;
-  float pad = thickness + buffer;
-  st = ctx_hip(timed(c, IPC_TIMER_BUILD_EDGE, s, [&] {
-                 return build_bvh(c->edge_bvh, BOX_EDGE, c->ne, c->edges.as<int>(), c->body.as<int>(), x, nullptr,
-                                  pad, s);
-               }),
-               "edge BVH");
-  if (st) return st;
+namespace {
+
+// The DCD queries (edge_edge_collision, point_triangle_collision,
+// contact.py:672-724) over the context's freshly built trees.
+QueryLaunch dcd_ee_query(ipc_ctx* c, const float* x, float thickness, int exclude_same_body, int* contacts,
+                         int* e0e1, int capacity, int* count) {
   QueryLaunch q{};
   q.mode = Q_DCD_EE;
   q.n_query = c->ne;
@@ -196,24 +162,11 @@ int ipc_detect_collision_ee(ipc_ctx* c, const float* x, float thickness, float b
   q.ids = reinterpret_cast<int2*>(e0e1);
   q.capacity = capacity;
   q.count = count;
-  return ctx_hip(timed(c, IPC_TIMER_QUERY_DCD_EE, s, [&] { return run_query(q, s); }), "detect_collision_ee query");
+  return q;
 }
 
-// point_triangle_collision over a freshly built triangle BVH
-// (contact.py:708-724): every vertex queries x_i +- 2 (thickness + buffer).
-int ipc_detect_collision_pt(ipc_ctx* c, const float* x, float thickness, float buffer, int exclude_same_body,
-                            int* contacts, int* e0e1, int capacity, int* count, void* stream) {
-  if (!c || !x || !count) return ctx_fail(IPC_EINVAL, "detect_collision_pt: null pointer");
-  if (capacity < 0 || (capacity > 0 && !contacts)) return ctx_fail(IPC_EINVAL, "detect_collision_pt: bad capacity");
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  int st = ctx_hip(launch_zero_i32(count, 1, s), "detect_collision_pt");
-  if (st || c->nf == 0 || c->nv == 0) return st;
-  st = ctx_hip(timed(c, IPC_TIMER_BUILD_TRI, s, [&] {
-                 return build_bvh(c->tri_bvh, BOX_TRI, c->nf, c->tris.as<int>(), c->body.as<int>(), x, nullptr, 0.0f,
-                                  s);
-               }),
-               "triangle BVH");
-  if (st) return st;
+QueryLaunch dcd_pt_query(ipc_ctx* c, const float* x, float thickness, float buffer, int exclude_same_body,
+                         int* contacts, int* e0e1, int capacity, int* count) {
   QueryLaunch q{};
   q.mode = Q_DCD_PT;
   q.n_query = c->nv;
@@ -228,23 +181,119 @@ int ipc_detect_collision_pt(ipc_ctx* c, const float* x, float thickness, float b
   q.ids = reinterpret_cast<int2*>(e0e1);
   q.capacity = capacity;
   q.count = count;
+  return q;
+}
+
+// The swept candidate queries of edge_edge_toi / point_triangle_toi.
+QueryLaunch ccd_ee_query(ipc_ctx* c, const float* x0, const float* x1, int exclude_same_body, int* quads, int* ids,
+                         int capacity, int* count) {
+  QueryLaunch q{};
+  q.mode = Q_CCD_EE;
+  q.n_query = c->ne;
+  q.order = c->edge_bvh.leaf_id.as<int>();
+  q.edges = c->edges.as<int>();
+  q.body = c->body.as<int>();
+  q.x0 = x0;
+  q.x1 = x1;
+  q.qlo = c->edge_bvh.node_lo.as<float4>();
+  q.qhi = c->edge_bvh.node_hi.as<float4>();
+  q.exclude_same_body = exclude_same_body;
+  q.bvh = &c->edge_bvh;
+  q.rec = reinterpret_cast<int4*>(quads);
+  q.ids = reinterpret_cast<int2*>(ids);
+  q.capacity = capacity;
+  q.count = count;
+  return q;
+}
+
+QueryLaunch ccd_pt_query(ipc_ctx* c, const float* x0, const float* x1, float padding, int exclude_same_body,
+                         int* quads, int* ids, int capacity, int* count) {
+  QueryLaunch q{};
+  q.mode = Q_CCD_PT;
+  q.n_query = c->nv;
+  q.tris = c->tris.as<int>();
+  q.body = c->body.as<int>();
+  q.x0 = x0;
+  q.x1 = x1;
+  q.pad = padding;
+  q.exclude_same_body = exclude_same_body;
+  q.bvh = &c->tri_bvh;
+  q.rec = reinterpret_cast<int4*>(quads);
+  q.ids = reinterpret_cast<int2*>(ids);
+  q.capacity = capacity;
+  q.count = count;
+  return q;
+}
+
+BuildPrologue zero_prologue(int* a, int* b = nullptr) {
+  BuildPrologue p;
+  if (a) p.zero[p.n_zero++] = a;
+  if (b) p.zero[p.n_zero++] = b;
+  return p;
+}
+
+}  // namespace
+
+// edge_edge_collision over a freshly built edge BVH (contact.py:672-704):
+// boxes +- (thickness + buffer), accepted pairs into contacts/e0e1 up to
+// capacity, total count into *count (device int).
+int ipc_detect_collision_ee(ipc_ctx* c, const float* x, float thickness, float buffer, int exclude_same_body,
+                            int* contacts, int* e0e1, int capacity, int* count, void* stream) {
+  if (!c || !x || !count) return ctx_fail(IPC_EINVAL, "detect_collision_ee: null pointer");
+  if (capacity < 0 || (capacity > 0 && !contacts)) return ctx_fail(IPC_EINVAL, "detect_collision_ee: bad capacity");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  BuildJob job{&c->edge_bvh, BOX_EDGE, c->ne, c->edges.as<int>(), nullptr, thickness + buffer};
+  int st = ctx_hip(timed(c, IPC_TIMER_BUILD_EDGE, s, [&] {
+                     return build_bvhs(&job, 1, c->body.as<int>(), x, zero_prologue(count), s);
+                   }),
+                   "edge BVH");
+  if (st || c->ne == 0) return st;
+  QueryLaunch q = dcd_ee_query(c, x, thickness, exclude_same_body, contacts, e0e1, capacity, count);
+  return ctx_hip(timed(c, IPC_TIMER_QUERY_DCD_EE, s, [&] { return run_query(q, s); }), "detect_collision_ee query");
+}
+
+// point_triangle_collision over a freshly built triangle BVH
+// (contact.py:708-724): every vertex queries x_i +- 2 (thickness + buffer).
+int ipc_detect_collision_pt(ipc_ctx* c, const float* x, float thickness, float buffer, int exclude_same_body,
+                            int* contacts, int* e0e1, int capacity, int* count, void* stream) {
+  if (!c || !x || !count) return ctx_fail(IPC_EINVAL, "detect_collision_pt: null pointer");
+  if (capacity < 0 || (capacity > 0 && !contacts)) return ctx_fail(IPC_EINVAL, "detect_collision_pt: bad capacity");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int n_tri = c->nv > 0 ? c->nf : 0;
+  BuildJob job{&c->tri_bvh, BOX_TRI, n_tri, c->tris.as<int>(), nullptr, 0.0f};
+  int st = ctx_hip(timed(c, IPC_TIMER_BUILD_TRI, s, [&] {
+                     return build_bvhs(&job, 1, c->body.as<int>(), x, zero_prologue(count), s);
+                   }),
+                   "triangle BVH");
+  if (st || n_tri == 0) return st;
+  QueryLaunch q = dcd_pt_query(c, x, thickness, buffer, exclude_same_body, contacts, e0e1, capacity, count);
   return ctx_hip(timed(c, IPC_TIMER_QUERY_DCD_PT, s, [&] { return run_query(q, s); }), "detect_collision_pt query");
 }
 
-// detect_collision (contact.py:687-725): the EE half on `stream`, the PT half
-// on the context's side stream, joined back; both counts stay on the device.
+// detect_collision (contact.py:687-725): both trees in one build launch and
+// both queries in one query launch on `stream` (the EE and PT halves share
+// the device instead of two streams, whose event fork and join cost more
+// host time than the launches they overlap); both counts stay on the
+// device. The launches are recorded under the EE timer kinds.
 int ipc_detect_collision(ipc_ctx* c, const float* x, float thickness, float buffer, int exclude_same_body, int* ee,
                          int* ee_ids, int ee_capacity, int* ee_count, int* pt, int* pt_ids, int pt_capacity,
                          int* pt_count, void* stream) {
-  if (!c) return ctx_fail(IPC_EINVAL, "detect_collision: null context");
+  if (!c || !x || !ee_count || !pt_count) return ctx_fail(IPC_EINVAL, "detect_collision: null pointer");
+  if (ee_capacity < 0 || (ee_capacity > 0 && !ee) || pt_capacity < 0 || (pt_capacity > 0 && !pt))
+    return ctx_fail(IPC_EINVAL, "detect_collision: bad capacity");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  int st = ctx_hip(fork_side(c, s), "detect_collision fork");
+  const int n_tri = c->nv > 0 ? c->nf : 0;
+  BuildJob jobs[2] = {{&c->edge_bvh, BOX_EDGE, c->ne, c->edges.as<int>(), nullptr, thickness + buffer},
+                      {&c->tri_bvh, BOX_TRI, n_tri, c->tris.as<int>(), nullptr, 0.0f}};
+  int st = ctx_hip(timed(c, IPC_TIMER_BUILD_EDGE, s, [&] {
+                     return build_bvhs(jobs, 2, c->body.as<int>(), x, zero_prologue(ee_count, pt_count), s);
+                   }),
+                   "detect_collision BVHs");
   if (st) return st;
-  st = ipc_detect_collision_ee(c, x, thickness, buffer, exclude_same_body, ee, ee_ids, ee_capacity, ee_count, stream);
-  int st2 = ipc_detect_collision_pt(c, x, thickness, buffer, exclude_same_body, pt, pt_ids, pt_capacity, pt_count,
-                                    c->side);
-  int st3 = ctx_hip(join_side(c, s), "detect_collision join");
-  return st ? st : (st2 ? st2 : st3);
+  QueryLaunch q[2] = {dcd_ee_query(c, x, thickness, exclude_same_body, ee, ee_ids, ee_capacity, ee_count),
+                      dcd_pt_query(c, x, thickness, buffer, exclude_same_body, pt, pt_ids, pt_capacity, pt_count)};
+  if (n_tri == 0) q[1].n_query = 0;
+  return ctx_hip(timed(c, IPC_TIMER_QUERY_DCD_EE, s, [&] { return run_queries(q, 2, s); }), "detect_collision query");
 }
 
 // Edge-triangle crossings of the surface at x: every edge's bounds against
@@ -281,30 +330,13 @@ int ipc_ccd_candidates_ee(ipc_ctx* c, const float* x0, const float* x1, float pa
   if (!c || !x0 || !x1 || !count) return ctx_fail(IPC_EINVAL, "ccd_candidates_ee: null pointer");
   if (capacity < 0 || (capacity > 0 && !quads)) return ctx_fail(IPC_EINVAL, "ccd_candidates_ee: bad capacity");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  int st = ctx_hip(launch_zero_i32(count, 1, s), "ccd_candidates_ee");
+  BuildJob job{&c->edge_bvh, BOX_SWEPT_EDGE, c->ne, c->edges.as<int>(), x1, padding};
+  int st = ctx_hip(timed(c, IPC_TIMER_BUILD_EDGE, s, [&] {
+                     return build_bvhs(&job, 1, c->body.as<int>(), x0, zero_prologue(count), s);
+                   }),
+                   "swept edge BVH");
   if (st || c->ne == 0) return st;
-  st = ctx_hip(timed(c, IPC_TIMER_BUILD_EDGE, s, [&] {
-                 return build_bvh(c->edge_bvh, BOX_SWEPT_EDGE, c->ne, c->edges.as<int>(), c->body.as<int>(), x0, x1,
-                                  padding, s);
-               }),
-               "swept edge BVH");
-  if (st) return st;
-  QueryLaunch q{};
-  q.mode = Q_CCD_EE;
-  q.n_query = c->ne;
-  q.order = c->edge_bvh.leaf_id.as<int>();
-  q.edges = c->edges.as<int>();
-  q.body = c->body.as<int>();
-  q.x0 = x0;
-  q.x1 = x1;
-  q.qlo = c->edge_bvh.node_lo.as<float4>();  // leaves are level 0, from slot 0
-  q.qhi = c->edge_bvh.node_hi.as<float4>();
-  q.exclude_same_body = exclude_same_body;
-  q.bvh = &c->edge_bvh;
-  q.rec = reinterpret_cast<int4*>(quads);
-  q.ids = reinterpret_cast<int2*>(ids);
-  q.capacity = capacity;
-  q.count = count;
+  QueryLaunch q = ccd_ee_query(c, x0, x1, exclude_same_body, quads, ids, capacity, count);
   return ctx_hip(timed(c, IPC_TIMER_QUERY_CCD_EE, s, [&] { return run_query(q, s); }), "ccd_candidates_ee query");
 }
 
@@ -313,36 +345,27 @@ int ipc_ccd_candidates_pt(ipc_ctx* c, const float* x0, const float* x1, float pa
   if (!c || !x0 || !x1 || !count) return ctx_fail(IPC_EINVAL, "ccd_candidates_pt: null pointer");
   if (capacity < 0 || (capacity > 0 && !quads)) return ctx_fail(IPC_EINVAL, "ccd_candidates_pt: bad capacity");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  int st = ctx_hip(launch_zero_i32(count, 1, s), "ccd_candidates_pt");
-  if (st || c->nf == 0 || c->nv == 0) return st;
-  st = ctx_hip(timed(c, IPC_TIMER_BUILD_TRI, s, [&] {
-                 return build_bvh(c->tri_bvh, BOX_SWEPT_TRI, c->nf, c->tris.as<int>(), c->body.as<int>(), x0, x1,
-                                  padding, s);
-               }),
-               "swept triangle BVH");
-  if (st) return st;
-  QueryLaunch q{};
-  q.mode = Q_CCD_PT;
-  q.n_query = c->nv;
-  q.tris = c->tris.as<int>();
-  q.body = c->body.as<int>();
-  q.x0 = x0;
-  q.x1 = x1;
-  q.pad = padding;
-  q.exclude_same_body = exclude_same_body;
-  q.bvh = &c->tri_bvh;
-  q.rec = reinterpret_cast<int4*>(quads);
-  q.ids = reinterpret_cast<int2*>(ids);
-  q.capacity = capacity;
-  q.count = count;
+  const int n_tri = c->nv > 0 ? c->nf : 0;
+  BuildJob job{&c->tri_bvh, BOX_SWEPT_TRI, n_tri, c->tris.as<int>(), x1, padding};
+  int st = ctx_hip(timed(c, IPC_TIMER_BUILD_TRI, s, [&] {
+                     return build_bvhs(&job, 1, c->body.as<int>(), x0, zero_prologue(count), s);
+                   }),
+                   "swept triangle BVH");
+  if (st || n_tri == 0) return st;
+  QueryLaunch q = ccd_pt_query(c, x0, x1, padding, exclude_same_body, quads, ids, capacity, count);
   return ctx_hip(timed(c, IPC_TIMER_QUERY_CCD_PT, s, [&] { return run_query(q, s); }), "ccd_candidates_pt query");
 }
 
 // collision_free_step (contact.py:539-585) before its 0.9 factor: x1 = x - dx,
-// swept-box candidates for EE then PT, FP64 TOI per candidate, minimum over
-// all of them. Candidate lists live in the context and grow on overflow (one
-// retry, like _grow_contact_list). Synchronises `stream` once to read the
-// result; writes the raw TOI (1.0 = no impact) to *toi.
+// swept-box candidates for EE and PT, FP64 TOI per candidate, minimum over
+// all of them. Five launches on `stream`: x1; both swept trees (the first
+// workgroup also clears the candidate counters and the TOI minimum); both
+// queries; both TOI lists (the TOI kernels read the candidate counts from
+// device memory, so no host round trip is needed between the phases); the
+// readback. The launches are recorded under the EE timer kinds. Candidate
+// lists live in the context and grow on overflow (one retry, like
+// _grow_contact_list). Synchronises `stream` once to read the result;
+// writes the raw TOI (1.0 = no impact) to *toi.
 int ipc_collision_free_step(ipc_ctx* c, const float* x, const float* dx, float padding, int exclude_same_body,
                             double* toi, void* stream) {
   if (!c || !x || !dx || !toi) return ctx_fail(IPC_EINVAL, "collision_free_step: null pointer");
@@ -351,38 +374,36 @@ int ipc_collision_free_step(ipc_ctx* c, const float* x, const float* dx, float p
   if (st) return st;
   if (c->cap_ee == 0) c->cap_ee = 1 << 16;
   if (c->cap_pt == 0) c->cap_pt = 1 << 16;
+  const int n_tri = c->nv > 0 ? c->nf : 0;
   for (int attempt = 0; attempt < 2; ++attempt) {
     st = ctx_hip(c->cand_ee.ensure(sizeof(int) * 4 * (size_t)c->cap_ee), "collision_free_step alloc");
     if (!st) st = ctx_hip(c->cand_pt.ensure(sizeof(int) * 4 * (size_t)c->cap_pt), "collision_free_step alloc");
     if (st) return st;
-    st = ctx_hip(launch_sub(3 * c->nv, x, dx, c->x1.as<float>(), s), "collision_free_step");
-    if (!st) st = ctx_hip(launch_fill_u64(toi_bits(c), 1, kOneBits, s), "collision_free_step");
-    if (st) return st;
+    const float* x1 = c->x1.as<float>();
     int* cnt = counters(c);
-    // EE half on `stream`, PT half on the side stream. The TOI kernels read
-    // the candidate counts from device memory, so no host round trip is
-    // needed between the phases.
-    st = ctx_hip(fork_side(c, s), "collision_free_step fork");
+    st = ctx_hip(launch_sub(3 * c->nv, x, dx, c->x1.as<float>(), s), "collision_free_step");
     if (st) return st;
-    int st_ee = ipc_ccd_candidates_ee(c, x, c->x1.as<float>(), padding, exclude_same_body, c->cand_ee.as<int>(), nullptr,
-                                      c->cap_ee, cnt + 0, stream);
-    if (!st_ee)
-      st_ee = ctx_hip(timed(c, IPC_TIMER_TOI_EE, s, [&] {
-                        return launch_collision_time_counted(true, x, c->x1.as<float>(), c->nv, c->cand_ee.as<int>(),
-                                                             cnt + 0, c->cap_ee, toi_bits(c), s);
-                      }),
-                      "collision_free_step EE TOI");
-    int st_pt = ipc_ccd_candidates_pt(c, x, c->x1.as<float>(), padding, exclude_same_body, c->cand_pt.as<int>(), nullptr,
-                                      c->cap_pt, cnt + 1, c->side);
-    if (!st_pt)
-      st_pt = ctx_hip(timed(c, IPC_TIMER_TOI_PT, c->side, [&] {
-                        return launch_collision_time_counted(false, x, c->x1.as<float>(), c->nv, c->cand_pt.as<int>(),
-                                                             cnt + 1, c->cap_pt, toi_bits(c), c->side);
-                      }),
-                      "collision_free_step PT TOI");
-    st = ctx_hip(join_side(c, s), "collision_free_step join");
-    if (st_ee) return st_ee;
-    if (st_pt) return st_pt;
+    BuildJob jobs[2] = {{&c->edge_bvh, BOX_SWEPT_EDGE, c->ne, c->edges.as<int>(), x1, padding},
+                        {&c->tri_bvh, BOX_SWEPT_TRI, n_tri, c->tris.as<int>(), x1, padding}};
+    BuildPrologue pro = zero_prologue(cnt + 0, cnt + 1);
+    pro.toi = toi_bits(c);
+    pro.toi_init = kOneBits;
+    st = ctx_hip(timed(c, IPC_TIMER_BUILD_EDGE, s, [&] { return build_bvhs(jobs, 2, c->body.as<int>(), x, pro, s); }),
+                 "collision_free_step BVHs");
+    if (st) return st;
+    QueryLaunch q[2] = {ccd_ee_query(c, x, x1, exclude_same_body, c->cand_ee.as<int>(), nullptr, c->cap_ee, cnt + 0),
+                        ccd_pt_query(c, x, x1, padding, exclude_same_body, c->cand_pt.as<int>(), nullptr, c->cap_pt,
+                                     cnt + 1)};
+    if (n_tri == 0) q[1].n_query = 0;
+    st = ctx_hip(timed(c, IPC_TIMER_QUERY_CCD_EE, s, [&] { return run_queries(q, 2, s); }),
+                 "collision_free_step query");
+    if (!st)
+      st = ctx_hip(timed(c, IPC_TIMER_TOI_EE, s, [&] {
+                     return launch_collision_time_pair(x, x1, c->nv, c->cand_ee.as<int>(), cnt + 0,
+                                                       c->ne > 0 ? c->cap_ee : 0, c->cand_pt.as<int>(), cnt + 1,
+                                                       n_tri > 0 ? c->cap_pt : 0, toi_bits(c), s);
+                   }),
+                   "collision_free_step TOI");
     if (st) return st;
     st = ctx_hip(hipMemcpyAsync(c->host, c->small.p, 128, hipMemcpyDeviceToHost, s), "collision_free_step readback");
     if (!st) st = ctx_hip(hipStreamSynchronize(s), "collision_free_step sync");
@@ -486,8 +507,15 @@ int ipc_ccd_intersections_pt(ipc_ctx* c, const float* x0, const float* x1, float
 
 // ------------------------------------------------------ fixed body BVHs ----
 // Counters in the small buffer: [0], [1] CCD candidates (EE, PT), [2] body
-// pairs, [3], [4] tasks (EE, PT), [5] singular-frame flag.
+// pairs, [3], [4] tasks (EE, PT), [5] singular-frame flag (the generation of
+// the last call that met a near-singular frame).
 namespace {
+
+// A new generation for a fixed-body call (never 0, the flag's initial value).
+int next_gen(ipc_ctx* c) {
+  c->fixed_gen = c->fixed_gen >= 0x3fffffff ? 1 : c->fixed_gen + 1;
+  return c->fixed_gen;
+}
 
 int grow_pow2(int cap, int need) {
   long long c = cap > 0 ? cap : 1;
@@ -502,15 +530,16 @@ int ensure_fixed_buffers(ipc_ctx* c) {
   return st;
 }
 
-// Body frames, body pairs and tasks of one fixed-body step (DCD: dq = NULL).
+// Body frames, body pairs and tasks of one fixed-body step (DCD: dq =
+// NULL): three launches, the first of which runs `pro` (the caller's
+// counters; the pair and task counters are added here). The singular-frame
+// flag (counter 5) is set to `gen` instead of being cleared.
 int fixed_cull(ipc_ctx* c, const float* x0, const float* x1, const float* q, const float* dq, const int* fixed,
-               float pair_pad, float task_pad, hipStream_t s) {
+               float pair_pad, float task_pad, BuildPrologue pro, int gen, hipStream_t s) {
   int* cnt = counters(c);
-  int st = ctx_hip(launch_zero_i32(cnt + 2, 4, s), "fixed cull");
-  if (st) return st;
   return ctx_hip(timed(c, IPC_TIMER_FIXED_CULL, s, [&] {
                    FixedStep f{c->nb, q, dq, c->body_lo.as<float4>(), c->body_hi.as<float4>(), c->world_lo.as<float4>(),
-                               c->world_hi.as<float4>(), c->frames.as<BodyFrame>(), cnt + 5};
+                               c->world_hi.as<float4>(), c->frames.as<BodyFrame>(), cnt + 5, gen, pro};
                    hipError_t e = launch_body_frames(f, s);
                    if (e == hipSuccess)
                      e = launch_body_pairs(c->nb, c->world_lo.as<float4>(), c->world_hi.as<float4>(), pair_pad,
@@ -522,6 +551,7 @@ int fixed_cull(ipc_ctx* c, const float* x0, const float* x1, const float* q, con
                    t.n_pairs = cnt + 2;
                    t.pair_cap = c->cap_pairs;
                    t.singular = cnt + 5;
+                   t.gen = gen;
                    t.x0 = x0;
                    t.x1 = x1;
                    t.world_lo = c->world_lo.as<float4>();
@@ -636,21 +666,23 @@ int ipc_detect_collision_fixed(ipc_ctx* c, const float* x, const float* q, float
   const float pad = 2.0f * thickness;
   for (int attempt = 0; attempt < 4; ++attempt) {
     int st = ensure_fixed_buffers(c);
-    if (!st) st = ctx_hip(launch_zero_i32(ee_count, 1, s), "detect_collision_fixed");
-    if (!st) st = ctx_hip(launch_zero_i32(pt_count, 1, s), "detect_collision_fixed");
-    if (!st) st = fixed_cull(c, x, nullptr, q, nullptr, nullptr, pad, pad, s);
+    if (st) return st;
+    const int gen = next_gen(c);
+    BuildPrologue pro;
+    pro.zero[0] = ee_count;
+    pro.zero[1] = pt_count;
+    pro.n_zero = 2;
+    pro.range = counters(c) + 2;  // pairs, tasks EE, tasks PT
+    pro.n_range = 3;
+    st = fixed_cull(c, x, nullptr, q, nullptr, nullptr, pad, pad, pro, gen, s);
     if (st) return st;
     st = ctx_hip(timed(c, IPC_TIMER_FIXED_QUERY_DCD, s, [&] {
-                   hipError_t e = launch_fixed_query(fixed_query(c, Q_DCD_EE, x, nullptr, thickness, 0.0f,
-                                                                 reinterpret_cast<int4*>(ee),
-                                                                 reinterpret_cast<int2*>(ee_ids), ee_capacity, ee_count),
-                                                     s);
-                   if (e == hipSuccess)
-                     e = launch_fixed_query(fixed_query(c, Q_DCD_PT, x, nullptr, thickness, 0.0f,
-                                                        reinterpret_cast<int4*>(pt), reinterpret_cast<int2*>(pt_ids),
-                                                        pt_capacity, pt_count),
-                                            s);
-                   return e;
+                   return launch_fixed_queries(
+                       fixed_query(c, Q_DCD_EE, x, nullptr, thickness, 0.0f, reinterpret_cast<int4*>(ee),
+                                   reinterpret_cast<int2*>(ee_ids), ee_capacity, ee_count),
+                       fixed_query(c, Q_DCD_PT, x, nullptr, thickness, 0.0f, reinterpret_cast<int4*>(pt),
+                                   reinterpret_cast<int2*>(pt_ids), pt_capacity, pt_count),
+                       s);
                  }),
                  "detect_collision_fixed query");
     if (!st) st = ctx_hip(hipMemcpyAsync(c->host, c->small.p, 128, hipMemcpyDeviceToHost, s), "detect_collision_fixed");
@@ -658,7 +690,7 @@ int ipc_detect_collision_fixed(ipc_ctx* c, const float* x, const float* q, float
     if (st) return st;
     int hc[6];
     std::memcpy(hc, static_cast<char*>(c->host) + 64, sizeof(hc));
-    if (hc[5]) {
+    if (hc[5] == gen) {
       // A (near-)singular body frame: world-space DCD instead (same pair set).
       st = ipc_detect_collision_ee(c, x, thickness, thickness, 1, ee, ee_ids, ee_capacity, ee_count, stream);
       if (!st) st = ipc_detect_collision_pt(c, x, thickness, thickness, 1, pt, pt_ids, pt_capacity, pt_count, stream);
@@ -687,41 +719,38 @@ int ipc_collision_free_step_fixed(ipc_ctx* c, const float* x, const float* dx, c
     if (!st) st = ctx_hip(c->cand_ee.ensure(sizeof(int) * 4 * (size_t)c->cap_ee), "collision_free_step_fixed alloc");
     if (!st) st = ctx_hip(c->cand_pt.ensure(sizeof(int) * 4 * (size_t)c->cap_pt), "collision_free_step_fixed alloc");
     if (!st) st = ctx_hip(launch_sub(3 * c->nv, x, dx, c->x1.as<float>(), s), "collision_free_step_fixed");
-    if (!st) st = ctx_hip(launch_fill_u64(toi_bits(c), 1, kOneBits, s), "collision_free_step_fixed");
-    if (!st) st = ctx_hip(launch_zero_i32(counters(c), 2, s), "collision_free_step_fixed");
-    if (!st) st = fixed_cull(c, x, c->x1.as<float>(), q, dq, fixed_bodies, padding, padding, s);
     if (st) return st;
     int* cnt = counters(c);
     const float* x1 = c->x1.as<float>();
+    const int gen = next_gen(c);
+    BuildPrologue pro;
+    pro.range = cnt;  // candidates EE, PT, pairs, tasks EE, tasks PT
+    pro.n_range = 5;
+    pro.toi = toi_bits(c);
+    pro.toi_init = kOneBits;
+    st = fixed_cull(c, x, x1, q, dq, fixed_bodies, padding, padding, pro, gen, s);
+    if (st) return st;
     st = ctx_hip(timed(c, IPC_TIMER_FIXED_QUERY_CCD, s, [&] {
-                   hipError_t e = launch_fixed_query(fixed_query(c, Q_CCD_EE, x, x1, 0.0f, padding,
-                                                                 c->cand_ee.as<int4>(), nullptr, c->cap_ee, cnt + 0),
-                                                     s);
-                   if (e == hipSuccess)
-                     e = launch_fixed_query(fixed_query(c, Q_CCD_PT, x, x1, 0.0f, padding, c->cand_pt.as<int4>(),
-                                                        nullptr, c->cap_pt, cnt + 1),
-                                            s);
-                   return e;
+                   return launch_fixed_queries(
+                       fixed_query(c, Q_CCD_EE, x, x1, 0.0f, padding, c->cand_ee.as<int4>(), nullptr, c->cap_ee,
+                                   cnt + 0),
+                       fixed_query(c, Q_CCD_PT, x, x1, 0.0f, padding, c->cand_pt.as<int4>(), nullptr, c->cap_pt,
+                                   cnt + 1),
+                       s);
                  }),
                  "collision_free_step_fixed query");
     if (!st)
       st = ctx_hip(timed(c, IPC_TIMER_TOI_EE, s, [&] {
-                     return launch_collision_time_counted(true, x, x1, c->nv, c->cand_ee.as<int>(), cnt + 0, c->cap_ee,
-                                                          toi_bits(c), s);
+                     return launch_collision_time_pair(x, x1, c->nv, c->cand_ee.as<int>(), cnt + 0, c->cap_ee,
+                                                       c->cand_pt.as<int>(), cnt + 1, c->cap_pt, toi_bits(c), s);
                    }),
-                   "collision_free_step_fixed EE TOI");
-    if (!st)
-      st = ctx_hip(timed(c, IPC_TIMER_TOI_PT, s, [&] {
-                     return launch_collision_time_counted(false, x, x1, c->nv, c->cand_pt.as<int>(), cnt + 1, c->cap_pt,
-                                                          toi_bits(c), s);
-                   }),
-                   "collision_free_step_fixed PT TOI");
+                   "collision_free_step_fixed TOI");
     if (!st) st = ctx_hip(hipMemcpyAsync(c->host, c->small.p, 128, hipMemcpyDeviceToHost, s), "collision_free_step_fixed");
     if (!st) st = ctx_hip(hipStreamSynchronize(s), "collision_free_step_fixed sync");
     if (st) return st;
     int hc[6];
     std::memcpy(hc, static_cast<char*>(c->host) + 64, sizeof(hc));
-    if (hc[5]) {
+    if (hc[5] == gen) {
       // min |det A| < 1e-8 at either end: the world-space swept CCD
       // (fixed_bvh_contacts.py:733-735); the fixed path only pairs
       // different bodies, so the fallback excludes same-body pairs too.

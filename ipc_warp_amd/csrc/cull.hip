@@ -167,96 +167,135 @@ __device__ __forceinline__ void lane_merge(float4& lo, float4& hi, int2& br, int
   br.y = max(br.y, __shfl_xor(br.y, off, 64));
 }
 
+// One tree of a build launch (device side).
+struct LeafJob {
+  int mode, n;
+  const int* prims;
+  const float* x1;
+  float pad;
+  const unsigned long long* keys;
+  int* leaf_id;
+  int4* leaf_vid;
+  float4* node_lo;
+  float4* node_hi;
+  int2* node_body;
+  LevelInfo info;
+};
+
+struct LeafSmem {
+  float4 wlo[kLeafBlock / 64], whi[kLeafBlock / 64];
+  int2 wbody[kLeafBlock / 64];
+};
+
 // Leaves in Morton order and levels 1..3 of the hierarchy, one thread per
-// leaf slot. Workgroup b owns leaf slots [512 b, 512 (b + 1)): level-1 nodes
-// are merged across 8-lane groups, level-2 nodes across a wave, the level-3
-// node across the workgroup's 8 waves in LDS. Leaf k's box is computed from
-// the topology and positions of primitive keys[k] (low 32 bits); slots past
-// n hold empty boxes. leaf_vid[k] holds the leaf's vertex ids (v2 = -1 for
-// an edge) and primitive id, so that a query reads one record per leaf
-// instead of chasing the topology.
-__global__ void __launch_bounds__(kLeafBlock) leaf_levels_kernel(
-    int mode, int n, const int* __restrict__ prims, const int* __restrict__ body, const float* __restrict__ x0,
-    const float* __restrict__ x1, float pad, const unsigned long long* __restrict__ keys, int* __restrict__ leaf_id,
-    int4* __restrict__ leaf_vid, float4* __restrict__ node_lo, float4* __restrict__ node_hi,
-    int2* __restrict__ node_body, LevelInfo info) {
-  __shared__ float4 wlo[kLeafBlock / 64], whi[kLeafBlock / 64];
-  __shared__ int2 wbody[kLeafBlock / 64];
+// leaf slot. Workgroup b of the job owns leaf slots [512 b, 512 (b + 1)):
+// level-1 nodes are merged across 8-lane groups, level-2 nodes across a
+// wave, the level-3 node across the workgroup's 8 waves in LDS. Leaf k's box
+// is computed from the topology and positions of primitive keys[k] (low 32
+// bits); slots past n hold empty boxes. leaf_vid[k] holds the leaf's vertex
+// ids (v2 = -1 for an edge) and primitive id, so that a query reads one
+// record per leaf instead of chasing the topology.
+__device__ __forceinline__ void leaf_block(const LeafJob& J, int bid, const int* __restrict__ body,
+                                           const float* __restrict__ x0, LeafSmem& sm) {
+  const LevelInfo& info = J.info;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int k = blockIdx.x * kLeafBlock + t;
-  const int stride = (mode == 0 || mode == 2) ? 2 : 3;
+  const int k = bid * kLeafBlock + t;
+  const int stride = (J.mode == 0 || J.mode == 2) ? 2 : 3;
   float4 lo, hi;
   box_empty(lo, hi);
   int2 br = make_int2(0x7fffffff, (int)0x80000000);
   int4 vid = make_int4(-1, -1, -1, -1);
-  if (k < n) {
-    int prim = static_cast<int>(keys[k] & 0xffffffffu);
-    prim_box(mode, prim, prims, x0, x1, pad, lo, hi);
-    int v0 = prims[stride * prim], v1 = prims[stride * prim + 1];
-    int v2 = stride == 3 ? prims[stride * prim + 2] : -1;
+  if (k < J.n) {
+    int prim = static_cast<int>(J.keys[k] & 0xffffffffu);
+    prim_box(J.mode, prim, J.prims, x0, J.x1, J.pad, lo, hi);
+    int v0 = J.prims[stride * prim], v1 = J.prims[stride * prim + 1];
+    int v2 = stride == 3 ? J.prims[stride * prim + 2] : -1;
     int bd = body[v0];
-    leaf_id[k] = prim;
+    J.leaf_id[k] = prim;
     vid = make_int4(v0, v1, v2, prim);
     br = make_int2(bd, bd);
   }
   if (k < info.slots[0]) {  // offset[0] = 0
-    node_lo[k] = lo;
-    node_hi[k] = hi;
-    node_body[k] = br;
-    leaf_vid[k] = vid;
+    J.node_lo[k] = lo;
+    J.node_hi[k] = hi;
+    J.node_body[k] = br;
+    J.leaf_vid[k] = vid;
   }
   // level 1: groups of 8 lanes
   lane_merge(lo, hi, br, 1);
   lane_merge(lo, hi, br, 2);
   lane_merge(lo, hi, br, 4);
-  int g1 = blockIdx.x * (kLeafBlock / kFan) + t / kFan;
+  int g1 = bid * (kLeafBlock / kFan) + t / kFan;
   if ((lane & (kFan - 1)) == 0 && g1 < info.slots[1]) {
-    node_lo[info.offset[1] + g1] = lo;
-    node_hi[info.offset[1] + g1] = hi;
-    node_body[info.offset[1] + g1] = br;
+    J.node_lo[info.offset[1] + g1] = lo;
+    J.node_hi[info.offset[1] + g1] = hi;
+    J.node_body[info.offset[1] + g1] = br;
   }
   // level 2: the wave's 8 level-1 nodes
   lane_merge(lo, hi, br, 8);
   lane_merge(lo, hi, br, 16);
   lane_merge(lo, hi, br, 32);
-  int g2 = blockIdx.x * (kLeafBlock / 64) + wave;
+  int g2 = bid * (kLeafBlock / 64) + wave;
   if (lane == 0) {
     if (g2 < info.slots[2]) {
-      node_lo[info.offset[2] + g2] = lo;
-      node_hi[info.offset[2] + g2] = hi;
-      node_body[info.offset[2] + g2] = br;
+      J.node_lo[info.offset[2] + g2] = lo;
+      J.node_hi[info.offset[2] + g2] = hi;
+      J.node_body[info.offset[2] + g2] = br;
     }
-    wlo[wave] = lo;
-    whi[wave] = hi;
-    wbody[wave] = br;
+    sm.wlo[wave] = lo;
+    sm.whi[wave] = hi;
+    sm.wbody[wave] = br;
   }
   __syncthreads();
   // level 3: the workgroup's 8 level-2 nodes
-  if (t == 0 && int(blockIdx.x) < info.slots[3]) {
-    float4 l = wlo[0], h = whi[0];
-    int2 r2 = wbody[0];
+  if (t == 0 && bid < info.slots[3]) {
+    float4 l = sm.wlo[0], h = sm.whi[0];
+    int2 r2 = sm.wbody[0];
     for (int w = 1; w < kLeafBlock / 64; ++w) {
-      box_merge(l, h, wlo[w], whi[w]);
-      r2.x = min(r2.x, wbody[w].x);
-      r2.y = max(r2.y, wbody[w].y);
+      box_merge(l, h, sm.wlo[w], sm.whi[w]);
+      r2.x = min(r2.x, sm.wbody[w].x);
+      r2.y = max(r2.y, sm.wbody[w].y);
     }
-    node_lo[info.offset[3] + blockIdx.x] = l;
-    node_hi[info.offset[3] + blockIdx.x] = h;
-    node_body[info.offset[3] + blockIdx.x] = r2;
+    J.node_lo[info.offset[3] + bid] = l;
+    J.node_hi[info.offset[3] + bid] = h;
+    J.node_body[info.offset[3] + bid] = r2;
   }
 }
 
-// Levels kInBlockLevels + 1 .. top in one workgroup, and the padding slots
-// of level kInBlockLevels past the leaf kernel's one node per workgroup.
-__global__ void __launch_bounds__(1024) reduce_top_kernel(float4* __restrict__ node_lo, float4* __restrict__ node_hi,
-                                                         int2* __restrict__ node_body, LevelInfo info) {
+// Workgroups [0, blocks0) build job 0, the rest job 1. The first workgroup
+// also runs the launch's prologue (counters of the queries that follow).
+__global__ void __launch_bounds__(kLeafBlock) leaf_levels_kernel(LeafJob J0, LeafJob J1, int blocks0,
+                                                                 const int* __restrict__ body,
+                                                                 const float* __restrict__ x0, BuildPrologue pro) {
+  __shared__ LeafSmem sm;
+  run_prologue(pro);
+  if (int(blockIdx.x) < blocks0) leaf_block(J0, blockIdx.x, body, x0, sm);
+  else leaf_block(J1, blockIdx.x - blocks0, body, x0, sm);
+}
+
+// The prologue alone, for a launch whose trees are all empty.
+__global__ void prologue_kernel(BuildPrologue pro) { run_prologue(pro); }
+
+// Levels kInBlockLevels + 1 .. top of one tree in one workgroup, and the
+// padding slots of level kInBlockLevels past the leaf kernel's one node per
+// workgroup.
+struct TopJob {
+  float4* node_lo;
+  float4* node_hi;
+  int2* node_body;
+  LevelInfo info;
+};
+
+__device__ __forceinline__ void top_block(const TopJob& T) {
+  const LevelInfo& info = T.info;
+  if (info.top <= kInBlockLevels) return;
   float4 el, eh;
   box_empty(el, eh);
   const int2 eb = make_int2(0x7fffffff, (int)0x80000000);
   for (int g = info.count[kInBlockLevels] + threadIdx.x; g < info.slots[kInBlockLevels]; g += blockDim.x) {
-    node_lo[info.offset[kInBlockLevels] + g] = el;
-    node_hi[info.offset[kInBlockLevels] + g] = eh;
-    node_body[info.offset[kInBlockLevels] + g] = eb;
+    T.node_lo[info.offset[kInBlockLevels] + g] = el;
+    T.node_hi[info.offset[kInBlockLevels] + g] = eh;
+    T.node_body[info.offset[kInBlockLevels] + g] = eb;
   }
   __syncthreads();
   for (int L = kInBlockLevels + 1; L <= info.top; ++L) {
@@ -266,18 +305,24 @@ __global__ void __launch_bounds__(1024) reduce_top_kernel(float4* __restrict__ n
       if (g < info.count[L]) {
         for (int c = 0; c < kFan; ++c) {
           int k = info.offset[L - 1] + g * kFan + c;
-          box_merge(l, h, node_lo[k], node_hi[k]);
-          int2 cb = node_body[k];
+          box_merge(l, h, T.node_lo[k], T.node_hi[k]);
+          int2 cb = T.node_body[k];
           r2.x = min(r2.x, cb.x);
           r2.y = max(r2.y, cb.y);
         }
       }
-      node_lo[info.offset[L] + g] = l;
-      node_hi[info.offset[L] + g] = h;
-      node_body[info.offset[L] + g] = r2;
+      T.node_lo[info.offset[L] + g] = l;
+      T.node_hi[info.offset[L] + g] = h;
+      T.node_body[info.offset[L] + g] = r2;
     }
     __syncthreads();
   }
+}
+
+// Workgroup j reduces the top levels of job j.
+__global__ void __launch_bounds__(1024) reduce_top_kernel(TopJob T0, TopJob T1) {
+  if (blockIdx.x == 0) top_block(T0);
+  else top_block(T1);
 }
 
 // ------------------------------------------------------------------ queries ----
@@ -369,11 +414,19 @@ __device__ __forceinline__ bool accept_pair(const QueryArgs& A, const QueryPrim&
 // group's ballot of overlapping children of a node becomes one new entry.
 // Every lane of a group computes and writes the same stack values, so each
 // lane only ever reads back its own LDS writes.
+struct QuerySmem {
+  unsigned long long stack[kGroupsPerBlock][kMaskStack];
+  int level_off[kMaxLevels], level_cnt[kMaxLevels];
+  BlockStage stage;
+};
+
+// Workgroup `bid` of a query launch over A.
 template <int MODE>
-__global__ void __launch_bounds__(kQueryBlock) query_kernel(QueryArgs A) {
-  __shared__ unsigned long long stack[kGroupsPerBlock][kMaskStack];
-  __shared__ int level_off[kMaxLevels], level_cnt[kMaxLevels];
-  __shared__ BlockStage stage;
+__device__ __forceinline__ void query_block(const QueryArgs& A, int bid, QuerySmem& sm) {
+  auto& stack = sm.stack;
+  int* level_off = sm.level_off;
+  int* level_cnt = sm.level_cnt;
+  BlockStage& stage = sm.stage;
   const StageOut out{A.rec, A.ids, A.count, A.capacity};
   int tid = threadIdx.x;
   if (tid < kMaxLevels) {
@@ -384,7 +437,7 @@ __global__ void __launch_bounds__(kQueryBlock) query_kernel(QueryArgs A) {
   int grp = tid / kFan, r = tid % kFan;
   int lane = __lane_id();
   int group_shift = lane & ~(kFan - 1);  // first lane of this group in the wave
-  int slot = blockIdx.x * kGroupsPerBlock + grp;
+  int slot = bid * kGroupsPerBlock + grp;
   bool live = slot < A.n_query && A.n_leaves > 0;
   QueryPrim Q;
   Q.i = Q.q0 = Q.q1 = 0;
@@ -506,6 +559,20 @@ __global__ void __launch_bounds__(kQueryBlock) query_kernel(QueryArgs A) {
   stage_flush(stage, out);
 }
 
+template <int MODE>
+__global__ void __launch_bounds__(kQueryBlock) query_kernel(QueryArgs A) {
+  __shared__ QuerySmem sm;
+  query_block<MODE>(A, blockIdx.x, sm);
+}
+
+// Two queries in one launch: workgroups [0, blocks_a) serve A, the rest B.
+template <int MA, int MB>
+__global__ void __launch_bounds__(kQueryBlock) query_pair_kernel(QueryArgs A, QueryArgs B, int blocks_a) {
+  __shared__ QuerySmem sm;
+  if (int(blockIdx.x) < blocks_a) query_block<MA>(A, blockIdx.x, sm);
+  else query_block<MB>(B, blockIdx.x - blocks_a, sm);
+}
+
 // x1 = x - dx (_add_dx, contact.py:49-52)
 __global__ void __launch_bounds__(256) sub_kernel(int n3, const float* __restrict__ x, const float* __restrict__ dx,
                                                   float* __restrict__ out) {
@@ -580,15 +647,12 @@ void Bvh::release() {
     if (_e != hipSuccess) return _e;     \
   } while (0)
 
-hipError_t build_bvh(Bvh& B, int mode, int n, const int* prims, const int* body, const float* x0, const float* x1,
-                     float pad, hipStream_t s) {
-  B.n = n;
-  if (n <= 0) return hipSuccess;
-  if (n > (1 << 27)) return hipErrorInvalidValue;
-  // level sizes: count[0] = n leaves, count[L] = ceil(count[L-1] / 8) up to
-  // a single root; level L occupies slots[L] = 8 count[L+1] slots from
-  // offset[L] (the root level one slot)
-  LevelInfo& I = B.info;
+namespace {
+
+// Level sizes of a tree of n leaves: count[0] = n, count[L] = ceil(count[L-1]
+// / 8) up to a single root; level L occupies slots[L] = 8 count[L+1] slots
+// from offset[L] (the root level one slot). Returns the total slot count.
+long long plan_levels(LevelInfo& I, int n) {
   for (int L = 0; L < kMaxLevels; ++L) { I.count[L] = 0; I.slots[L] = 0; I.offset[L] = 0; }
   I.count[0] = n;
   int L = 0;
@@ -603,20 +667,29 @@ hipError_t build_bvh(Bvh& B, int mode, int n, const int* prims, const int* body,
     I.offset[l] = static_cast<int>(total);
     total += I.slots[l];
   }
+  return total;
+}
+
+// Buffers of one job and, when due, the Morton re-sort of its leaf order.
+hipError_t prepare_job(const BuildJob& j, const int* body, const float* x0, hipStream_t s) {
+  Bvh& B = *j.bvh;
+  const int n = j.n;
+  (void)body;
+  long long total = plan_levels(B.info, n);
   IPC_TRY(B.bounds.ensure(sizeof(int) * 8));
   IPC_TRY(B.keys.ensure(sizeof(unsigned long long) * n));
   IPC_TRY(B.keys_sorted.ensure(sizeof(unsigned long long) * n));
   IPC_TRY(B.leaf_id.ensure(sizeof(int) * n));
-  IPC_TRY(B.leaf_vid.ensure(sizeof(int4) * I.slots[0]));
+  IPC_TRY(B.leaf_vid.ensure(sizeof(int4) * B.info.slots[0]));
   IPC_TRY(B.node_lo.ensure(sizeof(float4) * total));
   IPC_TRY(B.node_hi.ensure(sizeof(float4) * total));
   IPC_TRY(B.node_body.ensure(sizeof(int2) * total));
   bool resort = B.builds_since_sort < 0 || B.builds_since_sort >= kResortEvery || B.n_sorted != n;
   if (resort) {
     hipLaunchKernelGGL(init_bounds_kernel, dim3(1), dim3(64), 0, s, B.bounds.as<int>());
-    hipLaunchKernelGGL(centre_bounds_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, mode, n, prims, x0, x1, pad,
-                       B.bounds.as<int>());
-    hipLaunchKernelGGL(morton_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, mode, n, prims, x0, x1, pad,
+    hipLaunchKernelGGL(centre_bounds_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, j.mode, n, j.prims, x0, j.x1,
+                       j.pad, B.bounds.as<int>());
+    hipLaunchKernelGGL(morton_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, j.mode, n, j.prims, x0, j.x1, j.pad,
                        B.bounds.as<int>(), B.keys.as<unsigned long long>());
     IPC_TRY(hipGetLastError());
     size_t temp_bytes = 0;
@@ -631,18 +704,75 @@ hipError_t build_bvh(Bvh& B, int mode, int n, const int* prims, const int* body,
   } else {
     ++B.builds_since_sort;
   }
-  hipLaunchKernelGGL(leaf_levels_kernel, dim3(nblocks(n, kLeafBlock)), dim3(kLeafBlock), 0, s, mode, n, prims,
-                     body, x0, x1, pad, B.keys_sorted.as<unsigned long long>(), B.leaf_id.as<int>(),
-                     B.leaf_vid.as<int4>(), B.node_lo.as<float4>(), B.node_hi.as<float4>(), B.node_body.as<int2>(),
-                     I);
-  if (I.top > kInBlockLevels)
-    hipLaunchKernelGGL(reduce_top_kernel, dim3(1), dim3(1024), 0, s, B.node_lo.as<float4>(), B.node_hi.as<float4>(),
-                       B.node_body.as<int2>(), I);
+  return hipSuccess;
+}
+
+LeafJob leaf_job(const BuildJob& j) {
+  Bvh& B = *j.bvh;
+  LeafJob L{};
+  L.mode = j.mode;
+  L.n = j.n;
+  L.prims = j.prims;
+  L.x1 = j.x1;
+  L.pad = j.pad;
+  L.keys = B.keys_sorted.as<unsigned long long>();
+  L.leaf_id = B.leaf_id.as<int>();
+  L.leaf_vid = B.leaf_vid.as<int4>();
+  L.node_lo = B.node_lo.as<float4>();
+  L.node_hi = B.node_hi.as<float4>();
+  L.node_body = B.node_body.as<int2>();
+  L.info = B.info;
+  return L;
+}
+
+TopJob top_job(const Bvh& B) {
+  return TopJob{B.node_lo.as<float4>(), B.node_hi.as<float4>(), B.node_body.as<int2>(), B.info};
+}
+
+}  // namespace
+
+hipError_t build_bvhs(const BuildJob* jobs, int n_jobs, const int* body, const float* x0, const BuildPrologue& pro,
+                      hipStream_t s) {
+  if (n_jobs < 0 || n_jobs > 2 || pro.n_zero < 0 || pro.n_zero > 4 || pro.n_range < 0 || pro.n_range > 64)
+    return hipErrorInvalidValue;
+  const BuildJob* live[2] = {nullptr, nullptr};
+  int n_live = 0;
+  for (int k = 0; k < n_jobs; ++k) {
+    jobs[k].bvh->n = jobs[k].n;
+    if (jobs[k].n <= 0) continue;
+    if (jobs[k].n > (1 << 27)) return hipErrorInvalidValue;
+    IPC_TRY(prepare_job(jobs[k], body, x0, s));
+    live[n_live++] = &jobs[k];
+  }
+  if (n_live == 0) {
+    if (pro.n_zero > 0 || pro.n_range > 0 || pro.toi) hipLaunchKernelGGL(prologue_kernel, dim3(1), dim3(64), 0, s, pro);
+    return hipGetLastError();
+  }
+  LeafJob j0 = leaf_job(*live[0]);
+  LeafJob j1 = n_live > 1 ? leaf_job(*live[1]) : j0;
+  int blocks0 = nblocks(live[0]->n, kLeafBlock);
+  int blocks1 = n_live > 1 ? nblocks(live[1]->n, kLeafBlock) : 0;
+  hipLaunchKernelGGL(leaf_levels_kernel, dim3(blocks0 + blocks1), dim3(kLeafBlock), 0, s, j0, j1, blocks0, body, x0,
+                     pro);
+  bool top0 = live[0]->bvh->info.top > kInBlockLevels;
+  bool top1 = n_live > 1 && live[1]->bvh->info.top > kInBlockLevels;
+  if (top0 || top1) {
+    TopJob t0 = top_job(*live[0]->bvh);
+    TopJob t1 = n_live > 1 ? top_job(*live[1]->bvh) : t0;
+    hipLaunchKernelGGL(reduce_top_kernel, dim3(n_live), dim3(1024), 0, s, t0, t1);
+  }
   return hipGetLastError();
 }
 
-hipError_t run_query(const QueryLaunch& q, hipStream_t s) {
-  if (q.n_query <= 0 || q.bvh->n <= 0) return hipSuccess;
+hipError_t build_bvh(Bvh& B, int mode, int n, const int* prims, const int* body, const float* x0, const float* x1,
+                     float pad, hipStream_t s) {
+  BuildJob j{&B, mode, n, prims, x1, pad};
+  return build_bvhs(&j, 1, body, x0, BuildPrologue{}, s);
+}
+
+namespace {
+
+QueryArgs query_args(const QueryLaunch& q) {
   QueryArgs A;
   A.n_query = q.n_query;
   A.edges = q.edges;
@@ -666,17 +796,45 @@ hipError_t run_query(const QueryLaunch& q, hipStream_t s) {
   A.ids = q.ids;
   A.capacity = q.capacity;
   A.count = q.count;
-  dim3 grid(nblocks(q.n_query, kGroupsPerBlock)), block(kQueryBlock);
-  switch (q.mode) {
-    case Q_DCD_EE: hipLaunchKernelGGL(query_kernel<Q_DCD_EE>, grid, block, 0, s, A); break;
-    case Q_DCD_PT: hipLaunchKernelGGL(query_kernel<Q_DCD_PT>, grid, block, 0, s, A); break;
-    case Q_CCD_EE: hipLaunchKernelGGL(query_kernel<Q_CCD_EE>, grid, block, 0, s, A); break;
-    case Q_CCD_PT: hipLaunchKernelGGL(query_kernel<Q_CCD_PT>, grid, block, 0, s, A); break;
-    case Q_ISECT: hipLaunchKernelGGL(query_kernel<Q_ISECT>, grid, block, 0, s, A); break;
-    default: return hipErrorInvalidValue;
+  return A;
+}
+
+}  // namespace
+
+hipError_t run_queries(const QueryLaunch* q, int n_q, hipStream_t s) {
+  const QueryLaunch* live[2] = {nullptr, nullptr};
+  int n_live = 0;
+  for (int k = 0; k < n_q && k < 2; ++k)
+    if (q[k].n_query > 0 && q[k].bvh->n > 0) live[n_live++] = &q[k];
+  if (n_live == 0) return hipSuccess;
+  dim3 block(kQueryBlock);
+  if (n_live == 1) {
+    QueryArgs A = query_args(*live[0]);
+    dim3 grid(nblocks(live[0]->n_query, kGroupsPerBlock));
+    switch (live[0]->mode) {
+      case Q_DCD_EE: hipLaunchKernelGGL(query_kernel<Q_DCD_EE>, grid, block, 0, s, A); break;
+      case Q_DCD_PT: hipLaunchKernelGGL(query_kernel<Q_DCD_PT>, grid, block, 0, s, A); break;
+      case Q_CCD_EE: hipLaunchKernelGGL(query_kernel<Q_CCD_EE>, grid, block, 0, s, A); break;
+      case Q_CCD_PT: hipLaunchKernelGGL(query_kernel<Q_CCD_PT>, grid, block, 0, s, A); break;
+      case Q_ISECT: hipLaunchKernelGGL(query_kernel<Q_ISECT>, grid, block, 0, s, A); break;
+      default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
   }
+  // the two halves of a detection or CCD call: EE then PT
+  QueryArgs A = query_args(*live[0]), B = query_args(*live[1]);
+  int blocks_a = nblocks(live[0]->n_query, kGroupsPerBlock), blocks_b = nblocks(live[1]->n_query, kGroupsPerBlock);
+  dim3 grid(blocks_a + blocks_b);
+  if (live[0]->mode == Q_DCD_EE && live[1]->mode == Q_DCD_PT)
+    hipLaunchKernelGGL((query_pair_kernel<Q_DCD_EE, Q_DCD_PT>), grid, block, 0, s, A, B, blocks_a);
+  else if (live[0]->mode == Q_CCD_EE && live[1]->mode == Q_CCD_PT)
+    hipLaunchKernelGGL((query_pair_kernel<Q_CCD_EE, Q_CCD_PT>), grid, block, 0, s, A, B, blocks_a);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
+
+hipError_t run_query(const QueryLaunch& q, hipStream_t s) { return run_queries(&q, 1, s); }
 
 hipError_t launch_sub(int n3, const float* x, const float* dx, float* out, hipStream_t s) {
   if (n3 <= 0) return hipSuccess;

@@ -54,6 +54,43 @@ constexpr int kResortEvery = 16;
 enum BoxMode { BOX_EDGE = 0, BOX_TRI = 1, BOX_SWEPT_EDGE = 2, BOX_SWEPT_TRI = 3 };
 enum QueryMode { Q_DCD_EE = 0, Q_DCD_PT = 1, Q_CCD_EE = 2, Q_CCD_PT = 3, Q_ISECT = 4 };
 
+// One tree of a build launch: n primitives `prims` (2 or 3 vertex ids each
+// by mode) boxed at x0 (and x1 for the swept modes), +- pad.
+struct BuildJob {
+  Bvh* bvh;
+  int mode;  // BoxMode
+  int n;
+  const int* prims;
+  const float* x1;
+  float pad;
+};
+
+// Device writes folded into the first workgroup of a call's first launch,
+// for the queries and TOI kernels that follow on the same stream:
+// *zero[k] = 0 for k < n_zero, range[0 .. n_range) = 0, and *toi = toi_init
+// when toi is set.
+struct BuildPrologue {
+  int* zero[4] = {nullptr, nullptr, nullptr, nullptr};
+  int n_zero = 0;
+  int* range = nullptr;
+  int n_range = 0;  // <= 64
+  unsigned long long* toi = nullptr;
+  unsigned long long toi_init = 0;
+};
+
+// Applied by the first 64 threads of the launch's workgroup 0.
+__device__ __forceinline__ void run_prologue(const BuildPrologue& p) {
+  if (blockIdx.x != 0) return;
+  int t = threadIdx.x;
+  if (t < p.n_zero) *p.zero[t] = 0;
+  if (t < p.n_range) p.range[t] = 0;
+  if (t == 0 && p.toi) *p.toi = p.toi_init;
+}
+
+// Builds up to two trees with one leaf launch and one top-level launch
+// (plus a Morton re-sort for a tree whose leaf order is due).
+hipError_t build_bvhs(const BuildJob* jobs, int n_jobs, const int* body, const float* x0, const BuildPrologue& pro,
+                      hipStream_t s);
 hipError_t build_bvh(Bvh& B, int mode, int n, const int* prims, const int* body, const float* x0, const float* x1,
                      float pad, hipStream_t s);
 
@@ -79,6 +116,9 @@ struct QueryLaunch {
 };
 
 hipError_t run_query(const QueryLaunch& q, hipStream_t s);
+// Up to two queries in one launch (the EE and PT halves of a DCD or CCD
+// call: modes Q_DCD_EE + Q_DCD_PT or Q_CCD_EE + Q_CCD_PT, in that order).
+hipError_t run_queries(const QueryLaunch* q, int n_q, hipStream_t s);
 hipError_t launch_sub(int n3, const float* x, const float* dx, float* out, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t s);
 hipError_t launch_zero_i32(int* p, int n, hipStream_t s);

@@ -259,6 +259,7 @@ __device__ __forceinline__ void affine_box(const double A[9], const double p[3],
 }
 
 __global__ void __launch_bounds__(64) body_frames_kernel(FixedStep f) {
+  run_prologue(f.pro);
   int b = blockIdx.x * 64 + threadIdx.x;
   if (b >= f.n_bodies) return;
   double A0[9], p0[3];
@@ -303,7 +304,7 @@ __global__ void __launch_bounds__(64) body_frames_kernel(FixedStep f) {
       F.motion[i] = fabs(dc) + fabs(D[3 * i]) * e[0] + fabs(D[3 * i + 1]) * e[1] + fabs(D[3 * i + 2]) * e[2];
     }
   }
-  if (!(mindet >= 1.0e-8)) *f.singular = 1;
+  if (!(mindet >= 1.0e-8)) *f.singular = f.gen;
   // fp32 positions agree with (A, p) only to rounding: widen by 1e-6 relative
   for (int i = 0; i < 3; ++i) {
     double slack = 1.0e-6 * fmax(fabs(lo[i]), fabs(hi[i]));
@@ -362,13 +363,14 @@ __device__ __forceinline__ bool box_reaches(V3<float> lo, V3<float> hi, float4 b
   return !(lo.x > bh.x || lo.y > bh.y || lo.z > bh.z || bl.x > hi.x || bl.y > hi.y || bl.z > hi.z);
 }
 
-__global__ void __launch_bounds__(256) edge_tasks_kernel(TaskLaunch T) {
-  if (*T.singular) return;
+// Workgroup `bid` of `nblk` over the (pair, edge of the source body) space.
+__device__ __forceinline__ void edge_tasks_block(const TaskLaunch& T, int bid, int nblk) {
+  if (*T.singular == T.gen) return;
   int np = *T.n_pairs;
   np = np < T.pair_cap ? np : T.pair_cap;
   long long total = (long long)np * T.max_edges;
-  long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long t0 = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); t0 < total; t0 += stride) {
+  long long stride = (long long)nblk * blockDim.x;
+  for (long long t0 = (long long)bid * blockDim.x + (threadIdx.x & ~63); t0 < total; t0 += stride) {
     long long t = t0 + __lane_id();
     bool ok = false;
     int e = 0, tgt = 0;
@@ -400,14 +402,15 @@ __global__ void __launch_bounds__(256) edge_tasks_kernel(TaskLaunch T) {
   }
 }
 
-__global__ void __launch_bounds__(256) vertex_tasks_kernel(TaskLaunch T) {
-  if (*T.singular) return;
+// Workgroup `bid` of `nblk` over the (pair, direction, vertex) space.
+__device__ __forceinline__ void vertex_tasks_block(const TaskLaunch& T, int bid, int nblk) {
+  if (*T.singular == T.gen) return;
   int np = *T.n_pairs;
   np = np < T.pair_cap ? np : T.pair_cap;
   long long per = 2ll * T.max_verts;
   long long total = (long long)np * per;
-  long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long t0 = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); t0 < total; t0 += stride) {
+  long long stride = (long long)nblk * blockDim.x;
+  for (long long t0 = (long long)bid * blockDim.x + (threadIdx.x & ~63); t0 < total; t0 += stride) {
     long long t = t0 + __lane_id();
     bool ok = false;
     int v = 0, tgt = 0;
@@ -435,18 +438,20 @@ __global__ void __launch_bounds__(256) vertex_tasks_kernel(TaskLaunch T) {
   }
 }
 
+// Edge tasks in workgroups [0, blocks_ee), vertex tasks in the rest.
+__global__ void __launch_bounds__(256) tasks_kernel(TaskLaunch T, int blocks_ee) {
+  if (int(blockIdx.x) < blocks_ee) edge_tasks_block(T, blockIdx.x, blocks_ee);
+  else vertex_tasks_block(T, blockIdx.x - blocks_ee, gridDim.x - blocks_ee);
+}
+
 hipError_t launch_fixed_tasks(const TaskLaunch& t, hipStream_t s) {
   // grids sized by the largest possible task space; the kernels read the
   // device pair count and stride over what exists
   long long ee = (long long)t.pair_cap * t.max_edges, pt = 2ll * t.pair_cap * t.max_verts;
-  if (ee > 0 && t.tasks_ee) {
-    long long blocks = std::min<long long>((ee + 255) / 256, 4096);
-    hipLaunchKernelGGL(edge_tasks_kernel, dim3(unsigned(blocks)), dim3(256), 0, s, t);
-  }
-  if (pt > 0 && t.tasks_pt) {
-    long long blocks = std::min<long long>((pt + 255) / 256, 4096);
-    hipLaunchKernelGGL(vertex_tasks_kernel, dim3(unsigned(blocks)), dim3(256), 0, s, t);
-  }
+  long long blocks_ee = (ee > 0 && t.tasks_ee) ? std::min<long long>((ee + 255) / 256, 4096) : 0;
+  long long blocks_pt = (pt > 0 && t.tasks_pt) ? std::min<long long>((pt + 255) / 256, 4096) : 0;
+  if (blocks_ee + blocks_pt == 0) return hipSuccess;
+  hipLaunchKernelGGL(tasks_kernel, dim3(unsigned(blocks_ee + blocks_pt)), dim3(256), 0, s, t, int(blocks_ee));
   return hipGetLastError();
 }
 
@@ -511,10 +516,16 @@ __device__ __forceinline__ bool fixed_accept(const FixedArgs& A, const TaskPrim&
 // in bounds). The grid is persistent (tasks are counted on the device) and
 // each wave strides over groups of eight tasks, so every lane of a wave runs
 // the same trip count.
+struct FixedSmem {
+  unsigned long long stack[kGroupsPerBlock][kMaskStack];
+  BlockStage stage;
+};
+
+// Workgroup `bid` of the `nblk` serving A.
 template <int MODE>
-__global__ void __launch_bounds__(kQueryBlock) fixed_query_kernel(FixedArgs A) {
-  __shared__ unsigned long long stack[kGroupsPerBlock][kMaskStack];
-  __shared__ BlockStage stage;
+__device__ __forceinline__ void fixed_query_block(const FixedArgs& A, int bid, int nblk, FixedSmem& sm) {
+  auto& stack = sm.stack;
+  BlockStage& stage = sm.stage;
   const StageOut out{A.rec, A.ids, A.count, A.capacity};
   stage_begin(stage);
   const int tid = threadIdx.x;
@@ -525,8 +536,8 @@ __global__ void __launch_bounds__(kQueryBlock) fixed_query_kernel(FixedArgs A) {
   const int groups_per_wave = 64 / kFan;
   int n = *A.task_count;
   n = n < A.task_cap ? n : A.task_cap;
-  const int waves_total = gridDim.x * (kQueryBlock / 64);
-  for (int wbase = (blockIdx.x * (kQueryBlock / 64) + tid / 64) * groups_per_wave; wbase < n;
+  const int waves_total = nblk * (kQueryBlock / 64);
+  for (int wbase = (bid * (kQueryBlock / 64) + tid / 64) * groups_per_wave; wbase < n;
        wbase += waves_total * groups_per_wave) {
     int slot = wbase + gw;
     bool live = slot < n;
@@ -634,8 +645,23 @@ __global__ void __launch_bounds__(kQueryBlock) fixed_query_kernel(FixedArgs A) {
   stage_flush(stage, out);
 }
 
-hipError_t launch_fixed_query(const FixedQuery& q, hipStream_t s) {
-  if (q.task_cap <= 0 || q.forest->n_leaves == 0) return hipSuccess;
+template <int MODE>
+__global__ void __launch_bounds__(kQueryBlock) fixed_query_kernel(FixedArgs A) {
+  __shared__ FixedSmem sm;
+  fixed_query_block<MODE>(A, blockIdx.x, gridDim.x, sm);
+}
+
+// Workgroups [0, blocks_a) serve A, the rest B.
+template <int MA, int MB>
+__global__ void __launch_bounds__(kQueryBlock) fixed_query_pair_kernel(FixedArgs A, FixedArgs B, int blocks_a) {
+  __shared__ FixedSmem sm;
+  if (int(blockIdx.x) < blocks_a) fixed_query_block<MA>(A, blockIdx.x, blocks_a, sm);
+  else fixed_query_block<MB>(B, blockIdx.x - blocks_a, gridDim.x - blocks_a, sm);
+}
+
+namespace {
+
+FixedArgs fixed_args(const FixedQuery& q) {
   FixedArgs A;
   A.tasks = q.tasks;
   A.task_count = q.task_count;
@@ -655,14 +681,45 @@ hipError_t launch_fixed_query(const FixedQuery& q, hipStream_t s) {
   A.ids = q.ids;
   A.capacity = q.capacity;
   A.count = q.count;
+  return A;
+}
+
+// Persistent grid of a query: one workgroup per 32 tasks of capacity, at
+// most 2048.
+int fixed_blocks(const FixedQuery& q) {
   int blocks = (q.task_cap + kGroupsPerBlock - 1) / kGroupsPerBlock;
-  if (blocks > 2048) blocks = 2048;
+  return blocks > 2048 ? 2048 : blocks;
+}
+
+bool fixed_live(const FixedQuery& q) { return q.task_cap > 0 && q.forest->n_leaves > 0; }
+
+}  // namespace
+
+hipError_t launch_fixed_query(const FixedQuery& q, hipStream_t s) {
+  if (!fixed_live(q)) return hipSuccess;
+  FixedArgs A = fixed_args(q);
+  int blocks = fixed_blocks(q);
   switch (q.mode) {
     case Q_DCD_EE: hipLaunchKernelGGL(fixed_query_kernel<Q_DCD_EE>, dim3(blocks), dim3(kQueryBlock), 0, s, A); break;
     case Q_DCD_PT: hipLaunchKernelGGL(fixed_query_kernel<Q_DCD_PT>, dim3(blocks), dim3(kQueryBlock), 0, s, A); break;
     case Q_CCD_EE: hipLaunchKernelGGL(fixed_query_kernel<Q_CCD_EE>, dim3(blocks), dim3(kQueryBlock), 0, s, A); break;
     default: hipLaunchKernelGGL(fixed_query_kernel<Q_CCD_PT>, dim3(blocks), dim3(kQueryBlock), 0, s, A); break;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_fixed_queries(const FixedQuery& ee, const FixedQuery& pt, hipStream_t s) {
+  if (!fixed_live(ee)) return launch_fixed_query(pt, s);
+  if (!fixed_live(pt)) return launch_fixed_query(ee, s);
+  FixedArgs A = fixed_args(ee), B = fixed_args(pt);
+  int blocks_a = fixed_blocks(ee), blocks_b = fixed_blocks(pt);
+  dim3 grid(blocks_a + blocks_b), block(kQueryBlock);
+  if (ee.mode == Q_DCD_EE && pt.mode == Q_DCD_PT)
+    hipLaunchKernelGGL((fixed_query_pair_kernel<Q_DCD_EE, Q_DCD_PT>), grid, block, 0, s, A, B, blocks_a);
+  else if (ee.mode == Q_CCD_EE && pt.mode == Q_CCD_PT)
+    hipLaunchKernelGGL((fixed_query_pair_kernel<Q_CCD_EE, Q_CCD_PT>), grid, block, 0, s, A, B, blocks_a);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

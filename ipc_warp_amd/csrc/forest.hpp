@@ -58,7 +58,12 @@ struct FixedStep {
   float4* world_lo;  // out: world (DCD) or swept (CCD) body boxes
   float4* world_hi;
   BodyFrame* frames;
-  int* singular;     // set to 1 when a body's |det A| (either end for CCD) < 1e-8
+  // Set to `gen` when a body's |det A| (either end for CCD) < 1e-8. The
+  // caller gives each call a new generation instead of clearing the flag,
+  // which the same launch may set.
+  int* singular;
+  int gen;
+  BuildPrologue pro;  // run by the frames launch (the call's first)
 };
 hipError_t launch_body_frames(const FixedStep& f, hipStream_t s);
 
@@ -70,7 +75,8 @@ struct TaskLaunch {
   const int2* pairs;
   const int* n_pairs;
   int pair_cap;
-  const int* singular;
+  const int* singular;  // == gen: this call fell back, no tasks
+  int gen;
   const float* x0;
   const float* x1;
   const float4* world_lo;
@@ -114,5 +120,8 @@ struct FixedQuery {
   int* count;
 };
 hipError_t launch_fixed_query(const FixedQuery& q, hipStream_t s);
+// The EE and PT queries of one call (modes Q_DCD_EE + Q_DCD_PT or
+// Q_CCD_EE + Q_CCD_PT) in one launch.
+hipError_t launch_fixed_queries(const FixedQuery& ee, const FixedQuery& pt, hipStream_t s);
 
 }  // namespace ipcd

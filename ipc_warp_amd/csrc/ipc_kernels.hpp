@@ -16,6 +16,11 @@ hipError_t launch_collision_time(bool ee, const float* x0, const float* x1, int 
 
 hipError_t launch_collision_time_counted(bool ee, const float* x0, const float* x1, int nv, const int* quads,
                                          const int* n_dev, int capacity, unsigned long long* toi_min, hipStream_t s);
+// Both candidate lists of a CCD call (counts on the device, capacities
+// cap_ee / cap_pt) in one launch, folded into *toi_min.
+hipError_t launch_collision_time_pair(const float* x0, const float* x1, int nv, const int* quads_ee, const int* n_ee,
+                                      int cap_ee, const int* quads_pt, const int* n_pt, int cap_pt,
+                                      unsigned long long* toi_min, hipStream_t s);
 
 hipError_t launch_record_intersections(bool ee, const float* x0, const float* x1, int nv, const int* quads,
                                       const int* ids, const int* n_dev, int cap_in, int* rec_out, int* ids_out,

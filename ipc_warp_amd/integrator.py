@@ -13,7 +13,11 @@ counts with and without the learned warm start can be compared:
   decreases and the end state has no edge-triangle crossing (at most 64
   halvings, alpha = 0 ends the solve);
 * convergence: the relative decrease of E between iterations falls below
-  `tol` (1e-6). The reference's own test max|dx| <= tol_v h is reported too.
+  `tol` (1e-6), or the line search finds no decrease while the Newton
+  decrement is already within it (the full step's predicted decrease
+  g.dx / 2 <= tol |E|: the search then fails at the rounding floor of the
+  fp32 contact terms, not short of the minimum). The reference's own test
+  max|dx| <= tol_v h is reported too.
 
 The elasticity and the linear solve are harness code on PyTorch tensors;
 they are not part of the contact path.
@@ -266,6 +270,7 @@ class ContactIntegrator:
             stats.max_dx_over_h = float(dx.abs().max()) / self.h
             if alpha == 0.0:
                 stats.line_search_failures += 1
+                stats.converged = 0.5 * float(torch.dot(g, dx.reshape(-1))) <= self.tol * abs(E0)
                 break
             stats.energies.append(E1)
             rel = (E0 - E1) / max(abs(E1), 1e-300)

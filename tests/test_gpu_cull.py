@@ -122,16 +122,31 @@ def test_degenerate_soups(dev, oracle):
 
 
 def test_kernel_timing_hooks(dev, oracle):
+    from ipc_warp_amd._lib import check, lib
+
     soup, dx = _ccd_setup(16)
     S = _solver(dev, soup)
+    dxt = torch.from_numpy(dx).to(dev)
     S.set_kernel_timing(True)
     S.kernel_times()
+    # the combined calls launch each stage once for both halves (EE kinds)
     S.detect_collision()
-    S.collision_free_step(torch.from_numpy(dx).to(dev))
+    S.collision_free_step(dxt)
+    t = S.kernel_times()
+    for k in ("build_edge", "query_dcd_ee", "query_ccd_ee", "toi_ee"):
+        ms, n = t[k]
+        assert n >= 1 and ms > 0.0, k
+    for k in ("build_tri", "query_dcd_pt", "query_ccd_pt", "toi_pt"):
+        assert t[k][1] == 0, k
+    # the single-kind calls record their own kinds
+    S.new_intersections(S.x, (S.x - dxt).contiguous())
+    c = S.contacts_pt
+    check(lib().ipc_detect_collision_pt(S._ctx, S.x.data_ptr(), float(S.thickness), float(S.buffer), 1,
+                                        c.list.data_ptr(), c.e0e1.data_ptr(), int(c.capacity), c.cnt.data_ptr(),
+                                        S._stream()), "detect_collision_pt")
     t = S.kernel_times()
     S.set_kernel_timing(False)
-    for k in ("build_edge", "build_tri", "query_dcd_ee", "query_dcd_pt", "query_ccd_ee", "query_ccd_pt",
-              "toi_ee", "toi_pt"):
+    for k in ("build_edge", "build_tri", "query_dcd_pt", "query_ccd_ee", "query_ccd_pt", "toi_ee", "toi_pt"):
         ms, n = t[k]
         assert n >= 1 and ms > 0.0, k
     # reading resets the counters
