@@ -108,14 +108,24 @@ __device__ __forceinline__ bool pair_pt(const float* __restrict__ x, int4 c, flo
 // (dynamic_contacts.py:196-207,236-246); inactive contacts get zero blocks
 // with row = col = 0, which is what the reference's zero-initialised
 // triplet arrays hold for them. The force is atomically added into b.
+// Contact of this lane when each wave takes `lpw` contacts (lanes lpw..63
+// idle; -1 there). A short list is then spread over more SIMDs, and each
+// wave meets fewer of the feature dispatch's branch paths, which a wave runs
+// one after another.
+__device__ __forceinline__ int contact_slot(int lpw) {
+  int lane = threadIdx.x & 63;
+  int wave = int((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  return lane < lpw ? wave * lpw + lane : -1;
+}
+
 template <bool EE>
 __global__ void __launch_bounds__(256) contact_hessian_kernel(const float* __restrict__ x, const float* __restrict__ xcs,
                                                               int nv, const int4* __restrict__ contacts, int n, float l0,
                                                               double kappa, int offset, int* __restrict__ rows,
                                                               int* __restrict__ cols, float* __restrict__ vals,
-                                                              float* __restrict__ b) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+                                                              float* __restrict__ b, int lpw) {
+  int i = contact_slot(lpw);
+  if (i < 0 || i >= n) return;
   int4 c = contacts[i];
   CompressedHessian C;
   float force[12];
@@ -173,11 +183,11 @@ template <bool EE>
 __global__ void __launch_bounds__(256) contact_energy_kernel(const float* __restrict__ x,
                                                              const float* __restrict__ xcs, int nv,
                                                              const int4* __restrict__ contacts, int n, float l0,
-                                                             double kappa, double* __restrict__ energy) {
+                                                             double kappa, double* __restrict__ energy, int lpw) {
   __shared__ double partial[4];
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = contact_slot(lpw);
   double e = 0.0;
-  if (i < n) {
+  if (i >= 0 && i < n) {
     CompressedHessian C;
     float force[12];
     double ei = 0.0;
@@ -374,17 +384,26 @@ __global__ void __launch_bounds__(256) record_intersections_kernel(
 // --------------------------------------------------------------- launchers ----
 static inline int blocks_for(long long n, int bs) { return int((n + bs - 1) / bs); }
 
+// Contacts per wave: a list too short to fill the device several times over
+// is spread thinly (contact_slot); long lists run one contact per lane.
+constexpr int kSpreadBelow = 1 << 18;
+constexpr int kSpreadLanes = 16;        // Hessian (measured best of 4..64 at ~5e3 contacts)
+constexpr int kSpreadLanesEnergy = 32;  // energy (no Hessian, shorter chain)
+static inline int lanes_per_wave(int n, int spread = kSpreadLanes) { return n < kSpreadBelow ? spread : 64; }
+static inline int blocks_spread(int n, int lpw) { return blocks_for(blocks_for(n, lpw), 4); }  // 4 waves per block
+
 hipError_t launch_contact_hessian(bool ee, const float* x, const float* xcs, int nv, const int* contacts, int n, float l0,
                                   double kappa, int offset, int* rows, int* cols, float* vals, float* b,
                                   hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const int4* c4 = reinterpret_cast<const int4*>(contacts);
+  const int lpw = lanes_per_wave(n);
   if (ee)
-    hipLaunchKernelGGL(contact_hessian_kernel<true>, dim3(blocks_for(n, 256)), dim3(256), 0, s, x, xcs, nv, c4, n,
-                       l0, kappa, offset, rows, cols, vals, b);
+    hipLaunchKernelGGL(contact_hessian_kernel<true>, dim3(blocks_spread(n, lpw)), dim3(256), 0, s, x, xcs, nv, c4, n,
+                       l0, kappa, offset, rows, cols, vals, b, lpw);
   else
-    hipLaunchKernelGGL(contact_hessian_kernel<false>, dim3(blocks_for(n, 256)), dim3(256), 0, s, x, xcs, nv, c4, n,
-                       l0, kappa, offset, rows, cols, vals, b);
+    hipLaunchKernelGGL(contact_hessian_kernel<false>, dim3(blocks_spread(n, lpw)), dim3(256), 0, s, x, xcs, nv, c4,
+                       n, l0, kappa, offset, rows, cols, vals, b, lpw);
   return hipGetLastError();
 }
 
@@ -392,12 +411,13 @@ hipError_t launch_contact_energy(bool ee, const float* x, const float* xcs, int 
                                  double kappa, double* energy, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const int4* c4 = reinterpret_cast<const int4*>(contacts);
+  const int lpw = lanes_per_wave(n, kSpreadLanesEnergy);
   if (ee)
-    hipLaunchKernelGGL(contact_energy_kernel<true>, dim3(blocks_for(n, 256)), dim3(256), 0, s, x, xcs, nv, c4, n,
-                       l0, kappa, energy);
+    hipLaunchKernelGGL(contact_energy_kernel<true>, dim3(blocks_spread(n, lpw)), dim3(256), 0, s, x, xcs, nv, c4, n,
+                       l0, kappa, energy, lpw);
   else
-    hipLaunchKernelGGL(contact_energy_kernel<false>, dim3(blocks_for(n, 256)), dim3(256), 0, s, x, xcs, nv, c4, n,
-                       l0, kappa, energy);
+    hipLaunchKernelGGL(contact_energy_kernel<false>, dim3(blocks_spread(n, lpw)), dim3(256), 0, s, x, xcs, nv, c4,
+                       n, l0, kappa, energy, lpw);
   return hipGetLastError();
 }
 
