@@ -157,6 +157,7 @@ QueryLaunch dcd_ee_query(ipc_ctx* c, const float* x, float thickness, int exclud
   q.qhi = c->edge_bvh.node_hi.as<float4>();
   q.thickness = thickness;
   q.exclude_same_body = exclude_same_body;
+  q.self_query = 1;
   q.bvh = &c->edge_bvh;
   q.rec = reinterpret_cast<int4*>(contacts);
   q.ids = reinterpret_cast<int2*>(e0e1);
@@ -198,6 +199,7 @@ QueryLaunch ccd_ee_query(ipc_ctx* c, const float* x0, const float* x1, int exclu
   q.qlo = c->edge_bvh.node_lo.as<float4>();
   q.qhi = c->edge_bvh.node_hi.as<float4>();
   q.exclude_same_body = exclude_same_body;
+  q.self_query = 1;
   q.bvh = &c->edge_bvh;
   q.rec = reinterpret_cast<int4*>(quads);
   q.ids = reinterpret_cast<int2*>(ids);

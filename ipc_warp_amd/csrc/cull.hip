@@ -352,6 +352,9 @@ struct QueryArgs {
   float pad;          // mode 1: 2 (th + buf); mode 3: CCD padding
   float thickness;
   int exclude_same_body;
+  // modes 0, 2: query slot k is leaf k of the same tree; each unordered pair
+  // is then taken by the query at the lower leaf position only
+  int self_query;
   // hierarchy (every level in the node arrays, leaves from slot 0)
   int n_leaves;
   LevelInfo info;
@@ -375,8 +378,13 @@ struct QueryPrim {
   V3<float> p0, p1;
 };
 
+// rec and ids receive the pair's record and primitive ids. An EE pair is
+// written as the reference orders it, the lower edge id first (its distance
+// is evaluated in that argument order too), whichever of the two is the
+// query.
 template <int MODE>
-__device__ __forceinline__ bool accept_pair(const QueryArgs& A, const QueryPrim& Q, int4 lv, int4& rec) {
+__device__ __forceinline__ bool accept_pair(const QueryArgs& A, const QueryPrim& Q, int4 lv, int4& rec, int2& ids) {
+  ids = make_int2(Q.i, lv.w);
   if (MODE == Q_ISECT) {
     // edge i against triangle j: a crossing, not sharing a vertex
     int a0 = Q.q0, a1 = Q.q1, t0 = lv.x, t1 = lv.y, t2 = lv.z;
@@ -386,12 +394,17 @@ __device__ __forceinline__ bool accept_pair(const QueryArgs& A, const QueryPrim&
                                  to_d(ldx(A.x0, t2)));
   }
   if (MODE == 0 || MODE == 2) {
-    if (!(Q.i < lv.w)) return false;
-    int a0 = Q.q0, a1 = Q.q1, b0 = lv.x, b1 = lv.y;
+    const int j = lv.w;
+    const bool qlow = Q.i < j;  // the query edge has the lower id
+    if (!A.self_query && !qlow) return false;
+    int a0 = qlow ? Q.q0 : lv.x, a1 = qlow ? Q.q1 : lv.y, b0 = qlow ? lv.x : Q.q0, b1 = qlow ? lv.y : Q.q1;
     if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return false;
     rec = make_int4(a0, a1, b0, b1);
+    ids = qlow ? make_int2(Q.i, j) : make_int2(j, Q.i);
     if (MODE == 2) return true;
-    EdgeClosest ec = closest_point_edge_edge(Q.p0, Q.p1, ldx(A.x0, b0), ldx(A.x0, b1), 1.0e-6f);
+    V3<float> l0 = ldx(A.x0, lv.x), l1 = ldx(A.x0, lv.y);
+    EdgeClosest ec = closest_point_edge_edge(qlow ? Q.p0 : l0, qlow ? Q.p1 : l1, qlow ? l0 : Q.p0, qlow ? l1 : Q.p1,
+                                             1.0e-6f);
     return ec.dist < A.thickness * 2.0f;
   }
   int i = Q.i, t0 = lv.x, t1 = lv.y, t2 = lv.z;
@@ -439,6 +452,7 @@ __device__ __forceinline__ void query_block(const QueryArgs& A, int bid, QuerySm
   int group_shift = lane & ~(kFan - 1);  // first lane of this group in the wave
   int slot = bid * kGroupsPerBlock + grp;
   bool live = slot < A.n_query && A.n_leaves > 0;
+  const bool self = (MODE == 0 || MODE == 2) && A.self_query;
   QueryPrim Q;
   Q.i = Q.q0 = Q.q1 = 0;
   Q.p0 = Q.p1 = mk(0.f, 0.f, 0.f);
@@ -529,12 +543,16 @@ __device__ __forceinline__ void query_block(const QueryArgs& A, int bid, QuerySm
     const bool internal = L >= 2;
     const int child_off = level_off[L >= 1 ? L - 1 : 0], child_cnt = level_cnt[L >= 1 ? L - 1 : 0];
     const int grand_off = level_off[internal ? L - 2 : 0];
+    // Self-queries skip children whose leaves all sit at positions <= the
+    // query's own (one past the child's last leaf is (rel + 1) 8^(L-1)).
+    const int shift = L >= 1 ? 3 * (L - 1) : 0;
+    auto beyond = [&](int rel) { return !self || ((long long)(rel + 1) << shift) > (long long)slot + 1; };
     // (bitwise, so that no load is sunk behind a branch on another term)
     bool hit[kPopNodes];
 #pragma unroll
     for (int p = 0; p < kPopNodes; ++p)
       hit[p] = (cpos[p] >= 0) & (at[p] - child_off < child_cnt) & overlaps(qlo, qhi, clo[p], chi[p]) &
-               !((qbody >= 0) & (cbr[p].x == qbody) & (cbr[p].y == qbody));
+               !((qbody >= 0) & (cbr[p].x == qbody) & (cbr[p].y == qbody)) & beyond(at[p] - child_off);
     // internal level: one entry per node with overlapping children, the
     // first node's pushed last so that it is visited first
 #pragma unroll
@@ -551,8 +569,9 @@ __device__ __forceinline__ void query_block(const QueryArgs& A, int bid, QuerySm
     for (int p = 0; p < kPopNodes; ++p) {
       bool want = false;
       int4 rec = make_int4(0, 0, 0, 0);
-      if (L == 1 && hit[p]) want = accept_pair<MODE>(A, Q, cvid[p], rec);
-      stage_append(stage, out, want, rec, make_int2(Q.i, cvid[p].w));
+      int2 ids = make_int2(Q.i, cvid[p].w);
+      if (L == 1 && hit[p]) want = accept_pair<MODE>(A, Q, cvid[p], rec, ids);
+      stage_append(stage, out, want, rec, ids);
     }
     if (!__any(sp > 0)) break;
   }
@@ -786,6 +805,7 @@ QueryArgs query_args(const QueryLaunch& q) {
   A.pad = q.pad;
   A.thickness = q.thickness;
   A.exclude_same_body = q.exclude_same_body;
+  A.self_query = q.self_query;
   A.n_leaves = q.bvh->n;
   A.info = q.bvh->info;
   A.node_lo = q.bvh->node_lo.as<float4>();

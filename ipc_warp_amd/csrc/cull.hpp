@@ -108,6 +108,7 @@ struct QueryLaunch {
   float pad;
   float thickness;
   int exclude_same_body;
+  int self_query;  // EE: slot k is leaf k of bvh (order = its leaf_id, qlo/qhi = its leaf boxes)
   const Bvh* bvh;
   int4* rec;
   int2* ids;

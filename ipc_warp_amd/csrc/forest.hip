@@ -363,8 +363,10 @@ __device__ __forceinline__ bool box_reaches(V3<float> lo, V3<float> hi, float4 b
   return !(lo.x > bh.x || lo.y > bh.y || lo.z > bh.z || bl.x > hi.x || bl.y > hi.y || bl.z > hi.z);
 }
 
-// Workgroup `bid` of `nblk` over the (pair, edge of the source body) space.
-__device__ __forceinline__ void edge_tasks_block(const TaskLaunch& T, int bid, int nblk) {
+// Workgroup `bid` of `nblk` over the (pair, edge of the source body) space;
+// tasks are staged in `st` (the caller flushes it).
+__device__ __forceinline__ void edge_tasks_block(const TaskLaunch& T, int bid, int nblk, BlockStage& st) {
+  const StageOut out{nullptr, T.tasks_ee, T.count_ee, T.cap_ee};
   if (*T.singular == T.gen) return;
   int np = *T.n_pairs;
   np = np < T.pair_cap ? np : T.pair_cap;
@@ -397,13 +399,13 @@ __device__ __forceinline__ void edge_tasks_block(const TaskLaunch& T, int bid, i
         ok = box_reaches(lo - pv, hi + pv, T.world_lo[tgt], T.world_hi[tgt]);
       }
     }
-    int out = wave_append(ok, T.count_ee);
-    if (out >= 0 && out < T.cap_ee) T.tasks_ee[out] = make_int2(e, tgt);
+    stage_append(st, out, ok, make_int4(0, 0, 0, 0), make_int2(e, tgt));
   }
 }
 
 // Workgroup `bid` of `nblk` over the (pair, direction, vertex) space.
-__device__ __forceinline__ void vertex_tasks_block(const TaskLaunch& T, int bid, int nblk) {
+__device__ __forceinline__ void vertex_tasks_block(const TaskLaunch& T, int bid, int nblk, BlockStage& st) {
+  const StageOut out{nullptr, T.tasks_pt, T.count_pt, T.cap_pt};
   if (*T.singular == T.gen) return;
   int np = *T.n_pairs;
   np = np < T.pair_cap ? np : T.pair_cap;
@@ -433,15 +435,23 @@ __device__ __forceinline__ void vertex_tasks_block(const TaskLaunch& T, int bid,
         ok = box_reaches(lo - pv, hi + pv, T.world_lo[tgt], T.world_hi[tgt]);
       }
     }
-    int out = wave_append(ok, T.count_pt);
-    if (out >= 0 && out < T.cap_pt) T.tasks_pt[out] = make_int2(v, tgt);
+    stage_append(st, out, ok, make_int4(0, 0, 0, 0), make_int2(v, tgt));
   }
 }
 
-// Edge tasks in workgroups [0, blocks_ee), vertex tasks in the rest.
+// Edge tasks in workgroups [0, blocks_ee), vertex tasks in the rest. Each
+// workgroup stages its tasks in LDS and reserves its output range with one
+// global atomicAdd (the two counters share a cache line).
 __global__ void __launch_bounds__(256) tasks_kernel(TaskLaunch T, int blocks_ee) {
-  if (int(blockIdx.x) < blocks_ee) edge_tasks_block(T, blockIdx.x, blocks_ee);
-  else vertex_tasks_block(T, blockIdx.x - blocks_ee, gridDim.x - blocks_ee);
+  __shared__ BlockStage st;
+  stage_begin(st);
+  if (int(blockIdx.x) < blocks_ee) {
+    edge_tasks_block(T, blockIdx.x, blocks_ee, st);
+    stage_flush(st, StageOut{nullptr, T.tasks_ee, T.count_ee, T.cap_ee});
+  } else {
+    vertex_tasks_block(T, blockIdx.x - blocks_ee, gridDim.x - blocks_ee, st);
+    stage_flush(st, StageOut{nullptr, T.tasks_pt, T.count_pt, T.cap_pt});
+  }
 }
 
 hipError_t launch_fixed_tasks(const TaskLaunch& t, hipStream_t s) {
