@@ -17,6 +17,7 @@ struct ipc_ctx {
   int nv = 0, ne = 0, nf = 0;
   DevBuf edges, tris, body;
   Bvh edge_bvh, tri_bvh;
+  PointOrder vorder;       // Morton order of the vertices: the PT query order
   DevBuf x1;               // CCD end positions x - dx
   DevBuf cand_ee, cand_pt; // int4 candidate quadruples
   DevBuf cand_ids_ee, cand_ids_pt;  // int2 candidate primitive ids (new_intersections)
@@ -128,7 +129,7 @@ int ipc_ctx_destroy(ipc_ctx* c) {
   (void)hipDeviceSynchronize();
   for (auto& p : c->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
   c->edges.release(); c->tris.release(); c->body.release();
-  c->edge_bvh.release(); c->tri_bvh.release();
+  c->edge_bvh.release(); c->tri_bvh.release(); c->vorder.release();
   c->x1.release(); c->cand_ee.release(); c->cand_pt.release();
   c->cand_ids_ee.release(); c->cand_ids_pt.release(); c->small.release();
   c->fe.release(); c->ft.release();
@@ -167,10 +168,11 @@ QueryLaunch dcd_ee_query(ipc_ctx* c, const float* x, float thickness, int exclud
 }
 
 QueryLaunch dcd_pt_query(ipc_ctx* c, const float* x, float thickness, float buffer, int exclude_same_body,
-                         int* contacts, int* e0e1, int capacity, int* count) {
+                         int* contacts, int* e0e1, int capacity, int* count, const int* order) {
   QueryLaunch q{};
   q.mode = Q_DCD_PT;
   q.n_query = c->nv;
+  q.order = order;
   q.tris = c->tris.as<int>();
   q.body = c->body.as<int>();
   q.x0 = x;
@@ -209,10 +211,11 @@ QueryLaunch ccd_ee_query(ipc_ctx* c, const float* x0, const float* x1, int exclu
 }
 
 QueryLaunch ccd_pt_query(ipc_ctx* c, const float* x0, const float* x1, float padding, int exclude_same_body,
-                         int* quads, int* ids, int capacity, int* count) {
+                         int* quads, int* ids, int capacity, int* count, const int* order) {
   QueryLaunch q{};
   q.mode = Q_CCD_PT;
   q.n_query = c->nv;
+  q.order = order;
   q.tris = c->tris.as<int>();
   q.body = c->body.as<int>();
   q.x0 = x0;
@@ -268,7 +271,10 @@ int ipc_detect_collision_pt(ipc_ctx* c, const float* x, float thickness, float b
                    }),
                    "triangle BVH");
   if (st || n_tri == 0) return st;
-  QueryLaunch q = dcd_pt_query(c, x, thickness, buffer, exclude_same_body, contacts, e0e1, capacity, count);
+  const int* order = nullptr;
+  st = ctx_hip(point_order(c->vorder, c->nv, x, s, &order), "detect_collision_pt order");
+  if (st) return st;
+  QueryLaunch q = dcd_pt_query(c, x, thickness, buffer, exclude_same_body, contacts, e0e1, capacity, count, order);
   return ctx_hip(timed(c, IPC_TIMER_QUERY_DCD_PT, s, [&] { return run_query(q, s); }), "detect_collision_pt query");
 }
 
@@ -292,8 +298,12 @@ int ipc_detect_collision(ipc_ctx* c, const float* x, float thickness, float buff
                    }),
                    "detect_collision BVHs");
   if (st) return st;
+  const int* order = nullptr;
+  if (n_tri > 0) st = ctx_hip(point_order(c->vorder, c->nv, x, s, &order), "detect_collision order");
+  if (st) return st;
   QueryLaunch q[2] = {dcd_ee_query(c, x, thickness, exclude_same_body, ee, ee_ids, ee_capacity, ee_count),
-                      dcd_pt_query(c, x, thickness, buffer, exclude_same_body, pt, pt_ids, pt_capacity, pt_count)};
+                      dcd_pt_query(c, x, thickness, buffer, exclude_same_body, pt, pt_ids, pt_capacity, pt_count,
+                                   order)};
   if (n_tri == 0) q[1].n_query = 0;
   return ctx_hip(timed(c, IPC_TIMER_QUERY_DCD_EE, s, [&] { return run_queries(q, 2, s); }), "detect_collision query");
 }
@@ -354,7 +364,10 @@ int ipc_ccd_candidates_pt(ipc_ctx* c, const float* x0, const float* x1, float pa
                    }),
                    "swept triangle BVH");
   if (st || n_tri == 0) return st;
-  QueryLaunch q = ccd_pt_query(c, x0, x1, padding, exclude_same_body, quads, ids, capacity, count);
+  const int* order = nullptr;
+  st = ctx_hip(point_order(c->vorder, c->nv, x0, s, &order), "ccd_candidates_pt order");
+  if (st) return st;
+  QueryLaunch q = ccd_pt_query(c, x0, x1, padding, exclude_same_body, quads, ids, capacity, count, order);
   return ctx_hip(timed(c, IPC_TIMER_QUERY_CCD_PT, s, [&] { return run_query(q, s); }), "ccd_candidates_pt query");
 }
 
@@ -393,9 +406,12 @@ int ipc_collision_free_step(ipc_ctx* c, const float* x, const float* dx, float p
     st = ctx_hip(timed(c, IPC_TIMER_BUILD_EDGE, s, [&] { return build_bvhs(jobs, 2, c->body.as<int>(), x, pro, s); }),
                  "collision_free_step BVHs");
     if (st) return st;
+    const int* order = nullptr;
+    if (n_tri > 0) st = ctx_hip(point_order(c->vorder, c->nv, x, s, &order), "collision_free_step order");
+    if (st) return st;
     QueryLaunch q[2] = {ccd_ee_query(c, x, x1, exclude_same_body, c->cand_ee.as<int>(), nullptr, c->cap_ee, cnt + 0),
                         ccd_pt_query(c, x, x1, padding, exclude_same_body, c->cand_pt.as<int>(), nullptr, c->cap_pt,
-                                     cnt + 1)};
+                                     cnt + 1, order)};
     if (n_tri == 0) q[1].n_query = 0;
     st = ctx_hip(timed(c, IPC_TIMER_QUERY_CCD_EE, s, [&] { return run_queries(q, 2, s); }),
                  "collision_free_step query");

@@ -61,8 +61,14 @@ struct Box { float lo[3], hi[3]; };
 // mode 1: triangle bounds +- pad (wp.Mesh bounds when pad = 0)
 // mode 2: swept edge box over x0, x1, +- pad (swept_edge_aabbs, ccd/ccd.py:166-186)
 // mode 3: swept triangle box over x0, x1, +- pad (swept_triangle_aabbs, ccd/ccd.py:188-210)
+// mode 4: the point x0[p] (query ordering only)
 __device__ __forceinline__ void prim_box(int mode, int p, const int* __restrict__ prims, const float* __restrict__ x0,
                                          const float* __restrict__ x1, float pad, float4& lo4, float4& hi4) {
+  if (mode == BOX_POINT) {
+    V3<float> q = ldx(x0, p);
+    lo4 = hi4 = make_float4(q.x, q.y, q.z, 0.f);
+    return;
+  }
   int k = (mode == 0 || mode == 2) ? 2 : 3;
   V3<float> lo, hi;
   lo = hi = ldx(x0, prims[k * p]);
@@ -781,6 +787,53 @@ hipError_t build_bvhs(const BuildJob* jobs, int n_jobs, const int* body, const f
     hipLaunchKernelGGL(reduce_top_kernel, dim3(n_live), dim3(1024), 0, s, t0, t1);
   }
   return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) key_ids_kernel(int n, const unsigned long long* __restrict__ keys,
+                                                      int* __restrict__ ids) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) ids[i] = static_cast<int>(keys[i] & 0xffffffffu);
+}
+
+void PointOrder::release() {
+  keys.release(); keys_sorted.release(); order.release(); temp.release(); bounds.release();
+  n = 0;
+  age = -1;
+}
+
+hipError_t point_order(PointOrder& P, int n, const float* x, hipStream_t s, const int** order) {
+  *order = nullptr;
+  if (n <= 0) return hipSuccess;
+  if (P.age >= 0 && P.age < kResortEvery && P.n == n) {
+    ++P.age;
+    *order = P.order.as<int>();
+    return hipSuccess;
+  }
+  IPC_TRY(P.bounds.ensure(sizeof(int) * 8));
+  IPC_TRY(P.keys.ensure(sizeof(unsigned long long) * n));
+  IPC_TRY(P.keys_sorted.ensure(sizeof(unsigned long long) * n));
+  IPC_TRY(P.order.ensure(sizeof(int) * n));
+  hipLaunchKernelGGL(init_bounds_kernel, dim3(1), dim3(64), 0, s, P.bounds.as<int>());
+  hipLaunchKernelGGL(centre_bounds_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, int(BOX_POINT), n,
+                     static_cast<const int*>(nullptr), x, static_cast<const float*>(nullptr), 0.0f, P.bounds.as<int>());
+  hipLaunchKernelGGL(morton_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, int(BOX_POINT), n,
+                     static_cast<const int*>(nullptr), x, static_cast<const float*>(nullptr), 0.0f,
+                     P.bounds.as<int>(), P.keys.as<unsigned long long>());
+  IPC_TRY(hipGetLastError());
+  size_t temp_bytes = 0;
+  IPC_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, temp_bytes, P.keys.as<unsigned long long>(),
+                                            P.keys_sorted.as<unsigned long long>(), n, 0, 62, s));
+  IPC_TRY(P.temp.ensure(temp_bytes));
+  temp_bytes = P.temp.bytes;
+  IPC_TRY(hipcub::DeviceRadixSort::SortKeys(P.temp.p, temp_bytes, P.keys.as<unsigned long long>(),
+                                            P.keys_sorted.as<unsigned long long>(), n, 0, 62, s));
+  hipLaunchKernelGGL(key_ids_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, n,
+                     P.keys_sorted.as<unsigned long long>(), P.order.as<int>());
+  IPC_TRY(hipGetLastError());
+  P.n = n;
+  P.age = 0;
+  *order = P.order.as<int>();
+  return hipSuccess;
 }
 
 hipError_t build_bvh(Bvh& B, int mode, int n, const int* prims, const int* body, const float* x0, const float* x1,

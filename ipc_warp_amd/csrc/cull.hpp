@@ -51,7 +51,20 @@ struct Bvh {
 // kResortEvery builds; in between, builds reuse it and only recompute boxes.
 constexpr int kResortEvery = 16;
 
-enum BoxMode { BOX_EDGE = 0, BOX_TRI = 1, BOX_SWEPT_EDGE = 2, BOX_SWEPT_TRI = 3 };
+enum BoxMode { BOX_EDGE = 0, BOX_TRI = 1, BOX_SWEPT_EDGE = 2, BOX_SWEPT_TRI = 3, BOX_POINT = 4 };
+
+// Morton order of the points of a soup, the query order of the point
+// queries (a wave's eight queries then sit close together and walk the same
+// subtrees). Recomputed on the first call and every kResortEvery calls; the
+// order only affects speed, never the pairs found.
+struct PointOrder {
+  DevBuf keys, keys_sorted, order, temp, bounds;
+  int n = 0, age = -1;
+  void release();
+};
+
+// Refreshes P from positions x when due; *order receives slot -> point.
+hipError_t point_order(PointOrder& P, int n, const float* x, hipStream_t s, const int** order);
 enum QueryMode { Q_DCD_EE = 0, Q_DCD_PT = 1, Q_CCD_EE = 2, Q_CCD_PT = 3, Q_ISECT = 4 };
 
 // One tree of a build launch: n primitives `prims` (2 or 3 vertex ids each
