@@ -34,7 +34,7 @@ __device__ __forceinline__ bool pair_ee(const float* __restrict__ x, const float
   V3<double> xo = to_d(xf[0]);
   V3<double> xr[4] = {mk(0.0, 0.0, 0.0), to_d(xf[1]) - xo, to_d(xf[2]) - xo, to_d(xf[3]) - xo};
   FeatureGeom g = feature_geom(f, xr);
-  double d2 = want_hessian ? nsd_compressed(g, !near_parallel, C) : dot(g.r, g.r);
+  double d2 = dot(g.r, g.r);
   Barrier B = barrier(d2, double(l0) * double(l0), kappa);
   double m = 1.0;
   // grad m over the slots is (-ga, ga, -gb, gb)
@@ -52,6 +52,18 @@ __device__ __forceinline__ bool pair_ee(const float* __restrict__ x, const float
   }
   if (energy) *energy = m * B.b;
   if (!want_hessian) return true;
+  // force = -(m b' grad d^2 + b grad m), grad m zero outside the band. It is
+  // formed first, so that the feature geometry is dead before the spectral
+  // projection, the kernel's register peak.
+  double bm = near_parallel ? B.b : 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    double s = -2.0 * m * B.db * g.w[i];
+    V3<double> gmi = i == 0 ? mk(-ga.x, -ga.y, -ga.z) : (i == 1 ? ga : (i == 2 ? mk(-gb.x, -gb.y, -gb.z) : gb));
+    force[3 * i + 0] = float(s * g.r.x - bm * gmi.x);
+    force[3 * i + 1] = float(s * g.r.y - bm * gmi.y);
+    force[3 * i + 2] = float(s * g.r.z - bm * gmi.z);
+  }
   if (near_parallel) {
     // Near-parallel edges keep only the Gauss-Newton term m b'' g g^T.
     C.cn = 0.0f;
@@ -62,16 +74,7 @@ __device__ __forceinline__ bool pair_ee(const float* __restrict__ x, const float
     C.cn = float(B.db);
     C.cg = float(4.0 * B.d2b * d2);
   }
-  // force = -(m b' grad d^2 + b grad m); grad m is zero outside the band
-  double bm = near_parallel ? B.b : 0.0;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    double s = -2.0 * m * B.db * g.w[i];
-    V3<double> gmi = i == 0 ? mk(-ga.x, -ga.y, -ga.z) : (i == 1 ? ga : (i == 2 ? mk(-gb.x, -gb.y, -gb.z) : gb));
-    force[3 * i + 0] = float(s * g.r.x - bm * gmi.x);
-    force[3 * i + 1] = float(s * g.r.y - bm * gmi.y);
-    force[3 * i + 2] = float(s * g.r.z - bm * gmi.z);
-  }
+  nsd_compressed(g, !near_parallel, C);
   return true;
 }
 
@@ -87,7 +90,7 @@ __device__ __forceinline__ bool pair_pt(const float* __restrict__ x, int4 c, flo
   V3<double> xo = to_d(xf[0]);
   V3<double> xr[4] = {mk(0.0, 0.0, 0.0), to_d(xf[1]) - xo, to_d(xf[2]) - xo, to_d(xf[3]) - xo};
   FeatureGeom g = feature_geom(f, xr);
-  double d2 = want_hessian ? nsd_compressed(g, true, C) : dot(g.r, g.r);
+  double d2 = dot(g.r, g.r);
   Barrier B = barrier(d2, double(l0) * double(l0), kappa);
   if (energy) *energy = B.b;
   if (!want_hessian) return true;
@@ -100,6 +103,7 @@ __device__ __forceinline__ bool pair_pt(const float* __restrict__ x, int4 c, flo
     force[3 * i + 1] = float(s * g.r.y);
     force[3 * i + 2] = float(s * g.r.z);
   }
+  nsd_compressed(g, true, C);
   return true;
 }
 

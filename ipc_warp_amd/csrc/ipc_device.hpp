@@ -284,60 +284,71 @@ IPCD_FN double inv_sqrt(double x) {
 #endif
 }
 
+// Index of (i, j) in the packed upper triangle of a symmetric n x n matrix.
+template <int n> IPCD_FN constexpr int tri_idx(int i, int j) {
+  return i <= j ? i * n - i * (i - 1) / 2 + (j - i) : j * n - j * (j - 1) / 2 + (i - j);
+}
+
 // Cyclic Jacobi on a symmetric n x n (row-major), double. Returns
 // eigenvalues in ev and eigenvectors in the columns of V. Sweeps stop once
 // the off-diagonal mass is below 1e-24 of the diagonal's (eigenvalues then
-// carry relative errors far below the 1e-5 the Hessians are held to).
+// carry relative errors far below the 1e-5 the Hessians are held to). The
+// matrix is kept as its packed upper triangle and each rotation uses the
+// symmetric update (a_pp -= t a_pq, a_qq += t a_pq, a_pq = 0, the other rows
+// rotated); a rotation with a_pq = 0 runs with t = 0, the identity, so that
+// the unrolled sweep has no branches (and no duplicated live matrices).
 template <int n>
-IPCD_FN void jacobi_sym(double A[n * n], double ev[n], double V[n * n]) {
+IPCD_FN void jacobi_sym(const double A[n * n], double ev[n], double V[n * n]) {
 #pragma clang fp contract(fast)
+  double a[n * (n + 1) / 2];
+#pragma unroll
+  for (int i = 0; i < n; ++i)
+#pragma unroll
+    for (int j = i; j < n; ++j) a[tri_idx<n>(i, j)] = A[i * n + j];
 #pragma unroll
   for (int i = 0; i < n * n; ++i) V[i] = (i % (n + 1) == 0) ? 1.0 : 0.0;
   for (int sweep = 0; sweep < 12; ++sweep) {
     double off = 0.0, diag = 0.0;
 #pragma unroll
     for (int i = 0; i < n; ++i) {
-      diag += A[i * n + i] * A[i * n + i];
+      diag += a[tri_idx<n>(i, i)] * a[tri_idx<n>(i, i)];
 #pragma unroll
-      for (int j = i + 1; j < n; ++j) off += A[i * n + j] * A[i * n + j];
+      for (int j = i + 1; j < n; ++j) off += a[tri_idx<n>(i, j)] * a[tri_idx<n>(i, j)];
     }
     if (!(off > 1.0e-24 * diag)) break;
 #pragma unroll
     for (int p = 0; p < n - 1; ++p) {
 #pragma unroll
       for (int q = p + 1; q < n; ++q) {
-        double apq = A[p * n + q];
-        if (apq != 0.0) {
-          // tan of the rotation angle, t = sgn(d) e / (|d| + sqrt(d^2 + e^2))
-          // with d = a_qq - a_pp, e = 2 a_pq (the textbook theta = d / e form
-          // with its two divisions folded into one)
-          double d = A[q * n + q] - A[p * n + p], e = 2.0 * apq;
-          double t = (d >= 0.0 ? e : -e) / (fabs(d) + sqrt(d * d + e * e));
-          double c = inv_sqrt(t * t + 1.0), s = t * c;
+        const double apq = a[tri_idx<n>(p, q)];
+        // tan of the rotation angle, t = sgn(d) e / (|d| + sqrt(d^2 + e^2))
+        // with d = a_qq - a_pp, e = 2 a_pq (the textbook theta = d / e form
+        // with its two divisions folded into one)
+        const double d = a[tri_idx<n>(q, q)] - a[tri_idx<n>(p, p)], e = 2.0 * apq;
+        double t = (d >= 0.0 ? e : -e) / (fabs(d) + sqrt(d * d + e * e));
+        t = apq != 0.0 ? t : 0.0;
+        const double c = inv_sqrt(t * t + 1.0), s = t * c;
+        a[tri_idx<n>(p, p)] -= t * apq;
+        a[tri_idx<n>(q, q)] += t * apq;
+        a[tri_idx<n>(p, q)] = 0.0;
 #pragma unroll
-          for (int k = 0; k < n; ++k) {
-            double akp = A[k * n + p], akq = A[k * n + q];
-            A[k * n + p] = c * akp - s * akq;
-            A[k * n + q] = s * akp + c * akq;
-          }
+        for (int r = 0; r < n; ++r) {
+          if (r == p || r == q) continue;
+          const double arp = a[tri_idx<n>(r, p)], arq = a[tri_idx<n>(r, q)];
+          a[tri_idx<n>(r, p)] = c * arp - s * arq;
+          a[tri_idx<n>(r, q)] = s * arp + c * arq;
+        }
 #pragma unroll
-          for (int k = 0; k < n; ++k) {
-            double apk = A[p * n + k], aqk = A[q * n + k];
-            A[p * n + k] = c * apk - s * aqk;
-            A[q * n + k] = s * apk + c * aqk;
-          }
-#pragma unroll
-          for (int k = 0; k < n; ++k) {
-            double vkp = V[k * n + p], vkq = V[k * n + q];
-            V[k * n + p] = c * vkp - s * vkq;
-            V[k * n + q] = s * vkp + c * vkq;
-          }
+        for (int k = 0; k < n; ++k) {
+          const double vkp = V[k * n + p], vkq = V[k * n + q];
+          V[k * n + p] = c * vkp - s * vkq;
+          V[k * n + q] = s * vkp + c * vkq;
         }
       }
     }
   }
 #pragma unroll
-  for (int i = 0; i < n; ++i) ev[i] = A[i * n + i];
+  for (int i = 0; i < n; ++i) ev[i] = a[tri_idx<n>(i, i)];
 }
 
 // Compressed form of the 16 blocks of a contact Hessian
@@ -481,52 +492,49 @@ IPCD_FN double nsd_compressed(const FeatureGeom& g, bool want_nsd, CompressedHes
   }
   double ev[4], V[16];
   jacobi_sym<4>(K, ev, V);
-  // N4 = sum over negative eigenvalues of lambda v v^T
-  double N4[16];
+  // N = Q N4 Q^T with N4 = sum over negative eigenvalues of lambda v v^T,
+  // accumulated straight into the compressed fields: for an eigenvector v,
+  // y = Q v (y0..2 = v0 q1 + v1 q2, y3 = v2, y4 = v3) and zeta_i = Z_i.(y3, y4);
+  // then Ntt += lambda y y^T (3 x 3 block), u_i += lambda zeta_i y0..2 and
+  // s_ij += lambda zeta_i zeta_j.
+  float ntt[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, uu[4][3], ss[10];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) N4[i] = 0.0;
+  for (int i = 0; i < 4; ++i) uu[i][0] = uu[i][1] = uu[i][2] = 0.f;
+#pragma unroll
+  for (int i = 0; i < 10; ++i) ss[i] = 0.f;
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
     double lam = ev[m];
     if (lam < 0.0) {
+      double y[3], zeta[4];
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < 3; ++a) y[a] = V[0 * 4 + m] * q1[a] + V[1 * 4 + m] * q2[a];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) N4[a * 4 + b] += lam * V[a * 4 + m] * V[b * 4 + m];
+      for (int i = 0; i < 4; ++i) zeta[i] = Z[i][0] * V[2 * 4 + m] + Z[i][1] * V[3 * 4 + m];
+      ntt[0] += float(lam * y[0] * y[0]);
+      ntt[1] += float(lam * y[0] * y[1]);
+      ntt[2] += float(lam * y[0] * y[2]);
+      ntt[3] += float(lam * y[1] * y[1]);
+      ntt[4] += float(lam * y[1] * y[2]);
+      ntt[5] += float(lam * y[2] * y[2]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) uu[i][a] += float(lam * zeta[i] * y[a]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = i; j < 4; ++j) ss[sidx(i, j)] += float(lam * zeta[i] * zeta[j]);
     }
   }
-  // back to the 5 x 5 layout: N = Q N4 Q^T
-  double N[25];
-  {
-    const double Qa[3][2] = {{q1[0], q2[0]}, {q1[1], q2[1]}, {q1[2], q2[2]}};
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-#pragma unroll
-      for (int b = 0; b < 3; ++b)
-        N[a * 5 + b] = Qa[a][0] * (N4[0] * Qa[b][0] + N4[1] * Qa[b][1]) + Qa[a][1] * (N4[4] * Qa[b][0] + N4[5] * Qa[b][1]);
-#pragma unroll
-      for (int l = 0; l < 2; ++l) {
-        double v = Qa[a][0] * N4[0 * 4 + 2 + l] + Qa[a][1] * N4[1 * 4 + 2 + l];
-        N[a * 5 + 3 + l] = v;
-        N[(3 + l) * 5 + a] = v;
-      }
-    }
-    N[18] = N4[10]; N[19] = N4[11]; N[23] = N4[14]; N[24] = N4[15];
-  }
-  C.Ntt[0] = float(N[0]); C.Ntt[1] = float(N[1]); C.Ntt[2] = float(N[2]);
-  C.Ntt[3] = float(N[6]); C.Ntt[4] = float(N[7]); C.Ntt[5] = float(N[12]);
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) C.u[i][a] = float(N[a * 5 + 3] * Z[i][0] + N[a * 5 + 4] * Z[i][1]);
-  }
+  for (int i = 0; i < 6; ++i) C.Ntt[i] = ntt[i];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = i; j < 4; ++j) {
-      double v = Z[i][0] * (N[18] * Z[j][0] + N[19] * Z[j][1]) + Z[i][1] * (N[23] * Z[j][0] + N[24] * Z[j][1]);
-      C.s[sidx(i, j)] = float(v);
-    }
+    for (int a = 0; a < 3; ++a) C.u[i][a] = uu[i][a];
+#pragma unroll
+  for (int i = 0; i < 10; ++i) C.s[i] = ss[i];
   return d2;
 }
 
