@@ -164,12 +164,16 @@ __global__ void __launch_bounds__(256) contact_hessian_kernel(const float* __res
 #pragma unroll
     for (int k = 0; k < 9; ++k) v4[q * 9 + k] = make_float4(blk[4 * k], blk[4 * k + 1], blk[4 * k + 2], blk[4 * k + 3]);
   }
+  // Device-scope float atomics are the kernel's largest cost (each is its own
+  // fabric transaction); a slot with zero weight in the feature (the far
+  // vertices of edge and vertex features) has an exactly zero force, whose
+  // add is skipped.
 #pragma unroll
   for (int ii = 0; ii < 4; ++ii) {
     float* dst = b + 3 * (long long)cc[ii];
-    unsafeAtomicAdd(dst + 0, force[3 * ii + 0]);
-    unsafeAtomicAdd(dst + 1, force[3 * ii + 1]);
-    unsafeAtomicAdd(dst + 2, force[3 * ii + 2]);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      if (force[3 * ii + a] != 0.0f) unsafeAtomicAdd(dst + a, force[3 * ii + a]);
   }
 }
 
