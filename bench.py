@@ -244,6 +244,7 @@ def main() -> int:
                           disable_self_collision_=True)
     b = torch.zeros_like(x)
     e_dev = torch.zeros(1, dtype=torch.float64, device=dev)
+    e_host = torch.zeros(1, dtype=torch.float64, pin_memory=True)
     from ipc_warp_amd import ops
 
     counts = {}
@@ -276,8 +277,11 @@ def main() -> int:
         ops.contact_energy_pt(S.x, S.contacts_pt.list, n_pt, S.l0, S.contact_stiffness, e_dev)
         if timed:
             kev["energy"][1].record()
+        # the energy travels to pinned host memory ahead of the CCD, whose
+        # own synchronisation then covers it
+        e_host.copy_(e_dev, non_blocking=True)
         toi = S.raw_toi(dx)
-        e = float(e_dev.item())  # after the CCD's own synchronisation
+        e = float(e_host[0])
         if timed:
             torch.cuda.synchronize()
             for k in kev:

@@ -15,17 +15,20 @@ from ._lib import check, lib
 
 
 def _need(t: torch.Tensor, dtype: torch.dtype, name: str, shape_tail=None) -> int:
+    # One cheap combined test on the hot path (these wrappers run between a
+    # step's device synchronisations); the specific error only on failure.
+    if (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == dtype and t.is_contiguous()
+            and (shape_tail is None or t.shape[1:] == shape_tail)):
+        return t.data_ptr()
     if not isinstance(t, torch.Tensor):
         raise TypeError(f"{name} must be a torch.Tensor")
-    if t.device.type != "cuda":
+    if not t.is_cuda:
         raise ValueError(f"{name} must live on the GPU (got {t.device})")
     if t.dtype != dtype:
         raise TypeError(f"{name} must be {dtype} (got {t.dtype})")
     if not t.is_contiguous():
         raise ValueError(f"{name} must be contiguous")
-    if shape_tail is not None and tuple(t.shape[1:]) != tuple(shape_tail):
-        raise ValueError(f"{name} must have shape [n, {', '.join(map(str, shape_tail))}] (got {tuple(t.shape)})")
-    return t.data_ptr()
+    raise ValueError(f"{name} must have shape [n, {', '.join(map(str, shape_tail))}] (got {tuple(t.shape)})")
 
 
 def _opt(t, dtype, name, shape_tail=None):
@@ -45,7 +48,14 @@ def _vec_len(t, n, name):
         raise ValueError(f"{name} holds {t.shape[0]} entries, fewer than n = {n}")
 
 
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def stream_handle(device=None) -> int:
+    """Torch's current HIP stream on `device` (default: the current device)."""
+    if _raw_stream is not None:
+        index = device.index if isinstance(device, torch.device) else device
+        return _raw_stream(torch.cuda.current_device() if index is None else int(index))
     return torch.cuda.current_stream(device).cuda_stream
 
 
