@@ -587,15 +587,15 @@ __device__ __forceinline__ void query_block(const QueryArgs& A, int bid, QuerySm
 template <int MODE>
 __global__ void __launch_bounds__(kQueryBlock) query_kernel(QueryArgs A) {
   __shared__ QuerySmem sm;
-  query_block<MODE>(A, blockIdx.x, sm);
+  query_block<MODE>(A, xcd_local(blockIdx.x, gridDim.x), sm);
 }
 
 // Two queries in one launch: workgroups [0, blocks_a) serve A, the rest B.
 template <int MA, int MB>
 __global__ void __launch_bounds__(kQueryBlock) query_pair_kernel(QueryArgs A, QueryArgs B, int blocks_a) {
   __shared__ QuerySmem sm;
-  if (int(blockIdx.x) < blocks_a) query_block<MA>(A, blockIdx.x, sm);
-  else query_block<MB>(B, blockIdx.x - blocks_a, sm);
+  if (int(blockIdx.x) < blocks_a) query_block<MA>(A, xcd_local(blockIdx.x, blocks_a), sm);
+  else query_block<MB>(B, xcd_local(blockIdx.x - blocks_a, gridDim.x - blocks_a), sm);
 }
 
 // x1 = x - dx (_add_dx, contact.py:49-52)

@@ -16,6 +16,18 @@ constexpr int kPopNodes = 1;  // sibling nodes a query group takes from its stac
 constexpr int kMaskStack = kPopNodes * 9 + 1;
 constexpr int kBlockOut = 512;  // accepted pairs a query block stages in LDS
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup w runs on
+// XCD w mod 8), each XCD with its own L2. xcd_local renumbers the `nblk`
+// workgroups of a launch so that each XCD serves one contiguous range: with
+// Morton-ordered queries, each L2 then holds its region's share of a tree's
+// lower levels instead of the whole tree. A bijection of [0, nblk).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_local(int bid, int nblk) {
+  const int q = nblk / kXcds, r = nblk % kXcds;
+  const int x = bid % kXcds, k = bid / kXcds;
+  return x * q + (x < r ? x : r) + k;
+}
+
 // Inclusive box test (a lower bound above the other's upper bound
 // separates). The terms are combined bitwise: with short-circuit operators
 // the compiler splits the test into branches and defers each component's
