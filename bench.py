@@ -321,18 +321,18 @@ def main() -> int:
 
     # Algorithmic bytes per launch: barrier assembly 816 B (PT) / 864 B (EE) per
     # contact; TOI 112 B per candidate; a query reads the tree once (leaf:
-    # 32 B box + 4 B id + 4 B body; node: 32 B box + 8 B body range), one
-    # 32 B (EE) or 12/24 B (PT DCD/CCD) query box per query and writes 16 B
-    # (+8 B ids in DCD) per emitted pair. A DCD or CCD query launch serves
-    # both halves, so its bytes are the EE and PT sums.
+    # 24 B box + 4 B body + 4 B primitive id + 4 B per vertex id; node: 24 B
+    # box + 8 B body range), one 32 B (EE) or 12/24 B (PT DCD/CCD) query box
+    # per query and writes 16 B (+8 B ids in DCD) per emitted pair. A DCD or
+    # CCD query launch serves both halves, so its bytes are the EE and PT sums.
     E, F, V = int(soup.edges.shape[0]), int(soup.triangles.shape[0]), int(soup.x.shape[0])
-    tree = lambda n: 40 * n + 40 * (n + 6) // 7
+    tree = lambda n, nv: (32 + 4 * nv) * n + 32 * ((n + 6) // 7)
     alg = {
         "hessian_ee": 864 * counts["ee"],
         "hessian_pt": 816 * counts["pt"],
         "energy": 112 * counts["ee"] + 64 * counts["pt"],
-        "query_dcd": tree(E) + 32 * E + 24 * counts["ee"] + tree(F) + 12 * V + 24 * counts["pt"],
-        "query_ccd": tree(E) + 32 * E + 16 * counts["ccd_ee"] + tree(F) + 24 * V + 16 * counts["ccd_pt"],
+        "query_dcd": tree(E, 2) + 32 * E + 24 * counts["ee"] + tree(F, 3) + 12 * V + 24 * counts["pt"],
+        "query_ccd": tree(E, 2) + 32 * E + 16 * counts["ccd_ee"] + tree(F, 3) + 24 * V + 16 * counts["ccd_pt"],
         "toi": 112 * (counts["ccd_ee"] + counts["ccd_pt"]),
     }
     dominant = max((k for k in alg if kernel_ms.get(k, 0) > 0), key=lambda k: per_step_ms[k])

@@ -153,24 +153,31 @@ __global__ void __launch_bounds__(256) morton_kernel(int mode, int n, const int*
 constexpr int kLeafBlock = kFan * kFan * kFan;  // 512 leaves (and threads) per workgroup of the leaf kernel
 constexpr int kInBlockLevels = 3;               // levels 1..3 reduced inside that workgroup
 
+// A node is a pair of float4: xyz the box, and the w fields the bit patterns
+// of the (min, max) body id below it, so that one step of a query reads the
+// box and the body range in two 16-byte loads per child.
+__device__ __forceinline__ int body_min(float4 lo) { return __float_as_int(lo.w); }
+__device__ __forceinline__ int body_max(float4 hi) { return __float_as_int(hi.w); }
+
 __device__ __forceinline__ void box_merge(float4& lo, float4& hi, float4 l, float4 h) {
-  lo = make_float4(fminf(lo.x, l.x), fminf(lo.y, l.y), fminf(lo.z, l.z), 0.f);
-  hi = make_float4(fmaxf(hi.x, h.x), fmaxf(hi.y, h.y), fmaxf(hi.z, h.z), 0.f);
+  lo = make_float4(fminf(lo.x, l.x), fminf(lo.y, l.y), fminf(lo.z, l.z), __int_as_float(min(body_min(lo), body_min(l))));
+  hi = make_float4(fmaxf(hi.x, h.x), fmaxf(hi.y, h.y), fmaxf(hi.z, h.z), __int_as_float(max(body_max(hi), body_max(h))));
 }
 
+// An empty box and an empty body range.
 __device__ __forceinline__ void box_empty(float4& lo, float4& hi) {
-  lo = make_float4(INFINITY, INFINITY, INFINITY, 0.f);
-  hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+  lo = make_float4(INFINITY, INFINITY, INFINITY, __int_as_float(0x7fffffff));
+  hi = make_float4(-INFINITY, -INFINITY, -INFINITY, __int_as_float((int)0x80000000));
 }
 
-// Merge a box and body range with the lane `off` away (xor), so that after
-// offsets 1, 2, 4 every lane of an aligned group of 8 holds the group's merge.
-__device__ __forceinline__ void lane_merge(float4& lo, float4& hi, int2& br, int off) {
-  float4 l = make_float4(__shfl_xor(lo.x, off, 64), __shfl_xor(lo.y, off, 64), __shfl_xor(lo.z, off, 64), 0.f);
-  float4 h = make_float4(__shfl_xor(hi.x, off, 64), __shfl_xor(hi.y, off, 64), __shfl_xor(hi.z, off, 64), 0.f);
+// Merge a node with the lane `off` away (xor), so that after offsets 1, 2, 4
+// every lane of an aligned group of 8 holds the group's merge.
+__device__ __forceinline__ void lane_merge(float4& lo, float4& hi, int off) {
+  float4 l = make_float4(__shfl_xor(lo.x, off, 64), __shfl_xor(lo.y, off, 64), __shfl_xor(lo.z, off, 64),
+                         __shfl_xor(lo.w, off, 64));
+  float4 h = make_float4(__shfl_xor(hi.x, off, 64), __shfl_xor(hi.y, off, 64), __shfl_xor(hi.z, off, 64),
+                         __shfl_xor(hi.w, off, 64));
   box_merge(lo, hi, l, h);
-  br.x = min(br.x, __shfl_xor(br.x, off, 64));
-  br.y = max(br.y, __shfl_xor(br.y, off, 64));
 }
 
 // One tree of a build launch (device side).
@@ -184,13 +191,11 @@ struct LeafJob {
   int4* leaf_vid;
   float4* node_lo;
   float4* node_hi;
-  int2* node_body;
   LevelInfo info;
 };
 
 struct LeafSmem {
   float4 wlo[kLeafBlock / 64], whi[kLeafBlock / 64];
-  int2 wbody[kLeafBlock / 64];
 };
 
 // Leaves in Morton order and levels 1..3 of the hierarchy, one thread per
@@ -209,7 +214,6 @@ __device__ __forceinline__ void leaf_block(const LeafJob& J, int bid, const int*
   const int stride = (J.mode == 0 || J.mode == 2) ? 2 : 3;
   float4 lo, hi;
   box_empty(lo, hi);
-  int2 br = make_int2(0x7fffffff, (int)0x80000000);
   int4 vid = make_int4(-1, -1, -1, -1);
   if (k < J.n) {
     int prim = static_cast<int>(J.keys[k] & 0xffffffffu);
@@ -219,52 +223,43 @@ __device__ __forceinline__ void leaf_block(const LeafJob& J, int bid, const int*
     int bd = body[v0];
     J.leaf_id[k] = prim;
     vid = make_int4(v0, v1, v2, prim);
-    br = make_int2(bd, bd);
+    lo.w = __int_as_float(bd);
+    hi.w = __int_as_float(bd);
   }
   if (k < info.slots[0]) {  // offset[0] = 0
     J.node_lo[k] = lo;
     J.node_hi[k] = hi;
-    J.node_body[k] = br;
     J.leaf_vid[k] = vid;
   }
   // level 1: groups of 8 lanes
-  lane_merge(lo, hi, br, 1);
-  lane_merge(lo, hi, br, 2);
-  lane_merge(lo, hi, br, 4);
+  lane_merge(lo, hi, 1);
+  lane_merge(lo, hi, 2);
+  lane_merge(lo, hi, 4);
   int g1 = bid * (kLeafBlock / kFan) + t / kFan;
   if ((lane & (kFan - 1)) == 0 && g1 < info.slots[1]) {
     J.node_lo[info.offset[1] + g1] = lo;
     J.node_hi[info.offset[1] + g1] = hi;
-    J.node_body[info.offset[1] + g1] = br;
   }
   // level 2: the wave's 8 level-1 nodes
-  lane_merge(lo, hi, br, 8);
-  lane_merge(lo, hi, br, 16);
-  lane_merge(lo, hi, br, 32);
+  lane_merge(lo, hi, 8);
+  lane_merge(lo, hi, 16);
+  lane_merge(lo, hi, 32);
   int g2 = bid * (kLeafBlock / 64) + wave;
   if (lane == 0) {
     if (g2 < info.slots[2]) {
       J.node_lo[info.offset[2] + g2] = lo;
       J.node_hi[info.offset[2] + g2] = hi;
-      J.node_body[info.offset[2] + g2] = br;
     }
     sm.wlo[wave] = lo;
     sm.whi[wave] = hi;
-    sm.wbody[wave] = br;
   }
   __syncthreads();
   // level 3: the workgroup's 8 level-2 nodes
   if (t == 0 && bid < info.slots[3]) {
     float4 l = sm.wlo[0], h = sm.whi[0];
-    int2 r2 = sm.wbody[0];
-    for (int w = 1; w < kLeafBlock / 64; ++w) {
-      box_merge(l, h, sm.wlo[w], sm.whi[w]);
-      r2.x = min(r2.x, sm.wbody[w].x);
-      r2.y = max(r2.y, sm.wbody[w].y);
-    }
+    for (int w = 1; w < kLeafBlock / 64; ++w) box_merge(l, h, sm.wlo[w], sm.whi[w]);
     J.node_lo[info.offset[3] + bid] = l;
     J.node_hi[info.offset[3] + bid] = h;
-    J.node_body[info.offset[3] + bid] = r2;
   }
 }
 
@@ -288,7 +283,6 @@ __global__ void prologue_kernel(BuildPrologue pro) { run_prologue(pro); }
 struct TopJob {
   float4* node_lo;
   float4* node_hi;
-  int2* node_body;
   LevelInfo info;
 };
 
@@ -297,29 +291,22 @@ __device__ __forceinline__ void top_block(const TopJob& T) {
   if (info.top <= kInBlockLevels) return;
   float4 el, eh;
   box_empty(el, eh);
-  const int2 eb = make_int2(0x7fffffff, (int)0x80000000);
   for (int g = info.count[kInBlockLevels] + threadIdx.x; g < info.slots[kInBlockLevels]; g += blockDim.x) {
     T.node_lo[info.offset[kInBlockLevels] + g] = el;
     T.node_hi[info.offset[kInBlockLevels] + g] = eh;
-    T.node_body[info.offset[kInBlockLevels] + g] = eb;
   }
   __syncthreads();
   for (int L = kInBlockLevels + 1; L <= info.top; ++L) {
     for (int g = threadIdx.x; g < info.slots[L]; g += blockDim.x) {
       float4 l = el, h = eh;
-      int2 r2 = eb;
       if (g < info.count[L]) {
         for (int c = 0; c < kFan; ++c) {
           int k = info.offset[L - 1] + g * kFan + c;
           box_merge(l, h, T.node_lo[k], T.node_hi[k]);
-          int2 cb = T.node_body[k];
-          r2.x = min(r2.x, cb.x);
-          r2.y = max(r2.y, cb.y);
         }
       }
       T.node_lo[info.offset[L] + g] = l;
       T.node_hi[info.offset[L] + g] = h;
-      T.node_body[info.offset[L] + g] = r2;
     }
     __syncthreads();
   }
@@ -364,9 +351,8 @@ struct QueryArgs {
   // hierarchy (every level in the node arrays, leaves from slot 0)
   int n_leaves;
   LevelInfo info;
-  const float4* node_lo;
+  const float4* node_lo;  // xyz box, w body id bits (see body_min / body_max)
   const float4* node_hi;
-  const int2* node_body;
   const int4* leaf_vid;
   // output
   int4* rec;
@@ -534,15 +520,16 @@ __device__ __forceinline__ void query_block(const QueryArgs& A, int bid, QuerySm
         tk &= tk - 1u;
       }
     }
+    // The leaf record is fetched by lanes at the leaf level only: an
+    // internal step moves the two box halves, 32 B per lane.
     float4 clo[kPopNodes], chi[kPopNodes];
-    int2 cbr[kPopNodes];
     int4 cvid[kPopNodes];
 #pragma unroll
     for (int p = 0; p < kPopNodes; ++p) {
       clo[p] = A.node_lo[at[p]];
       chi[p] = A.node_hi[at[p]];
-      cbr[p] = A.node_body[at[p]];
-      cvid[p] = A.leaf_vid[L == 1 ? at[p] : 0];
+      cvid[p] = make_int4(0, 0, 0, 0);
+      if (L == 1) cvid[p] = A.leaf_vid[at[p]];
     }
     // Only real children count: a query box with NaN or infinite bounds
     // also "overlaps" the empty boxes of padding slots.
@@ -558,7 +545,7 @@ __device__ __forceinline__ void query_block(const QueryArgs& A, int bid, QuerySm
 #pragma unroll
     for (int p = 0; p < kPopNodes; ++p)
       hit[p] = (cpos[p] >= 0) & (at[p] - child_off < child_cnt) & overlaps(qlo, qhi, clo[p], chi[p]) &
-               !((qbody >= 0) & (cbr[p].x == qbody) & (cbr[p].y == qbody)) & beyond(at[p] - child_off);
+               !((qbody >= 0) & (body_min(clo[p]) == qbody) & (body_max(chi[p]) == qbody)) & beyond(at[p] - child_off);
     // internal level: one entry per node with overlapping children, the
     // first node's pushed last so that it is visited first
 #pragma unroll
@@ -660,7 +647,7 @@ void DevBuf::release() {
 
 void Bvh::release() {
   keys.release(); keys_sorted.release();
-  leaf_id.release(); node_lo.release(); node_hi.release(); leaf_vid.release(); node_body.release();
+  leaf_id.release(); node_lo.release(); node_hi.release(); leaf_vid.release();
   temp.release(); bounds.release();
   n = 0;
   builds_since_sort = -1;
@@ -708,7 +695,6 @@ hipError_t prepare_job(const BuildJob& j, const int* body, const float* x0, hipS
   IPC_TRY(B.leaf_vid.ensure(sizeof(int4) * B.info.slots[0]));
   IPC_TRY(B.node_lo.ensure(sizeof(float4) * total));
   IPC_TRY(B.node_hi.ensure(sizeof(float4) * total));
-  IPC_TRY(B.node_body.ensure(sizeof(int2) * total));
   bool resort = B.builds_since_sort < 0 || B.builds_since_sort >= kResortEvery || B.n_sorted != n;
   if (resort) {
     hipLaunchKernelGGL(init_bounds_kernel, dim3(1), dim3(64), 0, s, B.bounds.as<int>());
@@ -745,13 +731,12 @@ LeafJob leaf_job(const BuildJob& j) {
   L.leaf_vid = B.leaf_vid.as<int4>();
   L.node_lo = B.node_lo.as<float4>();
   L.node_hi = B.node_hi.as<float4>();
-  L.node_body = B.node_body.as<int2>();
   L.info = B.info;
   return L;
 }
 
 TopJob top_job(const Bvh& B) {
-  return TopJob{B.node_lo.as<float4>(), B.node_hi.as<float4>(), B.node_body.as<int2>(), B.info};
+  return TopJob{B.node_lo.as<float4>(), B.node_hi.as<float4>(), B.info};
 }
 
 }  // namespace
@@ -863,7 +848,6 @@ QueryArgs query_args(const QueryLaunch& q) {
   A.info = q.bvh->info;
   A.node_lo = q.bvh->node_lo.as<float4>();
   A.node_hi = q.bvh->node_hi.as<float4>();
-  A.node_body = q.bvh->node_body.as<int2>();
   A.leaf_vid = q.bvh->leaf_vid.as<int4>();
   A.rec = q.rec;
   A.ids = q.ids;

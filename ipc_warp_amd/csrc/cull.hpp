@@ -36,8 +36,8 @@ struct Bvh {
   int n = 0;
   LevelInfo info{};
   DevBuf keys, keys_sorted;   // uint64 Morton | id
-  DevBuf node_lo, node_hi;    // float4 per slot of every level (leaves first)
-  DevBuf node_body;           // int2 (min, max) body id per slot; (b, b) for a leaf
+  DevBuf node_lo, node_hi;    // float4 per slot of every level (leaves first): xyz the box,
+                              // w the bits of the (min, max) body id below; (b, b) for a leaf
   DevBuf leaf_id;             // primitive id per leaf
   DevBuf leaf_vid;            // int4 per leaf slot: vertex ids (v2 = -1 for an edge), primitive id
   DevBuf temp;                // radix sort workspace
