@@ -22,6 +22,8 @@ struct ipc_ctx {
   DevBuf cand_ee, cand_pt; // int4 candidate quadruples
   DevBuf cand_ids_ee, cand_ids_pt;  // int2 candidate primitive ids (new_intersections)
   int cap_ee = 0, cap_pt = 0;
+  // dispatch order of the paired DCD and CCD query launches (QueryOrder)
+  DevBuf qcost_dcd, qperm_dcd, qcost_ccd, qperm_ccd;
   // fixed per-body BVHs (ipc_ctx_init_fixed_bodies)
   int nb = 0, max_verts = 0, cap_pairs = 0, cap_tasks_ee = 0, cap_tasks_pt = 0;
   Forest fe, ft;
@@ -58,6 +60,21 @@ unsigned long long* toi_bits(ipc_ctx* c) { return c->small.as<unsigned long long
 int* counters(ipc_ctx* c) { return reinterpret_cast<int*>(c->small.as<char>() + 64); }
 
 constexpr unsigned long long kOneBits = 0x3FF0000000000000ull;
+
+// The dispatch order of a paired query launch over n_a EE and n_b PT
+// queries; the step counts start at zero with the buffers.
+hipError_t query_order(DevBuf& cost, DevBuf& perm, int n_a, int n_b, hipStream_t s, QueryOrder& o) {
+  o = QueryOrder{};
+  const int ba = query_blocks(n_a), bb = query_blocks(n_b);
+  if (ba <= 0 || bb <= 0) return hipSuccess;
+  const size_t need = sizeof(int) * 4 * size_t(ba + bb);
+  const bool fresh = cost.bytes < need;
+  hipError_t e = cost.ensure(need);
+  if (e == hipSuccess) e = perm.ensure(sizeof(int) * size_t(ba + bb));
+  if (e == hipSuccess && fresh) e = launch_zero_i32(cost.as<int>(), 4 * (ba + bb), s);
+  if (e == hipSuccess) o = QueryOrder{cost.as<int>(), perm.as<int>(), ba, bb};
+  return e;
+}
 
 // Runs `launch` between two events on `s` when the context is timing.
 template <typename F>
@@ -136,6 +153,7 @@ int ipc_ctx_destroy(ipc_ctx* c) {
   c->vert_base.release(); c->vert_ids.release(); c->body_lo.release(); c->body_hi.release();
   c->frames.release(); c->world_lo.release(); c->world_hi.release(); c->pairs.release();
   c->tasks_ee.release(); c->tasks_pt.release();
+  c->qcost_dcd.release(); c->qperm_dcd.release(); c->qcost_ccd.release(); c->qperm_ccd.release();
   if (c->host) (void)hipHostFree(c->host);
   delete c;
   return IPC_OK;
@@ -293,10 +311,14 @@ int ipc_detect_collision(ipc_ctx* c, const float* x, float thickness, float buff
   const int n_tri = c->nv > 0 ? c->nf : 0;
   BuildJob jobs[2] = {{&c->edge_bvh, BOX_EDGE, c->ne, c->edges.as<int>(), nullptr, thickness + buffer},
                       {&c->tri_bvh, BOX_TRI, n_tri, c->tris.as<int>(), nullptr, 0.0f}};
-  int st = ctx_hip(timed(c, IPC_TIMER_BUILD_EDGE, s, [&] {
-                     return build_bvhs(jobs, 2, c->body.as<int>(), x, zero_prologue(ee_count, pt_count), s);
-                   }),
-                   "detect_collision BVHs");
+  QueryOrder qo;
+  int st = ctx_hip(query_order(c->qcost_dcd, c->qperm_dcd, c->ne, n_tri > 0 ? c->nv : 0, s, qo),
+                   "detect_collision order buffers");
+  if (st) return st;
+  st = ctx_hip(timed(c, IPC_TIMER_BUILD_EDGE, s, [&] {
+                 return build_bvhs(jobs, 2, c->body.as<int>(), x, zero_prologue(ee_count, pt_count), s, &qo);
+               }),
+               "detect_collision BVHs");
   if (st) return st;
   const int* order = nullptr;
   if (n_tri > 0) st = ctx_hip(point_order(c->vorder, c->nv, x, s, &order), "detect_collision order");
@@ -305,7 +327,8 @@ int ipc_detect_collision(ipc_ctx* c, const float* x, float thickness, float buff
                       dcd_pt_query(c, x, thickness, buffer, exclude_same_body, pt, pt_ids, pt_capacity, pt_count,
                                    order)};
   if (n_tri == 0) q[1].n_query = 0;
-  return ctx_hip(timed(c, IPC_TIMER_QUERY_DCD_EE, s, [&] { return run_queries(q, 2, s); }), "detect_collision query");
+  return ctx_hip(timed(c, IPC_TIMER_QUERY_DCD_EE, s, [&] { return run_queries(q, 2, s, &qo); }),
+                 "detect_collision query");
 }
 
 // Edge-triangle crossings of the surface at x: every edge's bounds against
@@ -403,7 +426,11 @@ int ipc_collision_free_step(ipc_ctx* c, const float* x, const float* dx, float p
     BuildPrologue pro = zero_prologue(cnt + 0, cnt + 1);
     pro.toi = toi_bits(c);
     pro.toi_init = kOneBits;
-    st = ctx_hip(timed(c, IPC_TIMER_BUILD_EDGE, s, [&] { return build_bvhs(jobs, 2, c->body.as<int>(), x, pro, s); }),
+    QueryOrder qo;
+    st = ctx_hip(query_order(c->qcost_ccd, c->qperm_ccd, c->ne, n_tri > 0 ? c->nv : 0, s, qo),
+                 "collision_free_step order buffers");
+    if (st) return st;
+    st = ctx_hip(timed(c, IPC_TIMER_BUILD_EDGE, s, [&] { return build_bvhs(jobs, 2, c->body.as<int>(), x, pro, s, &qo); }),
                  "collision_free_step BVHs");
     if (st) return st;
     const int* order = nullptr;
@@ -413,7 +440,7 @@ int ipc_collision_free_step(ipc_ctx* c, const float* x, const float* dx, float p
                         ccd_pt_query(c, x, x1, padding, exclude_same_body, c->cand_pt.as<int>(), nullptr, c->cap_pt,
                                      cnt + 1, order)};
     if (n_tri == 0) q[1].n_query = 0;
-    st = ctx_hip(timed(c, IPC_TIMER_QUERY_CCD_EE, s, [&] { return run_queries(q, 2, s); }),
+    st = ctx_hip(timed(c, IPC_TIMER_QUERY_CCD_EE, s, [&] { return run_queries(q, 2, s, &qo); }),
                  "collision_free_step query");
     if (!st)
       st = ctx_hip(timed(c, IPC_TIMER_TOI_EE, s, [&] {

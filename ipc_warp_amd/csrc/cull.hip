@@ -23,7 +23,9 @@
 // per query: visiting a node, lane r tests the node's child r, and the
 // group's ballot of overlapping children becomes one stack entry (level,
 // first child, 8-bit mask) in LDS. Accepted pairs are staged in LDS and
-// each workgroup reserves its output range with one global atomicAdd.
+// each workgroup reserves its output range with one global atomicAdd. The
+// paired launches of a DCD or CCD call dispatch their workgroups heaviest
+// first, by the step counts of the previous call (QueryOrder).
 #include <hipcub/hipcub.hpp>
 
 #include "cull.hpp"
@@ -312,8 +314,80 @@ __device__ __forceinline__ void top_block(const TopJob& T) {
   }
 }
 
-// Workgroup j reduces the top levels of job j.
-__global__ void __launch_bounds__(1024) reduce_top_kernel(TopJob T0, TopJob T1) {
+// ------------------------------------------------------- dispatch order ----
+// A launch's workgroups go to the XCDs round-robin: raw workgroup b runs on
+// XCD b % 8, as that XCD's (b / 8)-th. In a paired query launch XCD x serves
+// a[x] EE workgroups (one contiguous range of the Morton-ordered queries,
+// from sa[x]; as many as there are raw b < n_a with b % 8 == x) and b[x] PT
+// workgroups from sb[x] likewise, so each XCD's L2 holds its region's part
+// of the trees.
+struct XcdShare {
+  int a[kXcds], b[kXcds], sa[kXcds], sb[kXcds];
+};
+
+__device__ __forceinline__ void xcd_share(int n_a, int n_b, XcdShare& X) {
+  const int n = n_a + n_b;
+  int sa = 0, sb = 0;
+  for (int x = 0; x < kXcds; ++x) {
+    const int cnt = (n + kXcds - 1 - x) / kXcds, a = (n_a + kXcds - 1 - x) / kXcds;
+    X.a[x] = a;
+    X.b[x] = cnt - a;
+    X.sa[x] = sa;
+    X.sb[x] = sb;
+    sa += a;
+    sb += cnt - a;
+  }
+}
+
+constexpr int kOrderBuckets = 16;  // 4 steps per bucket; 60 steps and more share the first
+
+struct OrderSmem {
+  int hist[kOrderBuckets];
+  XcdShare X;
+};
+
+// perm[raw] for XCD x's share of a paired launch, from the previous launch's
+// per-wave step counts: a counting sort of the XCD's workgroups by
+// decreasing cost (the largest of a workgroup's four waves'), written to
+// the XCD's raw slots x, x + 8, ... in order. One workgroup per XCD.
+__device__ __forceinline__ void order_block(const QueryOrder& O, int x, OrderSmem& sm) {
+  const int t = threadIdx.x, nt = blockDim.x;
+  if (t == 0) xcd_share(O.n_a, O.n_b, sm.X);
+  if (t < kOrderBuckets) sm.hist[t] = 0;
+  __syncthreads();
+  const int a = sm.X.a[x], cnt = a + sm.X.b[x], sa = sm.X.sa[x], sb = O.n_a + sm.X.sb[x];
+  auto item = [&](int i) { return i < a ? sa + i : sb + (i - a); };  // the XCD's i-th logical workgroup
+  auto bucket = [&](int c) {
+    const int4 w = reinterpret_cast<const int4*>(O.cost)[c];
+    const int m = min(max(max(max(w.x, w.y), max(w.z, w.w)), 0) / 4, kOrderBuckets - 1);
+    return kOrderBuckets - 1 - m;  // the most steps first
+  };
+  for (int i = t; i < cnt; i += nt) atomicAdd(&sm.hist[bucket(item(i))], 1);
+  __syncthreads();
+  if (t == 0) {  // exclusive scan
+    int run = 0;
+    for (int k = 0; k < kOrderBuckets; ++k) {
+      const int v = sm.hist[k];
+      sm.hist[k] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  for (int i = t; i < cnt; i += nt) {
+    const int c = item(i);
+    const int j = atomicAdd(&sm.hist[bucket(c)], 1);  // < cnt, so j * 8 + x < n_a + n_b
+    O.perm[j * kXcds + x] = c;
+  }
+}
+
+// Workgroup j < n_top reduces the top levels of job j; workgroups n_top +
+// x, when launched, compute the dispatch order O for XCD x.
+__global__ void __launch_bounds__(1024) reduce_top_kernel(TopJob T0, TopJob T1, int n_top, QueryOrder O) {
+  if (int(blockIdx.x) >= n_top) {
+    __shared__ OrderSmem sm;
+    order_block(O, int(blockIdx.x) - n_top, sm);
+    return;
+  }
   if (blockIdx.x == 0) top_block(T0);
   else top_block(T1);
 }
@@ -425,9 +499,10 @@ struct QuerySmem {
   BlockStage stage;
 };
 
-// Workgroup `bid` of a query launch over A.
+// Workgroup `bid` of a query launch over A. With wcost, each wave stores
+// its number of steps in wcost[wave].
 template <int MODE>
-__device__ __forceinline__ void query_block(const QueryArgs& A, int bid, QuerySmem& sm) {
+__device__ __forceinline__ void query_block(const QueryArgs& A, int bid, QuerySmem& sm, int* wcost) {
   auto& stack = sm.stack;
   int* level_off = sm.level_off;
   int* level_cnt = sm.level_cnt;
@@ -494,7 +569,9 @@ __device__ __forceinline__ void query_block(const QueryArgs& A, int bid, QuerySm
     stack[grp][0] = mask_entry(A.info.top, 1u, level_off[A.info.top - 1]);  // the root
     sp = 1;
   }
+  int steps = 0;
   while (true) {
+    ++steps;
     bool active = sp > 0;
     int L = 0, cbase = 0;
     unsigned take = 0u;
@@ -568,21 +645,34 @@ __device__ __forceinline__ void query_block(const QueryArgs& A, int bid, QuerySm
     }
     if (!__any(sp > 0)) break;
   }
+  if (wcost && lane == 0) wcost[tid >> 6] = steps;
   stage_flush(stage, out);
 }
 
 template <int MODE>
 __global__ void __launch_bounds__(kQueryBlock) query_kernel(QueryArgs A) {
   __shared__ QuerySmem sm;
-  query_block<MODE>(A, xcd_local(blockIdx.x, gridDim.x), sm);
+  query_block<MODE>(A, xcd_local(blockIdx.x, gridDim.x), sm, nullptr);
 }
 
-// Two queries in one launch: workgroups [0, blocks_a) serve A, the rest B.
+// Two queries in one launch: logical workgroups [0, blocks_a) serve A, the
+// rest B. With perm (a permutation of the launch's workgroups, see
+// QueryOrder), raw workgroup b serves logical workgroup perm[b] and records
+// its waves' step counts in cost; without, each half is spread over the
+// XCDs in contiguous ranges.
 template <int MA, int MB>
-__global__ void __launch_bounds__(kQueryBlock) query_pair_kernel(QueryArgs A, QueryArgs B, int blocks_a) {
+__global__ void __launch_bounds__(kQueryBlock) query_pair_kernel(QueryArgs A, QueryArgs B, int blocks_a,
+                                                                 const int* __restrict__ perm, int* cost) {
   __shared__ QuerySmem sm;
-  if (int(blockIdx.x) < blocks_a) query_block<MA>(A, xcd_local(blockIdx.x, blocks_a), sm);
-  else query_block<MB>(B, xcd_local(blockIdx.x - blocks_a, gridDim.x - blocks_a), sm);
+  const int n = gridDim.x, raw = blockIdx.x;
+  int c = raw < blocks_a ? xcd_local(raw, blocks_a) : blocks_a + xcd_local(raw - blocks_a, n - blocks_a);
+  if (perm) {
+    const int pc = perm[raw];
+    if (pc >= 0 && pc < n) c = pc;
+  }
+  int* wcost = cost ? cost + (kQueryBlock / 64) * c : nullptr;
+  if (c < blocks_a) query_block<MA>(A, c, sm, wcost);
+  else query_block<MB>(B, c - blocks_a, sm, wcost);
 }
 
 // x1 = x - dx (_add_dx, contact.py:49-52)
@@ -618,8 +708,7 @@ __global__ void init_range_kernel(int* out) {
   out[1] = (int)0x80000000;
 }
 __global__ void zero_i32_kernel(int* p, int n) {
-  int i = threadIdx.x;
-  if (i < n) p[i] = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
 // ------------------------------------------------------------- host side ----
@@ -742,7 +831,7 @@ TopJob top_job(const Bvh& B) {
 }  // namespace
 
 hipError_t build_bvhs(const BuildJob* jobs, int n_jobs, const int* body, const float* x0, const BuildPrologue& pro,
-                      hipStream_t s) {
+                      hipStream_t s, const QueryOrder* order) {
   if (n_jobs < 0 || n_jobs > 2 || pro.n_zero < 0 || pro.n_zero > 4 || pro.n_range < 0 || pro.n_range > 64)
     return hipErrorInvalidValue;
   const BuildJob* live[2] = {nullptr, nullptr};
@@ -766,10 +855,12 @@ hipError_t build_bvhs(const BuildJob* jobs, int n_jobs, const int* body, const f
                      pro);
   bool top0 = live[0]->bvh->info.top > kInBlockLevels;
   bool top1 = n_live > 1 && live[1]->bvh->info.top > kInBlockLevels;
-  if (top0 || top1) {
+  const bool ord = order && order->cost && order->perm && order->n_a > 0 && order->n_b > 0;
+  if (top0 || top1 || ord) {
     TopJob t0 = top_job(*live[0]->bvh);
     TopJob t1 = n_live > 1 ? top_job(*live[1]->bvh) : t0;
-    hipLaunchKernelGGL(reduce_top_kernel, dim3(n_live), dim3(1024), 0, s, t0, t1);
+    hipLaunchKernelGGL(reduce_top_kernel, dim3(n_live + (ord ? kXcds : 0)), dim3(1024), 0, s, t0, t1, n_live,
+                       ord ? *order : QueryOrder{});
   }
   return hipGetLastError();
 }
@@ -858,7 +949,9 @@ QueryArgs query_args(const QueryLaunch& q) {
 
 }  // namespace
 
-hipError_t run_queries(const QueryLaunch* q, int n_q, hipStream_t s) {
+int query_blocks(int n_query) { return n_query > 0 ? nblocks(n_query, kGroupsPerBlock) : 0; }
+
+hipError_t run_queries(const QueryLaunch* q, int n_q, hipStream_t s, const QueryOrder* order) {
   const QueryLaunch* live[2] = {nullptr, nullptr};
   int n_live = 0;
   for (int k = 0; k < n_q && k < 2; ++k)
@@ -882,10 +975,13 @@ hipError_t run_queries(const QueryLaunch* q, int n_q, hipStream_t s) {
   QueryArgs A = query_args(*live[0]), B = query_args(*live[1]);
   int blocks_a = nblocks(live[0]->n_query, kGroupsPerBlock), blocks_b = nblocks(live[1]->n_query, kGroupsPerBlock);
   dim3 grid(blocks_a + blocks_b);
+  const bool use = order && order->cost && order->perm && order->n_a == blocks_a && order->n_b == blocks_b;
+  const int* perm = use ? order->perm : nullptr;
+  int* cost = use ? order->cost : nullptr;
   if (live[0]->mode == Q_DCD_EE && live[1]->mode == Q_DCD_PT)
-    hipLaunchKernelGGL((query_pair_kernel<Q_DCD_EE, Q_DCD_PT>), grid, block, 0, s, A, B, blocks_a);
+    hipLaunchKernelGGL((query_pair_kernel<Q_DCD_EE, Q_DCD_PT>), grid, block, 0, s, A, B, blocks_a, perm, cost);
   else if (live[0]->mode == Q_CCD_EE && live[1]->mode == Q_CCD_PT)
-    hipLaunchKernelGGL((query_pair_kernel<Q_CCD_EE, Q_CCD_PT>), grid, block, 0, s, A, B, blocks_a);
+    hipLaunchKernelGGL((query_pair_kernel<Q_CCD_EE, Q_CCD_PT>), grid, block, 0, s, A, B, blocks_a, perm, cost);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
@@ -915,7 +1011,9 @@ hipError_t index_range(const int* a, long long n, int* out_dev, hipStream_t s) {
 }
 
 hipError_t launch_zero_i32(int* p, int n, hipStream_t s) {
-  hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, s, p, n);
+  if (n <= 0) return hipSuccess;
+  const int blocks = n <= 64 ? 1 : (nblocks(n, 256) < 1024 ? nblocks(n, 256) : 1024);
+  hipLaunchKernelGGL(zero_i32_kernel, dim3(blocks), dim3(n <= 64 ? 64 : 256), 0, s, p, n);
   return hipGetLastError();
 }
 

@@ -100,10 +100,24 @@ __device__ __forceinline__ void run_prologue(const BuildPrologue& p) {
   if (t == 0 && p.toi) *p.toi = p.toi_init;
 }
 
+// Dispatch order of a paired query launch (n_a workgroups of EE queries,
+// then n_b of PT queries). The launch records each wave's step count in
+// cost[4 c + w] (c the logical workgroup); the next build's top-level launch
+// turns those counts into perm[raw workgroup] = logical workgroup, ordering
+// each XCD's share of the workgroups by decreasing cost. A query launch's
+// length is set by its slowest waves: started first, they overlap the rest.
+// Any permutation returns the same pairs.
+struct QueryOrder {
+  int* cost = nullptr;  // 4 (n_a + n_b) ints, zero before the first launch
+  int* perm = nullptr;  // n_a + n_b ints
+  int n_a = 0, n_b = 0;
+};
+
 // Builds up to two trees with one leaf launch and one top-level launch
-// (plus a Morton re-sort for a tree whose leaf order is due).
+// (plus a Morton re-sort for a tree whose leaf order is due). With `order`,
+// the top-level launch also computes order->perm from order->cost.
 hipError_t build_bvhs(const BuildJob* jobs, int n_jobs, const int* body, const float* x0, const BuildPrologue& pro,
-                      hipStream_t s);
+                      hipStream_t s, const QueryOrder* order = nullptr);
 hipError_t build_bvh(Bvh& B, int mode, int n, const int* prims, const int* body, const float* x0, const float* x1,
                      float pad, hipStream_t s);
 
@@ -132,7 +146,11 @@ struct QueryLaunch {
 hipError_t run_query(const QueryLaunch& q, hipStream_t s);
 // Up to two queries in one launch (the EE and PT halves of a DCD or CCD
 // call: modes Q_DCD_EE + Q_DCD_PT or Q_CCD_EE + Q_CCD_PT, in that order).
-hipError_t run_queries(const QueryLaunch* q, int n_q, hipStream_t s);
+// `order` (from the same call's build_bvhs) applies when both halves run
+// and its block counts match the launch (query_blocks of each n_query).
+hipError_t run_queries(const QueryLaunch* q, int n_q, hipStream_t s, const QueryOrder* order = nullptr);
+// Workgroups of a world query launch over n queries.
+int query_blocks(int n_query);
 hipError_t launch_sub(int n3, const float* x, const float* dx, float* out, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t s);
 hipError_t launch_zero_i32(int* p, int n, hipStream_t s);

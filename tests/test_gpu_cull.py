@@ -108,6 +108,29 @@ def test_collision_free_step_matches_oracle(dev, oracle):
     assert S.collision_free_step(torch.zeros_like(dxt)) == 1.0
 
 
+def test_dispatch_order_keeps_the_sets(dev, oracle):
+    """The paired DCD and CCD query launches order their workgroups from the
+    previous launch's step counts: repeated calls on moving positions keep
+    the exact contact sets, CCD candidate counts and step bound."""
+    soup, vel = scenes.icosphere_grid(seed=16, per_axis=2, spacing=1.0003, subdivisions=3)
+    rng = np.random.default_rng(16)
+    S = _solver(dev, soup, exclude=True)
+    for _ in range(4):
+        soup.x = (soup.x + rng.uniform(-2e-4, 2e-4, soup.x.shape)).astype(np.float32)
+        S.x.copy_(torch.from_numpy(soup.x).to(dev))
+        n_ee, n_pt = _check_dcd(S, soup, True)
+        assert n_ee + n_pt > 0
+        dx = (-1e-2 * vel[soup.body]).astype(np.float32)
+        x1 = soup.x - dx
+        toi = S.raw_toi(torch.from_numpy(dx).to(dev))
+        ref = oracle.ccd_toi(soup.x, x1, soup.edges, soup.triangles, soup.body, 1e-7, True, relative=True, threads=8)
+        assert abs(toi - ref) <= 1e-12
+        r_ee, _ = oracle.ccd_candidates_ee(soup.x, x1, soup.edges, soup.body, 1e-7, True)
+        r_pt, _ = oracle.ccd_candidates_pt(soup.x, x1, soup.triangles, soup.body, 1e-7, True)
+        assert S.last_ccd_counts() == (len(r_ee), len(r_pt))
+        assert len(r_ee) > 0 and len(r_pt) > 0
+
+
 def test_degenerate_soups(dev, oracle):
     # one edge and no triangles; two crossing edges
     x = np.array([[0, 0, 0], [1, 0, 0], [0.5, -0.5, 4e-4], [0.5, 0.5, 4e-4]], np.float32)
