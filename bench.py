@@ -10,8 +10,9 @@ One step is one Newton iteration's contact work, in the reference's order
 (affine_body_dynamics.py:427-505, dynamic_contacts.py:299-308,
 contact.py:539-585):
   1. detect_collision: LBVH culling + fp32 distance filter, EE and PT;
-  2. contact_hessian_ee / _pt: barrier gradient + PSD Hessian into triplets;
-  3. contact_energy_ee / _pt: barrier energy (FP64 sum);
+  2. contact_hessian_ee + _pt (one launch): barrier gradient + PSD Hessian
+     into triplets;
+  3. contact_energy_ee + _pt (one launch): barrier energy (FP64 sum);
   4. collision_free_step(dx = -h v): swept culling + FP64 CCD TOI.
 value = (assembled contacts + CCD candidates) per second over the timed
 steps.
@@ -245,36 +246,28 @@ def main() -> int:
     b = torch.zeros_like(x)
     e_dev = torch.zeros(1, dtype=torch.float64, device=dev)
     e_host = torch.zeros(1, dtype=torch.float64, pin_memory=True)
-    from ipc_warp_amd import ops
 
     counts = {}
     # Single-kernel entry points are bracketed with events on torch's current
     # stream (the stream the library launches on); builds, queries and TOI
     # launches inside the context are bracketed by the library itself.
     kev = {k: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for k in ("hessian_ee", "hessian_pt", "energy")}
+           for k in ("hessian", "energy")}
     kacc = {k: 0.0 for k in kev}
 
     def step(timed: bool):
         b.zero_()
         S.detect_collision()
         n_ee, n_pt = S.n_contacts, S.n_contacts_pt
-        trip = S.new_triplets()
+        # the EE and PT halves of the Hessian and of the energy each share one
+        # launch; the FP64 energy is read after the CCD
         if timed:
-            kev["hessian_ee"][0].record()
-        S.contact_hessian_ee(trip, b)
+            kev["hessian"][0].record()
+        S.compute_collision_hessian(b)
         if timed:
-            kev["hessian_ee"][1].record()
-            kev["hessian_pt"][0].record()
-        S.contact_hessian_pt(trip, b)
-        if timed:
-            kev["hessian_pt"][1].record()
-        # the energy kernels alone are bracketed; the FP64 sum is read after
-        e_dev.zero_()
-        if timed:
+            kev["hessian"][1].record()
             kev["energy"][0].record()
-        ops.contact_energy_ee(S.x, S.xcs, S.contacts_new.list, n_ee, S.l0, S.contact_stiffness, e_dev)
-        ops.contact_energy_pt(S.x, S.contacts_pt.list, n_pt, S.l0, S.contact_stiffness, e_dev)
+        S.collision_energy_into(e_dev)
         if timed:
             kev["energy"][1].record()
         # the energy travels to pinned host memory ahead of the CCD, whose
@@ -328,8 +321,7 @@ def main() -> int:
     E, F, V = int(soup.edges.shape[0]), int(soup.triangles.shape[0]), int(soup.x.shape[0])
     tree = lambda n, nv: (32 + 4 * nv) * n + 32 * ((n + 6) // 7)
     alg = {
-        "hessian_ee": 864 * counts["ee"],
-        "hessian_pt": 816 * counts["pt"],
+        "hessian": 864 * counts["ee"] + 816 * counts["pt"],
         "energy": 112 * counts["ee"] + 64 * counts["pt"],
         "query_dcd": tree(E, 2) + 32 * E + 24 * counts["ee"] + tree(F, 3) + 12 * V + 24 * counts["pt"],
         "query_ccd": tree(E, 2) + 32 * E + 16 * counts["ccd_ee"] + tree(F, 3) + 24 * V + 16 * counts["ccd_pt"],

@@ -61,6 +61,15 @@ int ipc_contact_hessian_ee(const float* x, const float* xcs, int n_vertices, con
 int ipc_contact_hessian_pt(const float* x, int n_vertices, const int* contacts, int n_contacts, float l0, double kappa, int offset,
                            int* rows, int* cols, float* vals, float* b, void* stream);
 
+/* ------------------------------------- barrier Hessian, both lists ---
+ * Replaces the contact part of RodComplexBC.compute_A
+ * (dynamic_contacts.py:281-288): contact_hessian_ee over the n_ee EE
+ * contacts at triplet offset 0, then contact_hessian_pt over the n_pt PT
+ * contacts at offset n_ee, in one launch. rows/cols/vals hold
+ * 16 (n_ee + n_pt) blocks; results equal the two calls above. */
+int ipc_collision_hessian(const float* x, const float* xcs, int n_vertices, const int* ee, int n_ee, const int* pt,
+                          int n_pt, float l0, double kappa, int* rows, int* cols, float* vals, float* b, void* stream);
+
 /* ------------------------------------------------- barrier energy ---
  * Replace contact_energy_ee / contact_energy_pt(states, soup, contacts, e,
  * kappa) (dynamic_contacts.py:374-398, launched at :304-305). The sum over
@@ -70,6 +79,14 @@ int ipc_contact_energy_ee(const float* x, const float* xcs, int n_vertices, cons
                           double kappa, double* energy, void* stream);
 int ipc_contact_energy_pt(const float* x, int n_vertices, const int* contacts, int n_contacts, float l0, double kappa,
                           double* energy, void* stream);
+
+/* -------------------------------------- barrier energy, both lists ---
+ * Replaces the kernels of compute_collision_energy
+ * (dynamic_contacts.py:300-305): energy[0] is set to zero (the reference's
+ * wp.zeros), then the EE and PT sums are added, in FP64. The caller reads
+ * energy[0] (the reference's host transfer, :306-307). */
+int ipc_collision_energy(const float* x, const float* xcs, int n_vertices, const int* ee, int n_ee, const int* pt,
+                         int n_pt, float l0, double kappa, double* energy, void* stream);
 
 /* ----------------------------------------------- contact distance ---
  * Replace get_contact_points / get_contact_points_pt (contact.py:148-161)

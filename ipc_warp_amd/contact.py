@@ -298,18 +298,22 @@ class ContactSolverBase:
         triplets for (n_ee + n_pt) * 16 blocks, the EE kernel, then the PT
         kernel at offset n_ee; forces are added into b."""
         trip = self.new_triplets()
-        self.contact_hessian_ee(trip, b)
-        self.contact_hessian_pt(trip, b)
+        ops.collision_hessian(self.x, self.xcs, self.contacts_new.list, self.n_contacts, self.contacts_pt.list,
+                              self.n_contacts_pt, self.l0, self.contact_stiffness, trip.rows, trip.cols, trip.vals, b)
         self.collision_triplets = trip
         return trip
 
     def collision_energy_raw(self) -> float:
         """Barrier energy of the current contact lists (no DCD, no h^2)."""
-        e = torch.zeros(1, dtype=torch.float64, device=self.device)
-        ops.contact_energy_ee(self.x, self.xcs, self.contacts_new.list, self.n_contacts, self.l0,
-                              self.contact_stiffness, e)
-        ops.contact_energy_pt(self.x, self.contacts_pt.list, self.n_contacts_pt, self.l0, self.contact_stiffness, e)
+        e = torch.empty(1, dtype=torch.float64, device=self.device)
+        self.collision_energy_into(e)
         return float(e.item())
+
+    def collision_energy_into(self, e: torch.Tensor):
+        """The barrier energy of the current contact lists into the device
+        double e[0], without a host transfer (dynamic_contacts.py:300-305)."""
+        ops.collision_energy(self.x, self.xcs, self.contacts_new.list, self.n_contacts, self.contacts_pt.list,
+                             self.n_contacts_pt, self.l0, self.contact_stiffness, e)
 
     def compute_collision_energy(self) -> float:
         """dynamic_contacts.py:299-308: DCD, then h^2 * sum of barrier energies."""

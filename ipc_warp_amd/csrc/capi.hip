@@ -66,6 +66,17 @@ int ipc_contact_hessian_pt(const float* x, int n_vertices, const int* contacts, 
                     "contact_hessian_pt");
 }
 
+int ipc_collision_hessian(const float* x, const float* xcs, int n_vertices, const int* ee, int n_ee, const int* pt,
+                          int n_pt, float l0, double kappa, int* rows, int* cols, float* vals, float* b, void* stream) {
+  int st = check_hessian_args(x, n_vertices, ee, n_ee, l0, 0, rows, cols, vals, b);
+  if (st == IPC_OK) st = check_hessian_args(x, n_vertices, pt, n_pt, l0, n_ee, rows, cols, vals, b);
+  if (st != IPC_OK || (n_ee == 0 && n_pt == 0)) return st;
+  if (n_ee > 0 && !xcs) return fail(IPC_EINVAL, "collision_hessian: xcs (rest positions) is required");
+  return hip_status(ipcd::launch_collision_hessian(x, xcs, n_vertices, ee, n_ee, pt, n_pt, l0, kappa, rows, cols, vals, b,
+                                                   static_cast<hipStream_t>(stream)),
+                    "collision_hessian");
+}
+
 static int check_energy_args(const float* x, int nv, const int* contacts, int n, float l0, double* energy) {
   if (n < 0 || nv < 0) return fail(IPC_EINVAL, "contact_energy: negative n_vertices or n_contacts");
   if (n == 0) return IPC_OK;
@@ -92,6 +103,18 @@ int ipc_contact_energy_pt(const float* x, int n_vertices, const int* contacts, i
   return hip_status(ipcd::launch_contact_energy(false, x, nullptr, n_vertices, contacts, n_contacts, l0, kappa, energy,
                                                 static_cast<hipStream_t>(stream)),
                     "contact_energy_pt");
+}
+
+int ipc_collision_energy(const float* x, const float* xcs, int n_vertices, const int* ee, int n_ee, const int* pt,
+                         int n_pt, float l0, double kappa, double* energy, void* stream) {
+  if (!energy) return fail(IPC_EINVAL, "collision_energy: null energy");
+  int st = check_energy_args(x, n_vertices, ee, n_ee, l0, energy);
+  if (st == IPC_OK) st = check_energy_args(x, n_vertices, pt, n_pt, l0, energy);
+  if (st != IPC_OK) return st;
+  if (n_ee > 0 && !xcs) return fail(IPC_EINVAL, "collision_energy: xcs (rest positions) is required");
+  return hip_status(ipcd::launch_collision_energy(x, xcs, n_vertices, ee, n_ee, pt, n_pt, l0, kappa, energy,
+                                                  static_cast<hipStream_t>(stream)),
+                    "collision_energy");
 }
 
 static int check_points_args(const float* x, int nv, const int* contacts, int n) {

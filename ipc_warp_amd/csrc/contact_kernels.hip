@@ -116,21 +116,44 @@ __device__ __forceinline__ bool pair_pt(const float* __restrict__ x, int4 c, flo
 // idle; -1 there). A short list is then spread over more SIMDs, and each
 // wave meets fewer of the feature dispatch's branch paths, which a wave runs
 // one after another.
-__device__ __forceinline__ int contact_slot(int lpw) {
+// `block` is the workgroup's index within its half of a launch.
+__device__ __forceinline__ int contact_slot(int lpw, int block) {
   int lane = threadIdx.x & 63;
-  int wave = int((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  int wave = int((block * blockDim.x + threadIdx.x) >> 6);
   return lane < lpw ? wave * lpw + lane : -1;
 }
 
+// One contact list's Hessian launch (or half of a paired launch).
+struct HessianArgs {
+  const float* x;
+  const float* xcs;
+  int nv;
+  const int4* contacts;
+  int n;
+  float l0;
+  double kappa;
+  int offset;
+  int* rows;
+  int* cols;
+  float* vals;
+  float* b;
+  int lpw;
+};
+
 template <bool EE>
-__global__ void __launch_bounds__(256) contact_hessian_kernel(const float* __restrict__ x, const float* __restrict__ xcs,
-                                                              int nv, const int4* __restrict__ contacts, int n, float l0,
-                                                              double kappa, int offset, int* __restrict__ rows,
-                                                              int* __restrict__ cols, float* __restrict__ vals,
-                                                              float* __restrict__ b, int lpw) {
-  int i = contact_slot(lpw);
+__device__ __forceinline__ void hessian_block(const HessianArgs& A, int block) {
+  const float* __restrict__ x = A.x;
+  const float* __restrict__ xcs = A.xcs;
+  const int nv = A.nv, n = A.n, offset = A.offset;
+  const float l0 = A.l0;
+  const double kappa = A.kappa;
+  int* __restrict__ rows = A.rows;
+  int* __restrict__ cols = A.cols;
+  float* __restrict__ vals = A.vals;
+  float* __restrict__ b = A.b;
+  int i = contact_slot(A.lpw, block);
   if (i < 0 || i >= n) return;
-  int4 c = contacts[i];
+  const int4 c = A.contacts[i];
   CompressedHessian C;
   float force[12];
   bool active = valid4(c, nv) && (EE ? pair_ee(x, xcs, c, l0, kappa, true, C, force, nullptr)
@@ -177,6 +200,18 @@ __global__ void __launch_bounds__(256) contact_hessian_kernel(const float* __res
   }
 }
 
+template <bool EE>
+__global__ void __launch_bounds__(256) contact_hessian_kernel(HessianArgs A) {
+  hessian_block<EE>(A, blockIdx.x);
+}
+
+// Both halves of the contact Hessian in one launch: workgroups [0, blocks_e)
+// take the EE list, the rest the PT list.
+__global__ void __launch_bounds__(256) contact_hessian_pair_kernel(HessianArgs E, HessianArgs P, int blocks_e) {
+  if (int(blockIdx.x) < blocks_e) hessian_block<true>(E, blockIdx.x);
+  else hessian_block<false>(P, blockIdx.x - blocks_e);
+}
+
 // ------------------------------------------------------------ energy kernel ----
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -187,19 +222,33 @@ __device__ __forceinline__ double wave_sum(double v) {
 // contact_energy_ee / contact_energy_pt (dynamic_contacts.py:374-398): the
 // per-contact energies are summed in FP64 per wave, per block through LDS,
 // then one atomicAdd(double) per block into energy[0].
+struct EnergyArgs {
+  const float* x;
+  const float* xcs;
+  int nv;
+  const int4* contacts;
+  int n;
+  float l0;
+  double kappa;
+  double* energy;
+  int lpw;
+};
+
 template <bool EE>
-__global__ void __launch_bounds__(256) contact_energy_kernel(const float* __restrict__ x,
-                                                             const float* __restrict__ xcs, int nv,
-                                                             const int4* __restrict__ contacts, int n, float l0,
-                                                             double kappa, double* __restrict__ energy, int lpw) {
-  __shared__ double partial[4];
-  int i = contact_slot(lpw);
+__device__ __forceinline__ void energy_block(const EnergyArgs& A, int block, double* partial) {
+  const float* __restrict__ x = A.x;
+  const float* __restrict__ xcs = A.xcs;
+  const int nv = A.nv, n = A.n;
+  const float l0 = A.l0;
+  const double kappa = A.kappa;
+  double* __restrict__ energy = A.energy;
+  int i = contact_slot(A.lpw, block);
   double e = 0.0;
   if (i >= 0 && i < n) {
     CompressedHessian C;
     float force[12];
     double ei = 0.0;
-    int4 c = contacts[i];
+    int4 c = A.contacts[i];
     bool active = valid4(c, nv) && (EE ? pair_ee(x, xcs, c, l0, kappa, false, C, force, &ei)
                                        : pair_pt(x, c, l0, kappa, false, C, force, &ei));
     if (active) e = ei;
@@ -213,6 +262,20 @@ __global__ void __launch_bounds__(256) contact_energy_kernel(const float* __rest
     for (int w = 0; w < int(blockDim.x >> 6); ++w) s += partial[w];
     if (s != 0.0) unsafeAtomicAdd(energy, s);
   }
+}
+
+template <bool EE>
+__global__ void __launch_bounds__(256) contact_energy_kernel(EnergyArgs A) {
+  __shared__ double partial[4];
+  energy_block<EE>(A, blockIdx.x, partial);
+}
+
+// Both halves of the barrier energy in one launch (workgroups [0, blocks_e)
+// the EE list, the rest the PT list), both added into A.energy.
+__global__ void __launch_bounds__(256) contact_energy_pair_kernel(EnergyArgs E, EnergyArgs P, int blocks_e) {
+  __shared__ double partial[4];
+  if (int(blockIdx.x) < blocks_e) energy_block<true>(E, blockIdx.x, partial);
+  else energy_block<false>(P, blockIdx.x - blocks_e, partial);
 }
 
 // ----------------------------------------------------- contact points kernel ----
@@ -400,32 +463,72 @@ constexpr int kSpreadLanesEnergy = 32;  // energy (no Hessian, shorter chain)
 static inline int lanes_per_wave(int n, int spread = kSpreadLanes) { return n < kSpreadBelow ? spread : 64; }
 static inline int blocks_spread(int n, int lpw) { return blocks_for(blocks_for(n, lpw), 4); }  // 4 waves per block
 
+static HessianArgs hessian_args(const float* x, const float* xcs, int nv, const int* contacts, int n, float l0,
+                                double kappa, int offset, int* rows, int* cols, float* vals, float* b) {
+  return HessianArgs{x, xcs, nv, reinterpret_cast<const int4*>(contacts), n, l0, kappa, offset, rows, cols, vals, b,
+                     lanes_per_wave(n)};
+}
+
+static EnergyArgs energy_args(const float* x, const float* xcs, int nv, const int* contacts, int n, float l0,
+                              double kappa, double* energy) {
+  return EnergyArgs{x, xcs, nv, reinterpret_cast<const int4*>(contacts), n, l0, kappa, energy,
+                    lanes_per_wave(n, kSpreadLanesEnergy)};
+}
+
 hipError_t launch_contact_hessian(bool ee, const float* x, const float* xcs, int nv, const int* contacts, int n, float l0,
                                   double kappa, int offset, int* rows, int* cols, float* vals, float* b,
                                   hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const int4* c4 = reinterpret_cast<const int4*>(contacts);
-  const int lpw = lanes_per_wave(n);
+  const HessianArgs A = hessian_args(x, xcs, nv, contacts, n, l0, kappa, offset, rows, cols, vals, b);
   if (ee)
-    hipLaunchKernelGGL(contact_hessian_kernel<true>, dim3(blocks_spread(n, lpw)), dim3(256), 0, s, x, xcs, nv, c4, n,
-                       l0, kappa, offset, rows, cols, vals, b, lpw);
+    hipLaunchKernelGGL(contact_hessian_kernel<true>, dim3(blocks_spread(n, A.lpw)), dim3(256), 0, s, A);
   else
-    hipLaunchKernelGGL(contact_hessian_kernel<false>, dim3(blocks_spread(n, lpw)), dim3(256), 0, s, x, xcs, nv, c4,
-                       n, l0, kappa, offset, rows, cols, vals, b, lpw);
+    hipLaunchKernelGGL(contact_hessian_kernel<false>, dim3(blocks_spread(n, A.lpw)), dim3(256), 0, s, A);
+  return hipGetLastError();
+}
+
+hipError_t launch_collision_hessian(const float* x, const float* xcs, int nv, const int* ee, int n_ee, const int* pt,
+                                    int n_pt, float l0, double kappa, int* rows, int* cols, float* vals, float* b,
+                                    hipStream_t s) {
+  if (n_ee <= 0 || n_pt <= 0) {
+    hipError_t e = launch_contact_hessian(true, x, xcs, nv, ee, n_ee, l0, kappa, 0, rows, cols, vals, b, s);
+    if (e != hipSuccess) return e;
+    return launch_contact_hessian(false, x, nullptr, nv, pt, n_pt, l0, kappa, n_ee, rows, cols, vals, b, s);
+  }
+  const HessianArgs E = hessian_args(x, xcs, nv, ee, n_ee, l0, kappa, 0, rows, cols, vals, b);
+  const HessianArgs P = hessian_args(x, nullptr, nv, pt, n_pt, l0, kappa, n_ee, rows, cols, vals, b);
+  const int be = blocks_spread(n_ee, E.lpw), bp = blocks_spread(n_pt, P.lpw);
+  hipLaunchKernelGGL(contact_hessian_pair_kernel, dim3(be + bp), dim3(256), 0, s, E, P, be);
   return hipGetLastError();
 }
 
 hipError_t launch_contact_energy(bool ee, const float* x, const float* xcs, int nv, const int* contacts, int n, float l0,
                                  double kappa, double* energy, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const int4* c4 = reinterpret_cast<const int4*>(contacts);
-  const int lpw = lanes_per_wave(n, kSpreadLanesEnergy);
+  const EnergyArgs A = energy_args(x, xcs, nv, contacts, n, l0, kappa, energy);
   if (ee)
-    hipLaunchKernelGGL(contact_energy_kernel<true>, dim3(blocks_spread(n, lpw)), dim3(256), 0, s, x, xcs, nv, c4, n,
-                       l0, kappa, energy, lpw);
+    hipLaunchKernelGGL(contact_energy_kernel<true>, dim3(blocks_spread(n, A.lpw)), dim3(256), 0, s, A);
   else
-    hipLaunchKernelGGL(contact_energy_kernel<false>, dim3(blocks_spread(n, lpw)), dim3(256), 0, s, x, xcs, nv, c4,
-                       n, l0, kappa, energy, lpw);
+    hipLaunchKernelGGL(contact_energy_kernel<false>, dim3(blocks_spread(n, A.lpw)), dim3(256), 0, s, A);
+  return hipGetLastError();
+}
+
+__global__ void zero_f64_kernel(double* p) { *p = 0.0; }
+
+hipError_t launch_collision_energy(const float* x, const float* xcs, int nv, const int* ee, int n_ee, const int* pt,
+                                   int n_pt, float l0, double kappa, double* energy, hipStream_t s) {
+  hipLaunchKernelGGL(zero_f64_kernel, dim3(1), dim3(1), 0, s, energy);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || (n_ee <= 0 && n_pt <= 0)) return e;
+  if (n_ee <= 0 || n_pt <= 0) {
+    e = launch_contact_energy(true, x, xcs, nv, ee, n_ee, l0, kappa, energy, s);
+    if (e != hipSuccess) return e;
+    return launch_contact_energy(false, x, nullptr, nv, pt, n_pt, l0, kappa, energy, s);
+  }
+  const EnergyArgs E = energy_args(x, xcs, nv, ee, n_ee, l0, kappa, energy);
+  const EnergyArgs P = energy_args(x, nullptr, nv, pt, n_pt, l0, kappa, energy);
+  const int be = blocks_spread(n_ee, E.lpw), bp = blocks_spread(n_pt, P.lpw);
+  hipLaunchKernelGGL(contact_energy_pair_kernel, dim3(be + bp), dim3(256), 0, s, E, P, be);
   return hipGetLastError();
 }
 

@@ -9,6 +9,13 @@ hipError_t launch_contact_hessian(bool ee, const float* x, const float* xcs, int
                                   hipStream_t s);
 hipError_t launch_contact_energy(bool ee, const float* x, const float* xcs, int nv, const int* contacts, int n, float l0,
                                  double kappa, double* energy, hipStream_t s);
+// Both contact lists in one launch: the Hessian with the PT triplets at
+// offset n_ee; the energy after zeroing *energy.
+hipError_t launch_collision_hessian(const float* x, const float* xcs, int nv, const int* ee, int n_ee, const int* pt,
+                                    int n_pt, float l0, double kappa, int* rows, int* cols, float* vals, float* b,
+                                    hipStream_t s);
+hipError_t launch_collision_energy(const float* x, const float* xcs, int nv, const int* ee, int n_ee, const int* pt,
+                                   int n_pt, float l0, double kappa, double* energy, hipStream_t s);
 hipError_t launch_contact_points(bool ee, const float* x, const float* xcs, int nv, const int* contacts, int n, float* points,
                                  float* dists, int* features, hipStream_t s);
 hipError_t launch_collision_time(bool ee, const float* x0, const float* x1, int nv, const int* quads, int n, double* toi_out,

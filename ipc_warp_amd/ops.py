@@ -111,6 +111,32 @@ def contact_energy_pt(x, contacts, n, l0, kappa, e):
                                       _need(e, torch.float64, "e"), stream_handle(x.device)), "contact_energy_pt")
 
 
+def collision_hessian(x, xcs, ee, n_ee, pt, n_pt, l0, kappa, rows, cols, vals, b):
+    """The contact part of RodComplexBC.compute_A (dynamic_contacts.py:281-288):
+    contact_hessian_ee at triplet offset 0 and contact_hessian_pt at offset
+    n_ee, in one launch; rows/cols/vals hold 16 * (n_ee + n_pt) blocks."""
+    _check_triplets(rows, cols, vals, n_ee, n_pt, x, b)
+    if xcs.shape != x.shape:
+        raise ValueError("xcs must have the shape of x")
+    check(lib().ipc_collision_hessian(_need(x, torch.float32, "x", (3,)), _need(xcs, torch.float32, "xcs", (3,)),
+                                      x.shape[0], _rows(ee, n_ee, "ee"), int(n_ee), _rows(pt, n_pt, "pt"), int(n_pt),
+                                      float(l0), float(kappa), _need(rows, torch.int32, "rows"),
+                                      _need(cols, torch.int32, "cols"), _need(vals, torch.float32, "vals", (3, 3)),
+                                      _need(b, torch.float32, "b", (3,)), stream_handle(x.device)), "collision_hessian")
+
+
+def collision_energy(x, xcs, ee, n_ee, pt, n_pt, l0, kappa, e):
+    """The kernels of compute_collision_energy (dynamic_contacts.py:300-305):
+    e[0] (float64) is set to the EE + PT barrier energy, without a host
+    transfer."""
+    if xcs.shape != x.shape:
+        raise ValueError("xcs must have the shape of x")
+    check(lib().ipc_collision_energy(_need(x, torch.float32, "x", (3,)), _need(xcs, torch.float32, "xcs", (3,)),
+                                     x.shape[0], _rows(ee, n_ee, "ee"), int(n_ee), _rows(pt, n_pt, "pt"), int(n_pt),
+                                     float(l0), float(kappa), _need(e, torch.float64, "e"), stream_handle(x.device)),
+          "collision_energy")
+
+
 def contact_points_ee(x, contacts, n, points=None, dists=None, features=None, xcs=None):
     """get_contact_points (contact.py:148-153) over fetch_dist_v0v1 (contact.py:88-110)."""
     for t, nm in ((points, "points"), (dists, "dists"), (features, "features")):

@@ -94,6 +94,56 @@ def test_hessian_scene_with_offset_and_shared_vertices(dev, oracle):
              total)
 
 
+@pytest.mark.parametrize("lists", ["both", "ee", "pt"])
+def test_collision_hessian_both_lists_in_one_launch(dev, oracle, lists):
+    """compute_A's contact part (dynamic_contacts.py:281-288) in one call:
+    EE triplets from 0, PT triplets from n_ee; either list may be empty."""
+    soup = scenes.two_cubes_on_ground(seed=6, n=8, gap=4e-4, jitter=2e-4)
+    th = 5e-4
+    ee, _ = oracle.dcd_ee(soup.x, soup.edges, soup.body, th, th)
+    pt, _ = oracle.dcd_pt(soup.x, soup.triangles, soup.body, th, th)
+    if lists == "ee":
+        pt = pt[:0]
+    elif lists == "pt":
+        ee = ee[:0]
+    n_ee, n_pt = len(ee), len(pt)
+    total = n_ee + n_pt
+    xt = torch.from_numpy(soup.x).to(dev)
+    rows = torch.full((16 * total,), -7, dtype=torch.int32, device=dev)
+    cols = torch.full_like(rows, -7)
+    vals = torch.full((16 * total, 3, 3), float("nan"), dtype=torch.float32, device=dev)
+    b = torch.zeros_like(xt)
+    ee_t = torch.from_numpy(ee.reshape(-1, 4)).to(dev)
+    pt_t = torch.from_numpy(pt.reshape(-1, 4)).to(dev)
+    ops.collision_hessian(xt, xt, ee_t, n_ee, pt_t, n_pt, L0, KAPPA, rows, cols, vals, b)
+    torch.cuda.synchronize()
+    r_ee = oracle.contact_hessian("ee", soup.x, ee, L0, KAPPA, n_total=total, threads=8)
+    r_pt = oracle.contact_hessian("pt", soup.x, pt, L0, KAPPA, offset=n_ee, n_total=total, threads=8)
+    ref = (r_ee[0] + r_pt[0], r_ee[1] + r_pt[1], r_ee[2] + r_pt[2], r_ee[3] + r_pt[3], None)
+    _compare(rows.cpu().numpy(), cols.cpu().numpy(), vals.cpu().numpy().astype(np.float64), b.cpu().numpy(), ref, 0,
+             total)
+
+
+def test_collision_energy_both_lists_in_one_launch(dev, oracle):
+    """compute_collision_energy's kernels (dynamic_contacts.py:300-305): e[0]
+    is set (not added to) to the EE + PT energy."""
+    x_ee, c_ee = scenes.pair_stream("ee", 6000, seed=24)
+    x_pt, c_pt = scenes.pair_stream("pt", 5000, seed=25)
+    x = np.concatenate([x_ee, x_pt]).astype(np.float32)
+    c_pt = c_pt + len(x_ee)
+    xt = torch.from_numpy(x).to(dev)
+    ee_t, pt_t = torch.from_numpy(c_ee).to(dev), torch.from_numpy(c_pt).to(dev)
+    ref = oracle.contact_energy("ee", x, c_ee, L0, KAPPA, threads=8) + oracle.contact_energy("pt", x, c_pt, L0, KAPPA,
+                                                                                            threads=8)
+    e = torch.full((1,), 123.0, dtype=torch.float64, device=dev)
+    ops.collision_energy(xt, xt, ee_t, len(c_ee), pt_t, len(c_pt), L0, KAPPA, e)
+    assert e.item() == pytest.approx(ref, rel=1e-9)
+    ops.collision_energy(xt, xt, ee_t, len(c_ee), pt_t, len(c_pt), L0, KAPPA, e)
+    assert e.item() == pytest.approx(ref, rel=1e-9)
+    ops.collision_energy(xt, xt, ee_t, 0, pt_t, 0, L0, KAPPA, e)
+    assert e.item() == 0.0
+
+
 @pytest.mark.parametrize("kind", ["pt", "ee"])
 def test_energy_matches_oracle(dev, oracle, kind):
     x, c = scenes.pair_stream(kind, 20000, seed=22)
