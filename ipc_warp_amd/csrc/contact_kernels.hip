@@ -140,76 +140,107 @@ struct HessianArgs {
   int lpw;
 };
 
+// A wave's forces on their way to b: 12 per contact slot, and the slot's 4
+// vertex ids.
+struct ForceStage {
+  float f[4][64 * 12];
+  int v[4][64 * 4];
+};
+
 template <bool EE>
-__device__ __forceinline__ void hessian_block(const HessianArgs& A, int block) {
+__device__ __forceinline__ void hessian_block(const HessianArgs& A, int block, ForceStage& fs) {
   const float* __restrict__ x = A.x;
   const float* __restrict__ xcs = A.xcs;
-  const int nv = A.nv, n = A.n, offset = A.offset;
+  const int nv = A.nv, n = A.n, offset = A.offset, lpw = A.lpw;
   const float l0 = A.l0;
   const double kappa = A.kappa;
   int* __restrict__ rows = A.rows;
   int* __restrict__ cols = A.cols;
   float* __restrict__ vals = A.vals;
   float* __restrict__ b = A.b;
-  int i = contact_slot(A.lpw, block);
-  if (i < 0 || i >= n) return;
-  const int4 c = A.contacts[i];
-  CompressedHessian C;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i = contact_slot(lpw, block);
   float force[12];
-  bool active = valid4(c, nv) && (EE ? pair_ee(x, xcs, c, l0, kappa, true, C, force, nullptr)
-                                     : pair_pt(x, c, l0, kappa, true, C, force, nullptr));
-  long long base = (long long)(offset + i) * 16;
-  int4* rows4 = reinterpret_cast<int4*>(rows + base);
-  int4* cols4 = reinterpret_cast<int4*>(cols + base);
-  const int cc[4] = {c.x, c.y, c.z, c.w};
-  if (!active) {
-    int4 z = make_int4(0, 0, 0, 0);
 #pragma unroll
-    for (int ii = 0; ii < 4; ++ii) { rows4[ii] = z; cols4[ii] = z; }
+  for (int k = 0; k < 12; ++k) force[k] = 0.0f;
+  int4 c = make_int4(0, 0, 0, 0);
+  if (i >= 0 && i < n) {
+    c = A.contacts[i];
+    CompressedHessian C;
+    const bool active = valid4(c, nv) && (EE ? pair_ee(x, xcs, c, l0, kappa, true, C, force, nullptr)
+                                             : pair_pt(x, c, l0, kappa, true, C, force, nullptr));
+    const long long base = (long long)(offset + i) * 16;
+    int4* rows4 = reinterpret_cast<int4*>(rows + base);
+    int4* cols4 = reinterpret_cast<int4*>(cols + base);
     float4* v4 = reinterpret_cast<float4*>(vals + base * 9);
-    float4 zf = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!active) {
+      const int4 z = make_int4(0, 0, 0, 0);
 #pragma unroll
-    for (int k = 0; k < 36; ++k) v4[k] = zf;
-    return;
+      for (int ii = 0; ii < 4; ++ii) { rows4[ii] = z; cols4[ii] = z; }
+      const float4 zf = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int k = 0; k < 36; ++k) v4[k] = zf;
+#pragma unroll
+      for (int k = 0; k < 12; ++k) force[k] = 0.0f;
+    } else {
+      const int cc[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+      for (int ii = 0; ii < 4; ++ii) {
+        rows4[ii] = make_int4(cc[ii], cc[ii], cc[ii], cc[ii]);
+        cols4[ii] = c;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        // Four consecutive 3x3 blocks = 36 floats = 9 float4 stores.
+        float blk[36];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) expand_block(C, (q * 4 + r) >> 2, (q * 4 + r) & 3, blk + 9 * r);
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+          v4[q * 9 + k] = make_float4(blk[4 * k], blk[4 * k + 1], blk[4 * k + 2], blk[4 * k + 3]);
+      }
+    }
   }
+  // Forces into b. Device-scope float atomics are the kernel's largest cost,
+  // and one atomic instruction costs a memory transaction per cache line its
+  // lanes touch. The wave's 12 lpw force components (contact slot j / 12,
+  // component j % 12) go through LDS and are added transposed, value j by
+  // lane j % 64: an instruction then adds consecutive components of
+  // consecutive contacts, a few lines when their vertices are adjacent
+  // instead of one line per lane. A zero force (a vertex with zero weight in
+  // the contact's feature, or no active contact) is not added.
+  float* sf = fs.f[wave];
+  int* sv = fs.v[wave];
+  if (lane < lpw) {
 #pragma unroll
-  for (int ii = 0; ii < 4; ++ii) {
-    rows4[ii] = make_int4(cc[ii], cc[ii], cc[ii], cc[ii]);
-    cols4[ii] = c;
+    for (int k = 0; k < 12; ++k) sf[lane * 12 + k] = force[k];
+    sv[lane * 4 + 0] = c.x;
+    sv[lane * 4 + 1] = c.y;
+    sv[lane * 4 + 2] = c.z;
+    sv[lane * 4 + 3] = c.w;
   }
-  float4* v4 = reinterpret_cast<float4*>(vals + base * 9);
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    // Four consecutive 3x3 blocks = 36 floats = 9 float4 stores.
-    float blk[36];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) expand_block(C, (q * 4 + r) >> 2, (q * 4 + r) & 3, blk + 9 * r);
-#pragma unroll
-    for (int k = 0; k < 9; ++k) v4[q * 9 + k] = make_float4(blk[4 * k], blk[4 * k + 1], blk[4 * k + 2], blk[4 * k + 3]);
-  }
-  // Device-scope float atomics are the kernel's largest cost (each is its own
-  // fabric transaction); a slot with zero weight in the feature (the far
-  // vertices of edge and vertex features) has an exactly zero force, whose
-  // add is skipped.
-#pragma unroll
-  for (int ii = 0; ii < 4; ++ii) {
-    float* dst = b + 3 * (long long)cc[ii];
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-      if (force[3 * ii + a] != 0.0f) unsafeAtomicAdd(dst + a, force[3 * ii + a]);
+  __builtin_amdgcn_wave_barrier();
+  for (int j = lane; j < 12 * lpw; j += 64) {
+    const float v = sf[j];
+    if (v != 0.0f) {
+      const int slot = j / 12, comp = j - 12 * slot, vert = comp / 3;
+      unsafeAtomicAdd(b + 3 * (long long)sv[slot * 4 + vert] + (comp - 3 * vert), v);
+    }
   }
 }
 
 template <bool EE>
 __global__ void __launch_bounds__(256) contact_hessian_kernel(HessianArgs A) {
-  hessian_block<EE>(A, blockIdx.x);
+  __shared__ ForceStage fs;
+  hessian_block<EE>(A, blockIdx.x, fs);
 }
 
 // Both halves of the contact Hessian in one launch: workgroups [0, blocks_e)
 // take the EE list, the rest the PT list.
 __global__ void __launch_bounds__(256) contact_hessian_pair_kernel(HessianArgs E, HessianArgs P, int blocks_e) {
-  if (int(blockIdx.x) < blocks_e) hessian_block<true>(E, blockIdx.x);
-  else hessian_block<false>(P, blockIdx.x - blocks_e);
+  __shared__ ForceStage fs;
+  if (int(blockIdx.x) < blocks_e) hessian_block<true>(E, blockIdx.x, fs);
+  else hessian_block<false>(P, blockIdx.x - blocks_e, fs);
 }
 
 // ------------------------------------------------------------ energy kernel ----
