@@ -110,6 +110,21 @@ def measured_traffic(kernel: str, stream: bool = False):
     return pick[0]["fetch_bytes"] + pick[0]["write_bytes"]
 
 
+FP64_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "toi_stream_fp64.json")
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 vector
+
+
+def measured_flops64(kind: str):
+    """FP64 flops per launch of the TOI kernel on the 2^20-candidate CCD
+    stream from the committed rocprofv3 PMC summary (VALU FP64 instruction
+    counts x 64 lanes, FMA counted twice), or None when it has no entry."""
+    try:
+        with open(FP64_PROFILE) as f:
+            return float(json.load(f)["kernels"][kind]["flops_per_launch"])
+    except (OSError, ValueError, KeyError, TypeError):
+        return None
+
+
 WARMSTART_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ipc_warp_amd", "data",
                                "warmstart_cube_drop.npz")
 
@@ -351,7 +366,42 @@ def main() -> int:
     torch.cuda.synchronize()
     big_ms = e0.elapsed_time(e1) / 10
     big_gbs = 816 * n_big / (big_ms * 1e-3) / 1e9
-    del rows, cols, vals
+    del rows, cols, vals, xs_t, cs_t, bb
+
+    # The TOI kernel on synthetic CCD candidate streams (2^20 EE and 2^20 PT
+    # candidates, coordinates ~10, ~30 % impacts), the minimum folded on the
+    # device: HBM fraction from the 112 B per candidate, FP64-VALU fraction
+    # from the committed PMC counts.
+    n_ccd = 1 << 20
+    toi_stream = {}
+    for kind in ("ee", "pt"):
+        arrays = scenes.ccd_stream(kind, n_ccd, seed=args.seed)[:3]
+        x0t, x1t, qt = (torch.from_numpy(a).to(dev) for a in arrays)
+        tmin = ops.new_toi_min(dev)
+        for _ in range(3):
+            ops.collision_time(kind, x0t, x1t, qt, n_ccd, None, tmin)
+        e0.record()
+        for _ in range(10):
+            ops.collision_time(kind, x0t, x1t, qt, n_ccd, None, tmin)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 10
+        gbs = 112 * n_ccd / (ms * 1e-3) / 1e9
+        flops = measured_flops64(kind)
+        tflops = None if flops is None else flops / (ms * 1e-3) / 1e12
+        toi_stream[kind] = {
+            "kernel": f"collision_time_{kind}",
+            "candidates": n_ccd,
+            "ms": ms,
+            "candidates_per_s": n_ccd / (ms * 1e-3),
+            "hbm": {"achieved": gbs, "unit": "GB/s", "peak": HBM_PEAK_GBS, "frac": gbs / HBM_PEAK_GBS,
+                    "algorithmic_bytes_per_launch": 112 * n_ccd},
+            "fp64": None if flops is None else {"achieved": tflops, "unit": "TFLOP/s", "peak": FP64_PEAK_TFLOPS,
+                                                "frac": tflops / FP64_PEAK_TFLOPS, "flops_per_launch": flops,
+                                                "source": os.path.relpath(FP64_PROFILE, os.path.dirname(
+                                                    os.path.abspath(__file__)))},
+        }
+        del x0t, x1t, qt
 
     # The same step through the fixed per-body BVHs (the reference's
     # FixedBVHWreckingBalls option): identity frames, each body translated by
@@ -422,6 +472,7 @@ def main() -> int:
             "traffic": measured_traffic("hessian_pt", stream=True),
             "algorithmic_bytes_per_launch": 816 * n_big,
         },
+        "toi_stream": toi_stream,
         "hbm_copy_gbs": copy_gbs,
         "fixed_body_bvh_step": fixed,
     }

@@ -358,3 +358,60 @@ def pair_stream(kind: str, n: int, seed: int = 0, dhat: float = 1.0e-3, scale: f
         x[:, 0], x[:, 1], x[:, 2], x[:, 3] = a0, a1, b0, b1
     contacts = np.arange(4 * n, dtype=np.int32).reshape(n, 4)
     return x.reshape(-1, 3).astype(np.float32), contacts
+
+
+def ccd_stream(kind: str, n: int, seed: int = 0, hit_fraction: float = 0.3, coord: float = 10.0,
+               scale: float = 0.05):
+    """Synthetic CCD candidates with 4 private vertices each.
+
+    Primitives of size ~`scale` centred uniformly in [-coord, coord]^3. A
+    fraction `hit_fraction` of the candidates move into contact at a time
+    t ~ U(0.05, 0.95) of the step (PT: the point crosses the triangle's
+    plane inside the moving triangle; EE: edge b sweeps through an interior
+    point of edge a); the rest move apart (the point leaves along the face
+    normal, edge b leaves along the common normal) and report no impact.
+    Returns (x0, x1 float32 [4n,3], quads int32 [n,4], t float64 [n] with
+    t = 1 for the misses); quads[i] = 4i .. 4i+3 in the TOI argument order
+    (PT: p, t0, t1, t2; EE: a0, a1, b0, b1).
+    """
+    rng = np.random.default_rng(seed)
+    hit = rng.random(n) < hit_fraction
+    t = np.where(hit, rng.uniform(0.05, 0.95, n), 1.0)
+    centre = rng.uniform(-coord, coord, (n, 3))
+    if kind == "pt":
+        tri0 = centre[:, None, :] + scale * rng.standard_normal((n, 3, 3))
+        vel = 0.5 * scale * rng.standard_normal((n, 3))
+        w = rng.dirichlet([3, 3, 3], n)
+        nrm = np.cross(tri0[:, 1] - tri0[:, 0], tri0[:, 2] - tri0[:, 0])
+        nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+        speed = scale * rng.uniform(0.5, 2.0, (n, 1))
+        th = np.where(hit, t, 0.5)[:, None]
+        meet = np.einsum("ij,ijk->ik", w, tri0) + th * vel
+        # hits approach the plane against the normal; misses start on the
+        # other side and leave, so the point never reaches the plane
+        vp = np.where(hit[:, None], -nrm * speed, nrm * speed) + vel
+        p0 = np.where(hit[:, None], meet - th * vp, meet + nrm * speed)
+        X0 = np.concatenate([p0[:, None], tri0], axis=1)
+        X1 = np.concatenate([(p0 + vp)[:, None], tri0 + vel[:, None, :]], axis=1)
+    else:
+        a0 = centre + scale * rng.standard_normal((n, 3))
+        a1 = centre + scale * rng.standard_normal((n, 3))
+        b_dir = np.cross(a1 - a0, rng.standard_normal((n, 3)))
+        b_dir *= scale / np.linalg.norm(b_dir, axis=1, keepdims=True)
+        s, u = rng.uniform(0.2, 0.8, (2, n, 1))
+        meet = a0 + s * (a1 - a0)
+        common = np.cross(a1 - a0, b_dir)
+        common /= np.linalg.norm(common, axis=1, keepdims=True)
+        vb = scale * rng.standard_normal((n, 3))
+        th = t[:, None]
+        # hits pass through `meet` at time t; misses start one edge length off
+        # along the common normal and move further away
+        away = common * scale * (1.0 + rng.random((n, 1)))
+        vb = np.where(hit[:, None], vb, common * np.abs(vb).sum(axis=1, keepdims=True))
+        b0 = np.where(hit[:, None], meet - u * b_dir - th * vb, meet - u * b_dir + away)
+        b1 = b0 + b_dir
+        X0 = np.stack([a0, a1, b0, b1], axis=1)
+        X1 = np.stack([a0, a1, b0 + vb, b1 + vb], axis=1)
+    x0 = X0.reshape(-1, 3).astype(np.float32)
+    x1 = X1.reshape(-1, 3).astype(np.float32)
+    return x0, x1, np.arange(4 * n, dtype=np.int32).reshape(n, 4), t

@@ -71,6 +71,21 @@ def test_toi_straight_line_impacts(dev, oracle, kind):
     assert tmin == toi.min()
 
 
+@pytest.mark.parametrize("kind", ["pt", "ee"])
+def test_toi_on_the_ccd_stream(dev, oracle, kind):
+    """The bench's CCD stream (coordinates ~10, ~30 % impacts): per-candidate
+    TOI against the oracle's relative form, every constructed impact found."""
+    x0, x1, q, t = scenes.ccd_stream(kind, 1 << 14, seed=34)
+    toi, tmin = _gpu_toi(dev, kind, x0, x1, q)
+    rel, _ = oracle.collision_time(kind, x0, x1, q, relative=True, threads=8)
+    assert np.abs(toi - rel).max() <= 1e-12
+    hits = t < 1
+    assert 0.25 < hits.mean() < 0.35
+    assert (toi[hits] < 1).all()
+    assert np.median(np.abs(toi[hits] - t[hits])) <= 1e-5
+    assert tmin == toi.min()
+
+
 def test_toi_on_scene_candidates(dev, oracle):
     soup = scenes.two_cubes_on_ground(seed=7, n=8, gap=2e-3, jitter=1e-4)
     rng = np.random.default_rng(32)
