@@ -574,18 +574,72 @@ __device__ __forceinline__ void query_block(const QueryArgs& A, int bid, QuerySm
     ++steps;
     bool active = sp > 0;
     int L = 0, cbase = 0;
-    unsigned take = 0u;
+    unsigned take = 0u, rest = 0u;
     if (active) {
       unsigned long long e = stack[grp][sp - 1];
       L = int(e >> 40) & 0xf;
       unsigned mask = unsigned(e >> 32) & 0xffu;
       cbase = int(e & 0xffffffffu);
-      unsigned rest = mask;
+      rest = mask;
 #pragma unroll
       for (int p = 0; p < kPopNodes; ++p) rest &= rest - 1u;
       take = mask ^ rest;  // the kPopNodes lowest nodes
-      if (rest == 0u) --sp;
-      else stack[grp][sp - 1] = mask_entry(L, rest, cbase);
+      --sp;                // the rest goes back below, or to an idle group
+    }
+    // Work sharing: a group whose popped entry has nodes left hands them, with
+    // its query, to a group of the same wave that has run out of work (the
+    // k-th idle group takes the k-th such entry) and starts on them in the
+    // next step. A wave otherwise runs as long as its slowest query while
+    // the groups that finished wait. Every (query, node) is still visited
+    // exactly once.
+    bool gave = false;
+    {
+      constexpr unsigned long long kLead = 0x0101010101010101ull;  // the first lane of each group
+      const unsigned long long idle_b = __ballot(!active) & kLead;
+      const unsigned long long give_b = __ballot(rest != 0u) & kLead;
+      if (idle_b != 0ull && give_b != 0ull) {
+        const unsigned long long below = (1ull << group_shift) - 1ull;
+        const int pairs = min(__popcll(idle_b), __popcll(give_b));
+        const int idle_rank = __popcll(idle_b & below), give_rank = __popcll(give_b & below);
+        const bool adopt = !active && idle_rank < pairs;
+        gave = rest != 0u && give_rank < pairs;
+        unsigned long long m = give_b;
+        for (int j = 0; j < idle_rank && j < kFan; ++j) m &= m - 1ull;
+        const int src = (adopt ? __ffsll(static_cast<long long>(m)) - 1 : group_shift) + r;
+        const unsigned long long ent = mask_entry(L, rest, cbase);
+        const unsigned ent_lo = __shfl(unsigned(ent & 0xffffffffu), src, 64);
+        const unsigned ent_hi = __shfl(unsigned(ent >> 32), src, 64);
+        float s_lo[3], s_hi[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          s_lo[a] = __shfl(qlo[a], src, 64);
+          s_hi[a] = __shfl(qhi[a], src, 64);
+        }
+        const int s_body = __shfl(qbody, src, 64), s_slot = __shfl(slot, src, 64);
+        const int s_i = __shfl(Q.i, src, 64), s_q0 = __shfl(Q.q0, src, 64), s_q1 = __shfl(Q.q1, src, 64);
+        const V3<float> s_p0 = mk(__shfl(Q.p0.x, src, 64), __shfl(Q.p0.y, src, 64), __shfl(Q.p0.z, src, 64));
+        const V3<float> s_p1 = mk(__shfl(Q.p1.x, src, 64), __shfl(Q.p1.y, src, 64), __shfl(Q.p1.z, src, 64));
+        if (adopt) {
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            qlo[a] = s_lo[a];
+            qhi[a] = s_hi[a];
+          }
+          qbody = s_body;
+          slot = s_slot;
+          Q.i = s_i;
+          Q.q0 = s_q0;
+          Q.q1 = s_q1;
+          Q.p0 = s_p0;
+          Q.p1 = s_p1;
+          stack[grp][0] = (static_cast<unsigned long long>(ent_hi) << 32) | ent_lo;
+          sp = 1;
+        }
+      }
+    }
+    if (rest != 0u && !gave) {
+      stack[grp][sp] = mask_entry(L, rest, cbase);
+      ++sp;
     }
     int cpos[kPopNodes], at[kPopNodes];
     {
