@@ -64,24 +64,20 @@ struct Box { float lo[3], hi[3]; };
 // mode 2: swept edge box over x0, x1, +- pad (swept_edge_aabbs, ccd/ccd.py:166-186)
 // mode 3: swept triangle box over x0, x1, +- pad (swept_triangle_aabbs, ccd/ccd.py:188-210)
 // mode 4: the point x0[p] (query ordering only)
-__device__ __forceinline__ void prim_box(int mode, int p, const int* __restrict__ prims, const float* __restrict__ x0,
-                                         const float* __restrict__ x1, float pad, float4& lo4, float4& hi4) {
-  if (mode == BOX_POINT) {
-    V3<float> q = ldx(x0, p);
-    lo4 = hi4 = make_float4(q.x, q.y, q.z, 0.f);
-    return;
-  }
-  int k = (mode == 0 || mode == 2) ? 2 : 3;
+// The box of a primitive with vertex ids v[0 .. k) (k = 2 for an edge, 3 for
+// a triangle) in box mode 0..3.
+__device__ __forceinline__ void vertex_box(int mode, int k, const int v[3], const float* __restrict__ x0,
+                                           const float* __restrict__ x1, float pad, float4& lo4, float4& hi4) {
   V3<float> lo, hi;
-  lo = hi = ldx(x0, prims[k * p]);
+  lo = hi = ldx(x0, v[0]);
   for (int j = 1; j < k; ++j) {
-    V3<float> q = ldx(x0, prims[k * p + j]);
+    V3<float> q = ldx(x0, v[j]);
     lo = vmin(lo, q);
     hi = vmax(hi, q);
   }
   if (mode >= 2) {
     for (int j = 0; j < k; ++j) {
-      V3<float> q = ldx(x1, prims[k * p + j]);
+      V3<float> q = ldx(x1, v[j]);
       lo = vmin(lo, q);
       hi = vmax(hi, q);
     }
@@ -91,6 +87,18 @@ __device__ __forceinline__ void prim_box(int mode, int p, const int* __restrict_
   hi = hi + pv;
   lo4 = make_float4(lo.x, lo.y, lo.z, 0.f);
   hi4 = make_float4(hi.x, hi.y, hi.z, 0.f);
+}
+
+__device__ __forceinline__ void prim_box(int mode, int p, const int* __restrict__ prims, const float* __restrict__ x0,
+                                         const float* __restrict__ x1, float pad, float4& lo4, float4& hi4) {
+  if (mode == BOX_POINT) {
+    V3<float> q = ldx(x0, p);
+    lo4 = hi4 = make_float4(q.x, q.y, q.z, 0.f);
+    return;
+  }
+  const int k = (mode == 0 || mode == 2) ? 2 : 3;
+  const int v[3] = {prims[k * p], prims[k * p + 1], k == 3 ? prims[k * p + 2] : 0};
+  vertex_box(mode, k, v, x0, x1, pad, lo4, hi4);
 }
 
 // Scene bounds of the primitive box centres (ordered-int atomics), for the
@@ -194,6 +202,7 @@ struct LeafJob {
   float4* node_lo;
   float4* node_hi;
   LevelInfo info;
+  int reuse_vid;  // leaf_vid and the leaf bodies hold this leaf order's records
 };
 
 struct LeafSmem {
@@ -218,20 +227,28 @@ __device__ __forceinline__ void leaf_block(const LeafJob& J, int bid, const int*
   box_empty(lo, hi);
   int4 vid = make_int4(-1, -1, -1, -1);
   if (k < J.n) {
-    int prim = static_cast<int>(J.keys[k] & 0xffffffffu);
-    prim_box(J.mode, prim, J.prims, x0, J.x1, J.pad, lo, hi);
-    int v0 = J.prims[stride * prim], v1 = J.prims[stride * prim + 1];
-    int v2 = stride == 3 ? J.prims[stride * prim + 2] : -1;
-    int bd = body[v0];
-    J.leaf_id[k] = prim;
-    vid = make_int4(v0, v1, v2, prim);
+    int bd;
+    if (J.reuse_vid) {
+      // the leaf order is the previous build's: its record and body stand
+      vid = J.leaf_vid[k];
+      bd = __float_as_int(J.node_lo[k].w);
+    } else {
+      const int prim = static_cast<int>(J.keys[k] & 0xffffffffu);
+      const int v0 = J.prims[stride * prim], v1 = J.prims[stride * prim + 1];
+      const int v2 = stride == 3 ? J.prims[stride * prim + 2] : -1;
+      bd = body[v0];
+      J.leaf_id[k] = prim;
+      vid = make_int4(v0, v1, v2, prim);
+    }
+    const int v[3] = {vid.x, vid.y, stride == 3 ? vid.z : 0};
+    vertex_box(J.mode, stride, v, x0, J.x1, J.pad, lo, hi);
     lo.w = __int_as_float(bd);
     hi.w = __int_as_float(bd);
   }
   if (k < info.slots[0]) {  // offset[0] = 0
     J.node_lo[k] = lo;
     J.node_hi[k] = hi;
-    J.leaf_vid[k] = vid;
+    if (!J.reuse_vid) J.leaf_vid[k] = vid;
   }
   // level 1: groups of 8 lanes
   lane_merge(lo, hi, 1);
@@ -794,6 +811,9 @@ void Bvh::release() {
   temp.release(); bounds.release();
   n = 0;
   builds_since_sort = -1;
+  vid_prims = nullptr;
+  vid_body = nullptr;
+  reuse_vid = false;
 }
 
 #define IPC_TRY(expr)                    \
@@ -829,7 +849,6 @@ long long plan_levels(LevelInfo& I, int n) {
 hipError_t prepare_job(const BuildJob& j, const int* body, const float* x0, hipStream_t s) {
   Bvh& B = *j.bvh;
   const int n = j.n;
-  (void)body;
   long long total = plan_levels(B.info, n);
   IPC_TRY(B.bounds.ensure(sizeof(int) * 8));
   IPC_TRY(B.keys.ensure(sizeof(unsigned long long) * n));
@@ -839,6 +858,9 @@ hipError_t prepare_job(const BuildJob& j, const int* body, const float* x0, hipS
   IPC_TRY(B.node_lo.ensure(sizeof(float4) * total));
   IPC_TRY(B.node_hi.ensure(sizeof(float4) * total));
   bool resort = B.builds_since_sort < 0 || B.builds_since_sort >= kResortEvery || B.n_sorted != n;
+  B.reuse_vid = !resort && B.vid_prims == j.prims && B.vid_body == body;
+  B.vid_prims = j.prims;
+  B.vid_body = body;
   if (resort) {
     hipLaunchKernelGGL(init_bounds_kernel, dim3(1), dim3(64), 0, s, B.bounds.as<int>());
     hipLaunchKernelGGL(centre_bounds_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, j.mode, n, j.prims, x0, j.x1,
@@ -875,6 +897,7 @@ LeafJob leaf_job(const BuildJob& j) {
   L.node_lo = B.node_lo.as<float4>();
   L.node_hi = B.node_hi.as<float4>();
   L.info = B.info;
+  L.reuse_vid = B.reuse_vid ? 1 : 0;
   return L;
 }
 

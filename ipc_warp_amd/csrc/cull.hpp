@@ -44,6 +44,12 @@ struct Bvh {
   DevBuf bounds;              // 6 ordered ints
   int builds_since_sort = -1; // -1: no leaf order yet
   int n_sorted = 0;
+  // leaf_vid and the leaves' body ids were written for this leaf order from
+  // these topology and body arrays (a build that keeps the order reads them
+  // instead of following keys -> topology -> body)
+  const int* vid_prims = nullptr;
+  const int* vid_body = nullptr;
+  bool reuse_vid = false;     // this build reads them
   void release();
 };
 
