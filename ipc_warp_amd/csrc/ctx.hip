@@ -419,10 +419,10 @@ int ipc_collision_free_step(ipc_ctx* c, const float* x, const float* dx, float p
     if (st) return st;
     const float* x1 = c->x1.as<float>();
     int* cnt = counters(c);
-    st = ctx_hip(launch_sub(3 * c->nv, x, dx, c->x1.as<float>(), s), "collision_free_step");
-    if (st) return st;
-    BuildJob jobs[2] = {{&c->edge_bvh, BOX_SWEPT_EDGE, c->ne, c->edges.as<int>(), x1, padding},
-                        {&c->tri_bvh, BOX_SWEPT_TRI, n_tri, c->tris.as<int>(), x1, padding}};
+    // x1 = x - dx: the build reads the end positions as x - dx and its leaf
+    // launch writes x1 for the queries and the TOI launch that follow
+    BuildJob jobs[2] = {{&c->edge_bvh, BOX_SWEPT_EDGE, c->ne, c->edges.as<int>(), x1, padding, dx, c->nv},
+                        {&c->tri_bvh, BOX_SWEPT_TRI, n_tri, c->tris.as<int>(), x1, padding, dx, c->nv}};
     BuildPrologue pro = zero_prologue(cnt + 0, cnt + 1);
     pro.toi = toi_bits(c);
     pro.toi_init = kOneBits;

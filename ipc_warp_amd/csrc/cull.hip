@@ -64,10 +64,22 @@ struct Box { float lo[3], hi[3]; };
 // mode 2: swept edge box over x0, x1, +- pad (swept_edge_aabbs, ccd/ccd.py:166-186)
 // mode 3: swept triangle box over x0, x1, +- pad (swept_triangle_aabbs, ccd/ccd.py:188-210)
 // mode 4: the point x0[p] (query ordering only)
+// Vertex i at the end of a CCD step: x1[i], or, with dx, x0[i] - dx[i] (the
+// fp32 subtraction of _add_dx, contact.py:49-52, so the same value).
+__device__ __forceinline__ V3<float> end_pos(const float* __restrict__ x0, const float* __restrict__ x1,
+                                             const float* __restrict__ dx, int i) {
+  if (dx) {
+    const V3<float> a = ldx(x0, i), d = ldx(dx, i);
+    return mk(a.x - d.x, a.y - d.y, a.z - d.z);
+  }
+  return ldx(x1, i);
+}
+
 // The box of a primitive with vertex ids v[0 .. k) (k = 2 for an edge, 3 for
 // a triangle) in box mode 0..3.
 __device__ __forceinline__ void vertex_box(int mode, int k, const int v[3], const float* __restrict__ x0,
-                                           const float* __restrict__ x1, float pad, float4& lo4, float4& hi4) {
+                                           const float* __restrict__ x1, const float* __restrict__ dx, float pad,
+                                           float4& lo4, float4& hi4) {
   V3<float> lo, hi;
   lo = hi = ldx(x0, v[0]);
   for (int j = 1; j < k; ++j) {
@@ -77,7 +89,7 @@ __device__ __forceinline__ void vertex_box(int mode, int k, const int v[3], cons
   }
   if (mode >= 2) {
     for (int j = 0; j < k; ++j) {
-      V3<float> q = ldx(x1, v[j]);
+      V3<float> q = end_pos(x0, x1, dx, v[j]);
       lo = vmin(lo, q);
       hi = vmax(hi, q);
     }
@@ -90,7 +102,8 @@ __device__ __forceinline__ void vertex_box(int mode, int k, const int v[3], cons
 }
 
 __device__ __forceinline__ void prim_box(int mode, int p, const int* __restrict__ prims, const float* __restrict__ x0,
-                                         const float* __restrict__ x1, float pad, float4& lo4, float4& hi4) {
+                                         const float* __restrict__ x1, const float* __restrict__ dx, float pad,
+                                         float4& lo4, float4& hi4) {
   if (mode == BOX_POINT) {
     V3<float> q = ldx(x0, p);
     lo4 = hi4 = make_float4(q.x, q.y, q.z, 0.f);
@@ -98,20 +111,21 @@ __device__ __forceinline__ void prim_box(int mode, int p, const int* __restrict_
   }
   const int k = (mode == 0 || mode == 2) ? 2 : 3;
   const int v[3] = {prims[k * p], prims[k * p + 1], k == 3 ? prims[k * p + 2] : 0};
-  vertex_box(mode, k, v, x0, x1, pad, lo4, hi4);
+  vertex_box(mode, k, v, x0, x1, dx, pad, lo4, hi4);
 }
 
 // Scene bounds of the primitive box centres (ordered-int atomics), for the
 // Morton quantisation of a re-sort.
 __global__ void __launch_bounds__(256) centre_bounds_kernel(int mode, int n, const int* __restrict__ prims,
                                                             const float* __restrict__ x0, const float* __restrict__ x1,
-                                                            float pad, int* __restrict__ bounds) {
+                                                            const float* __restrict__ dx, float pad,
+                                                            int* __restrict__ bounds) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   float c[3] = {0.f, 0.f, 0.f};
   bool live = i < n;
   if (live) {
     float4 lo, hi;
-    prim_box(mode, i, prims, x0, x1, pad, lo, hi);
+    prim_box(mode, i, prims, x0, x1, dx, pad, lo, hi);
     c[0] = 0.5f * (lo.x + hi.x);
     c[1] = 0.5f * (lo.y + hi.y);
     c[2] = 0.5f * (lo.z + hi.z);
@@ -139,12 +153,13 @@ __global__ void __launch_bounds__(256) centre_bounds_kernel(int mode, int n, con
 
 __global__ void __launch_bounds__(256) morton_kernel(int mode, int n, const int* __restrict__ prims,
                                                      const float* __restrict__ x0, const float* __restrict__ x1,
-                                                     float pad, const int* __restrict__ bounds,
+                                                     const float* __restrict__ dx, float pad,
+                                                     const int* __restrict__ bounds,
                                                      unsigned long long* __restrict__ keys) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float4 lo, hi;
-  prim_box(mode, i, prims, x0, x1, pad, lo, hi);
+  prim_box(mode, i, prims, x0, x1, dx, pad, lo, hi);
   float c[3] = {0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z)};
   unsigned q[3];
   for (int a = 0; a < 3; ++a) {
@@ -195,6 +210,7 @@ struct LeafJob {
   int mode, n;
   const int* prims;
   const float* x1;
+  const float* dx;  // with dx, end positions are x0 - dx (x1 unused)
   float pad;
   const unsigned long long* keys;
   int* leaf_id;
@@ -241,7 +257,7 @@ __device__ __forceinline__ void leaf_block(const LeafJob& J, int bid, const int*
       vid = make_int4(v0, v1, v2, prim);
     }
     const int v[3] = {vid.x, vid.y, stride == 3 ? vid.z : 0};
-    vertex_box(J.mode, stride, v, x0, J.x1, J.pad, lo, hi);
+    vertex_box(J.mode, stride, v, x0, J.x1, J.dx, J.pad, lo, hi);
     lo.w = __int_as_float(bd);
     hi.w = __int_as_float(bd);
   }
@@ -284,11 +300,24 @@ __device__ __forceinline__ void leaf_block(const LeafJob& J, int bid, const int*
 
 // Workgroups [0, blocks0) build job 0, the rest job 1. The first workgroup
 // also runs the launch's prologue (counters of the queries that follow).
+// With end.out set, job 0's workgroups also write end.out[k] = x0[k] - dx[k]
+// for the 3 nv floats of the positions (the CCD end positions that the
+// queries and the TOI launch read next).
+struct EndWrite {
+  float* out;
+  const float* dx;
+  int n3;
+};
+
 __global__ void __launch_bounds__(kLeafBlock) leaf_levels_kernel(LeafJob J0, LeafJob J1, int blocks0,
                                                                  const int* __restrict__ body,
-                                                                 const float* __restrict__ x0, BuildPrologue pro) {
+                                                                 const float* __restrict__ x0, BuildPrologue pro,
+                                                                 EndWrite end) {
   __shared__ LeafSmem sm;
   run_prologue(pro);
+  if (end.out && int(blockIdx.x) < blocks0)
+    for (int f = blockIdx.x * kLeafBlock + threadIdx.x; f < end.n3; f += blocks0 * kLeafBlock)
+      end.out[f] = x0[f] - end.dx[f];
   if (int(blockIdx.x) < blocks0) leaf_block(J0, blockIdx.x, body, x0, sm);
   else leaf_block(J1, blockIdx.x - blocks0, body, x0, sm);
 }
@@ -863,9 +892,9 @@ hipError_t prepare_job(const BuildJob& j, const int* body, const float* x0, hipS
   B.vid_body = body;
   if (resort) {
     hipLaunchKernelGGL(init_bounds_kernel, dim3(1), dim3(64), 0, s, B.bounds.as<int>());
-    hipLaunchKernelGGL(centre_bounds_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, j.mode, n, j.prims, x0, j.x1,
+    hipLaunchKernelGGL(centre_bounds_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, j.mode, n, j.prims, x0, j.x1, j.dx,
                        j.pad, B.bounds.as<int>());
-    hipLaunchKernelGGL(morton_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, j.mode, n, j.prims, x0, j.x1, j.pad,
+    hipLaunchKernelGGL(morton_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, j.mode, n, j.prims, x0, j.x1, j.dx, j.pad,
                        B.bounds.as<int>(), B.keys.as<unsigned long long>());
     IPC_TRY(hipGetLastError());
     size_t temp_bytes = 0;
@@ -890,6 +919,7 @@ LeafJob leaf_job(const BuildJob& j) {
   L.n = j.n;
   L.prims = j.prims;
   L.x1 = j.x1;
+  L.dx = j.dx;
   L.pad = j.pad;
   L.keys = B.keys_sorted.as<unsigned long long>();
   L.leaf_id = B.leaf_id.as<int>();
@@ -920,16 +950,23 @@ hipError_t build_bvhs(const BuildJob* jobs, int n_jobs, const int* body, const f
     IPC_TRY(prepare_job(jobs[k], body, x0, s));
     live[n_live++] = &jobs[k];
   }
+  // end positions x0 - dx, written by the leaf launch (or on their own when
+  // no tree is built)
+  const BuildJob* ej = nullptr;
+  for (int k = 0; k < n_jobs; ++k)
+    if (jobs[k].dx && jobs[k].x1 && jobs[k].n_end > 0) ej = &jobs[k];
   if (n_live == 0) {
     if (pro.n_zero > 0 || pro.n_range > 0 || pro.toi) hipLaunchKernelGGL(prologue_kernel, dim3(1), dim3(64), 0, s, pro);
+    if (ej) IPC_TRY(launch_sub(3 * ej->n_end, x0, ej->dx, const_cast<float*>(ej->x1), s));
     return hipGetLastError();
   }
+  const EndWrite end = ej ? EndWrite{const_cast<float*>(ej->x1), ej->dx, 3 * ej->n_end} : EndWrite{nullptr, nullptr, 0};
   LeafJob j0 = leaf_job(*live[0]);
   LeafJob j1 = n_live > 1 ? leaf_job(*live[1]) : j0;
   int blocks0 = nblocks(live[0]->n, kLeafBlock);
   int blocks1 = n_live > 1 ? nblocks(live[1]->n, kLeafBlock) : 0;
   hipLaunchKernelGGL(leaf_levels_kernel, dim3(blocks0 + blocks1), dim3(kLeafBlock), 0, s, j0, j1, blocks0, body, x0,
-                     pro);
+                     pro, end);
   bool top0 = live[0]->bvh->info.top > kInBlockLevels;
   bool top1 = n_live > 1 && live[1]->bvh->info.top > kInBlockLevels;
   const bool ord = order && order->cost && order->perm && order->n_a > 0 && order->n_b > 0;
@@ -968,10 +1005,11 @@ hipError_t point_order(PointOrder& P, int n, const float* x, hipStream_t s, cons
   IPC_TRY(P.order.ensure(sizeof(int) * n));
   hipLaunchKernelGGL(init_bounds_kernel, dim3(1), dim3(64), 0, s, P.bounds.as<int>());
   hipLaunchKernelGGL(centre_bounds_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, int(BOX_POINT), n,
-                     static_cast<const int*>(nullptr), x, static_cast<const float*>(nullptr), 0.0f, P.bounds.as<int>());
+                     static_cast<const int*>(nullptr), x, static_cast<const float*>(nullptr),
+                     static_cast<const float*>(nullptr), 0.0f, P.bounds.as<int>());
   hipLaunchKernelGGL(morton_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, int(BOX_POINT), n,
-                     static_cast<const int*>(nullptr), x, static_cast<const float*>(nullptr), 0.0f,
-                     P.bounds.as<int>(), P.keys.as<unsigned long long>());
+                     static_cast<const int*>(nullptr), x, static_cast<const float*>(nullptr),
+                     static_cast<const float*>(nullptr), 0.0f, P.bounds.as<int>(), P.keys.as<unsigned long long>());
   IPC_TRY(hipGetLastError());
   size_t temp_bytes = 0;
   IPC_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, temp_bytes, P.keys.as<unsigned long long>(),

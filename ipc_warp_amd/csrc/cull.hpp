@@ -82,6 +82,11 @@ struct BuildJob {
   const int* prims;
   const float* x1;
   float pad;
+  // With dx, the swept modes take the end positions as x0 - dx, and job 0
+  // of the launch also writes them to x1 (3 n_end floats) for the launches
+  // that follow.
+  const float* dx = nullptr;
+  int n_end = 0;
 };
 
 // Device writes folded into the first workgroup of a call's first launch,
