@@ -149,3 +149,21 @@ def test_pt_collision_before_toi_is_free(oracle):
             _, typ = oracle.closest_point_triangle(xt[0], xt[1], xt[2], xt[3], double=True)
             d = oracle.closest_point_triangle(xt[0], xt[1], xt[2], xt[3], double=True)[0][0, 2]
             assert d > 0
+
+
+@pytest.mark.parametrize("kind", ["pt", "ee"])
+def test_ccd_stream_impacts_are_where_they_were_built(oracle, kind):
+    """The bench's synthetic CCD stream: ~30 % of the candidates are built to
+    meet at a known time, the rest move apart; the oracle finds each built
+    impact near its time (fp32 rounding of coordinates ~10 moves it by
+    ~1e-6) and no TOI for nearly all of the rest."""
+    from ipc_warp_amd import scenes
+    x0, x1, q, t = scenes.ccd_stream(kind, 4096, seed=41)
+    assert x0.dtype == np.float32 and q.shape == (4096, 4)
+    assert 8.0 < np.abs(x0).max() < 11.0
+    rel, capped = oracle.collision_time(kind, x0, x1, q, relative=True, threads=4)
+    hits = t < 1
+    assert 0.25 < hits.mean() < 0.35 and capped == 0
+    assert (rel[hits] < 1).all()
+    assert np.median(np.abs(rel[hits] - t[hits])) < 1e-5
+    assert (rel[~hits] >= 1).mean() > 0.99
