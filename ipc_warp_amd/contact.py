@@ -121,12 +121,33 @@ class ContactSolverBase:
     # ------------------------------------------------------------- helpers ----
     @property
     def l0(self) -> float:
-        import numpy as np
+        """2 * thickness in fp32 (the reference's l0), cached per thickness."""
+        cached = getattr(self, "_l0_cache", None)
+        if cached is None or cached[0] != self.thickness:
+            import numpy as np
 
-        return float(np.float32(self.thickness) * np.float32(2.0))
+            cached = (self.thickness, float(np.float32(self.thickness) * np.float32(2.0)))
+            self._l0_cache = cached
+        return cached[1]
 
     def _stream(self):
         return ops.stream_handle(self.device)
+
+    def _ptr(self, name: str, t: torch.Tensor, dtype: torch.dtype, like: torch.Tensor | None = None) -> int:
+        """t.data_ptr() after the ops argument checks (float [n, 3] or a 1-d
+        tensor of dtype; shaped like `like` when given), which run again only
+        when the tensor under this name is a different object or storage."""
+        cache = self.__dict__.setdefault("_checked", {})
+        hit = cache.get(name)
+        if hit is not None and hit[0] is t:
+            ptr = t.data_ptr()
+            if hit[1] == ptr:
+                return ptr
+        ptr = ops._need(t, dtype, name, (3,) if dtype == torch.float32 else None)
+        if like is not None and t.shape != like.shape:
+            raise ValueError(f"{name} must have the shape of x")
+        cache[name] = (t, ptr)
+        return ptr
 
     def _grow_contact_list(self, contacts: Contacts, required: int) -> Contacts:
         """contact.py:523-537: regrow to the next power of two >= required."""
@@ -298,8 +319,17 @@ class ContactSolverBase:
         triplets for (n_ee + n_pt) * 16 blocks, the EE kernel, then the PT
         kernel at offset n_ee; forces are added into b."""
         trip = self.new_triplets()
-        ops.collision_hessian(self.x, self.xcs, self.contacts_new.list, self.n_contacts, self.contacts_pt.list,
-                              self.n_contacts_pt, self.l0, self.contact_stiffness, trip.rows, trip.cols, trip.vals, b)
+        # The contact lists and triplets are the solver's own (sized by
+        # detect_collision and new_triplets); x, xcs and b are checked once
+        # per tensor (the library checks pointers and sizes on every call).
+        x = self.x
+        xp = self._ptr("x", x, torch.float32)
+        check(lib().ipc_collision_hessian(xp, self._ptr("xcs", self.xcs, torch.float32, x), x.shape[0],
+                                          self.contacts_new.list.data_ptr(), self.n_contacts,
+                                          self.contacts_pt.list.data_ptr(), self.n_contacts_pt, self.l0,
+                                          self.contact_stiffness, trip.rows.data_ptr(), trip.cols.data_ptr(),
+                                          trip.vals.data_ptr(), self._ptr("b", b, torch.float32, x),
+                                          self._stream()), "collision_hessian")
         self.collision_triplets = trip
         return trip
 
@@ -312,8 +342,13 @@ class ContactSolverBase:
     def collision_energy_into(self, e: torch.Tensor):
         """The barrier energy of the current contact lists into the device
         double e[0], without a host transfer (dynamic_contacts.py:300-305)."""
-        ops.collision_energy(self.x, self.xcs, self.contacts_new.list, self.n_contacts, self.contacts_pt.list,
-                             self.n_contacts_pt, self.l0, self.contact_stiffness, e)
+        x = self.x
+        xp = self._ptr("x", x, torch.float32)
+        check(lib().ipc_collision_energy(xp, self._ptr("xcs", self.xcs, torch.float32, x), x.shape[0],
+                                         self.contacts_new.list.data_ptr(), self.n_contacts,
+                                         self.contacts_pt.list.data_ptr(), self.n_contacts_pt, self.l0,
+                                         self.contact_stiffness, self._ptr("e", e, torch.float64), self._stream()),
+              "collision_energy")
 
     def compute_collision_energy(self) -> float:
         """dynamic_contacts.py:299-308: DCD, then h^2 * sum of barrier energies."""

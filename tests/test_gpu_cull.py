@@ -247,3 +247,34 @@ def test_count_intersections_matches_oracle(dev, oracle, shift):
     g = ids.cpu().numpy()
     g = g[np.lexsort((g[:, 1], g[:, 0]))]
     assert np.array_equal(g, r_ids)
+
+
+def test_solver_hessian_and_energy_match_oracle(dev, oracle):
+    """ContactSolverBase.compute_collision_hessian / collision_energy_raw (the
+    contact part of compute_A and compute_collision_energy,
+    dynamic_contacts.py:281-305) after its own DCD, against the oracle on the
+    same contact lists; a new b tensor is checked afresh, a misshaped one
+    refused."""
+    soup = scenes.two_cubes_on_ground(seed=17, n=8, gap=4e-4, jitter=2e-4)
+    S = _solver(dev, soup)
+    S.detect_collision()
+    ee = S.contacts_new.list[:S.n_contacts].cpu().numpy()
+    pt = S.contacts_pt.list[:S.n_contacts_pt].cpu().numpy()
+    n_ee, n_pt = len(ee), len(pt)
+    assert n_ee > 50 and n_pt > 50
+    l0 = np.float32(TH) * np.float32(2)
+    r_ee = oracle.contact_hessian("ee", soup.x, ee, l0, 1e8, n_total=n_ee + n_pt, threads=8)
+    r_pt = oracle.contact_hessian("pt", soup.x, pt, l0, 1e8, offset=n_ee, n_total=n_ee + n_pt, threads=8)
+    r_b = r_ee[3] + r_pt[3]
+    for _ in range(2):
+        b = torch.zeros_like(S.x)
+        trip = S.compute_collision_hessian(b)
+        torch.cuda.synchronize()
+        assert np.array_equal(trip.rows.cpu().numpy(), r_ee[0] + r_pt[0])
+        assert np.array_equal(trip.cols.cpu().numpy(), r_ee[1] + r_pt[1])
+        assert np.abs(b.cpu().numpy() - r_b).max() <= 1e-5 * np.abs(r_b).max()
+    e_ref = oracle.contact_energy("ee", soup.x, ee, l0, 1e8, threads=8) + oracle.contact_energy("pt", soup.x, pt, l0,
+                                                                                               1e8, threads=8)
+    assert S.collision_energy_raw() == pytest.approx(e_ref, rel=1e-9)
+    with pytest.raises(ValueError):
+        S.compute_collision_hessian(torch.zeros((3, 3), dtype=torch.float32, device=dev))
