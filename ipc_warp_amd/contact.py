@@ -85,6 +85,7 @@ class ContactSolverBase:
         self.device = dev
         self.x = x
         self.xcs = xcs
+        self._nv = int(x.shape[0])  # the vertex count the library context is built for
         self.edges = edges.to(dev, torch.int32).contiguous().reshape(-1, 2)
         self.triangles = triangles.to(dev, torch.int32).contiguous().reshape(-1, 3)
         self.body = body.to(dev, torch.int32).contiguous()
@@ -134,9 +135,10 @@ class ContactSolverBase:
         return ops.stream_handle(self.device)
 
     def _ptr(self, name: str, t: torch.Tensor, dtype: torch.dtype, like: torch.Tensor | None = None) -> int:
-        """t.data_ptr() after the ops argument checks (float [n, 3] or a 1-d
-        tensor of dtype; shaped like `like` when given), which run again only
-        when the tensor under this name is a different object or storage."""
+        """t.data_ptr() after the ops argument checks (float32 [V, 3] with the
+        context's V, or a tensor of dtype; shaped like `like` when given),
+        which run again only when the tensor under this name is a different
+        object or storage."""
         cache = self.__dict__.setdefault("_checked", {})
         hit = cache.get(name)
         if hit is not None and hit[0] is t:
@@ -144,6 +146,8 @@ class ContactSolverBase:
             if hit[1] == ptr:
                 return ptr
         ptr = ops._need(t, dtype, name, (3,) if dtype == torch.float32 else None)
+        if dtype == torch.float32 and t.shape[0] != self._nv:
+            raise ValueError(f"{name} must have {self._nv} rows (got {t.shape[0]})")
         if like is not None and t.shape != like.shape:
             raise ValueError(f"{name} must have the shape of x")
         cache[name] = (t, ptr)
@@ -167,9 +171,10 @@ class ContactSolverBase:
         run concurrently) and one readback of both counts; a list that overflowed
         grows to the next power of two and the call runs again."""
         self.compute_V(ret=False)
+        xp = self._ptr("x", self.x, torch.float32)
         while True:
             ee, pt = self.contacts_new, self.contacts_pt
-            check(lib().ipc_detect_collision(self._ctx, self.x.data_ptr(), float(self.thickness), float(self.buffer),
+            check(lib().ipc_detect_collision(self._ctx, xp, float(self.thickness), float(self.buffer),
                                              int(self.disable_self_collision), ee.list.data_ptr(), ee.e0e1.data_ptr(),
                                              int(ee.capacity), ee.cnt.data_ptr(), pt.list.data_ptr(),
                                              pt.e0e1.data_ptr(), int(pt.capacity) if self.has_triangles else 0,
@@ -195,10 +200,10 @@ class ContactSolverBase:
 
     def raw_toi(self, dx, x=None) -> float:
         x = self.x if x is None else x
-        if dx.shape != x.shape or dx.dtype != torch.float32 or not dx.is_contiguous():
-            raise ValueError("dx must be a contiguous float32 tensor shaped like x")
+        xp = self._ptr("x" if x is self.x else "x_toi", x, torch.float32)
+        dxp = self._ptr("dx", dx, torch.float32, x)
         out = ctypes.c_double(1.0)
-        check(lib().ipc_collision_free_step(self._ctx, x.data_ptr(), dx.data_ptr(), float(CCD_PADDING),
+        check(lib().ipc_collision_free_step(self._ctx, xp, dxp, float(CCD_PADDING),
                                             int(self.disable_self_collision), ctypes.byref(out), self._stream()),
               "collision_free_step")
         return float(out.value)
