@@ -21,6 +21,7 @@ libipc_warp_amd. Module globals mirror contact.py:14-22.
 from __future__ import annotations
 
 import ctypes
+import weakref
 from dataclasses import dataclass
 
 import torch
@@ -138,10 +139,11 @@ class ContactSolverBase:
         """t.data_ptr() after the ops argument checks (float32 [V, 3] with the
         context's V, or a tensor of dtype; shaped like `like` when given),
         which run again only when the tensor under this name is a different
-        object or storage."""
+        object or storage. The cache holds weak references: a caller's
+        tensor is not kept alive by it."""
         cache = self.__dict__.setdefault("_checked", {})
         hit = cache.get(name)
-        if hit is not None and hit[0] is t:
+        if hit is not None and hit[0]() is t:
             ptr = t.data_ptr()
             if hit[1] == ptr:
                 return ptr
@@ -150,7 +152,7 @@ class ContactSolverBase:
             raise ValueError(f"{name} must have {self._nv} rows (got {t.shape[0]})")
         if like is not None and t.shape != like.shape:
             raise ValueError(f"{name} must have the shape of x")
-        cache[name] = (t, ptr)
+        cache[name] = (weakref.ref(t), ptr)
         return ptr
 
     def _grow_contact_list(self, contacts: Contacts, required: int) -> Contacts:
