@@ -41,10 +41,16 @@ __device__ __forceinline__ int ord_of(float f) {
 }
 __device__ __forceinline__ float float_of(int k) { return __int_as_float(k >= 0 ? k : k ^ 0x7fffffff); }
 
+// The scene bounds are kept in kBoundCopies copies of (min xyz, max xyz),
+// workgroup b of the bounds launch folding into copy b % 8 (its XCD under
+// round-robin dispatch): atomics of every workgroup on the same six words
+// serialised and crossed XCDs. The Morton launch folds the copies.
+constexpr int kBoundCopies = 8;
+constexpr int kBoundInts = 6 * kBoundCopies;
+
 __global__ void init_bounds_kernel(int* bounds) {
   int t = threadIdx.x;
-  if (t < 3) bounds[t] = 0x7fffffff;           // min
-  else if (t < 6) bounds[t] = (int)0x80000000;  // max
+  if (t < kBoundInts) bounds[t] = t % 6 < 3 ? 0x7fffffff : (int)0x80000000;  // min xyz, max xyz
 }
 
 __device__ __forceinline__ unsigned expand_bits10(unsigned v) {
@@ -146,8 +152,9 @@ __global__ void __launch_bounds__(256) centre_bounds_kernel(int mode, int n, con
     int a = threadIdx.x;
     int mn = smin_[a][0], mx = smax_[a][0];
     for (int w = 1; w < int(blockDim.x >> 6); ++w) { mn = min(mn, smin_[a][w]); mx = max(mx, smax_[a][w]); }
-    atomicMin(bounds + a, mn);
-    atomicMax(bounds + 3 + a, mx);
+    int* copy = bounds + 6 * (blockIdx.x % kBoundCopies);
+    atomicMin(copy + a, mn);
+    atomicMax(copy + 3 + a, mx);
   }
 }
 
@@ -156,6 +163,14 @@ __global__ void __launch_bounds__(256) morton_kernel(int mode, int n, const int*
                                                      const float* __restrict__ dx, float pad,
                                                      const int* __restrict__ bounds,
                                                      unsigned long long* __restrict__ keys) {
+  __shared__ int sb[6];
+  if (threadIdx.x < 6) {  // fold the copies
+    const int t = threadIdx.x;
+    int v = bounds[t];
+    for (int k = 1; k < kBoundCopies; ++k) v = t < 3 ? min(v, bounds[6 * k + t]) : max(v, bounds[6 * k + t]);
+    sb[t] = v;
+  }
+  __syncthreads();
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float4 lo, hi;
@@ -163,7 +178,7 @@ __global__ void __launch_bounds__(256) morton_kernel(int mode, int n, const int*
   float c[3] = {0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z)};
   unsigned q[3];
   for (int a = 0; a < 3; ++a) {
-    float b0 = float_of(bounds[a]), b1 = float_of(bounds[3 + a]);
+    float b0 = float_of(sb[a]), b1 = float_of(sb[3 + a]);
     float ext = b1 - b0;
     float u = ext > 0.f ? (c[a] - b0) / ext : 0.5f;
     float s = u * 1024.f;
@@ -879,7 +894,7 @@ hipError_t prepare_job(const BuildJob& j, const int* body, const float* x0, hipS
   Bvh& B = *j.bvh;
   const int n = j.n;
   long long total = plan_levels(B.info, n);
-  IPC_TRY(B.bounds.ensure(sizeof(int) * 8));
+  IPC_TRY(B.bounds.ensure(sizeof(int) * kBoundInts));
   IPC_TRY(B.keys.ensure(sizeof(unsigned long long) * n));
   IPC_TRY(B.keys_sorted.ensure(sizeof(unsigned long long) * n));
   IPC_TRY(B.leaf_id.ensure(sizeof(int) * n));
@@ -999,7 +1014,7 @@ hipError_t point_order(PointOrder& P, int n, const float* x, hipStream_t s, cons
     *order = P.order.as<int>();
     return hipSuccess;
   }
-  IPC_TRY(P.bounds.ensure(sizeof(int) * 8));
+  IPC_TRY(P.bounds.ensure(sizeof(int) * kBoundInts));
   IPC_TRY(P.keys.ensure(sizeof(unsigned long long) * n));
   IPC_TRY(P.keys_sorted.ensure(sizeof(unsigned long long) * n));
   IPC_TRY(P.order.ensure(sizeof(int) * n));

@@ -41,7 +41,7 @@ struct Bvh {
   DevBuf leaf_id;             // primitive id per leaf
   DevBuf leaf_vid;            // int4 per leaf slot: vertex ids (v2 = -1 for an edge), primitive id
   DevBuf temp;                // radix sort workspace
-  DevBuf bounds;              // 6 ordered ints
+  DevBuf bounds;              // ordered ints: copies of (min xyz, max xyz)
   int builds_since_sort = -1; // -1: no leaf order yet
   int n_sorted = 0;
   // leaf_vid and the leaves' body ids were written for this leaf order from
