@@ -161,8 +161,8 @@ __global__ void __launch_bounds__(256) centre_bounds_kernel(int mode, int n, con
 __global__ void __launch_bounds__(256) morton_kernel(int mode, int n, const int* __restrict__ prims,
                                                      const float* __restrict__ x0, const float* __restrict__ x1,
                                                      const float* __restrict__ dx, float pad,
-                                                     const int* __restrict__ bounds,
-                                                     unsigned long long* __restrict__ keys) {
+                                                     const int* __restrict__ bounds, unsigned* __restrict__ codes,
+                                                     int* __restrict__ ids) {
   __shared__ int sb[6];
   if (threadIdx.x < 6) {  // fold the copies
     const int t = threadIdx.x;
@@ -185,8 +185,23 @@ __global__ void __launch_bounds__(256) morton_kernel(int mode, int n, const int*
     s = s < 0.f ? 0.f : (s > 1023.f ? 1023.f : s);
     q[a] = unsigned(s);
   }
-  unsigned m = (expand_bits10(q[0]) << 2) | (expand_bits10(q[1]) << 1) | expand_bits10(q[2]);
-  keys[i] = (static_cast<unsigned long long>(m) << 32) | static_cast<unsigned>(i);
+  codes[i] = (expand_bits10(q[0]) << 2) | (expand_bits10(q[1]) << 1) | expand_bits10(q[2]);
+  ids[i] = i;
+}
+
+// Stable radix sort of the 30-bit Morton codes carrying the primitive ids:
+// equal codes keep increasing id order, the order of a sort of (code, id).
+// keys and keys_sorted hold n codes followed by n ids each.
+static hipError_t sort_codes(DevBuf& keys, DevBuf& keys_sorted, DevBuf& temp, int n, int* ids_out, hipStream_t s) {
+  unsigned* codes = keys.as<unsigned>();
+  int* ids = keys.as<int>() + n;
+  unsigned* codes_sorted = keys_sorted.as<unsigned>();
+  size_t temp_bytes = 0;
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, codes, codes_sorted, ids, ids_out, n, 0, 30, s);
+  if (e == hipSuccess) e = temp.ensure(temp_bytes);
+  if (e != hipSuccess) return e;
+  temp_bytes = temp.bytes;
+  return hipcub::DeviceRadixSort::SortPairs(temp.p, temp_bytes, codes, codes_sorted, ids, ids_out, n, 0, 30, s);
 }
 
 // ------------------------------------------------------------ hierarchy ----
@@ -227,7 +242,7 @@ struct LeafJob {
   const float* x1;
   const float* dx;  // with dx, end positions are x0 - dx (x1 unused)
   float pad;
-  const unsigned long long* keys;
+  const int* ids;  // sorted primitive ids (leaf order)
   int* leaf_id;
   int4* leaf_vid;
   float4* node_lo;
@@ -244,8 +259,8 @@ struct LeafSmem {
 // leaf slot. Workgroup b of the job owns leaf slots [512 b, 512 (b + 1)):
 // level-1 nodes are merged across 8-lane groups, level-2 nodes across a
 // wave, the level-3 node across the workgroup's 8 waves in LDS. Leaf k's box
-// is computed from the topology and positions of primitive keys[k] (low 32
-// bits); slots past n hold empty boxes. leaf_vid[k] holds the leaf's vertex
+// is computed from the topology and positions of primitive ids[k]; slots
+// past n hold empty boxes. leaf_vid[k] holds the leaf's vertex
 // ids (v2 = -1 for an edge) and primitive id, so that a query reads one
 // record per leaf instead of chasing the topology.
 __device__ __forceinline__ void leaf_block(const LeafJob& J, int bid, const int* __restrict__ body,
@@ -264,7 +279,7 @@ __device__ __forceinline__ void leaf_block(const LeafJob& J, int bid, const int*
       vid = J.leaf_vid[k];
       bd = __float_as_int(J.node_lo[k].w);
     } else {
-      const int prim = static_cast<int>(J.keys[k] & 0xffffffffu);
+      const int prim = J.ids[k];
       const int v0 = J.prims[stride * prim], v1 = J.prims[stride * prim + 1];
       const int v2 = stride == 3 ? J.prims[stride * prim + 2] : -1;
       bd = body[v0];
@@ -910,15 +925,9 @@ hipError_t prepare_job(const BuildJob& j, const int* body, const float* x0, hipS
     hipLaunchKernelGGL(centre_bounds_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, j.mode, n, j.prims, x0, j.x1, j.dx,
                        j.pad, B.bounds.as<int>());
     hipLaunchKernelGGL(morton_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, j.mode, n, j.prims, x0, j.x1, j.dx, j.pad,
-                       B.bounds.as<int>(), B.keys.as<unsigned long long>());
+                       B.bounds.as<int>(), B.keys.as<unsigned>(), B.keys.as<int>() + n);
     IPC_TRY(hipGetLastError());
-    size_t temp_bytes = 0;
-    IPC_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, temp_bytes, B.keys.as<unsigned long long>(),
-                                              B.keys_sorted.as<unsigned long long>(), n, 0, 62, s));
-    IPC_TRY(B.temp.ensure(temp_bytes));
-    temp_bytes = B.temp.bytes;
-    IPC_TRY(hipcub::DeviceRadixSort::SortKeys(B.temp.p, temp_bytes, B.keys.as<unsigned long long>(),
-                                              B.keys_sorted.as<unsigned long long>(), n, 0, 62, s));
+    IPC_TRY(sort_codes(B.keys, B.keys_sorted, B.temp, n, B.keys_sorted.as<int>() + n, s));
     B.builds_since_sort = 0;
     B.n_sorted = n;
   } else {
@@ -936,7 +945,7 @@ LeafJob leaf_job(const BuildJob& j) {
   L.x1 = j.x1;
   L.dx = j.dx;
   L.pad = j.pad;
-  L.keys = B.keys_sorted.as<unsigned long long>();
+  L.ids = B.keys_sorted.as<int>() + j.n;
   L.leaf_id = B.leaf_id.as<int>();
   L.leaf_vid = B.leaf_vid.as<int4>();
   L.node_lo = B.node_lo.as<float4>();
@@ -994,12 +1003,6 @@ hipError_t build_bvhs(const BuildJob* jobs, int n_jobs, const int* body, const f
   return hipGetLastError();
 }
 
-__global__ void __launch_bounds__(256) key_ids_kernel(int n, const unsigned long long* __restrict__ keys,
-                                                      int* __restrict__ ids) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) ids[i] = static_cast<int>(keys[i] & 0xffffffffu);
-}
-
 void PointOrder::release() {
   keys.release(); keys_sorted.release(); order.release(); temp.release(); bounds.release();
   n = 0;
@@ -1024,18 +1027,10 @@ hipError_t point_order(PointOrder& P, int n, const float* x, hipStream_t s, cons
                      static_cast<const float*>(nullptr), 0.0f, P.bounds.as<int>());
   hipLaunchKernelGGL(morton_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, int(BOX_POINT), n,
                      static_cast<const int*>(nullptr), x, static_cast<const float*>(nullptr),
-                     static_cast<const float*>(nullptr), 0.0f, P.bounds.as<int>(), P.keys.as<unsigned long long>());
+                     static_cast<const float*>(nullptr), 0.0f, P.bounds.as<int>(), P.keys.as<unsigned>(),
+                     P.keys.as<int>() + n);
   IPC_TRY(hipGetLastError());
-  size_t temp_bytes = 0;
-  IPC_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, temp_bytes, P.keys.as<unsigned long long>(),
-                                            P.keys_sorted.as<unsigned long long>(), n, 0, 62, s));
-  IPC_TRY(P.temp.ensure(temp_bytes));
-  temp_bytes = P.temp.bytes;
-  IPC_TRY(hipcub::DeviceRadixSort::SortKeys(P.temp.p, temp_bytes, P.keys.as<unsigned long long>(),
-                                            P.keys_sorted.as<unsigned long long>(), n, 0, 62, s));
-  hipLaunchKernelGGL(key_ids_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, n,
-                     P.keys_sorted.as<unsigned long long>(), P.order.as<int>());
-  IPC_TRY(hipGetLastError());
+  IPC_TRY(sort_codes(P.keys, P.keys_sorted, P.temp, n, P.order.as<int>(), s));
   P.n = n;
   P.age = 0;
   *order = P.order.as<int>();

@@ -35,7 +35,7 @@ struct LevelInfo {
 struct Bvh {
   int n = 0;
   LevelInfo info{};
-  DevBuf keys, keys_sorted;   // uint64 Morton | id
+  DevBuf keys, keys_sorted;   // n uint32 Morton codes, then n primitive ids
   DevBuf node_lo, node_hi;    // float4 per slot of every level (leaves first): xyz the box,
                               // w the bits of the (min, max) body id below; (b, b) for a leaf
   DevBuf leaf_id;             // primitive id per leaf
