@@ -131,6 +131,27 @@ def test_dispatch_order_keeps_the_sets(dev, oracle):
         assert len(r_ee) > 0 and len(r_pt) > 0
 
 
+def test_leaf_order_refresh_keeps_the_sets(dev, oracle):
+    """A tree's leaf order is re-sorted every 16 builds and reused, with
+    only the boxes recomputed, in between. Forty DCD + CCD rounds (two
+    builds of each tree per round) on drifting positions cross several
+    re-sorts; every round keeps the exact contact sets and CCD candidate
+    counts."""
+    soup, vel = scenes.icosphere_grid(seed=17, per_axis=2, spacing=1.0003, subdivisions=2)
+    rng = np.random.default_rng(17)
+    S = _solver(dev, soup, exclude=True)
+    for _ in range(40):
+        soup.x = (soup.x + rng.uniform(-2e-3, 2e-3, soup.x.shape)).astype(np.float32)
+        S.x.copy_(torch.from_numpy(soup.x).to(dev))
+        _check_dcd(S, soup, True)
+        dx = (-1e-2 * vel[soup.body]).astype(np.float32)
+        x1 = soup.x - dx
+        S.raw_toi(torch.from_numpy(dx).to(dev))
+        r_ee, _ = oracle.ccd_candidates_ee(soup.x, x1, soup.edges, soup.body, 1e-7, True)
+        r_pt, _ = oracle.ccd_candidates_pt(soup.x, x1, soup.triangles, soup.body, 1e-7, True)
+        assert S.last_ccd_counts() == (len(r_ee), len(r_pt))
+
+
 def test_degenerate_soups(dev, oracle):
     # one edge and no triangles; two crossing edges
     x = np.array([[0, 0, 0], [1, 0, 0], [0.5, -0.5, 4e-4], [0.5, 0.5, 4e-4]], np.float32)
