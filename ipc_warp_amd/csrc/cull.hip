@@ -189,19 +189,54 @@ __global__ void __launch_bounds__(256) morton_kernel(int mode, int n, const int*
   ids[i] = i;
 }
 
-// Stable radix sort of the 30-bit Morton codes carrying the primitive ids:
-// equal codes keep increasing id order, the order of a sort of (code, id).
-// keys and keys_sorted hold n codes followed by n ids each.
-static hipError_t sort_codes(DevBuf& keys, DevBuf& keys_sorted, DevBuf& temp, int n, int* ids_out, hipStream_t s) {
-  unsigned* codes = keys.as<unsigned>();
-  int* ids = keys.as<int>() + n;
-  unsigned* codes_sorted = keys_sorted.as<unsigned>();
-  size_t temp_bytes = 0;
-  hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, codes, codes_sorted, ids, ids_out, n, 0, 30, s);
-  if (e == hipSuccess) e = temp.ensure(temp_bytes);
+// Leaf order by Morton cell: primitives are bucketed by the top 18 of the 30
+// code bits (64 cells per axis) with a counting sort, cells in Morton order.
+// The order inside a cell (a 1/64 slice of the scene per axis, a few
+// primitives) is the order of the scatter's atomics, so it may differ from
+// run to run; the order only affects speed, never the pairs found. Four
+// launches (clear, count, scan, scatter), against a block sort and about
+// seven merge launches for a full radix sort of the codes at these sizes.
+constexpr int kCellShift = 12;
+constexpr int kCells = 1 << (30 - kCellShift);
+// masked: a code built from a NaN centre may carry bits above 30
+__device__ __forceinline__ int cell_of(unsigned code) { return static_cast<int>((code >> kCellShift) & (kCells - 1)); }
+
+__global__ void __launch_bounds__(256) cell_count_kernel(int n, const unsigned* __restrict__ codes,
+                                                         int* __restrict__ count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicAdd(&count[cell_of(codes[i])], 1);
+}
+
+__global__ void __launch_bounds__(256) cell_scatter_kernel(int n, const unsigned* __restrict__ codes,
+                                                           int* __restrict__ cursor, int* __restrict__ ids_out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) ids_out[atomicAdd(&cursor[cell_of(codes[i])], 1)] = i;
+}
+
+// keys holds the n codes (then n ids, unused here); ids_out receives the
+// n primitive ids in cell order. temp holds the cell counts and cursors and
+// the scan's workspace.
+static hipError_t sort_codes(DevBuf& keys, DevBuf& temp, int n, int* ids_out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const unsigned* codes = keys.as<unsigned>();
+  size_t scan_bytes = 0;
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, static_cast<const int*>(nullptr),
+                                                  static_cast<int*>(nullptr), kCells, s);
+  const size_t cell_bytes = sizeof(int) * kCells;
+  if (e == hipSuccess) e = temp.ensure(2 * cell_bytes + scan_bytes);
   if (e != hipSuccess) return e;
-  temp_bytes = temp.bytes;
-  return hipcub::DeviceRadixSort::SortPairs(temp.p, temp_bytes, codes, codes_sorted, ids, ids_out, n, 0, 30, s);
+  int* count = temp.as<int>();
+  int* cursor = count + kCells;
+  void* scan_temp = static_cast<char*>(temp.p) + 2 * cell_bytes;
+  e = launch_zero_i32(count, kCells, s);
+  if (e != hipSuccess) return e;
+  const int blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(cell_count_kernel, dim3(blocks), dim3(256), 0, s, n, codes, count);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(scan_temp, scan_bytes, count, cursor, kCells, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(cell_scatter_kernel, dim3(blocks), dim3(256), 0, s, n, codes, cursor, ids_out);
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------ hierarchy ----
@@ -865,7 +900,7 @@ void DevBuf::release() {
 }
 
 void Bvh::release() {
-  keys.release(); keys_sorted.release();
+  keys.release(); leaf_order.release();
   leaf_id.release(); node_lo.release(); node_hi.release(); leaf_vid.release();
   temp.release(); bounds.release();
   n = 0;
@@ -911,7 +946,7 @@ hipError_t prepare_job(const BuildJob& j, const int* body, const float* x0, hipS
   long long total = plan_levels(B.info, n);
   IPC_TRY(B.bounds.ensure(sizeof(int) * kBoundInts));
   IPC_TRY(B.keys.ensure(sizeof(unsigned long long) * n));
-  IPC_TRY(B.keys_sorted.ensure(sizeof(unsigned long long) * n));
+  IPC_TRY(B.leaf_order.ensure(sizeof(int) * n));
   IPC_TRY(B.leaf_id.ensure(sizeof(int) * n));
   IPC_TRY(B.leaf_vid.ensure(sizeof(int4) * B.info.slots[0]));
   IPC_TRY(B.node_lo.ensure(sizeof(float4) * total));
@@ -927,7 +962,7 @@ hipError_t prepare_job(const BuildJob& j, const int* body, const float* x0, hipS
     hipLaunchKernelGGL(morton_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, j.mode, n, j.prims, x0, j.x1, j.dx, j.pad,
                        B.bounds.as<int>(), B.keys.as<unsigned>(), B.keys.as<int>() + n);
     IPC_TRY(hipGetLastError());
-    IPC_TRY(sort_codes(B.keys, B.keys_sorted, B.temp, n, B.keys_sorted.as<int>() + n, s));
+    IPC_TRY(sort_codes(B.keys, B.temp, n, B.leaf_order.as<int>(), s));
     B.builds_since_sort = 0;
     B.n_sorted = n;
   } else {
@@ -945,7 +980,7 @@ LeafJob leaf_job(const BuildJob& j) {
   L.x1 = j.x1;
   L.dx = j.dx;
   L.pad = j.pad;
-  L.ids = B.keys_sorted.as<int>() + j.n;
+  L.ids = B.leaf_order.as<int>();
   L.leaf_id = B.leaf_id.as<int>();
   L.leaf_vid = B.leaf_vid.as<int4>();
   L.node_lo = B.node_lo.as<float4>();
@@ -1004,7 +1039,7 @@ hipError_t build_bvhs(const BuildJob* jobs, int n_jobs, const int* body, const f
 }
 
 void PointOrder::release() {
-  keys.release(); keys_sorted.release(); order.release(); temp.release(); bounds.release();
+  keys.release(); order.release(); temp.release(); bounds.release();
   n = 0;
   age = -1;
 }
@@ -1019,7 +1054,6 @@ hipError_t point_order(PointOrder& P, int n, const float* x, hipStream_t s, cons
   }
   IPC_TRY(P.bounds.ensure(sizeof(int) * kBoundInts));
   IPC_TRY(P.keys.ensure(sizeof(unsigned long long) * n));
-  IPC_TRY(P.keys_sorted.ensure(sizeof(unsigned long long) * n));
   IPC_TRY(P.order.ensure(sizeof(int) * n));
   hipLaunchKernelGGL(init_bounds_kernel, dim3(1), dim3(64), 0, s, P.bounds.as<int>());
   hipLaunchKernelGGL(centre_bounds_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, int(BOX_POINT), n,
@@ -1030,7 +1064,7 @@ hipError_t point_order(PointOrder& P, int n, const float* x, hipStream_t s, cons
                      static_cast<const float*>(nullptr), 0.0f, P.bounds.as<int>(), P.keys.as<unsigned>(),
                      P.keys.as<int>() + n);
   IPC_TRY(hipGetLastError());
-  IPC_TRY(sort_codes(P.keys, P.keys_sorted, P.temp, n, P.order.as<int>(), s));
+  IPC_TRY(sort_codes(P.keys, P.temp, n, P.order.as<int>(), s));
   P.n = n;
   P.age = 0;
   *order = P.order.as<int>();

@@ -35,12 +35,13 @@ struct LevelInfo {
 struct Bvh {
   int n = 0;
   LevelInfo info{};
-  DevBuf keys, keys_sorted;   // n uint32 Morton codes, then n primitive ids
+  DevBuf keys;                // n uint32 Morton codes, then n primitive ids
+  DevBuf leaf_order;          // primitive id per leaf slot from the last re-sort (Morton cell order)
   DevBuf node_lo, node_hi;    // float4 per slot of every level (leaves first): xyz the box,
                               // w the bits of the (min, max) body id below; (b, b) for a leaf
   DevBuf leaf_id;             // primitive id per leaf
   DevBuf leaf_vid;            // int4 per leaf slot: vertex ids (v2 = -1 for an edge), primitive id
-  DevBuf temp;                // radix sort workspace
+  DevBuf temp;                // re-sort workspace: cell counts, cursors, scan
   DevBuf bounds;              // ordered ints: copies of (min xyz, max xyz)
   int builds_since_sort = -1; // -1: no leaf order yet
   int n_sorted = 0;
@@ -53,7 +54,7 @@ struct Bvh {
   void release();
 };
 
-// Leaf order is recomputed (Morton sort) on the first build and then every
+// Leaf order is recomputed (by Morton cell) on the first build and then every
 // kResortEvery builds; in between, builds reuse it and only recompute boxes.
 constexpr int kResortEvery = 16;
 
@@ -64,7 +65,7 @@ enum BoxMode { BOX_EDGE = 0, BOX_TRI = 1, BOX_SWEPT_EDGE = 2, BOX_SWEPT_TRI = 3,
 // subtrees). Recomputed on the first call and every kResortEvery calls; the
 // order only affects speed, never the pairs found.
 struct PointOrder {
-  DevBuf keys, keys_sorted, order, temp, bounds;
+  DevBuf keys, order, temp, bounds;
   int n = 0, age = -1;
   void release();
 };
