@@ -152,6 +152,39 @@ def test_leaf_order_refresh_keeps_the_sets(dev, oracle):
         assert S.last_ccd_counts() == (len(r_ee), len(r_pt))
 
 
+def test_non_finite_vertices_drop_out(dev, oracle):
+    """A NaN vertex and a vertex at +inf (one coordinate), each of them in a
+    contact of the finite scene: the contacts through them drop out and
+    every other contact stays, as in the oracle, through a fresh build
+    (NaN and infinite box centres in the leaf order) and a second build
+    that reuses it. The CCD candidates with the infinite vertex moving (its
+    boxes reach to +inf) are the oracle's too."""
+    soup, vel = scenes.icosphere_grid(seed=18, per_axis=2, spacing=1.0003, subdivisions=2)
+    r_ee, _ = oracle.dcd_ee(soup.x, soup.edges, soup.body, TH, TH, True)
+    r_pt, _ = oracle.dcd_pt(soup.x, soup.triangles, soup.body, TH, TH, True)
+    v_nan, v_inf = int(r_ee[0][0]), int(r_pt[-1][0])
+    assert v_nan != v_inf
+    x = soup.x.copy()
+    x[v_inf, 1] = np.inf
+    x_nan = x.copy()
+    x_nan[v_nan] = np.nan
+    S = _solver(dev, soup, exclude=True)
+    for xs in (x_nan, x_nan, x):
+        soup.x = xs
+        S.x.copy_(torch.from_numpy(xs).to(dev))
+        n_ee, n_pt = _check_dcd(S, soup, True)
+        assert n_ee < len(r_ee) and n_pt < len(r_pt)
+    dx = (-1e-2 * vel[soup.body]).astype(np.float32)
+    x1 = x - dx
+    for kind in ("ee", "pt"):
+        quads, ids = S.ccd_candidates(kind, S.x, torch.from_numpy(x1).to(dev))
+        q, i = _sorted(quads.cpu().numpy(), ids.cpu().numpy())
+        fn = oracle.ccd_candidates_ee if kind == "ee" else oracle.ccd_candidates_pt
+        rq, ri = fn(x, x1, soup.edges if kind == "ee" else soup.triangles, soup.body, 1e-7, True)
+        assert len(rq) > 0
+        assert np.array_equal(i, ri) and np.array_equal(q, rq)
+
+
 def test_degenerate_soups(dev, oracle):
     # one edge and no triangles; two crossing edges
     x = np.array([[0, 0, 0], [1, 0, 0], [0.5, -0.5, 4e-4], [0.5, 0.5, 4e-4]], np.float32)
