@@ -48,9 +48,18 @@ __device__ __forceinline__ float float_of(int k) { return __int_as_float(k >= 0 
 constexpr int kBoundCopies = 8;
 constexpr int kBoundInts = 6 * kBoundCopies;
 
-__global__ void init_bounds_kernel(int* bounds) {
-  int t = threadIdx.x;
-  if (t < kBoundInts) bounds[t] = t % 6 < 3 ? 0x7fffffff : (int)0x80000000;  // min xyz, max xyz
+// Leaf order cells: the top 18 of the 30 Morton code bits (64 per axis).
+constexpr int kCellShift = 12;
+constexpr int kCells = 1 << (30 - kCellShift);
+// masked: a code built from a NaN centre may carry bits above 30
+__device__ __forceinline__ int cell_of(unsigned code) { return static_cast<int>((code >> kCellShift) & (kCells - 1)); }
+
+// Resets the bound copies and clears the kCells cell counts of a re-sort.
+constexpr int kInitBlocks = 256;
+__global__ void __launch_bounds__(256) init_bounds_kernel(int* __restrict__ bounds, int* __restrict__ count) {
+  const int t = threadIdx.x;
+  if (blockIdx.x == 0 && t < kBoundInts) bounds[t] = t % 6 < 3 ? 0x7fffffff : (int)0x80000000;  // min xyz, max xyz
+  for (int i = blockIdx.x * blockDim.x + t; i < kCells; i += kInitBlocks * 256) count[i] = 0;
 }
 
 __device__ __forceinline__ unsigned expand_bits10(unsigned v) {
@@ -162,7 +171,7 @@ __global__ void __launch_bounds__(256) morton_kernel(int mode, int n, const int*
                                                      const float* __restrict__ x0, const float* __restrict__ x1,
                                                      const float* __restrict__ dx, float pad,
                                                      const int* __restrict__ bounds, unsigned* __restrict__ codes,
-                                                     int* __restrict__ ids) {
+                                                     int* __restrict__ count) {
   __shared__ int sb[6];
   if (threadIdx.x < 6) {  // fold the copies
     const int t = threadIdx.x;
@@ -185,57 +194,55 @@ __global__ void __launch_bounds__(256) morton_kernel(int mode, int n, const int*
     s = s < 0.f ? 0.f : (s > 1023.f ? 1023.f : s);
     q[a] = unsigned(s);
   }
-  codes[i] = (expand_bits10(q[0]) << 2) | (expand_bits10(q[1]) << 1) | expand_bits10(q[2]);
-  ids[i] = i;
+  const unsigned code = (expand_bits10(q[0]) << 2) | (expand_bits10(q[1]) << 1) | expand_bits10(q[2]);
+  codes[i] = code;
+  atomicAdd(&count[cell_of(code)], 1);
 }
 
 // Leaf order by Morton cell: primitives are bucketed by the top 18 of the 30
 // code bits (64 cells per axis) with a counting sort, cells in Morton order.
 // The order inside a cell (a 1/64 slice of the scene per axis, a few
 // primitives) is the order of the scatter's atomics, so it may differ from
-// run to run; the order only affects speed, never the pairs found. Four
-// launches (clear, count, scan, scatter), against a block sort and about
-// seven merge launches for a full radix sort of the codes at these sizes.
-constexpr int kCellShift = 12;
-constexpr int kCells = 1 << (30 - kCellShift);
-// masked: a code built from a NaN centre may carry bits above 30
-__device__ __forceinline__ int cell_of(unsigned code) { return static_cast<int>((code >> kCellShift) & (kCells - 1)); }
-
-__global__ void __launch_bounds__(256) cell_count_kernel(int n, const unsigned* __restrict__ codes,
-                                                         int* __restrict__ count) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) atomicAdd(&count[cell_of(codes[i])], 1);
-}
-
+// run to run; the order only affects speed, never the pairs found. The
+// counts are cleared in the bounds launch and counted in the Morton launch,
+// so the sort adds two launches (scan, scatter), against a block sort and
+// about seven merge launches for a full radix sort of the codes at these
+// sizes.
 __global__ void __launch_bounds__(256) cell_scatter_kernel(int n, const unsigned* __restrict__ codes,
                                                            int* __restrict__ cursor, int* __restrict__ ids_out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) ids_out[atomicAdd(&cursor[cell_of(codes[i])], 1)] = i;
 }
 
-// keys holds the n codes (then n ids, unused here); ids_out receives the
-// n primitive ids in cell order. temp holds the cell counts and cursors and
-// the scan's workspace.
-static hipError_t sort_codes(DevBuf& keys, DevBuf& temp, int n, int* ids_out, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  const unsigned* codes = keys.as<unsigned>();
-  size_t scan_bytes = 0;
-  hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, static_cast<const int*>(nullptr),
-                                                  static_cast<int*>(nullptr), kCells, s);
+// temp of a re-sort: cell counts (cleared by init_bounds_kernel, counted by
+// morton_kernel), cell cursors, the scan's workspace.
+struct CellBufs {
+  int* count;
+  int* cursor;
+  void* scan_temp;
+  size_t scan_bytes;
+};
+
+static hipError_t cell_buffers(DevBuf& temp, CellBufs& c) {
+  c.scan_bytes = 0;
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, c.scan_bytes, static_cast<const int*>(nullptr),
+                                                  static_cast<int*>(nullptr), kCells, hipStream_t(nullptr));
   const size_t cell_bytes = sizeof(int) * kCells;
-  if (e == hipSuccess) e = temp.ensure(2 * cell_bytes + scan_bytes);
+  if (e == hipSuccess) e = temp.ensure(2 * cell_bytes + c.scan_bytes);
   if (e != hipSuccess) return e;
-  int* count = temp.as<int>();
-  int* cursor = count + kCells;
-  void* scan_temp = static_cast<char*>(temp.p) + 2 * cell_bytes;
-  e = launch_zero_i32(count, kCells, s);
+  c.count = temp.as<int>();
+  c.cursor = c.count + kCells;
+  c.scan_temp = static_cast<char*>(temp.p) + 2 * cell_bytes;
+  return hipSuccess;
+}
+
+// ids_out receives the n primitive ids in cell order, from the counts.
+static hipError_t sort_codes(const CellBufs& c, const unsigned* codes, int n, int* ids_out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  size_t scan_bytes = c.scan_bytes;
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(c.scan_temp, scan_bytes, c.count, c.cursor, kCells, s);
   if (e != hipSuccess) return e;
-  const int blocks = (n + 255) / 256;
-  hipLaunchKernelGGL(cell_count_kernel, dim3(blocks), dim3(256), 0, s, n, codes, count);
-  e = hipGetLastError();
-  if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(scan_temp, scan_bytes, count, cursor, kCells, s);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(cell_scatter_kernel, dim3(blocks), dim3(256), 0, s, n, codes, cursor, ids_out);
+  hipLaunchKernelGGL(cell_scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, codes, c.cursor, ids_out);
   return hipGetLastError();
 }
 
@@ -945,7 +952,7 @@ hipError_t prepare_job(const BuildJob& j, const int* body, const float* x0, hipS
   const int n = j.n;
   long long total = plan_levels(B.info, n);
   IPC_TRY(B.bounds.ensure(sizeof(int) * kBoundInts));
-  IPC_TRY(B.keys.ensure(sizeof(unsigned long long) * n));
+  IPC_TRY(B.keys.ensure(sizeof(unsigned) * n));
   IPC_TRY(B.leaf_order.ensure(sizeof(int) * n));
   IPC_TRY(B.leaf_id.ensure(sizeof(int) * n));
   IPC_TRY(B.leaf_vid.ensure(sizeof(int4) * B.info.slots[0]));
@@ -956,13 +963,15 @@ hipError_t prepare_job(const BuildJob& j, const int* body, const float* x0, hipS
   B.vid_prims = j.prims;
   B.vid_body = body;
   if (resort) {
-    hipLaunchKernelGGL(init_bounds_kernel, dim3(1), dim3(64), 0, s, B.bounds.as<int>());
+    CellBufs cb;
+    IPC_TRY(cell_buffers(B.temp, cb));
+    hipLaunchKernelGGL(init_bounds_kernel, dim3(kInitBlocks), dim3(256), 0, s, B.bounds.as<int>(), cb.count);
     hipLaunchKernelGGL(centre_bounds_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, j.mode, n, j.prims, x0, j.x1, j.dx,
                        j.pad, B.bounds.as<int>());
     hipLaunchKernelGGL(morton_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, j.mode, n, j.prims, x0, j.x1, j.dx, j.pad,
-                       B.bounds.as<int>(), B.keys.as<unsigned>(), B.keys.as<int>() + n);
+                       B.bounds.as<int>(), B.keys.as<unsigned>(), cb.count);
     IPC_TRY(hipGetLastError());
-    IPC_TRY(sort_codes(B.keys, B.temp, n, B.leaf_order.as<int>(), s));
+    IPC_TRY(sort_codes(cb, B.keys.as<unsigned>(), n, B.leaf_order.as<int>(), s));
     B.builds_since_sort = 0;
     B.n_sorted = n;
   } else {
@@ -1053,18 +1062,19 @@ hipError_t point_order(PointOrder& P, int n, const float* x, hipStream_t s, cons
     return hipSuccess;
   }
   IPC_TRY(P.bounds.ensure(sizeof(int) * kBoundInts));
-  IPC_TRY(P.keys.ensure(sizeof(unsigned long long) * n));
+  IPC_TRY(P.keys.ensure(sizeof(unsigned) * n));
   IPC_TRY(P.order.ensure(sizeof(int) * n));
-  hipLaunchKernelGGL(init_bounds_kernel, dim3(1), dim3(64), 0, s, P.bounds.as<int>());
+  CellBufs cb;
+  IPC_TRY(cell_buffers(P.temp, cb));
+  hipLaunchKernelGGL(init_bounds_kernel, dim3(kInitBlocks), dim3(256), 0, s, P.bounds.as<int>(), cb.count);
   hipLaunchKernelGGL(centre_bounds_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, int(BOX_POINT), n,
                      static_cast<const int*>(nullptr), x, static_cast<const float*>(nullptr),
                      static_cast<const float*>(nullptr), 0.0f, P.bounds.as<int>());
   hipLaunchKernelGGL(morton_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, int(BOX_POINT), n,
                      static_cast<const int*>(nullptr), x, static_cast<const float*>(nullptr),
-                     static_cast<const float*>(nullptr), 0.0f, P.bounds.as<int>(), P.keys.as<unsigned>(),
-                     P.keys.as<int>() + n);
+                     static_cast<const float*>(nullptr), 0.0f, P.bounds.as<int>(), P.keys.as<unsigned>(), cb.count);
   IPC_TRY(hipGetLastError());
-  IPC_TRY(sort_codes(P.keys, P.temp, n, P.order.as<int>(), s));
+  IPC_TRY(sort_codes(cb, P.keys.as<unsigned>(), n, P.order.as<int>(), s));
   P.n = n;
   P.age = 0;
   *order = P.order.as<int>();

@@ -35,7 +35,7 @@ struct LevelInfo {
 struct Bvh {
   int n = 0;
   LevelInfo info{};
-  DevBuf keys;                // n uint32 Morton codes, then n primitive ids
+  DevBuf keys;                // n uint32 Morton codes of the last re-sort
   DevBuf leaf_order;          // primitive id per leaf slot from the last re-sort (Morton cell order)
   DevBuf node_lo, node_hi;    // float4 per slot of every level (leaves first): xyz the box,
                               // w the bits of the (min, max) body id below; (b, b) for a leaf
