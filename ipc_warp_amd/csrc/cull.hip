@@ -7,10 +7,10 @@
 // refits a BVH built once; a rebuild keeps query cost independent of how far
 // the mesh has moved):
 //   1. one AABB per primitive (fp32, the reference's inflation),
-//   2. 30-bit Morton codes of box centres in the scene bounds, made unique by
-//      the primitive id in the low 32 bits of a 64-bit key, sorted with the
-//      hipCUB radix sort on the first build and every kResortEvery builds
-//      (in between, the leaf order is kept and only the boxes are refreshed),
+//   2. 30-bit Morton codes of box centres in the scene bounds, the leaves
+//      ordered by Morton cell (a counting sort on the top 18 code bits) on
+//      the first build and every kResortEvery builds (in between, the leaf
+//      order is kept and only the boxes are refreshed),
 //   3. an implicit 8-ary hierarchy over the sorted leaves: node k of level L
 //      bounds nodes 8k .. 8k+7 of level L-1 (level 0 = leaves). Levels 1-3
 //      are reduced by one kernel, one thread per leaf (a 512-thread
