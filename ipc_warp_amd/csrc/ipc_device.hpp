@@ -262,15 +262,26 @@ IPCD_FN FeatureGeom feature_geom(const Feature& f, const V3<double> x[4]) {
 // det[q - p, a - p, b - p], det[q - p, b - p, c - p], det[q - p, c - p, a - p]
 // do not take both signs). A segment lying in the triangle's plane, or a
 // degenerate triangle, does not count: that is contact, not a crossing.
-IPCD_FN double orient3(V3<double> a, V3<double> b, V3<double> c, V3<double> d) {
-  return dot(cross(b - a, c - a), d - a);
+// Each volume's sign is certain or taken as zero: a value within the forward
+// error bound of the FP64 evaluation (Shewchuk's orient3d bound,
+// (7 + 56 eps) eps times the permanent, eps = 2^-53) counts as zero, so
+// exactly coplanar inputs (planar mesh regions) never produce a crossing
+// from rounding noise.
+IPCD_FN int orient3_sign(V3<double> a, V3<double> b, V3<double> c, V3<double> d) {
+  const V3<double> u = b - a, v = c - a, w = d - a;
+  const double m0 = u.y * v.z, m1 = u.z * v.y, m2 = u.z * v.x, m3 = u.x * v.z, m4 = u.x * v.y, m5 = u.y * v.x;
+  const double det = w.x * (m0 - m1) + w.y * (m2 - m3) + w.z * (m4 - m5);
+  const double perm = fabs(w.x) * (fabs(m0) + fabs(m1)) + fabs(w.y) * (fabs(m2) + fabs(m3)) +
+                      fabs(w.z) * (fabs(m4) + fabs(m5));
+  const double bound = (7.0 + 56.0 * 0x1p-53) * 0x1p-53 * perm;
+  return det > bound ? 1 : (det < -bound ? -1 : 0);
 }
 IPCD_FN bool segment_hits_triangle(V3<double> p, V3<double> q, V3<double> a, V3<double> b, V3<double> c) {
-  double sp = orient3(a, b, c, p), sq = orient3(a, b, c, q);
-  if ((sp > 0.0 && sq > 0.0) || (sp < 0.0 && sq < 0.0) || (sp == 0.0 && sq == 0.0)) return false;
-  double u = orient3(p, q, a, b), v = orient3(p, q, b, c), w = orient3(p, q, c, a);
-  if (u == 0.0 && v == 0.0 && w == 0.0) return false;
-  bool neg = u < 0.0 || v < 0.0 || w < 0.0, pos = u > 0.0 || v > 0.0 || w > 0.0;
+  const int sp = orient3_sign(a, b, c, p), sq = orient3_sign(a, b, c, q);
+  if (sp * sq > 0 || (sp == 0 && sq == 0)) return false;
+  const int u = orient3_sign(p, q, a, b), v = orient3_sign(p, q, b, c), w = orient3_sign(p, q, c, a);
+  if (u == 0 && v == 0 && w == 0) return false;
+  const bool neg = u < 0 || v < 0 || w < 0, pos = u > 0 || v > 0 || w > 0;
   return !(neg && pos);
 }
 

@@ -415,3 +415,368 @@ def ccd_stream(kind: str, n: int, seed: int = 0, hit_fraction: float = 0.3, coor
     x0 = X0.reshape(-1, 3).astype(np.float32)
     x1 = X1.reshape(-1, 3).astype(np.float32)
     return x0, x1, np.arange(4 * n, dtype=np.int32).reshape(n, 4), t
+
+
+# ------------------------------------------------------- rod bundle (twist) ----
+def hex_disk(rings: int, radius: float):
+    """Triangulated regular hexagon of circumradius `radius`: the triangular
+    lattice points with |i|, |j|, |i + j| <= rings (spacing radius / rings),
+    corners at angles 0, 60, ..., 300 degrees. Returns (P float64 [n, 2],
+    triangles int32 [6 rings^2, 3]), counter-clockwise."""
+    pts, index = [], {}
+    for i in range(-rings, rings + 1):
+        for j in range(-rings, rings + 1):
+            if abs(i + j) <= rings:
+                index[(i, j)] = len(pts)
+                pts.append((i + 0.5 * j, (3.0 ** 0.5 / 2.0) * j))
+    tris = []
+    for i in range(-rings - 1, rings + 1):
+        for j in range(-rings - 1, rings + 1):
+            a, b = index.get((i, j)), index.get((i + 1, j))
+            c, d = index.get((i, j + 1)), index.get((i + 1, j + 1))
+            if a is not None and b is not None and c is not None:
+                tris.append((a, b, c))
+            if b is not None and c is not None and d is not None:
+                tris.append((b, d, c))
+    return np.asarray(pts, np.float64) * (radius / rings), np.asarray(tris, np.int32)
+
+
+def extruded_prism_tets(section_tris: np.ndarray, n_section: int, segments: int) -> np.ndarray:
+    """Tets of a triangle section extruded over `segments` layers, 3 per prism.
+
+    Layer s holds vertices s * n_section + k. A prism over triangle (a, b, c)
+    with a < b < c (section indices) splits into (a0 b0 c0 c1), (a0 b0 b1 c1),
+    (a0 a1 b1 c1): the diagonal of each side quad runs from the smaller
+    index's lower vertex, so neighbouring prisms agree on it (conforming)."""
+    tri = np.sort(section_tris, axis=1).astype(np.int64)
+    s = np.arange(segments, dtype=np.int64)[:, None, None]
+    lo = tri[None] + s * n_section
+    hi = lo + n_section
+    a0, b0, c0 = lo[..., 0], lo[..., 1], lo[..., 2]
+    a1, b1, c1 = hi[..., 0], hi[..., 1], hi[..., 2]
+    T = np.stack([np.stack([a0, b0, c0, c1], -1), np.stack([a0, b0, b1, c1], -1),
+                  np.stack([a0, a1, b1, c1], -1)], axis=2)
+    return T.reshape(-1, 4).astype(np.int32)
+
+
+def _smoothstep(t):
+    t = np.clip(t, 0.0, 1.0)
+    return t * t * (3.0 - 2.0 * t)
+
+
+ROD_TWIST_RATE = 2.0  # rad/s at each end cap (the config); rods_twist.py:36 uses 0.4 pi
+ROD_TWIST_DT = 0.025  # RodsTwist's default time step (rods_twist.py:85)
+ROD_THICKNESS = 1.0e-3  # rods_twist.py:38,91-92 (thickness and buffer, unit-scaled scene)
+
+
+def rod_bundle(seed: int = 0, per_side: int = 4, length: float = 10.0, radius: float = 0.1,
+               segments: int = 400, spacing: float = 0.25, rings: int = 2, steps: int = 200,
+               twist_rate: float = ROD_TWIST_RATE, dt: float = ROD_TWIST_DT, thickness: float = ROD_THICKNESS,
+               gap: float = 0.5, gap_jitter: float = 0.15, press: float = 0.6, vertex_noise: float = 0.05):
+    """Twisted bundle of hexagonal-prism tet rods at the peak of a rod-twist run.
+
+    per_side^2 rods (length `length`, hexagon circumradius `radius`, `rings`
+    lattice rings per section, `segments` layers) along x on a square grid
+    `spacing` apart, scaled to unit extent like rods_twist.py:108-113 (so
+    `thickness` is in unit-scene coordinates, as the reference's contact
+    thickness is). Hexagon corners face the y neighbours, flats the z
+    neighbours.
+
+    The state after `steps` steps of `dt` with both end caps turned at
+    `twist_rate` in opposite senses (set_twist_displacement,
+    rods_twist.py:46-75) is generated rather than simulated: over the middle
+    of the bundle the rods are pressed together, each y (z) neighbour pair
+    `gap` * l0 apart across corners (flats) with a smooth seeded per-rod
+    offset of up to `gap_jitter` * l0, easing back to the rest grid over the
+    outer 10 % at each end; then every cross-section is turned about the
+    bundle axis by theta(x) = theta_end * (2 x / L - 1). Each cross-section
+    is placed rigidly and x is kept, so the rods stay disjoint (the map is
+    one-to-one) while neighbours sit within l0 = 2 thickness.
+
+    The step's motion (x1 = x - dx) turns the ends one step further and
+    presses the middle by another `press` * l0 (with new per-rod offsets)
+    plus U(-1, 1) * `vertex_noise` * l0 per vertex, so the closest
+    neighbours meet within the step.
+
+    Returns (soup, dx float32 [V, 3], fixed mask [V] of end-cap vertices).
+    """
+    rng = np.random.default_rng(seed)
+    P2, tri2 = hex_disk(rings, radius)
+    nsec = P2.shape[0]
+    T1 = extruded_prism_tets(tri2, nsec, segments)
+    xs = np.linspace(0.0, length, segments + 1)
+    rod_rest = np.zeros(((segments + 1) * nsec, 3))
+    rod_rest[:, 0] = np.repeat(xs, nsec)
+    rod_rest[:, 1:] = np.tile(P2, (segments + 1, 1))
+    # orient every tet positively and find the outward boundary once
+    e1 = rod_rest[T1[:, 1]] - rod_rest[T1[:, 0]]
+    e2 = rod_rest[T1[:, 2]] - rod_rest[T1[:, 0]]
+    e3 = rod_rest[T1[:, 3]] - rod_rest[T1[:, 0]]
+    neg = np.einsum("ij,ij->i", np.cross(e1, e2), e3) < 0
+    T1[neg] = T1[neg][:, [0, 2, 1, 3]]
+    F1 = boundary_faces(rod_rest, T1)
+
+    n_rods = per_side * per_side
+    grid = np.array([(a - (per_side - 1) / 2.0, b - (per_side - 1) / 2.0)
+                     for a in range(per_side) for b in range(per_side)])
+    scale = 1.0 / max(length, spacing * (per_side - 1) + 2.0 * radius)
+    L, r, l0 = length * scale, radius * scale, 2.0 * thickness
+    rest_c = grid * spacing * scale
+    pitch_pressed = np.array([2.0 * r + gap * l0, 3.0 ** 0.5 * r + gap * l0])
+    pressed_c = grid * pitch_pressed
+
+    def offsets(rng_):
+        # smooth per-rod in-plane offsets, |offset| <= gap_jitter * l0 / 2 per axis
+        m = np.arange(1, 4)
+        amp = rng_.uniform(-1.0, 1.0, (n_rods, 2, 3)) / m
+        phase = rng_.uniform(0, 2 * np.pi, (n_rods, 2, 3))
+        norm = 0.5 * gap_jitter * l0 / np.abs(1.0 / m).sum()
+        u = (xs * scale / L)[:, None, None, None]
+        return norm * (amp[None] * np.sin(2 * np.pi * m * u + phase[None])).sum(-1)  # [S+1, rods, 2]
+
+    w = _smoothstep((xs * scale / L) / 0.1) * _smoothstep((1.0 - xs * scale / L) / 0.1)  # [S+1]
+    theta_end = twist_rate * steps * dt
+
+    def place(theta_e, press_by, offs):
+        c = (1.0 - w)[:, None, None] * rest_c[None] + w[:, None, None] * (
+            pressed_c[None] - press_by * l0 * grid[None] + offs)
+        th = theta_e * (2.0 * xs * scale / L - 1.0)
+        cs, sn = np.cos(th)[:, None], np.sin(th)[:, None]
+        out = np.zeros((n_rods, segments + 1, nsec, 3))
+        for k in range(n_rods):
+            yz = P2[None] * scale + c[:, k][:, None, :]
+            y, z = yz[..., 0], yz[..., 1]
+            out[k, ..., 0] = (xs * scale)[:, None]
+            out[k, ..., 1] = cs * y - sn * z
+            out[k, ..., 2] = sn * y + cs * z
+        return out.reshape(n_rods, -1, 3)
+
+    x0 = place(theta_end, 0.0, offsets(rng))
+    x1 = place(theta_end + twist_rate * dt, press, offsets(rng))
+    x1 = x1 + vertex_noise * l0 * rng.uniform(-1.0, 1.0, x1.shape)
+    rest = [rod_rest * scale + np.array([0.0, rest_c[k, 0], rest_c[k, 1]]) for k in range(n_rods)]
+    soup = assemble([(rest[k], F1, T1) for k in range(n_rods)])
+    soup.x = x0.reshape(-1, 3).astype(np.float32)
+    x1 = x1.reshape(-1, 3)
+    xr = soup.xcs.astype(np.float64)
+    fixed = (xr[:, 0] < 1e-5 * scale) | (xr[:, 0] > L - 1e-5 * scale)
+    x1[fixed] = x0.reshape(-1, 3)[fixed]
+    dx = (soup.x - x1.astype(np.float32)).astype(np.float32)
+    return soup, dx, fixed
+
+
+# ------------------------------------------------------- screw and nut ----
+SCREW_PITCH = 0.2
+SCREW_RADIUS = 1.0
+# thread profile over one pitch, in sixteenths: crest 2, flank 5, root 4, flank 5
+# (a 60-degree ISO-like profile: depth = (5/16) pitch / tan 30 deg)
+_PROFILE_16 = np.array([0, 2, 7, 11, 16])
+
+
+def thread_radius(phase: np.ndarray, major: float = SCREW_RADIUS, pitch: float = SCREW_PITCH) -> np.ndarray:
+    """Radius of the screw's thread surface at helical phase `phase` in [0, 1)
+    of a pitch: crest (major radius), falling flank, root, rising flank."""
+    depth = (5.0 / 16.0) * pitch / np.tan(np.pi / 6.0)
+    knots = _PROFILE_16 / 16.0
+    level = np.array([0.0, 0.0, 1.0, 1.0, 0.0])
+    return major - depth * np.interp(np.mod(phase, 1.0), knots, level)
+
+
+def _helical_band(n_theta: int, rows: int, turns: int, u0: float, pitch: float, radius_of_u):
+    """A surface band wound `turns` times about the y axis.
+
+    Vertex (i, j), i in [0, turns * n_theta], j in [0, rows]: angle
+    2 pi i / n_theta, y = u + pitch * i / n_theta with the helical
+    coordinate u = u0 + pitch * j / rows, radius radius_of_u(u). (i, rows) is the same point as (i + n_theta, 0)
+    and is stored once. Rows follow the helix, so kinks of a thread profile
+    at multiples of 1 / rows lie on mesh rows. Returns (V [n, 3], vid(i, j)
+    index function over arrays, quads [m, 4] (i, j), (i+1, j), (i+1, j+1),
+    (i, j+1))."""
+    imax = turns * n_theta
+    # stored: j in [0, rows) for every i, plus (i, rows) for i > imax - n_theta
+    n_low = (imax + 1) * rows
+    top_i = np.arange(imax - n_theta + 1, imax + 1)
+
+    def vid(i, j):
+        i = np.asarray(i)
+        j = np.asarray(j)
+        ii = np.where(j == rows, i + n_theta, i)
+        jj = np.where(j == rows, 0, j)
+        low = ii * rows + jj
+        top = n_low + (i - (imax - n_theta + 1))
+        return np.where((j == rows) & (ii > imax), top, low)
+
+    I = np.concatenate([np.repeat(np.arange(imax + 1), rows), top_i])
+    J = np.concatenate([np.tile(np.arange(rows), imax + 1), np.full(top_i.size, rows)])
+    ang = 2.0 * np.pi * I / n_theta
+    rad = radius_of_u(u0 + pitch * (J / rows))
+    V = np.stack([rad * np.cos(ang), u0 + pitch * (J / rows + I / n_theta), rad * np.sin(ang)], -1)
+    qi, qj = np.meshgrid(np.arange(imax), np.arange(rows), indexing="ij")
+    qi, qj = qi.ravel(), qj.ravel()
+    quads = np.stack([vid(qi, qj), vid(qi + 1, qj), vid(qi + 1, qj + 1), vid(qi, qj + 1)], -1)
+    return V, vid, quads
+
+
+def _quads_to_tris(q: np.ndarray) -> np.ndarray:
+    return np.concatenate([q[:, [0, 1, 2]], q[:, [0, 2, 3]]])
+
+
+def _stitch(a: np.ndarray, b: np.ndarray, ta: np.ndarray, tb: np.ndarray) -> np.ndarray:
+    """Triangle strip between polylines a and b (vertex ids) with increasing
+    parameters ta, tb over the same interval (equal end parameters). Where
+    both next parameters are equal the quad a_i a_i+1 b_j+1 b_j is split
+    along a_i - b_j+1, so matching points are joined by an edge."""
+    tris, i, j = [], 0, 0
+    while i < len(a) - 1 or j < len(b) - 1:
+        if i < len(a) - 1 and j < len(b) - 1 and ta[i + 1] == tb[j + 1]:
+            tris += [(a[i], a[i + 1], b[j + 1]), (a[i], b[j + 1], b[j])]
+            i += 1
+            j += 1
+        elif j == len(b) - 1 or (i < len(a) - 1 and ta[i + 1] < tb[j + 1]):
+            tris.append((a[i], a[i + 1], b[j]))
+            i += 1
+        else:
+            tris.append((a[i], b[j + 1], b[j]))
+            j += 1
+    return np.asarray(tris, np.int64)
+
+
+def _band_ends(vid, n_theta, rows, turns):
+    """Boundary polylines of a band: (bottom row, start wall column, top row,
+    end wall column), each ordered by increasing angle or height."""
+    imax = turns * n_theta
+    bottom = vid(np.arange(n_theta + 1), np.zeros(n_theta + 1, int))
+    start = vid(np.zeros(rows + 1, int), np.arange(rows + 1))
+    top = vid(np.arange(imax - n_theta, imax + 1), np.full(n_theta + 1, rows))
+    end = vid(np.full(rows + 1, imax), np.arange(rows + 1))
+    return bottom, start, top, end
+
+
+def screw_body(n_theta: int, rows: int, turns: int, u0: float, pitch: float = SCREW_PITCH,
+               major: float = SCREW_RADIUS, thread_origin: float = 0.0):
+    """Solid threaded screw along y: the thread band closed at each end by a
+    helicoid ramp (one turn of the band's end row joined to the y axis) and
+    a wall in the plane of angle 0 (the band's end column joined to the
+    axis). Ramp and wall share their axis points (n_theta must be a multiple
+    of rows, so the wall's heights are among the ramp's); the ramp's first
+    quad is split so that no ramp triangle lies in the wall's plane.
+    Returns (V, F) with F oriented outward."""
+    if n_theta % rows:
+        raise ValueError("n_theta must be a multiple of rows")
+    V, vid, quads = _helical_band(n_theta, rows, turns, u0, pitch,
+                                  lambda u: thread_radius((u - thread_origin) / pitch, major, pitch))
+    F = [_quads_to_tris(quads)]
+    Vs = [V]
+    base = V.shape[0]
+    bottom, start, top, end = _band_ends(vid, n_theta, rows, turns)
+    for ramp, wall, flip in ((bottom, start, False), (top, end, True)):
+        ys = V[ramp, 1]
+        axis = base + np.arange(ys.size)
+        Vs.append(np.stack([np.zeros_like(ys), ys, np.zeros_like(ys)], -1))
+        base += ys.size
+        a0, a1, b0, b1 = ramp[:-1], ramp[1:], axis[:-1], axis[1:]
+        r = np.concatenate([np.stack([a0, a1, b1], -1), np.stack([a0, b1, b0], -1)])
+        m = ys.size - 1  # the first quad: split along a1 - b0 instead
+        r[0] = (a0[0], a1[0], b0[0])
+        r[m] = (a1[0], b1[0], b0[0])
+        w = _stitch(wall, axis, V[wall, 1], ys)
+        part = np.concatenate([r[:, ::-1], w])
+        F.append(part[:, ::-1] if flip else part)
+    return np.concatenate(Vs), np.concatenate(F)[:, ::-1]
+
+
+def nut_body(n_theta_in: int, n_theta_out: int, rows: int, turns: int, u0: float, clearance: float,
+             outer: float, pitch: float = SCREW_PITCH, major: float = SCREW_RADIUS, thread_origin: float = 0.0):
+    """Threaded nut along y: inner band = the thread surface of a screw whose
+    thread starts at helical coordinate `thread_origin`, pushed out radially
+    by `clearance`, outer band a cylinder of radius `outer`,
+    joined by ramps (stitched by angle) and walls at angle 0. Returns (V, F)
+    with F oriented outward."""
+    Vi, vi, qi = _helical_band(n_theta_in, rows, turns, u0, pitch,
+                               lambda u: thread_radius((u - thread_origin) / pitch, major, pitch) + clearance)
+    Vo, vo, qo = _helical_band(n_theta_out, rows, turns, u0, pitch, lambda u: np.full_like(u, outer))
+    off = Vi.shape[0]
+    V = np.concatenate([Vi, Vo])
+    F = [_quads_to_tris(qi)[:, ::-1], _quads_to_tris(qo + off)]
+    bi, si, ti, ei = _band_ends(vi, n_theta_in, rows, turns)
+    bo, so, to, eo = _band_ends(vo, n_theta_out, rows, turns)
+    tin = np.arange(n_theta_in + 1) / n_theta_in
+    tout = np.arange(n_theta_out + 1) / n_theta_out
+    F.append(_stitch(bi, bo + off, tin, tout))
+    F.append(_stitch(ti, to + off, tin, tout)[:, ::-1])
+    F.append(_quads_to_tris(np.stack([si[:-1], so[:-1] + off, so[1:] + off, si[1:]], -1)))
+    F.append(_quads_to_tris(np.stack([ei[:-1], eo[:-1] + off, eo[1:] + off, ei[1:]], -1))[:, ::-1])
+    return V, np.concatenate(F)[:, ::-1]
+
+
+def _weld(V: np.ndarray, F: np.ndarray):
+    """Merge vertices with equal coordinates (to 1e-12), keeping first-seen order."""
+    _, first, inv = np.unique(np.round(V, 12), axis=0, return_index=True, return_inverse=True)
+    order = np.argsort(first)
+    remap = np.empty_like(order)
+    remap[order] = np.arange(order.size)
+    return V[first[order]], remap[inv.reshape(-1)][F]
+
+
+SCREW_DT = 0.01  # ScrewAndNut's h (affine_body_dynamics.py:764)
+SCREW_MOTOR = -np.pi  # rad/s about y (motors={0: (0, -pi, 0)}, affine_body_dynamics.py:770)
+
+
+def screw_and_nut(seed: int = 0, n_theta: int = 512, n_theta_nut: int = 460, n_theta_outer: int = 64,
+                  rows: int = 32, screw_turns: int = 4, nut_turns: int = 3, clearance: float = 1.0e-3,
+                  outer: float = 1.4, shift: float = 2.0e-4, h: float = SCREW_DT, omega: float = SCREW_MOTOR):
+    """Threaded screw (body 0) inside a matching fixed nut (body 1), as affine bodies.
+
+    The screw is a solid of major radius 1 with an ISO-like 60-degree thread
+    of pitch 0.2 over `screw_turns` turns; the nut's threaded bore is the
+    screw's thread surface pushed out radially by `clearance` (a normal gap
+    of clearance/2 on the flanks), `nut_turns` turns long and centred, so
+    the screw's thread shows half a turn beyond each end. Mesh rows follow
+    the helix (`rows` per pitch, a multiple of 16 so the profile's kinks lie
+    on rows); the screw, the nut's bore and the nut's outside use
+    `n_theta`, `n_theta_nut` and `n_theta_outer` columns per turn, so the
+    two threaded surfaces are not tessellated alike.
+
+    Seeded placement: the screw sits off the nut's axis by U(-shift, shift)
+    in x, y and z (every gap stays positive). The step turns the screw about
+    its axis by omega * h (the reference's motor, affine_body_dynamics.py:
+    760-770), which drives its flanks into the nut's within the step.
+
+    Returns (soup, q float32 [2 * 4, 3] = (p, A[:,0], A[:,1], A[:,2]) per
+    body, dq float32 like q (q1 = q - dq), dx float32 [V, 3] (x1 = x - dx),
+    fixed_bodies int32 [2] = (0, 1)). Body-local rest coordinates are in
+    soup.xcs; soup.x = A xcs + p.
+    """
+    if rows % 16:
+        raise ValueError("rows must be a multiple of 16")
+    rng = np.random.default_rng(seed)
+    P = SCREW_PITCH
+    u0_s = -(screw_turns + 1) * P / 2.0
+    Vs, Fs = screw_body(n_theta, rows, screw_turns, u0_s, thread_origin=u0_s)
+    u0_n = u0_s + P * (screw_turns - nut_turns) / 2.0
+    Vn, Fn = nut_body(n_theta_nut, n_theta_outer, rows, nut_turns, u0_n, clearance, outer, thread_origin=u0_s)
+    soup = assemble([(Vs, Fs, None), (Vn, Fn, None)])
+    # body frames: origin on the axis at the middle of each body, A = I
+    p = np.array([rng.uniform(-shift, shift, 3), [0.0, 0.0, 0.0]])
+    local = soup.xcs.astype(np.float64)
+    soup.xcs = local.astype(np.float32)
+    q = np.zeros((2, 4, 3))
+    q[:, 0] = p
+    q[:, 1:] = np.eye(3)[None]
+    q32 = q.astype(np.float32)
+    qd = q32.astype(np.float64)
+    b = soup.body
+    rest = soup.xcs.astype(np.float64)
+    x = qd[b, 0] + np.einsum("vij,vj->vi", qd[b, 1:].transpose(0, 2, 1), rest)
+    soup.x = x.astype(np.float32)
+    ang = omega * h
+    c, s = np.cos(ang), np.sin(ang)
+    Ry = np.array([[c, 0.0, s], [0.0, 1.0, 0.0], [-s, 0.0, c]])
+    q1 = qd.copy()
+    q1[0, 1:] = (Ry @ qd[0, 1:].T).T  # A1 = Ry A0 (rows of q hold A's columns)
+    q1 = q1.astype(np.float32)
+    q1d = q1.astype(np.float64)
+    x1 = q1d[b, 0] + np.einsum("vij,vj->vi", q1d[b, 1:].transpose(0, 2, 1), rest)
+    dq = (q32 - q1).reshape(-1, 3)
+    dx = (soup.x - x1.astype(np.float32)).astype(np.float32)
+    return soup, q32.reshape(-1, 3), dq.astype(np.float32), dx, np.array([0, 1], np.int32)

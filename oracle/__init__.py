@@ -60,6 +60,9 @@ def lib():
                                              I32P, I32P, F64P, F64P, U8P, c_int]
         L.oracle_contact_energy.argtypes = [c_int, c_int, c_int, F32P, F32P, I32P, c_int, c_flt, c_dbl, c_int]
         L.oracle_contact_energy.restype = c_dbl
+        L.oracle_contact_pair.argtypes = [c_int, c_int, c_int, F64P, F64P, c_flt, c_dbl, F64P, F64P, I32P]
+        L.oracle_contact_pair.restype = c_dbl
+        L.oracle_contact_points.argtypes = [c_int, c_int, F32P, F32P, I32P, c_int, F64P, F64P, I32P]
         L.oracle_cubic_roots.argtypes = [c_int, F64P, c_dbl, c_dbl, F64P, I32P]
         L.oracle_cubic_roots.restype = c_int
         L.oracle_coplanarity_cubic.argtypes = [c_int, F64P, F64P, F64P]
@@ -200,6 +203,36 @@ def contact_energy(kind, x, contacts, l0, kappa, xcs=None, mode="mixed", threads
     return float(lib().oracle_contact_energy(int(kind == "ee"), prec, pol, _p(x, F32P), _p(xcs, F32P),
                                              _p(contacts, I32P), contacts.shape[0], float(l0), float(kappa),
                                              int(threads)))
+
+
+def contact_pair(kind, x12, l0, kappa, rest12=None, mode="fp64"):
+    """One contact at double positions x12 [12] (EE: rest12 for the mollifier
+    threshold; default x12): (energy, force [12] = -gradient, H [12, 12],
+    feature (kind, 4 slots) with kind -1 when inactive)."""
+    x12 = _f64(np.asarray(x12).reshape(12))
+    r12 = None if rest12 is None else _f64(np.asarray(rest12).reshape(12))
+    force = np.zeros(12)
+    H = np.zeros((12, 12))
+    feat = np.zeros(5, dtype=np.int32)
+    prec, pol = _MODES[mode]
+    e = lib().oracle_contact_pair(int(kind == "ee"), prec, pol, _p(x12, F64P), None if r12 is None else _p(r12, F64P),
+                                  float(l0), float(kappa), _p(force, F64P), _p(H, F64P), _p(feat, I32P))
+    return float(e), force, H, (int(feat[0]), [int(v) for v in feat[1:]])
+
+
+def contact_points(kind, x, contacts, xcs=None, double=False):
+    """get_contact_points(_pt) (contact.py:148-161): midpoints [n, 3], distances
+    [n] and dispatched feature codes [n] (the layout of ipc_contact_points_*)."""
+    x = _f32(x).reshape(-1, 3)
+    xcs = None if xcs is None else _f32(xcs).reshape(-1, 3)
+    contacts = _i32(contacts).reshape(-1, 4)
+    n = contacts.shape[0]
+    pts = np.zeros((n, 3))
+    dist = np.zeros(n)
+    feat = np.zeros(n, dtype=np.int32)
+    lib().oracle_contact_points(int(kind == "ee"), int(double), _p(x, F32P), None if xcs is None else _p(xcs, F32P),
+                                _p(contacts, I32P), n, _p(pts, F64P), _p(dist, F64P), _p(feat, I32P))
+    return pts, dist, feat
 
 
 # --------------------------------------------------------------------- CCD ----

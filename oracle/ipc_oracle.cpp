@@ -26,10 +26,12 @@ inline bool overlap(const Box& a, const Box& b) {
 // Calls fn(a_id, b_id) once for every overlapping pair with one box from A and
 // one from B (every unordered pair when self = true), found through a uniform
 // hash grid; every candidate is confirmed with `overlap`, so the enumeration
-// is exact and independent of the grid's parameters. Cell size
-// is twice the median box extent; boxes spanning more than kBigCells cells on
-// an axis are tested against everything directly. A pair of grid boxes is
-// reported only from the cell holding the lower corner of their
+// is exact and independent of the grid's parameters. Cell size is twice the
+// median box extent. Boxes spanning more than kBigCells cells on an axis
+// ("big") stay out of the grid: each one visits the grid cells it covers
+// (or, when those exceed kMaxVisit, scans every small box), and big boxes
+// are tested against each other directly. A pair found through the grid is
+// reported only from the cell holding the lower corner of the two boxes'
 // intersection, so each pair is reported once.
 template <typename Fn>
 void grid_pairs(const std::vector<Box>& A, const std::vector<Box>& B, bool self, Fn fn) {
@@ -43,6 +45,7 @@ void grid_pairs(const std::vector<Box>& A, const std::vector<Box>& B, bool self,
   std::nth_element(ext.begin(), ext.begin() + ext.size() / 2, ext.end());
   double h = std::max(2.0 * double(ext[ext.size() / 2]), 1e-6);
   const int kBigCells = 16;
+  const long long kMaxVisit = 1 << 16;
   auto cell = [h](float v) { return (long long)std::floor(double(v) / h); };
   auto is_big = [&](const Box& b) {
     for (int k = 0; k < 3; ++k)
@@ -52,47 +55,67 @@ void grid_pairs(const std::vector<Box>& A, const std::vector<Box>& B, bool self,
   auto key = [](long long i, long long j, long long k) {
     return (unsigned long long)((i & 0x1FFFFF) | ((j & 0x1FFFFF) << 21) | ((k & 0x1FFFFF) << 42));
   };
-  std::vector<int> bigA, smallA, bigB, smallB;
-  for (size_t i = 0; i < A.size(); ++i) (is_big(A[i]) ? bigA : smallA).push_back(int(i));
-  if (self) { bigB = bigA; smallB = smallA; }
-  else for (size_t i = 0; i < Bs.size(); ++i) (is_big(Bs[i]) ? bigB : smallB).push_back(int(i));
-  // big A against all B; small A against big B (self: each unordered pair once)
-  for (int a : bigA)
-    for (size_t b = 0; b < Bs.size(); ++b) {
-      if (self && (int(b) == a || (is_big(Bs[b]) && int(b) < a))) continue;
-      if (overlap(A[a], Bs[b])) fn(A[a].id, Bs[b].id);
+  using Grid = std::unordered_map<unsigned long long, std::vector<int>>;
+  auto build = [&](const std::vector<Box>& S, const std::vector<int>& ids) {
+    Grid g;
+    g.reserve(ids.size() * 2);
+    for (int b : ids) {
+      const Box& x = S[b];
+      for (long long i = cell(x.lo[0]); i <= cell(x.hi[0]); ++i)
+        for (long long j = cell(x.lo[1]); j <= cell(x.hi[1]); ++j)
+          for (long long k = cell(x.lo[2]); k <= cell(x.hi[2]); ++k) g[key(i, j, k)].push_back(b);
     }
-  for (int a : smallA)
-    for (int b : bigB) {
-      if (self) continue;  // covered above from the big side
-      if (overlap(A[a], Bs[b])) fn(A[a].id, Bs[b].id);
+    return g;
+  };
+  // visit(x, S, g, ids, skip, emit): every box y of S in grid g (built from
+  // `ids`) overlapping x, reported once per pair, skip(y) excluded.
+  auto visit = [&](const Box& x, const std::vector<Box>& S, const Grid& g, const std::vector<int>& ids, auto skip,
+                   auto emit) {
+    long long n = 1;
+    for (int k = 0; k < 3; ++k) n *= cell(x.hi[k]) - cell(x.lo[k]) + 1;
+    if (n > kMaxVisit) {
+      for (int b : ids)
+        if (!skip(b) && overlap(x, S[b])) emit(b);
+      return;
     }
-  // small-small through the grid
-  std::unordered_map<unsigned long long, std::vector<int>> grid;
-  grid.reserve(smallB.size() * 2);
-  for (int b : smallB) {
-    const Box& x = Bs[b];
-    for (long long i = cell(x.lo[0]); i <= cell(x.hi[0]); ++i)
-      for (long long j = cell(x.lo[1]); j <= cell(x.hi[1]); ++j)
-        for (long long k = cell(x.lo[2]); k <= cell(x.hi[2]); ++k) grid[key(i, j, k)].push_back(b);
-  }
-  for (int a : smallA) {
-    const Box& x = A[a];
     for (long long i = cell(x.lo[0]); i <= cell(x.hi[0]); ++i)
       for (long long j = cell(x.lo[1]); j <= cell(x.hi[1]); ++j)
         for (long long k = cell(x.lo[2]); k <= cell(x.hi[2]); ++k) {
-          auto it = grid.find(key(i, j, k));
-          if (it == grid.end()) continue;
+          auto it = g.find(key(i, j, k));
+          if (it == g.end()) continue;
           for (int b : it->second) {
-            if (self && b <= a) continue;
-            const Box& y = Bs[b];
+            if (skip(b)) continue;
+            const Box& y = S[b];
             if (!overlap(x, y)) continue;
             if (cell(std::max(x.lo[0], y.lo[0])) != i || cell(std::max(x.lo[1], y.lo[1])) != j ||
                 cell(std::max(x.lo[2], y.lo[2])) != k)
               continue;
-            fn(x.id, y.id);
+            emit(b);
           }
         }
+  };
+  std::vector<int> bigA, smallA, bigB, smallB;
+  for (size_t i = 0; i < A.size(); ++i) (is_big(A[i]) ? bigA : smallA).push_back(int(i));
+  if (self) { bigB = bigA; smallB = smallA; }
+  else for (size_t i = 0; i < Bs.size(); ++i) (is_big(Bs[i]) ? bigB : smallB).push_back(int(i));
+  const Grid gridB = build(Bs, smallB);
+  // small A against small B
+  for (int a : smallA)
+    visit(A[a], Bs, gridB, smallB, [&](int b) { return self && b <= a; }, [&](int b) { fn(A[a].id, Bs[b].id); });
+  // big A against small B (self: this also covers small-big)
+  for (int a : bigA)
+    visit(A[a], Bs, gridB, smallB, [&](int) { return false; }, [&](int b) { fn(A[a].id, Bs[b].id); });
+  // big A against big B
+  for (int a : bigA)
+    for (int b : bigB) {
+      if (self && b <= a) continue;
+      if (overlap(A[a], Bs[b])) fn(A[a].id, Bs[b].id);
+    }
+  // small A against big B
+  if (!self && !bigB.empty()) {
+    const Grid gridA = build(A, smallA);
+    for (int b : bigB)
+      visit(Bs[b], A, gridA, smallA, [&](int) { return false; }, [&](int a) { fn(A[a].id, Bs[b].id); });
   }
 }
 
@@ -118,6 +141,48 @@ struct Out {
     ++n;
   }
 };
+
+// get_contact_points / get_contact_points_pt (contact.py:148-161) over
+// fetch_dist_v0v1 / fetch_dist_v0v1_pt (contact.py:88-145), in precision
+// T: midpoint (v0 + v1) / 2 of the closest points (wp.lerp(a, b, t) =
+// a (1 - t) + b t; PT v1 = alpha t0 + beta t1 + (1 - alpha - beta) t2 over
+// slots 1..3), the distance, and the feature code the dispatch picks (PT:
+// closest_point_triangle's type 0..6; EE: kind * 16 + slot0 * 4 + slot1,
+// the layout of ipc_contact_points_ee; xcs NULL disables the near-parallel
+// fallback).
+template <typename T>
+static void contact_points_t(int is_ee, const float* x, const float* xcs, const int* contacts, int n, double* points,
+                             double* dists, int* features) {
+  for (int i = 0; i < n; ++i) {
+    const int* c = contacts + 4 * i;
+    V3<T> p[4];
+    for (int k = 0; k < 4; ++k) p[k] = load3<T>(x, c[k]);
+    V3<T> v0, v1;
+    T dist;
+    int feat;
+    if (is_ee) {
+      EdgeClosest<T> ec = closest_point_edge_edge(p[0], p[1], p[2], p[3], T(1.0e-6));
+      v0 = p[0] * (T(1) - ec.s) + p[1] * ec.s;
+      v1 = p[2] * (T(1) - ec.t) + p[3] * ec.t;
+      dist = ec.dist;
+      T eps_x = xcs ? ee_mollifier_threshold(load3<T>(xcs, c[0]), load3<T>(xcs, c[1]), load3<T>(xcs, c[2]),
+                                             load3<T>(xcs, c[3]))
+                    : T(0);
+      Feature f = ee_feature(p[0], p[1], p[2], p[3], eps_x);
+      feat = f.kind * 16 + f.slot[0] * 4 + (f.slot[1] < 0 ? 0 : f.slot[1]);
+    } else {
+      TriClosest<T> tc = closest_point_triangle(p[0], p[1], p[2], p[3]);
+      v0 = p[0];
+      v1 = tc.wa * p[1] + tc.wb * p[2] + (T(1) - tc.wa - tc.wb) * p[3];
+      dist = tc.dist;
+      feat = tc.type;
+    }
+    V3<T> mid = (v0 + v1) * T(0.5);
+    if (points) { points[3 * i] = mid.x; points[3 * i + 1] = mid.y; points[3 * i + 2] = mid.z; }
+    if (dists) dists[i] = dist;
+    if (features) features[i] = feat;
+  }
+}
 
 }  // namespace
 
@@ -229,6 +294,37 @@ void oracle_contact_hessian(int is_ee, int prec, int d2_policy, const float* x, 
       }
     }
   }
+}
+
+// One contact at double positions x12 (rest12 for the EE mollifier
+// threshold): energy (returned), force (-gradient, 12), 12x12 Hessian and
+// the dispatched feature (kind, 4 slots; kind -1 when inactive). prec and
+// d2_policy as in oracle_contact_hessian. Lets tests differentiate the
+// pair's energy in double.
+double oracle_contact_pair(int is_ee, int prec, int d2_policy, const double* x12, const double* rest12, float l0,
+                           double kappa, double* force, double* H, int* feature) {
+  V3<double> xd[4], rd[4];
+  for (int k = 0; k < 4; ++k) {
+    xd[k] = mk(x12[3 * k], x12[3 * k + 1], x12[3 * k + 2]);
+    rd[k] = rest12 ? mk(rest12[3 * k], rest12[3 * k + 1], rest12[3 * k + 2]) : xd[k];
+  }
+  PairResult r;
+  if (is_ee) r = prec ? contact_ee_at<double>(xd, rd, l0, kappa, d2_policy) : contact_ee_at<float>(xd, rd, l0, kappa, d2_policy);
+  else r = prec ? contact_pt_at<double>(xd, l0, kappa, d2_policy) : contact_pt_at<float>(xd, l0, kappa, d2_policy);
+  if (force) std::memcpy(force, r.grad_force, sizeof(r.grad_force));
+  if (H) std::memcpy(H, r.H, sizeof(r.H));
+  if (feature) {
+    feature[0] = r.active ? r.feature.kind : -1;
+    for (int k = 0; k < 4; ++k) feature[1 + k] = r.active ? r.feature.slot[k] : -1;
+  }
+  return r.active ? r.energy : 0.0;
+}
+
+// Midpoints, distances and feature codes of a contact list (contact_points_t).
+void oracle_contact_points(int is_ee, int use_double, const float* x, const float* xcs, const int* contacts, int n,
+                           double* points, double* dists, int* features) {
+  if (use_double) contact_points_t<double>(is_ee, x, xcs, contacts, n, points, dists, features);
+  else contact_points_t<float>(is_ee, x, xcs, contacts, n, points, dists, features);
 }
 
 // contact_energy_ee / contact_energy_pt (dynamic_contacts.py:374-398):
@@ -403,13 +499,26 @@ int64_t oracle_ccd_candidates_pt(const float* x0, const float* x1, int n_nodes, 
   return out.n;
 }
 
+// Distance of point o from the plane of (v0, v0 + e1, v0 + e2), scaled by
+// |e1 x e2|, is certainly nonzero: |n . (o - v0)| above the forward error
+// bound of its double evaluation, 8 * 2^-53 times the sum of the products'
+// magnitudes (Shewchuk's orient3d filter, rounded up).
+static bool off_plane(const V3<double>& n, const V3<double>& e1, const V3<double>& e2, const V3<double>& r) {
+  double perm = std::fabs(r.x) * (std::fabs(e1.y * e2.z) + std::fabs(e1.z * e2.y)) +
+                std::fabs(r.y) * (std::fabs(e1.z * e2.x) + std::fabs(e1.x * e2.z)) +
+                std::fabs(r.z) * (std::fabs(e1.x * e2.y) + std::fabs(e1.y * e2.x));
+  return std::fabs(dot(n, r)) > 8.0 * 0x1p-53 * perm;
+}
+
 // Edge-triangle crossings (layout of ipc_count_intersections; no reference
 // counterpart). Written independently of the GPU's signed-volume test:
 // Moller-Trumbore ray-triangle intersection in double, the segment
 // parameter in [0, 1] and the barycentrics in the closed triangle; a segment
-// parallel to the triangle's plane (|det| <= 1e-300) or sharing a vertex
-// does not count. Pairs come from the hash-grid box enumeration. Writes the
-// (edge, triangle) ids, sorted, up to cap; returns the count.
+// parallel to the triangle's plane (|det| <= 1e-300), one whose end points
+// both lie in the plane within rounding (coplanar: contact, not a
+// crossing), or one sharing a vertex does not count. Pairs come from the
+// hash-grid box enumeration. Writes the (edge, triangle) ids, sorted, up to
+// cap; returns the count.
 int64_t oracle_count_intersections(const float* x, const int* edges, int n_edges, const int* tris, int n_tris,
                                    int* ids, int64_t cap) {
   std::vector<Box> eb(n_edges), tb(n_tris);
@@ -432,6 +541,8 @@ int64_t oracle_count_intersections(const float* x, const int* edges, int n_edges
     if (a0 == t0 || a0 == t1 || a0 == t2 || a1 == t0 || a1 == t1 || a1 == t2) continue;
     V3<double> o = load3<double>(x, a0), d = load3<double>(x, a1) - o;
     V3<double> v0 = load3<double>(x, t0), e1 = load3<double>(x, t1) - v0, e2 = load3<double>(x, t2) - v0;
+    V3<double> nrm = cross(e1, e2);
+    if (!off_plane(nrm, e1, e2, o - v0) && !off_plane(nrm, e1, e2, o + d - v0)) continue;
     V3<double> pv = cross(d, e2);
     double det = dot(e1, pv);
     if (std::fabs(det) <= 1.0e-300) continue;

@@ -394,27 +394,33 @@ inline void symmetric_outer_product_psd_12(const double u[12], const double v[12
 // double (the HIP library's policy).
 struct PairResult {
   bool active;
+  Feature feature;        // the dispatched feature (valid when active)
   double grad_force[12];  // the value atomically added into `b` (force = -grad E)
   double H[144];
   double energy;
 };
 
-// contact_hessian_ee (dynamic_contacts.py:151-207) + contact_energy_ee (:374-388).
+template <typename TC> inline V3<TC> cast3(const V3<double>& v) { return V3<TC>{TC(v.x), TC(v.y), TC(v.z)}; }
+
+// contact_hessian_ee (dynamic_contacts.py:151-207) + contact_energy_ee (:374-388)
+// at positions xd (rest positions rd for the mollifier threshold). The
+// reference's buffers are fp32: a caller passing fp32 values widened to
+// double gets them back exactly in the TC = float classification.
 template <typename TC>
-PairResult contact_ee(const float* x, const float* xcs, const int c[4], float l0f, double kappa, int d2_policy) {
+PairResult contact_ee_at(const V3<double> xd[4], const V3<double> rd[4], float l0f, double kappa, int d2_policy) {
   PairResult R;
   std::memset(&R, 0, sizeof(R));
   V3<TC> xc[4];
-  V3<double> xd[4];
-  for (int k = 0; k < 4; ++k) { xc[k] = load3<TC>(x, c[k]); xd[k] = load3<double>(x, c[k]); }
+  for (int k = 0; k < 4; ++k) xc[k] = cast3<TC>(xd[k]);
   EdgeClosest<TC> ec = closest_point_edge_edge(xc[0], xc[1], xc[2], xc[3], TC(1.0e-6));
   TC l0 = TC(l0f);
   if (!(ec.dist < l0)) return R;
   R.active = true;
-  TC eps_x = ee_mollifier_threshold(load3<TC>(xcs, c[0]), load3<TC>(xcs, c[1]), load3<TC>(xcs, c[2]), load3<TC>(xcs, c[3]));
+  TC eps_x = ee_mollifier_threshold(cast3<TC>(rd[0]), cast3<TC>(rd[1]), cast3<TC>(rd[2]), cast3<TC>(rd[3]));
   V3<TC> cr = cross(xc[1] - xc[0], xc[3] - xc[2]);
   bool near_parallel = eps_x > TC(0) && dot(cr, cr) < eps_x;
   Feature f = ee_feature(xc[0], xc[1], xc[2], xc[3], eps_x);
+  R.feature = f;
   double g[12], Hn[144];
   double d2m = feature_grad_nsd(f, xd, g, Hn);
   double d2 = d2_policy == 0 ? double(ec.dist * ec.dist) : d2m;
@@ -437,22 +443,29 @@ PairResult contact_ee(const float* x, const float* xcs, const int c[4], float l0
   return R;
 }
 
+template <typename TC>
+PairResult contact_ee(const float* x, const float* xcs, const int c[4], float l0f, double kappa, int d2_policy) {
+  V3<double> xd[4], rd[4];
+  for (int k = 0; k < 4; ++k) { xd[k] = load3<double>(x, c[k]); rd[k] = load3<double>(xcs, c[k]); }
+  return contact_ee_at<TC>(xd, rd, l0f, kappa, d2_policy);
+}
+
 // contact_hessian_pt (dynamic_contacts.py:209-246) + contact_energy_pt (:390-398).
 // Slot order is the contact record (p, t1, t0, t2) written by
 // point_triangle_collision (contact.py:336); fetch_dist_v0v1_pt
 // (contact.py:116-145) treats slots 1..3 as the triangle (a, b, c).
 template <typename TC>
-PairResult contact_pt(const float* x, const int c[4], float l0f, double kappa, int d2_policy) {
+PairResult contact_pt_at(const V3<double> xd[4], float l0f, double kappa, int d2_policy) {
   PairResult R;
   std::memset(&R, 0, sizeof(R));
   V3<TC> xc[4];
-  V3<double> xd[4];
-  for (int k = 0; k < 4; ++k) { xc[k] = load3<TC>(x, c[k]); xd[k] = load3<double>(x, c[k]); }
+  for (int k = 0; k < 4; ++k) xc[k] = cast3<TC>(xd[k]);
   TriClosest<TC> tc = closest_point_triangle(xc[0], xc[1], xc[2], xc[3]);
   TC l0 = TC(l0f);
   if (!(tc.dist < l0)) return R;
   R.active = true;
   Feature f = pt_feature_from_type(tc.type);
+  R.feature = f;
   double g[12], Hn[144];
   double d2m = feature_grad_nsd(f, xd, g, Hn);
   double d2 = d2_policy == 0 ? double(tc.dist * tc.dist) : d2m;
@@ -462,6 +475,13 @@ PairResult contact_pt(const float* x, const int c[4], float l0f, double kappa, i
   for (int i = 0; i < 12; ++i) R.grad_force[i] = -B.db * g[i];
   R.energy = B.b;
   return R;
+}
+
+template <typename TC>
+PairResult contact_pt(const float* x, const int c[4], float l0f, double kappa, int d2_policy) {
+  V3<double> xd[4];
+  for (int k = 0; k < 4; ++k) xd[k] = load3<double>(x, c[k]);
+  return contact_pt_at<TC>(xd, l0f, kappa, d2_policy);
 }
 
 // --------------------------------------------------------------- CCD ----
