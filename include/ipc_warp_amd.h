@@ -279,6 +279,17 @@ int ipc_ctx_kernel_times(ipc_ctx* ctx, double* total_ms, long long* calls);
  * a backward pass; with work = NULL, scratch (float [3 * 256 * n]) holds the
  * activations. */
 int ipc_mlp_forward(const float* params, int n, const float* x, float* y, float* work, float* scratch, void* stream);
+/* Inference that allocates nothing: ipc_mlp_forward with work = NULL, its
+ * activations held in a device array inside the library (3 * 256 *
+ * IPC_MLP_NOALLOC_MAX_ROWS floats, 12 MiB, one per GPU the library runs on,
+ * kept until the process ends) instead of a scratch argument. No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops). n must not exceed
+ * IPC_MLP_NOALLOC_MAX_ROWS (IPC_EINVAL otherwise). Every call on one GPU
+ * shares that array, so calls that overlap on one GPU (other streams,
+ * other threads) race; calls on one stream are ordered and safe. */
+#define IPC_MLP_NOALLOC_MAX_ROWS 4096
+int ipc_mlp_forward_noalloc(const float* params, int n, const float* x, float* y, void* stream);
 /* Backward pass of the forward pass that filled `work`: ADDS the gradient of
  * the loss with respect to the parameters into grad (IPC_MLP_PARAMS floats)
  * given dy = dLoss/dy; scratch float [2 * 256 * n]. */

@@ -39,6 +39,7 @@ _SIGNATURES = {
     "ipc_ctx_set_timing": ([_vp, _c_int], _c_int),
     "ipc_ctx_kernel_times": ([_vp, ctypes.POINTER(_c_double), ctypes.POINTER(ctypes.c_longlong)], _c_int),
     "ipc_mlp_forward": ([_vp, _c_int, _vp, _vp, _vp, _vp, _vp], _c_int),
+    "ipc_mlp_forward_noalloc": ([_vp, _c_int, _vp, _vp, _vp], _c_int),
     "ipc_mlp_backward": ([_vp, _c_int, _vp, _vp, _vp, _vp, _vp, _vp], _c_int),
     "ipc_mlp_mse": ([_vp, _vp, _vp, _c_int, _c_float, _vp, _vp, _vp], _c_int),
     "ipc_mlp_adam": ([_vp, _vp, _vp, _vp, ctypes.c_longlong, _c_float, _c_float, _c_float, _c_float, _c_int, _vp],

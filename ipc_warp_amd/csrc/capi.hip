@@ -172,6 +172,15 @@ int ipc_mlp_forward(const float* params, int n, const float* x, float* y, float*
                     "mlp_forward");
 }
 
+int ipc_mlp_forward_noalloc(const float* params, int n, const float* x, float* y, void* stream) {
+  if (n < 0) return fail(IPC_EINVAL, "mlp_forward_noalloc: negative n");
+  if (n > IPC_MLP_NOALLOC_MAX_ROWS) return fail(IPC_EINVAL, "mlp_forward_noalloc: n exceeds IPC_MLP_NOALLOC_MAX_ROWS");
+  if (n == 0) return IPC_OK;
+  if (!params || !x || !y) return fail(IPC_EINVAL, "mlp_forward_noalloc: null pointer");
+  return hip_status(ipcd::mlp_forward_noalloc(params, n, x, y, static_cast<hipStream_t>(stream)),
+                    "mlp_forward_noalloc");
+}
+
 int ipc_mlp_backward(const float* params, int n, const float* x, const float* work, const float* dy, float* grad,
                      float* scratch, void* stream) {
   if (n < 0) return fail(IPC_EINVAL, "mlp_backward: negative n");

@@ -34,7 +34,8 @@ hipError_t launch_record_intersections(bool ee, const float* x0, const float* x1
                                       double* alpha_out, int cap_out, int* count_out, unsigned long long* toi_min,
                                       hipStream_t s);
 
-hipError_t mlp_forward(const float* prm, int n, const float* x, float* y, float* work, float* scratch, hipStream_t s);
+hipError_t mlp_forward(const float* prm, int n, const float* x, float* y, float* work, float* tmp, hipStream_t s);
+hipError_t mlp_forward_noalloc(const float* prm, int n, const float* x, float* y, hipStream_t s);
 hipError_t mlp_backward(const float* prm, int n, const float* x, const float* work, const float* dy, float* grad,
                         float* scratch, hipStream_t s);
 hipError_t mlp_mse(const float* y, const float* t, const float* w, int n, float wsum, float* dy, double* loss,

@@ -8,9 +8,17 @@
 // Activations of a forward pass with backward in mind ("work") hold
 // Z1, H1, Z2, H2, Z3, H3, each [n][256] (pre-activation and GELU output).
 #include <cmath>
+#include <mutex>
 
 #include "../../include/ipc_warp_amd.h"
 #include "ipc_kernels.hpp"
+
+// Activations of ipc_mlp_forward_noalloc: a file-scope device array (one
+// copy per GPU, held until the process ends) instead of a caller or library
+// buffer, so that inference allocates nothing. Host code must use the
+// address hipGetSymbolAddress returns on each device (scratch_on_device),
+// never this name: on the host it is a stand-in, not a device address.
+static __device__ __align__(256) float scratch[3 * IPC_MLP_HIDDEN * IPC_MLP_NOALLOC_MAX_ROWS];
 
 namespace ipcd {
 
@@ -195,16 +203,16 @@ static hipError_t gemm(int R, int Cn, int K, const float* A, int lda, const floa
     if (_e != hipSuccess) return _e; \
   } while (0)
 
-hipError_t mlp_forward(const float* prm, int n, const float* x, float* y, float* work, float* scratch, hipStream_t s) {
-  // Without a work buffer (inference) the activations ping-pong in scratch.
+hipError_t mlp_forward(const float* prm, int n, const float* x, float* y, float* work, float* tmp, hipStream_t s) {
+  // Without a work buffer (inference) the activations ping-pong in tmp.
   long long L = (long long)n * kHid;
   float *Z1, *H1, *Z2, *H2, *Z3, *H3;
   if (work) {
     Z1 = work; H1 = work + L; Z2 = work + 2 * L; H2 = work + 3 * L; Z3 = work + 4 * L; H3 = work + 5 * L;
   } else {
-    Z1 = Z2 = Z3 = scratch;
-    H1 = H3 = scratch + L;
-    H2 = scratch + 2 * L;
+    Z1 = Z2 = Z3 = tmp;
+    H1 = H3 = tmp + L;
+    H2 = tmp + 2 * L;
   }
   IPC_TRY((gemm<false, true, EPI_BIAS_GELU>(n, kHid, kIn, x, kIn, prm + kOffW1, kIn, H1, kHid, prm + kOffB1, Z1, s)));
   IPC_TRY((gemm<false, true, EPI_BIAS_GELU>(n, kHid, kHid, H1, kHid, prm + kOffW2, kHid, H2, kHid, prm + kOffB2, Z2, s)));
@@ -213,11 +221,11 @@ hipError_t mlp_forward(const float* prm, int n, const float* x, float* y, float*
 }
 
 hipError_t mlp_backward(const float* prm, int n, const float* x, const float* work, const float* dy, float* grad,
-                        float* scratch, hipStream_t s) {
+                        float* tmp, hipStream_t s) {
   long long L = (long long)n * kHid;
   const float *Z1 = work, *H1 = work + L, *Z2 = work + 2 * L, *H2 = work + 3 * L, *Z3 = work + 4 * L,
               *H3 = work + 5 * L;
-  float *dA = scratch, *dB = scratch + L;
+  float *dA = tmp, *dB = tmp + L;
   int splits = n / 2048 + 1;
   if (splits > 64) splits = 64;
   // layer 4: dW4 += dy^T H3, db4 += sum dy, dZ3 = (dy W4) * gelu'(Z3)
@@ -257,6 +265,37 @@ hipError_t mlp_adam(float* p, const float* g, float* m, float* v, long long n, f
   float c1 = 1.0f - powf(b1, float(step)), c2 = 1.0f - powf(b2, float(step));
   hipLaunchKernelGGL(adam_kernel, dim3(gblocks(n, 256)), dim3(256), 0, s, n, p, g, m, v, lr, b1, b2, eps, c1, c2);
   return hipGetLastError();
+}
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept; nullptr with *err set when the
+// lookup fails.
+static float* scratch_on_device(hipError_t* err) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static float* addr[kMaxDevices] = {};
+  int dev = 0;
+  *err = hipGetDevice(&dev);
+  if (*err != hipSuccess) return nullptr;
+  if (dev < 0 || dev >= kMaxDevices) {
+    *err = hipErrorInvalidDevice;
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    *err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (*err != hipSuccess) return nullptr;
+    addr[dev] = static_cast<float*>(p);
+  }
+  return addr[dev];
+}
+
+hipError_t mlp_forward_noalloc(const float* prm, int n, const float* x, float* y, hipStream_t s) {
+  hipError_t err = hipSuccess;
+  float* tmp = scratch_on_device(&err);
+  if (!tmp) return err;
+  return mlp_forward(prm, n, x, y, nullptr, tmp, s);
 }
 
 hipError_t zero_f32(float* p, long long n, hipStream_t s) {
