@@ -41,7 +41,10 @@ __device__ __forceinline__ bool pair_ee(const float* __restrict__ x, const float
   V3<double> ga = mk(0.0, 0.0, 0.0), gb = mk(0.0, 0.0, 0.0);
   if (near_parallel) {
     // IPC edge-edge mollifier, m(c) = (c/eps_x)(2 - c/eps_x), c = |ea x eb|^2.
-    V3<double> ea = xr[1], eb = xr[3] - xr[2];
+    // c is a cancellation of nearly parallel edges: eb is formed from the
+    // absolute coordinates (one rounding of x3 - x2), not as the difference
+    // of the two offsets from slot 0, which round at the scale of |x - x0|.
+    V3<double> ea = xr[1], eb = to_d(xf[3]) - to_d(xf[2]);
     V3<double> n = cross(ea, eb);
     double ex = double(eps_x);
     double q = dot(n, n) / ex;
