@@ -493,9 +493,9 @@ def main() -> int:
     }
     del S
 
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     if not args.no_extras:
         from ipc_warp_amd import scenes
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         # the barrier Hessian kernel on a large synthetic contact stream (2^22 PT contacts)
         n_big = 1 << 22
         xs, cs = scenes.pair_stream("pt", n_big, seed=args.seed)
@@ -574,20 +574,20 @@ def main() -> int:
             **frozen_step(sg, dxg, qg.reshape(-1, 3), dqg.reshape(-1, 3), True, GRID_THICKNESS, GRID_KAPPA, 100, 10,
                           dev))
 
-        # STREAM-style copy of 256 MiB: the measured HBM ceiling, and the
-        # calibration kernel for the FETCH_SIZE / WRITE_SIZE counters.
-        copy_src = torch.empty(64 << 20, dtype=torch.float32, device=dev)
-        copy_dst = torch.empty_like(copy_src)
-        for _ in range(3):
-            copy_dst.copy_(copy_src)
-        e0.record()
-        for _ in range(10):
-            copy_dst.copy_(copy_src)
-        e1.record()
-        torch.cuda.synchronize()
-        copy_ms = e0.elapsed_time(e1) / 10
-        result["hbm_copy_gbs"] = 2 * copy_src.numel() * 4 / (copy_ms * 1e-3) / 1e9
-        del copy_src, copy_dst
+    # STREAM-style copy of 256 MiB: the measured HBM ceiling, and the
+    # calibration kernel for the FETCH_SIZE / WRITE_SIZE counters.
+    copy_src = torch.empty(64 << 20, dtype=torch.float32, device=dev)
+    copy_dst = torch.empty_like(copy_src)
+    for _ in range(3):
+        copy_dst.copy_(copy_src)
+    e0.record()
+    for _ in range(10):
+        copy_dst.copy_(copy_src)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_ms = e0.elapsed_time(e1) / 10
+    result["hbm_copy_gbs"] = 2 * copy_src.numel() * 4 / (copy_ms * 1e-3) / 1e9
+    del copy_src, copy_dst
 
     if not args.no_newton:
         result["newton"] = newton_comparison(args.newton_seeds, args.newton_steps, dev)
