@@ -143,23 +143,32 @@ struct HessianArgs {
   int lpw;
 };
 
-// A wave's forces on their way to b: 12 per contact slot, and the slot's 4
-// vertex ids.
-struct ForceStage {
+// Per wave in LDS: the forces on their way to b (12 per contact slot), the
+// slots' 4 vertex ids and active flags, and one block row (4 blocks, 9
+// float4) of every slot's triplet values on their way to memory.
+struct HessianStage {
   float f[4][64 * 12];
   int v[4][64 * 4];
+  int act[4][64];
+  float4 row[4][64 * 9];
 };
 
+// The wave's triplets are one contiguous range of memory (16 blocks per
+// contact, its contacts consecutive). A lane computes its own contact, but
+// the stores are issued for consecutive addresses across the wave: a lane
+// writing its own contact's 576 contiguous bytes made every store
+// instruction touch 64 cache lines with 16 bytes each, and those stores
+// took about half of the kernel's time. Row and column indices are stored
+// from the staged vertex ids; the values go block row by block row through
+// LDS (each lane expands its contact's row q, then the wave stores the 64
+// contacts' rows q, 144 contiguous bytes per contact).
 template <bool EE>
-__device__ __forceinline__ void hessian_block(const HessianArgs& A, int block, ForceStage& fs) {
+__device__ __forceinline__ void hessian_block(const HessianArgs& A, int block, HessianStage& st) {
   const float* __restrict__ x = A.x;
   const float* __restrict__ xcs = A.xcs;
   const int nv = A.nv, n = A.n, offset = A.offset, lpw = A.lpw;
   const float l0 = A.l0;
   const double kappa = A.kappa;
-  int* __restrict__ rows = A.rows;
-  int* __restrict__ cols = A.cols;
-  float* __restrict__ vals = A.vals;
   float* __restrict__ b = A.b;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i = contact_slot(lpw, block);
@@ -167,53 +176,21 @@ __device__ __forceinline__ void hessian_block(const HessianArgs& A, int block, F
 #pragma unroll
   for (int k = 0; k < 12; ++k) force[k] = 0.0f;
   int4 c = make_int4(0, 0, 0, 0);
+  CompressedHessian C;
+  bool active = false;
   if (i >= 0 && i < n) {
     c = A.contacts[i];
-    CompressedHessian C;
-    const bool active = valid4(c, nv) && (EE ? pair_ee(x, xcs, c, l0, kappa, true, C, force, nullptr)
-                                             : pair_pt(x, c, l0, kappa, true, C, force, nullptr));
-    const long long base = (long long)(offset + i) * 16;
-    int4* rows4 = reinterpret_cast<int4*>(rows + base);
-    int4* cols4 = reinterpret_cast<int4*>(cols + base);
-    float4* v4 = reinterpret_cast<float4*>(vals + base * 9);
-    if (!active) {
-      const int4 z = make_int4(0, 0, 0, 0);
-#pragma unroll
-      for (int ii = 0; ii < 4; ++ii) { rows4[ii] = z; cols4[ii] = z; }
-      const float4 zf = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-      for (int k = 0; k < 36; ++k) v4[k] = zf;
-#pragma unroll
-      for (int k = 0; k < 12; ++k) force[k] = 0.0f;
-    } else {
-      const int cc[4] = {c.x, c.y, c.z, c.w};
-#pragma unroll
-      for (int ii = 0; ii < 4; ++ii) {
-        rows4[ii] = make_int4(cc[ii], cc[ii], cc[ii], cc[ii]);
-        cols4[ii] = c;
-      }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        // Four consecutive 3x3 blocks = 36 floats = 9 float4 stores.
-        float blk[36];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) expand_block(C, (q * 4 + r) >> 2, (q * 4 + r) & 3, blk + 9 * r);
-#pragma unroll
-        for (int k = 0; k < 9; ++k)
-          v4[q * 9 + k] = make_float4(blk[4 * k], blk[4 * k + 1], blk[4 * k + 2], blk[4 * k + 3]);
-      }
-    }
+    active = valid4(c, nv) && (EE ? pair_ee(x, xcs, c, l0, kappa, true, C, force, nullptr)
+                                  : pair_pt(x, c, l0, kappa, true, C, force, nullptr));
   }
-  // Forces into b. Device-scope float atomics are the kernel's largest cost,
-  // and one atomic instruction costs a memory transaction per cache line its
-  // lanes touch. The wave's 12 lpw force components (contact slot j / 12,
-  // component j % 12) go through LDS and are added transposed, value j by
-  // lane j % 64: an instruction then adds consecutive components of
-  // consecutive contacts, a few lines when their vertices are adjacent
-  // instead of one line per lane. A zero force (a vertex with zero weight in
-  // the contact's feature, or no active contact) is not added.
-  float* sf = fs.f[wave];
-  int* sv = fs.v[wave];
+  if (!active) {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) force[k] = 0.0f;
+  }
+  float* sf = st.f[wave];
+  int* sv = st.v[wave];
+  int* sa = st.act[wave];
+  float4* sr = st.row[wave];
   if (lane < lpw) {
 #pragma unroll
     for (int k = 0; k < 12; ++k) sf[lane * 12 + k] = force[k];
@@ -221,8 +198,62 @@ __device__ __forceinline__ void hessian_block(const HessianArgs& A, int block, F
     sv[lane * 4 + 1] = c.y;
     sv[lane * 4 + 2] = c.z;
     sv[lane * 4 + 3] = c.w;
+    sa[lane] = active ? 1 : 0;
   }
-  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  // the wave's contacts [first, first + nw) own triplets [offset + first, ...) * 16
+  const int first = int((block * blockDim.x + threadIdx.x) >> 6) * lpw;
+  const int nw = first < n ? min(lpw, n - first) : 0;
+  const long long tbase = (long long)(offset + first) * 16;
+  // rows and columns: 4 int4 per contact each (block ii * 4 + jj: row c[ii], column c[jj])
+  {
+    int4* rows4 = reinterpret_cast<int4*>(A.rows + tbase);
+    int4* cols4 = reinterpret_cast<int4*>(A.cols + tbase);
+    for (int t = lane; t < nw * 4; t += 64) {
+      const int s = t >> 2, ii = t & 3;
+      int4 rv = make_int4(0, 0, 0, 0), cv = make_int4(0, 0, 0, 0);
+      if (sa[s]) {
+        const int vi = sv[s * 4 + ii];
+        rv = make_int4(vi, vi, vi, vi);
+        cv = make_int4(sv[s * 4 + 0], sv[s * 4 + 1], sv[s * 4 + 2], sv[s * 4 + 3]);
+      }
+      rows4[t] = rv;
+      cols4[t] = cv;
+    }
+  }
+  // values, block row by block row
+  float4* v4 = reinterpret_cast<float4*>(A.vals + tbase * 9);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (lane < lpw) {
+      // Four consecutive 3x3 blocks = 36 floats = 9 float4.
+      float blk[36];
+      if (active) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) expand_block(C, q, r, blk + 9 * r);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 36; ++k) blk[k] = 0.0f;
+      }
+#pragma unroll
+      for (int k = 0; k < 9; ++k)
+        sr[lane * 9 + k] = make_float4(blk[4 * k], blk[4 * k + 1], blk[4 * k + 2], blk[4 * k + 3]);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    // slot s's row q: float4 s * 36 + q * 9 + k of the wave's range
+    for (int t = lane; t < nw * 9; t += 64) {
+      const int s = t / 9, k = t - 9 * s;
+      v4[s * 36 + q * 9 + k] = sr[t];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  }
+  // Forces into b. Device-scope float atomics cost a memory transaction per
+  // cache line an atomic instruction touches. The wave's 12 lpw force
+  // components (contact slot j / 12, component j % 12) are added transposed
+  // from LDS, value j by lane j % 64: an instruction then adds consecutive
+  // components of consecutive contacts, a few lines when their vertices are
+  // adjacent instead of one line per lane. A zero force (a vertex with zero
+  // weight in the contact's feature, or no active contact) is not added.
   for (int j = lane; j < 12 * lpw; j += 64) {
     const float v = sf[j];
     if (v != 0.0f) {
@@ -234,16 +265,16 @@ __device__ __forceinline__ void hessian_block(const HessianArgs& A, int block, F
 
 template <bool EE>
 __global__ void __launch_bounds__(256) contact_hessian_kernel(HessianArgs A) {
-  __shared__ ForceStage fs;
-  hessian_block<EE>(A, blockIdx.x, fs);
+  __shared__ HessianStage st;
+  hessian_block<EE>(A, blockIdx.x, st);
 }
 
 // Both halves of the contact Hessian in one launch: workgroups [0, blocks_e)
 // take the EE list, the rest the PT list.
 __global__ void __launch_bounds__(256) contact_hessian_pair_kernel(HessianArgs E, HessianArgs P, int blocks_e) {
-  __shared__ ForceStage fs;
-  if (int(blockIdx.x) < blocks_e) hessian_block<true>(E, blockIdx.x, fs);
-  else hessian_block<false>(P, blockIdx.x - blocks_e, fs);
+  __shared__ HessianStage st;
+  if (int(blockIdx.x) < blocks_e) hessian_block<true>(E, blockIdx.x, st);
+  else hessian_block<false>(P, blockIdx.x - blocks_e, st);
 }
 
 // ------------------------------------------------------------ energy kernel ----

@@ -294,64 +294,70 @@ IPCD_FN double inv_sqrt(double x) {
   return 1.0 / sqrt(x);
 #endif
 }
+IPCD_FN float inv_sqrt(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return rsqrtf(x);
+#else
+  return 1.0f / sqrtf(x);
+#endif
+}
 
 // Index of (i, j) in the packed upper triangle of a symmetric n x n matrix.
 template <int n> IPCD_FN constexpr int tri_idx(int i, int j) {
   return i <= j ? i * n - i * (i - 1) / 2 + (j - i) : j * n - j * (j - 1) / 2 + (i - j);
 }
 
-// Cyclic Jacobi on a symmetric n x n (row-major), double. Returns
+// Cyclic Jacobi on a symmetric n x n (row-major) in precision T. Returns
 // eigenvalues in ev and eigenvectors in the columns of V. Sweeps stop once
-// the off-diagonal mass is below 1e-24 of the diagonal's (eigenvalues then
-// carry relative errors far below the 1e-5 the Hessians are held to). The
-// matrix is kept as its packed upper triangle and each rotation uses the
-// symmetric update (a_pp -= t a_pq, a_qq += t a_pq, a_pq = 0, the other rows
-// rotated); a rotation with a_pq = 0 runs with t = 0, the identity, so that
-// the unrolled sweep has no branches (and no duplicated live matrices).
-template <int n>
-IPCD_FN void jacobi_sym(const double A[n * n], double ev[n], double V[n * n]) {
+// the off-diagonal mass is below tol2 of the diagonal's. The matrix is kept
+// as its packed upper triangle and each rotation uses the symmetric update
+// (a_pp -= t a_pq, a_qq += t a_pq, a_pq = 0, the other rows rotated); a
+// rotation with a_pq = 0 runs with t = 0, the identity, so that the
+// unrolled sweep has no branches (and no duplicated live matrices).
+template <int n, typename T>
+IPCD_FN void jacobi_sym(const T A[n * n], T ev[n], T V[n * n], T tol2) {
 #pragma clang fp contract(fast)
-  double a[n * (n + 1) / 2];
+  T a[n * (n + 1) / 2];
 #pragma unroll
   for (int i = 0; i < n; ++i)
 #pragma unroll
     for (int j = i; j < n; ++j) a[tri_idx<n>(i, j)] = A[i * n + j];
 #pragma unroll
-  for (int i = 0; i < n * n; ++i) V[i] = (i % (n + 1) == 0) ? 1.0 : 0.0;
+  for (int i = 0; i < n * n; ++i) V[i] = (i % (n + 1) == 0) ? T(1) : T(0);
   for (int sweep = 0; sweep < 12; ++sweep) {
-    double off = 0.0, diag = 0.0;
+    T off = T(0), diag = T(0);
 #pragma unroll
     for (int i = 0; i < n; ++i) {
       diag += a[tri_idx<n>(i, i)] * a[tri_idx<n>(i, i)];
 #pragma unroll
       for (int j = i + 1; j < n; ++j) off += a[tri_idx<n>(i, j)] * a[tri_idx<n>(i, j)];
     }
-    if (!(off > 1.0e-24 * diag)) break;
+    if (!(off > tol2 * diag)) break;
 #pragma unroll
     for (int p = 0; p < n - 1; ++p) {
 #pragma unroll
       for (int q = p + 1; q < n; ++q) {
-        const double apq = a[tri_idx<n>(p, q)];
+        const T apq = a[tri_idx<n>(p, q)];
         // tan of the rotation angle, t = sgn(d) e / (|d| + sqrt(d^2 + e^2))
         // with d = a_qq - a_pp, e = 2 a_pq (the textbook theta = d / e form
         // with its two divisions folded into one)
-        const double d = a[tri_idx<n>(q, q)] - a[tri_idx<n>(p, p)], e = 2.0 * apq;
-        double t = (d >= 0.0 ? e : -e) / (fabs(d) + sqrt(d * d + e * e));
-        t = apq != 0.0 ? t : 0.0;
-        const double c = inv_sqrt(t * t + 1.0), s = t * c;
+        const T d = a[tri_idx<n>(q, q)] - a[tri_idx<n>(p, p)], e = T(2) * apq;
+        T t = (d >= T(0) ? e : -e) / (fabs(d) + sqrt(d * d + e * e));
+        t = apq != T(0) ? t : T(0);
+        const T c = inv_sqrt(t * t + T(1)), s = t * c;
         a[tri_idx<n>(p, p)] -= t * apq;
         a[tri_idx<n>(q, q)] += t * apq;
-        a[tri_idx<n>(p, q)] = 0.0;
+        a[tri_idx<n>(p, q)] = T(0);
 #pragma unroll
         for (int r = 0; r < n; ++r) {
           if (r == p || r == q) continue;
-          const double arp = a[tri_idx<n>(r, p)], arq = a[tri_idx<n>(r, q)];
+          const T arp = a[tri_idx<n>(r, p)], arq = a[tri_idx<n>(r, q)];
           a[tri_idx<n>(r, p)] = c * arp - s * arq;
           a[tri_idx<n>(r, q)] = s * arp + c * arq;
         }
 #pragma unroll
         for (int k = 0; k < n; ++k) {
-          const double vkp = V[k * n + p], vkq = V[k * n + q];
+          const T vkp = V[k * n + p], vkq = V[k * n + q];
           V[k * n + p] = c * vkp - s * vkq;
           V[k * n + q] = s * vkp + c * vkq;
         }
@@ -501,8 +507,21 @@ IPCD_FN double nsd_compressed(const FeatureGeom& g, bool want_nsd, CompressedHes
       K[a * 4 + b] = v;
     }
   }
-  double ev[4], V[16];
-  jacobi_sym<4>(K, ev, V);
+  // The 4 x 4 eigenproblem runs in fp32 (native square roots and
+  // reciprocals; in FP64 their multi-instruction sequences were most of the
+  // kernel's VALU work): its projection's errors stay near fp32 rounding of
+  // the largest entry, below the 1e-5 bound of the blocks, while every input
+  // of K (feature geometry, distances) stays FP64. K is scaled by its
+  // largest diagonal magnitude first so that fp32's range cannot matter.
+  double kmax = fmax(fmax(fabs(K[0]), fabs(K[5])), fmax(fabs(K[10]), fabs(K[15])));
+  const double ks = kmax > 0.0 ? 1.0 / kmax : 1.0;
+  float Kf[16], evf[4], V[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) Kf[i] = float(K[i] * ks);
+  jacobi_sym<4, float>(Kf, evf, V, 1.0e-14f);
+  double ev[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) ev[i] = double(evf[i]) * kmax;
   // N = Q N4 Q^T with N4 = sum over negative eigenvalues of lambda v v^T,
   // accumulated straight into the compressed fields: for an eigenvector v,
   // y = Q v (y0..2 = v0 q1 + v1 q2, y3 = v2, y4 = v3) and zeta_i = Z_i.(y3, y4);
