@@ -402,38 +402,98 @@ __device__ __forceinline__ double wave_min(double v) {
 // pt_collision_time / ee_collision_time (ccd/ccd.py:304-357) for a list of
 // candidate quadruples, in FP64. The number of candidates is n, or
 // min(*n_dev, n) when n_dev is given (a count produced on the device; n is
-// then the list's capacity), walked with a grid-stride loop. Writes
-// per-candidate TOI (optional) and folds TOI < 1 into *toi_min with one
-// 64-bit atomicMin per wave (TOI >= 0, so the IEEE bit pattern orders like
-// an unsigned integer).
-// Workgroup `bid` of `nblk` walking one candidate list.
+// then the list's capacity), walked by each wave 64 candidates at a time.
+// Writes per-candidate TOI (optional) and folds TOI < 1 into *toi_min with
+// one 64-bit atomicMin per wave (TOI >= 0, so the IEEE bit pattern orders
+// like an unsigned integer).
+//
+// Two phases with compaction: most candidates' coplanarity cubic has no
+// bracketed root in [0, 1] and their TOI is 1 after the coefficients and
+// the case analysis. The others need the iterative root search, about 30
+// rounds of FP64 Newton / bisection, which a wave ran for its slowest lane
+// with the other lanes idle. Here a workgroup first classifies a range of
+// candidates, 256 at a time, and queues the indices of those with a
+// bracketed root in LDS (ballot and prefix count per wave, one LDS atomic per
+// wave); when the queue holds kToiDense or more, or at the end, its four
+// waves run the full collision_time on the queued candidates, 64 at a time,
+// one per lane. The second phase recomputes a candidate from its positions
+// with the single-pass code, so the values are the single pass's.
+constexpr int kToiBlock = 256;
+constexpr int kToiQueue = 1024;
+constexpr int kToiDense = kToiQueue - kToiBlock;  // run the queue before it could overflow
+struct ToiQueue {
+  int idx[kToiQueue];
+  int n;
+};
+
+template <bool EE>
+__device__ __forceinline__ double toi_of(const float* __restrict__ x0, const float* __restrict__ x1, int nv, int4 q,
+                                         bool full) {
+  if (!valid4(q, nv)) return 1.0;
+  const int id[4] = {q.x, q.y, q.z, q.w};
+  V3<double> p0[4], p1[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) { p0[k] = to_d(ldx(x0, id[k])); p1[k] = to_d(ldx(x1, id[k])); }
+  if (!full) return collision_time_may_hit(p0, p1) ? -1.0 : 1.0;
+  return collision_time<EE, double>(p0, p1);
+}
+
+// Root isolation of the queued candidates by the whole workgroup; empties
+// the queue. Called by every thread of the workgroup.
+template <bool EE>
+__device__ __forceinline__ void toi_drain(const float* __restrict__ x0, const float* __restrict__ x1, int nv,
+                                          const int4* __restrict__ quads, double* __restrict__ toi_out,
+                                          ToiQueue& tq, double& tmin) {
+  __syncthreads();
+  const int qn = tq.n;
+  for (int k = threadIdx.x; k < qn; k += kToiBlock) {
+    const int j = tq.idx[k];
+    const double t = toi_of<EE>(x0, x1, nv, quads[j], true);
+    if (toi_out) toi_out[j] = t;
+    tmin = fmin(tmin, t);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) tq.n = 0;
+  __syncthreads();
+}
+
+// Workgroup `bid` of `nblk` walking one candidate list (every thread of the
+// workgroup takes the same number of trips).
 template <bool EE>
 __device__ __forceinline__ void collision_time_block(const float* __restrict__ x0, const float* __restrict__ x1, int nv,
                                                      const int4* __restrict__ quads, int n,
                                                      const int* __restrict__ n_dev, double* __restrict__ toi_out,
-                                                     unsigned long long* __restrict__ toi_min, int bid, int nblk) {
+                                                     unsigned long long* __restrict__ toi_min, int bid, int nblk,
+                                                     ToiQueue& tq) {
   int count = n;
   if (n_dev) {
     int d = *n_dev;
     count = d < n ? d : n;
   }
+  const int lane = threadIdx.x & 63;
+  if (threadIdx.x == 0) tq.n = 0;
+  __syncthreads();
   double tmin = 1.0;
-  for (int i = bid * blockDim.x + threadIdx.x; i < count; i += nblk * blockDim.x) {
-    int4 q = quads[i];
-    double t = 1.0;
-    if (valid4(q, nv)) {
-      const int id[4] = {q.x, q.y, q.z, q.w};
-      V3<double> p0[4], p1[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) { p0[k] = to_d(ldx(x0, id[k])); p1[k] = to_d(ldx(x1, id[k])); }
-      t = collision_time<EE, double>(p0, p1);
+  const int stride = nblk * kToiBlock;
+  for (int i0 = bid * kToiBlock; i0 < count; i0 += stride) {
+    const int i = i0 + threadIdx.x;
+    bool need = false;
+    if (i < count) {
+      need = toi_of<EE>(x0, x1, nv, quads[i], false) < 0.0;
+      if (!need && toi_out) toi_out[i] = 1.0;
     }
-    if (toi_out) toi_out[i] = t;
-    tmin = fmin(tmin, t);
+    const unsigned long long m = __ballot(need);
+    int base = 0;
+    if (lane == 0 && m) base = atomicAdd(&tq.n, __popcll(m));
+    base = __shfl(base, 0, 64);
+    if (need) tq.idx[base + __popcll(m & ((1ull << lane) - 1ull))] = i;
+    __syncthreads();
+    if (tq.n >= kToiDense) toi_drain<EE>(x0, x1, nv, quads, toi_out, tq, tmin);
   }
+  toi_drain<EE>(x0, x1, nv, quads, toi_out, tq, tmin);
   if (toi_min) {
     double m = wave_min(tmin);
-    if ((threadIdx.x & 63) == 0 && m < 1.0) atomicMin(toi_min, (unsigned long long)__double_as_longlong(m));
+    if (lane == 0 && m < 1.0) atomicMin(toi_min, (unsigned long long)__double_as_longlong(m));
   }
 }
 
@@ -443,7 +503,8 @@ __global__ void __launch_bounds__(256) collision_time_kernel(const float* __rest
                                                              const int* __restrict__ n_dev,
                                                              double* __restrict__ toi_out,
                                                              unsigned long long* __restrict__ toi_min) {
-  collision_time_block<EE>(x0, x1, nv, quads, n, n_dev, toi_out, toi_min, blockIdx.x, gridDim.x);
+  __shared__ ToiQueue tq;
+  collision_time_block<EE>(x0, x1, nv, quads, n, n_dev, toi_out, toi_min, blockIdx.x, gridDim.x, tq);
 }
 
 // The EE and PT candidate lists of one CCD call in one launch: workgroups
@@ -453,11 +514,12 @@ __global__ void __launch_bounds__(256) collision_time_pair_kernel(
     const float* __restrict__ x0, const float* __restrict__ x1, int nv, const int4* __restrict__ quads_ee,
     const int* __restrict__ n_ee, int cap_ee, int blocks_ee, const int4* __restrict__ quads_pt,
     const int* __restrict__ n_pt, int cap_pt, unsigned long long* __restrict__ toi_min) {
+  __shared__ ToiQueue tq;
   if (int(blockIdx.x) < blocks_ee)
-    collision_time_block<true>(x0, x1, nv, quads_ee, cap_ee, n_ee, nullptr, toi_min, blockIdx.x, blocks_ee);
+    collision_time_block<true>(x0, x1, nv, quads_ee, cap_ee, n_ee, nullptr, toi_min, blockIdx.x, blocks_ee, tq);
   else
     collision_time_block<false>(x0, x1, nv, quads_pt, cap_pt, n_pt, nullptr, toi_min, blockIdx.x - blocks_ee,
-                                gridDim.x - blocks_ee);
+                                gridDim.x - blocks_ee, tq);
 }
 
 // edge_edge_intersections / point_triangle_intersections (contact.py:370-442)
@@ -610,12 +672,21 @@ hipError_t launch_contact_points(bool ee, const float* x, const float* xcs, int 
   return hipGetLastError();
 }
 
+// Workgroups of a TOI launch over (a capacity of) n candidates: one 256-
+// candidate range each for short lists; long lists give each workgroup
+// about four ranges (so its queue gathers enough root searches to fill its
+// waves) while still keeping at least 1536 workgroups (6 per CU) in flight.
+static inline int toi_blocks(int n) {
+  const int one = blocks_for(n, kToiBlock), four = blocks_for(n, 4 * kToiBlock);
+  const int g = one < 1536 ? one : (four > 1536 ? four : 1536);
+  return g > 8192 ? 8192 : g;
+}
+
 static hipError_t collision_time_grid(bool ee, const float* x0, const float* x1, int nv, const int* quads, int n,
                                       const int* n_dev, double* toi_out, unsigned long long* toi_min, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const int4* q4 = reinterpret_cast<const int4*>(quads);
-  int grid = blocks_for(n, 256);
-  if (grid > 8192) grid = 8192;
+  const int grid = toi_blocks(n);
   if (ee)
     hipLaunchKernelGGL(collision_time_kernel<true>, dim3(grid), dim3(256), 0, s, x0, x1, nv, q4, n, n_dev, toi_out,
                        toi_min);
@@ -640,9 +711,7 @@ hipError_t launch_collision_time_pair(const float* x0, const float* x1, int nv, 
                                       unsigned long long* toi_min, hipStream_t s) {
   if (cap_ee <= 0) return launch_collision_time_counted(false, x0, x1, nv, quads_pt, n_pt, cap_pt, toi_min, s);
   if (cap_pt <= 0) return launch_collision_time_counted(true, x0, x1, nv, quads_ee, n_ee, cap_ee, toi_min, s);
-  int blocks_ee = blocks_for(cap_ee, 256), blocks_pt = blocks_for(cap_pt, 256);
-  if (blocks_ee > 8192) blocks_ee = 8192;
-  if (blocks_pt > 8192) blocks_pt = 8192;
+  const int blocks_ee = toi_blocks(cap_ee), blocks_pt = toi_blocks(cap_pt);
   hipLaunchKernelGGL(collision_time_pair_kernel, dim3(blocks_ee + blocks_pt), dim3(256), 0, s, x0, x1, nv,
                      reinterpret_cast<const int4*>(quads_ee), n_ee, cap_ee, blocks_ee,
                      reinterpret_cast<const int4*>(quads_pt), n_pt, cap_pt, toi_min);

@@ -658,18 +658,23 @@ IPCD_FN int quadratic_roots(const T coef[3], T x0, T x1, T roots[2]) {
 // The case analysis of cubic_roots.py picks one bracket for the iterative
 // root and, in some cases, an interval for the deflated quadratic's roots;
 // neither choice depends on the root's value, so the cases are decided
-// first and find_closed runs at a single call site. Lanes of a wave that
-// take different cases then share one loop instead of running one inlined
-// loop per case after another.
+// first (cubic_case) and find_closed runs at a single call site. Lanes of a
+// wave that take different cases then share one loop instead of running one
+// inlined loop per case after another.
+template <typename T> struct CubicCase {
+  bool search, deflated;
+  T lo, hi, ylo, dlo;  // bracket of the root; lower end of the quadratic's interval
+  T deriv[4];
+};
 template <typename T>
-IPCD_FN int cubic_roots(const T coef[4], T x0, T x1, T roots[3]) {
-  roots[0] = roots[1] = roots[2] = T(0);
+IPCD_FN CubicCase<T> cubic_case(const T coef[4], T x0, T x1) {
+  CubicCase<T> K;
   T y0 = cevaluate(coef, x0), y1 = cevaluate(coef, x1);
   T a = coef[3] * T(3), b_2 = coef[2], c = coef[1];
-  T deriv[4] = {c, T(2) * b_2, a, T(0)};
+  K.deriv[0] = c; K.deriv[1] = T(2) * b_2; K.deriv[2] = a; K.deriv[3] = T(0);
   T delta_4 = b_2 * b_2 - a * c;
   bool search = false, deflated = false;
-  T lo = x0, hi = x1, ylo = y0, dlo = x0;  // bracket of the root; lower end of the quadratic's interval
+  T lo = x0, hi = x1, ylo = y0, dlo = x0;
   if (delta_4 > T(0)) {
     T d_2 = sqrt(delta_4);
     T q = -(b_2 + multi_sign(d_2, b_2));
@@ -720,12 +725,21 @@ IPCD_FN int cubic_roots(const T coef[4], T x0, T x1, T roots[3]) {
   } else {
     search = diff_sign(y0, y1);
   }
-  if (!search) return 0;
-  roots[0] = find_closed(coef, deriv, lo, hi, ylo);
-  if (!deflated) return 1;
+  K.search = search;
+  K.deflated = deflated;
+  K.lo = lo; K.hi = hi; K.ylo = ylo; K.dlo = dlo;
+  return K;
+}
+template <typename T>
+IPCD_FN int cubic_roots(const T coef[4], T x0, T x1, T roots[3]) {
+  roots[0] = roots[1] = roots[2] = T(0);
+  const CubicCase<T> K = cubic_case(coef, x0, x1);
+  if (!K.search) return 0;
+  roots[0] = find_closed(coef, K.deriv, K.lo, K.hi, K.ylo);
+  if (!K.deflated) return 1;
   T defpoly[3], qroots[2];
   deflate(coef, roots[0], defpoly);
-  int r = quadratic_roots(defpoly, dlo, x1, qroots);
+  int r = quadratic_roots(defpoly, K.dlo, x1, qroots);
   roots[1] = qroots[0];
   roots[2] = qroots[1];
   return r + 1;
@@ -745,18 +759,35 @@ template <typename T> IPCD_FN void ls2(V3<T> e1, V3<T> e2, V3<T> q, T* beta, T* 
   *gamma = (a11 * b2 - a12 * b1) / det;
 }
 
+// Coplanarity cubic of the 4 points at t = 0 (p0) and t = 1 (p1), relative
+// to point 0: u_k, v_k with point k + 1 at u_k + t v_k.
+template <typename T>
+IPCD_FN void coplanarity(const V3<T> p0[4], const V3<T> p1[4], V3<T> u[3], V3<T> v[3], T coef[4]) {
+  V3<T> d0 = p1[0] - p0[0];
+  for (int k = 0; k < 3; ++k) { u[k] = p0[k + 1] - p0[0]; v[k] = (p1[k + 1] - p0[k + 1]) - d0; }
+  coef[0] = det3(u[0], u[1], u[2]);
+  coef[1] = det3(v[0], u[1], u[2]) + det3(u[0], v[1], u[2]) + det3(u[0], u[1], v[2]);
+  coef[2] = det3(u[0], v[1], v[2]) + det3(v[0], u[1], v[2]) + det3(v[0], v[1], u[2]);
+  coef[3] = det3(v[0], v[1], v[2]);
+}
+
+// Whether the coplanarity cubic has a root in [0, 1] at all (cubic_roots'
+// case analysis finds a bracket): without one, collision_time returns 1.
+template <typename T>
+IPCD_FN bool collision_time_may_hit(const V3<T> p0[4], const V3<T> p1[4]) {
+  V3<T> u[3], v[3];
+  T coef[4];
+  coplanarity(p0, p1, u, v, coef);
+  return cubic_case(coef, T(0), T(1)).search;
+}
+
 // pt_collision_time / ee_collision_time (ccd/ccd.py:304-357) on the 4 points
 // at t = 0 (p0) and t = 1 (p1). PT: point, t0, t1, t2; EE: a0, a1, b0, b1.
 template <bool EE, typename T>
 IPCD_FN T collision_time(const V3<T> p0[4], const V3<T> p1[4]) {
   V3<T> u[3], v[3];
-  V3<T> d0 = p1[0] - p0[0];
-  for (int k = 0; k < 3; ++k) { u[k] = p0[k + 1] - p0[0]; v[k] = (p1[k + 1] - p0[k + 1]) - d0; }
   T coef[4];
-  coef[0] = det3(u[0], u[1], u[2]);
-  coef[1] = det3(v[0], u[1], u[2]) + det3(u[0], v[1], u[2]) + det3(u[0], u[1], v[2]);
-  coef[2] = det3(u[0], v[1], v[2]) + det3(v[0], u[1], v[2]) + det3(v[0], v[1], u[2]);
-  coef[3] = det3(v[0], v[1], v[2]);
+  coplanarity(p0, p1, u, v, coef);
   T roots[3];
   int n = cubic_roots(coef, T(0), T(1), roots);
   const T tol = T(1e-8);
