@@ -26,6 +26,11 @@
  * pair-producing entry point is the capacity it needed (grow and call again,
  * as ContactSolverBase._grow_contact_list does, contact.py:523-537).
  * ipc_last_error() describes the last failure on the calling thread.
+ *
+ * Alignment: contact, candidate and record lists (int32 [n][4]) and the
+ * triplet arrays are read and written as 16-byte vectors and must be
+ * 16-byte aligned; primitive-id lists (int32 [n][2]) 8-byte aligned. A
+ * misaligned pointer is refused with IPC_EINVAL before any launch.
  */
 #ifndef IPC_WARP_AMD_H
 #define IPC_WARP_AMD_H
@@ -120,7 +125,12 @@ int ipc_collision_time_pt(const float* x0, const float* x1, int n_vertices, cons
  * The device state ContactSolverBase owns (contact.py:444-521): topology
  * (copied at creation from device or host pointers: edges int32 [E][2],
  * triangles int32 [F][3], body int32 [V]), the BVHs and the CCD buffers.
- * One context per mesh soup; calls on one context must not overlap. */
+ * One context per mesh soup. A context's buffers (BVHs, leaf and query
+ * orders, candidate lists) are reused from one call to the next, so calls on
+ * one context must run one after another on the GPU: issue them on one
+ * stream, or order calls on different streams with events; calls that
+ * overlap on the GPU race on those buffers. Different contexts are
+ * independent. */
 typedef struct ipc_ctx ipc_ctx;
 int ipc_ctx_create(ipc_ctx** ctx, int n_vertices, const int* edges, int n_edges, const int* triangles,
                    int n_triangles, const int* body);

@@ -84,6 +84,9 @@ def lib() -> ctypes.CDLL:
     return _lib
 
 
+IPC_EINVAL = -100000  # status of a refused argument (include/ipc_warp_amd.h)
+
+
 def check(status: int, what: str) -> int:
     """Raise on a negative status; return a positive one (a required capacity)."""
     if status < 0:

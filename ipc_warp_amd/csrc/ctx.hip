@@ -248,6 +248,12 @@ QueryLaunch ccd_pt_query(ipc_ctx* c, const float* x0, const float* x1, float pad
   return q;
 }
 
+// Record lists are written as int4 and id lists as int2.
+bool lists_aligned(const int* rec, const int* ids) {
+  return (reinterpret_cast<unsigned long long>(rec) & 15ull) == 0 &&
+         (reinterpret_cast<unsigned long long>(ids) & 7ull) == 0;
+}
+
 BuildPrologue zero_prologue(int* a, int* b = nullptr) {
   BuildPrologue p;
   if (a) p.zero[p.n_zero++] = a;
@@ -264,6 +270,8 @@ int ipc_detect_collision_ee(ipc_ctx* c, const float* x, float thickness, float b
                             int* contacts, int* e0e1, int capacity, int* count, void* stream) {
   if (!c || !x || !count) return ctx_fail(IPC_EINVAL, "detect_collision_ee: null pointer");
   if (capacity < 0 || (capacity > 0 && !contacts)) return ctx_fail(IPC_EINVAL, "detect_collision_ee: bad capacity");
+  if (!lists_aligned(contacts, e0e1))
+    return ctx_fail(IPC_EINVAL, "detect_collision_ee: records must be 16-byte and ids 8-byte aligned");
   hipStream_t s = static_cast<hipStream_t>(stream);
   BuildJob job{&c->edge_bvh, BOX_EDGE, c->ne, c->edges.as<int>(), nullptr, thickness + buffer};
   int st = ctx_hip(timed(c, IPC_TIMER_BUILD_EDGE, s, [&] {
@@ -281,6 +289,8 @@ int ipc_detect_collision_pt(ipc_ctx* c, const float* x, float thickness, float b
                             int* contacts, int* e0e1, int capacity, int* count, void* stream) {
   if (!c || !x || !count) return ctx_fail(IPC_EINVAL, "detect_collision_pt: null pointer");
   if (capacity < 0 || (capacity > 0 && !contacts)) return ctx_fail(IPC_EINVAL, "detect_collision_pt: bad capacity");
+  if (!lists_aligned(contacts, e0e1))
+    return ctx_fail(IPC_EINVAL, "detect_collision_pt: records must be 16-byte and ids 8-byte aligned");
   hipStream_t s = static_cast<hipStream_t>(stream);
   const int n_tri = c->nv > 0 ? c->nf : 0;
   BuildJob job{&c->tri_bvh, BOX_TRI, n_tri, c->tris.as<int>(), nullptr, 0.0f};
@@ -307,6 +317,8 @@ int ipc_detect_collision(ipc_ctx* c, const float* x, float thickness, float buff
   if (!c || !x || !ee_count || !pt_count) return ctx_fail(IPC_EINVAL, "detect_collision: null pointer");
   if (ee_capacity < 0 || (ee_capacity > 0 && !ee) || pt_capacity < 0 || (pt_capacity > 0 && !pt))
     return ctx_fail(IPC_EINVAL, "detect_collision: bad capacity");
+  if (!lists_aligned(ee, ee_ids) || !lists_aligned(pt, pt_ids))
+    return ctx_fail(IPC_EINVAL, "detect_collision: records must be 16-byte and ids 8-byte aligned");
   hipStream_t s = static_cast<hipStream_t>(stream);
   const int n_tri = c->nv > 0 ? c->nf : 0;
   BuildJob jobs[2] = {{&c->edge_bvh, BOX_EDGE, c->ne, c->edges.as<int>(), nullptr, thickness + buffer},
@@ -339,6 +351,7 @@ int ipc_detect_collision(ipc_ctx* c, const float* x, float thickness, float buff
 int ipc_count_intersections(ipc_ctx* c, const float* x, int* ids, int capacity, int* count, void* stream) {
   if (!c || !x || !count) return ctx_fail(IPC_EINVAL, "count_intersections: null pointer");
   if (capacity < 0 || (capacity > 0 && !ids)) return ctx_fail(IPC_EINVAL, "count_intersections: bad capacity");
+  if (!lists_aligned(nullptr, ids)) return ctx_fail(IPC_EINVAL, "count_intersections: ids must be 8-byte aligned");
   hipStream_t s = static_cast<hipStream_t>(stream);
   int st = ctx_hip(launch_zero_i32(count, 1, s), "count_intersections");
   if (st || c->ne == 0 || c->nf == 0) return st;
@@ -364,6 +377,8 @@ int ipc_ccd_candidates_ee(ipc_ctx* c, const float* x0, const float* x1, float pa
                           int* quads, int* ids, int capacity, int* count, void* stream) {
   if (!c || !x0 || !x1 || !count) return ctx_fail(IPC_EINVAL, "ccd_candidates_ee: null pointer");
   if (capacity < 0 || (capacity > 0 && !quads)) return ctx_fail(IPC_EINVAL, "ccd_candidates_ee: bad capacity");
+  if (!lists_aligned(quads, ids))
+    return ctx_fail(IPC_EINVAL, "ccd_candidates_ee: records must be 16-byte and ids 8-byte aligned");
   hipStream_t s = static_cast<hipStream_t>(stream);
   BuildJob job{&c->edge_bvh, BOX_SWEPT_EDGE, c->ne, c->edges.as<int>(), x1, padding};
   int st = ctx_hip(timed(c, IPC_TIMER_BUILD_EDGE, s, [&] {
@@ -379,6 +394,8 @@ int ipc_ccd_candidates_pt(ipc_ctx* c, const float* x0, const float* x1, float pa
                           int* quads, int* ids, int capacity, int* count, void* stream) {
   if (!c || !x0 || !x1 || !count) return ctx_fail(IPC_EINVAL, "ccd_candidates_pt: null pointer");
   if (capacity < 0 || (capacity > 0 && !quads)) return ctx_fail(IPC_EINVAL, "ccd_candidates_pt: bad capacity");
+  if (!lists_aligned(quads, ids))
+    return ctx_fail(IPC_EINVAL, "ccd_candidates_pt: records must be 16-byte and ids 8-byte aligned");
   hipStream_t s = static_cast<hipStream_t>(stream);
   const int n_tri = c->nv > 0 ? c->nf : 0;
   BuildJob job{&c->tri_bvh, BOX_SWEPT_TRI, n_tri, c->tris.as<int>(), x1, padding};

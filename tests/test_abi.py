@@ -44,3 +44,22 @@ def test_abi_version_and_validation():
     assert L.ipc_contact_energy_pt(None, 4, None, 0, ctypes.c_float(1e-3), 1e8, None, None) == 0
     with pytest.raises(_lib.IPCError):
         _lib.check(-100000, "probe")
+
+
+def test_misaligned_pointers_are_refused_before_any_launch():
+    """The Hessian and energy entry points check 16-byte alignment of the
+    record and triplet arrays on the host; the stand-in addresses below are
+    never dereferenced."""
+    L = ipc_warp_amd.lib()
+    ok, off = 1 << 20, (1 << 20) + 4
+    st = L.ipc_contact_hessian_ee(ok, ok, 4, off, 1, ctypes.c_float(1e-3), 1e8, 0, ok, ok, ok, ok, None)
+    assert st == _lib.IPC_EINVAL and b"aligned" in L.ipc_last_error()
+    st = L.ipc_contact_hessian_pt(ok, 4, ok, 1, ctypes.c_float(1e-3), 1e8, 0, ok, ok, off, ok, None)
+    assert st == _lib.IPC_EINVAL and b"aligned" in L.ipc_last_error()
+    st = L.ipc_contact_energy_pt(ok, 4, off, 1, ctypes.c_float(1e-3), 1e8, ok, None)
+    assert st == _lib.IPC_EINVAL and b"aligned" in L.ipc_last_error()
+
+
+def test_status_constant_matches_header():
+    src = open(os.path.join(ROOT, "include", "ipc_warp_amd.h")).read()
+    assert int(re.search(r"#define IPC_EINVAL \((-?\d+)\)", src).group(1)) == _lib.IPC_EINVAL

@@ -332,3 +332,28 @@ def test_solver_hessian_and_energy_match_oracle(dev, oracle):
     assert S.collision_energy_raw() == pytest.approx(e_ref, rel=1e-9)
     with pytest.raises(ValueError):
         S.compute_collision_hessian(torch.zeros((3, 3), dtype=torch.float32, device=dev))
+
+
+def test_misaligned_lists_are_refused(dev, oracle):
+    """Record lists are written as 16-byte vectors and id lists as 8-byte
+    pairs: a pointer off that alignment is refused with IPC_EINVAL before any
+    launch, and the count stays untouched."""
+    from ipc_warp_amd._lib import IPC_EINVAL, lib
+
+    soup = scenes.two_cubes_on_ground(seed=3, n=4)
+    S = _solver(dev, soup)
+    rec = torch.zeros(4 * 64 + 4, dtype=torch.int32, device=dev)
+    ids = torch.zeros(2 * 64 + 2, dtype=torch.int32, device=dev)
+    cnt = torch.full((1,), -7, dtype=torch.int32, device=dev)
+    xp = S.x.data_ptr()
+    for r_off, i_off in ((4, 0), (0, 4)):
+        r, i = rec.data_ptr() + r_off, ids.data_ptr() + i_off
+        assert lib().ipc_detect_collision_ee(S._ctx, xp, TH, TH, 0, r, i, 64, cnt.data_ptr(), None) == IPC_EINVAL
+        assert lib().ipc_detect_collision_pt(S._ctx, xp, TH, TH, 0, r, i, 64, cnt.data_ptr(), None) == IPC_EINVAL
+        assert lib().ipc_ccd_candidates_ee(S._ctx, xp, xp, 1e-7, 0, r, i, 64, cnt.data_ptr(), None) == IPC_EINVAL
+        assert lib().ipc_ccd_candidates_pt(S._ctx, xp, xp, 1e-7, 0, r, i, 64, cnt.data_ptr(), None) == IPC_EINVAL
+        assert lib().ipc_detect_collision(S._ctx, xp, TH, TH, 0, r, i, 64, cnt.data_ptr(), rec.data_ptr(),
+                                          ids.data_ptr(), 64, cnt.data_ptr(), None) == IPC_EINVAL
+        assert b"aligned" in lib().ipc_last_error()
+    torch.cuda.synchronize()
+    assert int(cnt.item()) == -7
