@@ -333,6 +333,47 @@ int ipc_zero_f32(float* p, long long n, void* stream);
 int ipc_warmstart_features(const float* x, int n_vertices, const float* base, const int* ee, int n_ee, const int* pt,
                            int n_pt, float dhat, unsigned long long* slots, float* features, void* stream);
 
+/* ============================================== block-sparse assembly ===
+ * BSR matrices of 3x3 fp32 blocks as warp.sparse keeps them: offsets int32
+ * [n_rows + 1], columns int32 [nnz] sorted and distinct within each row,
+ * values float [nnz][3][3] row-major. An ipc_bsr is the workspace of these
+ * calls and keeps the pattern of its last successful ipc_bsr_from_triplets;
+ * calls on one workspace must run one after another on the GPU (one stream,
+ * or ordered with events). */
+typedef struct ipc_bsr ipc_bsr;
+int ipc_bsr_create(ipc_bsr** bsr);
+int ipc_bsr_destroy(ipc_bsr* bsr);
+/* Replaces bsr_from_triplets(n_rows, n_cols, rows, cols, vals)
+ * (dynamic_contacts.py:290-296; affine_body_dynamics.py:500-503;
+ * augmented_lagrangian.py:631): triplet t = (rows[t], cols[t], vals[t]
+ * float [3][3]) is dropped when its row or column is out of range or its
+ * nine values are all zero (NaN is nonzero); the kept triplets of one
+ * (row, column) are summed in fp32 in ascending t from +0. Writes offsets
+ * always and columns/values when the block count fits `capacity`. Returns 0,
+ * or, when capacity < n_triplets and the pattern needs more blocks, the
+ * block count (columns/values untouched; the call synchronises `stream`
+ * only in that case). */
+int ipc_bsr_from_triplets(ipc_bsr* bsr, int n_rows, int n_cols, const int* rows, const int* cols, const float* vals,
+                          int n_triplets, int* offsets, int* columns, float* values, int capacity, void* stream);
+/* Values of the last pattern ipc_bsr_from_triplets built on this workspace
+ * for new triplet values with the same rows and columns (the pattern reuse of
+ * solve_ldlt, dynamic_contacts.py:314-333): every block sums its triplets in
+ * the same order as the build, a triplet dropped by the build stays out and a
+ * block whose triplets are now zero stays in the pattern as zeros. */
+int ipc_bsr_refill(ipc_bsr* bsr, const float* vals, int n_triplets, float* values, void* stream);
+/* z = alpha x + beta y over the union of the two patterns (bsr_axpy(x, y,
+ * alpha, beta), dynamic_contacts.py:297; augmented_lagrangian.py:632, which
+ * writes the sum into y): both inputs canonical (columns sorted and distinct
+ * per row); a block present on one side only counts as zero on the other;
+ * each value is alpha * xv + beta * yv in fp32. Writes z_offsets always and
+ * z_columns / z_values when the union fits `capacity`. x_nnz / y_nnz are the
+ * block counts of x and y or upper bounds of them (their capacities). Returns
+ * 0, or, when capacity < x_nnz + y_nnz and the union needs more blocks, its
+ * block count (after synchronising `stream`). z must not alias x or y. */
+int ipc_bsr_axpy(ipc_bsr* bsr, int n_rows, const int* x_offsets, const int* x_columns, const float* x_values, int x_nnz,
+                 float alpha, const int* y_offsets, const int* y_columns, const float* y_values, int y_nnz, float beta,
+                 int* z_offsets, int* z_columns, float* z_values, int capacity, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

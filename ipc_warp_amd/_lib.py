@@ -55,6 +55,12 @@ _SIGNATURES = {
                              _c_int),
     "ipc_count_intersections": ([_vp, _vp, _vp, _c_int, _vp, _vp], _c_int),
     "ipc_warmstart_features": ([_vp, _c_int, _vp, _vp, _c_int, _vp, _c_int, _c_float, _vp, _vp, _vp], _c_int),
+    "ipc_bsr_create": ([ctypes.POINTER(_vp)], _c_int),
+    "ipc_bsr_destroy": ([_vp], _c_int),
+    "ipc_bsr_from_triplets": ([_vp, _c_int, _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _vp], _c_int),
+    "ipc_bsr_refill": ([_vp, _vp, _c_int, _vp, _vp], _c_int),
+    "ipc_bsr_axpy": ([_vp, _c_int, _vp, _vp, _vp, _c_int, _c_float, _vp, _vp, _vp, _c_int, _c_float, _vp, _vp, _vp,
+                      _c_int, _vp], _c_int),
 }
 
 TIMER_KINDS = ("build_edge", "build_tri", "query_dcd_ee", "query_dcd_pt", "query_ccd_ee", "query_ccd_pt",

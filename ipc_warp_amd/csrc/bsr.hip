@@ -1,0 +1,556 @@
+// Block-sparse (BSR, 3x3 fp32 blocks) assembly of the Newton system: the
+// triplet -> BSR conversion the contact Hessian goes through, a value-only
+// refill over a kept pattern, and the sum of two BSR matrices
+// (warp.sparse.bsr_from_triplets / bsr_axpy at dynamic_contacts.py:290-297,
+// affine_body_dynamics.py:500-503, augmented_lagrangian.py:631-632).
+//
+// Triplet -> BSR in row buckets instead of a global key sort:
+//   count    one lane per triplet: drop it (row or column out of range, or
+//            all nine values zero), else take its slot in the row's bucket
+//            with an atomic on the row's counter;
+//   scan     bucket starts = exclusive scan of the row counts;
+//   scatter  (column, triplet) into the row's bucket;
+//   sort     one wave per row: sort the bucket by (column, triplet) — in
+//            registers up to 64 entries, bitonic in LDS up to 1024, by rank
+//            in memory beyond — and count the distinct columns;
+//   scan     row offsets = exclusive scan of the distinct counts;
+//   emit     one wave per row: the first entry of each column run writes the
+//            column and the fp32 sum of its run, added in ascending triplet
+//            order from +0, and remembers where its run starts (the pattern
+//            a refill reuses).
+// Every step reads or writes each triplet a bounded number of times; the
+// result does not depend on the order the atomics hand out slots.
+#include <cstdio>
+
+#include "../../include/ipc_warp_amd.h"
+#include "cull.hpp"
+#include "ipc_kernels.hpp"
+
+using namespace ipcd;
+
+struct ipc_bsr {
+  DevBuf cnt;      // int [n_rows + 1]: per-row kept triplets, then distinct columns
+  DevBuf start;    // int [n_rows + 1]: bucket start of each row
+  DevBuf slot;     // int [n_triplets]: slot in the row's bucket, -1 if dropped
+  DevBuf bucket;   // int2 [n_triplets]: (column, triplet) by row, unsorted
+  DevBuf sorted;   // int2 [n_triplets]: by row, then column, then triplet
+  DevBuf seg;      // int [capacity + 1]: first sorted entry of each block
+  DevBuf part;     // scan partial sums
+  DevBuf nnz;      // int [1]: blocks of the kept pattern
+  int* host = nullptr;  // pinned readback of the block count
+  int n_rows = 0, n_triplets = 0, max_blocks = 0;
+  bool planned = false;
+};
+
+namespace {
+
+constexpr int kB = 256;
+constexpr int kWaves = kB / 64;
+constexpr int kSortCap = 1024;  // entries of one row staged in LDS by its wave
+constexpr int kWin = 256;       // triplet values staged per window and wave
+constexpr int kScanPer = 16;
+constexpr int kScanTile = kB * kScanPer;
+
+int bsr_fail(int code, const char* what) {
+  set_error(what);
+  return code;
+}
+
+int bsr_hip(hipError_t e, const char* where) {
+  if (e == hipSuccess) return IPC_OK;
+  char buf[512];
+  std::snprintf(buf, sizeof(buf), "%s: %s", where, hipGetErrorString(e));
+  set_error(buf);
+  return -int(e);
+}
+
+inline int nblk(long long n, int per) { return int((n + per - 1) / per); }
+
+__global__ void fill_i32_kernel(int* p, int n, int v) {
+  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) p[i] = v;
+}
+
+hipError_t fill_i32(int* p, int n, int v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int g = nblk(n, kB) < 1024 ? nblk(n, kB) : 1024;
+  hipLaunchKernelGGL(fill_i32_kernel, dim3(g), dim3(kB), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- scan ---
+// out[0..n) = exclusive prefix sums of in[0..n) (in and out distinct), in
+// tiles of kScanTile, then the tile sums, then the tile offsets added.
+__device__ int block_exclusive(int v, int* sh, int& total) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int d = 1; d < kB; d <<= 1) {
+    const int u = tid >= d ? sh[tid - d] : 0;
+    __syncthreads();
+    sh[tid] += u;
+    __syncthreads();
+  }
+  total = sh[kB - 1];
+  const int r = sh[tid] - v;
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(kB) void scan_tiles_kernel(const int* in, int n, int* out, int* tile_sum) {
+  __shared__ int sh[kB];
+  const long long base = (long long)blockIdx.x * kScanTile + threadIdx.x * kScanPer;
+  int v[kScanPer];
+  int acc = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    v[k] = base + k < n ? in[base + k] : 0;
+    acc += v[k];
+  }
+  int total;
+  int run = block_exclusive(acc, sh, total);
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kB) void scan_sums_kernel(int* tile_sum, int nt) {
+  __shared__ int sh[kB];
+  int carry = 0;
+  for (int b = 0; b < nt; b += kB) {
+    const int i = b + threadIdx.x;
+    const int v = i < nt ? tile_sum[i] : 0;
+    int total;
+    const int e = block_exclusive(v, sh, total);
+    if (i < nt) tile_sum[i] = carry + e;
+    carry += total;
+  }
+}
+
+__global__ void scan_add_kernel(int* out, int n, const int* tile_sum) {
+  const int tile = blockIdx.x + 1;
+  const int add = tile_sum[tile];
+  const long long base = (long long)tile * kScanTile;
+  for (int k = threadIdx.x; k < kScanTile; k += kB)
+    if (base + k < n) out[base + k] += add;
+}
+
+hipError_t exclusive_scan(const int* in, int n, int* out, DevBuf& part, hipStream_t s) {
+  const int nt = nblk(n, kScanTile);
+  hipError_t e = part.ensure(sizeof(int) * size_t(nt));
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(scan_tiles_kernel, dim3(nt), dim3(kB), 0, s, in, n, out, part.as<int>());
+  if (nt > 1) {
+    hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(kB), 0, s, part.as<int>(), nt);
+    hipLaunchKernelGGL(scan_add_kernel, dim3(nt - 1), dim3(kB), 0, s, out, n, part.as<int>());
+  }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------- triplets -> BSR ---
+__device__ inline bool block_nonzero(const float* v) {
+  bool nz = false;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) nz |= !(v[k] == 0.0f);  // NaN counts as nonzero
+  return nz;
+}
+
+// Lanes of a wave whose triplets are kept in one row take consecutive slots
+// with one atomic per run of neighbouring lanes (the contact Hessian writes
+// the four blocks of a pair's block row next to each other).
+__global__ __launch_bounds__(kB) void bsr_count_kernel(const int* rows, const int* cols, const float* vals, int m,
+                                                       int nr, int nc, int* cnt, int* slot) {
+  const int t = blockIdx.x * kB + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  int key = -1;
+  if (t < m) {
+    const int r = rows[t], c = cols[t];
+    if (unsigned(r) < unsigned(nr) && unsigned(c) < unsigned(nc) && block_nonzero(vals + 9ll * t)) key = r;
+  }
+  const bool keep = key >= 0;
+  const int prev = __shfl_up(key, 1);
+  const bool head = keep && (lane == 0 || prev != key);
+  const unsigned long long hm = __ballot(head), km = __ballot(keep);
+  // this lane's run starts at the last head at or below it and ends before
+  // the next head or dropped lane
+  const unsigned long long upto = lane == 63 ? ~0ull : (2ull << lane) - 1;
+  const int h = keep ? 63 - __clzll(hm & upto) : lane;
+  const unsigned long long brk = (hm | ~km) & ~((h == 63 ? ~0ull : (2ull << h) - 1));
+  const int end = brk ? __ffsll(static_cast<long long>(brk)) - 1 : 64;
+  int base = 0;
+  if (head) base = atomicAdd(cnt + key, end - h);
+  base = __shfl(base, h);
+  if (t < m) slot[t] = keep ? base + (lane - h) : -1;
+}
+
+__global__ void bsr_scatter_kernel(const int* rows, const int* cols, const int* slot, const int* start, int m,
+                                   int2* bucket) {
+  const int t = blockIdx.x * kB + threadIdx.x;
+  if (t >= m) return;
+  const int s = slot[t];
+  if (s >= 0) bucket[start[rows[t]] + s] = make_int2(cols[t], t);
+}
+
+__device__ inline unsigned long long entry_key(int2 e) {
+  return (static_cast<unsigned long long>(unsigned(e.x)) << 32) | unsigned(e.y);
+}
+
+__device__ inline int2 key_entry(unsigned long long k) { return make_int2(int(k >> 32), int(unsigned(k))); }
+
+// Rank of K keys per lane (entries g + lane + 64 k of the row) among the
+// row's L keys, read one at a time at a wave-uniform index from `at`. A key
+// starts its column's run (a head) when its rank equals the number of keys
+// with a smaller column.
+template <int K, typename At>
+__device__ inline int rank_pass(At at, int L, int g, int lane, int b, const int2* bucket, int2* sorted) {
+  unsigned long long key[K];
+  unsigned col[K];
+  int rank[K], below[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = g + lane + 64 * k;
+    key[k] = i < L ? entry_key(bucket[b + i]) : ~0ull;
+    col[k] = unsigned(key[k] >> 32);
+    rank[k] = 0;
+    below[k] = 0;
+  }
+#pragma unroll 4
+  for (int q = 0; q < L; ++q) {
+    const unsigned long long kq = at(q);
+    const unsigned cq = unsigned(kq >> 32);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      rank[k] += kq < key[k];
+      below[k] += cq < col[k];
+    }
+  }
+  int heads = 0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const bool in = g + lane + 64 * k < L;
+    if (in) sorted[b + rank[k]] = key_entry(key[k]);
+    heads += __popcll(__ballot(in && rank[k] == below[k]));
+  }
+  return heads;
+}
+
+template <typename At>
+__device__ inline int rank_row(At at, int L, int lane, int b, const int2* bucket, int2* sorted) {
+  int heads = 0;
+  int g = 0;
+  for (; L - g > 192; g += 256) heads += rank_pass<4>(at, L, g, lane, b, bucket, sorted);
+  if (L - g > 128) heads += rank_pass<3>(at, L, g, lane, b, bucket, sorted);
+  else if (L - g > 64) heads += rank_pass<2>(at, L, g, lane, b, bucket, sorted);
+  else if (L - g > 0) heads += rank_pass<1>(at, L, g, lane, b, bucket, sorted);
+  return heads;
+}
+
+// One wave per row: the row's bucket sorted by (column, triplet) into
+// `sorted`, its distinct column count into rnnz. Keys are distinct (the
+// triplet index is), so each key's rank is its sorted position.
+__global__ __launch_bounds__(kB) void bsr_sort_rows_kernel(const int* start, int nr, const int2* bucket, int2* sorted,
+                                                           int* rnnz) {
+  __shared__ unsigned long long sk[kWaves][kSortCap];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  unsigned long long* s = sk[w];
+  for (int r = blockIdx.x * kWaves + w; r < nr; r += gridDim.x * kWaves) {
+    const int b = start[r], L = start[r + 1] - b;
+    int heads = 0;
+    if (L == 0) {
+    } else if (L <= 64) {
+      const unsigned long long k = lane < L ? entry_key(bucket[b + lane]) : ~0ull;
+      const int khi = int(k >> 32), klo = int(unsigned(k));
+      int rank = 0, below = 0;
+      for (int q = 0; q < L; ++q) {
+        const int hi = __shfl(khi, q), lo = __shfl(klo, q);
+        const unsigned long long kq = (static_cast<unsigned long long>(unsigned(hi)) << 32) | unsigned(lo);
+        rank += kq < k;
+        below += unsigned(hi) < unsigned(khi);
+      }
+      if (lane < L) sorted[b + rank] = key_entry(k);
+      heads = __popcll(__ballot(lane < L && rank == below));
+    } else if (L <= kSortCap) {
+      for (int i = lane; i < L; i += 64) s[i] = entry_key(bucket[b + i]);
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      heads = rank_row([&](int q) { return s[q]; }, L, lane, b, bucket, sorted);
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    } else {
+      heads = rank_row([&](int q) { return entry_key(bucket[b + q]); }, L, lane, b, bucket, sorted);
+    }
+    if (lane == 0) rnnz[r] = heads;
+  }
+}
+
+// One wave per row: the first entry of each column run writes its block's
+// column and where its run starts; the last row also closes the run list.
+__global__ __launch_bounds__(kB) void bsr_pattern_kernel(const int* start, const int* offsets, int nr,
+                                                         const int2* sorted, int cap, int* columns, int* seg,
+                                                         int* nnz_out) {
+  const int total = offsets[nr];
+  if (blockIdx.x == 0 && threadIdx.x == 0) *nnz_out = total <= cap ? total : -1;
+  if (total > cap) return;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const unsigned long long below = (1ull << lane) - 1;
+  for (int r = blockIdx.x * kWaves + w; r < nr; r += gridDim.x * kWaves) {
+    const int b = start[r], L = start[r + 1] - b;
+    int base = offsets[r];
+    for (int i0 = 0; i0 < L; i0 += 64) {
+      const int i = i0 + lane;
+      int c = -1;
+      bool head = false;
+      if (i < L) {
+        c = sorted[b + i].x;
+        head = i == 0 || sorted[b + i - 1].x != c;
+      }
+      const unsigned long long hb = __ballot(head);
+      if (head) {
+        const int o = base + __popcll(hb & below);
+        columns[o] = c;
+        seg[o] = b + i;
+      }
+      base += __popcll(hb);
+    }
+    if (r == nr - 1 && lane == 0) seg[total] = start[nr];
+  }
+}
+
+// Block values: one wave per 64 consecutive blocks. The triplet values of
+// their runs (one contiguous range of the sorted entries) are gathered into
+// LDS a window at a time by the whole wave, then each lane adds its block's
+// entries of the window in order, so every block is the fp32 sum of its
+// triplets in ascending triplet order from +0 however the windows fall.
+__global__ __launch_bounds__(kB) void bsr_sum_kernel(const int* nnz_p, const int* seg, const int2* sorted,
+                                                     const float* vals, float* values) {
+  __shared__ float sv[kWaves][kWin * 9];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  float* s = sv[w];
+  const int n = *nnz_p;
+  for (int o0 = (blockIdx.x * kWaves + w) * 64; o0 < n; o0 += gridDim.x * kWaves * 64) {
+    const int o = o0 + lane;
+    const bool active = o < n;
+    const int mb = active ? seg[o] : 0, me = active ? seg[o + 1] : 0;
+    const int e0 = seg[o0], e1 = seg[o0 + 64 < n ? o0 + 64 : n];
+    float acc[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int w0 = e0; w0 < e1; w0 += kWin) {
+      const int cnt = e1 - w0 < kWin ? e1 - w0 : kWin;
+      for (int k = lane; k < cnt; k += 64) {
+        const float* v = vals + 9ll * sorted[w0 + k].y;
+#pragma unroll
+        for (int j = 0; j < 9; ++j) s[k * 9 + j] = v[j];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      const int lo = mb > w0 ? mb : w0, hi = me < w0 + cnt ? me : w0 + cnt;
+      for (int q = lo; q < hi; ++q) {
+#pragma unroll
+        for (int j = 0; j < 9; ++j) acc[j] += s[(q - w0) * 9 + j];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    }
+    if (active) {
+#pragma unroll
+      for (int j = 0; j < 9; ++j) values[9ll * o + j] = acc[j];
+    }
+  }
+}
+
+hipError_t launch_sum(const ipc_bsr* w, int max_blocks, const float* vals, float* values, hipStream_t s) {
+  const int need = nblk(max_blocks, kB);
+  const int g = need < 1 ? 1 : (need < 4096 ? need : 4096);
+  hipLaunchKernelGGL(bsr_sum_kernel, dim3(g), dim3(kB), 0, s, w->nnz.as<int>(), w->seg.as<int>(),
+                     w->sorted.as<int2>(), vals, values);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------ axpy ---
+// Row r of z = alpha x + beta y over the union of the two rows' columns
+// (both sorted and distinct per row); a block missing from one side counts
+// as zero there.
+__global__ void bsr_union_count_kernel(int nr, const int* xo, const int* xc, const int* yo, const int* yc, int* cnt) {
+  const int r = blockIdx.x * kB + threadIdx.x;
+  if (r > nr) return;
+  if (r == nr) {
+    cnt[r] = 0;
+    return;
+  }
+  int i = xo[r], j = yo[r];
+  const int ie = xo[r + 1], je = yo[r + 1];
+  int n = 0;
+  while (i < ie && j < je) {
+    const int a = xc[i], c = yc[j];
+    ++n;
+    i += a <= c;
+    j += c <= a;
+  }
+  cnt[r] = n + (ie - i) + (je - j);
+}
+
+__global__ void bsr_union_fill_kernel(int nr, const int* xo, const int* xc, const float* xv, float alpha, const int* yo,
+                                      const int* yc, const float* yv, float beta, const int* zo, int* zc, float* zv,
+                                      int cap, int* nnz_out) {
+  const int total = zo[nr];
+  if (blockIdx.x == 0 && threadIdx.x == 0 && nnz_out) *nnz_out = total;
+  const int r = blockIdx.x * kB + threadIdx.x;
+  if (r >= nr || total > cap) return;
+  int i = xo[r], j = yo[r], o = zo[r];
+  const int ie = xo[r + 1], je = yo[r + 1];
+  while (i < ie || j < je) {
+    const int a = i < ie ? xc[i] : 0x7fffffff, c = j < je ? yc[j] : 0x7fffffff;
+    const bool tx = a <= c, ty = c <= a;
+    float* out = zv + 9ll * o;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const float xs = tx ? xv[9ll * i + k] : 0.0f;
+      const float ys = ty ? yv[9ll * j + k] : 0.0f;
+      out[k] = alpha * xs + beta * ys;
+    }
+    zc[o] = tx ? a : c;
+    ++o;
+    i += tx;
+    j += ty;
+  }
+}
+
+}  // namespace
+
+// ============================================================== C ABI ===
+extern "C" {
+
+int ipc_bsr_create(ipc_bsr** out) {
+  if (!out) return bsr_fail(IPC_EINVAL, "bsr_create: null output");
+  ipc_bsr* w = new (std::nothrow) ipc_bsr();
+  if (!w) return bsr_fail(IPC_EINVAL, "bsr_create: out of host memory");
+  hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&w->host), sizeof(int));
+  if (e != hipSuccess) {
+    delete w;
+    return bsr_hip(e, "bsr_create");
+  }
+  *out = w;
+  return IPC_OK;
+}
+
+int ipc_bsr_destroy(ipc_bsr* w) {
+  if (!w) return IPC_OK;
+  for (DevBuf* d : {&w->cnt, &w->start, &w->slot, &w->bucket, &w->sorted, &w->seg, &w->part, &w->nnz}) d->release();
+  if (w->host) hipHostFree(w->host);
+  delete w;
+  return IPC_OK;
+}
+
+int ipc_bsr_from_triplets(ipc_bsr* w, int n_rows, int n_cols, const int* rows, const int* cols, const float* vals,
+                          int n_triplets, int* offsets, int* columns, float* values, int capacity, void* stream) {
+  if (!w) return bsr_fail(IPC_EINVAL, "bsr_from_triplets: null workspace");
+  if (n_rows < 0 || n_cols < 0 || n_triplets < 0 || capacity < 0)
+    return bsr_fail(IPC_EINVAL, "bsr_from_triplets: negative size");
+  if (!offsets) return bsr_fail(IPC_EINVAL, "bsr_from_triplets: null offsets");
+  if (n_triplets > 0 && (!rows || !cols || !vals)) return bsr_fail(IPC_EINVAL, "bsr_from_triplets: null triplets");
+  if (capacity > 0 && (!columns || !values)) return bsr_fail(IPC_EINVAL, "bsr_from_triplets: null columns/values");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  w->planned = false;
+  const int m = n_triplets, nr = n_rows;
+  hipError_t e = hipSuccess;
+  for (DevBuf* d : {&w->cnt, &w->start}) {
+    if (e == hipSuccess) e = d->ensure(sizeof(int) * (size_t(nr) + 1));
+  }
+  if (e == hipSuccess) e = w->slot.ensure(sizeof(int) * size_t(m > 0 ? m : 1));
+  if (e == hipSuccess) e = w->bucket.ensure(sizeof(int2) * size_t(m > 0 ? m : 1));
+  if (e == hipSuccess) e = w->sorted.ensure(sizeof(int2) * size_t(m > 0 ? m : 1));
+  if (e == hipSuccess) e = w->seg.ensure(sizeof(int) * (size_t(capacity) + 1));
+  if (e == hipSuccess) e = w->nnz.ensure(sizeof(int));
+  if (e != hipSuccess) return bsr_hip(e, "bsr_from_triplets: workspace");
+  int* cnt = w->cnt.as<int>();
+  int* start = w->start.as<int>();
+  e = fill_i32(cnt, nr + 1, 0, s);
+  if (e == hipSuccess && m > 0)
+    hipLaunchKernelGGL(bsr_count_kernel, dim3(nblk(m, kB)), dim3(kB), 0, s, rows, cols, vals, m, nr, n_cols, cnt,
+                       w->slot.as<int>());
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e == hipSuccess) e = exclusive_scan(cnt, nr + 1, start, w->part, s);
+  if (e == hipSuccess && m > 0)
+    hipLaunchKernelGGL(bsr_scatter_kernel, dim3(nblk(m, kB)), dim3(kB), 0, s, rows, cols, w->slot.as<int>(), start, m,
+                       w->bucket.as<int2>());
+  const int rg = nr > 0 ? (nblk(nr, kWaves) < 65536 ? nblk(nr, kWaves) : 65536) : 0;
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e == hipSuccess && nr > 0) {
+    // cnt[nr] stays 0 from the fill: the row's distinct counts go to cnt[0..nr)
+    hipLaunchKernelGGL(bsr_sort_rows_kernel, dim3(rg), dim3(kB), 0, s, start, nr, w->bucket.as<int2>(),
+                       w->sorted.as<int2>(), cnt);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = exclusive_scan(cnt, nr + 1, offsets, w->part, s);
+  if (e == hipSuccess) {
+    if (nr > 0) {
+      hipLaunchKernelGGL(bsr_pattern_kernel, dim3(rg), dim3(kB), 0, s, start, offsets, nr, w->sorted.as<int2>(),
+                         capacity, columns, w->seg.as<int>(), w->nnz.as<int>());
+      e = hipGetLastError();
+      if (e == hipSuccess && m > 0) e = launch_sum(w, capacity < m ? capacity : m, vals, values, s);
+    } else {
+      e = fill_i32(w->nnz.as<int>(), 1, 0, s);
+      if (e == hipSuccess) e = fill_i32(w->seg.as<int>(), 1, 0, s);
+    }
+  }
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e != hipSuccess) return bsr_hip(e, "bsr_from_triplets");
+  w->n_rows = nr;
+  w->n_triplets = m;
+  w->max_blocks = capacity < m ? capacity : m;
+  if (capacity >= m) {
+    w->planned = true;
+    return IPC_OK;
+  }
+  // the pattern may not fit: read the block count back
+  e = hipMemcpyAsync(w->host, offsets + nr, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return bsr_hip(e, "bsr_from_triplets: block count readback");
+  if (*w->host > capacity) return *w->host;
+  w->planned = true;
+  return IPC_OK;
+}
+
+int ipc_bsr_refill(ipc_bsr* w, const float* vals, int n_triplets, float* values, void* stream) {
+  if (!w) return bsr_fail(IPC_EINVAL, "bsr_refill: null workspace");
+  if (!w->planned) return bsr_fail(IPC_EINVAL, "bsr_refill: no pattern (ipc_bsr_from_triplets has not succeeded)");
+  if (n_triplets != w->n_triplets) return bsr_fail(IPC_EINVAL, "bsr_refill: triplet count differs from the pattern's");
+  if (n_triplets == 0) return IPC_OK;
+  if (!vals || !values) return bsr_fail(IPC_EINVAL, "bsr_refill: null pointer");
+  hipError_t e = launch_sum(w, w->max_blocks, vals, values, static_cast<hipStream_t>(stream));
+  return bsr_hip(e, "bsr_refill");
+}
+
+int ipc_bsr_axpy(ipc_bsr* w, int n_rows, const int* x_offsets, const int* x_columns, const float* x_values, int x_nnz,
+                 float alpha, const int* y_offsets, const int* y_columns, const float* y_values, int y_nnz, float beta,
+                 int* z_offsets, int* z_columns, float* z_values, int capacity, void* stream) {
+  if (!w) return bsr_fail(IPC_EINVAL, "bsr_axpy: null workspace");
+  if (n_rows < 0 || x_nnz < 0 || y_nnz < 0 || capacity < 0) return bsr_fail(IPC_EINVAL, "bsr_axpy: negative size");
+  if (!x_offsets || !y_offsets || !z_offsets) return bsr_fail(IPC_EINVAL, "bsr_axpy: null offsets");
+  if ((x_nnz > 0 && (!x_columns || !x_values)) || (y_nnz > 0 && (!y_columns || !y_values)) ||
+      (capacity > 0 && (!z_columns || !z_values)))
+    return bsr_fail(IPC_EINVAL, "bsr_axpy: null columns/values");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e = w->cnt.ensure(sizeof(int) * (size_t(n_rows) + 1));
+  if (e != hipSuccess) return bsr_hip(e, "bsr_axpy: workspace");
+  // the row counts share the triplet path's count buffer; a kept pattern
+  // (sorted entries, runs, block count) stays valid
+  int* cnt = w->cnt.as<int>();
+  hipLaunchKernelGGL(bsr_union_count_kernel, dim3(nblk(n_rows + 1, kB)), dim3(kB), 0, s, n_rows, x_offsets, x_columns,
+                     y_offsets, y_columns, cnt);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = exclusive_scan(cnt, n_rows + 1, z_offsets, w->part, s);
+  if (e == hipSuccess && n_rows > 0)
+    hipLaunchKernelGGL(bsr_union_fill_kernel, dim3(nblk(n_rows, kB)), dim3(kB), 0, s, n_rows, x_offsets, x_columns,
+                       x_values, alpha, y_offsets, y_columns, y_values, beta, z_offsets, z_columns, z_values, capacity,
+                       nullptr);
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e != hipSuccess) return bsr_hip(e, "bsr_axpy");
+  if (static_cast<long long>(x_nnz) + y_nnz <= capacity) return IPC_OK;
+  e = hipMemcpyAsync(w->host, z_offsets + n_rows, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return bsr_hip(e, "bsr_axpy: block count readback");
+  return *w->host > capacity ? *w->host : IPC_OK;
+}
+
+}  // extern "C"

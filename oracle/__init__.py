@@ -80,6 +80,10 @@ def lib():
         L.oracle_count_intersections.argtypes = [F32P, I32P, c_int, I32P, c_int, I32P, c_i64]
         L.oracle_count_intersections.restype = c_i64
         L.oracle_warmstart_features.argtypes = [F32P, c_int, F32P, I32P, c_int, I32P, c_int, c_flt, F32P]
+        L.oracle_bsr_from_triplets.argtypes = [c_int, c_int, I32P, I32P, F32P, c_int, I32P, I32P, F32P]
+        L.oracle_bsr_from_triplets.restype = c_int
+        L.oracle_bsr_axpy.argtypes = [c_int, I32P, I32P, F32P, c_flt, I32P, I32P, F32P, c_flt, I32P, I32P, F32P]
+        L.oracle_bsr_axpy.restype = c_int
         _lib = L
     return _lib
 
@@ -353,3 +357,31 @@ def count_intersections(x, edges, triangles):
         if n <= cap:
             return int(n), ids[:n].copy()
         cap = 1 << int(n - 1).bit_length()
+
+
+def bsr_from_triplets(n_rows, n_cols, rows, cols, vals):
+    """(offsets, columns, values float32 [nnz, 3, 3]) of the triplets, summed
+    in fp32 in triplet order (warp.sparse.bsr_from_triplets's contract)."""
+    rows, cols = _i32(rows).reshape(-1), _i32(cols).reshape(-1)
+    vals = _f32(vals).reshape(-1)
+    n = rows.size
+    offsets = np.zeros(n_rows + 1, np.int32)
+    columns = np.zeros(max(n, 1), np.int32)
+    values = np.zeros(max(n, 1) * 9, np.float32)
+    nnz = lib().oracle_bsr_from_triplets(n_rows, n_cols, _p(rows, I32P), _p(cols, I32P), _p(vals, F32P), n,
+                                         _p(offsets, I32P), _p(columns, I32P), _p(values, F32P))
+    return offsets, columns[:nnz].copy(), values[:9 * nnz].reshape(nnz, 3, 3).copy()
+
+
+def bsr_axpy(x, y, alpha=1.0, beta=1.0):
+    """alpha x + beta y of two (offsets, columns, values) BSR triples."""
+    xo, xc, xv = _i32(x[0]), _i32(x[1]), _f32(x[2]).reshape(-1)
+    yo, yc, yv = _i32(y[0]), _i32(y[1]), _f32(y[2]).reshape(-1)
+    nr = xo.size - 1
+    cap = max(xc.size + yc.size, 1)
+    zo = np.zeros(nr + 1, np.int32)
+    zc = np.zeros(cap, np.int32)
+    zv = np.zeros(cap * 9, np.float32)
+    nnz = lib().oracle_bsr_axpy(nr, _p(xo, I32P), _p(xc, I32P), _p(xv, F32P), alpha, _p(yo, I32P), _p(yc, I32P),
+                                _p(yv, F32P), beta, _p(zo, I32P), _p(zc, I32P), _p(zv, F32P))
+    return zo, zc[:nnz].copy(), zv[:9 * nnz].reshape(nnz, 3, 3).copy()

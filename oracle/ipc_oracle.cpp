@@ -5,9 +5,11 @@
 
 #include <omp.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <unordered_map>
 #include <utility>
+#include <vector>
 
 using namespace ipco;
 
@@ -647,6 +649,68 @@ void oracle_warmstart_features(const float* x, int nv, const float* base, const 
     }
     i = j;
   }
+}
+
+// warp.sparse.bsr_from_triplets as called at dynamic_contacts.py:290-296
+// (Warp is absent; its published contract): triplets with a row or column
+// out of range, or nine zero values, are dropped; the rest are ordered by
+// (row, column) keeping triplet order within a key and summed per key in
+// fp32 from +0. values holds 9 * n floats (the pattern has at most n blocks).
+int oracle_bsr_from_triplets(int nr, int nc, const int* rows, const int* cols, const float* vals, int n, int* offsets,
+                             int* columns, float* values) {
+  std::vector<int> keep;
+  keep.reserve(n);
+  for (int t = 0; t < n; ++t) {
+    if (rows[t] < 0 || rows[t] >= nr || cols[t] < 0 || cols[t] >= nc) continue;
+    bool nz = false;
+    for (int k = 0; k < 9; ++k) nz |= !(vals[9ll * t + k] == 0.0f);
+    if (nz) keep.push_back(t);
+  }
+  std::stable_sort(keep.begin(), keep.end(), [&](int a, int b) {
+    return rows[a] != rows[b] ? rows[a] < rows[b] : cols[a] < cols[b];
+  });
+  for (int r = 0; r <= nr; ++r) offsets[r] = 0;
+  int nnz = 0;
+  for (size_t i = 0; i < keep.size();) {
+    const int r = rows[keep[i]], c = cols[keep[i]];
+    float acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    size_t j = i;
+    for (; j < keep.size() && rows[keep[j]] == r && cols[keep[j]] == c; ++j)
+      for (int k = 0; k < 9; ++k) acc[k] += vals[9ll * keep[j] + k];
+    columns[nnz] = c;
+    for (int k = 0; k < 9; ++k) values[9ll * nnz + k] = acc[k];
+    ++offsets[r + 1];
+    ++nnz;
+    i = j;
+  }
+  for (int r = 0; r < nr; ++r) offsets[r + 1] += offsets[r];
+  return nnz;
+}
+
+// warp.sparse.bsr_axpy(x, y, alpha, beta) (dynamic_contacts.py:297): the
+// union of two canonical patterns, a block missing on one side read as zero
+// there, each value alpha * xv + beta * yv in fp32. zc / zv hold
+// x_nnz + y_nnz blocks.
+int oracle_bsr_axpy(int nr, const int* xo, const int* xc, const float* xv, float alpha, const int* yo, const int* yc,
+                    const float* yv, float beta, int* zo, int* zc, float* zv) {
+  int o = 0;
+  zo[0] = 0;
+  for (int r = 0; r < nr; ++r) {
+    int i = xo[r], j = yo[r];
+    while (i < xo[r + 1] || j < yo[r + 1]) {
+      const int a = i < xo[r + 1] ? xc[i] : INT32_MAX, b = j < yo[r + 1] ? yc[j] : INT32_MAX;
+      const bool tx = a <= b, ty = b <= a;
+      for (int k = 0; k < 9; ++k) {
+        const float xs = tx ? xv[9ll * i + k] : 0.0f, ys = ty ? yv[9ll * j + k] : 0.0f;
+        zv[9ll * o + k] = alpha * xs + beta * ys;
+      }
+      zc[o++] = tx ? a : b;
+      i += tx;
+      j += ty;
+    }
+    zo[r + 1] = o;
+  }
+  return o;
 }
 
 int oracle_max_threads(void) { return omp_get_max_threads(); }
