@@ -310,6 +310,67 @@ def frozen_step(soup, dx_np, q, dq, exclude, thickness, kappa, steps, warmup, de
     return out
 
 
+def bsr_assembly(S, x0, soup, dev, reps: int = 20):
+    """compute_A's sparse assembly on the workload's first state
+    (dynamic_contacts.py:290-297): the contact triplets to BSR, the same
+    pattern refilled, and A = h^2 H_c + K over the union with an elastic
+    pattern K (the tets' 16 vertex blocks each, random values)."""
+    import torch
+
+    from ipc_warp_amd import sparse
+
+    S.x.copy_(x0)
+    S.detect_collision()
+    b = torch.zeros_like(S.x)
+    trip = S.compute_collision_hessian(b)
+    nv = S.x.shape[0]
+    m = int(trip.rows.shape[0])
+    H = sparse.bsr_from_triplets(nv, nv, trip.rows, trip.cols, trip.vals)
+    tets = torch.from_numpy(soup.tets.astype(np.int32)).to(dev)
+    kr = tets[:, :, None].expand(-1, 4, 4).reshape(-1).contiguous()
+    kc = tets[:, None, :].expand(-1, 4, 4).reshape(-1).contiguous()
+    g = torch.Generator(device=dev).manual_seed(0)
+    K = sparse.bsr_from_triplets(nv, nv, kr, kc, torch.randn((kr.shape[0], 3, 3), device=dev, generator=g))
+    A = sparse.BsrMatrix(nv, nv, None, None, None)
+
+    def timed(fn):
+        for _ in range(3):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    build_ms = timed(lambda: sparse.bsr_set_from_triplets(H, trip.rows, trip.cols, trip.vals))
+    refill_ms = timed(lambda: H.refill(trip.vals))
+
+    def axpy():
+        A.offsets, A.columns, A.values = K.offsets, K.columns, K.values
+        sparse.bsr_axpy(H, A, 0.025 ** 2, 1.0)
+
+    axpy_ms = timed(axpy)
+    nnz_h, nnz_k = H.nnz_sync(), K.nnz_sync()
+    nnz_a = A.nnz_sync()
+    kept = int((trip.vals.reshape(m, 9) != 0).any(1).sum())
+    # algorithmic bytes: triplets read once (rows, cols, values: 44 B) and
+    # blocks written once (column + values: 40 B) plus the offsets; a refill
+    # reads the kept triplets' values and writes the block values; axpy reads
+    # both operands' blocks and writes the union's
+    alg = {"build": 44 * m + 40 * nnz_h + 4 * (nv + 1), "refill": 36 * kept + 36 * nnz_h,
+           "axpy": 40 * (nnz_h + nnz_k + nnz_a) + 12 * (nv + 1)}
+    ms = {"build": build_ms, "refill": refill_ms, "axpy": axpy_ms}
+    out = {"triplets": m, "kept_triplets": kept, "contact_blocks": nnz_h, "elastic_blocks": nnz_k,
+           "system_blocks": nnz_a}
+    for k in ms:
+        gbs = alg[k] / (ms[k] * 1e-3) / 1e9
+        out[k] = {"ms": ms[k], "algorithmic_bytes": alg[k], "achieved": gbs, "unit": "GB/s",
+                  "frac": gbs / HBM_PEAK_GBS}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -491,9 +552,11 @@ def main() -> int:
                                 "traffic": measured_traffic("hessian"), "avg_launch_ms": kernel_ms["hessian"],
                                 "algorithmic_bytes_per_launch": alg["hessian"]},
     }
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    if not args.no_extras:
+        result["bsr_assembly"] = bsr_assembly(S, X0[0], soup, dev)
     del S
 
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     if not args.no_extras:
         from ipc_warp_amd import scenes
         # the barrier Hessian kernel on a large synthetic contact stream (2^22 PT contacts)

@@ -333,6 +333,27 @@ int ipc_zero_f32(float* p, long long n, void* stream);
 int ipc_warmstart_features(const float* x, int n_vertices, const float* base, const int* ee, int n_ee, const int* pt,
                            int n_pt, float dhat, unsigned long long* slots, float* features, void* stream);
 
+/* ================================================== contact records ===
+ * The reference's Contacts.list records (XConstraint, contact.py:61-69):
+ * a1a2b1b2 int32[4] at byte 0, e0e1 int32[2] at 16, then l0, alpha, lam as
+ * `scalar` from byte 24: 36-byte records with fp32 scalars
+ * (scalar_types.py USE_FLOAT32), 48-byte records with fp64 scalars. */
+#define IPC_XCONSTRAINT_BYTES_F32 36
+#define IPC_XCONSTRAINT_BYTES_F64 48
+/* contacts[i] = records[i].a1a2b1b2 (int32 [n][4], 16-byte aligned), so a
+ * caller's own contact lists (contacts_new.list / contacts_pt.list) feed the
+ * Hessian, energy and contact-point entry points. The kernels take one l0
+ * for all contacts; when l0_mismatch (device int32, may be NULL) is given,
+ * the number of records whose l0 differs from `l0` is ADDED to it. */
+int ipc_contact_records_pack(const void* records, int record_bytes, int n, float l0, int* contacts, int* l0_mismatch,
+                             void* stream);
+/* The fields append writes (contact.py:163-183): records[i].a1a2b1b2 =
+ * contacts[i], .e0e1 = e0e1[i], .l0 = l0, .alpha = alpha[i] (append_ccd,
+ * contact.py:185-197) or 1e-6 when alpha is NULL (append); .lam is left
+ * untouched (the augmented-Lagrangian multiplier). */
+int ipc_contact_records_unpack(const int* contacts, const int* e0e1, int n, float l0, const double* alpha,
+                               void* records, int record_bytes, void* stream);
+
 /* ============================================== block-sparse assembly ===
  * BSR matrices of 3x3 fp32 blocks as warp.sparse keeps them: offsets int32
  * [n_rows + 1], columns int32 [nnz] sorted and distinct within each row,

@@ -63,3 +63,15 @@ def test_misaligned_pointers_are_refused_before_any_launch():
 def test_status_constant_matches_header():
     src = open(os.path.join(ROOT, "include", "ipc_warp_amd.h")).read()
     assert int(re.search(r"#define IPC_EINVAL \((-?\d+)\)", src).group(1)) == _lib.IPC_EINVAL
+
+
+def test_contact_record_layout_is_checked():
+    L = ipc_warp_amd.lib()
+    assert L.ipc_contact_records_pack(1 << 20, 40, 4, ctypes.c_float(1e-3), 1 << 20, None, None) == _lib.IPC_EINVAL
+    assert b"36" in L.ipc_last_error()
+    assert L.ipc_contact_records_unpack(1 << 20, 1 << 20, 4, ctypes.c_float(1e-3), None, 1 << 20, 12, None) == \
+        _lib.IPC_EINVAL
+    # misaligned output list
+    assert L.ipc_contact_records_pack(1 << 20, 36, 4, ctypes.c_float(1e-3), (1 << 20) + 4, None, None) == \
+        _lib.IPC_EINVAL
+    assert L.ipc_contact_records_pack(None, 36, 0, ctypes.c_float(1e-3), None, None, None) == 0
