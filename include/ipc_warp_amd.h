@@ -395,6 +395,36 @@ int ipc_bsr_axpy(ipc_bsr* bsr, int n_rows, const int* x_offsets, const int* x_co
                  float alpha, const int* y_offsets, const int* y_columns, const float* y_values, int y_nnz, float beta,
                  int* z_offsets, int* z_columns, float* z_values, int capacity, void* stream);
 
+/* ==================================================== affine bodies ===
+ * Contact reduction onto affine-body degrees of freedom
+ * (affine_body_dynamics.py:144-190; body b has q = [p, a0, a1, a2] and
+ * vertex i of body b sits at p + r_i.x a0 + r_i.y a1 + r_i.z a2). The
+ * reference writes 16 body triplets per vertex triplet and sorts them; this
+ * sums them in place. An ipc_abd is the call's workspace (one stream at a
+ * time). */
+typedef struct ipc_abd ipc_abd;
+int ipc_abd_create(ipc_abd** abd);
+int ipc_abd_destroy(ipc_abd* abd);
+/* Vertex triplets (rows, cols int32 [n], vals float [n][3][3], e.g. from
+ * ipc_collision_hessian) reduced like _reduce_contact_hessian: triplet
+ * (v_i, v_j, H) adds scale * c_a(v_i) c_c(v_j) H to block (4 b_i + a,
+ * 4 b_j + c), c_0 = 1, c_k = rest[v][k-1]; triplets out of range, all zero,
+ * or touching a fixed body (fixed int32 [n_bodies] != 0) add nothing. Output
+ * BSR over 4 n_bodies block rows: body pair (A, B) receiving a block owns all
+ * 16 blocks (4A + a, 4B + c), rows sorted by column; offsets int32
+ * [4 n_bodies + 1] always written, columns / values when 16 * pairs <=
+ * capacity. Blocks are summed in fp32 per lane and wave and in fp64 across
+ * waves, then scaled and rounded to fp32 (the cross-wave order is that of
+ * the atomics: results agree to fp64 rounding). When force (float [V][3],
+ * the vertex force b) is given, _reduce_contact_force's sums are ADDED to
+ * reduced_force (float [4 n_bodies][3], unscaled). Synchronises `stream`
+ * once (the pattern size). Returns 0, or the block count when it exceeds
+ * capacity (nothing else written). n_bodies <= 8192. */
+int ipc_abd_reduce_contacts(ipc_abd* abd, int n_bodies, const int* body, const float* rest, const int* fixed,
+                            int n_vertices, const int* rows, const int* cols, const float* vals, int n_triplets,
+                            const float* force, double scale, int* offsets, int* columns, float* values, int capacity,
+                            float* reduced_force, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -84,6 +84,9 @@ def lib():
         L.oracle_bsr_from_triplets.restype = c_int
         L.oracle_bsr_axpy.argtypes = [c_int, I32P, I32P, F32P, c_flt, I32P, I32P, F32P, c_flt, I32P, I32P, F32P]
         L.oracle_bsr_axpy.restype = c_int
+        L.oracle_abd_reduce.argtypes = [c_int, I32P, F32P, I32P, c_int, I32P, I32P, F32P, c_int, F32P, c_dbl, c_int,
+                                        I32P, I32P, F64P, F64P, F64P]
+        L.oracle_abd_reduce.restype = c_int
         _lib = L
     return _lib
 
@@ -385,3 +388,29 @@ def bsr_axpy(x, y, alpha=1.0, beta=1.0):
     nnz = lib().oracle_bsr_axpy(nr, _p(xo, I32P), _p(xc, I32P), _p(xv, F32P), alpha, _p(yo, I32P), _p(yc, I32P),
                                 _p(yv, F32P), beta, _p(zo, I32P), _p(zc, I32P), _p(zv, F32P))
     return zo, zc[:nnz].copy(), zv[:9 * nnz].reshape(nnz, 3, 3).copy()
+
+
+def abd_reduce(n_bodies, body, rest, fixed, rows, cols, vals, force=None, scale=1.0):
+    """affine_body_dynamics.py:144-190 in fp64: (offsets, columns, values
+    [nnz,3,3], abssum [nnz,3,3], reduced force [4 n_bodies, 3] or None) over
+    the structural body-pair pattern (16 DOF blocks per body pair)."""
+    body, fixed = _i32(body), _i32(fixed)
+    rest = _f32(rest).reshape(-1)
+    rows, cols = _i32(rows).reshape(-1), _i32(cols).reshape(-1)
+    vals = _f32(vals).reshape(-1)
+    nv, m = body.size, rows.size
+    cap = max(1, min(n_bodies * n_bodies, m))
+    offsets = np.zeros(4 * n_bodies + 1, np.int32)
+    columns = np.zeros(16 * cap, np.int32)
+    values = np.zeros(16 * cap * 9)
+    abssum = np.zeros(16 * cap * 9)
+    rforce = np.zeros(12 * n_bodies)
+    f = None if force is None else _f32(force).reshape(-1)
+    pairs = lib().oracle_abd_reduce(n_bodies, _p(body, I32P), _p(rest, F32P), _p(fixed, I32P), nv, _p(rows, I32P),
+                                    _p(cols, I32P), _p(vals, F32P), m, None if f is None else _p(f, F32P),
+                                    float(scale), cap, _p(offsets, I32P), _p(columns, I32P), _p(values, F64P),
+                                    _p(abssum, F64P), None if f is None else _p(rforce, F64P))
+    assert pairs >= 0
+    n = 16 * pairs
+    return (offsets, columns[:n].copy(), values[:9 * n].reshape(n, 3, 3), abssum[:9 * n].reshape(n, 3, 3),
+            None if f is None else rforce.reshape(-1, 3))

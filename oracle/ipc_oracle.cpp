@@ -6,7 +6,9 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
+#include <map>
 #include <unordered_map>
 #include <utility>
 #include <vector>
@@ -711,6 +713,79 @@ int oracle_bsr_axpy(int nr, const int* xo, const int* xc, const float* xv, float
     zo[r + 1] = o;
   }
   return o;
+}
+
+// affine_body_dynamics.py:144-190: _reduce_contact_hessian's 16 body blocks
+// per vertex triplet (scale * c_a(v_i) c_c(v_j) H, c_0 = 1, c_k = rest[v][k-1],
+// zero when a body is fixed) summed in fp64 per block over the structural
+// pattern the library uses (every body pair receiving a nonzero block owns
+// its 16 DOF blocks), plus the sum of |terms| per value (the summation error
+// bound), and _reduce_contact_force into rforce [4 nb][3] (fp64). Outputs
+// sized by the caller: offsets [4 nb + 1], columns / values / abssum for up to
+// cap_pairs body pairs. Returns the body-pair count (or -1 if above cap).
+int oracle_abd_reduce(int nb, const int* body, const float* rest, const int* fixed, int nv, const int* rows,
+                      const int* cols, const float* vals, int m, const float* force, double scale, int cap_pairs,
+                      int* offsets, int* columns, double* values, double* abssum, double* rforce) {
+  std::map<std::pair<int, int>, std::vector<double>> acc;
+  for (int t = 0; t < m; ++t) {
+    const int r = rows[t], c = cols[t];
+    if (r < 0 || r >= nv || c < 0 || c >= nv) continue;
+    const int bi = body[r], bj = body[c];
+    if (bi < 0 || bi >= nb || bj < 0 || bj >= nb || fixed[bi] || fixed[bj]) continue;
+    bool nz = false;
+    for (int k = 0; k < 9; ++k) nz |= !(vals[9ll * t + k] == 0.0f);
+    if (!nz) continue;
+    auto& v = acc[{bi, bj}];
+    if (v.empty()) v.assign(288, 0.0);
+    const double ca[4] = {1.0, rest[3ll * r], rest[3ll * r + 1], rest[3ll * r + 2]};
+    const double cc[4] = {1.0, rest[3ll * c], rest[3ll * c + 1], rest[3ll * c + 2]};
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b)
+        for (int k = 0; k < 9; ++k) {
+          const double term = scale * ca[a] * cc[b] * double(vals[9ll * t + k]);
+          v[(a * 4 + b) * 9 + k] += term;
+          v[144 + (a * 4 + b) * 9 + k] += std::fabs(term);
+        }
+  }
+  const int pairs = int(acc.size());
+  if (rforce) {
+    for (int k = 0; k < 12 * nb; ++k) rforce[k] = 0.0;
+    for (int i = 0; i < nv; ++i) {
+      const int b = body[i];
+      if (b < 0 || b >= nb || fixed[b]) continue;
+      const double cf[4] = {1.0, rest[3ll * i], rest[3ll * i + 1], rest[3ll * i + 2]};
+      for (int a = 0; a < 4; ++a)
+        for (int k = 0; k < 3; ++k) rforce[12ll * b + 3 * a + k] += cf[a] * double(force[3ll * i + k]);
+    }
+  }
+  if (pairs > cap_pairs) return -1;
+  // body-level rows (std::map iterates keys in (A, B) order)
+  std::vector<int> poff(nb + 1, 0), pcol;
+  std::vector<const std::vector<double>*> pv;
+  for (auto& kv : acc) {
+    ++poff[kv.first.first + 1];
+    pcol.push_back(kv.first.second);
+    pv.push_back(&kv.second);
+  }
+  for (int A = 0; A < nb; ++A) poff[A + 1] += poff[A];
+  for (int A = 0; A < nb; ++A) {
+    const int p0 = poff[A], len = poff[A + 1] - p0;
+    for (int a = 0; a < 4; ++a) {
+      const long long o = 16ll * p0 + 4ll * a * len;
+      offsets[4 * A + a] = int(o);
+      for (int j = 0; j < len; ++j)
+        for (int c = 0; c < 4; ++c) {
+          const long long blk = o + 4 * j + c;
+          columns[blk] = 4 * pcol[p0 + j] + c;
+          for (int k = 0; k < 9; ++k) {
+            values[9 * blk + k] = (*pv[p0 + j])[(a * 4 + c) * 9 + k];
+            abssum[9 * blk + k] = (*pv[p0 + j])[144 + (a * 4 + c) * 9 + k];
+          }
+        }
+    }
+  }
+  offsets[4 * nb] = 16 * pairs;
+  return pairs;
 }
 
 int oracle_max_threads(void) { return omp_get_max_threads(); }
