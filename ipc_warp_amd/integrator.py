@@ -47,6 +47,7 @@ class StepStats:
     warm_start_accepted: bool = False
     warm_start_alpha: float = 0.0
     warm_start_choice: int = -1
+    reference_criterion_iterations: int = 0
 
 
 class NeoHookeanTets:
