@@ -371,6 +371,59 @@ def bsr_assembly(S, x0, soup, dev, reps: int = 20):
     return out
 
 
+def abd_reduction(dev, seed: int, reps: int = 10):
+    """The affine-body contact reduction (affine_body_dynamics.py:144-190)
+    of the screw-and-nut contact Hessian: vertex triplets onto the screw's
+    12 degrees of freedom (nut fixed)."""
+    import torch
+
+    from ipc_warp_amd.abd import screw_and_nut_integrator
+
+    I = screw_and_nut_integrator(seed, dev)
+    S = I.contact
+    S.detect_collision()
+    force = torch.zeros_like(I.x)
+    trip = S.compute_collision_hessian(force)
+    m = int(trip.rows.shape[0])
+    args = (I.nb, I.body32, I.rest32, I.fixed, trip.rows, trip.cols, trip.vals, force, I.h * I.h)
+    for _ in range(2):
+        I.reducer.reduce(*args)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(reps):
+        I.reducer.reduce(*args)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    # algorithmic bytes: the triplets once (44 B) and the body/rest gathers of
+    # their two vertices (2 x 16 B); the force pass reads 28 B per vertex
+    alg = 76 * m + 28 * I.V
+    return {"triplets": m, "bodies": I.nb, "ms_per_call": ms, "includes": "pattern + one host readback + reduction",
+            "achieved": alg / (ms * 1e-3) / 1e9, "unit": "GB/s", "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "algorithmic_bytes": alg}
+
+
+def newton_screw(dev, seed: int, steps: int):
+    """Newton iterations per step of the screw turning in its nut (affine
+    bodies, nut fixed), from the previous state and from the inertial
+    extrapolation; both the 1e-6 relative-potential criterion and the
+    reference's max|dq| <= 0.05 h (time_integrator_base.py:392-396)."""
+    from ipc_warp_amd.abd import compare_newton
+
+    r = compare_newton(seed, steps, dev)
+    out = {"workload": "screw and nut as affine bodies: h=1e-2, motor -pi rad/s about y, nut fixed, affine "
+                       "stiffness 1e8, kappa 1e9, thickness 5e-4; Newton to 1e-6 relative potential decrease",
+           "seed": seed, "steps": steps}
+    for mode, v in r.items():
+        out[mode] = {"iters_per_step": float(np.mean(v["iterations"])),
+                     "reference_criterion_iters_per_step": float(np.mean(v["reference_criterion_iterations"])),
+                     "all_converged": all(v["converged"]), "max_edge_triangle_crossings": max(v["intersections"]),
+                     "iterations": v["iterations"]}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -624,6 +677,7 @@ def main() -> int:
                      "(h=1e-2) in a fixed 3-turn nut with 1e-3 radial clearance; 117,998 v, 353,092 e, 235,032 t; "
                      "thickness 5e-4, kappa 1e8, same-body pairs excluded; one state",
             **frozen_step(s3, dx3, q3, dq3, True, GRID_THICKNESS, GRID_KAPPA, 50, 5, dev, fixed_bodies=fb3))
+        result["screw_and_nut_abd_reduction"] = abd_reduction(dev, args.seed)
         sg, vel = scenes.icosphere_grid(seed=args.seed)
         nb = sg.n_bodies
         qg = np.zeros((nb, 4, 3), np.float32)
@@ -654,6 +708,7 @@ def main() -> int:
 
     if not args.no_newton:
         result["newton"] = newton_comparison(args.newton_seeds, args.newton_steps, dev)
+        result["newton_screw_and_nut"] = newton_screw(dev, args.seed, 8)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         result["cpu_baseline"], result["precision"] = cpu_baseline_and_precision(states[0][0], states[0][1], first,

@@ -10,15 +10,17 @@
 // and sorts them into BSR; with ~1e7 vertex triplets onto a handful of
 // bodies that is ~1.6e8 triplets falling into a few rows. Here the blocks
 // are summed where they are produced:
-//   mark     one bit per (body, body) pair that receives a nonzero block;
+//   mark     one bit per (body, body) pair that receives a nonzero block,
+//            each workgroup setting the distinct pairs of its triplets once;
 //   pattern  the pairs as a BSR over bodies (rows sorted) and its DOF-level
 //            BSR: body pair (A, B) owns the 16 blocks (4A + a, 4B + c);
 //   reduce   each wave walks a contiguous run of vertex triplets, each lane
 //            a fixed stride of it, summing its blocks in fp32 registers
 //            while its body pair stays the same; on a change the wave adds
-//            the lanes' sums per body pair (fixed-shape fp32 tree) into an
-//            fp64 accumulator with one atomic per value;
-//   finish   values = scale * accumulator rounded to fp32.
+//            the lanes' sums per body pair (lane order, fp32) into one of
+//            up to 64 fp64 accumulator copies (wave index modulo the copy
+//            count) with one atomic per value;
+//   finish   values = scale * (sum of the copies) rounded to fp32.
 // The forces reduce the same way per vertex (one body per vertex).
 #include <cstdio>
 
@@ -169,16 +171,40 @@ __device__ inline bool triplet_pair(const int* rows, const int* cols, const floa
   return nz;
 }
 
-__global__ void mark_kernel(const int* rows, const int* cols, const float* vals, int m, int nv, int nb, const int* body,
-                            const int* fixed, int words, unsigned* bits) {
-  const int t = blockIdx.x * kB + threadIdx.x;
-  if (t >= m) return;
-  int bi, bj;
-  if (!triplet_pair(rows, cols, vals, t, nv, nb, body, fixed, bi, bj)) return;
-  unsigned* w = bits + (long long)bi * words + (bj >> 5);
-  const unsigned bit = 1u << (bj & 31);
-  // most triplets find their bit already set: read before the atomic
-  if (!(__atomic_load_n(w, __ATOMIC_RELAXED) & bit)) atomicOr(w, bit);
+constexpr int kMarkPer = 64;     // triplets per thread of the mark kernel
+constexpr int kMarkSet = 256;    // LDS set of body pairs per workgroup
+
+// Bits of the body pairs receiving a nonzero block. Each workgroup collects
+// the distinct pairs of its kB * kMarkPer triplets in an LDS set and sets
+// each bit once; a pair that finds the set full sets its bit itself.
+__global__ __launch_bounds__(kB) void mark_kernel(const int* rows, const int* cols, const float* vals, int m, int nv,
+                                                  int nb, const int* body, const int* fixed, int words,
+                                                  unsigned* bits) {
+  __shared__ unsigned set[kMarkSet];
+  for (int i = threadIdx.x; i < kMarkSet; i += kB) set[i] = 0xffffffffu;
+  __syncthreads();
+  const long long base = (long long)blockIdx.x * kB * kMarkPer;
+  for (int k = 0; k < kMarkPer; ++k) {
+    const long long t = base + (long long)k * kB + threadIdx.x;
+    int bi, bj;
+    if (t >= m || !triplet_pair(rows, cols, vals, int(t), nv, nb, body, fixed, bi, bj)) continue;
+    const unsigned key = unsigned(bi) * unsigned(nb) + unsigned(bj);
+    unsigned h = (key * 2654435761u) >> 24;  // 8 bits: kMarkSet slots
+    bool placed = false;
+    for (int probe = 0; probe < 16 && !placed; ++probe) {
+      const unsigned slot = (h + probe) & (kMarkSet - 1);
+      const unsigned old = atomicCAS(&set[slot], 0xffffffffu, key);
+      placed = old == 0xffffffffu || old == key;
+    }
+    if (!placed) atomicOr(bits + (long long)bi * words + (bj >> 5), 1u << (bj & 31));
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kMarkSet; i += kB) {
+    const unsigned key = set[i];
+    if (key == 0xffffffffu) continue;
+    const int bi = int(key / unsigned(nb)), bj = int(key % unsigned(nb));
+    atomicOr(bits + (long long)bi * words + (bj >> 5), 1u << (bj & 31));
+  }
 }
 
 __global__ void row_count_kernel(const unsigned* bits, int nb, int words, int* cnt) {
@@ -244,12 +270,14 @@ __device__ inline float wave_sum(float v) {
   return v;
 }
 
-constexpr int kPad = 65;  // LDS row pitch of the flush table (conflict-free both ways)
+constexpr int kPad = 65;    // LDS row pitch of the flush table (conflict-free both ways)
+constexpr int kChunk = 48;  // values per LDS pass of a flush (3 passes of the 144)
 
 // Adds every pending lane sum into the accumulator, one body pair at a time:
-// the lanes of that pair write their 144 sums to LDS (others write zeros),
-// then lane k adds value k (and k + 64, k + 128) over the 64 lanes in lane
-// order and issues the fp64 atomic.
+// per pass of 48 values the lanes of that pair write their sums to LDS
+// (others write zeros), then lane k < 48 adds value k over the 64 lanes in
+// lane order and issues the fp64 atomic. `out` is this wave's accumulator
+// copy.
 __device__ void flush_pairs(int& cur_a, int cur_b, float* acc, int lane, const int* poff, const int* pcol,
                             float* sh, double* out) {
   while (true) {
@@ -259,16 +287,20 @@ __device__ void flush_pairs(int& cur_a, int cur_b, float* acc, int lane, const i
     const int A = __shfl(cur_a, leader), B = __shfl(cur_b, leader);
     const bool mine = cur_a == A && cur_b == B;
 #pragma unroll
-    for (int k = 0; k < 144; ++k) sh[k * kPad + lane] = mine ? acc[k] : 0.0f;
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    for (int k = lane; k < 144; k += 64) {
-      float sum = 0.0f;
-      for (int j = 0; j < 64; ++j) sum += sh[k * kPad + j];
-      const int blk = k / 9, e = k - 9 * blk;
-      const long long o = dof_block(poff, pcol, A, B, blk >> 2, blk & 3);
-      atomicAdd(out + 9 * o + e, static_cast<double>(sum));
+    for (int c = 0; c < 144 / kChunk; ++c) {
+#pragma unroll
+      for (int k = 0; k < kChunk; ++k) sh[k * kPad + lane] = mine ? acc[c * kChunk + k] : 0.0f;
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      if (lane < kChunk) {
+        float sum = 0.0f;
+        for (int j = 0; j < 64; ++j) sum += sh[lane * kPad + j];
+        const int k = c * kChunk + lane;
+        const int blk = k / 9, e = k - 9 * blk;
+        const long long o = dof_block(poff, pcol, A, B, blk >> 2, blk & 3);
+        atomicAdd(out + 9 * o + e, static_cast<double>(sum));
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     if (mine) {
       cur_a = -1;
 #pragma unroll
@@ -278,12 +310,16 @@ __device__ void flush_pairs(int& cur_a, int cur_b, float* acc, int lane, const i
 }
 
 // One wave per block, each a contiguous run of per_wave triplets.
-__global__ __launch_bounds__(64) void reduce_hessian_kernel(const int* rows, const int* cols, const float* vals, int m,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void reduce_hessian_kernel(const int* rows, const int* cols, const float* vals, int m,
                                                             int per_wave, int nv, int nb, const int* body,
                                                             const int* fixed, const float* rest, const int* poff,
-                                                            const int* pcol, double* out) {
-  __shared__ float sh[144 * kPad];
+                                                            const int* pcol, double* acc_reps, int reps,
+                                                            long long rep_stride) {
+  __shared__ float sh[kChunk * kPad];
   const int lane = threadIdx.x;
+  // waves spread their atomics over `reps` copies of the accumulator, so a
+  // body pair shared by every wave is not one chain of atomics per value
+  double* out = acc_reps + (blockIdx.x % reps) * rep_stride;
   const long long w0 = (long long)blockIdx.x * per_wave;
   if (w0 >= m) return;
   const long long w1 = w0 + per_wave < m ? w0 + per_wave : m;
@@ -320,9 +356,12 @@ __global__ __launch_bounds__(64) void reduce_hessian_kernel(const int* rows, con
   flush_pairs(cur_a, cur_b, acc, lane, poff, pcol, sh, out);
 }
 
-__global__ void finish_values_kernel(const double* acc, long long n, double scale, float* values) {
-  for (long long i = blockIdx.x * (long long)kB + threadIdx.x; i < n; i += (long long)gridDim.x * kB)
-    values[i] = static_cast<float>(scale * acc[i]);
+__global__ void finish_values_kernel(const double* acc, long long n, int reps, double scale, float* values) {
+  for (long long i = blockIdx.x * (long long)kB + threadIdx.x; i < n; i += (long long)gridDim.x * kB) {
+    double v = 0.0;
+    for (int r = 0; r < reps; ++r) v += acc[r * n + i];
+    values[i] = static_cast<float>(scale * v);
+  }
 }
 
 // Forces: vertex i of a free body adds (f, r.x f, r.y f, r.z f) to its
@@ -414,7 +453,7 @@ int ipc_abd_reduce_contacts(ipc_abd* w, int n_bodies, const int* body, const flo
   if (e != hipSuccess) return abd_hip(e, "abd_reduce_contacts: workspace");
   e = zero_u32(w->bits.as<unsigned>(), (long long)nb * words, s);
   if (e == hipSuccess && n_triplets > 0)
-    hipLaunchKernelGGL(mark_kernel, dim3(nblk(n_triplets, kB)), dim3(kB), 0, s, rows, cols, vals, n_triplets,
+    hipLaunchKernelGGL(mark_kernel, dim3(nblk(n_triplets, kB * kMarkPer)), dim3(kB), 0, s, rows, cols, vals, n_triplets,
                        n_vertices, nb, body, fixed, words, w->bits.as<unsigned>());
   if (e == hipSuccess) e = hipGetLastError();
   if (e == hipSuccess)
@@ -428,8 +467,11 @@ int ipc_abd_reduce_contacts(ipc_abd* w, int n_bodies, const int* body, const flo
   const int pairs = *w->host;
   const long long blocks = 16ll * pairs;
   if (blocks > 0x7fffffffll) return abd_fail(IPC_EINVAL, "abd_reduce_contacts: pattern too large");
+  // accumulator replicas: up to 64, within 64 MiB
+  long long reps = (64ll << 20) / (72 * (blocks > 0 ? blocks : 1));
+  reps = reps < 1 ? 1 : (reps > 64 ? 64 : reps);
   e = w->pcol.ensure(sizeof(int) * size_t(pairs > 0 ? pairs : 1));
-  if (e == hipSuccess) e = w->acc.ensure(sizeof(double) * size_t(blocks > 0 ? blocks : 1) * 9);
+  if (e == hipSuccess) e = w->acc.ensure(sizeof(double) * size_t(blocks > 0 ? blocks : 1) * 9 * size_t(reps));
   if (e == hipSuccess) e = w->facc.ensure(sizeof(double) * size_t(nb) * 12);
   if (e != hipSuccess) return abd_hip(e, "abd_reduce_contacts: accumulators");
   hipLaunchKernelGGL(row_fill_kernel, dim3(nblk(nb, kB)), dim3(kB), 0, s, w->bits.as<unsigned>(), nb, words,
@@ -454,7 +496,7 @@ int ipc_abd_reduce_contacts(ipc_abd* w, int n_bodies, const int* body, const flo
     if (e != hipSuccess) return abd_hip(e, "abd_reduce_contacts: force");
   }
   if (blocks == 0) return IPC_OK;
-  e = zero_f64(w->acc.as<double>(), 9 * blocks, s);
+  e = zero_f64(w->acc.as<double>(), 9 * blocks * reps, s);
   if (e == hipSuccess && n_triplets > 0) {
     // each wave a contiguous run of at least 64 triplets, about 4096 waves
     long long per = (n_triplets + 4095) / 4096;
@@ -462,13 +504,14 @@ int ipc_abd_reduce_contacts(ipc_abd* w, int n_bodies, const int* body, const flo
     const long long waves = (n_triplets + per - 1) / per;
     hipLaunchKernelGGL(reduce_hessian_kernel, dim3(int(waves)), dim3(64), 0, s, rows, cols, vals, n_triplets,
                        int(per), n_vertices, nb, body, fixed, rest, w->poff.as<int>(), w->pcol.as<int>(),
-                       w->acc.as<double>());
+                       w->acc.as<double>(), int(reps), 9 * blocks);
     e = hipGetLastError();
   }
   if (e == hipSuccess) {
     const long long n = 9 * blocks;
     const int g = nblk(n, kB) < 4096 ? nblk(n, kB) : 4096;
-    hipLaunchKernelGGL(finish_values_kernel, dim3(g), dim3(kB), 0, s, w->acc.as<double>(), n, scale, values);
+    hipLaunchKernelGGL(finish_values_kernel, dim3(g), dim3(kB), 0, s, w->acc.as<double>(), n, int(reps), scale,
+                       values);
     e = hipGetLastError();
   }
   return abd_hip(e, "abd_reduce_contacts: reduce");
