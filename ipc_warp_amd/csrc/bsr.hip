@@ -10,9 +10,10 @@
 //            with an atomic on the row's counter;
 //   scan     bucket starts = exclusive scan of the row counts;
 //   scatter  (column, triplet) into the row's bucket;
-//   sort     one wave per row: sort the bucket by (column, triplet) — in
-//            registers up to 64 entries, bitonic in LDS up to 1024, by rank
-//            in memory beyond — and count the distinct columns;
+//   sort     one wave per row: sort the bucket by (column, triplet) — by
+//            rank in registers up to 64 entries, by rank against the row
+//            staged in LDS up to 1024 — and count the distinct columns; the
+//            (rare) longer rows go to a segmented radix sort (hipCUB);
 //   scan     row offsets = exclusive scan of the distinct counts;
 //   emit     one wave per row: the first entry of each column run writes the
 //            column and the fp32 sum of its run, added in ascending triplet
@@ -21,6 +22,8 @@
 // Every step reads or writes each triplet a bounded number of times; the
 // result does not depend on the order the atomics hand out slots.
 #include <cstdio>
+
+#include <hipcub/hipcub.hpp>
 
 #include "../../include/ipc_warp_amd.h"
 #include "cull.hpp"
@@ -32,8 +35,10 @@ struct ipc_bsr {
   DevBuf cnt;      // int [n_rows + 1]: per-row kept triplets, then distinct columns
   DevBuf start;    // int [n_rows + 1]: bucket start of each row
   DevBuf slot;     // int [n_triplets]: slot in the row's bucket, -1 if dropped
-  DevBuf bucket;   // int2 [n_triplets]: (column, triplet) by row, unsorted
-  DevBuf sorted;   // int2 [n_triplets]: by row, then column, then triplet
+  DevBuf bucket;   // u64 [n_triplets]: column << 32 | triplet, by row, unsorted
+  DevBuf sorted;   // u64 [n_triplets]: by row, then column, then triplet
+  DevBuf big;      // int [3 * big_cap + 1]: rows longer than kSortCap (begin, end, row), count
+  DevBuf temp;     // segmented radix sort workspace
   DevBuf seg;      // int [capacity + 1]: first sorted entry of each block
   DevBuf part;     // scan partial sums
   DevBuf nnz;      // int [1]: blocks of the kept pattern
@@ -186,32 +191,30 @@ __global__ __launch_bounds__(kB) void bsr_count_kernel(const int* rows, const in
 }
 
 __global__ void bsr_scatter_kernel(const int* rows, const int* cols, const int* slot, const int* start, int m,
-                                   int2* bucket) {
+                                   unsigned long long* bucket) {
   const int t = blockIdx.x * kB + threadIdx.x;
   if (t >= m) return;
   const int s = slot[t];
-  if (s >= 0) bucket[start[rows[t]] + s] = make_int2(cols[t], t);
+  if (s >= 0) bucket[start[rows[t]] + s] = (static_cast<unsigned long long>(unsigned(cols[t])) << 32) | unsigned(t);
 }
 
-__device__ inline unsigned long long entry_key(int2 e) {
-  return (static_cast<unsigned long long>(unsigned(e.x)) << 32) | unsigned(e.y);
-}
-
-__device__ inline int2 key_entry(unsigned long long k) { return make_int2(int(k >> 32), int(unsigned(k))); }
+__device__ inline int key_col(unsigned long long k) { return int(k >> 32); }
+__device__ inline int key_triplet(unsigned long long k) { return int(unsigned(k)); }
 
 // Rank of K keys per lane (entries g + lane + 64 k of the row) among the
 // row's L keys, read one at a time at a wave-uniform index from `at`. A key
 // starts its column's run (a head) when its rank equals the number of keys
 // with a smaller column.
 template <int K, typename At>
-__device__ inline int rank_pass(At at, int L, int g, int lane, int b, const int2* bucket, int2* sorted) {
+__device__ inline int rank_pass(At at, int L, int g, int lane, int b, const unsigned long long* bucket,
+                                unsigned long long* sorted) {
   unsigned long long key[K];
   unsigned col[K];
   int rank[K], below[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int i = g + lane + 64 * k;
-    key[k] = i < L ? entry_key(bucket[b + i]) : ~0ull;
+    key[k] = i < L ? bucket[b + i] : ~0ull;
     col[k] = unsigned(key[k] >> 32);
     rank[k] = 0;
     below[k] = 0;
@@ -230,14 +233,15 @@ __device__ inline int rank_pass(At at, int L, int g, int lane, int b, const int2
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const bool in = g + lane + 64 * k < L;
-    if (in) sorted[b + rank[k]] = key_entry(key[k]);
+    if (in) sorted[b + rank[k]] = key[k];
     heads += __popcll(__ballot(in && rank[k] == below[k]));
   }
   return heads;
 }
 
 template <typename At>
-__device__ inline int rank_row(At at, int L, int lane, int b, const int2* bucket, int2* sorted) {
+__device__ inline int rank_row(At at, int L, int lane, int b, const unsigned long long* bucket,
+                               unsigned long long* sorted) {
   int heads = 0;
   int g = 0;
   for (; L - g > 192; g += 256) heads += rank_pass<4>(at, L, g, lane, b, bucket, sorted);
@@ -249,9 +253,12 @@ __device__ inline int rank_row(At at, int L, int lane, int b, const int2* bucket
 
 // One wave per row: the row's bucket sorted by (column, triplet) into
 // `sorted`, its distinct column count into rnnz. Keys are distinct (the
-// triplet index is), so each key's rank is its sorted position.
-__global__ __launch_bounds__(kB) void bsr_sort_rows_kernel(const int* start, int nr, const int2* bucket, int2* sorted,
-                                                           int* rnnz) {
+// triplet index is), so each key's rank is its sorted position. Rows longer
+// than kSortCap are listed in `big` (begin, end, row) for the segmented radix
+// sort instead.
+__global__ __launch_bounds__(kB) void bsr_sort_rows_kernel(const int* start, int nr, const unsigned long long* bucket,
+                                                           unsigned long long* sorted, int* rnnz, int* big,
+                                                           int big_cap) {
   __shared__ unsigned long long sk[kWaves][kSortCap];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -261,7 +268,7 @@ __global__ __launch_bounds__(kB) void bsr_sort_rows_kernel(const int* start, int
     int heads = 0;
     if (L == 0) {
     } else if (L <= 64) {
-      const unsigned long long k = lane < L ? entry_key(bucket[b + lane]) : ~0ull;
+      const unsigned long long k = lane < L ? bucket[b + lane] : ~0ull;
       const int khi = int(k >> 32), klo = int(unsigned(k));
       int rank = 0, below = 0;
       for (int q = 0; q < L; ++q) {
@@ -270,24 +277,50 @@ __global__ __launch_bounds__(kB) void bsr_sort_rows_kernel(const int* start, int
         rank += kq < k;
         below += unsigned(hi) < unsigned(khi);
       }
-      if (lane < L) sorted[b + rank] = key_entry(k);
+      if (lane < L) sorted[b + rank] = k;
       heads = __popcll(__ballot(lane < L && rank == below));
     } else if (L <= kSortCap) {
-      for (int i = lane; i < L; i += 64) s[i] = entry_key(bucket[b + i]);
+      for (int i = lane; i < L; i += 64) s[i] = bucket[b + i];
       __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
       heads = rank_row([&](int q) { return s[q]; }, L, lane, b, bucket, sorted);
       __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     } else {
-      heads = rank_row([&](int q) { return entry_key(bucket[b + q]); }, L, lane, b, bucket, sorted);
+      // at most n_triplets / (kSortCap + 1) rows are this long: a slot exists
+      if (lane == 0) {
+        const int i = atomicAdd(big + 3 * big_cap, 1);
+        big[i] = b;
+        big[big_cap + i] = b + L;
+        big[2 * big_cap + i] = r;
+      }
+      continue;
     }
     if (lane == 0) rnnz[r] = heads;
   }
 }
 
+// One workgroup per long row after its segmented sort: its distinct columns.
+__global__ __launch_bounds__(kB) void bsr_big_heads_kernel(const int* big, int big_cap,
+                                                           const unsigned long long* sorted, int* rnnz) {
+  __shared__ int sh[kB];
+  const int i = blockIdx.x;
+  if (i >= big[3 * big_cap]) return;
+  const int b = big[i], e = big[big_cap + i], r = big[2 * big_cap + i];
+  int heads = 0;
+  for (int q = b + threadIdx.x; q < e; q += kB) heads += q == b || key_col(sorted[q]) != key_col(sorted[q - 1]);
+  sh[threadIdx.x] = heads;
+  __syncthreads();
+  for (int d = kB / 2; d > 0; d >>= 1) {
+    if (threadIdx.x < d) sh[threadIdx.x] += sh[threadIdx.x + d];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) rnnz[r] = sh[0];
+}
+
 // One wave per row: the first entry of each column run writes its block's
 // column and where its run starts; the last row also closes the run list.
 __global__ __launch_bounds__(kB) void bsr_pattern_kernel(const int* start, const int* offsets, int nr,
-                                                         const int2* sorted, int cap, int* columns, int* seg,
+                                                         const unsigned long long* sorted, int cap, int* columns,
+                                                         int* seg,
                                                          int* nnz_out) {
   const int total = offsets[nr];
   if (blockIdx.x == 0 && threadIdx.x == 0) *nnz_out = total <= cap ? total : -1;
@@ -303,8 +336,8 @@ __global__ __launch_bounds__(kB) void bsr_pattern_kernel(const int* start, const
       int c = -1;
       bool head = false;
       if (i < L) {
-        c = sorted[b + i].x;
-        head = i == 0 || sorted[b + i - 1].x != c;
+        c = key_col(sorted[b + i]);
+        head = i == 0 || key_col(sorted[b + i - 1]) != c;
       }
       const unsigned long long hb = __ballot(head);
       if (head) {
@@ -323,7 +356,7 @@ __global__ __launch_bounds__(kB) void bsr_pattern_kernel(const int* start, const
 // LDS a window at a time by the whole wave, then each lane adds its block's
 // entries of the window in order, so every block is the fp32 sum of its
 // triplets in ascending triplet order from +0 however the windows fall.
-__global__ __launch_bounds__(kB) void bsr_sum_kernel(const int* nnz_p, const int* seg, const int2* sorted,
+__global__ __launch_bounds__(kB) void bsr_sum_kernel(const int* nnz_p, const int* seg, const unsigned long long* sorted,
                                                      const float* vals, float* values) {
   __shared__ float sv[kWaves][kWin * 9];
   const int lane = threadIdx.x & 63;
@@ -339,7 +372,7 @@ __global__ __launch_bounds__(kB) void bsr_sum_kernel(const int* nnz_p, const int
     for (int w0 = e0; w0 < e1; w0 += kWin) {
       const int cnt = e1 - w0 < kWin ? e1 - w0 : kWin;
       for (int k = lane; k < cnt; k += 64) {
-        const float* v = vals + 9ll * sorted[w0 + k].y;
+        const float* v = vals + 9ll * key_triplet(sorted[w0 + k]);
 #pragma unroll
         for (int j = 0; j < 9; ++j) s[k * 9 + j] = v[j];
       }
@@ -362,7 +395,7 @@ hipError_t launch_sum(const ipc_bsr* w, int max_blocks, const float* vals, float
   const int need = nblk(max_blocks, kB);
   const int g = need < 1 ? 1 : (need < 4096 ? need : 4096);
   hipLaunchKernelGGL(bsr_sum_kernel, dim3(g), dim3(kB), 0, s, w->nnz.as<int>(), w->seg.as<int>(),
-                     w->sorted.as<int2>(), vals, values);
+                     w->sorted.as<unsigned long long>(), vals, values);
   return hipGetLastError();
 }
 
@@ -435,7 +468,8 @@ int ipc_bsr_create(ipc_bsr** out) {
 
 int ipc_bsr_destroy(ipc_bsr* w) {
   if (!w) return IPC_OK;
-  for (DevBuf* d : {&w->cnt, &w->start, &w->slot, &w->bucket, &w->sorted, &w->seg, &w->part, &w->nnz}) d->release();
+  for (DevBuf* d : {&w->cnt, &w->start, &w->slot, &w->bucket, &w->sorted, &w->seg, &w->part, &w->nnz, &w->big, &w->temp})
+    d->release();
   if (w->host) hipHostFree(w->host);
   delete w;
   return IPC_OK;
@@ -457,8 +491,18 @@ int ipc_bsr_from_triplets(ipc_bsr* w, int n_rows, int n_cols, const int* rows, c
     if (e == hipSuccess) e = d->ensure(sizeof(int) * (size_t(nr) + 1));
   }
   if (e == hipSuccess) e = w->slot.ensure(sizeof(int) * size_t(m > 0 ? m : 1));
-  if (e == hipSuccess) e = w->bucket.ensure(sizeof(int2) * size_t(m > 0 ? m : 1));
-  if (e == hipSuccess) e = w->sorted.ensure(sizeof(int2) * size_t(m > 0 ? m : 1));
+  if (e == hipSuccess) e = w->bucket.ensure(sizeof(unsigned long long) * size_t(m > 0 ? m : 1));
+  if (e == hipSuccess) e = w->sorted.ensure(sizeof(unsigned long long) * size_t(m > 0 ? m : 1));
+  // rows longer than kSortCap go through a segmented radix sort; there are
+  // at most m / (kSortCap + 1) of them
+  const int big_cap = m / (kSortCap + 1);
+  size_t temp_bytes = 0;
+  if (e == hipSuccess) e = w->big.ensure(sizeof(int) * (3 * size_t(big_cap) + 1));
+  if (e == hipSuccess && big_cap > 0)
+    e = hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, temp_bytes, (const unsigned long long*)nullptr,
+                                                   (unsigned long long*)nullptr, m, big_cap, (const int*)nullptr,
+                                                   (const int*)nullptr, 0, 64, s);
+  if (e == hipSuccess && big_cap > 0) e = w->temp.ensure(temp_bytes);
   if (e == hipSuccess) e = w->seg.ensure(sizeof(int) * (size_t(capacity) + 1));
   if (e == hipSuccess) e = w->nnz.ensure(sizeof(int));
   if (e != hipSuccess) return bsr_hip(e, "bsr_from_triplets: workspace");
@@ -472,19 +516,33 @@ int ipc_bsr_from_triplets(ipc_bsr* w, int n_rows, int n_cols, const int* rows, c
   if (e == hipSuccess) e = exclusive_scan(cnt, nr + 1, start, w->part, s);
   if (e == hipSuccess && m > 0)
     hipLaunchKernelGGL(bsr_scatter_kernel, dim3(nblk(m, kB)), dim3(kB), 0, s, rows, cols, w->slot.as<int>(), start, m,
-                       w->bucket.as<int2>());
+                       w->bucket.as<unsigned long long>());
   const int rg = nr > 0 ? (nblk(nr, kWaves) < 65536 ? nblk(nr, kWaves) : 65536) : 0;
   if (e == hipSuccess) e = hipGetLastError();
   if (e == hipSuccess && nr > 0) {
     // cnt[nr] stays 0 from the fill: the row's distinct counts go to cnt[0..nr)
-    hipLaunchKernelGGL(bsr_sort_rows_kernel, dim3(rg), dim3(kB), 0, s, start, nr, w->bucket.as<int2>(),
-                       w->sorted.as<int2>(), cnt);
-    e = hipGetLastError();
+    int* big = w->big.as<int>();
+    e = fill_i32(big, 3 * big_cap + 1, 0, s);
+    if (e == hipSuccess)
+      hipLaunchKernelGGL(bsr_sort_rows_kernel, dim3(rg), dim3(kB), 0, s, start, nr,
+                         w->bucket.as<unsigned long long>(), w->sorted.as<unsigned long long>(), cnt, big, big_cap);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess && big_cap > 0) {
+      // unused slots are empty segments (begin = end = 0)
+      e = hipcub::DeviceSegmentedRadixSort::SortKeys(w->temp.p, temp_bytes, w->bucket.as<unsigned long long>(),
+                                                     w->sorted.as<unsigned long long>(), m, big_cap, big,
+                                                     big + big_cap, 0, 64, s);
+      if (e == hipSuccess)
+        hipLaunchKernelGGL(bsr_big_heads_kernel, dim3(big_cap), dim3(kB), 0, s, big, big_cap,
+                           w->sorted.as<unsigned long long>(), cnt);
+      if (e == hipSuccess) e = hipGetLastError();
+    }
   }
   if (e == hipSuccess) e = exclusive_scan(cnt, nr + 1, offsets, w->part, s);
   if (e == hipSuccess) {
     if (nr > 0) {
-      hipLaunchKernelGGL(bsr_pattern_kernel, dim3(rg), dim3(kB), 0, s, start, offsets, nr, w->sorted.as<int2>(),
+      hipLaunchKernelGGL(bsr_pattern_kernel, dim3(rg), dim3(kB), 0, s, start, offsets, nr,
+                         w->sorted.as<unsigned long long>(),
                          capacity, columns, w->seg.as<int>(), w->nnz.as<int>());
       e = hipGetLastError();
       if (e == hipSuccess && m > 0) e = launch_sum(w, capacity < m ? capacity : m, vals, values, s);

@@ -210,3 +210,24 @@ def test_empty_inputs(dev, oracle):
     assert _host(M)[0].tolist() == [0, 0, 0]
     Z = sparse.bsr_axpy(M, None, 2.0, 1.0)
     assert _host(Z)[0].tolist() == [0, 0, 0]
+
+
+def test_few_long_rows(dev, oracle):
+    """The affine-body pattern the reference assembles (16 body triplets per
+    vertex triplet onto a few body rows): 4 rows of ~500k triplets each over
+    32 columns, through the segmented radix sort of long rows."""
+    import time
+
+    rng = np.random.default_rng(9)
+    n = 2_000_000
+    rows = rng.integers(0, 4, n).astype(np.int32)
+    cols = rng.integers(0, 32, n).astype(np.int32)
+    vals = rng.normal(size=(n, 3, 3)).astype(np.float32)
+    r, c, v = _to_dev(dev, rows, cols, vals)
+    M = sparse.bsr_from_triplets(8, 32, r, c, v)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sparse.bsr_set_from_triplets(M, r, c, v)
+    torch.cuda.synchronize()
+    _say(f"bsr few long rows: {time.perf_counter() - t0:.4f} s")
+    _same(_host(M), oracle.bsr_from_triplets(8, 32, rows, cols, vals))
