@@ -96,7 +96,7 @@ def algorithmic_bytes(E, F, V, c):
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "rod_twist_hbm_traffic.json")
-STREAM_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "c2_contact_step_hbm_traffic.json")
+STREAM_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "kernel_streams_hbm_traffic.json")
 
 
 def measured_traffic(kernel: str, path: str = TRAFFIC_PROFILE, grid: int | None = None):
@@ -661,7 +661,8 @@ def main() -> int:
                 "kernel": f"collision_time_{kind}", "candidates": n_ccd, "ms": ms,
                 "candidates_per_s": n_ccd / (ms * 1e-3),
                 "hbm": {"achieved": gbs, "unit": "GB/s", "peak": HBM_PEAK_GBS, "frac": gbs / HBM_PEAK_GBS,
-                        "algorithmic_bytes_per_launch": 112 * n_ccd},
+                        "algorithmic_bytes_per_launch": 112 * n_ccd,
+                        "traffic": measured_traffic(f"toi_{kind}", STREAM_TRAFFIC_PROFILE)},
                 "fp64": None if flops is None else {"achieved": tflops, "unit": "TFLOP/s", "peak": FP64_PEAK_TFLOPS,
                                                     "frac": tflops / FP64_PEAK_TFLOPS, "flops_per_launch": flops,
                                                     "source": os.path.relpath(FP64_PROFILE, ROOT)},
