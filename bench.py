@@ -371,6 +371,62 @@ def bsr_assembly(S, x0, soup, dev, reps: int = 20):
     return out
 
 
+def newton_system_cg(S, x0, soup, dev, h: float = 0.025):
+    """The linear solve of one rod-twist Newton iteration (time_integrator_base.py:
+    500-507): A = M + h^2 (K + H_c) assembled with ipc_bsr_from_triplets (K the
+    PSD-projected Neo-Hookean tet Hessian of the harness, E = 1e5, nu = 0.4,
+    rho = 1000; H_c the contact Hessian) and solved by ipc_bsr_cg to the
+    reference's tolerance 1e-4, unpreconditioned (as the reference calls cg)
+    and with the Jacobi preconditioner, at most 5000 iterations (the barrier
+    terms make this system far too ill-conditioned for CG to reach 1e-4; the
+    reference's rod scenes use its direct solver)."""
+    import torch
+
+    from ipc_warp_amd import sparse
+    from ipc_warp_amd.integrator import NeoHookeanTets, lame
+
+    S.x.copy_(x0)
+    S.detect_collision()
+    trip = S.compute_collision_hessian(torch.zeros_like(S.x))
+    nv = S.x.shape[0]
+    T = torch.from_numpy(soup.tets.astype(np.int64)).to(dev)
+    mu, lam = lame(1e5, 0.4)
+    el = NeoHookeanTets(torch.from_numpy(soup.xcs).to(dev), T, mu, lam)
+    _, He = el.gradient_hessian(S.x.double())
+    m = torch.zeros(nv, dtype=torch.float64, device=dev)
+    m.index_add_(0, T.reshape(-1), (1000.0 * el.W / 4.0).repeat_interleave(4))
+    m[m == 0] = 1.0
+    Kb = (h * h * He).reshape(-1, 4, 3, 4, 3).permute(0, 1, 3, 2, 4).reshape(-1, 3, 3).float()
+    idx = torch.arange(nv, device=dev)
+    rows = torch.cat([T[:, :, None].expand(-1, 4, 4).reshape(-1), trip.rows.long(), idx]).int().contiguous()
+    cols = torch.cat([T[:, None, :].expand(-1, 4, 4).reshape(-1), trip.cols.long(), idx]).int().contiguous()
+    vals = torch.cat([Kb, h * h * trip.vals, m.float()[:, None, None] * torch.eye(3, device=dev)]).contiguous()
+    A = sparse.bsr_from_triplets(nv, nv, rows, cols, vals)
+    nnz = A.nnz_sync()
+    g = torch.Generator(device=dev).manual_seed(0)
+    b = torch.randn((nv, 3), device=dev, generator=g)
+    out = {"block_rows": nv, "blocks": nnz, "tol": 1e-4}
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    bn = float(torch.linalg.norm(b.double()))
+    for name, jac in (("plain", False), ("jacobi", True)):
+        x = torch.zeros_like(b)
+        sparse.cg(A, b, x, tol=1e-4, maxiter=20, jacobi=jac)
+        x.zero_()
+        torch.cuda.synchronize()
+        e0.record()
+        it, res, goal = sparse.cg(A, b, x, tol=1e-4, maxiter=5000, jacobi=jac)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        # per iteration: the product reads each block (36 B values + 4 B column)
+        # and gathers x; the vector passes move about 14 floats per unknown
+        alg = 40 * nnz + 4 * (nv + 1) + 14 * 4 * 3 * nv
+        out[name] = {"iterations": it, "converged": res <= goal, "relative_residual": res / bn, "ms": ms,
+                     "ms_per_iteration": ms / max(it, 1),
+                     "achieved_per_iteration": alg / (ms / max(it, 1) * 1e-3) / 1e9, "unit": "GB/s"}
+    return out
+
+
 def abd_reduction(dev, seed: int, reps: int = 10):
     """The affine-body contact reduction (affine_body_dynamics.py:144-190)
     of the screw-and-nut contact Hessian: vertex triplets onto the screw's
@@ -608,6 +664,7 @@ def main() -> int:
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     if not args.no_extras:
         result["bsr_assembly"] = bsr_assembly(S, X0[0], soup, dev)
+        result["newton_system_cg"] = newton_system_cg(S, X0[0], soup, dev)
     del S
 
     if not args.no_extras:

@@ -395,6 +395,30 @@ int ipc_bsr_axpy(ipc_bsr* bsr, int n_rows, const int* x_offsets, const int* x_co
                  float alpha, const int* y_offsets, const int* y_columns, const float* y_values, int y_nnz, float beta,
                  int* z_offsets, int* z_columns, float* z_values, int capacity, void* stream);
 
+/* y = alpha A x + beta y for a BSR matrix of n_rows block rows (x float
+ * [n_cols][3], y float [n_rows][3]; beta = 0 does not read y); warp.sparse's
+ * bsr_mv, the product inside the reference's cg. */
+int ipc_bsr_mv(int n_rows, const int* offsets, const int* columns, const float* values, const float* x, float* y,
+               float alpha, float beta, void* stream);
+/* Conjugate gradients on a symmetric positive (semi)definite BSR matrix
+ * (warp.optim.linear.cg as called at time_integrator_base.py:500-507 and
+ * affine_body_dynamics.py:580-592): solves A x = b from the initial guess in
+ * x (float [n_rows][3]), stopping when |r| <= tol |b| (checked on the host
+ * every check_every iterations, default 10, as the reference's cg does) or
+ * after maxiter iterations (<= 0: 3 n_rows); iterations run in whole groups
+ * of check_every rounded up to even, captured once as a HIP graph for the
+ * given pointers and size and replayed. jacobi != 0 preconditions with
+ * the inverse diagonal of the diagonal blocks. Vectors are fp32, dot
+ * products fp64 and summed in a fixed order. Writes the iteration count of
+ * the last check and |r| there. Synchronises `stream` at each check. An
+ * ipc_cg is the solver's workspace (one stream at a time). */
+typedef struct ipc_cg ipc_cg;
+int ipc_cg_create(ipc_cg** cg);
+int ipc_cg_destroy(ipc_cg* cg);
+int ipc_bsr_cg(ipc_cg* cg, int n_rows, const int* offsets, const int* columns, const float* values, const float* b,
+               float* x, float tol, int maxiter, int check_every, int jacobi, int* iterations, double* residual,
+               void* stream);
+
 /* ==================================================== affine bodies ===
  * Contact reduction onto affine-body degrees of freedom
  * (affine_body_dynamics.py:144-190; body b has q = [p, a0, a1, a2] and

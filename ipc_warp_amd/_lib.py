@@ -57,6 +57,11 @@ _SIGNATURES = {
     "ipc_warmstart_features": ([_vp, _c_int, _vp, _vp, _c_int, _vp, _c_int, _c_float, _vp, _vp, _vp], _c_int),
     "ipc_contact_records_pack": ([_vp, _c_int, _c_int, _c_float, _vp, _vp, _vp], _c_int),
     "ipc_contact_records_unpack": ([_vp, _vp, _c_int, _c_float, _vp, _vp, _c_int, _vp], _c_int),
+    "ipc_bsr_mv": ([_c_int, _vp, _vp, _vp, _vp, _vp, _c_float, _c_float, _vp], _c_int),
+    "ipc_cg_create": ([ctypes.POINTER(_vp)], _c_int),
+    "ipc_cg_destroy": ([_vp], _c_int),
+    "ipc_bsr_cg": ([_vp, _c_int, _vp, _vp, _vp, _vp, _vp, _c_float, _c_int, _c_int, _c_int, ctypes.POINTER(_c_int),
+                    ctypes.POINTER(_c_double), _vp], _c_int),
     "ipc_abd_create": ([ctypes.POINTER(_vp)], _c_int),
     "ipc_abd_destroy": ([_vp], _c_int),
     "ipc_abd_reduce_contacts": ([_vp, _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _vp, _c_double, _vp, _vp,

@@ -145,3 +145,56 @@ def bsr_axpy(x: BsrMatrix, y: BsrMatrix | None = None, alpha: float = 1.0, beta:
     # y's pattern changed: a refill over its old pattern would be wrong
     y._ws, y._n_triplets = None, None
     return y
+
+
+def bsr_mv(A: BsrMatrix, x: torch.Tensor, y: torch.Tensor | None = None, alpha: float = 1.0,
+           beta: float = 0.0) -> torch.Tensor:
+    """warp.sparse.bsr_mv: y = alpha A x + beta y (x, y float32 [n, 3])."""
+    if y is None:
+        y = torch.zeros((A.nrow, 3), dtype=torch.float32, device=x.device)
+        beta = 0.0
+    if x.shape[0] != A.ncol or y.shape[0] != A.nrow:
+        raise ValueError("bsr_mv: x / y do not match the matrix shape")
+    check(lib().ipc_bsr_mv(A.nrow, _need(A.offsets, torch.int32, "offsets"), A.columns.data_ptr(),
+                           A.values.data_ptr(), _need(x, torch.float32, "x", (3,)), _need(y, torch.float32, "y", (3,)),
+                           float(alpha), float(beta), stream_handle(x.device)), "bsr_mv")
+    return y
+
+
+class _CgWorkspace:
+    def __init__(self):
+        h = ctypes.c_void_p()
+        check(lib().ipc_cg_create(ctypes.byref(h)), "cg_create")
+        self.handle = h
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                lib().ipc_cg_destroy(h)
+            except Exception:  # interpreter shutdown: the process frees it
+                pass
+            self.handle = None
+
+
+_cg_ws: dict = {}
+
+
+def cg(A: BsrMatrix, b: torch.Tensor, x: torch.Tensor, tol: float = 1e-4, maxiter: int = 0, check_every: int = 10,
+       jacobi: bool = False):
+    """warp.optim.linear.cg(A, b, x, tol, maxiter) (time_integrator_base.py:
+    500-507): x (the initial guess) is overwritten with the solution; returns
+    (iterations, |r|, tol |b|) at the last convergence check."""
+    if b.shape != x.shape or x.shape[0] != A.nrow:
+        raise ValueError("cg: b and x must be float32 [n_rows, 3]")
+    dev = x.device
+    ws = _cg_ws.get(dev)
+    if ws is None:
+        ws = _cg_ws[dev] = _CgWorkspace()
+    it = ctypes.c_int(0)
+    res = ctypes.c_double(0.0)
+    check(lib().ipc_bsr_cg(ws.handle, A.nrow, _need(A.offsets, torch.int32, "offsets"), A.columns.data_ptr(),
+                           A.values.data_ptr(), _need(b, torch.float32, "b", (3,)), _need(x, torch.float32, "x", (3,)),
+                           float(tol), int(maxiter), int(check_every), int(bool(jacobi)), ctypes.byref(it),
+                           ctypes.byref(res), stream_handle(dev)), "bsr_cg")
+    return it.value, res.value, float(tol) * float(torch.linalg.norm(b.double()))
