@@ -602,6 +602,43 @@ __device__ __forceinline__ bool accept_pair(const QueryArgs& A, const QueryPrim&
   return tc.dist < A.thickness * 2.0f;
 }
 
+// Modes 0 and 1 test the distance of a candidate away from the traversal:
+// pair_candidate applies the id filters and orients the record exactly as
+// accept_pair does, and pair_distance_test evaluates accept_pair's distance
+// test from the record's vertex ids (the same arguments in the same order,
+// so the same result).
+template <int MODE>
+__device__ __forceinline__ bool pair_candidate(const QueryArgs& A, const QueryPrim& Q, int4 lv, int4& rec, int2& ids) {
+  ids = make_int2(Q.i, lv.w);
+  if (MODE == 0) {
+    const int j = lv.w;
+    const bool qlow = Q.i < j;
+    if (!A.self_query && !qlow) return false;
+    int a0 = qlow ? Q.q0 : lv.x, a1 = qlow ? Q.q1 : lv.y, b0 = qlow ? lv.x : Q.q0, b1 = qlow ? lv.y : Q.q1;
+    if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return false;
+    rec = make_int4(a0, a1, b0, b1);
+    ids = qlow ? make_int2(Q.i, j) : make_int2(j, Q.i);
+    return true;
+  }
+  int i = Q.i, t0 = lv.x, t1 = lv.y, t2 = lv.z;
+  if (t0 == i || t1 == i || t2 == i) return false;
+  rec = make_int4(i, t1, t0, t2);
+  return true;
+}
+
+template <int MODE>
+__device__ __forceinline__ bool pair_distance_test(const QueryArgs& A, int4 rec) {
+  if (MODE == 0) {
+    EdgeClosest ec = closest_point_edge_edge(ldx(A.x0, rec.x), ldx(A.x0, rec.y), ldx(A.x0, rec.z), ldx(A.x0, rec.w),
+                                             1.0e-6f);
+    return ec.dist < A.thickness * 2.0f;
+  }
+  TriClosest tc = closest_point_triangle(ldx(A.x0, rec.x), ldx(A.x0, rec.z), ldx(A.x0, rec.y), ldx(A.x0, rec.w));
+  return tc.dist < A.thickness * 2.0f;
+}
+
+constexpr int kCandQueue = 128;  // candidate pairs a wave holds for its dense distance tests
+
 // Eight lanes share one query. A stack entry (level L, mask, cbase) stands
 // for the level-L nodes picked by the mask bits c of one parent; node c's
 // children sit at slots cbase + 8 c + 0..7 of level L - 1 (cbase is an
@@ -611,16 +648,24 @@ __device__ __forceinline__ bool accept_pair(const QueryArgs& A, const QueryPrim&
 // group's ballot of overlapping children of a node becomes one new entry.
 // Every lane of a group computes and writes the same stack values, so each
 // lane only ever reads back its own LDS writes.
-struct QuerySmem {
+// Q = kCandQueue for launches with a DCD mode (each wave's candidate pairs
+// awaiting the distance test), 1 otherwise (no LDS spent on it).
+template <int Q>
+struct QuerySmemT {
   unsigned long long stack[kGroupsPerBlock][kMaskStack];
   int level_off[kMaxLevels], level_cnt[kMaxLevels];
   BlockStage stage;
+  int4 cand_rec[kQueryBlock / 64][Q];
+  int2 cand_ids[kQueryBlock / 64][Q];
 };
+using QuerySmem = QuerySmemT<1>;
+template <int MA, int MB>
+using QuerySmemFor = QuerySmemT<(MA == 0 || MA == 1 || MB == 0 || MB == 1) ? kCandQueue : 1>;
 
 // Workgroup `bid` of a query launch over A. With wcost, each wave stores
 // its number of steps in wcost[wave].
-template <int MODE>
-__device__ __forceinline__ void query_block(const QueryArgs& A, int bid, QuerySmem& sm, int* wcost) {
+template <int MODE, typename Smem>
+__device__ __forceinline__ void query_block(const QueryArgs& A, int bid, Smem& sm, int* wcost) {
   auto& stack = sm.stack;
   int* level_off = sm.level_off;
   int* level_cnt = sm.level_cnt;
@@ -688,6 +733,10 @@ __device__ __forceinline__ void query_block(const QueryArgs& A, int bid, QuerySm
     sp = 1;
   }
   int steps = 0;
+  constexpr bool kDense = (MODE == 0 || MODE == 1) && sizeof(sm.cand_rec[0]) >= sizeof(int4) * 64;
+  int4* cq_rec = sm.cand_rec[tid >> 6];
+  int2* cq_ids = sm.cand_ids[tid >> 6];
+  int qn = 0;  // queued candidates (wave-uniform)
   while (true) {
     ++steps;
     bool active = sp > 0;
@@ -806,16 +855,50 @@ __device__ __forceinline__ void query_block(const QueryArgs& A, int bid, QuerySm
         ++sp;
       }
     }
-    // leaf level: test and stage the pairs
+    // leaf level: test and stage the pairs. The DCD modes queue the
+    // candidates that pass the id filters and test their distances 64 at a
+    // time with every lane of the wave busy, instead of inside the step
+    // where only the lanes with a leaf hit work.
 #pragma unroll
     for (int p = 0; p < kPopNodes; ++p) {
-      bool want = false;
       int4 rec = make_int4(0, 0, 0, 0);
       int2 ids = make_int2(Q.i, cvid[p].w);
-      if (L == 1 && hit[p]) want = accept_pair<MODE>(A, Q, cvid[p], rec, ids);
-      stage_append(stage, out, want, rec, ids);
+      if (kDense) {
+        const bool cand = L == 1 && hit[p] && pair_candidate<MODE>(A, Q, cvid[p], rec, ids);
+        const unsigned long long cm = __ballot(cand);
+        if (cand) {
+          const int at_q = qn + __popcll(cm & ((1ull << lane) - 1ull));
+          cq_rec[at_q] = rec;
+          cq_ids[at_q] = ids;
+        }
+        qn += __popcll(cm);
+        if (qn >= 64) {
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+          const int k = qn - 64 + lane;
+          const int4 crec = cq_rec[k];
+          const int2 cids = cq_ids[k];
+          __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+          qn -= 64;
+          stage_append(stage, out, pair_distance_test<MODE>(A, crec), crec, cids);
+        }
+      } else {
+        bool want = false;
+        if (L == 1 && hit[p]) want = accept_pair<MODE>(A, Q, cvid[p], rec, ids);
+        stage_append(stage, out, want, rec, ids);
+      }
     }
     if (!__any(sp > 0)) break;
+  }
+  if (kDense) {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    const bool mine = lane < qn;
+    int4 crec = make_int4(0, 0, 0, 0);
+    int2 cids = make_int2(0, 0);
+    if (mine) {
+      crec = cq_rec[lane];
+      cids = cq_ids[lane];
+    }
+    stage_append(stage, out, mine && pair_distance_test<MODE>(A, crec), crec, cids);
   }
   if (wcost && lane == 0) wcost[tid >> 6] = steps;
   stage_flush(stage, out);
@@ -823,7 +906,7 @@ __device__ __forceinline__ void query_block(const QueryArgs& A, int bid, QuerySm
 
 template <int MODE>
 __global__ void __launch_bounds__(kQueryBlock) query_kernel(QueryArgs A) {
-  __shared__ QuerySmem sm;
+  __shared__ QuerySmemFor<MODE, MODE> sm;
   query_block<MODE>(A, xcd_local(blockIdx.x, gridDim.x), sm, nullptr);
 }
 
@@ -835,7 +918,7 @@ __global__ void __launch_bounds__(kQueryBlock) query_kernel(QueryArgs A) {
 template <int MA, int MB>
 __global__ void __launch_bounds__(kQueryBlock) query_pair_kernel(QueryArgs A, QueryArgs B, int blocks_a,
                                                                  const int* __restrict__ perm, int* cost) {
-  __shared__ QuerySmem sm;
+  __shared__ QuerySmemFor<MA, MB> sm;
   const int n = gridDim.x, raw = blockIdx.x;
   int c = raw < blocks_a ? xcd_local(raw, blocks_a) : blocks_a + xcd_local(raw - blocks_a, n - blocks_a);
   if (perm) {

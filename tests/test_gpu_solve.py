@@ -81,3 +81,26 @@ def test_cg_zero_rhs_and_warm_start(dev, system):
     b = sparse.bsr_mv(A, xs)
     it, res, goal = sparse.cg(A, b, xs.clone(), tol=1e-4)
     assert it == 0 and res <= goal
+
+
+def test_cg_graph_replays_are_identical(dev, system):
+    """The captured iterations replayed by repeated solves, with eager work
+    and synchronisations between them, give the same bits each time (the
+    sums are in a fixed order) as the first solve."""
+    A, ref, V = system
+    rng = np.random.default_rng(3)
+    b = torch.from_numpy(rng.normal(size=(V, 3)).astype(np.float32)).to(dev)
+    outs = []
+    for k in range(3):
+        x = torch.zeros_like(b)
+        it, res, goal = sparse.cg(A, b, x, tol=1e-5, maxiter=2000, jacobi=True)
+        torch.cuda.synchronize()
+        outs.append((it, res, x.cpu().numpy().copy()))
+        # eager work between replays
+        junk = torch.randn(1 << 20, device=dev).sum()
+        sparse.bsr_mv(A, b)
+        torch.cuda.synchronize()
+        assert torch.isfinite(junk)
+    for it, res, x in outs[1:]:
+        assert it == outs[0][0] and res == outs[0][1]
+        assert np.array_equal(x.view(np.uint32), outs[0][2].view(np.uint32))
