@@ -120,6 +120,17 @@ int ipc_collision_time_ee(const float* x0, const float* x1, int n_vertices, cons
                           unsigned long long* toi_min, void* stream);
 int ipc_collision_time_pt(const float* x0, const float* x1, int n_vertices, const int* quads, int n, double* toi,
                           unsigned long long* toi_min, void* stream);
+/* The reference's distance-based conservative advancement
+ * (conservative_ee_toi / conservative_pt_toi, ccd/ccd.py:86-163; defined
+ * there but not called on its path, ccd/ccd.py:388,421): per candidate, in
+ * FP64, steps of 0.9 * distance / (motion bound relative to the mean
+ * motion) until the distance drops below a tenth of the initial one or the
+ * steps reach 1. Same arguments and outputs as ipc_collision_time_ee/pt
+ * (quads EE (a0, a1, b0, b1), PT (p, t0, t1, t2)). */
+int ipc_conservative_time_ee(const float* x0, const float* x1, int n_vertices, const int* quads, int n, double* toi,
+                             unsigned long long* toi_min, void* stream);
+int ipc_conservative_time_pt(const float* x0, const float* x1, int n_vertices, const int* quads, int n, double* toi,
+                             unsigned long long* toi_min, void* stream);
 
 /* ===================================================== contact context ===
  * The device state ContactSolverBase owns (contact.py:444-521): topology

@@ -28,6 +28,8 @@ _SIGNATURES = {
     "ipc_contact_points_pt": ([_vp, _c_int, _vp, _c_int, _vp, _vp, _vp, _vp], _c_int),
     "ipc_collision_time_ee": ([_vp, _vp, _c_int, _vp, _c_int, _vp, _vp, _vp], _c_int),
     "ipc_collision_time_pt": ([_vp, _vp, _c_int, _vp, _c_int, _vp, _vp, _vp], _c_int),
+    "ipc_conservative_time_ee": ([_vp, _vp, _c_int, _vp, _c_int, _vp, _vp, _vp], _c_int),
+    "ipc_conservative_time_pt": ([_vp, _vp, _c_int, _vp, _c_int, _vp, _vp, _vp], _c_int),
     "ipc_ctx_create": ([ctypes.POINTER(_vp), _c_int, _vp, _c_int, _vp, _c_int, _vp], _c_int),
     "ipc_ctx_destroy": ([_vp], _c_int),
     "ipc_detect_collision_ee": ([_vp, _vp, _c_float, _c_float, _c_int, _vp, _vp, _c_int, _vp, _vp], _c_int),

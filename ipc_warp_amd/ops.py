@@ -182,3 +182,16 @@ def collision_time(kind, x0, x1, quads, n, toi=None, toi_min=None):
     check(fn(_need(x0, torch.float32, "x0", (3,)), _need(x1, torch.float32, "x1", (3,)), x0.shape[0],
              _rows(quads, n, "quads"), int(n), _opt(toi, torch.float64, "toi"),
              _opt(toi_min, torch.int64, "toi_min"), stream_handle(x0.device)), f"collision_time_{kind}")
+
+
+def conservative_time(kind: str, x0, x1, quads, n, toi=None, toi_min=None):
+    """conservative_ee_toi / conservative_pt_toi (ccd/ccd.py:86-163) per
+    candidate: toi float64 [>=n] (optional), toi_min from new_toi_min()
+    (optional, folded in as the minimum)."""
+    fn = lib().ipc_conservative_time_ee if kind == "ee" else lib().ipc_conservative_time_pt
+    if x1.shape != x0.shape:
+        raise ValueError("x1 must have the shape of x0")
+    _vec_len(toi, n, "toi")
+    check(fn(_need(x0, torch.float32, "x0", (3,)), _need(x1, torch.float32, "x1", (3,)), x0.shape[0],
+             _rows(quads, n, "quads"), int(n), _opt(toi, torch.float64, "toi"), _opt(toi_min, torch.int64, "toi_min"),
+             stream_handle(x0.device)), "conservative_time")

@@ -87,6 +87,11 @@ def lib():
         L.oracle_abd_reduce.argtypes = [c_int, I32P, F32P, I32P, c_int, I32P, I32P, F32P, c_int, F32P, c_dbl, c_int,
                                         I32P, I32P, F64P, F64P, F64P]
         L.oracle_abd_reduce.restype = c_int
+        L.oracle_conservative_time.argtypes = [c_int, c_int, c_int, F32P, F32P, I32P, F64P]
+        L.oracle_segment_distance_sq.argtypes = [F64P, F64P, F64P, F64P]
+        L.oracle_segment_distance_sq.restype = c_dbl
+        L.oracle_triangle_distance_sq.argtypes = [F64P, F64P, F64P, F64P]
+        L.oracle_triangle_distance_sq.restype = c_dbl
         _lib = L
     return _lib
 
@@ -414,3 +419,24 @@ def abd_reduce(n_bodies, body, rest, fixed, rows, cols, vals, force=None, scale=
     n = 16 * pairs
     return (offsets, columns[:n].copy(), values[:9 * n].reshape(n, 3, 3), abssum[:9 * n].reshape(n, 3, 3),
             None if f is None else rforce.reshape(-1, 3))
+
+
+def conservative_time(kind, x0, x1, quads, double=True):
+    """ccd/ccd.py:86-163 per candidate quadruple (float64 array)."""
+    x0, x1 = _f32(x0).reshape(-1), _f32(x1).reshape(-1)
+    q = _i32(quads).reshape(-1)
+    n = q.size // 4
+    out = np.zeros(n)
+    lib().oracle_conservative_time(int(kind == "ee"), int(double), n, _p(x0, F32P), _p(x1, F32P), _p(q, I32P),
+                                   _p(out, F64P))
+    return out
+
+
+def segment_distance_sq(a0, a1, b0, b1):
+    a = [_f64(v) for v in (a0, a1, b0, b1)]
+    return lib().oracle_segment_distance_sq(*[_p(v, F64P) for v in a])
+
+
+def triangle_distance_sq(p, a, b, c):
+    a = [_f64(v) for v in (p, a, b, c)]
+    return lib().oracle_triangle_distance_sq(*[_p(v, F64P) for v in a])

@@ -788,6 +788,39 @@ int oracle_abd_reduce(int nb, const int* body, const float* rest, const int* fix
   return pairs;
 }
 
+// ccd/ccd.py:86-163 per candidate (quads EE (a0, a1, b0, b1), PT (p, t0, t1, t2)).
+void oracle_conservative_time(int is_ee, int use_double, int n, const float* x0, const float* x1, const int* quads,
+                              double* out) {
+  for (int i = 0; i < n; ++i) {
+    const int* q = quads + 4ll * i;
+    if (use_double) {
+      V3<double> x[4], d[4];
+      for (int k = 0; k < 4; ++k) {
+        x[k] = load3<double>(x0, q[k]);
+        d[k] = load3<double>(x1, q[k]) - x[k];
+      }
+      out[i] = conservative_toi<double>(is_ee != 0, x, d);
+    } else {
+      V3<float> x[4], d[4];
+      for (int k = 0; k < 4; ++k) {
+        x[k] = load3<float>(x0, q[k]);
+        d[k] = load3<float>(x1, q[k]) - x[k];
+      }
+      out[i] = conservative_toi<float>(is_ee != 0, x, d);
+    }
+  }
+}
+
+double oracle_segment_distance_sq(const double* a0, const double* a1, const double* b0, const double* b1) {
+  return ca_segment_distance_sq(mk(a0[0], a0[1], a0[2]), mk(a1[0], a1[1], a1[2]), mk(b0[0], b0[1], b0[2]),
+                                mk(b1[0], b1[1], b1[2]));
+}
+
+double oracle_triangle_distance_sq(const double* p, const double* a, const double* b, const double* c) {
+  return ca_triangle_distance_sq(mk(p[0], p[1], p[2]), mk(a[0], a[1], a[2]), mk(b[0], b[1], b[2]),
+                                 mk(c[0], c[1], c[2]));
+}
+
 int oracle_max_threads(void) { return omp_get_max_threads(); }
 
 }  // extern "C"

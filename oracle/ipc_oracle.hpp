@@ -741,4 +741,110 @@ T collision_time(bool is_ee, bool relative, const V3<T> p0[4], const V3<T> p1[4]
   return T(1);
 }
 
+
+// ccd/ccd.py:7-163: the distance-based conservative advancement alternates
+// (restated from the reference's text; T = float reproduces its fp32 default,
+// T = double the library's FP64 evaluation).
+template <typename T>
+T ca_clamp01(T v) { return v < T(0) ? T(0) : (v > T(1) ? T(1) : v); }
+
+template <typename T>
+T ca_segment_distance_sq(V3<T> a0, V3<T> a1, V3<T> b0, V3<T> b1) {
+  const V3<T> da = a1 - a0, db = b1 - b0, r = a0 - b0;
+  const T aa = dot(da, da), bb = dot(db, db), f = dot(db, r);
+  T s = T(0), t = T(0);
+  if (aa <= T(1e-20) && bb <= T(1e-20)) return dot(r, r);
+  if (aa <= T(1e-20)) {
+    t = ca_clamp01(f / bb);
+  } else if (bb <= T(1e-20)) {
+    s = ca_clamp01(-dot(da, r) / aa);
+  } else {
+    const T ab = dot(da, db), c = dot(da, r);
+    const T denom = aa * bb - ab * ab;
+    if (denom > T(1e-20)) s = ca_clamp01((ab * f - c * bb) / denom);
+    t = (ab * s + f) / bb;
+    if (t < T(0)) {
+      t = T(0);
+      s = ca_clamp01(-c / aa);
+    } else if (t > T(1)) {
+      t = T(1);
+      s = ca_clamp01((ab - c) / aa);
+    }
+  }
+  const V3<T> d = (a0 + s * da) - (b0 + t * db);
+  return dot(d, d);
+}
+
+template <typename T>
+T ca_triangle_distance_sq(V3<T> p, V3<T> a, V3<T> b, V3<T> c) {
+  const V3<T> ab = b - a, ac = c - a, ap = p - a;
+  const T d1 = dot(ab, ap), d2 = dot(ac, ap);
+  if (d1 <= T(0) && d2 <= T(0)) return dot(ap, ap);
+  const V3<T> bp = p - b;
+  const T d3 = dot(ab, bp), d4 = dot(ac, bp);
+  if (d3 >= T(0) && d4 <= d3) return dot(bp, bp);
+  const T vc = d1 * d4 - d3 * d2;
+  if (vc <= T(0) && d1 >= T(0) && d3 <= T(0)) {
+    const T v = d1 / (d1 - d3);
+    const V3<T> e = p - (a + v * ab);
+    return dot(e, e);
+  }
+  const V3<T> cp = p - c;
+  const T d5 = dot(ab, cp), d6 = dot(ac, cp);
+  if (d6 >= T(0) && d5 <= d6) return dot(cp, cp);
+  const T vb = d5 * d2 - d1 * d6;
+  if (vb <= T(0) && d2 >= T(0) && d6 <= T(0)) {
+    const T w = d2 / (d2 - d6);
+    const V3<T> e = p - (a + w * ac);
+    return dot(e, e);
+  }
+  const T va = d3 * d6 - d5 * d4;
+  if (va <= T(0) && d4 >= d3 && d5 >= d6) {
+    const T w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    const V3<T> e = p - (b + w * (c - b));
+    return dot(e, e);
+  }
+  const T denom = va + vb + vc;
+  if (std::fabs(denom) <= T(1e-20)) {
+    const T dab = ca_segment_distance_sq(p, p, a, b), dac = ca_segment_distance_sq(p, p, a, c),
+            dbc = ca_segment_distance_sq(p, p, b, c);
+    return std::min(dab, std::min(dac, dbc));
+  }
+  const T inv = T(1) / denom;
+  const T v = vb * inv, w = vc * inv;
+  const V3<T> e = p - (a + v * ab + w * ac);
+  return dot(e, e);
+}
+
+template <typename T>
+T ca_distance(bool ee, const V3<T>* x) {
+  return ee ? std::sqrt(ca_segment_distance_sq(x[0], x[1], x[2], x[3]))
+            : std::sqrt(ca_triangle_distance_sq(x[0], x[1], x[2], x[3]));
+}
+
+template <typename T>
+T conservative_toi(bool ee, V3<T> x[4], V3<T> d[4]) {
+  const V3<T> mean = (((d[0] + d[1]) + d[2]) + d[3]) * T(0.25);
+  for (int k = 0; k < 4; ++k) d[k] = d[k] - mean;
+  T max_disp;
+  if (ee)
+    max_disp = std::sqrt(std::max(dot(d[0], d[0]), dot(d[1], d[1]))) + std::sqrt(std::max(dot(d[2], d[2]), dot(d[3], d[3])));
+  else
+    max_disp = std::sqrt(dot(d[0], d[0])) + std::sqrt(std::max(dot(d[1], d[1]), std::max(dot(d[2], d[2]), dot(d[3], d[3]))));
+  if (max_disp <= T(1e-20)) return T(1);
+  T distance = ca_distance(ee, x);
+  const T gap = T(0.1) * distance;
+  T toi = T(0);
+  for (int it = 0; it < 100; ++it) {
+    const T step = T(0.9) * distance / max_disp;
+    if (step <= T(1e-15)) return toi;
+    for (int k = 0; k < 4; ++k) x[k] = x[k] + step * d[k];
+    distance = ca_distance(ee, x);
+    if (toi > T(0) && distance < gap) return toi;
+    toi += step;
+    if (toi >= T(1)) return T(1);
+  }
+  return std::min(toi, T(1));
+}
+
 }  // namespace ipco
