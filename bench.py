@@ -427,6 +427,42 @@ def newton_system_cg(S, x0, soup, dev, h: float = 0.025):
     return out
 
 
+def tet_elasticity(soup, x0, dev, reps: int = 20):
+    """The rod twist's elastic tets (460,800; fem/fem.py:138-203 and
+    time_integrator_base.py:124-137) at the workload's first state: the
+    stable Neo-Hookean Hessian into triplets with the forces, and the
+    energy. Algorithmic bytes per tet: tet 16 B, 4 vertices 48 B, Bm 36 B,
+    W 4 B read; 16 triplets 704 B and 4 forces 48 B written."""
+    import torch
+
+    from ipc_warp_amd import ops
+
+    T = torch.from_numpy(soup.tets.astype(np.int32)).to(dev)
+    n = int(T.shape[0])
+    Bm, W = ops.tet_rest(torch.from_numpy(soup.xcs).to(dev), T)
+    rows = torch.empty(16 * n, dtype=torch.int32, device=dev)
+    cols = torch.empty_like(rows)
+    vals = torch.empty((16 * n, 3, 3), dtype=torch.float32, device=dev)
+    b = torch.zeros_like(x0)
+    e = torch.zeros(1, dtype=torch.float64, device=dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    out = {"tets": n}
+    for name, fn, alg in (("hessian", lambda: ops.tet_hessian(x0, T, Bm, W, 2.0e6, 2.0e7, rows, cols, vals, b), 856),
+                          ("energy", lambda: ops.tet_energy(x0, T, Bm, W, 2.0e6, 2.0e7, None, e), 104)):
+        for _ in range(3):
+            fn()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        gbs = alg * n / (ms * 1e-3) / 1e9
+        out[name] = {"ms": ms, "tets_per_s": n / (ms * 1e-3), "achieved": gbs, "unit": "GB/s",
+                     "frac": gbs / HBM_PEAK_GBS, "algorithmic_bytes": alg * n}
+    return out
+
+
 def abd_reduction(dev, seed: int, reps: int = 10):
     """The affine-body contact reduction (affine_body_dynamics.py:144-190)
     of the screw-and-nut contact Hessian: vertex triplets onto the screw's
@@ -665,6 +701,7 @@ def main() -> int:
     if not args.no_extras:
         result["bsr_assembly"] = bsr_assembly(S, X0[0], soup, dev)
         result["newton_system_cg"] = newton_system_cg(S, X0[0], soup, dev)
+        result["tet_elasticity"] = tet_elasticity(soup, X0[0], dev)
     del S
 
     if not args.no_extras:

@@ -430,6 +430,25 @@ int ipc_bsr_cg(ipc_cg* cg, int n_rows, const int* offsets, const int* columns, c
                float* x, float tol, int maxiter, int check_every, int jacobi, int* iterations, double* residual,
                void* stream);
 
+/* ================================================= elastic tetrahedra ===
+ * Stable Neo-Hookean tets (fem/snh.py:1-84): tets int32 [n][4] (16-byte
+ * aligned), rest shapes Bm float [n][3][3] (the inverse of Dm = [x0 - x3,
+ * x1 - x3, x2 - x3]) and volumes W float [n]. mu / lam are the Lamé
+ * parameters the reference keeps as module constants (fem/params.py:7),
+ * reparametrised inside as fem/snh.py:6-12 does. */
+/* compute_Dm (fem/fem.py:21-34). */
+int ipc_tet_rest(const float* xcs, int n_vertices, const int* tets, int n_tets, float* Bm, float* W, void* stream);
+/* tet_kernel_sparse (fem/fem.py:138-203): block (offset + e) * 16 + 4 i + j
+ * = d(-f_i)/dx_j of tet e (row tets[e][i], column tets[e][j]), the elastic
+ * force f added into b (float [V][3], may be NULL). */
+int ipc_tet_hessian(const float* x, int n_vertices, const int* tets, int n_tets, const float* Bm, const float* W,
+                    float mu, float lam, int offset, int* rows, int* cols, float* vals, float* b, void* stream);
+/* compute_Psi (time_integrator_base.py:124-137): psi[e] = W[e] psi(F_e)
+ * (float [n], may be NULL); their sum ADDED to *energy (double, device,
+ * may be NULL). */
+int ipc_tet_energy(const float* x, int n_vertices, const int* tets, int n_tets, const float* Bm, const float* W,
+                   float mu, float lam, float* psi, double* energy, void* stream);
+
 /* ==================================================== affine bodies ===
  * Contact reduction onto affine-body degrees of freedom
  * (affine_body_dynamics.py:144-190; body b has q = [p, a0, a1, a2] and

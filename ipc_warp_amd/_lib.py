@@ -64,6 +64,10 @@ _SIGNATURES = {
     "ipc_cg_destroy": ([_vp], _c_int),
     "ipc_bsr_cg": ([_vp, _c_int, _vp, _vp, _vp, _vp, _vp, _c_float, _c_int, _c_int, _c_int, ctypes.POINTER(_c_int),
                     ctypes.POINTER(_c_double), _vp], _c_int),
+    "ipc_tet_rest": ([_vp, _c_int, _vp, _c_int, _vp, _vp, _vp], _c_int),
+    "ipc_tet_hessian": ([_vp, _c_int, _vp, _c_int, _vp, _vp, _c_float, _c_float, _c_int, _vp, _vp, _vp, _vp, _vp],
+                        _c_int),
+    "ipc_tet_energy": ([_vp, _c_int, _vp, _c_int, _vp, _vp, _c_float, _c_float, _vp, _vp, _vp], _c_int),
     "ipc_abd_create": ([ctypes.POINTER(_vp)], _c_int),
     "ipc_abd_destroy": ([_vp], _c_int),
     "ipc_abd_reduce_contacts": ([_vp, _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _vp, _c_double, _vp, _vp,

@@ -195,3 +195,36 @@ def conservative_time(kind: str, x0, x1, quads, n, toi=None, toi_min=None):
     check(fn(_need(x0, torch.float32, "x0", (3,)), _need(x1, torch.float32, "x1", (3,)), x0.shape[0],
              _rows(quads, n, "quads"), int(n), _opt(toi, torch.float64, "toi"), _opt(toi_min, torch.int64, "toi_min"),
              stream_handle(x0.device)), "conservative_time")
+
+
+def tet_rest(xcs, tets):
+    """compute_Dm (fem/fem.py:21-34): (Bm float32 [n,3,3], W float32 [n])."""
+    n = int(tets.shape[0])
+    Bm = torch.empty((n, 3, 3), dtype=torch.float32, device=xcs.device)
+    W = torch.empty(n, dtype=torch.float32, device=xcs.device)
+    check(lib().ipc_tet_rest(_need(xcs, torch.float32, "xcs", (3,)), xcs.shape[0], _need(tets, torch.int32, "tets", (4,)),
+                             n, Bm.data_ptr(), W.data_ptr(), stream_handle(xcs.device)), "tet_rest")
+    return Bm, W
+
+
+def tet_hessian(x, tets, Bm, W, mu, lam, rows, cols, vals, b=None, offset=0):
+    """tet_kernel_sparse (fem/fem.py:138-203) into triplets at `offset`."""
+    n = int(tets.shape[0])
+    need = 16 * (int(offset) + n)
+    if rows.numel() < need or cols.numel() < need or vals.shape[0] < need:
+        raise ValueError(f"triplet arrays hold fewer than {need} blocks")
+    check(lib().ipc_tet_hessian(_need(x, torch.float32, "x", (3,)), x.shape[0], _need(tets, torch.int32, "tets", (4,)), n,
+                                _need(Bm, torch.float32, "Bm", (3, 3)), _need(W, torch.float32, "W"), float(mu),
+                                float(lam), int(offset), _need(rows, torch.int32, "rows"), _need(cols, torch.int32, "cols"),
+                                _need(vals, torch.float32, "vals", (3, 3)), _opt(b, torch.float32, "b", (3,)),
+                                stream_handle(x.device)), "tet_hessian")
+
+
+def tet_energy(x, tets, Bm, W, mu, lam, psi=None, energy=None):
+    """compute_Psi (time_integrator_base.py:124-137): per-tet W psi(F) into
+    psi and their sum added into energy (float64 [1])."""
+    n = int(tets.shape[0])
+    check(lib().ipc_tet_energy(_need(x, torch.float32, "x", (3,)), x.shape[0], _need(tets, torch.int32, "tets", (4,)), n,
+                               _need(Bm, torch.float32, "Bm", (3, 3)), _need(W, torch.float32, "W"), float(mu),
+                               float(lam), _opt(psi, torch.float32, "psi"), _opt(energy, torch.float64, "energy"),
+                               stream_handle(x.device)), "tet_energy")

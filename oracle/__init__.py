@@ -88,6 +88,9 @@ def lib():
                                         I32P, I32P, F64P, F64P, F64P]
         L.oracle_abd_reduce.restype = c_int
         L.oracle_conservative_time.argtypes = [c_int, c_int, c_int, F32P, F32P, I32P, F64P]
+        L.oracle_tet_rest.argtypes = [F32P, I32P, c_int, F32P, F32P]
+        L.oracle_tet_hessian.argtypes = [F32P, I32P, c_int, F32P, F32P, c_flt, c_flt, I32P, I32P, F32P, F32P]
+        L.oracle_tet_energy.argtypes = [F32P, I32P, c_int, F32P, F32P, c_flt, c_flt, F32P]
         L.oracle_segment_distance_sq.argtypes = [F64P, F64P, F64P, F64P]
         L.oracle_segment_distance_sq.restype = c_dbl
         L.oracle_triangle_distance_sq.argtypes = [F64P, F64P, F64P, F64P]
@@ -440,3 +443,40 @@ def segment_distance_sq(a0, a1, b0, b1):
 def triangle_distance_sq(p, a, b, c):
     a = [_f64(v) for v in (p, a, b, c)]
     return lib().oracle_triangle_distance_sq(*[_p(v, F64P) for v in a])
+
+
+def tet_rest(xcs, tets):
+    """fem/fem.py:21-34 -> (Bm float32 [n,3,3], W float32 [n])."""
+    x = _f32(xcs).reshape(-1)
+    T = _i32(tets).reshape(-1)
+    n = T.size // 4
+    Bm = np.zeros(9 * n, np.float32)
+    W = np.zeros(n, np.float32)
+    lib().oracle_tet_rest(_p(x, F32P), _p(T, I32P), n, _p(Bm, F32P), _p(W, F32P))
+    return Bm.reshape(n, 3, 3), W
+
+
+def tet_hessian(x, tets, Bm, W, mu, lam, with_force=True):
+    """fem/fem.py:138-203 per reference thread -> (rows, cols, vals [16n,3,3], force [V,3])."""
+    xf = _f32(x).reshape(-1)
+    T = _i32(tets).reshape(-1)
+    n = T.size // 4
+    rows = np.zeros(16 * n, np.int32)
+    cols = np.zeros(16 * n, np.int32)
+    vals = np.zeros(16 * n * 9, np.float32)
+    b = np.zeros(xf.size, np.float32)
+    lib().oracle_tet_hessian(_p(xf, F32P), _p(T, I32P), n, _p(_f32(Bm).reshape(-1), F32P), _p(_f32(W), F32P),
+                             float(mu), float(lam), _p(rows, I32P), _p(cols, I32P), _p(vals, F32P),
+                             _p(b, F32P) if with_force else None)
+    return rows, cols, vals.reshape(-1, 3, 3), b.reshape(-1, 3)
+
+
+def tet_energy(x, tets, Bm, W, mu, lam):
+    """time_integrator_base.py:124-137 -> per-tet W psi(F) (float32 [n])."""
+    xf = _f32(x).reshape(-1)
+    T = _i32(tets).reshape(-1)
+    n = T.size // 4
+    psi = np.zeros(n, np.float32)
+    lib().oracle_tet_energy(_p(xf, F32P), _p(T, I32P), n, _p(_f32(Bm).reshape(-1), F32P), _p(_f32(W), F32P),
+                            float(mu), float(lam), _p(psi, F32P))
+    return psi

@@ -821,6 +821,197 @@ double oracle_triangle_distance_sq(const double* p, const double* a, const doubl
                                  mk(c[0], c[1], c[2]));
 }
 
+// ---------------------------------------------------------------- FEM ----
+// fem/fem.py:21-34 (compute_Dm), :138-203 (tet_kernel_sparse, one thread per
+// block (e, i, j) as the reference launches it) and
+// time_integrator_base.py:124-137 (compute_Psi), stable Neo-Hookean
+// (fem/snh.py:1-84), fp32 in the reference's operation order.
+namespace {
+struct M3f { float m[3][3]; };
+M3f m3_zero() { M3f a{}; return a; }
+M3f m3_cols(const float* c0, const float* c1, const float* c2) {
+  M3f a;
+  for (int r = 0; r < 3; ++r) { a.m[r][0] = c0[r]; a.m[r][1] = c1[r]; a.m[r][2] = c2[r]; }
+  return a;
+}
+M3f m3_mul(const M3f& a, const M3f& b) {
+  M3f o;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      float s = 0.0f;
+      for (int k = 0; k < 3; ++k) s += a.m[r][k] * b.m[k][c];
+      o.m[r][c] = s;
+    }
+  return o;
+}
+M3f m3_transpose(const M3f& a) {
+  M3f o;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) o.m[r][c] = a.m[c][r];
+  return o;
+}
+M3f m3_scale(float s, const M3f& a) {
+  M3f o;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) o.m[r][c] = s * a.m[r][c];
+  return o;
+}
+M3f m3_add(const M3f& a, const M3f& b) {
+  M3f o;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) o.m[r][c] = a.m[r][c] + b.m[r][c];
+  return o;
+}
+float m3_ddot(const M3f& a, const M3f& b) {
+  float s = 0.0f;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) s += a.m[r][c] * b.m[r][c];
+  return s;
+}
+float m3_det(const M3f& a) {
+  return a.m[0][0] * (a.m[1][1] * a.m[2][2] - a.m[1][2] * a.m[2][1]) -
+         a.m[0][1] * (a.m[1][0] * a.m[2][2] - a.m[1][2] * a.m[2][0]) +
+         a.m[0][2] * (a.m[1][0] * a.m[2][1] - a.m[1][1] * a.m[2][0]);
+}
+void v3_cross(const float* a, const float* b, float* o) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+void m3_col(const M3f& a, int c, float* v) { v[0] = a.m[0][c]; v[1] = a.m[1][c]; v[2] = a.m[2][c]; }
+struct SnhP { float mu, lam, alpha; };
+SnhP snh_reparam(float mu, float lam) {
+  SnhP s;
+  s.mu = float(4.0 / 3.0) * mu;
+  s.lam = lam + float(5.0 / 6.0) * mu;
+  s.alpha = 1.0f + 0.75f * s.mu / s.lam;
+  return s;
+}
+M3f snh_cofactor(const M3f& F) {
+  // partialJpartialF: the rows of F^T are F's columns
+  float f0[3], f1[3], f2[3], c0[3], c1[3], c2[3];
+  m3_col(F, 0, f0); m3_col(F, 1, f1); m3_col(F, 2, f2);
+  v3_cross(f1, f2, c0); v3_cross(f2, f0, c1); v3_cross(f0, f1, c2);
+  return m3_cols(c0, c1, c2);
+}
+M3f snh_pk1(const M3f& F, const SnhP& s) {
+  const float Ic = m3_ddot(F, F);
+  const M3f cof = snh_cofactor(F);
+  const float dc = s.mu * (1.0f - 1.0f / (Ic + 1.0f));
+  const float vc = s.lam * (m3_det(F) - s.alpha);
+  return m3_add(m3_scale(dc, F), m3_scale(vc, cof));
+}
+M3f snh_tangent(const M3f& F, const M3f& dF, const SnhP& s) {
+  const M3f cof = snh_cofactor(F);
+  const float Ic = m3_ddot(F, F);
+  const float Ic1 = Ic + 1.0f;
+  const float dIc = 2.0f * m3_ddot(F, dF);
+  const float dc = s.mu * (1.0f - 1.0f / Ic1);
+  const float ddc = s.mu * dIc / (Ic1 * Ic1);
+  const float dJ = m3_ddot(cof, dF);
+  const float vc = s.lam * (m3_det(F) - s.alpha);
+  float f0[3], f1[3], f2[3], g0[3], g1[3], g2[3], a[3], b[3], c0[3], c1[3], c2[3];
+  m3_col(F, 0, f0); m3_col(F, 1, f1); m3_col(F, 2, f2);
+  m3_col(dF, 0, g0); m3_col(dF, 1, g1); m3_col(dF, 2, g2);
+  v3_cross(g1, f2, a); v3_cross(f1, g2, b);
+  for (int k = 0; k < 3; ++k) c0[k] = a[k] + b[k];
+  v3_cross(g2, f0, a); v3_cross(f2, g0, b);
+  for (int k = 0; k < 3; ++k) c1[k] = a[k] + b[k];
+  v3_cross(g0, f1, a); v3_cross(f0, g1, b);
+  for (int k = 0; k < 3; ++k) c2[k] = a[k] + b[k];
+  const M3f dcof = m3_cols(c0, c1, c2);
+  return m3_add(m3_add(m3_add(m3_scale(dc, dF), m3_scale(ddc, F)), m3_scale(s.lam * dJ, cof)), m3_scale(vc, dcof));
+}
+M3f tet_F(const float* x, const int* t, const float* bm, M3f& Bm) {
+  float d[3][3];
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r) d[c][r] = x[3ll * t[c] + r] - x[3ll * t[3] + r];
+  const M3f Ds = m3_cols(d[0], d[1], d[2]);
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) Bm.m[r][c] = bm[3 * r + c];
+  return m3_mul(Ds, Bm);
+}
+}  // namespace
+
+void oracle_tet_rest(const float* xcs, const int* T, int n, float* Bm, float* W) {
+  for (int e = 0; e < n; ++e) {
+    const int* t = T + 4ll * e;
+    float d[3][3];
+    for (int c = 0; c < 3; ++c)
+      for (int r = 0; r < 3; ++r) d[c][r] = xcs[3ll * t[c] + r] - xcs[3ll * t[3] + r];
+    const M3f Dm = m3_cols(d[0], d[1], d[2]);
+    const float det = m3_det(Dm);
+    const float inv = 1.0f / det;
+    const auto& m = Dm.m;
+    float* bm = Bm + 9ll * e;
+    bm[0] = (m[1][1] * m[2][2] - m[1][2] * m[2][1]) * inv;
+    bm[1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) * inv;
+    bm[2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * inv;
+    bm[3] = (m[1][2] * m[2][0] - m[1][0] * m[2][2]) * inv;
+    bm[4] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * inv;
+    bm[5] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) * inv;
+    bm[6] = (m[1][0] * m[2][1] - m[1][1] * m[2][0]) * inv;
+    bm[7] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) * inv;
+    bm[8] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * inv;
+    W[e] = std::fabs(det) / 6.0f;
+  }
+}
+
+// One iteration per reference thread ej = 16 e + 4 i + j.
+void oracle_tet_hessian(const float* x, const int* T, int n, const float* Bmv, const float* Wv, float mu, float lam,
+                        int* rows, int* cols, float* vals, float* b) {
+  const SnhP s = snh_reparam(mu, lam);
+  for (long long ej = 0; ej < 16ll * n; ++ej) {
+    const long long e = ej / 16;
+    const int jj = int(ej % 4), ii = int((ej / 4) % 4);
+    const int* t = T + 4 * e;
+    M3f Bm;
+    const M3f F = tet_F(x, t, Bmv + 9 * e, Bm);
+    M3f a = m3_zero();
+    const M3f P = snh_pk1(F, s);
+    const M3f H = m3_transpose(m3_mul(m3_scale(-Wv[e], P), m3_transpose(Bm)));
+    for (int k = 0; k < 3; ++k) {
+      M3f dDs = m3_zero();
+      if (jj < 3) {
+        dDs.m[k][jj] = -1.0f;
+      } else {
+        dDs.m[k][0] = 1.0f;
+        dDs.m[k][1] = 1.0f;
+        dDs.m[k][2] = 1.0f;
+      }
+      const M3f dF = m3_mul(dDs, Bm);
+      const M3f dP = snh_tangent(F, dF, s);
+      const M3f dH = m3_mul(m3_scale(-Wv[e], dP), m3_transpose(Bm));
+      float df[3];
+      for (int l = 0; l < 3; ++l) df[l] = ii == 3 ? -(dH.m[l][0] + dH.m[l][1] + dH.m[l][2]) : dH.m[l][ii];
+      for (int l = 0; l < 3; ++l) a.m[l][k] += df[l];
+    }
+    rows[ej] = t[ii];
+    cols[ej] = t[jj];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) vals[9 * ej + 3 * r + c] = a.m[r][c];
+    if (b && jj == 0) {
+      for (int l = 0; l < 3; ++l) {
+        const float fi = ii == 3 ? ((-H.m[0][l]) - H.m[1][l]) - H.m[2][l] : H.m[ii][l];
+        b[3ll * t[ii] + l] += fi;
+      }
+    }
+  }
+}
+
+void oracle_tet_energy(const float* x, const int* T, int n, const float* Bmv, const float* Wv, float mu, float lam,
+                       float* psi) {
+  const SnhP s = snh_reparam(mu, lam);
+  for (int e = 0; e < n; ++e) {
+    M3f Bm;
+    const M3f F = tet_F(x, T + 4ll * e, Bmv + 9ll * e, Bm);
+    const float Ic = m3_ddot(F, F);
+    const float JmA = m3_det(F) - s.alpha;
+    const float p = 0.5f * (s.lam * JmA * JmA + s.mu * (Ic - 3.0f) - s.mu * std::log(Ic + 1.0f));
+    psi[e] = Wv[e] * p;
+  }
+}
+
 int oracle_max_threads(void) { return omp_get_max_threads(); }
 
 }  // extern "C"
