@@ -463,6 +463,59 @@ def tet_elasticity(soup, x0, dev, reps: int = 20):
     return out
 
 
+def al_active_set(S, X0, DX, dev, reps: int = 20):
+    """update_active_set (augmented_lagrangian.py:705-717) on the rod twist,
+    the geometry RodsTwistAL runs the barrier-free solver on (:750-767):
+    the active sets hold the pairs met between the first state and its full
+    Newton step; the pairs met by the second state's full step are then
+    recorded once (new_intersections), and the timed hash work is the rest
+    of the update: per-vertex earliest TOI, create_hash (EE, PT) and merge
+    into the active sets, with the host count readbacks ActiveSet.count
+    makes."""
+    import torch
+
+    from ipc_warp_amd.active_set import ALActiveSets, accumulate_vertex_toi, xconstraint_records
+
+    sets = ALActiveSets(X0[0].shape[0], dev, initial_capacity=1 << 12)
+    sets.update_active_set(S, X0[0], (X0[0] - DX[0]).contiguous())
+    old = (sets.active_set_ee.hash, sets.active_set_pt.hash)
+    n_old = (sets.active_set_ee.count(), sets.active_set_pt.count())
+    x1 = (X0[1] - DX[1]).contiguous()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        S.new_intersections(X0[1], x1)
+    torch.cuda.synchronize()
+    ccd_ms = (time.perf_counter() - t0) * 1e3 / reps
+    recs = [xconstraint_records(C.list, C.e0e1, C.alpha, n, S.l0) for C, n in
+            ((S.contacts_new, S.n_contacts), (S.contacts_pt, S.n_contacts_pt))]
+
+    def hash_work():
+        sets.active_set_ee.hash, sets.active_set_pt.hash = old
+        sets.vertex_toi.fill_(1.0)
+        for r in recs:
+            accumulate_vertex_toi(r, r.shape[0], sets.vertex_toi)
+        sets.detected_set_ee.create_hash(recs[0], recs[0].shape[0], sets.vertex_toi)
+        sets.detected_set_pt.create_hash(recs[1], recs[1].shape[0], sets.vertex_toi)
+        sets.active_set_ee.merge(sets.detected_set_ee)
+        sets.active_set_pt.merge(sets.detected_set_pt)
+
+    for _ in range(3):
+        hash_work()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        hash_work()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    detected = recs[0].shape[0] + recs[1].shape[0]
+    return {"recorded_pairs": [int(recs[0].shape[0]), int(recs[1].shape[0])],
+            "accepted": [sets.detected_set_ee.count(), sets.detected_set_pt.count()],
+            "old_active": list(n_old), "merged_active": [sets.active_set_ee.count(), sets.active_set_pt.count()],
+            "new_intersections_ms": ccd_ms, "hash_ms_per_update": ms,
+            "recorded_pairs_per_s": detected / (ms * 1e-3)}
+
+
 def abd_reduction(dev, seed: int, reps: int = 10):
     """The affine-body contact reduction (affine_body_dynamics.py:144-190)
     of the screw-and-nut contact Hessian: vertex triplets onto the screw's
@@ -702,6 +755,7 @@ def main() -> int:
         result["bsr_assembly"] = bsr_assembly(S, X0[0], soup, dev)
         result["newton_system_cg"] = newton_system_cg(S, X0[0], soup, dev)
         result["tet_elasticity"] = tet_elasticity(soup, X0[0], dev)
+        result["al_active_set"] = al_active_set(S, X0, DX, dev)
     del S
 
     if not args.no_extras:

@@ -479,6 +479,47 @@ int ipc_abd_reduce_contacts(ipc_abd* abd, int n_bodies, const int* body, const f
                             const float* force, double scale, int* offsets, int* columns, float* values, int capacity,
                             float* reduced_force, void* stream);
 
+/* =========================================== augmented-Lagrangian set ===
+ * The active set of the barrier-free AL solver (augmented_lagrangian.py:
+ * 42-197, 506-547): a hash table (the reference's Hash) of table_size
+ * buckets, cell_start int32 [table_size + 1] (bucket b holds entries
+ * cell_start[b] .. cell_start[b+1]; cell_start[table_size] is the entry
+ * count) and entries, ALConstraint records in the reference's layout:
+ * a1a2b1b2 int32[4], e0e1 int32[2], then l0, alpha, lam, gamma, slack,
+ * mollifier as `scalar`, distance_type int32, d0 `scalar`, grad `scalar`[12]
+ * (104 bytes with fp32 scalars, 184 with fp64: 4 bytes of padding after
+ * distance_type). Constraints come in as XConstraint records
+ * (IPC_XCONSTRAINT_BYTES_*), vertex TOIs as `scalar` [n_vertices]. Within a
+ * bucket, entries are ordered by decreasing input index (the reference's
+ * order when its threads run one after another). An ipc_active_set is the
+ * workspace of these calls (one stream at a time); tables are the
+ * caller's. */
+#define IPC_ALCONSTRAINT_BYTES_F32 104
+#define IPC_ALCONSTRAINT_BYTES_F64 184
+typedef struct ipc_active_set ipc_active_set;
+int ipc_active_set_create(ipc_active_set** ws);
+int ipc_active_set_destroy(ipc_active_set* ws);
+/* accumulate_vertex_toi (augmented_lagrangian.py:105-113): vertex_toi[v] =
+ * min(vertex_toi[v], alpha) over the constraints on v (the caller fills it
+ * with 1 first, as update_active_set does at :707-712). */
+int ipc_al_vertex_toi(const void* constraints, int record_bytes, int n, void* vertex_toi, int n_vertices,
+                      void* stream);
+/* ActiveSet.create_hash (augmented_lagrangian.py:525-535, count_constraints
+ * / register_constraints :124-149): the constraints with alpha <= vertex_toi
+ * + 1e-12 at one of their vertices become entries with lam 0, gamma 1,
+ * slack 0, mollifier 1, distance_type 0, d0 0, grad 0; entries past the
+ * count are zeroed. capacity >= n. */
+int ipc_al_create_hash(ipc_active_set* ws, const void* constraints, int record_bytes, int n, const void* vertex_toi,
+                       int n_vertices, int table_size, int* cell_start, void* entries, int capacity, void* stream);
+/* ActiveSet.merge (augmented_lagrangian.py:537-547, count_merge /
+ * register_merge :167-197): the old table's n_old entries with gamma >=
+ * 0.01, then the new table's n_new entries whose e0e1 the old table does
+ * not hold, into the destination table (capacity >= n_old + n_new; entries
+ * past the count zeroed). The destination must not overlap the inputs. */
+int ipc_al_merge(ipc_active_set* ws, int entry_bytes, const int* old_cell_start, const void* old_entries,
+                 int old_table_size, int n_old, const void* new_entries, int n_new, int table_size, int* cell_start,
+                 void* entries, int capacity, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -72,6 +72,11 @@ _SIGNATURES = {
     "ipc_abd_destroy": ([_vp], _c_int),
     "ipc_abd_reduce_contacts": ([_vp, _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _vp, _c_double, _vp, _vp,
                                  _vp, _c_int, _vp, _vp], _c_int),
+    "ipc_active_set_create": ([ctypes.POINTER(_vp)], _c_int),
+    "ipc_active_set_destroy": ([_vp], _c_int),
+    "ipc_al_vertex_toi": ([_vp, _c_int, _c_int, _vp, _c_int, _vp], _c_int),
+    "ipc_al_create_hash": ([_vp, _vp, _c_int, _c_int, _vp, _c_int, _c_int, _vp, _vp, _c_int, _vp], _c_int),
+    "ipc_al_merge": ([_vp, _c_int, _vp, _vp, _c_int, _c_int, _vp, _c_int, _c_int, _vp, _vp, _c_int, _vp], _c_int),
     "ipc_bsr_create": ([ctypes.POINTER(_vp)], _c_int),
     "ipc_bsr_destroy": ([_vp], _c_int),
     "ipc_bsr_from_triplets": ([_vp, _c_int, _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp, _c_int, _vp], _c_int),

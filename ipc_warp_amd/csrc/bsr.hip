@@ -28,6 +28,7 @@
 #include "../../include/ipc_warp_amd.h"
 #include "cull.hpp"
 #include "ipc_kernels.hpp"
+#include "scan.hpp"
 
 using namespace ipcd;
 
@@ -53,8 +54,6 @@ constexpr int kB = 256;
 constexpr int kWaves = kB / 64;
 constexpr int kSortCap = 1024;  // entries of one row staged in LDS by its wave
 constexpr int kWin = 256;       // triplet values staged per window and wave
-constexpr int kScanPer = 16;
-constexpr int kScanTile = kB * kScanPer;
 
 int bsr_fail(int code, const char* what) {
   set_error(what);
@@ -70,89 +69,6 @@ int bsr_hip(hipError_t e, const char* where) {
 }
 
 inline int nblk(long long n, int per) { return int((n + per - 1) / per); }
-
-__global__ void fill_i32_kernel(int* p, int n, int v) {
-  for (int i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) p[i] = v;
-}
-
-hipError_t fill_i32(int* p, int n, int v, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  const int g = nblk(n, kB) < 1024 ? nblk(n, kB) : 1024;
-  hipLaunchKernelGGL(fill_i32_kernel, dim3(g), dim3(kB), 0, s, p, n, v);
-  return hipGetLastError();
-}
-
-// ---------------------------------------------------------------- scan ---
-// out[0..n) = exclusive prefix sums of in[0..n) (in and out distinct), in
-// tiles of kScanTile, then the tile sums, then the tile offsets added.
-__device__ int block_exclusive(int v, int* sh, int& total) {
-  const int tid = threadIdx.x;
-  sh[tid] = v;
-  __syncthreads();
-  for (int d = 1; d < kB; d <<= 1) {
-    const int u = tid >= d ? sh[tid - d] : 0;
-    __syncthreads();
-    sh[tid] += u;
-    __syncthreads();
-  }
-  total = sh[kB - 1];
-  const int r = sh[tid] - v;
-  __syncthreads();
-  return r;
-}
-
-__global__ __launch_bounds__(kB) void scan_tiles_kernel(const int* in, int n, int* out, int* tile_sum) {
-  __shared__ int sh[kB];
-  const long long base = (long long)blockIdx.x * kScanTile + threadIdx.x * kScanPer;
-  int v[kScanPer];
-  int acc = 0;
-#pragma unroll
-  for (int k = 0; k < kScanPer; ++k) {
-    v[k] = base + k < n ? in[base + k] : 0;
-    acc += v[k];
-  }
-  int total;
-  int run = block_exclusive(acc, sh, total);
-#pragma unroll
-  for (int k = 0; k < kScanPer; ++k) {
-    if (base + k < n) out[base + k] = run;
-    run += v[k];
-  }
-  if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
-}
-
-__global__ __launch_bounds__(kB) void scan_sums_kernel(int* tile_sum, int nt) {
-  __shared__ int sh[kB];
-  int carry = 0;
-  for (int b = 0; b < nt; b += kB) {
-    const int i = b + threadIdx.x;
-    const int v = i < nt ? tile_sum[i] : 0;
-    int total;
-    const int e = block_exclusive(v, sh, total);
-    if (i < nt) tile_sum[i] = carry + e;
-    carry += total;
-  }
-}
-
-__global__ void scan_add_kernel(int* out, int n, const int* tile_sum) {
-  const int tile = blockIdx.x + 1;
-  const int add = tile_sum[tile];
-  const long long base = (long long)tile * kScanTile;
-  for (int k = threadIdx.x; k < kScanTile; k += kB)
-    if (base + k < n) out[base + k] += add;
-}
-
-hipError_t exclusive_scan(const int* in, int n, int* out, DevBuf& part, hipStream_t s) {
-  const int nt = nblk(n, kScanTile);
-  hipError_t e = part.ensure(sizeof(int) * size_t(nt));
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(scan_tiles_kernel, dim3(nt), dim3(kB), 0, s, in, n, out, part.as<int>());
-  if (nt > 1) {
-    hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(kB), 0, s, part.as<int>(), nt);
-    hipLaunchKernelGGL(scan_add_kernel, dim3(nt - 1), dim3(kB), 0, s, out, n, part.as<int>());
-  }
-  return hipGetLastError();
-}
 
 // ---------------------------------------------------- triplets -> BSR ---
 __device__ inline bool block_nonzero(const float* v) {

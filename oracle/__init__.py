@@ -91,6 +91,11 @@ def lib():
         L.oracle_tet_rest.argtypes = [F32P, I32P, c_int, F32P, F32P]
         L.oracle_tet_hessian.argtypes = [F32P, I32P, c_int, F32P, F32P, c_flt, c_flt, I32P, I32P, F32P, F32P]
         L.oracle_tet_energy.argtypes = [F32P, I32P, c_int, F32P, F32P, c_flt, c_flt, F32P]
+        L.oracle_al_hash_pos.argtypes = [c_int, c_int, c_int]
+        L.oracle_al_hash_pos.restype = c_int
+        L.oracle_al_vertex_toi.argtypes = [c_int, I32P, c_int, ctypes.c_void_p, c_int]
+        L.oracle_al_create_hash.argtypes = [c_int, I32P, c_int, ctypes.c_void_p, c_int, c_int, I32P, I32P, c_int]
+        L.oracle_al_merge.argtypes = [c_int, I32P, I32P, c_int, c_int, I32P, c_int, c_int, I32P, I32P, c_int]
         L.oracle_segment_distance_sq.argtypes = [F64P, F64P, F64P, F64P]
         L.oracle_segment_distance_sq.restype = c_dbl
         L.oracle_triangle_distance_sq.argtypes = [F64P, F64P, F64P, F64P]
@@ -480,3 +485,52 @@ def tet_energy(x, tets, Bm, W, mu, lam):
     lib().oracle_tet_energy(_p(xf, F32P), _p(T, I32P), n, _p(_f32(Bm).reshape(-1), F32P), _p(_f32(W), F32P),
                             float(mu), float(lam), _p(psi, F32P))
     return psi
+
+
+# ------------------------------------------------ augmented-Lagrangian set ---
+XCONSTRAINT_WORDS = {np.float32: 9, np.float64: 12}
+ALCONSTRAINT_WORDS = {np.float32: 26, np.float64: 46}
+
+
+def al_hash_pos(e0, e1, table_size):
+    """augmented_lagrangian.py:68-73."""
+    return int(lib().oracle_al_hash_pos(int(e0), int(e1), int(table_size)))
+
+
+def al_vertex_toi(records, vertex_toi):
+    """augmented_lagrangian.py:105-113 on XConstraint records (int32 [n, 9]
+    fp32 / [n, 12] fp64); returns the updated copy of vertex_toi."""
+    r = _i32(records)
+    use_double = r.shape[1] == 12
+    vt = np.array(vertex_toi, dtype=np.float64 if use_double else np.float32)
+    lib().oracle_al_vertex_toi(int(use_double), _p(r.reshape(-1), I32P), r.shape[0], vt.ctypes.data, vt.size)
+    return vt
+
+
+def al_create_hash(records, vertex_toi, table_size, capacity):
+    """ActiveSet.create_hash (augmented_lagrangian.py:525-535) -> (count,
+    cell_start int32 [T+1], entries int32 [capacity, 26 | 46])."""
+    r = _i32(records)
+    use_double = r.shape[1] == 12
+    vt = np.ascontiguousarray(vertex_toi, dtype=np.float64 if use_double else np.float32)
+    cs = np.zeros(table_size + 1, np.int32)
+    ent = np.zeros((capacity, 46 if use_double else 26), np.int32)
+    cnt = lib().oracle_al_create_hash(int(use_double), _p(r.reshape(-1), I32P), r.shape[0], vt.ctypes.data, vt.size,
+                                      int(table_size), _p(cs, I32P), _p(ent, I32P), int(capacity))
+    return cnt, cs, ent
+
+
+def al_merge(old_cell_start, old_entries, old_table_size, n_old, new_entries, n_new, table_size, capacity):
+    """ActiveSet.merge (augmented_lagrangian.py:537-547) -> (count,
+    cell_start, entries)."""
+    oe = _i32(old_entries)
+    ne = _i32(new_entries)
+    words = oe.shape[1]
+    use_double = words == 46
+    ocs = _i32(old_cell_start)
+    cs = np.zeros(table_size + 1, np.int32)
+    ent = np.zeros((capacity, words), np.int32)
+    cnt = lib().oracle_al_merge(int(use_double), _p(ocs, I32P), _p(oe.reshape(-1), I32P), int(old_table_size),
+                                int(n_old), _p(ne.reshape(-1), I32P), int(n_new), int(table_size), _p(cs, I32P),
+                                _p(ent, I32P), int(capacity))
+    return cnt, cs, ent

@@ -1015,3 +1015,166 @@ void oracle_tet_energy(const float* x, const int* T, int n, const float* Bmv, co
 int oracle_max_threads(void) { return omp_get_max_threads(); }
 
 }  // extern "C"
+
+// ------------------------------------------- augmented-Lagrangian set ---
+// The active-set hash kernels of augmented_lagrangian.py run with their
+// threads one after another in index order (as Warp's CPU device runs a
+// launch), on records in the reference's layout seen as int32 words.
+namespace {
+
+template <typename S> struct AlLayout {
+  static constexpr int sw = sizeof(S) / 4;
+  static constexpr int xwords = 6 + 3 * sw;                  // XConstraint
+  static constexpr int awords = (sw == 1 ? 13 : 20) + 13 * sw;  // ALConstraint
+  static constexpr int alpha = 6 + sw, lam = 6 + 2 * sw, gamma = 6 + 3 * sw, slack = 6 + 4 * sw,
+                       mollifier = 6 + 5 * sw, type = 6 + 6 * sw;
+};
+
+template <typename S> S get_s(const uint32_t* r, int w) {
+  S v;
+  std::memcpy(&v, r + w, sizeof(S));
+  return v;
+}
+template <typename S> void put_s(uint32_t* r, int w, S v) { std::memcpy(r + w, &v, sizeof(S)); }
+
+// hash_pos (augmented_lagrangian.py:68-73): bit_xor of the int32 products,
+// negated when negative, modulo table_size. The products wrap; the one XOR
+// whose negation int32 cannot hold (INT_MIN) is taken as 2^31.
+int al_hash(int e0, int e1, int table_size) {
+  const int32_t v = int32_t(uint32_t(e0) * 73856093u ^ uint32_t(e1) * 19349663u);
+  const int64_t mag = v < 0 ? -int64_t(v) : int64_t(v);
+  return int(mag % table_size);
+}
+
+// filter_accept (augmented_lagrangian.py:116-121).
+template <typename S> bool al_filter(const uint32_t* c, const S* vt, int nv) {
+  const S alpha = get_s<S>(c, AlLayout<S>::alpha);
+  for (int k = 0; k < 4; ++k) {
+    const int v = int(c[k]);
+    if (v >= 0 && v < nv && alpha <= S(vt[v] + S(1.0e-12))) return true;
+  }
+  return false;
+}
+
+// to_al_constraint (augmented_lagrangian.py:88-102).
+template <typename S> void al_fresh(const uint32_t* c, uint32_t* out) {
+  using L = AlLayout<S>;
+  std::memset(out, 0, sizeof(uint32_t) * L::awords);
+  std::memcpy(out, c, sizeof(uint32_t) * 6);
+  put_s<S>(out, 6, get_s<S>(c, 6));
+  put_s<S>(out, L::alpha, get_s<S>(c, L::alpha));
+  put_s<S>(out, L::gamma, S(1));
+  put_s<S>(out, L::mollifier, S(1));
+}
+
+// ActiveSet.create_hash (augmented_lagrangian.py:525-535) with
+// count_constraints, array_scan(inclusive) and register_constraints
+// (:124-149).
+template <typename S>
+int al_create(const uint32_t* rec, int n, const S* vt, int nv, int T, int* cs, uint32_t* entries, int cap) {
+  using L = AlLayout<S>;
+  std::fill(cs, cs + T + 1, 0);
+  std::fill(entries, entries + size_t(cap) * L::awords, 0u);
+  for (int i = 0; i < n; ++i) {
+    const uint32_t* c = rec + size_t(i) * L::xwords;
+    if (al_filter<S>(c, vt, nv)) ++cs[al_hash(int(c[4]), int(c[5]), T)];
+  }
+  for (int b = 1; b <= T; ++b) cs[b] += cs[b - 1];
+  for (int i = 0; i < n; ++i) {
+    const uint32_t* c = rec + size_t(i) * L::xwords;
+    if (!al_filter<S>(c, vt, nv)) continue;
+    const int out = --cs[al_hash(int(c[4]), int(c[5]), T)];
+    al_fresh<S>(c, entries + size_t(out) * L::awords);
+  }
+  return cs[T];
+}
+
+// is_new_constraint (augmented_lagrangian.py:157-164).
+template <typename S> bool al_is_new(const int* ocs, const uint32_t* oe, int oT, const uint32_t* c) {
+  const int b = al_hash(int(c[4]), int(c[5]), oT);
+  for (int k = ocs[b]; k < ocs[b + 1]; ++k) {
+    const uint32_t* o = oe + size_t(k) * AlLayout<S>::awords;
+    if (o[4] == c[4] && o[5] == c[5]) return false;
+  }
+  return true;
+}
+
+// ActiveSet.merge (augmented_lagrangian.py:537-547) with count_merge,
+// array_scan(inclusive) and register_merge (:167-197); keep_old :152-154.
+template <typename S>
+int al_merge(const int* ocs, const uint32_t* oe, int oT, int n_old, const uint32_t* ne, int n_new, int T, int* cs,
+             uint32_t* entries, int cap) {
+  using L = AlLayout<S>;
+  std::fill(cs, cs + T + 1, 0);
+  std::fill(entries, entries + size_t(cap) * L::awords, 0u);
+  auto source = [&](int i, bool& accept) -> const uint32_t* {
+    if (i < n_old) {
+      const uint32_t* c = oe + size_t(i) * L::awords;
+      accept = get_s<S>(c, L::gamma) >= S(0.01);
+      return c;
+    }
+    const uint32_t* c = ne + size_t(i - n_old) * L::awords;
+    accept = al_is_new<S>(ocs, oe, oT, c);
+    return c;
+  };
+  for (int i = 0; i < n_old + n_new; ++i) {
+    bool accept;
+    const uint32_t* c = source(i, accept);
+    if (accept) ++cs[al_hash(int(c[4]), int(c[5]), T)];
+  }
+  for (int b = 1; b <= T; ++b) cs[b] += cs[b - 1];
+  for (int i = 0; i < n_old + n_new; ++i) {
+    bool accept;
+    const uint32_t* c = source(i, accept);
+    if (!accept) continue;
+    const int out = --cs[al_hash(int(c[4]), int(c[5]), T)];
+    std::memcpy(entries + size_t(out) * L::awords, c, sizeof(uint32_t) * L::awords);
+  }
+  return cs[T];
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_al_hash_pos(int e0, int e1, int table_size) { return al_hash(e0, e1, table_size); }
+
+// accumulate_vertex_toi (augmented_lagrangian.py:105-113).
+void oracle_al_vertex_toi(int use_double, const uint32_t* rec, int n, void* vt, int nv) {
+  for (int i = 0; i < n; ++i) {
+    if (use_double) {
+      const uint32_t* c = rec + size_t(i) * AlLayout<double>::xwords;
+      const double a = get_s<double>(c, AlLayout<double>::alpha);
+      for (int k = 0; k < 4; ++k) {
+        const int v = int(c[k]);
+        if (v >= 0 && v < nv) static_cast<double*>(vt)[v] = std::min(static_cast<double*>(vt)[v], a);
+      }
+    } else {
+      const uint32_t* c = rec + size_t(i) * AlLayout<float>::xwords;
+      const float a = get_s<float>(c, AlLayout<float>::alpha);
+      for (int k = 0; k < 4; ++k) {
+        const int v = int(c[k]);
+        if (v >= 0 && v < nv) static_cast<float*>(vt)[v] = std::min(static_cast<float*>(vt)[v], a);
+      }
+    }
+  }
+}
+
+int oracle_al_create_hash(int use_double, const uint32_t* rec, int n, const void* vt, int nv, int table_size,
+                          int* cell_start, uint32_t* entries, int capacity) {
+  return use_double ? al_create<double>(rec, n, static_cast<const double*>(vt), nv, table_size, cell_start, entries,
+                                        capacity)
+                    : al_create<float>(rec, n, static_cast<const float*>(vt), nv, table_size, cell_start, entries,
+                                       capacity);
+}
+
+int oracle_al_merge(int use_double, const int* old_cs, const uint32_t* old_e, int old_table_size, int n_old,
+                    const uint32_t* new_e, int n_new, int table_size, int* cell_start, uint32_t* entries,
+                    int capacity) {
+  return use_double ? al_merge<double>(old_cs, old_e, old_table_size, n_old, new_e, n_new, table_size, cell_start,
+                                       entries, capacity)
+                    : al_merge<float>(old_cs, old_e, old_table_size, n_old, new_e, n_new, table_size, cell_start,
+                                      entries, capacity);
+}
+
+}  // extern "C"
