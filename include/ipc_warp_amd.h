@@ -93,6 +93,23 @@ int ipc_contact_energy_pt(const float* x, int n_vertices, const int* contacts, i
 int ipc_collision_energy(const float* x, const float* xcs, int n_vertices, const int* ee, int n_ee, const int* pt,
                          int n_pt, float l0, double kappa, double* energy, void* stream);
 
+/* ------------------------------------------------ fp64 scalars ---
+ * The four contact kernels above with the reference's fp64 scalars
+ * (scalar_types.py:6-28 with USE_FLOAT32 = False: vec3d positions, mat33d
+ * blocks, vec3d forces): x / xcs double [V][3], vals double [16 n][3][3]
+ * (16-byte aligned), b double [V][3], l0 double. The distance filter and the
+ * feature dispatch run in double; the layout of the triplets and the zero
+ * blocks of inactive contacts are as in fp32. */
+int ipc_contact_hessian_ee_f64(const double* x, const double* xcs, int n_vertices, const int* contacts,
+                               int n_contacts, double l0, double kappa, int offset, int* rows, int* cols, double* vals,
+                               double* b, void* stream);
+int ipc_contact_hessian_pt_f64(const double* x, int n_vertices, const int* contacts, int n_contacts, double l0,
+                               double kappa, int offset, int* rows, int* cols, double* vals, double* b, void* stream);
+int ipc_contact_energy_ee_f64(const double* x, const double* xcs, int n_vertices, const int* contacts, int n_contacts,
+                              double l0, double kappa, double* energy, void* stream);
+int ipc_contact_energy_pt_f64(const double* x, int n_vertices, const int* contacts, int n_contacts, double l0,
+                              double kappa, double* energy, void* stream);
+
 /* ----------------------------------------------- contact distance ---
  * Replace get_contact_points / get_contact_points_pt (contact.py:148-161)
  * over fetch_dist_v0v1 / fetch_dist_v0v1_pt (contact.py:88-145): midpoint of

@@ -24,6 +24,12 @@ _SIGNATURES = {
     "ipc_collision_hessian": ([_vp, _vp, _c_int, _vp, _c_int, _vp, _c_int, _c_float, _c_double, _vp, _vp, _vp, _vp, _vp],
                               _c_int),
     "ipc_collision_energy": ([_vp, _vp, _c_int, _vp, _c_int, _vp, _c_int, _c_float, _c_double, _vp, _vp], _c_int),
+    "ipc_contact_hessian_ee_f64": ([_vp, _vp, _c_int, _vp, _c_int, _c_double, _c_double, _c_int, _vp, _vp, _vp, _vp,
+                                    _vp], _c_int),
+    "ipc_contact_hessian_pt_f64": ([_vp, _c_int, _vp, _c_int, _c_double, _c_double, _c_int, _vp, _vp, _vp, _vp, _vp],
+                                   _c_int),
+    "ipc_contact_energy_ee_f64": ([_vp, _vp, _c_int, _vp, _c_int, _c_double, _c_double, _vp, _vp], _c_int),
+    "ipc_contact_energy_pt_f64": ([_vp, _c_int, _vp, _c_int, _c_double, _c_double, _vp, _vp], _c_int),
     "ipc_contact_points_ee": ([_vp, _vp, _c_int, _vp, _c_int, _vp, _vp, _vp, _vp], _c_int),
     "ipc_contact_points_pt": ([_vp, _c_int, _vp, _c_int, _vp, _vp, _vp, _vp], _c_int),
     "ipc_collision_time_ee": ([_vp, _vp, _c_int, _vp, _c_int, _vp, _vp, _vp], _c_int),

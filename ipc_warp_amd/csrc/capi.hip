@@ -34,13 +34,13 @@ int ipc_abi_version(void) { return 2; }
 
 const char* ipc_last_error(void) { return g_err; }
 
-static int check_hessian_args(const float* x, int nv, const int* contacts, int n, float l0, int offset, int* rows,
-                              int* cols, float* vals, float* b) {
+static int check_hessian_args(const void* x, int nv, const int* contacts, int n, double l0, int offset, int* rows,
+                              int* cols, const void* vals, const void* b) {
   if (n < 0 || offset < 0 || nv < 0) return fail(IPC_EINVAL, "contact_hessian: negative n_vertices, n_contacts or offset");
   if ((long long)(offset) + n > (1ll << 27)) return fail(IPC_EINVAL, "contact_hessian: offset + n_contacts exceeds 2^27");
   if (n == 0) return IPC_OK;
   if (!x || !contacts || !rows || !cols || !vals || !b) return fail(IPC_EINVAL, "contact_hessian: null pointer");
-  if (!(l0 > 0.0f)) return fail(IPC_EINVAL, "contact_hessian: l0 must be > 0");
+  if (!(l0 > 0.0)) return fail(IPC_EINVAL, "contact_hessian: l0 must be > 0");
   if (!aligned16(contacts) || !aligned16(rows) || !aligned16(cols) || !aligned16(vals))
     return fail(IPC_EINVAL, "contact_hessian: contacts/rows/cols/vals must be 16-byte aligned");
   return IPC_OK;
@@ -77,11 +77,11 @@ int ipc_collision_hessian(const float* x, const float* xcs, int n_vertices, cons
                     "collision_hessian");
 }
 
-static int check_energy_args(const float* x, int nv, const int* contacts, int n, float l0, double* energy) {
+static int check_energy_args(const void* x, int nv, const int* contacts, int n, double l0, double* energy) {
   if (n < 0 || nv < 0) return fail(IPC_EINVAL, "contact_energy: negative n_vertices or n_contacts");
   if (n == 0) return IPC_OK;
   if (!x || !contacts || !energy) return fail(IPC_EINVAL, "contact_energy: null pointer");
-  if (!(l0 > 0.0f)) return fail(IPC_EINVAL, "contact_energy: l0 must be > 0");
+  if (!(l0 > 0.0)) return fail(IPC_EINVAL, "contact_energy: l0 must be > 0");
   if (!aligned16(contacts)) return fail(IPC_EINVAL, "contact_energy: contacts must be 16-byte aligned");
   return IPC_OK;
 }
@@ -115,6 +115,45 @@ int ipc_collision_energy(const float* x, const float* xcs, int n_vertices, const
   return hip_status(ipcd::launch_collision_energy(x, xcs, n_vertices, ee, n_ee, pt, n_pt, l0, kappa, energy,
                                                   static_cast<hipStream_t>(stream)),
                     "collision_energy");
+}
+
+int ipc_contact_hessian_ee_f64(const double* x, const double* xcs, int n_vertices, const int* contacts,
+                               int n_contacts, double l0, double kappa, int offset, int* rows, int* cols, double* vals,
+                               double* b, void* stream) {
+  int st = check_hessian_args(x, n_vertices, contacts, n_contacts, l0, offset, rows, cols, vals, b);
+  if (st != IPC_OK || n_contacts == 0) return st;
+  if (!xcs) return fail(IPC_EINVAL, "contact_hessian_ee_f64: xcs (rest positions) is required");
+  return hip_status(ipcd::launch_contact_hessian_f64(true, x, xcs, n_vertices, contacts, n_contacts, l0, kappa, offset,
+                                                     rows, cols, vals, b, static_cast<hipStream_t>(stream)),
+                    "contact_hessian_ee_f64");
+}
+
+int ipc_contact_hessian_pt_f64(const double* x, int n_vertices, const int* contacts, int n_contacts, double l0,
+                               double kappa, int offset, int* rows, int* cols, double* vals, double* b, void* stream) {
+  int st = check_hessian_args(x, n_vertices, contacts, n_contacts, l0, offset, rows, cols, vals, b);
+  if (st != IPC_OK || n_contacts == 0) return st;
+  return hip_status(ipcd::launch_contact_hessian_f64(false, x, nullptr, n_vertices, contacts, n_contacts, l0, kappa,
+                                                     offset, rows, cols, vals, b, static_cast<hipStream_t>(stream)),
+                    "contact_hessian_pt_f64");
+}
+
+int ipc_contact_energy_ee_f64(const double* x, const double* xcs, int n_vertices, const int* contacts, int n_contacts,
+                              double l0, double kappa, double* energy, void* stream) {
+  int st = check_energy_args(x, n_vertices, contacts, n_contacts, l0, energy);
+  if (st != IPC_OK || n_contacts == 0) return st;
+  if (!xcs) return fail(IPC_EINVAL, "contact_energy_ee_f64: xcs (rest positions) is required");
+  return hip_status(ipcd::launch_contact_energy_f64(true, x, xcs, n_vertices, contacts, n_contacts, l0, kappa, energy,
+                                                    static_cast<hipStream_t>(stream)),
+                    "contact_energy_ee_f64");
+}
+
+int ipc_contact_energy_pt_f64(const double* x, int n_vertices, const int* contacts, int n_contacts, double l0,
+                              double kappa, double* energy, void* stream) {
+  int st = check_energy_args(x, n_vertices, contacts, n_contacts, l0, energy);
+  if (st != IPC_OK || n_contacts == 0) return st;
+  return hip_status(ipcd::launch_contact_energy_f64(false, x, nullptr, n_vertices, contacts, n_contacts, l0, kappa,
+                                                    energy, static_cast<hipStream_t>(stream)),
+                    "contact_energy_pt_f64");
 }
 
 static int check_points_args(const float* x, int nv, const int* contacts, int n) {

@@ -20,16 +20,18 @@ __device__ __forceinline__ bool valid4(int4 c, int nv) {
 // contact_hessian_ee (dynamic_contacts.py:151-207). Returns false for an
 // inactive contact (fp32 distance >= l0). On an active contact, fills the
 // compressed Hessian and the force (-gradient) contribution of each slot.
-__device__ __forceinline__ bool pair_ee(const float* __restrict__ x, const float* __restrict__ xcs, int4 c, float l0,
-                                        double kappa, bool want_hessian, CompressedHessian& C, float force[12],
+// P: the positions' (and classification's) type; R: the outputs' type.
+template <typename P, typename R>
+__device__ __forceinline__ bool pair_ee(const P* __restrict__ x, const P* __restrict__ xcs, int4 c, P l0,
+                                        double kappa, bool want_hessian, CompressedHessianT<R>& C, R force[12],
                                         double* energy) {
-  V3<float> xf[4] = {ldx(x, c.x), ldx(x, c.y), ldx(x, c.z), ldx(x, c.w)};
-  EdgeClosest ec = closest_point_edge_edge(xf[0], xf[1], xf[2], xf[3], 1.0e-6f);
+  V3<P> xf[4] = {ldx(x, c.x), ldx(x, c.y), ldx(x, c.z), ldx(x, c.w)};
+  EdgeClosestT<P> ec = closest_point_edge_edge(xf[0], xf[1], xf[2], xf[3], lit<P>(1.0e-6f, 1.0e-6));
   if (!(ec.dist < l0)) return false;
-  V3<float> ra = ldx(xcs, c.y) - ldx(xcs, c.x), rb = ldx(xcs, c.w) - ldx(xcs, c.z);
-  float eps_x = 1.0e-3f * dot(ra, ra) * dot(rb, rb);
-  V3<float> cr = cross(xf[1] - xf[0], xf[3] - xf[2]);
-  bool near_parallel = eps_x > 0.0f && dot(cr, cr) < eps_x;
+  V3<P> ra = ldx(xcs, c.y) - ldx(xcs, c.x), rb = ldx(xcs, c.w) - ldx(xcs, c.z);
+  P eps_x = lit<P>(1.0e-3f, 1.0e-3) * dot(ra, ra) * dot(rb, rb);
+  V3<P> cr = cross(xf[1] - xf[0], xf[3] - xf[2]);
+  bool near_parallel = eps_x > P(0) && dot(cr, cr) < eps_x;
   Feature f = ee_feature(ec, cr, eps_x);
   V3<double> xo = to_d(xf[0]);
   V3<double> xr[4] = {mk(0.0, 0.0, 0.0), to_d(xf[1]) - xo, to_d(xf[2]) - xo, to_d(xf[3]) - xo};
@@ -63,19 +65,19 @@ __device__ __forceinline__ bool pair_ee(const float* __restrict__ x, const float
   for (int i = 0; i < 4; ++i) {
     double s = -2.0 * m * B.db * g.w[i];
     V3<double> gmi = i == 0 ? mk(-ga.x, -ga.y, -ga.z) : (i == 1 ? ga : (i == 2 ? mk(-gb.x, -gb.y, -gb.z) : gb));
-    force[3 * i + 0] = float(s * g.r.x - bm * gmi.x);
-    force[3 * i + 1] = float(s * g.r.y - bm * gmi.y);
-    force[3 * i + 2] = float(s * g.r.z - bm * gmi.z);
+    force[3 * i + 0] = R(s * g.r.x - bm * gmi.x);
+    force[3 * i + 1] = R(s * g.r.y - bm * gmi.y);
+    force[3 * i + 2] = R(s * g.r.z - bm * gmi.z);
   }
   if (near_parallel) {
     // Near-parallel edges keep only the Gauss-Newton term m b'' g g^T.
-    C.cn = 0.0f;
-    C.cg = float(4.0 * m * B.d2b * d2);
+    C.cn = R(0);
+    C.cg = R(4.0 * m * B.d2b * d2);
   } else {
     // Outside the mollified band m = 1 and grad m = 0, so the reference's
     // b * H_m and PSD(sym(grad m, b' g)) terms are exactly zero.
-    C.cn = float(B.db);
-    C.cg = float(4.0 * B.d2b * d2);
+    C.cn = R(B.db);
+    C.cg = R(4.0 * B.d2b * d2);
   }
   nsd_compressed(g, !near_parallel, C);
   return true;
@@ -84,10 +86,11 @@ __device__ __forceinline__ bool pair_ee(const float* __restrict__ x, const float
 // contact_hessian_pt (dynamic_contacts.py:209-246). The record is
 // (p, t1, t0, t2) and slots 1..3 are the triangle (a, b, c) of
 // fetch_dist_v0v1_pt (contact.py:116-145).
-__device__ __forceinline__ bool pair_pt(const float* __restrict__ x, int4 c, float l0, double kappa,
-                                        bool want_hessian, CompressedHessian& C, float force[12], double* energy) {
-  V3<float> xf[4] = {ldx(x, c.x), ldx(x, c.y), ldx(x, c.z), ldx(x, c.w)};
-  TriClosest tc = closest_point_triangle(xf[0], xf[1], xf[2], xf[3]);
+template <typename P, typename R>
+__device__ __forceinline__ bool pair_pt(const P* __restrict__ x, int4 c, P l0, double kappa, bool want_hessian,
+                                        CompressedHessianT<R>& C, R force[12], double* energy) {
+  V3<P> xf[4] = {ldx(x, c.x), ldx(x, c.y), ldx(x, c.z), ldx(x, c.w)};
+  TriClosestT<P> tc = closest_point_triangle(xf[0], xf[1], xf[2], xf[3]);
   if (!(tc.dist < l0)) return false;
   Feature f = pt_feature(tc.type);
   V3<double> xo = to_d(xf[0]);
@@ -97,14 +100,14 @@ __device__ __forceinline__ bool pair_pt(const float* __restrict__ x, int4 c, flo
   Barrier B = barrier(d2, double(l0) * double(l0), kappa);
   if (energy) *energy = B.b;
   if (!want_hessian) return true;
-  C.cn = float(B.db);
-  C.cg = float(4.0 * B.d2b * d2);
+  C.cn = R(B.db);
+  C.cg = R(4.0 * B.d2b * d2);
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     double s = -2.0 * B.db * g.w[i];
-    force[3 * i + 0] = float(s * g.r.x);
-    force[3 * i + 1] = float(s * g.r.y);
-    force[3 * i + 2] = float(s * g.r.z);
+    force[3 * i + 0] = R(s * g.r.x);
+    force[3 * i + 1] = R(s * g.r.y);
+    force[3 * i + 2] = R(s * g.r.z);
   }
   nsd_compressed(g, true, C);
   return true;
@@ -126,28 +129,34 @@ __device__ __forceinline__ int contact_slot(int lpw, int block) {
   return lane < lpw ? wave * lpw + lane : -1;
 }
 
-// One contact list's Hessian launch (or half of a paired launch).
-struct HessianArgs {
-  const float* x;
-  const float* xcs;
+// One contact list's Hessian launch (or half of a paired launch). P is the
+// positions' type, R the outputs' (vals, b): float and float, or double and
+// double with fp64 scalars.
+template <typename P, typename R>
+struct HessianArgsT {
+  const P* x;
+  const P* xcs;
   int nv;
   const int4* contacts;
   int n;
-  float l0;
+  P l0;
   double kappa;
   int offset;
   int* rows;
   int* cols;
-  float* vals;
-  float* b;
+  R* vals;
+  R* b;
   int lpw;
 };
+using HessianArgs = HessianArgsT<float, float>;
 
 // Per wave in LDS: the forces on their way to b (12 per contact slot), the
-// slots' 4 vertex ids and active flags, and one block row (4 blocks, 9
-// float4) of every slot's triplet values on their way to memory.
-struct HessianStage {
-  float f[4][64 * 12];
+// slots' 4 vertex ids and active flags, and 144 bytes of every slot's
+// triplet values on their way to memory (one block row in fp32, half of one
+// in fp64).
+template <typename R>
+struct HessianStageT {
+  R f[4][64 * 12];
   int v[4][64 * 4];
   int act[4][64];
   float4 row[4][64 * 9];
@@ -161,33 +170,34 @@ struct HessianStage {
 // took about half of the kernel's time. Row and column indices are stored
 // from the staged vertex ids; the values go block row by block row through
 // LDS (each lane expands its contact's row q, then the wave stores the 64
-// contacts' rows q, 144 contiguous bytes per contact).
-template <bool EE>
-__device__ __forceinline__ void hessian_block(const HessianArgs& A, int block, HessianStage& st) {
-  const float* __restrict__ x = A.x;
-  const float* __restrict__ xcs = A.xcs;
+// contacts' rows q, 144 contiguous bytes per contact; in fp64 a row goes in
+// two halves of 144 bytes).
+template <bool EE, typename P, typename R>
+__device__ __forceinline__ void hessian_block(const HessianArgsT<P, R>& A, int block, HessianStageT<R>& st) {
+  const P* __restrict__ x = A.x;
+  const P* __restrict__ xcs = A.xcs;
   const int nv = A.nv, n = A.n, offset = A.offset, lpw = A.lpw;
-  const float l0 = A.l0;
+  const P l0 = A.l0;
   const double kappa = A.kappa;
-  float* __restrict__ b = A.b;
+  R* __restrict__ b = A.b;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i = contact_slot(lpw, block);
-  float force[12];
+  R force[12];
 #pragma unroll
-  for (int k = 0; k < 12; ++k) force[k] = 0.0f;
+  for (int k = 0; k < 12; ++k) force[k] = R(0);
   int4 c = make_int4(0, 0, 0, 0);
-  CompressedHessian C;
+  CompressedHessianT<R> C;
   bool active = false;
   if (i >= 0 && i < n) {
     c = A.contacts[i];
-    active = valid4(c, nv) && (EE ? pair_ee(x, xcs, c, l0, kappa, true, C, force, nullptr)
-                                  : pair_pt(x, c, l0, kappa, true, C, force, nullptr));
+    active = valid4(c, nv) && (EE ? pair_ee<P, R>(x, xcs, c, l0, kappa, true, C, force, nullptr)
+                                  : pair_pt<P, R>(x, c, l0, kappa, true, C, force, nullptr));
   }
   if (!active) {
 #pragma unroll
-    for (int k = 0; k < 12; ++k) force[k] = 0.0f;
+    for (int k = 0; k < 12; ++k) force[k] = R(0);
   }
-  float* sf = st.f[wave];
+  R* sf = st.f[wave];
   int* sv = st.v[wave];
   int* sa = st.act[wave];
   float4* sr = st.row[wave];
@@ -221,29 +231,31 @@ __device__ __forceinline__ void hessian_block(const HessianArgs& A, int block, H
       cols4[t] = cv;
     }
   }
-  // values, block row by block row
+  // values, 144 bytes at a time: kPer consecutive 3x3 blocks of a block row
+  // (4 in fp32, 2 in fp64) = 9 16-byte units; a contact holds 36 kHalves
+  // units, kHalves per block row.
+  constexpr int kHalves = int(sizeof(R)) / 4;
+  constexpr int kPer = 4 / kHalves;
   float4* v4 = reinterpret_cast<float4*>(A.vals + tbase * 9);
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < 4 * kHalves; ++q) {
     if (lane < lpw) {
-      // Four consecutive 3x3 blocks = 36 floats = 9 float4.
-      float blk[36];
+      R blk[9 * kPer];
       if (active) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) expand_block(C, q, r, blk + 9 * r);
+        for (int r = 0; r < kPer; ++r) expand_block(C, q / kHalves, (q % kHalves) * kPer + r, blk + 9 * r);
       } else {
 #pragma unroll
-        for (int k = 0; k < 36; ++k) blk[k] = 0.0f;
+        for (int k = 0; k < 9 * kPer; ++k) blk[k] = R(0);
       }
 #pragma unroll
-      for (int k = 0; k < 9; ++k)
-        sr[lane * 9 + k] = make_float4(blk[4 * k], blk[4 * k + 1], blk[4 * k + 2], blk[4 * k + 3]);
+      for (int k = 0; k < 9; ++k) __builtin_memcpy(&sr[lane * 9 + k], blk + k * (16 / int(sizeof(R))), 16);
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    // slot s's row q: float4 s * 36 + q * 9 + k of the wave's range
+    // slot s's part q: unit s * 36 kHalves + q * 9 + k of the wave's range
     for (int t = lane; t < nw * 9; t += 64) {
       const int s = t / 9, k = t - 9 * s;
-      v4[s * 36 + q * 9 + k] = sr[t];
+      v4[s * 36 * kHalves + q * 9 + k] = sr[t];
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   }
@@ -255,26 +267,28 @@ __device__ __forceinline__ void hessian_block(const HessianArgs& A, int block, H
   // adjacent instead of one line per lane. A zero force (a vertex with zero
   // weight in the contact's feature, or no active contact) is not added.
   for (int j = lane; j < 12 * lpw; j += 64) {
-    const float v = sf[j];
-    if (v != 0.0f) {
+    const R v = sf[j];
+    if (v != R(0)) {
       const int slot = j / 12, comp = j - 12 * slot, vert = comp / 3;
       unsafeAtomicAdd(b + 3 * (long long)sv[slot * 4 + vert] + (comp - 3 * vert), v);
     }
   }
 }
 
-template <bool EE>
-__global__ void __launch_bounds__(256) contact_hessian_kernel(HessianArgs A) {
-  __shared__ HessianStage st;
+template <bool EE, typename P = float, typename R = float>
+__global__ void __launch_bounds__(256) contact_hessian_kernel(HessianArgsT<P, R> A) {
+  __shared__ HessianStageT<R> st;
   hessian_block<EE>(A, blockIdx.x, st);
 }
 
 // Both halves of the contact Hessian in one launch: workgroups [0, blocks_e)
 // take the EE list, the rest the PT list.
-__global__ void __launch_bounds__(256) contact_hessian_pair_kernel(HessianArgs E, HessianArgs P, int blocks_e) {
-  __shared__ HessianStage st;
+template <typename P = float, typename R = float>
+__global__ void __launch_bounds__(256) contact_hessian_pair_kernel(HessianArgsT<P, R> E, HessianArgsT<P, R> T,
+                                                                   int blocks_e) {
+  __shared__ HessianStageT<R> st;
   if (int(blockIdx.x) < blocks_e) hessian_block<true>(E, blockIdx.x, st);
-  else hessian_block<false>(P, blockIdx.x - blocks_e, st);
+  else hessian_block<false>(T, blockIdx.x - blocks_e, st);
 }
 
 // ------------------------------------------------------------ energy kernel ----
@@ -287,35 +301,37 @@ __device__ __forceinline__ double wave_sum(double v) {
 // contact_energy_ee / contact_energy_pt (dynamic_contacts.py:374-398): the
 // per-contact energies are summed in FP64 per wave, per block through LDS,
 // then one atomicAdd(double) per block into energy[0].
-struct EnergyArgs {
-  const float* x;
-  const float* xcs;
+template <typename P>
+struct EnergyArgsT {
+  const P* x;
+  const P* xcs;
   int nv;
   const int4* contacts;
   int n;
-  float l0;
+  P l0;
   double kappa;
   double* energy;
   int lpw;
 };
+using EnergyArgs = EnergyArgsT<float>;
 
-template <bool EE>
-__device__ __forceinline__ void energy_block(const EnergyArgs& A, int block, double* partial) {
-  const float* __restrict__ x = A.x;
-  const float* __restrict__ xcs = A.xcs;
+template <bool EE, typename P>
+__device__ __forceinline__ void energy_block(const EnergyArgsT<P>& A, int block, double* partial) {
+  const P* __restrict__ x = A.x;
+  const P* __restrict__ xcs = A.xcs;
   const int nv = A.nv, n = A.n;
-  const float l0 = A.l0;
+  const P l0 = A.l0;
   const double kappa = A.kappa;
   double* __restrict__ energy = A.energy;
   int i = contact_slot(A.lpw, block);
   double e = 0.0;
   if (i >= 0 && i < n) {
-    CompressedHessian C;
-    float force[12];
+    CompressedHessianT<P> C;
+    P force[12];
     double ei = 0.0;
     int4 c = A.contacts[i];
-    bool active = valid4(c, nv) && (EE ? pair_ee(x, xcs, c, l0, kappa, false, C, force, &ei)
-                                       : pair_pt(x, c, l0, kappa, false, C, force, &ei));
+    bool active = valid4(c, nv) && (EE ? pair_ee<P, P>(x, xcs, c, l0, kappa, false, C, force, &ei)
+                                       : pair_pt<P, P>(x, c, l0, kappa, false, C, force, &ei));
     if (active) e = ei;
   }
   e = wave_sum(e);
@@ -329,18 +345,19 @@ __device__ __forceinline__ void energy_block(const EnergyArgs& A, int block, dou
   }
 }
 
-template <bool EE>
-__global__ void __launch_bounds__(256) contact_energy_kernel(EnergyArgs A) {
+template <bool EE, typename P = float>
+__global__ void __launch_bounds__(256) contact_energy_kernel(EnergyArgsT<P> A) {
   __shared__ double partial[4];
   energy_block<EE>(A, blockIdx.x, partial);
 }
 
 // Both halves of the barrier energy in one launch (workgroups [0, blocks_e)
 // the EE list, the rest the PT list), both added into A.energy.
-__global__ void __launch_bounds__(256) contact_energy_pair_kernel(EnergyArgs E, EnergyArgs P, int blocks_e) {
+template <typename P = float>
+__global__ void __launch_bounds__(256) contact_energy_pair_kernel(EnergyArgsT<P> E, EnergyArgsT<P> T, int blocks_e) {
   __shared__ double partial[4];
   if (int(blockIdx.x) < blocks_e) energy_block<true>(E, blockIdx.x, partial);
-  else energy_block<false>(P, blockIdx.x - blocks_e, partial);
+  else energy_block<false>(T, blockIdx.x - blocks_e, partial);
 }
 
 // ----------------------------------------------------- contact points kernel ----
@@ -625,7 +642,35 @@ hipError_t launch_collision_hessian(const float* x, const float* xcs, int nv, co
   const HessianArgs E = hessian_args(x, xcs, nv, ee, n_ee, l0, kappa, 0, rows, cols, vals, b);
   const HessianArgs P = hessian_args(x, nullptr, nv, pt, n_pt, l0, kappa, n_ee, rows, cols, vals, b);
   const int be = blocks_spread(n_ee, E.lpw), bp = blocks_spread(n_pt, P.lpw);
-  hipLaunchKernelGGL(contact_hessian_pair_kernel, dim3(be + bp), dim3(256), 0, s, E, P, be);
+  hipLaunchKernelGGL(contact_hessian_pair_kernel<>, dim3(be + bp), dim3(256), 0, s, E, P, be);
+  return hipGetLastError();
+}
+
+// fp64 scalars: positions, blocks and forces in double.
+hipError_t launch_contact_hessian_f64(bool ee, const double* x, const double* xcs, int nv, const int* contacts, int n,
+                                      double l0, double kappa, int offset, int* rows, int* cols, double* vals,
+                                      double* b, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const HessianArgsT<double, double> A{x, xcs, nv, reinterpret_cast<const int4*>(contacts), n, l0, kappa, offset,
+                                       rows, cols, vals, b, lanes_per_wave(n)};
+  if (ee)
+    hipLaunchKernelGGL((contact_hessian_kernel<true, double, double>), dim3(blocks_spread(n, A.lpw)), dim3(256), 0, s,
+                       A);
+  else
+    hipLaunchKernelGGL((contact_hessian_kernel<false, double, double>), dim3(blocks_spread(n, A.lpw)), dim3(256), 0,
+                       s, A);
+  return hipGetLastError();
+}
+
+hipError_t launch_contact_energy_f64(bool ee, const double* x, const double* xcs, int nv, const int* contacts, int n,
+                                     double l0, double kappa, double* energy, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const EnergyArgsT<double> A{x, xcs, nv, reinterpret_cast<const int4*>(contacts), n, l0, kappa, energy,
+                              lanes_per_wave(n, kSpreadLanesEnergy)};
+  if (ee)
+    hipLaunchKernelGGL((contact_energy_kernel<true, double>), dim3(blocks_spread(n, A.lpw)), dim3(256), 0, s, A);
+  else
+    hipLaunchKernelGGL((contact_energy_kernel<false, double>), dim3(blocks_spread(n, A.lpw)), dim3(256), 0, s, A);
   return hipGetLastError();
 }
 
@@ -655,7 +700,7 @@ hipError_t launch_collision_energy(const float* x, const float* xcs, int nv, con
   const EnergyArgs E = energy_args(x, xcs, nv, ee, n_ee, l0, kappa, energy);
   const EnergyArgs P = energy_args(x, nullptr, nv, pt, n_pt, l0, kappa, energy);
   const int be = blocks_spread(n_ee, E.lpw), bp = blocks_spread(n_pt, P.lpw);
-  hipLaunchKernelGGL(contact_energy_pair_kernel, dim3(be + bp), dim3(256), 0, s, E, P, be);
+  hipLaunchKernelGGL(contact_energy_pair_kernel<>, dim3(be + bp), dim3(256), 0, s, E, P, be);
   return hipGetLastError();
 }
 

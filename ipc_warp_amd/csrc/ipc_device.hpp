@@ -41,81 +41,96 @@ template <typename T> IPCD_FN T smax(T a, T b) { return a > b ? a : b; }
 template <typename T> IPCD_FN V3<T> vmin(V3<T> a, V3<T> b) { return {smin(a.x, b.x), smin(a.y, b.y), smin(a.z, b.z)}; }
 template <typename T> IPCD_FN V3<T> vmax(V3<T> a, V3<T> b) { return {smax(a.x, b.x), smax(a.y, b.y), smax(a.z, b.z)}; }
 IPCD_FN V3<double> to_d(V3<float> a) { return {double(a.x), double(a.y), double(a.z)}; }
+IPCD_FN V3<double> to_d(V3<double> a) { return a; }
 
 // Vertex positions are the reference's vec3 array: float32 [V, 3], AoS.
 IPCD_FN V3<float> ldx(const float* __restrict__ x, int i) {
   const float* p = x + 3 * (long long)i;
   return {p[0], p[1], p[2]};
 }
+// With fp64 scalars (scalar_types.py USE_FLOAT32 = False) they are vec3d.
+IPCD_FN V3<double> ldx(const double* __restrict__ x, int i) {
+  const double* p = x + 3 * (long long)i;
+  return {p[0], p[1], p[2]};
+}
+
+// A constant of the reference's `scalar` type: the float literal in fp32,
+// the double literal in fp64 (not the float value widened).
+template <typename T> IPCD_FN T lit(float f, double d);
+template <> IPCD_FN float lit<float>(float f, double) { return f; }
+template <> IPCD_FN double lit<double>(float, double d) { return d; }
 
 // ------------------------------------------------- closest point, triangle ----
 // Voronoi-region closest point (contact.py:230-302). Feature: 0/1/2 vertex
 // a/b/c, 3 edge ab, 4 edge ac, 5 edge bc, 6 face. Returns the distance.
-struct TriClosest { float wa, wb, dist; int type; };
-IPCD_FN TriClosest closest_point_triangle(V3<float> p, V3<float> a, V3<float> b, V3<float> c) {
-  const float zero = 0.0f, one = 1.0f;
-  V3<float> ab = b - a, ac = c - a, ap = p - a;
-  float d1 = dot(ab, ap), d2 = dot(ac, ap);
+template <typename T> struct TriClosestT { T wa, wb, dist; int type; };
+using TriClosest = TriClosestT<float>;
+template <typename T>
+IPCD_FN TriClosestT<T> closest_point_triangle(V3<T> p, V3<T> a, V3<T> b, V3<T> c) {
+  const T zero = T(0), one = T(1);
+  V3<T> ab = b - a, ac = c - a, ap = p - a;
+  T d1 = dot(ab, ap), d2 = dot(ac, ap);
   if (d1 <= zero && d2 <= zero) return {one, zero, length(p - a), 0};
-  V3<float> bp = p - b;
-  float d3 = dot(ab, bp), d4 = dot(ac, bp);
+  V3<T> bp = p - b;
+  T d3 = dot(ab, bp), d4 = dot(ac, bp);
   if (d3 >= zero && d4 <= d3) return {zero, one, length(p - b), 1};
-  float vc = d1 * d4 - d3 * d2;
+  T vc = d1 * d4 - d3 * d2;
   if (vc <= zero && d1 >= zero && d3 <= zero) {
-    float v = d1 / (d1 - d3);
-    V3<float> point = a + v * ab;
+    T v = d1 / (d1 - d3);
+    V3<T> point = a + v * ab;
     return {one - v, v, length(p - point), 3};
   }
-  V3<float> cp = p - c;
-  float d5 = dot(ab, cp), d6 = dot(ac, cp);
+  V3<T> cp = p - c;
+  T d5 = dot(ab, cp), d6 = dot(ac, cp);
   if (d6 >= zero && d5 <= d6) return {zero, zero, length(p - c), 2};
-  float vb = d5 * d2 - d1 * d6;
+  T vb = d5 * d2 - d1 * d6;
   if (vb <= zero && d2 >= zero && d6 <= zero) {
-    float w = d2 / (d2 - d6);
-    V3<float> point = a + w * ac;
+    T w = d2 / (d2 - d6);
+    V3<T> point = a + w * ac;
     return {one - w, zero, length(p - point), 4};
   }
-  float va = d3 * d6 - d5 * d4;
+  T va = d3 * d6 - d5 * d4;
   if (va <= zero && (d4 - d3) >= zero && (d5 - d6) >= zero) {
-    float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
-    V3<float> point = b + w * (c - b);
+    T w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    V3<T> point = b + w * (c - b);
     return {zero, one - w, length(p - point), 5};
   }
-  float denom = one / (va + vb + vc);
-  float v = vb * denom, w = vc * denom;
-  float u = one - v - w;
-  V3<float> point = a + ab * v + ac * w;
+  T denom = one / (va + vb + vc);
+  T v = vb * denom, w = vc * denom;
+  T u = one - v - w;
+  V3<T> point = a + ab * v + ac * w;
   return {u, v, length(p - point), 6};
 }
 
 // ------------------------------------------------- closest points, segments ----
 // Segment-segment closest points (Ericson, Real-Time Collision Detection
 // 5.1.9), the role of Warp's closest_point_edge_edge(p1, q1, p2, q2, eps).
-struct EdgeClosest { float s, t, dist; };
-IPCD_FN float clamp01(float v) { return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v); }
-IPCD_FN EdgeClosest closest_point_edge_edge(V3<float> p1, V3<float> q1, V3<float> p2, V3<float> q2,
-                                                               float eps) {
-  V3<float> d1 = q1 - p1, d2 = q2 - p2, r = p1 - p2;
-  float a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
-  float s, t;
+template <typename T> struct EdgeClosestT { T s, t, dist; };
+using EdgeClosest = EdgeClosestT<float>;
+template <typename T> IPCD_FN T clamp_unit(T v) { return v < T(0) ? T(0) : (v > T(1) ? T(1) : v); }
+template <typename T>
+IPCD_FN EdgeClosestT<T> closest_point_edge_edge(V3<T> p1, V3<T> q1, V3<T> p2, V3<T> q2, T eps) {
+  V3<T> d1 = q1 - p1, d2 = q2 - p2, r = p1 - p2;
+  T a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
+  T s, t;
   if (a <= eps && e <= eps) {
-    s = 0.0f; t = 0.0f;
+    s = T(0); t = T(0);
   } else if (a <= eps) {
-    s = 0.0f; t = clamp01(f / e);
+    s = T(0); t = clamp_unit(f / e);
   } else {
-    float c = dot(d1, r);
+    T c = dot(d1, r);
     if (e <= eps) {
-      t = 0.0f; s = clamp01(-c / a);
+      t = T(0); s = clamp_unit(-c / a);
     } else {
-      float b = dot(d1, d2);
-      float denom = a * e - b * b;
-      s = denom != 0.0f ? clamp01((b * f - c * e) / denom) : 0.0f;
+      T b = dot(d1, d2);
+      T denom = a * e - b * b;
+      s = denom != T(0) ? clamp_unit((b * f - c * e) / denom) : T(0);
       t = (b * s + f) / e;
-      if (t < 0.0f) { t = 0.0f; s = clamp01(-c / a); }
-      else if (t > 1.0f) { t = 1.0f; s = clamp01((b - c) / a); }
+      if (t < T(0)) { t = T(0); s = clamp_unit(-c / a); }
+      else if (t > T(1)) { t = T(1); s = clamp_unit((b - c) / a); }
     }
   }
-  V3<float> c1 = p1 + d1 * s, c2 = p2 + d2 * t;
+  V3<T> c1 = p1 + d1 * s, c2 = p2 + d2 * t;
   return {s, t, length(c1 - c2)};
 }
 
@@ -140,18 +155,19 @@ enum { F_PP = 0, F_PE = 1, F_PT = 2, F_EE = 3 };
 struct Feature { int kind; int s0, s1, s2, s3; };
 
 // edge_edge_distance_gradient_hessian's dispatch (dynamic_contacts.py:48-108)
-// on the already computed fp32 closest points.
-IPCD_FN Feature ee_feature(EdgeClosest ec, V3<float> cr, float parallel_eps) {
-  const float fe = 1.0e-6f, one = 1.0f;
-  float s = ec.s, t = ec.t;
+// on the closest points, computed in the reference's `scalar` type.
+template <typename T>
+IPCD_FN Feature ee_feature(EdgeClosestT<T> ec, V3<T> cr, T parallel_eps) {
+  const T fe = lit<T>(1.0e-6f, 1.0e-6), one = T(1);
+  T s = ec.s, t = ec.t;
   bool sb = s <= fe || s >= one - fe;
   bool tb = t <= fe || t >= one - fe;
   if (sb && tb) return {F_PP, (s >= one - fe) ? 1 : 0, (t >= one - fe) ? 3 : 2, -1, -1};
   if (sb) return {F_PE, (s >= one - fe) ? 1 : 0, 2, 3, -1};
   if (tb) return {F_PE, (t >= one - fe) ? 3 : 2, 0, 1, -1};
-  if (parallel_eps > 0.0f && dot(cr, cr) < parallel_eps) {
-    float ds0 = s, ds1 = one - s, dt0 = t, dt1 = one - t;
-    float m = smin(smin(ds0, ds1), smin(dt0, dt1));
+  if (parallel_eps > T(0) && dot(cr, cr) < parallel_eps) {
+    T ds0 = s, ds1 = one - s, dt0 = t, dt1 = one - t;
+    T m = smin(smin(ds0, ds1), smin(dt0, dt1));
     if (m == ds0) return {F_PE, 0, 2, 3, -1};
     if (m == ds1) return {F_PE, 1, 2, 3, -1};
     if (m == dt0) return {F_PE, 2, 0, 1, -1};
@@ -370,15 +386,18 @@ IPCD_FN void jacobi_sym(const T A[n * n], T ev[n], T V[n * n], T tol2) {
 
 // Compressed form of the 16 blocks of a contact Hessian
 //     H_ij = c_n (w^_i w^_j Ntt + w^_i u_j r^T + w^_j r u_i^T) + (cg w_i w_j + c_n s_ij) r r^T
-// with r the unit closest-point direction (see feature_geom). Stored in fp32.
-struct CompressedHessian {
-  float what[4], w[4];
-  float rh[3];
-  float Ntt[6];     // xx, xy, xz, yy, yz, zz of the NSD part in the w^ (x) I3 block
-  float u[4][3];    // Ntz z_i
-  float s[10];      // z_i^T Nzz z_j, packed upper triangle (i <= j)
-  float cn, cg;
+// with r the unit closest-point direction (see feature_geom). Stored in the
+// output's precision R (fp32, or fp64 with fp64 scalars).
+template <typename R>
+struct CompressedHessianT {
+  R what[4], w[4];
+  R rh[3];
+  R Ntt[6];     // xx, xy, xz, yy, yz, zz of the NSD part in the w^ (x) I3 block
+  R u[4][3];    // Ntz z_i
+  R s[10];      // z_i^T Nzz z_j, packed upper triangle (i <= j)
+  R cn, cg;
 };
+using CompressedHessian = CompressedHessianT<float>;
 
 IPCD_FN int sidx(int i, int j) {
   // packed index of (min, max) in a 4x4 upper triangle
@@ -390,7 +409,8 @@ IPCD_FN int sidx(int i, int j) {
 // (3 + k)-dimensional subspace span{w^ (x) R^3, Z (x) r^}, Z an orthonormal
 // basis of the D_j projected orthogonal to w. Fills Ntt, u, s and the
 // geometry fields of `C`; returns |r|^2 (the feature's squared distance).
-IPCD_FN double nsd_compressed(const FeatureGeom& g, bool want_nsd, CompressedHessian& C) {
+template <typename R>
+IPCD_FN double nsd_compressed(const FeatureGeom& g, bool want_nsd, CompressedHessianT<R>& C) {
 #pragma clang fp contract(fast)
   double d2 = dot(g.r, g.r);
   double rn = sqrt(d2);
@@ -399,14 +419,14 @@ IPCD_FN double nsd_compressed(const FeatureGeom& g, bool want_nsd, CompressedHes
   double wn = sqrt(wn2), iwn = inv_sqrt(wn2);
   double wh[4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) { wh[i] = g.w[i] * iwn; C.what[i] = float(wh[i]); C.w[i] = float(g.w[i]); }
-  C.rh[0] = float(rh.x); C.rh[1] = float(rh.y); C.rh[2] = float(rh.z);
+  for (int i = 0; i < 4; ++i) { wh[i] = g.w[i] * iwn; C.what[i] = R(wh[i]); C.w[i] = R(g.w[i]); }
+  C.rh[0] = R(rh.x); C.rh[1] = R(rh.y); C.rh[2] = R(rh.z);
 #pragma unroll
-  for (int i = 0; i < 6; ++i) C.Ntt[i] = 0.0f;
+  for (int i = 0; i < 6; ++i) C.Ntt[i] = R(0);
 #pragma unroll
-  for (int i = 0; i < 4; ++i) { C.u[i][0] = C.u[i][1] = C.u[i][2] = 0.0f; }
+  for (int i = 0; i < 4; ++i) { C.u[i][0] = C.u[i][1] = C.u[i][2] = R(0); }
 #pragma unroll
-  for (int i = 0; i < 10; ++i) C.s[i] = 0.0f;
+  for (int i = 0; i < 10; ++i) C.s[i] = R(0);
   if (!want_nsd || g.k == 0) return d2;
 
   // Z: Gram-Schmidt of the D_j orthogonal to w^.
@@ -507,18 +527,19 @@ IPCD_FN double nsd_compressed(const FeatureGeom& g, bool want_nsd, CompressedHes
       K[a * 4 + b] = v;
     }
   }
-  // The 4 x 4 eigenproblem runs in fp32 (native square roots and
-  // reciprocals; in FP64 their multi-instruction sequences were most of the
-  // kernel's VALU work): its projection's errors stay near fp32 rounding of
-  // the largest entry, below the 1e-5 bound of the blocks, while every input
-  // of K (feature geometry, distances) stays FP64. K is scaled by its
-  // largest diagonal magnitude first so that fp32's range cannot matter.
+  // The 4 x 4 eigenproblem runs in the output's precision. In fp32 (native
+  // square roots and reciprocals; in FP64 their multi-instruction sequences
+  // were most of the kernel's VALU work) its projection's errors stay near
+  // fp32 rounding of the largest entry, below the 1e-5 bound of the blocks,
+  // while every input of K (feature geometry, distances) stays FP64. K is
+  // scaled by its largest diagonal magnitude first so that fp32's range
+  // cannot matter.
   double kmax = fmax(fmax(fabs(K[0]), fabs(K[5])), fmax(fabs(K[10]), fabs(K[15])));
   const double ks = kmax > 0.0 ? 1.0 / kmax : 1.0;
-  float Kf[16], evf[4], V[16];
+  R Kf[16], evf[4], V[16];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) Kf[i] = float(K[i] * ks);
-  jacobi_sym<4, float>(Kf, evf, V, 1.0e-14f);
+  for (int i = 0; i < 16; ++i) Kf[i] = R(K[i] * ks);
+  jacobi_sym<4, R>(Kf, evf, V, lit<R>(1.0e-14f, 1.0e-28));
   double ev[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) ev[i] = double(evf[i]) * kmax;
@@ -527,11 +548,11 @@ IPCD_FN double nsd_compressed(const FeatureGeom& g, bool want_nsd, CompressedHes
   // y = Q v (y0..2 = v0 q1 + v1 q2, y3 = v2, y4 = v3) and zeta_i = Z_i.(y3, y4);
   // then Ntt += lambda y y^T (3 x 3 block), u_i += lambda zeta_i y0..2 and
   // s_ij += lambda zeta_i zeta_j.
-  float ntt[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, uu[4][3], ss[10];
+  R ntt[6] = {R(0), R(0), R(0), R(0), R(0), R(0)}, uu[4][3], ss[10];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) uu[i][0] = uu[i][1] = uu[i][2] = 0.f;
+  for (int i = 0; i < 4; ++i) uu[i][0] = uu[i][1] = uu[i][2] = R(0);
 #pragma unroll
-  for (int i = 0; i < 10; ++i) ss[i] = 0.f;
+  for (int i = 0; i < 10; ++i) ss[i] = R(0);
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
     double lam = ev[m];
@@ -541,20 +562,20 @@ IPCD_FN double nsd_compressed(const FeatureGeom& g, bool want_nsd, CompressedHes
       for (int a = 0; a < 3; ++a) y[a] = V[0 * 4 + m] * q1[a] + V[1 * 4 + m] * q2[a];
 #pragma unroll
       for (int i = 0; i < 4; ++i) zeta[i] = Z[i][0] * V[2 * 4 + m] + Z[i][1] * V[3 * 4 + m];
-      ntt[0] += float(lam * y[0] * y[0]);
-      ntt[1] += float(lam * y[0] * y[1]);
-      ntt[2] += float(lam * y[0] * y[2]);
-      ntt[3] += float(lam * y[1] * y[1]);
-      ntt[4] += float(lam * y[1] * y[2]);
-      ntt[5] += float(lam * y[2] * y[2]);
+      ntt[0] += R(lam * y[0] * y[0]);
+      ntt[1] += R(lam * y[0] * y[1]);
+      ntt[2] += R(lam * y[0] * y[2]);
+      ntt[3] += R(lam * y[1] * y[1]);
+      ntt[4] += R(lam * y[1] * y[2]);
+      ntt[5] += R(lam * y[2] * y[2]);
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int a = 0; a < 3; ++a) uu[i][a] += float(lam * zeta[i] * y[a]);
+        for (int a = 0; a < 3; ++a) uu[i][a] += R(lam * zeta[i] * y[a]);
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = i; j < 4; ++j) ss[sidx(i, j)] += float(lam * zeta[i] * zeta[j]);
+        for (int j = i; j < 4; ++j) ss[sidx(i, j)] += R(lam * zeta[i] * zeta[j]);
     }
   }
 #pragma unroll
@@ -569,12 +590,13 @@ IPCD_FN double nsd_compressed(const FeatureGeom& g, bool want_nsd, CompressedHes
 }
 
 // Block (i, j) of the compressed Hessian, row-major 3x3.
-IPCD_FN void expand_block(const CompressedHessian& C, int i, int j, float out[9]) {
+template <typename R>
+IPCD_FN void expand_block(const CompressedHessianT<R>& C, int i, int j, R out[9]) {
 #pragma clang fp contract(fast)
-  float ww = C.what[i] * C.what[j];
-  float rr = C.cg * C.w[i] * C.w[j] + C.cn * C.s[sidx(i, j)];
-  const float* r = C.rh;
-  const float Nt[9] = {C.Ntt[0], C.Ntt[1], C.Ntt[2], C.Ntt[1], C.Ntt[3], C.Ntt[4], C.Ntt[2], C.Ntt[4], C.Ntt[5]};
+  R ww = C.what[i] * C.what[j];
+  R rr = C.cg * C.w[i] * C.w[j] + C.cn * C.s[sidx(i, j)];
+  const R* r = C.rh;
+  const R Nt[9] = {C.Ntt[0], C.Ntt[1], C.Ntt[2], C.Ntt[1], C.Ntt[3], C.Ntt[4], C.Ntt[2], C.Ntt[4], C.Ntt[5]};
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll

@@ -9,6 +9,12 @@ hipError_t launch_contact_hessian(bool ee, const float* x, const float* xcs, int
                                   hipStream_t s);
 hipError_t launch_contact_energy(bool ee, const float* x, const float* xcs, int nv, const int* contacts, int n, float l0,
                                  double kappa, double* energy, hipStream_t s);
+// The same with fp64 scalars (positions, blocks and forces in double).
+hipError_t launch_contact_hessian_f64(bool ee, const double* x, const double* xcs, int nv, const int* contacts, int n,
+                                      double l0, double kappa, int offset, int* rows, int* cols, double* vals,
+                                      double* b, hipStream_t s);
+hipError_t launch_contact_energy_f64(bool ee, const double* x, const double* xcs, int nv, const int* contacts, int n,
+                                     double l0, double kappa, double* energy, hipStream_t s);
 // Both contact lists in one launch: the Hessian with the PT triplets at
 // offset n_ee; the energy after zeroing *energy.
 hipError_t launch_collision_hessian(const float* x, const float* xcs, int nv, const int* ee, int n_ee, const int* pt,

@@ -111,6 +111,38 @@ def contact_energy_pt(x, contacts, n, l0, kappa, e):
                                       _need(e, torch.float64, "e"), stream_handle(x.device)), "contact_energy_pt")
 
 
+def contact_hessian_f64(kind, x, contacts, n, l0, kappa, rows, cols, vals, b, offset=0, xcs=None):
+    """contact_hessian_ee / _pt with fp64 scalars (scalar_types.py with
+    USE_FLOAT32 = False): x, xcs, b float64 [V,3], vals float64 [16*total,3,3],
+    l0 in double. kind is 'ee' (xcs required) or 'pt'."""
+    _check_triplets(rows, cols, vals, offset, n, x, b)
+    args = (_rows(contacts, n, "contacts"), int(n), float(l0), float(kappa), int(offset),
+            _need(rows, torch.int32, "rows"), _need(cols, torch.int32, "cols"),
+            _need(vals, torch.float64, "vals", (3, 3)), _need(b, torch.float64, "b", (3,)), stream_handle(x.device))
+    if kind == "ee":
+        if xcs is None or xcs.shape != x.shape:
+            raise ValueError("xcs (float64, shaped like x) is required for edge-edge contacts")
+        check(lib().ipc_contact_hessian_ee_f64(_need(x, torch.float64, "x", (3,)), _need(xcs, torch.float64, "xcs", (3,)),
+                                               x.shape[0], *args), "contact_hessian_ee_f64")
+    else:
+        check(lib().ipc_contact_hessian_pt_f64(_need(x, torch.float64, "x", (3,)), x.shape[0], *args),
+              "contact_hessian_pt_f64")
+
+
+def contact_energy_f64(kind, x, contacts, n, l0, kappa, e, xcs=None):
+    """contact_energy_ee / _pt with fp64 scalars: adds the sum into e[0]."""
+    args = (_rows(contacts, n, "contacts"), int(n), float(l0), float(kappa), _need(e, torch.float64, "e"),
+            stream_handle(x.device))
+    if kind == "ee":
+        if xcs is None or xcs.shape != x.shape:
+            raise ValueError("xcs (float64, shaped like x) is required for edge-edge contacts")
+        check(lib().ipc_contact_energy_ee_f64(_need(x, torch.float64, "x", (3,)), _need(xcs, torch.float64, "xcs", (3,)),
+                                              x.shape[0], *args), "contact_energy_ee_f64")
+    else:
+        check(lib().ipc_contact_energy_pt_f64(_need(x, torch.float64, "x", (3,)), x.shape[0], *args),
+              "contact_energy_pt_f64")
+
+
 def collision_hessian(x, xcs, ee, n_ee, pt, n_pt, l0, kappa, rows, cols, vals, b):
     """The contact part of RodComplexBC.compute_A (dynamic_contacts.py:281-288):
     contact_hessian_ee at triplet offset 0 and contact_hessian_pt at offset

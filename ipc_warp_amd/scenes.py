@@ -281,7 +281,7 @@ def _orthonormal_frames(rng, n):
 
 
 def pair_stream(kind: str, n: int, seed: int = 0, dhat: float = 1.0e-3, scale: float = 0.05,
-                center_spread: float = 1.0):
+                center_spread: float = 1.0, dtype=np.float32):
     """Synthetic active contacts with 4 private vertices each.
 
     Distances d/dhat are log-uniform on [1e-3, 1) with 15% forced below
@@ -289,8 +289,9 @@ def pair_stream(kind: str, n: int, seed: int = 0, dhat: float = 1.0e-3, scale: f
     fp32 spacing of coordinates in the unit cube). PT feature mix: 50% face, 30% edge, 20% vertex. EE mix: 70%
     interior, 20% endpoint, 10% near-parallel. Primitives have size
     ~`scale`, positions spread over a cube of side `center_spread`.
-    Returns (x float32 [4n,3], contacts int32 [n,4]) with contacts[i] =
-    4i .. 4i+3 in the reference's slot order (PT: p, t1, t0, t2).
+    Returns (x [4n,3] in `dtype` (float32, or float64 for the fp64-scalar
+    kernels), contacts int32 [n,4]) with contacts[i] = 4i .. 4i+3 in the
+    reference's slot order (PT: p, t1, t0, t2).
     """
     rng = np.random.default_rng(seed)
     u = rng.random(n)
@@ -357,7 +358,7 @@ def pair_stream(kind: str, n: int, seed: int = 0, dhat: float = 1.0e-3, scale: f
         b0[e] = pb[e] + L[e] * rise[e]
         x[:, 0], x[:, 1], x[:, 2], x[:, 3] = a0, a1, b0, b1
     contacts = np.arange(4 * n, dtype=np.int32).reshape(n, 4)
-    return x.reshape(-1, 3).astype(np.float32), contacts
+    return x.reshape(-1, 3).astype(dtype), contacts
 
 
 def ccd_stream(kind: str, n: int, seed: int = 0, hit_fraction: float = 0.3, coord: float = 10.0,

@@ -91,6 +91,10 @@ def lib():
         L.oracle_tet_rest.argtypes = [F32P, I32P, c_int, F32P, F32P]
         L.oracle_tet_hessian.argtypes = [F32P, I32P, c_int, F32P, F32P, c_flt, c_flt, I32P, I32P, F32P, F32P]
         L.oracle_tet_energy.argtypes = [F32P, I32P, c_int, F32P, F32P, c_flt, c_flt, F32P]
+        L.oracle_contact_hessian_f64.argtypes = [c_int, F64P, F64P, I32P, c_int, c_dbl, c_dbl, c_int, I32P, I32P,
+                                                 F64P, F64P, U8P, c_int]
+        L.oracle_contact_energy_f64.argtypes = [c_int, F64P, F64P, I32P, c_int, c_dbl, c_dbl]
+        L.oracle_contact_energy_f64.restype = c_dbl
         L.oracle_al_hash_pos.argtypes = [c_int, c_int, c_int]
         L.oracle_al_hash_pos.restype = c_int
         L.oracle_al_vertex_toi.argtypes = [c_int, I32P, c_int, ctypes.c_void_p, c_int]
@@ -212,6 +216,35 @@ def contact_hessian(kind, x, contacts, l0, kappa, xcs=None, offset=0, n_total=No
                                  float(l0), float(kappa), int(offset), _p(rows, I32P), _p(cols, I32P), _p(vals, F64P),
                                  _p(b, F64P), _p(active, U8P), int(threads))
     return rows, cols, vals, b, active.astype(bool)
+
+
+def contact_hessian_f64(kind, x, contacts, l0, kappa, xcs=None, offset=0, n_total=None, threads=1):
+    """contact_hessian_ee / _pt with fp64 scalars (double positions and l0).
+
+    Returns (rows, cols, vals [16*n_total,3,3], b [V,3], active [n])."""
+    x = _f64(x).reshape(-1, 3)
+    xcs = x if xcs is None else _f64(xcs).reshape(-1, 3)
+    contacts = _i32(contacts).reshape(-1, 4)
+    n = contacts.shape[0]
+    n_total = offset + n if n_total is None else n_total
+    rows = np.zeros(16 * n_total, dtype=np.int32)
+    cols = np.zeros(16 * n_total, dtype=np.int32)
+    vals = np.zeros((16 * n_total, 3, 3))
+    b = np.zeros((x.shape[0], 3))
+    active = np.zeros(max(n, 1), dtype=np.uint8)
+    lib().oracle_contact_hessian_f64(int(kind == "ee"), _p(x, F64P), _p(xcs, F64P), _p(contacts, I32P), n, float(l0),
+                                     float(kappa), int(offset), _p(rows, I32P), _p(cols, I32P), _p(vals, F64P),
+                                     _p(b, F64P), _p(active, U8P), int(threads))
+    return rows, cols, vals, b, active[:n].astype(bool)
+
+
+def contact_energy_f64(kind, x, contacts, l0, kappa, xcs=None):
+    """contact_energy_ee / _pt with fp64 scalars."""
+    x = _f64(x).reshape(-1, 3)
+    xcs = x if xcs is None else _f64(xcs).reshape(-1, 3)
+    contacts = _i32(contacts).reshape(-1, 4)
+    return float(lib().oracle_contact_energy_f64(int(kind == "ee"), _p(x, F64P), _p(xcs, F64P), _p(contacts, I32P),
+                                                 contacts.shape[0], float(l0), float(kappa)))
 
 
 def contact_energy(kind, x, contacts, l0, kappa, xcs=None, mode="mixed", threads=1):

@@ -300,6 +300,46 @@ void oracle_contact_hessian(int is_ee, int prec, int d2_policy, const float* x, 
   }
 }
 
+// The same with fp64 scalars (scalar_types.py with USE_FLOAT32 = False):
+// double positions, classification and barrier argument in double.
+void oracle_contact_hessian_f64(int is_ee, const double* x, const double* xcs, const int* contacts, int n, double l0,
+                                double kappa, int offset, int* rows, int* cols, double* vals, double* b,
+                                unsigned char* active, int threads) {
+  std::vector<PairResult> res(n > 0 ? n : 0);
+#pragma omp parallel for num_threads(threads > 0 ? threads : 1) schedule(dynamic, 64)
+  for (int i = 0; i < n; ++i) {
+    const int* c = contacts + 4 * i;
+    res[i] = is_ee ? contact_ee<double>(x, xcs, c, l0, kappa, 1) : contact_pt<double>(x, c, l0, kappa, 1);
+  }
+  for (int i = 0; i < n; ++i) {
+    const int* c = contacts + 4 * i;
+    const PairResult& r = res[i];
+    if (active) active[i] = r.active ? 1 : 0;
+    if (!r.active) continue;
+    for (int ii = 0; ii < 4; ++ii) {
+      for (int k = 0; k < 3; ++k) b[3 * c[ii] + k] += r.grad_force[3 * ii + k];
+      for (int jj = 0; jj < 4; ++jj) {
+        int64_t idx = int64_t(offset + i) * 16 + ii * 4 + jj;
+        rows[idx] = c[ii];
+        cols[idx] = c[jj];
+        for (int r3 = 0; r3 < 3; ++r3)
+          for (int c3 = 0; c3 < 3; ++c3) vals[idx * 9 + r3 * 3 + c3] = r.H[(ii * 3 + r3) * 12 + jj * 3 + c3];
+      }
+    }
+  }
+}
+
+double oracle_contact_energy_f64(int is_ee, const double* x, const double* xcs, const int* contacts, int n, double l0,
+                                 double kappa) {
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const int* c = contacts + 4 * i;
+    const PairResult r = is_ee ? contact_ee<double>(x, xcs, c, l0, kappa, 1) : contact_pt<double>(x, c, l0, kappa, 1);
+    if (r.active) s += r.energy;
+  }
+  return s;
+}
+
 // One contact at double positions x12 (rest12 for the EE mollifier
 // threshold): energy (returned), force (-gradient, 12), 12x12 Hessian and
 // the dispatched feature (kind, 4 slots; kind -1 when inactive). prec and

@@ -39,6 +39,7 @@ template <typename T> inline V3<T> vmin(V3<T> a, V3<T> b) { return {std::min(a.x
 template <typename T> inline V3<T> vmax(V3<T> a, V3<T> b) { return {std::max(a.x, b.x), std::max(a.y, b.y), std::max(a.z, b.z)}; }
 template <typename T, typename S> inline V3<T> cast3(V3<S> a) { return {T(a.x), T(a.y), T(a.z)}; }
 template <typename T> inline V3<T> load3(const float* p, int i) { return {T(p[3 * i]), T(p[3 * i + 1]), T(p[3 * i + 2])}; }
+template <typename T> inline V3<T> load3(const double* p, int i) { return {T(p[3 * i]), T(p[3 * i + 1]), T(p[3 * i + 2])}; }
 
 // --------------------------------------------- closest point on triangle ----
 // contact.py:230-302 (Ericson, Real-Time Collision Detection 5.1.5).
@@ -405,9 +406,10 @@ template <typename TC> inline V3<TC> cast3(const V3<double>& v) { return V3<TC>{
 // contact_hessian_ee (dynamic_contacts.py:151-207) + contact_energy_ee (:374-388)
 // at positions xd (rest positions rd for the mollifier threshold). The
 // reference's buffers are fp32: a caller passing fp32 values widened to
-// double gets them back exactly in the TC = float classification.
+// double gets them back exactly in the TC = float classification. l0 is the
+// `scalar` 2 * thickness (a float widened to double in fp32).
 template <typename TC>
-PairResult contact_ee_at(const V3<double> xd[4], const V3<double> rd[4], float l0f, double kappa, int d2_policy) {
+PairResult contact_ee_at(const V3<double> xd[4], const V3<double> rd[4], double l0f, double kappa, int d2_policy) {
   PairResult R;
   std::memset(&R, 0, sizeof(R));
   V3<TC> xc[4];
@@ -443,8 +445,8 @@ PairResult contact_ee_at(const V3<double> xd[4], const V3<double> rd[4], float l
   return R;
 }
 
-template <typename TC>
-PairResult contact_ee(const float* x, const float* xcs, const int c[4], float l0f, double kappa, int d2_policy) {
+template <typename TC, typename XT>
+PairResult contact_ee(const XT* x, const XT* xcs, const int c[4], double l0f, double kappa, int d2_policy) {
   V3<double> xd[4], rd[4];
   for (int k = 0; k < 4; ++k) { xd[k] = load3<double>(x, c[k]); rd[k] = load3<double>(xcs, c[k]); }
   return contact_ee_at<TC>(xd, rd, l0f, kappa, d2_policy);
@@ -455,7 +457,7 @@ PairResult contact_ee(const float* x, const float* xcs, const int c[4], float l0
 // point_triangle_collision (contact.py:336); fetch_dist_v0v1_pt
 // (contact.py:116-145) treats slots 1..3 as the triangle (a, b, c).
 template <typename TC>
-PairResult contact_pt_at(const V3<double> xd[4], float l0f, double kappa, int d2_policy) {
+PairResult contact_pt_at(const V3<double> xd[4], double l0f, double kappa, int d2_policy) {
   PairResult R;
   std::memset(&R, 0, sizeof(R));
   V3<TC> xc[4];
@@ -477,8 +479,8 @@ PairResult contact_pt_at(const V3<double> xd[4], float l0f, double kappa, int d2
   return R;
 }
 
-template <typename TC>
-PairResult contact_pt(const float* x, const int c[4], float l0f, double kappa, int d2_policy) {
+template <typename TC, typename XT>
+PairResult contact_pt(const XT* x, const int c[4], double l0f, double kappa, int d2_policy) {
   V3<double> xd[4];
   for (int k = 0; k < 4; ++k) xd[k] = load3<double>(x, c[k]);
   return contact_pt_at<TC>(xd, l0f, kappa, d2_policy);
