@@ -457,7 +457,8 @@ int ipc_bsr_cg(ipc_cg* cg, int n_rows, const int* offsets, const int* columns, c
 int ipc_tet_rest(const float* xcs, int n_vertices, const int* tets, int n_tets, float* Bm, float* W, void* stream);
 /* tet_kernel_sparse (fem/fem.py:138-203): block (offset + e) * 16 + 4 i + j
  * = d(-f_i)/dx_j of tet e (row tets[e][i], column tets[e][j]), the elastic
- * force f added into b (float [V][3], may be NULL). */
+ * force f added into b (float [V][3], may be NULL). rows, cols and vals are
+ * written as 16-byte vectors and must be 16-byte aligned. */
 int ipc_tet_hessian(const float* x, int n_vertices, const int* tets, int n_tets, const float* Bm, const float* W,
                     float mu, float lam, int offset, int* rows, int* cols, float* vals, float* b, void* stream);
 /* compute_Psi (time_integrator_base.py:124-137): psi[e] = W[e] psi(F_e)

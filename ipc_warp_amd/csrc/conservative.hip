@@ -152,7 +152,9 @@ __global__ void __launch_bounds__(256) conservative_time_kernel(const float* __r
       const unsigned long long o = __shfl_xor(bits, off, 64);
       bits = o < bits ? o : bits;
     }
-    if ((threadIdx.x & 63) == 0 && bits < 0x3FF0000000000000ull) atomicMin(toi_min, bits);
+    if ((threadIdx.x & 63) == 0 && bits < 0x3FF0000000000000ull &&
+        bits < __hip_atomic_load(toi_min, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMin(toi_min, bits);
   }
 }
 

@@ -315,27 +315,28 @@ struct EnergyArgsT {
 };
 using EnergyArgs = EnergyArgsT<float>;
 
+// Energy of the contact this lane takes in (virtual) workgroup `block`.
 template <bool EE, typename P>
-__device__ __forceinline__ void energy_block(const EnergyArgsT<P>& A, int block, double* partial) {
-  const P* __restrict__ x = A.x;
-  const P* __restrict__ xcs = A.xcs;
-  const int nv = A.nv, n = A.n;
-  const P l0 = A.l0;
-  const double kappa = A.kappa;
-  double* __restrict__ energy = A.energy;
-  int i = contact_slot(A.lpw, block);
+__device__ __forceinline__ double energy_of(const EnergyArgsT<P>& A, int block) {
+  const int i = contact_slot(A.lpw, block);
   double e = 0.0;
-  if (i >= 0 && i < n) {
+  if (i >= 0 && i < A.n) {
     CompressedHessianT<P> C;
     P force[12];
     double ei = 0.0;
-    int4 c = A.contacts[i];
-    bool active = valid4(c, nv) && (EE ? pair_ee<P, P>(x, xcs, c, l0, kappa, false, C, force, &ei)
-                                       : pair_pt<P, P>(x, c, l0, kappa, false, C, force, &ei));
+    const int4 c = A.contacts[i];
+    const bool active = valid4(c, A.nv) && (EE ? pair_ee<P, P>(A.x, A.xcs, c, A.l0, A.kappa, false, C, force, &ei)
+                                               : pair_pt<P, P>(A.x, c, A.l0, A.kappa, false, C, force, &ei));
     if (active) e = ei;
   }
+  return e;
+}
+
+// The workgroup's sum, added into *energy with one atomic: atomics on one
+// address serialise, so a launch makes at most its grid's worth of them.
+__device__ __forceinline__ void energy_flush(double e, double* partial, double* energy) {
   e = wave_sum(e);
-  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane == 0) partial[wave] = e;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -345,19 +346,34 @@ __device__ __forceinline__ void energy_block(const EnergyArgsT<P>& A, int block,
   }
 }
 
+// A launch covers `blocks` virtual workgroups (blocks_spread's count) with
+// at most kEnergyGrid real ones, each walking every gridDim-th.
+constexpr int kEnergyGrid = 512;
+static inline int energy_grid(int blocks) { return blocks < kEnergyGrid ? blocks : kEnergyGrid; }
+
 template <bool EE, typename P = float>
-__global__ void __launch_bounds__(256) contact_energy_kernel(EnergyArgsT<P> A) {
+__global__ void __launch_bounds__(256) contact_energy_kernel(EnergyArgsT<P> A, int blocks) {
   __shared__ double partial[4];
-  energy_block<EE>(A, blockIdx.x, partial);
+  double e = 0.0;
+  for (int vb = blockIdx.x; vb < blocks; vb += gridDim.x) e += energy_of<EE>(A, vb);
+  energy_flush(e, partial, A.energy);
 }
 
-// Both halves of the barrier energy in one launch (workgroups [0, blocks_e)
-// the EE list, the rest the PT list), both added into A.energy.
+// Both halves of the barrier energy in one launch: workgroups [0, grid_e)
+// walk the blocks_e virtual workgroups of the EE list, the rest the
+// blocks_t of the PT list; both sums go into E.energy.
 template <typename P = float>
-__global__ void __launch_bounds__(256) contact_energy_pair_kernel(EnergyArgsT<P> E, EnergyArgsT<P> T, int blocks_e) {
+__global__ void __launch_bounds__(256) contact_energy_pair_kernel(EnergyArgsT<P> E, EnergyArgsT<P> T, int blocks_e,
+                                                                  int grid_e, int blocks_t) {
   __shared__ double partial[4];
-  if (int(blockIdx.x) < blocks_e) energy_block<true>(E, blockIdx.x, partial);
-  else energy_block<false>(T, blockIdx.x - blocks_e, partial);
+  double e = 0.0;
+  if (int(blockIdx.x) < grid_e) {
+    for (int vb = blockIdx.x; vb < blocks_e; vb += grid_e) e += energy_of<true>(E, vb);
+  } else {
+    const int grid_t = int(gridDim.x) - grid_e;
+    for (int vb = int(blockIdx.x) - grid_e; vb < blocks_t; vb += grid_t) e += energy_of<false>(T, vb);
+  }
+  energy_flush(e, partial, E.energy);
 }
 
 // ----------------------------------------------------- contact points kernel ----
@@ -410,6 +426,15 @@ __global__ void __launch_bounds__(256) contact_points_kernel(const float* __rest
 }
 
 // -------------------------------------------------------------- CCD kernel ----
+// Folds t (>= 0) into the running minimum *toi_min (IEEE bits, which order
+// like the values): the atomic only when t is below what is there already,
+// so the many waves that finish after the minimum has dropped make no
+// atomic on the one address (those serialise).
+__device__ __forceinline__ void toi_min_add(unsigned long long* toi_min, double t) {
+  const unsigned long long bits = (unsigned long long)__double_as_longlong(t);
+  if (bits < __hip_atomic_load(toi_min, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(toi_min, bits);
+}
+
 __device__ __forceinline__ double wave_min(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
@@ -510,7 +535,7 @@ __device__ __forceinline__ void collision_time_block(const float* __restrict__ x
   toi_drain<EE>(x0, x1, nv, quads, toi_out, tq, tmin);
   if (toi_min) {
     double m = wave_min(tmin);
-    if (lane == 0 && m < 1.0) atomicMin(toi_min, (unsigned long long)__double_as_longlong(m));
+    if (lane == 0 && m < 1.0) toi_min_add(toi_min, m);
   }
 }
 
@@ -592,7 +617,7 @@ __global__ void __launch_bounds__(256) record_intersections_kernel(
   }
   if (toi_min) {
     double mn = wave_min(tmin);
-    if (lane == 0 && mn < 1.0) atomicMin(toi_min, (unsigned long long)__double_as_longlong(mn));
+    if (lane == 0 && mn < 1.0) toi_min_add(toi_min, mn);
   }
 }
 
@@ -667,10 +692,11 @@ hipError_t launch_contact_energy_f64(bool ee, const double* x, const double* xcs
   if (n <= 0) return hipSuccess;
   const EnergyArgsT<double> A{x, xcs, nv, reinterpret_cast<const int4*>(contacts), n, l0, kappa, energy,
                               lanes_per_wave(n, kSpreadLanesEnergy)};
+  const int blocks = blocks_spread(n, A.lpw);
   if (ee)
-    hipLaunchKernelGGL((contact_energy_kernel<true, double>), dim3(blocks_spread(n, A.lpw)), dim3(256), 0, s, A);
+    hipLaunchKernelGGL((contact_energy_kernel<true, double>), dim3(energy_grid(blocks)), dim3(256), 0, s, A, blocks);
   else
-    hipLaunchKernelGGL((contact_energy_kernel<false, double>), dim3(blocks_spread(n, A.lpw)), dim3(256), 0, s, A);
+    hipLaunchKernelGGL((contact_energy_kernel<false, double>), dim3(energy_grid(blocks)), dim3(256), 0, s, A, blocks);
   return hipGetLastError();
 }
 
@@ -678,10 +704,11 @@ hipError_t launch_contact_energy(bool ee, const float* x, const float* xcs, int 
                                  double kappa, double* energy, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const EnergyArgs A = energy_args(x, xcs, nv, contacts, n, l0, kappa, energy);
+  const int blocks = blocks_spread(n, A.lpw);
   if (ee)
-    hipLaunchKernelGGL(contact_energy_kernel<true>, dim3(blocks_spread(n, A.lpw)), dim3(256), 0, s, A);
+    hipLaunchKernelGGL(contact_energy_kernel<true>, dim3(energy_grid(blocks)), dim3(256), 0, s, A, blocks);
   else
-    hipLaunchKernelGGL(contact_energy_kernel<false>, dim3(blocks_spread(n, A.lpw)), dim3(256), 0, s, A);
+    hipLaunchKernelGGL(contact_energy_kernel<false>, dim3(energy_grid(blocks)), dim3(256), 0, s, A, blocks);
   return hipGetLastError();
 }
 
@@ -700,7 +727,8 @@ hipError_t launch_collision_energy(const float* x, const float* xcs, int nv, con
   const EnergyArgs E = energy_args(x, xcs, nv, ee, n_ee, l0, kappa, energy);
   const EnergyArgs P = energy_args(x, nullptr, nv, pt, n_pt, l0, kappa, energy);
   const int be = blocks_spread(n_ee, E.lpw), bp = blocks_spread(n_pt, P.lpw);
-  hipLaunchKernelGGL(contact_energy_pair_kernel<>, dim3(be + bp), dim3(256), 0, s, E, P, be);
+  const int ge = energy_grid(be), gp = energy_grid(bp);
+  hipLaunchKernelGGL(contact_energy_pair_kernel<>, dim3(ge + gp), dim3(256), 0, s, E, P, be, ge, bp);
   return hipGetLastError();
 }
 

@@ -236,93 +236,175 @@ __global__ void tet_rest_kernel(const float* __restrict__ xcs, int nv, const int
   W[e] = fabsf(det) / 6.0f;
 }
 
-// One lane per tet: blocks (offset + e) * 16 + 4 i + j with row T[e][i],
-// column T[e][j]; the force added into b. A tet naming a vertex out of range
-// writes zero blocks with row = col = 0 and no force.
+// Per wave in LDS: its 32 tets' vertex ids and flags, their forces on the
+// way to b, and one block row (blocks (i, 0..3), 36 floats) of each tet's
+// values on the way to memory.
+struct TetStage {
+  float f[4][32 * 12];
+  int v[4][32 * 4];
+  int ok[4][32];
+  float4 row[4][32 * 9];
+};
+
+// Two lanes per tet, lanes p and p + 32 of a wave: the first takes the
+// directional derivatives of columns j = 0, 1, the second those of j = 2, 3
+// (six each), and the first the force. Blocks (offset + e) * 16 + 4 i + j
+// with row T[e][i], column T[e][j]; the force added into b. A tet naming a
+// vertex out of range writes zero blocks with row = col = 0 and no force.
+//
+// A tet's 16 blocks are 576 contiguous bytes and a wave's tets are
+// consecutive, so the wave's output is one contiguous range. The blocks go
+// out block row by block row through LDS (each lane puts its two blocks of
+// row i there), and the wave stores 32 tets' row i with consecutive lanes
+// on consecutive 16-byte words; a lane storing its own tet's 576 bytes
+// touched 64 cache lines per store instruction. Row and column indices go
+// out the same way from the staged vertex ids.
+constexpr int kTetsPerWave = 32;
+constexpr int kTetsPerBlock = 4 * kTetsPerWave;
+
 __global__ void __launch_bounds__(256) tet_hessian_kernel(const float* __restrict__ x, int nv,
                                                           const int4* __restrict__ T, int n,
                                                           const float* __restrict__ Bmv, const float* __restrict__ Wv,
                                                           float mu, float lam, long long base, int* __restrict__ rows,
                                                           int* __restrict__ cols, float* __restrict__ vals, float* b) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  const int4 t = T[e];
-  const long long t0 = (base + e) * 16;
-  float* v = vals + 9 * t0;
-  if (!tet_ok(t, nv)) {
-    for (int q = 0; q < 16; ++q) {
-      rows[t0 + q] = 0;
-      cols[t0 + q] = 0;
-#pragma unroll
-      for (int k = 0; k < 9; ++k) v[9 * q + k] = 0.0f;
-    }
-    return;
+  __shared__ TetStage st;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int p = lane & 31, half = lane >> 5;
+  const int first = blockIdx.x * kTetsPerBlock + wave * kTetsPerWave;
+  if (first >= n) return;  // the whole wave is past the end
+  const int nw = min(kTetsPerWave, n - first);
+  const int e = first + p;
+  int4 t = make_int4(0, 0, 0, 0);
+  bool ok = false;
+  if (e < n) {
+    t = T[e];
+    ok = tet_ok(t, nv);
   }
-  const int id[4] = {t.x, t.y, t.z, t.w};
-  const Snh s = reparam(mu, lam);
-  M3 Bm;
-  const M3 F = deformation(x, t, Bmv + 9ll * e, Bm);
-  const float W = Wv[e];
-  // rows / columns of the 16 blocks
+  // h[jj][k][l][i] = dH(j, k).m[l][i] with j = 2 half + jj, i < 3: entry
+  // (l, k) of block (i, j)
+  float h[2][3][3][3];
+  float force[12];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    reinterpret_cast<int4*>(rows + t0)[i] = make_int4(id[i], id[i], id[i], id[i]);
-    reinterpret_cast<int4*>(cols + t0)[i] = make_int4(id[0], id[1], id[2], id[3]);
-  }
-  // blocks of column j: dH for (j, k), k = 0..2
+  for (int q = 0; q < 12; ++q) force[q] = 0.0f;
+  if (ok) {
+    const Snh s = reparam(mu, lam);
+    M3 Bm;
+    const M3 F = deformation(x, t, Bmv + 9ll * e, Bm);
+    const float W = Wv[e];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    float blk[4][9];
+    for (int jj = 0; jj < 2; ++jj)
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      M3 dDs = mzero();
-      if (j < 3) {
-        dDs.m[k][j] = -1.0f;
-      } else {
-        dDs.m[k][0] = 1.0f;
-        dDs.m[k][1] = 1.0f;
-        dDs.m[k][2] = 1.0f;
+      for (int k = 0; k < 3; ++k) {
+        const int j = 2 * half + jj;
+        M3 dDs = mzero();
+        if (j < 3) {
+          dDs.m[k][0] = j == 0 ? -1.0f : 0.0f;
+          dDs.m[k][1] = j == 1 ? -1.0f : 0.0f;
+          dDs.m[k][2] = j == 2 ? -1.0f : 0.0f;
+        } else {
+          dDs.m[k][0] = 1.0f;
+          dDs.m[k][1] = 1.0f;
+          dDs.m[k][2] = 1.0f;
+        }
+        const M3 dF = matmul(dDs, Bm);
+        const M3 dP = tangent(F, dF, s);
+        const M3 dH = matmul_t(scale(-W, dP), Bm);
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+#pragma unroll
+          for (int i = 0; i < 3; ++i) h[jj][k][l][i] = dH.m[l][i];
       }
-      const M3 dF = matmul(dDs, Bm);
-      const M3 dP = tangent(F, dF, s);
-      const M3 dH = matmul_t(scale(-W, dP), Bm);
+    if (b && half == 0) {
+      // columns of H = -W P Bm^T, the last the negated sum
+      const M3 H = matmul_t(scale(-W, pk1(F, s)), Bm);
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int l = 0; l < 3; ++l) {
-          const float df = i == 3 ? -(dH.m[l][0] + dH.m[l][1] + dH.m[l][2]) : dH.m[l][i];
-          blk[i][3 * l + k] = 0.0f + df;
-        }
+        for (int l = 0; l < 3; ++l)
+          force[3 * i + l] = i == 3 ? ((-H.m[l][0]) - H.m[l][1]) - H.m[l][2] : H.m[l][i];
     }
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int k = 0; k < 9; ++k) v[9 * (4 * i + j) + k] = blk[i][k];
   }
-  // forces: columns of H = -W P Bm^T, the last the negated sum
-  if (b) {
-    const M3 H = matmul_t(scale(-W, pk1(F, s)), Bm);
+  float* sf = st.f[wave];
+  int* sv = st.v[wave];
+  int* so = st.ok[wave];
+  float* sr = reinterpret_cast<float*>(st.row[wave]);
+  if (half == 0) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int l = 0; l < 3; ++l) {
-        const float fi = i == 3 ? ((-H.m[l][0]) - H.m[l][1]) - H.m[l][2] : H.m[l][i];
-        atomicAdd(b + 3ll * id[i] + l, fi);
+    for (int q = 0; q < 12; ++q) sf[p * 12 + q] = force[q];
+    sv[p * 4 + 0] = t.x;
+    sv[p * 4 + 1] = t.y;
+    sv[p * 4 + 2] = t.z;
+    sv[p * 4 + 3] = t.w;
+    so[p] = ok ? 1 : 0;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  const long long t0 = (base + first) * 16;
+  {
+    int4* rows4 = reinterpret_cast<int4*>(rows + t0);
+    int4* cols4 = reinterpret_cast<int4*>(cols + t0);
+    for (int q = lane; q < nw * 4; q += 64) {
+      const int sl = q >> 2, i = q & 3;
+      int4 rv = make_int4(0, 0, 0, 0), cv = make_int4(0, 0, 0, 0);
+      if (so[sl]) {
+        const int vi = sv[sl * 4 + i];
+        rv = make_int4(vi, vi, vi, vi);
+        cv = make_int4(sv[sl * 4 + 0], sv[sl * 4 + 1], sv[sl * 4 + 2], sv[sl * 4 + 3]);
       }
+      rows4[q] = rv;
+      cols4[q] = cv;
+    }
+  }
+  float4* v4 = reinterpret_cast<float4*>(vals + 9 * t0);
+  const float4* sr4 = st.row[wave];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    float blk[18];  // blocks (i, 2 half), (i, 2 half + 1): entry (l, k) of block jj at 9 jj + 3 l + k
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+      for (int l = 0; l < 3; ++l)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const float df = i == 3 ? -(h[jj][k][l][0] + h[jj][k][l][1] + h[jj][k][l][2]) : h[jj][k][l][i];
+          blk[9 * jj + 3 * l + k] = ok ? 0.0f + df : 0.0f;
+        }
+    float2* dst = reinterpret_cast<float2*>(sr + p * 36 + 18 * half);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) dst[q] = make_float2(blk[2 * q], blk[2 * q + 1]);
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    for (int q = lane; q < nw * 9; q += 64) {
+      const int sl = q / 9, u = q - 9 * sl;
+      v4[sl * 36 + i * 9 + u] = sr4[q];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  }
+  // forces into b, component q of the wave's 12 nw by lane q % 64, so that
+  // one atomic instruction adds consecutive components of consecutive tets
+  if (b) {
+    for (int q = lane; q < 12 * nw; q += 64) {
+      const float v = sf[q];
+      const int sl = q / 12, comp = q - 12 * sl, vert = comp / 3;
+      if (so[sl] && v != 0.0f) atomicAdd(b + 3ll * sv[sl * 4 + vert] + (comp - 3 * vert), v);
+    }
   }
 }
+
+// Grid-stride over the tets: each lane sums its tets' energies in double,
+// then the block's sum goes into *energy with one atomic (the atomics on
+// one address serialise, so their number is kept to the grid's size).
+constexpr int kEnergyGrid = 1024;
 
 __global__ void __launch_bounds__(256) tet_energy_kernel(const float* __restrict__ x, int nv,
                                                          const int4* __restrict__ T, int n,
                                                          const float* __restrict__ Bmv, const float* __restrict__ Wv,
                                                          float mu, float lam, float* psi_out, double* energy) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double partial[4];
   double ed = 0.0;
-  if (e < n) {
+  const Snh s = reparam(mu, lam);
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
     const int4 t = T[e];
     float pe = 0.0f;
     if (tet_ok(t, nv)) {
-      const Snh s = reparam(mu, lam);
       M3 Bm;
       const M3 F = deformation(x, t, Bmv + 9ll * e, Bm);
       const float Ic = ddot(F, F);
@@ -331,11 +413,16 @@ __global__ void __launch_bounds__(256) tet_energy_kernel(const float* __restrict
       pe = Wv[e] * psi;
     }
     if (psi_out) psi_out[e] = pe;
-    ed = pe;
+    ed += pe;
   }
   if (energy) {
     for (int off = 32; off > 0; off >>= 1) ed += __shfl_xor(ed, off, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(energy, ed);
+    if ((threadIdx.x & 63) == 0) partial[threadIdx.x >> 6] = ed;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const double sum = ((partial[0] + partial[1]) + partial[2]) + partial[3];
+      if (sum != 0.0) atomicAdd(energy, sum);
+    }
   }
 }
 
@@ -380,8 +467,10 @@ int ipc_tet_hessian(const float* x, int n_vertices, const int* tets, int n_tets,
   if (n_vertices < 0 || n_tets < 0 || offset < 0) return fem_fail("tet_hessian: negative size");
   if (n_tets == 0) return IPC_OK;
   if (!x || !tets || !Bm || !W || !rows || !cols || !vals) return fem_fail("tet_hessian: null pointer");
-  if (!a16(tets) || !a16(rows) || !a16(cols)) return fem_fail("tet_hessian: tets/rows/cols must be 16-byte aligned");
-  hipLaunchKernelGGL(tet_hessian_kernel, dim3((n_tets + 255) / 256), dim3(256), 0, static_cast<hipStream_t>(stream), x,
+  if (!a16(tets) || !a16(rows) || !a16(cols) || !a16(vals))
+    return fem_fail("tet_hessian: tets/rows/cols/vals must be 16-byte aligned");
+  hipLaunchKernelGGL(tet_hessian_kernel, dim3((n_tets + kTetsPerBlock - 1) / kTetsPerBlock), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), x,
                      n_vertices, reinterpret_cast<const int4*>(tets), n_tets, Bm, W, mu, lam, (long long)offset, rows,
                      cols, vals, b);
   return fem_hip(hipGetLastError(), "tet_hessian");
@@ -393,7 +482,8 @@ int ipc_tet_energy(const float* x, int n_vertices, const int* tets, int n_tets, 
   if (n_tets == 0) return IPC_OK;
   if (!x || !tets || !Bm || !W) return fem_fail("tet_energy: null pointer");
   if (!a16(tets)) return fem_fail("tet_energy: tets must be 16-byte aligned");
-  hipLaunchKernelGGL(tet_energy_kernel, dim3((n_tets + 255) / 256), dim3(256), 0, static_cast<hipStream_t>(stream), x,
+  const int grid = (n_tets + 255) / 256 < kEnergyGrid ? (n_tets + 255) / 256 : kEnergyGrid;
+  hipLaunchKernelGGL(tet_energy_kernel, dim3(grid), dim3(256), 0, static_cast<hipStream_t>(stream), x,
                      n_vertices, reinterpret_cast<const int4*>(tets), n_tets, Bm, W, mu, lam, psi, energy);
   return fem_hip(hipGetLastError(), "tet_energy");
 }
