@@ -199,50 +199,89 @@ __global__ void __launch_bounds__(256) morton_kernel(int mode, int n, const int*
   atomicAdd(&count[cell_of(code)], 1);
 }
 
-// Leaf order by Morton cell: primitives are bucketed by the top 18 of the 30
-// code bits (64 cells per axis) with a counting sort, cells in Morton order.
-// The order inside a cell (a 1/64 slice of the scene per axis, a few
-// primitives) is the order of the scatter's atomics, so it may differ from
-// run to run; the order only affects speed, never the pairs found. The
-// counts are cleared in the bounds launch and counted in the Morton launch,
-// so the sort adds two launches (scan, scatter), against a block sort and
-// about seven merge launches for a full radix sort of the codes at these
-// sizes.
+// Leaf order by Morton code: primitives are bucketed by the top 18 of the 30
+// code bits (64 cells per axis) with a counting sort, cells in Morton order,
+// then each cell's primitives are put in (code, id) order by ranking each
+// within its cell. Where two surfaces lie close together (the screw and its
+// nut, 1e-3 apart) a cell holds a few dozen primitives of both bodies, and
+// the scatter's atomic order would give 8-leaf nodes cell-sized boxes and
+// mixed body ranges. A cell of more than kRankCap primitives keeps the
+// atomic order (the ranking is quadratic in the cell's size); the order only
+// affects speed, never the pairs found. The counts are cleared in the
+// bounds launch and counted in the Morton launch, so the sort adds three
+// launches (scan, scatter, rank), against a block sort and about seven merge
+// launches for a full radix sort of the codes at these sizes.
 __global__ void __launch_bounds__(256) cell_scatter_kernel(int n, const unsigned* __restrict__ codes,
                                                            int* __restrict__ cursor, int* __restrict__ ids_out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) ids_out[atomicAdd(&cursor[cell_of(codes[i])], 1)] = i;
 }
 
+constexpr int kRankCap = 128;
+
+// Slot s of the scattered order: its primitive goes to its cell's start plus
+// the number of the cell's primitives with a smaller (code, id). After the
+// scatter, cursor[c] is cell c's end and count[c] its size.
+__global__ void __launch_bounds__(256) cell_rank_kernel(int n, const unsigned* __restrict__ codes,
+                                                        const int* __restrict__ count, const int* __restrict__ cursor,
+                                                        const int* __restrict__ scattered, int* __restrict__ ids_out) {
+  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= n) return;
+  const int i = scattered[slot];
+  const unsigned code = codes[i];
+  const int c = cell_of(code);
+  const int hi = cursor[c], len = count[c], lo = hi - len;
+  if (len > kRankCap) {
+    ids_out[slot] = i;
+    return;
+  }
+  const unsigned long long key = (static_cast<unsigned long long>(code) << 32) | unsigned(i);
+  int rank = 0;
+  for (int k = lo; k < hi; ++k) {
+    const int j = scattered[k];
+    rank += ((static_cast<unsigned long long>(codes[j]) << 32) | unsigned(j)) < key;
+  }
+  ids_out[lo + rank] = i;
+}
+
 // temp of a re-sort: cell counts (cleared by init_bounds_kernel, counted by
-// morton_kernel), cell cursors, the scan's workspace.
+// morton_kernel), cell cursors, the scan's workspace, the scattered order (n
+// ids). Each part starts on a 256-byte boundary: the scan's workspace holds
+// its look-back state, which hipCUB expects aligned.
 struct CellBufs {
   int* count;
   int* cursor;
+  int* scattered;
   void* scan_temp;
   size_t scan_bytes;
 };
 
-static hipError_t cell_buffers(DevBuf& temp, CellBufs& c) {
+static hipError_t cell_buffers(DevBuf& temp, int n, CellBufs& c) {
   c.scan_bytes = 0;
   hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, c.scan_bytes, static_cast<const int*>(nullptr),
                                                   static_cast<int*>(nullptr), kCells, hipStream_t(nullptr));
-  const size_t cell_bytes = sizeof(int) * kCells;
-  if (e == hipSuccess) e = temp.ensure(2 * cell_bytes + c.scan_bytes);
+  auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t cell_bytes = sizeof(int) * kCells;  // a multiple of 256
+  const size_t scan_at = 2 * cell_bytes, order_at = scan_at + up(c.scan_bytes);
+  if (e == hipSuccess) e = temp.ensure(order_at + sizeof(int) * size_t(n > 0 ? n : 1));
   if (e != hipSuccess) return e;
   c.count = temp.as<int>();
   c.cursor = c.count + kCells;
-  c.scan_temp = static_cast<char*>(temp.p) + 2 * cell_bytes;
+  c.scan_temp = static_cast<char*>(temp.p) + scan_at;
+  c.scattered = reinterpret_cast<int*>(static_cast<char*>(temp.p) + order_at);
   return hipSuccess;
 }
 
-// ids_out receives the n primitive ids in cell order, from the counts.
+// ids_out receives the n primitive ids in Morton order (cells by counting
+// sort, each cell ranked by (code, id)).
 static hipError_t sort_codes(const CellBufs& c, const unsigned* codes, int n, int* ids_out, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   size_t scan_bytes = c.scan_bytes;
   hipError_t e = hipcub::DeviceScan::ExclusiveSum(c.scan_temp, scan_bytes, c.count, c.cursor, kCells, s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(cell_scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, codes, c.cursor, ids_out);
+  hipLaunchKernelGGL(cell_scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, codes, c.cursor, c.scattered);
+  hipLaunchKernelGGL(cell_rank_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, codes, c.count, c.cursor,
+                     c.scattered, ids_out);
   return hipGetLastError();
 }
 
@@ -1047,7 +1086,7 @@ hipError_t prepare_job(const BuildJob& j, const int* body, const float* x0, hipS
   B.vid_body = body;
   if (resort) {
     CellBufs cb;
-    IPC_TRY(cell_buffers(B.temp, cb));
+    IPC_TRY(cell_buffers(B.temp, n, cb));
     hipLaunchKernelGGL(init_bounds_kernel, dim3(kInitBlocks), dim3(256), 0, s, B.bounds.as<int>(), cb.count);
     hipLaunchKernelGGL(centre_bounds_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, j.mode, n, j.prims, x0, j.x1, j.dx,
                        j.pad, B.bounds.as<int>());
@@ -1148,7 +1187,7 @@ hipError_t point_order(PointOrder& P, int n, const float* x, hipStream_t s, cons
   IPC_TRY(P.keys.ensure(sizeof(unsigned) * n));
   IPC_TRY(P.order.ensure(sizeof(int) * n));
   CellBufs cb;
-  IPC_TRY(cell_buffers(P.temp, cb));
+  IPC_TRY(cell_buffers(P.temp, n, cb));
   hipLaunchKernelGGL(init_bounds_kernel, dim3(kInitBlocks), dim3(256), 0, s, P.bounds.as<int>(), cb.count);
   hipLaunchKernelGGL(centre_bounds_kernel, dim3(nblocks(n, 256)), dim3(256), 0, s, int(BOX_POINT), n,
                      static_cast<const int*>(nullptr), x, static_cast<const float*>(nullptr),
