@@ -204,12 +204,12 @@ struct FeatureGeom {
 };
 
 // Slot selection without a runtime-indexed local array (the compiler would
-// place it in scratch memory): a 0/1-weighted blend of the four slots, exact
-// because every product is either the value or zero.
+// place it in scratch memory): two levels of selects (v_cndmask on each
+// 32-bit half), exact, and far cheaper than a 0/1-weighted blend in FP64.
 IPCD_FN V3<double> pick(const V3<double> x[4], int s) {
-  double c0 = s == 0 ? 1.0 : 0.0, c1 = s == 1 ? 1.0 : 0.0, c2 = s == 2 ? 1.0 : 0.0, c3 = s == 3 ? 1.0 : 0.0;
-  return mk(x[0].x * c0 + x[1].x * c1 + x[2].x * c2 + x[3].x * c3, x[0].y * c0 + x[1].y * c1 + x[2].y * c2 + x[3].y * c3,
-            x[0].z * c0 + x[1].z * c1 + x[2].z * c2 + x[3].z * c3);
+  const V3<double> lo = s == 1 ? x[1] : x[0];
+  const V3<double> hi = s == 3 ? x[3] : x[2];
+  return s >= 2 ? hi : lo;
 }
 
 // Positions are given relative to slot 0 (x[0] = 0) in double.
@@ -593,18 +593,25 @@ IPCD_FN double nsd_compressed(const FeatureGeom& g, bool want_nsd, CompressedHes
 template <typename R>
 IPCD_FN void expand_block(const CompressedHessianT<R>& C, int i, int j, R out[9]) {
 #pragma clang fp contract(fast)
-  R ww = C.what[i] * C.what[j];
-  R rr = C.cg * C.w[i] * C.w[j] + C.cn * C.s[sidx(i, j)];
+  // H_ij[a][b] = c_n w^_i w^_j Ntt[a][b] + r[b] (c_n w^_i u_j[a])
+  //            + r[a] (c_n w^_j u_i[b] + rr r[b]): per block three 3-vectors
+  //            and the six distinct scaled Ntt entries, then two FMAs an entry.
+  const R rr = C.cg * C.w[i] * C.w[j] + C.cn * C.s[sidx(i, j)];
   const R* r = C.rh;
-  const R Nt[9] = {C.Ntt[0], C.Ntt[1], C.Ntt[2], C.Ntt[1], C.Ntt[3], C.Ntt[4], C.Ntt[2], C.Ntt[4], C.Ntt[5]};
+  const R pi = C.cn * C.what[i], pj = C.cn * C.what[j];
+  const R cw = pi * C.what[j];
+  R A[3], B[3], N[6];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) A[a] = pi * C.u[j][a];
+#pragma unroll
+  for (int b = 0; b < 3; ++b) B[b] = pj * C.u[i][b] + rr * r[b];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) N[k] = cw * C.Ntt[k];
+  const R Nt[9] = {N[0], N[1], N[2], N[1], N[3], N[4], N[2], N[4], N[5]};
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
-    for (int b = 0; b < 3; ++b) {
-      // w^_i u_j r^T has entry w^_i u_j[a] r[b]; w^_j r u_i^T has w^_j r[a] u_i[b].
-      out[a * 3 + b] = C.cn * (ww * Nt[a * 3 + b] + C.what[i] * C.u[j][a] * r[b] + C.what[j] * r[a] * C.u[i][b]) +
-                       rr * r[a] * r[b];
-    }
+    for (int b = 0; b < 3; ++b) out[a * 3 + b] = Nt[a * 3 + b] + r[b] * A[a] + r[a] * B[b];
 }
 
 }  // namespace ipcd
