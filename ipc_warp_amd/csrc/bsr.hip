@@ -187,8 +187,11 @@ __global__ __launch_bounds__(kB) void bsr_sort_rows_kernel(const int* start, int
       const unsigned long long k = lane < L ? bucket[b + lane] : ~0ull;
       const int khi = int(k >> 32), klo = int(unsigned(k));
       int rank = 0, below = 0;
+      // q is wave-uniform: each key is read from its lane into a scalar
+      // register (v_readlane), not through the LDS crossbar
+#pragma unroll 4
       for (int q = 0; q < L; ++q) {
-        const int hi = __shfl(khi, q), lo = __shfl(klo, q);
+        const int hi = __builtin_amdgcn_readlane(khi, q), lo = __builtin_amdgcn_readlane(klo, q);
         const unsigned long long kq = (static_cast<unsigned long long>(unsigned(hi)) << 32) | unsigned(lo);
         rank += kq < k;
         below += unsigned(hi) < unsigned(khi);
