@@ -537,6 +537,18 @@ int ipc_al_create_hash(ipc_active_set* ws, const void* constraints, int record_b
 int ipc_al_merge(ipc_active_set* ws, int entry_bytes, const int* old_cell_start, const void* old_entries,
                  int old_table_size, int n_old, const void* new_entries, int n_new, int table_size, int* cell_start,
                  void* entries, int capacity, void* stream);
+/* ActiveSet.update_slack / update_forces (augmented_lagrangian.py:564-572,
+ * kernels update_slack_variables / update_active_set_forces :326-345) on
+ * the n entries of a table, in place. With c = d0 + grad . x_hat (x_hat
+ * `scalar` [n_vertices][3]; constraint_value :76-85) and mu the AL
+ * penalty (cast to `scalar`): update_slack sets slack = max(0, c - lam/mu);
+ * update_forces sets lam -= mu c and gamma = 1 where slack == 0, else
+ * lam = 0 and gamma *= 0.9. An entry naming a vertex out of range is left
+ * unchanged. */
+int ipc_al_update_slack(void* entries, int entry_bytes, int n, const void* x_hat, int n_vertices, double mu,
+                        void* stream);
+int ipc_al_update_forces(void* entries, int entry_bytes, int n, const void* x_hat, int n_vertices, double mu,
+                         void* stream);
 
 #ifdef __cplusplus
 }

@@ -210,6 +210,23 @@ class ActiveSet:
                                  int(destination.cell_entries.shape[0]), stream_handle(self.device)), "al_merge")
         self.hash = destination
 
+    def update_forces(self, x_hat: torch.Tensor, mu: float) -> None:
+        """augmented_lagrangian.py:564-567 (update_active_set_forces)."""
+        _update("forces", self.hash, self.count(), self.scalar, x_hat, mu)
+
+    def update_slack(self, x_hat: torch.Tensor, mu: float) -> None:
+        """augmented_lagrangian.py:569-572 (update_slack_variables)."""
+        _update("slack", self.hash, self.count(), self.scalar, x_hat, mu)
+
+
+def _update(kind: str, hash_: Hash, n: int, scalar: torch.dtype, x_hat: torch.Tensor, mu: float) -> None:
+    if n == 0:
+        return
+    fn = lib().ipc_al_update_forces if kind == "forces" else lib().ipc_al_update_slack
+    check(fn(hash_.cell_entries.data_ptr(), 4 * ALCONSTRAINT_WORDS[scalar], int(n),
+             _need(x_hat, scalar, "x_hat", (3,)), int(x_hat.shape[0]), float(mu), stream_handle(x_hat.device)),
+          f"al_update_{kind}")
+
 
 class ALActiveSets:
     """The four tables and the per-vertex TOI of RodComplexAL

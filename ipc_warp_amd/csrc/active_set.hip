@@ -253,6 +253,66 @@ __global__ __launch_bounds__(kB) void gather_kernel(const unsigned* in_a, int n_
   }
 }
 
+// constraint_value (augmented_lagrangian.py:76-85): c = d0 + sum over the
+// four vertices of dot(grad_k, x_k), each dot x*x + y*y + z*z in that order.
+// False when a vertex is out of range (the entry is then left unchanged).
+template <typename S>
+__device__ inline bool constraint_value(const unsigned* e, const S* x, int nv, S& c) {
+  using L = Layout<S>;
+  c = load_scalar<S>(e, L::kD0Word);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int v = int(e[k]);
+    if (unsigned(v) >= unsigned(nv)) return false;
+    const int g = L::kD0Word + L::kSw * (1 + 3 * k);
+    const S d = load_scalar<S>(e, g) * x[3ll * v] + load_scalar<S>(e, g + L::kSw) * x[3ll * v + 1] +
+                load_scalar<S>(e, g + 2 * L::kSw) * x[3ll * v + 2];
+    c = c + d;
+  }
+  return true;
+}
+
+template <typename S>
+__device__ inline void store_scalar(unsigned* rec, int word, S v) {
+  *reinterpret_cast<S*>(rec + word) = v;
+}
+
+// update_slack_variables (augmented_lagrangian.py:326-331): slack =
+// max(0, c(x_hat) - lam / mu).
+template <typename S>
+__global__ __launch_bounds__(kB) void update_slack_kernel(unsigned* entries, int n, const S* x, int nv, S mu) {
+  using L = Layout<S>;
+  const int i = blockIdx.x * kB + threadIdx.x;
+  if (i >= n) return;
+  unsigned* e = entries + static_cast<long long>(i) * L::kAWords;
+  S c;
+  if (!constraint_value<S>(e, x, nv, c)) return;
+  const S lam = load_scalar<S>(e, 6 + 2 * L::kSw);
+  const S v = c - lam / mu;
+  store_scalar<S>(e, 6 + 4 * L::kSw, v > S(0) ? v : S(0));  // NaN and -0 give +0
+}
+
+// update_active_set_forces (augmented_lagrangian.py:334-345): a constraint
+// with zero slack updates its multiplier, lam -= mu c(x_hat), and resets
+// gamma to 1; otherwise lam = 0 and gamma decays by 0.9.
+template <typename S>
+__global__ __launch_bounds__(kB) void update_forces_kernel(unsigned* entries, int n, const S* x, int nv, S mu) {
+  using L = Layout<S>;
+  const int i = blockIdx.x * kB + threadIdx.x;
+  if (i >= n) return;
+  unsigned* e = entries + static_cast<long long>(i) * L::kAWords;
+  S c;
+  if (!constraint_value<S>(e, x, nv, c)) return;
+  const S lam = load_scalar<S>(e, 6 + 2 * L::kSw), gamma = load_scalar<S>(e, L::kGammaWord);
+  if (load_scalar<S>(e, 6 + 4 * L::kSw) == S(0)) {
+    store_scalar<S>(e, 6 + 2 * L::kSw, lam - mu * c);
+    store_scalar<S>(e, L::kGammaWord, S(1));
+  } else {
+    store_scalar<S>(e, 6 + 2 * L::kSw, S(0));
+    store_scalar<S>(e, L::kGammaWord, gamma * S(0.9));
+  }
+}
+
 int scalar_of(int bytes, int f32, int f64) { return bytes == f32 ? 4 : (bytes == f64 ? 8 : 0); }
 
 inline bool aligned(const void* p, unsigned long long a) { return (reinterpret_cast<unsigned long long>(p) & (a - 1)) == 0; }
@@ -395,6 +455,48 @@ int ipc_al_merge(ipc_active_set* ws, int entry_bytes, const int* old_cell_start,
   return sb == 4 ? finish<float, false>(ws, m, oe, n_old, ne, table_size, cell_start, entries, capacity, s, "al_merge")
                  : finish<double, false>(ws, m, oe, n_old, ne, table_size, cell_start, entries, capacity, s,
                                          "al_merge");
+}
+
+static int update_entries(bool forces, void* entries, int entry_bytes, int n, const void* x_hat, int n_vertices,
+                          double mu, void* stream) {
+  const char* who = forces ? "al_update_forces" : "al_update_slack";
+  const int sb = scalar_of(entry_bytes, IPC_ALCONSTRAINT_BYTES_F32, IPC_ALCONSTRAINT_BYTES_F64);
+  if (!sb) return as_fail(IPC_EINVAL, forces ? "al_update_forces: entry_bytes must be 104 or 184"
+                                             : "al_update_slack: entry_bytes must be 104 or 184");
+  if (n < 0 || n_vertices < 0) return as_fail(IPC_EINVAL, forces ? "al_update_forces: negative size"
+                                                                 : "al_update_slack: negative size");
+  if (n == 0) return IPC_OK;
+  if (!entries || !x_hat) return as_fail(IPC_EINVAL, forces ? "al_update_forces: null pointer"
+                                                            : "al_update_slack: null pointer");
+  if (!aligned(entries, sb) || !aligned(x_hat, sb))
+    return as_fail(IPC_EINVAL, forces ? "al_update_forces: misaligned pointer" : "al_update_slack: misaligned pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  unsigned* e = static_cast<unsigned*>(entries);
+  const dim3 g(nblk(n, kB));
+  if (sb == 4) {
+    const float* x = static_cast<const float*>(x_hat);
+    if (forces)
+      hipLaunchKernelGGL(update_forces_kernel<float>, g, dim3(kB), 0, s, e, n, x, n_vertices, float(mu));
+    else
+      hipLaunchKernelGGL(update_slack_kernel<float>, g, dim3(kB), 0, s, e, n, x, n_vertices, float(mu));
+  } else {
+    const double* x = static_cast<const double*>(x_hat);
+    if (forces)
+      hipLaunchKernelGGL(update_forces_kernel<double>, g, dim3(kB), 0, s, e, n, x, n_vertices, mu);
+    else
+      hipLaunchKernelGGL(update_slack_kernel<double>, g, dim3(kB), 0, s, e, n, x, n_vertices, mu);
+  }
+  return as_hip(hipGetLastError(), who);
+}
+
+int ipc_al_update_slack(void* entries, int entry_bytes, int n, const void* x_hat, int n_vertices, double mu,
+                        void* stream) {
+  return update_entries(false, entries, entry_bytes, n, x_hat, n_vertices, mu, stream);
+}
+
+int ipc_al_update_forces(void* entries, int entry_bytes, int n, const void* x_hat, int n_vertices, double mu,
+                         void* stream) {
+  return update_entries(true, entries, entry_bytes, n, x_hat, n_vertices, mu, stream);
 }
 
 }  // extern "C"

@@ -96,6 +96,7 @@ def lib():
         L.oracle_contact_energy_f64.argtypes = [c_int, F64P, F64P, I32P, c_int, c_dbl, c_dbl]
         L.oracle_contact_energy_f64.restype = c_dbl
         L.oracle_al_hash_pos.argtypes = [c_int, c_int, c_int]
+        L.oracle_al_update.argtypes = [c_int, c_int, I32P, c_int, ctypes.c_void_p, c_int, c_dbl]
         L.oracle_al_hash_pos.restype = c_int
         L.oracle_al_vertex_toi.argtypes = [c_int, I32P, c_int, ctypes.c_void_p, c_int]
         L.oracle_al_create_hash.argtypes = [c_int, I32P, c_int, ctypes.c_void_p, c_int, c_int, I32P, I32P, c_int]
@@ -567,3 +568,15 @@ def al_merge(old_cell_start, old_entries, old_table_size, n_old, new_entries, n_
                                 int(n_old), _p(ne.reshape(-1), I32P), int(n_new), int(table_size), _p(cs, I32P),
                                 _p(ent, I32P), int(capacity))
     return cnt, cs, ent
+
+
+def al_update(entries, x, mu, forces):
+    """update_slack_variables (forces=False) or update_active_set_forces
+    (forces=True), augmented_lagrangian.py:326-345, on a copy of the
+    ALConstraint records (int32 [n, 26 | 46]); x `scalar` [V, 3]."""
+    e = np.array(entries, dtype=np.int32, copy=True)
+    use_double = e.shape[1] == 46
+    xs = np.ascontiguousarray(x, dtype=np.float64 if use_double else np.float32)
+    lib().oracle_al_update(int(bool(forces)), int(use_double), _p(e.reshape(-1), I32P), e.shape[0], xs.ctypes.data,
+                           xs.shape[0], float(mu))
+    return e

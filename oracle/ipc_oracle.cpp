@@ -1173,9 +1173,52 @@ int al_merge(const int* ocs, const uint32_t* oe, int oT, int n_old, const uint32
   return cs[T];
 }
 
+// constraint_value (augmented_lagrangian.py:76-85); false when a vertex is
+// out of range.
+template <typename S> bool al_value(const uint32_t* e, const S* x, int nv, S& c) {
+  using L = AlLayout<S>;
+  const int d0 = L::sw == 1 ? 13 : 20;
+  c = get_s<S>(e, d0);
+  for (int k = 0; k < 4; ++k) {
+    const int v = int(e[k]);
+    if (v < 0 || v >= nv) return false;
+    const int g = d0 + L::sw * (1 + 3 * k);
+    const S d = get_s<S>(e, g) * x[3ll * v] + get_s<S>(e, g + L::sw) * x[3ll * v + 1] +
+                get_s<S>(e, g + 2 * L::sw) * x[3ll * v + 2];
+    c = c + d;
+  }
+  return true;
+}
+
+// update_slack_variables (:326-331) and update_active_set_forces (:334-345).
+template <typename S> void al_update(bool forces, uint32_t* entries, int n, const S* x, int nv, S mu) {
+  using L = AlLayout<S>;
+  for (int i = 0; i < n; ++i) {
+    uint32_t* e = entries + size_t(i) * L::awords;
+    S c;
+    if (!al_value<S>(e, x, nv, c)) continue;
+    const S lam = get_s<S>(e, L::lam);
+    if (!forces) {
+      const S v = c - lam / mu;
+      put_s<S>(e, L::slack, v > S(0) ? v : S(0));
+    } else if (get_s<S>(e, L::slack) == S(0)) {
+      put_s<S>(e, L::lam, lam - mu * c);
+      put_s<S>(e, L::gamma, S(1));
+    } else {
+      put_s<S>(e, L::lam, S(0));
+      put_s<S>(e, L::gamma, get_s<S>(e, L::gamma) * S(0.9));
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" {
+
+void oracle_al_update(int forces, int use_double, uint32_t* entries, int n, const void* x, int nv, double mu) {
+  if (use_double) al_update<double>(forces != 0, entries, n, static_cast<const double*>(x), nv, mu);
+  else al_update<float>(forces != 0, entries, n, static_cast<const float*>(x), nv, float(mu));
+}
 
 int oracle_al_hash_pos(int e0, int e1, int table_size) { return al_hash(e0, e1, table_size); }
 

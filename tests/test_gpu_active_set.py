@@ -140,6 +140,36 @@ def test_update_active_set_on_ccd_pairs(dev, oracle):
         x = x1
 
 
+@pytest.mark.parametrize("scalar", [torch.float32, torch.float64])
+def test_slack_and_force_updates_match_oracle(dev, oracle, scalar):
+    """ActiveSet.update_slack / update_forces (augmented_lagrangian.py:
+    326-345, 564-572) on a created set given random linearisations,
+    multipliers and slacks (a third of them zero), bit for bit."""
+    rng = np.random.default_rng(17)
+    nv = 400
+    words = random_records(rng, 3000, nv, scalar)
+    A, n, _, _ = check_against_oracle(oracle, dev, words, nv, scalar, 8192, 4096)
+    view = ALConstraintView(A.hash.cell_entries[:n])
+    view.set_scalar("d0", torch.from_numpy(rng.normal(size=n)).to(dev))
+    view.set_scalar("lam", torch.from_numpy(rng.normal(size=n)).to(dev))
+    view.set_scalar("gamma", torch.from_numpy(rng.uniform(0, 1, n)).to(dev))
+    view.set_scalar("slack", torch.from_numpy(np.where(rng.random(n) < 0.33, 0.0, rng.uniform(0, 1, n))).to(dev))
+    g = torch.from_numpy(rng.normal(size=(n, 12))).to(scalar).contiguous().view(torch.int32)
+    sw = 1 if scalar == torch.float32 else 2
+    d0w = 13 if sw == 1 else 20
+    A.hash.cell_entries[:n, d0w + sw:d0w + 13 * sw] = g.reshape(n, 12 * sw)
+    x = torch.from_numpy(rng.normal(size=(nv, 3))).to(dev).to(scalar)
+    before = A.hash.cell_entries[:n].cpu().numpy()
+    for forces in (False, True):
+        if forces:
+            A.update_forces(x, 3.0)
+        else:
+            A.update_slack(x, 3.0)
+        torch.cuda.synchronize()
+        before = oracle.al_update(before, x.cpu().numpy(), 3.0, forces)
+        assert np.array_equal(A.hash.cell_entries[:n].cpu().numpy(), before)
+
+
 def test_empty_inputs(dev, oracle):
     A = ActiveSet(8, dev)
     A.hash.cell_entries.fill_(5)

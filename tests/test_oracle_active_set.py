@@ -170,3 +170,34 @@ def test_merge_of_empty_tables():
     count, cs, ent = oracle.al_merge(np.zeros(3, np.int32), np.zeros((1, 26), np.int32), 2, 0,
                                      np.zeros((1, 26), np.int32), 0, 2, 1)
     assert count == 0 and not cs.any() and not ent.any()
+
+
+def _entry(scalar, ids, d0, grad, lam, gamma, slack):
+    e = np.zeros(1, AL[scalar])
+    e["a"] = ids
+    e["d0"], e["grad"], e["lam"], e["gamma"], e["slack"] = d0, grad, lam, gamma, slack
+    return e.view(np.int32).reshape(1, -1)
+
+
+@pytest.mark.parametrize("scalar", [np.float32, np.float64])
+def test_slack_and_forces_by_hand(scalar):
+    """c = d0 + grad . x, by hand on dyadic values (exact in both precisions):
+    d0 = 0.5, the vertices' gradients (1, 0, 0), (0, 2, 0), (0, 0, 3),
+    (1, 1, 1) at x = (1, 0, 0), (0, 1, 0), (0, 0, 1), (0.5, 0.25, 0.25):
+    c = 0.5 + 1 + 2 + 3 + 1 = 7.5 (augmented_lagrangian.py:76-85)."""
+    x = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [0.5, 0.25, 0.25]], dtype=scalar)
+    grad = [1, 0, 0, 0, 2, 0, 0, 0, 3, 1, 1, 1]
+    e = _entry(scalar, [0, 1, 2, 3], 0.5, grad, lam=2.0, gamma=0.5, slack=9.0)
+    s = oracle.al_update(e, x, mu=4.0, forces=False).view(AL[scalar])[0]
+    assert s["slack"] == 7.0  # max(0, 7.5 - 2 / 4)
+    f = oracle.al_update(e, x, mu=4.0, forces=True).view(AL[scalar])[0]
+    assert f["lam"] == 0.0 and f["gamma"] == scalar(0.5) * scalar(0.9)  # slack 9 != 0: released, gamma decays
+    e0 = _entry(scalar, [0, 1, 2, 3], 0.5, grad, lam=2.0, gamma=0.5, slack=0.0)
+    f0 = oracle.al_update(e0, x, mu=4.0, forces=True).view(AL[scalar])[0]
+    assert f0["lam"] == 2.0 - 4.0 * 7.5 and f0["gamma"] == 1.0  # slack 0: lam -= mu c, gamma reset
+    # a constraint the multiplier outweighs has zero slack
+    e1 = _entry(scalar, [0, 1, 2, 3], 0.5, grad, lam=64.0, gamma=1.0, slack=1.0)
+    assert oracle.al_update(e1, x, mu=4.0, forces=False).view(AL[scalar])[0]["slack"] == 0.0
+    # an entry naming a vertex out of range stays as it was
+    e2 = _entry(scalar, [0, 1, 2, 9], 0.5, grad, lam=2.0, gamma=0.5, slack=3.0)
+    assert np.array_equal(oracle.al_update(e2, x, mu=4.0, forces=True), e2)
