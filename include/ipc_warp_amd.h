@@ -545,6 +545,17 @@ int ipc_al_merge(ipc_active_set* ws, int entry_bytes, const int* old_cell_start,
  * update_forces sets lam -= mu c and gamma = 1 where slack == 0, else
  * lam = 0 and gamma *= 0.9. An entry naming a vertex out of range is left
  * unchanged. */
+/* ActiveSet.linearize (augmented_lagrangian.py:549-562, kernel
+ * linearize_constraints :270-323) on the n entries, in place: at x_safe
+ * (`scalar` [n_vertices][3]) each constraint's feature — PT: the region of
+ * closest_point_triangle; EE: where the mollifier (rest edges x_rest, eps_x
+ * = 1e-3 |ea|^2 |eb|^2) is below point_edge_threshold, the parallel
+ * point-edge choice of :208-267 (distance_type 1..4), else
+ * edge_edge_distance_gradient_hessian's dispatch — gives grad = grad(d^2) /
+ * (2 d) and d0 = d - l0 - grad . x_safe; mollifier and distance_type are
+ * stored too. Classification in fp32, the feature gradient in FP64. */
+int ipc_al_linearize(void* entries, int entry_bytes, int n, const void* x_safe, const void* x_rest, int n_vertices,
+                     int is_point_triangle, double point_edge_threshold, void* stream);
 int ipc_al_update_slack(void* entries, int entry_bytes, int n, const void* x_hat, int n_vertices, double mu,
                         void* stream);
 int ipc_al_update_forces(void* entries, int entry_bytes, int n, const void* x_hat, int n_vertices, double mu,

@@ -83,6 +83,7 @@ _SIGNATURES = {
     "ipc_al_vertex_toi": ([_vp, _c_int, _c_int, _vp, _c_int, _vp], _c_int),
     "ipc_al_create_hash": ([_vp, _vp, _c_int, _c_int, _vp, _c_int, _c_int, _vp, _vp, _c_int, _vp], _c_int),
     "ipc_al_merge": ([_vp, _c_int, _vp, _vp, _c_int, _c_int, _vp, _c_int, _c_int, _vp, _vp, _c_int, _vp], _c_int),
+    "ipc_al_linearize": ([_vp, _c_int, _c_int, _vp, _vp, _c_int, _c_int, _c_double, _vp], _c_int),
     "ipc_al_update_slack": ([_vp, _c_int, _c_int, _vp, _c_int, _c_double, _vp], _c_int),
     "ipc_al_update_forces": ([_vp, _c_int, _c_int, _vp, _c_int, _c_double, _vp], _c_int),
     "ipc_bsr_create": ([ctypes.POINTER(_vp)], _c_int),

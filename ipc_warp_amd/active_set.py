@@ -24,6 +24,7 @@ ALCONSTRAINT_WORDS = {torch.float32: 26, torch.float64: 46}
 DECAY_FACTOR = 0.9
 ACTIVE_SET_DECAY_THRESHOLD = 0.01
 TOI_FILTER_EPS = 1.0e-12
+EE_MOLLIFIER_POINT_EDGE_THRESHOLD = 1.0
 
 
 def _scalar_of(words: int, table: dict) -> torch.dtype:
@@ -209,6 +210,18 @@ class ActiveSet:
                                  destination.cell_start.data_ptr(), destination.cell_entries.data_ptr(),
                                  int(destination.cell_entries.shape[0]), stream_handle(self.device)), "al_merge")
         self.hash = destination
+
+    def linearize(self, x_safe: torch.Tensor, x_rest: torch.Tensor, is_point_triangle: bool,
+                  point_edge_threshold: float = EE_MOLLIFIER_POINT_EDGE_THRESHOLD) -> None:
+        """augmented_lagrangian.py:549-562 (linearize_constraints, :270-323)."""
+        n = self.count()
+        if n == 0:
+            return
+        check(lib().ipc_al_linearize(self.hash.cell_entries.data_ptr(), 4 * ALCONSTRAINT_WORDS[self.scalar], n,
+                                     _need(x_safe, self.scalar, "x_safe", (3,)),
+                                     _need(x_rest, self.scalar, "x_rest", (3,)), int(x_safe.shape[0]),
+                                     int(bool(is_point_triangle)), float(point_edge_threshold),
+                                     stream_handle(x_safe.device)), "al_linearize")
 
     def update_forces(self, x_hat: torch.Tensor, mu: float) -> None:
         """augmented_lagrangian.py:564-567 (update_active_set_forces)."""

@@ -23,6 +23,7 @@
 
 #include "../../include/ipc_warp_amd.h"
 #include "cull.hpp"
+#include "ipc_device.hpp"
 #include "ipc_kernels.hpp"
 #include "scan.hpp"
 
@@ -313,6 +314,140 @@ __global__ __launch_bounds__(kB) void update_forces_kernel(unsigned* entries, in
   }
 }
 
+// ---------------------------------------------------------- linearise ---
+// point_line_distance_squared (augmented_lagrangian.py:199-204), in S.
+template <typename S>
+__device__ inline S point_line(V3<S> p, V3<S> e0, V3<S> e1, S& alpha) {
+  const V3<S> edge = e1 - e0;
+  const S ee = dot(edge, edge);
+  alpha = dot(p - e0, edge) / (ee > S(1.0e-30) ? ee : S(1.0e-30));
+  const V3<S> off = p - (e0 + alpha * edge);
+  return dot(off, off);
+}
+
+// The dispatch of edge_edge_distance_gradient_hessian
+// (dynamic_contacts.py:48-108) on fp32 closest-point parameters s, t, the
+// comparisons in S; `parallel` is its near-parallel test.
+template <typename S>
+__device__ inline Feature ee_dispatch(S s, S t, bool parallel) {
+  const S fe = lit<S>(1.0e-6f, 1.0e-6), one = S(1);
+  const bool sb = s <= fe || s >= one - fe, tb = t <= fe || t >= one - fe;
+  if (sb && tb) return {F_PP, s >= one - fe ? 1 : 0, t >= one - fe ? 3 : 2, -1, -1};
+  if (sb) return {F_PE, s >= one - fe ? 1 : 0, 2, 3, -1};
+  if (tb) return {F_PE, t >= one - fe ? 3 : 2, 0, 1, -1};
+  if (parallel) {
+    const S ds0 = s, ds1 = one - s, dt0 = t, dt1 = one - t;
+    const S m = fmin(fmin(ds0, ds1), fmin(dt0, dt1));
+    if (m == ds0) return {F_PE, 0, 2, 3, -1};
+    if (m == ds1) return {F_PE, 1, 2, 3, -1};
+    if (m == dt0) return {F_PE, 2, 0, 1, -1};
+    return {F_PE, 3, 0, 1, -1};
+  }
+  return {F_EE, 0, 1, 2, 3};
+}
+
+// linearize_constraints (augmented_lagrangian.py:270-323): at x_safe, the
+// constraint's feature (PT: closest_point_triangle's region; EE: the
+// parallel point-edge choice of :208-267 where the mollifier is below the
+// threshold, else edge_edge_distance_gradient_hessian's dispatch) gives the
+// gradient of the squared distance; grad = that / (2 d) and d0 = d - l0 -
+// grad . x_safe, so that c(x) = d0 + grad . x is the distance's first-order
+// model minus l0. Classification in fp32 (the reference's vec3 casts),
+// mollifier and point-line distances in S, the feature gradient in FP64.
+template <typename S>
+__global__ __launch_bounds__(kB) void linearize_kernel(unsigned* entries, int n, const S* __restrict__ xs,
+                                                       const S* __restrict__ xrest, int nv, int is_pt, S pe_threshold) {
+  using L = Layout<S>;
+  const int i = blockIdx.x * kB + threadIdx.x;
+  if (i >= n) return;
+  unsigned* e = entries + static_cast<long long>(i) * L::kAWords;
+  int id[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    id[k] = int(e[k]);
+    if (unsigned(id[k]) >= unsigned(nv)) return;
+  }
+  V3<S> x[4];
+  V3<float> xf[4];
+  V3<double> xr[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    x[k] = ldx(xs, id[k]);
+    xf[k] = mk(float(x[k].x), float(x[k].y), float(x[k].z));
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) xr[k] = to_d(x[k]) - to_d(x[0]);
+  Feature f;
+  S d2;
+  S mollifier = S(1);
+  int type = 0;
+  if (is_pt) {
+    const TriClosestT<float> tc = closest_point_triangle(xf[0], xf[1], xf[2], xf[3]);
+    f = pt_feature(tc.type);
+    d2 = S(tc.dist) * S(tc.dist);
+  } else {
+    const V3<S> ra = ldx(xrest, id[1]) - ldx(xrest, id[0]), rb = ldx(xrest, id[3]) - ldx(xrest, id[2]);
+    const S eps_x = lit<S>(1.0e-3f, 1.0e-3) * dot(ra, ra) * dot(rb, rb);
+    const V3<S> cr = cross(x[1] - x[0], x[3] - x[2]);
+    const S c = dot(cr, cr);
+    if (c < eps_x) {
+      const S q = c / eps_x;
+      mollifier = q * (S(2) - q);
+    }
+    if (mollifier < pe_threshold) {
+      S a[4], dist[4];
+      dist[0] = point_line(x[0], x[2], x[3], a[0]);
+      dist[1] = point_line(x[1], x[2], x[3], a[1]);
+      dist[2] = point_line(x[2], x[0], x[1], a[2]);
+      dist[3] = point_line(x[3], x[0], x[1], a[3]);
+      const S large = S(1.0e30);
+      S score[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) score[k] = (a[k] < S(0) || a[k] > S(1)) ? large : dist[k];
+      if (fmin(fmin(score[0], score[1]), fmin(score[2], score[3])) == large) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) score[k] = dist[k];
+      }
+      int best = 0;
+      S b = score[0];
+#pragma unroll
+      for (int k = 1; k < 4; ++k)
+        if (score[k] < b) {
+          b = score[k];
+          best = k;
+        }
+      type = best + 1;
+      d2 = best == 0 ? dist[0] : (best == 1 ? dist[1] : (best == 2 ? dist[2] : dist[3]));
+      f = best == 0 ? Feature{F_PE, 0, 2, 3, -1}
+                    : (best == 1 ? Feature{F_PE, 1, 2, 3, -1}
+                                 : (best == 2 ? Feature{F_PE, 2, 0, 1, -1} : Feature{F_PE, 3, 0, 1, -1}));
+    } else {
+      const EdgeClosestT<float> ec = closest_point_edge_edge(xf[0], xf[1], xf[2], xf[3], 1.0e-6f);
+      const bool parallel = eps_x > S(0) && c < eps_x;
+      f = ee_dispatch<S>(S(ec.s), S(ec.t), parallel);
+      d2 = S(ec.dist) * S(ec.dist);
+    }
+  }
+  const FeatureGeom g = feature_geom(f, xr);
+  const S dist = sqrt(d2 > S(1.0e-30) ? d2 : S(1.0e-30));
+  const double inv = 1.0 / (2.0 * double(dist));
+  S grad[12];
+  S off = dist - load_scalar<S>(e, 6);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double s2 = 2.0 * g.w[k];
+    grad[3 * k] = S(s2 * g.r.x * inv);
+    grad[3 * k + 1] = S(s2 * g.r.y * inv);
+    grad[3 * k + 2] = S(s2 * g.r.z * inv);
+    off = off - (grad[3 * k] * x[k].x + grad[3 * k + 1] * x[k].y + grad[3 * k + 2] * x[k].z);
+  }
+  store_scalar<S>(e, L::kD0Word, off);
+#pragma unroll
+  for (int j = 0; j < 12; ++j) store_scalar<S>(e, L::kD0Word + L::kSw * (1 + j), grad[j]);
+  store_scalar<S>(e, L::kMollifierWord, mollifier);
+  e[L::kTypeWord] = unsigned(type);
+}
+
 int scalar_of(int bytes, int f32, int f64) { return bytes == f32 ? 4 : (bytes == f64 ? 8 : 0); }
 
 inline bool aligned(const void* p, unsigned long long a) { return (reinterpret_cast<unsigned long long>(p) & (a - 1)) == 0; }
@@ -487,6 +622,29 @@ static int update_entries(bool forces, void* entries, int entry_bytes, int n, co
       hipLaunchKernelGGL(update_slack_kernel<double>, g, dim3(kB), 0, s, e, n, x, n_vertices, mu);
   }
   return as_hip(hipGetLastError(), who);
+}
+
+int ipc_al_linearize(void* entries, int entry_bytes, int n, const void* x_safe, const void* x_rest, int n_vertices,
+                     int is_point_triangle, double point_edge_threshold, void* stream) {
+  const int sb = scalar_of(entry_bytes, IPC_ALCONSTRAINT_BYTES_F32, IPC_ALCONSTRAINT_BYTES_F64);
+  if (!sb) return as_fail(IPC_EINVAL, "al_linearize: entry_bytes must be 104 or 184");
+  if (n < 0 || n_vertices < 0) return as_fail(IPC_EINVAL, "al_linearize: negative size");
+  if (n == 0) return IPC_OK;
+  if (!entries || !x_safe || (!is_point_triangle && !x_rest)) return as_fail(IPC_EINVAL, "al_linearize: null pointer");
+  if (!aligned(entries, sb) || !aligned(x_safe, sb) || !aligned(x_rest, sb))
+    return as_fail(IPC_EINVAL, "al_linearize: misaligned pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  unsigned* e = static_cast<unsigned*>(entries);
+  const dim3 g(nblk(n, kB));
+  if (sb == 4)
+    hipLaunchKernelGGL(linearize_kernel<float>, g, dim3(kB), 0, s, e, n, static_cast<const float*>(x_safe),
+                       static_cast<const float*>(x_rest), n_vertices, is_point_triangle ? 1 : 0,
+                       float(point_edge_threshold));
+  else
+    hipLaunchKernelGGL(linearize_kernel<double>, g, dim3(kB), 0, s, e, n, static_cast<const double*>(x_safe),
+                       static_cast<const double*>(x_rest), n_vertices, is_point_triangle ? 1 : 0,
+                       point_edge_threshold);
+  return as_hip(hipGetLastError(), "al_linearize");
 }
 
 int ipc_al_update_slack(void* entries, int entry_bytes, int n, const void* x_hat, int n_vertices, double mu,

@@ -97,6 +97,7 @@ def lib():
         L.oracle_contact_energy_f64.restype = c_dbl
         L.oracle_al_hash_pos.argtypes = [c_int, c_int, c_int]
         L.oracle_al_update.argtypes = [c_int, c_int, I32P, c_int, ctypes.c_void_p, c_int, c_dbl]
+        L.oracle_al_linearize.argtypes = [c_int, I32P, c_int, ctypes.c_void_p, ctypes.c_void_p, c_int, c_int, c_dbl]
         L.oracle_al_hash_pos.restype = c_int
         L.oracle_al_vertex_toi.argtypes = [c_int, I32P, c_int, ctypes.c_void_p, c_int]
         L.oracle_al_create_hash.argtypes = [c_int, I32P, c_int, ctypes.c_void_p, c_int, c_int, I32P, I32P, c_int]
@@ -579,4 +580,17 @@ def al_update(entries, x, mu, forces):
     xs = np.ascontiguousarray(x, dtype=np.float64 if use_double else np.float32)
     lib().oracle_al_update(int(bool(forces)), int(use_double), _p(e.reshape(-1), I32P), e.shape[0], xs.ctypes.data,
                            xs.shape[0], float(mu))
+    return e
+
+
+def al_linearize(entries, x_safe, x_rest, is_point_triangle, threshold=1.0):
+    """linearize_constraints (augmented_lagrangian.py:270-323) on a copy of
+    the ALConstraint records; x_safe / x_rest `scalar` [V, 3]."""
+    e = np.array(entries, dtype=np.int32, copy=True)
+    use_double = e.shape[1] == 46
+    dt = np.float64 if use_double else np.float32
+    xs = np.ascontiguousarray(x_safe, dtype=dt)
+    xr = np.ascontiguousarray(x_rest, dtype=dt)
+    lib().oracle_al_linearize(int(use_double), _p(e.reshape(-1), I32P), e.shape[0], xs.ctypes.data, xr.ctypes.data,
+                              xs.shape[0], int(bool(is_point_triangle)), float(threshold))
     return e

@@ -1211,9 +1211,129 @@ template <typename S> void al_update(bool forces, uint32_t* entries, int n, cons
   }
 }
 
+// linearize_constraints (augmented_lagrangian.py:270-323) with the helpers
+// it calls: closest_point_triangle / wp.closest_point_edge_edge on fp32
+// casts of x_safe, point_line_distance_squared and
+// parallel_edge_point_edge_distance (:199-267) in S, the mollifier
+// m(c) = (c/eps)(2 - c/eps) for c = |ea x eb|^2 < eps = 1e-3 |ea0|^2 |eb0|^2
+// (ipctkwp's ee_mollifier_value / _threshold, restated), the dispatch of
+// point_triangle_distance_gradient_hessian / edge_edge_distance_gradient_
+// hessian (dynamic_contacts.py:48-125), and the gradient of the feature's
+// squared distance in double (feature_grad_nsd).
+template <typename S> S al_point_line(V3<S> p, V3<S> e0, V3<S> e1, S& alpha) {
+  const V3<S> edge = e1 - e0;
+  const S ee = dot(edge, edge);
+  alpha = dot(p - e0, edge) / (ee > S(1.0e-30) ? ee : S(1.0e-30));
+  const V3<S> off = p - (e0 + edge * alpha);
+  return dot(off, off);
+}
+
+template <typename S>
+void al_linearize(uint32_t* entries, int n, const S* xs, const S* xrest, int nv, bool is_pt, S threshold) {
+  using L = AlLayout<S>;
+  const int d0w = L::sw == 1 ? 13 : 20;
+  for (int i = 0; i < n; ++i) {
+    uint32_t* e = entries + size_t(i) * L::awords;
+    int id[4];
+    bool ok = true;
+    for (int k = 0; k < 4; ++k) {
+      id[k] = int(e[k]);
+      ok = ok && id[k] >= 0 && id[k] < nv;
+    }
+    if (!ok) continue;
+    V3<S> x[4];
+    V3<float> xf[4];
+    V3<double> xd[4];
+    for (int k = 0; k < 4; ++k) {
+      x[k] = V3<S>{xs[3 * id[k]], xs[3 * id[k] + 1], xs[3 * id[k] + 2]};
+      xf[k] = V3<float>{float(x[k].x), float(x[k].y), float(x[k].z)};
+      xd[k] = V3<double>{double(x[k].x), double(x[k].y), double(x[k].z)};
+    }
+    Feature f;
+    S d2;
+    S mollifier = S(1);
+    int type = 0;
+    if (is_pt) {
+      const TriClosest<float> tc = closest_point_triangle(xf[0], xf[1], xf[2], xf[3]);
+      f = pt_feature_from_type(tc.type);
+      d2 = S(tc.dist) * S(tc.dist);
+    } else {
+      auto rest = [&](int k) { return V3<S>{xrest[3 * id[k]], xrest[3 * id[k] + 1], xrest[3 * id[k] + 2]}; };
+      const V3<S> ra = rest(1) - rest(0), rb = rest(3) - rest(2);
+      const S eps_x = S(1.0e-3) * dot(ra, ra) * dot(rb, rb);
+      const V3<S> cr = cross(x[1] - x[0], x[3] - x[2]);
+      const S c = dot(cr, cr);
+      if (c < eps_x) mollifier = (c / eps_x) * (S(2) - c / eps_x);
+      if (mollifier < threshold) {
+        // parallel_edge_point_edge_distance (:208-267)
+        S a[4], dist[4];
+        dist[0] = al_point_line(x[0], x[2], x[3], a[0]);
+        dist[1] = al_point_line(x[1], x[2], x[3], a[1]);
+        dist[2] = al_point_line(x[2], x[0], x[1], a[2]);
+        dist[3] = al_point_line(x[3], x[0], x[1], a[3]);
+        S score[4];
+        const S large = S(1.0e30);
+        for (int k = 0; k < 4; ++k) score[k] = (a[k] < S(0) || a[k] > S(1)) ? large : dist[k];
+        if (std::min(std::min(score[0], score[1]), std::min(score[2], score[3])) == large)
+          for (int k = 0; k < 4; ++k) score[k] = dist[k];
+        int best = 0;
+        for (int k = 1; k < 4; ++k)
+          if (score[k] < score[best]) best = k;
+        type = best + 1;
+        d2 = dist[best];
+        static const int slots[4][3] = {{0, 2, 3}, {1, 2, 3}, {2, 0, 1}, {3, 0, 1}};
+        f = Feature{F_PE, {slots[best][0], slots[best][1], slots[best][2], -1}};
+      } else {
+        const EdgeClosest<float> ec = closest_point_edge_edge(xf[0], xf[1], xf[2], xf[3], 1.0e-6f);
+        const S sp = S(ec.s), tp = S(ec.t), fe = S(1.0e-6), one = S(1);
+        const bool sb = sp <= fe || sp >= one - fe, tb = tp <= fe || tp >= one - fe;
+        if (sb && tb) {
+          f = Feature{F_PP, {sp >= one - fe ? 1 : 0, tp >= one - fe ? 3 : 2, -1, -1}};
+        } else if (sb) {
+          f = Feature{F_PE, {sp >= one - fe ? 1 : 0, 2, 3, -1}};
+        } else if (tb) {
+          f = Feature{F_PE, {tp >= one - fe ? 3 : 2, 0, 1, -1}};
+        } else if (eps_x > S(0) && c < eps_x) {
+          const S m = std::min(std::min(sp, one - sp), std::min(tp, one - tp));
+          if (m == sp) f = Feature{F_PE, {0, 2, 3, -1}};
+          else if (m == one - sp) f = Feature{F_PE, {1, 2, 3, -1}};
+          else if (m == tp) f = Feature{F_PE, {2, 0, 1, -1}};
+          else f = Feature{F_PE, {3, 0, 1, -1}};
+        } else {
+          f = Feature{F_EE, {0, 1, 2, 3}};
+        }
+        d2 = S(ec.dist) * S(ec.dist);
+      }
+    }
+    double g[12], H[144];
+    feature_grad_nsd(f, xd, g, H);
+    const S dist = std::sqrt(d2 > S(1.0e-30) ? d2 : S(1.0e-30));
+    S grad[12];
+    S off = dist - get_s<S>(e, 6);
+    for (int k = 0; k < 4; ++k) {
+      for (int a3 = 0; a3 < 3; ++a3) grad[3 * k + a3] = S(g[3 * k + a3] / (2.0 * double(dist)));
+      off = off - (grad[3 * k] * x[k].x + grad[3 * k + 1] * x[k].y + grad[3 * k + 2] * x[k].z);
+    }
+    put_s<S>(e, d0w, off);
+    for (int j = 0; j < 12; ++j) put_s<S>(e, d0w + L::sw * (1 + j), grad[j]);
+    put_s<S>(e, L::mollifier, mollifier);
+    e[L::type] = uint32_t(type);
+  }
+}
+
 }  // namespace
 
 extern "C" {
+
+void oracle_al_linearize(int use_double, uint32_t* entries, int n, const void* xs, const void* xrest, int nv,
+                         int is_pt, double threshold) {
+  if (use_double)
+    al_linearize<double>(entries, n, static_cast<const double*>(xs), static_cast<const double*>(xrest), nv, is_pt != 0,
+                         threshold);
+  else
+    al_linearize<float>(entries, n, static_cast<const float*>(xs), static_cast<const float*>(xrest), nv, is_pt != 0,
+                        float(threshold));
+}
 
 void oracle_al_update(int forces, int use_double, uint32_t* entries, int n, const void* x, int nv, double mu) {
   if (use_double) al_update<double>(forces != 0, entries, n, static_cast<const double*>(x), nv, mu);

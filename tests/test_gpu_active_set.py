@@ -170,6 +170,47 @@ def test_slack_and_force_updates_match_oracle(dev, oracle, scalar):
         assert np.array_equal(A.hash.cell_entries[:n].cpu().numpy(), before)
 
 
+@pytest.mark.parametrize("scalar", [torch.float32, torch.float64])
+@pytest.mark.parametrize("kind", ["pt", "ee"])
+def test_linearize_matches_oracle(dev, oracle, scalar, kind):
+    """ActiveSet.linearize (augmented_lagrangian.py:270-323, 549-562) on a
+    synthetic pair stream (EE with near-parallel pairs, whose mollifier
+    selects the point-edge form): mollifier and distance_type bit for bit,
+    gradients within 2e-6 of each pair's largest (fp32 records; 1e-9 in
+    fp64, where the kernel's closest-point form of grad(d^2) and the oracle's
+    jets round differently), d0 within 2e-6 (1e-9) absolute."""
+    s = SCALARS[scalar]
+    x, c = scenes.pair_stream(kind, 4096, seed=23, dhat=1e-3, dtype=np.float64)
+    quads = c.copy()
+    if kind == "pt":
+        quads[:, [1, 2]] = c[:, [2, 1]]
+    dt = np.dtype([("a", "<i4", 4), ("e", "<i4", 2), ("l0", s), ("alpha", s), ("lam", s), ("gamma", s),
+                   ("slack", s), ("mollifier", s), ("distance_type", "<i4"), ("d0", s), ("grad", s, 12)], align=True)
+    rec = np.zeros(len(quads), dt)
+    rec["a"] = quads
+    rec["e"][:, 0] = np.arange(len(quads))
+    rec["l0"] = 1e-3
+    rec["gamma"] = 1.0
+    words = rec.view(np.int32).reshape(len(quads), -1)
+    A = ActiveSet(len(quads), dev, scalar)
+    A.hash.cell_entries.copy_(torch.from_numpy(words))
+    A.hash.cell_start[-1] = len(quads)
+    xt = torch.from_numpy(x).to(dev).to(scalar)
+    A.linearize(xt, xt, kind == "pt")
+    torch.cuda.synchronize()
+    got = A.hash.cell_entries.cpu().numpy().view(dt).reshape(-1)
+    ref = oracle.al_linearize(words, x.astype(s), x.astype(s), kind == "pt").view(dt).reshape(-1)
+    assert np.array_equal(got["distance_type"], ref["distance_type"])
+    assert np.array_equal(got["mollifier"], ref["mollifier"])
+    if kind == "ee":
+        assert (ref["distance_type"] > 0).sum() > 50
+    tol = 2e-6 if scalar == torch.float32 else 1e-9
+    gg, rg = got["grad"].astype(np.float64), ref["grad"].astype(np.float64)
+    scale = np.abs(rg).max(axis=1)
+    assert (np.abs(gg - rg).max(axis=1) <= tol * scale).all()
+    assert np.abs(got["d0"].astype(np.float64) - ref["d0"]).max() <= tol
+
+
 def test_empty_inputs(dev, oracle):
     A = ActiveSet(8, dev)
     A.hash.cell_entries.fill_(5)

@@ -201,3 +201,89 @@ def test_slack_and_forces_by_hand(scalar):
     # an entry naming a vertex out of range stays as it was
     e2 = _entry(scalar, [0, 1, 2, 9], 0.5, grad, lam=2.0, gamma=0.5, slack=3.0)
     assert np.array_equal(oracle.al_update(e2, x, mu=4.0, forces=True), e2)
+
+
+def _al_entries(scalar, quads, l0):
+    e = np.zeros(len(quads), AL[scalar])
+    e["a"] = quads
+    e["l0"] = l0
+    e["gamma"] = 1.0
+    e["mollifier"] = 1.0
+    return e.view(np.int32).reshape(len(quads), -1)
+
+
+PE_OF_TYPE = {1: (0, 2, 3), 2: (1, 2, 3), 3: (2, 0, 1), 4: (3, 0, 1)}
+
+
+def _line_distance(p, a, b):
+    e = b - a
+    t = np.dot(p - a, e) / max(np.dot(e, e), 1e-30)
+    return np.linalg.norm(p - (a + t * e))
+
+
+@pytest.mark.parametrize("scalar", [np.float32, np.float64])
+@pytest.mark.parametrize("kind", ["pt", "ee"])
+def test_linearize_is_the_distance_first_order_model(oracle, scalar, kind):
+    """linearize_constraints (augmented_lagrangian.py:270-323): the
+    constraint c(x) = d0 + grad . x is the first-order model of the
+    linearised distance minus l0 -- the Ericson distance (separately pinned)
+    for a regular feature, the point-line distance of the chosen endpoint
+    for a mollified edge pair (distance_type 1..4) -- so c(x_safe) equals
+    that distance - l0 (up to the fp32 casts the reference classifies and
+    measures with), grad is its gradient (central differences, fp64 records)
+    and the four vertices' gradients sum to zero."""
+    from ipc_warp_amd import scenes
+    x, c = scenes.pair_stream(kind, 64, seed=3, dhat=1e-3, dtype=np.float64)
+    quads = c.copy()
+    if kind == "pt":  # the AL records (p, t0, t1, t2); pair_stream writes (p, t1, t0, t2)
+        quads[:, [1, 2]] = c[:, [2, 1]]
+    e = _al_entries(scalar, quads, 1e-3)
+    out = oracle.al_linearize(e, x, x, kind == "pt").view(AL[scalar]).reshape(-1)
+    xs = x.astype(scalar).astype(np.float64)
+    assert kind == "pt" or (out["distance_type"] > 0).any()
+
+    def dist(xx, q, t):
+        if t > 0:
+            i, j, k = PE_OF_TYPE[t]
+            return _line_distance(xx[q[i]], xx[q[j]], xx[q[k]])
+        if kind == "pt":
+            return np.sqrt(oracle.triangle_distance_sq(xx[q[0]], xx[q[1]], xx[q[2]], xx[q[3]]))
+        return np.sqrt(oracle.segment_distance_sq(xx[q[0]], xx[q[1]], xx[q[2]], xx[q[3]]))
+
+    for i, q in enumerate(quads):
+        t = int(out["distance_type"][i])
+        g = out["grad"][i].astype(np.float64).reshape(4, 3)
+        d = dist(xs, q, t)
+        assert np.abs(g.sum(axis=0)).max() <= (1e-4 if scalar == np.float32 else 1e-9) * max(1.0, np.abs(g).max())
+        model = float(out["d0"][i]) + float((g * xs[q]).sum())
+        assert abs(model - (d - 1e-3)) <= (2e-6 if scalar == np.float32 else 2e-7), (i, t, model, d)
+        if scalar == np.float32 or d < 5e-5:
+            continue
+        for k in range(4):
+            for a in range(3):
+                h = 1e-3 * d  # well inside the distance's curvature radius
+                xp, xm = xs.copy(), xs.copy()
+                xp[q[k], a] += h
+                xm[q[k], a] -= h
+                fd = (dist(xp, q, t) - dist(xm, q, t)) / (xp[q[k], a] - xm[q[k], a])
+                # grad = grad(d^2) / (2 d_stored), d_stored the fp32 distance
+                # the reference divides by (model + l0 in a fp64 record)
+                scaled = g[k, a] * (model + 1e-3) / d
+                assert abs(fd - scaled) <= 1e-6 * max(1.0, abs(fd)), (i, k, a, fd, scaled)
+
+
+def test_linearize_parallel_edges_choose_an_overlapping_endpoint(oracle):
+    """Two parallel edges 1e-4 apart (|ea x eb|^2 = 0 < eps_x, so the
+    mollifier is 0): the point-edge representation takes an endpoint whose
+    projection lies on the other edge, the first by the rule's order
+    (augmented_lagrangian.py:208-267) -- here x2, projecting into edge a --
+    and the distance is the 1e-4 gap."""
+    x = np.array([[0.0, 0, 0], [1.0, 0, 0], [0.5, 1e-4, 0], [1.5, 1e-4, 0]])
+    e = _al_entries(np.float64, np.array([[0, 1, 2, 3]]), 5e-5)
+    out = oracle.al_linearize(e, x, x, False).view(AL[np.float64]).reshape(-1)[0]
+    # x0 projects before x2..x3 (alpha < 0); x1 projects inside it; x2 inside x0..x1
+    assert out["mollifier"] == 0.0
+    assert out["distance_type"] == 2  # x1 onto edge b: the first in-range candidate of equal distance
+    g = out["grad"].reshape(4, 3)
+    assert np.allclose(g[1], [0, -1, 0]) and np.allclose(g[2] + g[3], [0, 1, 0])
+    assert abs(out["d0"] + (g * x).sum() - (1e-4 - 5e-5)) < 1e-15
