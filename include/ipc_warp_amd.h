@@ -109,6 +109,16 @@ int ipc_contact_energy_ee_f64(const double* x, const double* xcs, int n_vertices
                               double l0, double kappa, double* energy, void* stream);
 int ipc_contact_energy_pt_f64(const double* x, int n_vertices, const int* contacts, int n_contacts, double l0,
                               double kappa, double* energy, void* stream);
+/* pt_collision_time / ee_collision_time with fp64 scalars: x0, x1 double
+ * [V][3] (vec3d), the same FP64 arithmetic as ipc_collision_time_ee/pt.
+ * The candidate culling keeps fp32 positions in both modes: the reference
+ * casts to wp.vec3 for its boxes and distance tests (contact.py:207-229,
+ * 313-336), so fp64 callers pass fp32 casts of x to the culling and DCD
+ * entry points. */
+int ipc_collision_time_ee_f64(const double* x0, const double* x1, int n_vertices, const int* quads, int n,
+                              double* toi, unsigned long long* toi_min, void* stream);
+int ipc_collision_time_pt_f64(const double* x0, const double* x1, int n_vertices, const int* quads, int n,
+                              double* toi, unsigned long long* toi_min, void* stream);
 
 /* ----------------------------------------------- contact distance ---
  * Replace get_contact_points / get_contact_points_pt (contact.py:148-161)

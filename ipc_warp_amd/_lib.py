@@ -30,6 +30,8 @@ _SIGNATURES = {
                                    _c_int),
     "ipc_contact_energy_ee_f64": ([_vp, _vp, _c_int, _vp, _c_int, _c_double, _c_double, _vp, _vp], _c_int),
     "ipc_contact_energy_pt_f64": ([_vp, _c_int, _vp, _c_int, _c_double, _c_double, _vp, _vp], _c_int),
+    "ipc_collision_time_ee_f64": ([_vp, _vp, _c_int, _vp, _c_int, _vp, _vp, _vp], _c_int),
+    "ipc_collision_time_pt_f64": ([_vp, _vp, _c_int, _vp, _c_int, _vp, _vp, _vp], _c_int),
     "ipc_contact_points_ee": ([_vp, _vp, _c_int, _vp, _c_int, _vp, _vp, _vp, _vp], _c_int),
     "ipc_contact_points_pt": ([_vp, _c_int, _vp, _c_int, _vp, _vp, _vp, _vp], _c_int),
     "ipc_collision_time_ee": ([_vp, _vp, _c_int, _vp, _c_int, _vp, _vp, _vp], _c_int),

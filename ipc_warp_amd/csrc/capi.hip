@@ -202,6 +202,27 @@ int ipc_collision_time_pt(const float* x0, const float* x1, int n_vertices, cons
   return collision_time(false, x0, x1, n_vertices, quads, n, toi, toi_min, stream);
 }
 
+static int collision_time_f64(bool ee, const double* x0, const double* x1, int nv, const int* quads, int n,
+                              double* toi, unsigned long long* toi_min, void* stream) {
+  if (n < 0 || nv < 0) return fail(IPC_EINVAL, "collision_time_f64: negative n_vertices or n");
+  if (n == 0) return IPC_OK;
+  if (!x0 || !x1 || !quads) return fail(IPC_EINVAL, "collision_time_f64: null pointer");
+  if (!aligned16(quads)) return fail(IPC_EINVAL, "collision_time_f64: quads must be 16-byte aligned");
+  return hip_status(
+      ipcd::launch_collision_time_f64(ee, x0, x1, nv, quads, n, toi, toi_min, static_cast<hipStream_t>(stream)),
+      ee ? "collision_time_ee_f64" : "collision_time_pt_f64");
+}
+
+int ipc_collision_time_ee_f64(const double* x0, const double* x1, int n_vertices, const int* quads, int n,
+                              double* toi, unsigned long long* toi_min, void* stream) {
+  return collision_time_f64(true, x0, x1, n_vertices, quads, n, toi, toi_min, stream);
+}
+
+int ipc_collision_time_pt_f64(const double* x0, const double* x1, int n_vertices, const int* quads, int n,
+                              double* toi, unsigned long long* toi_min, void* stream) {
+  return collision_time_f64(false, x0, x1, n_vertices, quads, n, toi, toi_min, stream);
+}
+
 int ipc_mlp_forward(const float* params, int n, const float* x, float* y, float* work, float* scratch, void* stream) {
   if (n < 0) return fail(IPC_EINVAL, "mlp_forward: negative n");
   if (n == 0) return IPC_OK;

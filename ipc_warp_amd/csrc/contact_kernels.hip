@@ -468,8 +468,8 @@ struct ToiQueue {
   int n;
 };
 
-template <bool EE>
-__device__ __forceinline__ double toi_of(const float* __restrict__ x0, const float* __restrict__ x1, int nv, int4 q,
+template <bool EE, typename P = float>
+__device__ __forceinline__ double toi_of(const P* __restrict__ x0, const P* __restrict__ x1, int nv, int4 q,
                                          bool full) {
   if (!valid4(q, nv)) return 1.0;
   const int id[4] = {q.x, q.y, q.z, q.w};
@@ -482,15 +482,15 @@ __device__ __forceinline__ double toi_of(const float* __restrict__ x0, const flo
 
 // Root isolation of the queued candidates by the whole workgroup; empties
 // the queue. Called by every thread of the workgroup.
-template <bool EE>
-__device__ __forceinline__ void toi_drain(const float* __restrict__ x0, const float* __restrict__ x1, int nv,
+template <bool EE, typename P = float>
+__device__ __forceinline__ void toi_drain(const P* __restrict__ x0, const P* __restrict__ x1, int nv,
                                           const int4* __restrict__ quads, double* __restrict__ toi_out,
                                           ToiQueue& tq, double& tmin) {
   __syncthreads();
   const int qn = tq.n;
   for (int k = threadIdx.x; k < qn; k += kToiBlock) {
     const int j = tq.idx[k];
-    const double t = toi_of<EE>(x0, x1, nv, quads[j], true);
+    const double t = toi_of<EE, P>(x0, x1, nv, quads[j], true);
     if (toi_out) toi_out[j] = t;
     tmin = fmin(tmin, t);
   }
@@ -501,8 +501,8 @@ __device__ __forceinline__ void toi_drain(const float* __restrict__ x0, const fl
 
 // Workgroup `bid` of `nblk` walking one candidate list (every thread of the
 // workgroup takes the same number of trips).
-template <bool EE>
-__device__ __forceinline__ void collision_time_block(const float* __restrict__ x0, const float* __restrict__ x1, int nv,
+template <bool EE, typename P = float>
+__device__ __forceinline__ void collision_time_block(const P* __restrict__ x0, const P* __restrict__ x1, int nv,
                                                      const int4* __restrict__ quads, int n,
                                                      const int* __restrict__ n_dev, double* __restrict__ toi_out,
                                                      unsigned long long* __restrict__ toi_min, int bid, int nblk,
@@ -521,7 +521,7 @@ __device__ __forceinline__ void collision_time_block(const float* __restrict__ x
     const int i = i0 + threadIdx.x;
     bool need = false;
     if (i < count) {
-      need = toi_of<EE>(x0, x1, nv, quads[i], false) < 0.0;
+      need = toi_of<EE, P>(x0, x1, nv, quads[i], false) < 0.0;
       if (!need && toi_out) toi_out[i] = 1.0;
     }
     const unsigned long long m = __ballot(need);
@@ -530,23 +530,23 @@ __device__ __forceinline__ void collision_time_block(const float* __restrict__ x
     base = __shfl(base, 0, 64);
     if (need) tq.idx[base + __popcll(m & ((1ull << lane) - 1ull))] = i;
     __syncthreads();
-    if (tq.n >= kToiDense) toi_drain<EE>(x0, x1, nv, quads, toi_out, tq, tmin);
+    if (tq.n >= kToiDense) toi_drain<EE, P>(x0, x1, nv, quads, toi_out, tq, tmin);
   }
-  toi_drain<EE>(x0, x1, nv, quads, toi_out, tq, tmin);
+  toi_drain<EE, P>(x0, x1, nv, quads, toi_out, tq, tmin);
   if (toi_min) {
     double m = wave_min(tmin);
     if (lane == 0 && m < 1.0) toi_min_add(toi_min, m);
   }
 }
 
-template <bool EE>
-__global__ void __launch_bounds__(256) collision_time_kernel(const float* __restrict__ x0, const float* __restrict__ x1,
+template <bool EE, typename P = float>
+__global__ void __launch_bounds__(256) collision_time_kernel(const P* __restrict__ x0, const P* __restrict__ x1,
                                                              int nv, const int4* __restrict__ quads, int n,
                                                              const int* __restrict__ n_dev,
                                                              double* __restrict__ toi_out,
                                                              unsigned long long* __restrict__ toi_min) {
   __shared__ ToiQueue tq;
-  collision_time_block<EE>(x0, x1, nv, quads, n, n_dev, toi_out, toi_min, blockIdx.x, gridDim.x, tq);
+  collision_time_block<EE, P>(x0, x1, nv, quads, n, n_dev, toi_out, toi_min, blockIdx.x, gridDim.x, tq);
 }
 
 // The EE and PT candidate lists of one CCD call in one launch: workgroups
@@ -766,6 +766,21 @@ static hipError_t collision_time_grid(bool ee, const float* x0, const float* x1,
   else
     hipLaunchKernelGGL(collision_time_kernel<false>, dim3(grid), dim3(256), 0, s, x0, x1, nv, q4, n, n_dev, toi_out,
                        toi_min);
+  return hipGetLastError();
+}
+
+// fp64 scalars: vec3d positions, the same FP64 arithmetic.
+hipError_t launch_collision_time_f64(bool ee, const double* x0, const double* x1, int nv, const int* quads, int n,
+                                     double* toi_out, unsigned long long* toi_min, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int4* q4 = reinterpret_cast<const int4*>(quads);
+  const int grid = toi_blocks(n);
+  if (ee)
+    hipLaunchKernelGGL((collision_time_kernel<true, double>), dim3(grid), dim3(256), 0, s, x0, x1, nv, q4, n, nullptr,
+                       toi_out, toi_min);
+  else
+    hipLaunchKernelGGL((collision_time_kernel<false, double>), dim3(grid), dim3(256), 0, s, x0, x1, nv, q4, n, nullptr,
+                       toi_out, toi_min);
   return hipGetLastError();
 }
 

@@ -26,6 +26,8 @@ hipError_t launch_contact_points(bool ee, const float* x, const float* xcs, int 
                                  float* dists, int* features, hipStream_t s);
 hipError_t launch_collision_time(bool ee, const float* x0, const float* x1, int nv, const int* quads, int n, double* toi_out,
                                  unsigned long long* toi_min, hipStream_t s);
+hipError_t launch_collision_time_f64(bool ee, const double* x0, const double* x1, int nv, const int* quads, int n,
+                                     double* toi_out, unsigned long long* toi_min, hipStream_t s);
 
 hipError_t launch_collision_time_counted(bool ee, const float* x0, const float* x1, int nv, const int* quads,
                                          const int* n_dev, int capacity, unsigned long long* toi_min, hipStream_t s);

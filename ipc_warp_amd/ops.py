@@ -216,6 +216,18 @@ def collision_time(kind, x0, x1, quads, n, toi=None, toi_min=None):
              _opt(toi_min, torch.int64, "toi_min"), stream_handle(x0.device)), f"collision_time_{kind}")
 
 
+def collision_time_f64(kind, x0, x1, quads, n, toi=None, toi_min=None):
+    """pt_collision_time / ee_collision_time (ccd/ccd.py:304-357) with fp64
+    scalars: x0, x1 float64 [V,3]; otherwise as collision_time."""
+    fn = lib().ipc_collision_time_ee_f64 if kind == "ee" else lib().ipc_collision_time_pt_f64
+    if x1.shape != x0.shape:
+        raise ValueError("x1 must have the shape of x0")
+    _vec_len(toi, n, "toi")
+    check(fn(_need(x0, torch.float64, "x0", (3,)), _need(x1, torch.float64, "x1", (3,)), x0.shape[0],
+             _rows(quads, n, "quads"), int(n), _opt(toi, torch.float64, "toi"), _opt(toi_min, torch.int64, "toi_min"),
+             stream_handle(x0.device)), f"collision_time_{kind}_f64")
+
+
 def conservative_time(kind: str, x0, x1, quads, n, toi=None, toi_min=None):
     """conservative_ee_toi / conservative_pt_toi (ccd/ccd.py:86-163) per
     candidate: toi float64 [>=n] (optional), toi_min from new_toi_min()

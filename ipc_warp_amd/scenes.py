@@ -362,7 +362,7 @@ def pair_stream(kind: str, n: int, seed: int = 0, dhat: float = 1.0e-3, scale: f
 
 
 def ccd_stream(kind: str, n: int, seed: int = 0, hit_fraction: float = 0.3, coord: float = 10.0,
-               scale: float = 0.05):
+               scale: float = 0.05, dtype=np.float32):
     """Synthetic CCD candidates with 4 private vertices each.
 
     Primitives of size ~`scale` centred uniformly in [-coord, coord]^3. A
@@ -371,9 +371,11 @@ def ccd_stream(kind: str, n: int, seed: int = 0, hit_fraction: float = 0.3, coor
     plane inside the moving triangle; EE: edge b sweeps through an interior
     point of edge a); the rest move apart (the point leaves along the face
     normal, edge b leaves along the common normal) and report no impact.
-    Returns (x0, x1 float32 [4n,3], quads int32 [n,4], t float64 [n] with
+    Returns (x0, x1 `dtype` [4n,3], quads int32 [n,4], t float64 [n] with
     t = 1 for the misses); quads[i] = 4i .. 4i+3 in the TOI argument order
-    (PT: p, t0, t1, t2; EE: a0, a1, b0, b1).
+    (PT: p, t0, t1, t2; EE: a0, a1, b0, b1). With dtype float64 the
+    positions are the unrounded construction, so t is exact up to double
+    rounding.
     """
     rng = np.random.default_rng(seed)
     hit = rng.random(n) < hit_fraction
@@ -413,8 +415,8 @@ def ccd_stream(kind: str, n: int, seed: int = 0, hit_fraction: float = 0.3, coor
         b1 = b0 + b_dir
         X0 = np.stack([a0, a1, b0, b1], axis=1)
         X1 = np.stack([a0, a1, b0 + vb, b1 + vb], axis=1)
-    x0 = X0.reshape(-1, 3).astype(np.float32)
-    x1 = X1.reshape(-1, 3).astype(np.float32)
+    x0 = X0.reshape(-1, 3).astype(dtype)
+    x1 = X1.reshape(-1, 3).astype(dtype)
     return x0, x1, np.arange(4 * n, dtype=np.int32).reshape(n, 4), t
 
 

@@ -95,6 +95,7 @@ def lib():
                                                  F64P, F64P, U8P, c_int]
         L.oracle_contact_energy_f64.argtypes = [c_int, F64P, F64P, I32P, c_int, c_dbl, c_dbl]
         L.oracle_contact_energy_f64.restype = c_dbl
+        L.oracle_collision_time_f64.argtypes = [c_int, c_int, c_int, F64P, F64P, I32P, F64P, c_int]
         L.oracle_al_hash_pos.argtypes = [c_int, c_int, c_int]
         L.oracle_al_update.argtypes = [c_int, c_int, I32P, c_int, ctypes.c_void_p, c_int, c_dbl]
         L.oracle_al_linearize.argtypes = [c_int, I32P, c_int, ctypes.c_void_p, ctypes.c_void_p, c_int, c_int, c_dbl]
@@ -307,6 +308,17 @@ def coplanarity_cubic(p0, p1, relative=False):
     coef = np.zeros(4)
     lib().oracle_coplanarity_cubic(int(relative), _p(p0, F64P), _p(p1, F64P), _p(coef, F64P))
     return coef
+
+
+def collision_time_f64(kind, x0, x1, quads, relative=True, threads=1):
+    """ccd/ccd.py:304-357 with fp64 scalars (double positions) -> toi float64 [n]."""
+    a, b = _f64(x0).reshape(-1), _f64(x1).reshape(-1)
+    q = _i32(quads).reshape(-1)
+    n = q.size // 4
+    toi = np.zeros(n)
+    lib().oracle_collision_time_f64(int(kind == "ee"), int(relative), n, _p(a, F64P), _p(b, F64P), _p(q, I32P),
+                                    _p(toi, F64P), int(threads))
+    return toi
 
 
 def collision_time(kind, x0, x1, quads, relative=False, double=True, threads=1):

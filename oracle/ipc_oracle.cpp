@@ -434,6 +434,22 @@ int oracle_collision_time(int is_ee, int relative, int use_double, int n, const 
   return capped_total;
 }
 
+// The same with fp64 scalars: double positions (vec3d), FP64 arithmetic.
+int oracle_collision_time_f64(int is_ee, int relative, int n, const double* x0, const double* x1, const int* quads,
+                              double* toi, int threads) {
+  int capped_total = 0;
+#pragma omp parallel for num_threads(threads > 0 ? threads : 1) schedule(dynamic, 64) reduction(+ : capped_total)
+  for (int i = 0; i < n; ++i) {
+    const int* q = quads + 4 * i;
+    int capped = 0;
+    V3<double> a[4], b[4];
+    for (int k = 0; k < 4; ++k) { a[k] = load3<double>(x0, q[k]); b[k] = load3<double>(x1, q[k]); }
+    toi[i] = collision_time<double>(is_ee != 0, relative != 0, a, b, &capped);
+    capped_total += capped;
+  }
+  return capped_total;
+}
+
 // ------------------------------------------------------------------- DCD ----
 // edge_edge_collision (contact.py:199-229) with the BVH replaced by an exact
 // overlap enumeration of the same inflated boxes (edge_aabb, contact.py:30-37).

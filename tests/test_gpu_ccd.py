@@ -137,3 +137,33 @@ def test_conservative_advancement_matches_oracle(dev, oracle, kind):
     assert ops.toi_value(tmin) == ref.min()
     cubic, _ = oracle.collision_time(kind, x0, x1, q)
     assert (got <= cubic + 1e-12).all()
+
+
+@pytest.mark.parametrize("kind", ["pt", "ee"])
+def test_toi_f64_on_the_unrounded_stream(dev, oracle, kind):
+    """fp64-scalar TOI (double positions) on the CCD stream kept in double:
+    per-candidate TOI within 1e-12 of the oracle's fp64 driver, every
+    constructed impact within 2e-10 of its time, the folded minimum equal
+    to the smallest TOI; on fp32-representable positions the result is the
+    fp32-position kernel's, bit for bit."""
+    x0, x1, q, t = scenes.ccd_stream(kind, 1 << 14, seed=35, dtype=np.float64)
+    n = q.shape[0]
+    qt = torch.from_numpy(q).to(dev)
+
+    def run(a, b):
+        toi = torch.full((n,), -1.0, dtype=torch.float64, device=dev)
+        tmin = ops.new_toi_min(dev)
+        ops.collision_time_f64(kind, torch.from_numpy(a).to(dev), torch.from_numpy(b).to(dev), qt, n, toi, tmin)
+        torch.cuda.synchronize()
+        return toi.cpu().numpy(), ops.toi_value(tmin)
+
+    toi, tmin = run(x0, x1)
+    ref = oracle.collision_time_f64(kind, x0, x1, q, threads=8)
+    assert np.abs(toi - ref).max() <= 1e-12
+    hits = t < 1
+    assert np.abs(toi[hits] - t[hits]).max() <= 2e-10
+    assert tmin == toi.min()
+    a32, b32 = x0.astype(np.float32), x1.astype(np.float32)
+    toi64, _ = run(a32.astype(np.float64), b32.astype(np.float64))
+    toi32, _ = _gpu_toi(dev, kind, a32, b32, q)
+    assert np.array_equal(toi64.view(np.uint64), toi32.view(np.uint64))

@@ -55,3 +55,28 @@ def test_inactive_contacts_write_nothing(oracle):
     rows, cols, vals, b, act = oracle.contact_hessian_f64("pt", x, c, L0, KAPPA)
     assert not act[3] and act[[0, 1, 2, 4, 5, 6, 7]].all()
     assert not rows[48:64].any() and not vals[48:64].any() and not b[c[3]].any()
+
+
+@pytest.mark.parametrize("kind", ["pt", "ee"])
+def test_toi_f64_equals_the_double_driver_on_fp32_inputs(oracle, kind):
+    """collision_time_f64 evaluates ccd/ccd.py:304-357 in double exactly like
+    the fp32-position driver's double mode: on positions that fp32 holds
+    exactly the two agree bit for bit."""
+    x0, x1, q, _ = scenes.ccd_stream(kind, 4096, seed=12)
+    r32, _ = oracle.collision_time(kind, x0, x1, q, relative=True, threads=4)
+    r64 = oracle.collision_time_f64(kind, x0.astype(np.float64), x1.astype(np.float64), q, threads=4)
+    assert np.array_equal(r64.view(np.uint64), r32.view(np.uint64))
+
+
+@pytest.mark.parametrize("kind", ["pt", "ee"])
+def test_toi_f64_resolves_the_unrounded_construction(oracle, kind):
+    """On the unrounded double construction (coordinates ~10, impacts at a
+    known time t) every impact comes back within 2e-10 of t; the same
+    stream rounded to fp32 positions misses t by ~1e-6 at the median, so a
+    driver that dropped the double positions would fail."""
+    x0, x1, q, t = scenes.ccd_stream(kind, 4096, seed=13, dtype=np.float64)
+    hits = t < 1
+    r64 = oracle.collision_time_f64(kind, x0, x1, q, threads=4)
+    assert np.abs(r64[hits] - t[hits]).max() <= 2e-10
+    r32, _ = oracle.collision_time(kind, x0.astype(np.float32), x1.astype(np.float32), q, relative=True, threads=4)
+    assert np.median(np.abs(r32[hits] - t[hits])) > 1e-7
