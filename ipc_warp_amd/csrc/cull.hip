@@ -677,6 +677,10 @@ __device__ __forceinline__ bool pair_distance_test(const QueryArgs& A, int4 rec)
 }
 
 constexpr int kCandQueue = 128;  // candidate pairs a wave holds for its dense distance tests
+// A DCD launch stages fewer accepted pairs than the other queries, so that
+// its LDS (stack, stage and candidate queues) stays within 20 KiB: eight
+// workgroups of four waves per CU, the most a CU runs of them.
+constexpr int kDcdBlockOut = 192;
 
 // Eight lanes share one query. A stack entry (level L, mask, cbase) stands
 // for the level-L nodes picked by the mask bits c of one parent; node c's
@@ -693,7 +697,7 @@ template <int Q>
 struct QuerySmemT {
   unsigned long long stack[kGroupsPerBlock][kMaskStack];
   int level_off[kMaxLevels], level_cnt[kMaxLevels];
-  BlockStage stage;
+  BlockStageN<Q == 1 ? kBlockOut : kDcdBlockOut> stage;
   int4 cand_rec[kQueryBlock / 64][Q];
   int2 cand_ids[kQueryBlock / 64][Q];
 };
@@ -708,7 +712,7 @@ __device__ __forceinline__ void query_block(const QueryArgs& A, int bid, Smem& s
   auto& stack = sm.stack;
   int* level_off = sm.level_off;
   int* level_cnt = sm.level_cnt;
-  BlockStage& stage = sm.stage;
+  auto& stage = sm.stage;
   const StageOut out{A.rec, A.ids, A.count, A.capacity};
   int tid = threadIdx.x;
   if (tid < kMaxLevels) {

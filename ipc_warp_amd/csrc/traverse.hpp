@@ -57,16 +57,19 @@ __device__ __forceinline__ int wave_append(bool want, int* count) {
   return base + __popcll(m & ((1ull << lane) - 1ull));
 }
 
-// Accepted pairs staged in LDS per workgroup. Appends from every wave go to
-// an LDS counter; the workgroup reserves its output range with one global
-// atomicAdd at the end (a single global counter hit by every wave
-// serialises). Pairs past kBlockOut go straight to the output, one global
-// atomicAdd per wave.
-struct BlockStage {
-  int4 rec[kBlockOut];
-  int2 ids[kBlockOut];
+// Accepted pairs staged in LDS per workgroup, N of them. Appends from every
+// wave go to an LDS counter; the workgroup reserves its output range with
+// one global atomicAdd at the end (a single global counter hit by every wave
+// serialises). Pairs past N go straight to the output, one global atomicAdd
+// per wave.
+template <int N>
+struct BlockStageN {
+  static constexpr int kCap = N;
+  int4 rec[N];
+  int2 ids[N];
   int n, base;
 };
+using BlockStage = BlockStageN<kBlockOut>;
 
 struct StageOut {
   int4* rec;  // may be null (count only)
@@ -76,15 +79,17 @@ struct StageOut {
 };
 
 // Called by the whole workgroup before the first stage_append.
-__device__ __forceinline__ void stage_begin(BlockStage& S) {
+template <typename Stage>
+__device__ __forceinline__ void stage_begin(Stage& S) {
   if (threadIdx.x == 0) S.n = 0;
   __syncthreads();
 }
 
 // Called by every lane of a wave together (ballots inside).
-__device__ __forceinline__ void stage_append(BlockStage& S, const StageOut& O, bool want, int4 rec, int2 id) {
+template <typename Stage>
+__device__ __forceinline__ void stage_append(Stage& S, const StageOut& O, bool want, int4 rec, int2 id) {
   int s = wave_append(want, &S.n);
-  bool spill = want && s >= kBlockOut;
+  bool spill = want && s >= Stage::kCap;
   int g = wave_append(spill, O.count);
   if (want && !spill) {
     S.rec[s] = rec;
@@ -96,9 +101,10 @@ __device__ __forceinline__ void stage_append(BlockStage& S, const StageOut& O, b
 }
 
 // Called by the whole workgroup once every wave has finished appending.
-__device__ __forceinline__ void stage_flush(BlockStage& S, const StageOut& O) {
+template <typename Stage>
+__device__ __forceinline__ void stage_flush(Stage& S, const StageOut& O) {
   __syncthreads();
-  int staged = S.n < kBlockOut ? S.n : kBlockOut;
+  int staged = S.n < Stage::kCap ? S.n : Stage::kCap;
   if (threadIdx.x == 0) S.base = staged ? atomicAdd(O.count, staged) : 0;
   __syncthreads();
   for (int k = threadIdx.x; k < staged; k += blockDim.x) {
