@@ -185,10 +185,16 @@ __global__ void __launch_bounds__(256) morton_kernel(int mode, int n, const int*
   float4 lo, hi;
   prim_box(mode, i, prims, x0, x1, dx, pad, lo, hi);
   float c[3] = {0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z)};
+  // Cubic cells: every axis is quantised by the scene's largest extent, so
+  // that 8 consecutive codes group primitives that are close in every
+  // direction (per-axis scaling stretches the cells of an elongated scene:
+  // 8:1 on the rod bundle, where it made EE queries pop 43 nodes each
+  // instead of 30).
+  float ext = 0.f;
+  for (int a = 0; a < 3; ++a) ext = fmaxf(ext, float_of(sb[3 + a]) - float_of(sb[a]));
   unsigned q[3];
   for (int a = 0; a < 3; ++a) {
-    float b0 = float_of(sb[a]), b1 = float_of(sb[3 + a]);
-    float ext = b1 - b0;
+    float b0 = float_of(sb[a]);
     float u = ext > 0.f ? (c[a] - b0) / ext : 0.5f;
     float s = u * 1024.f;
     s = s < 0.f ? 0.f : (s > 1023.f ? 1023.f : s);
@@ -212,46 +218,49 @@ __global__ void __launch_bounds__(256) morton_kernel(int mode, int n, const int*
 // launches (scan, scatter, rank), against a block sort and about seven merge
 // launches for a full radix sort of the codes at these sizes.
 __global__ void __launch_bounds__(256) cell_scatter_kernel(int n, const unsigned* __restrict__ codes,
-                                                           int* __restrict__ cursor, int* __restrict__ ids_out) {
+                                                           int* __restrict__ cursor,
+                                                           unsigned long long* __restrict__ keys_out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) ids_out[atomicAdd(&cursor[cell_of(codes[i])], 1)] = i;
+  if (i < n) {
+    const unsigned code = codes[i];
+    keys_out[atomicAdd(&cursor[cell_of(code)], 1)] = (static_cast<unsigned long long>(code) << 32) | unsigned(i);
+  }
 }
 
-constexpr int kRankCap = 128;
+constexpr int kRankCap = 512;
 
 // Slot s of the scattered order: its primitive goes to its cell's start plus
-// the number of the cell's primitives with a smaller (code, id). After the
-// scatter, cursor[c] is cell c's end and count[c] its size.
-__global__ void __launch_bounds__(256) cell_rank_kernel(int n, const unsigned* __restrict__ codes,
-                                                        const int* __restrict__ count, const int* __restrict__ cursor,
-                                                        const int* __restrict__ scattered, int* __restrict__ ids_out) {
+// the number of the cell's (code, id) keys below its own; the cell's keys lie
+// contiguously in the scattered order, so a workgroup's threads of one cell
+// read the same run. After the scatter, cursor[c] is cell c's end and
+// count[c] its size.
+__global__ void __launch_bounds__(256) cell_rank_kernel(int n, const int* __restrict__ count,
+                                                        const int* __restrict__ cursor,
+                                                        const unsigned long long* __restrict__ scattered,
+                                                        int* __restrict__ ids_out) {
   const int slot = blockIdx.x * blockDim.x + threadIdx.x;
   if (slot >= n) return;
-  const int i = scattered[slot];
-  const unsigned code = codes[i];
-  const int c = cell_of(code);
+  const unsigned long long key = scattered[slot];
+  const int i = static_cast<int>(key & 0xffffffffull);
+  const int c = cell_of(static_cast<unsigned>(key >> 32));
   const int hi = cursor[c], len = count[c], lo = hi - len;
   if (len > kRankCap) {
     ids_out[slot] = i;
     return;
   }
-  const unsigned long long key = (static_cast<unsigned long long>(code) << 32) | unsigned(i);
   int rank = 0;
-  for (int k = lo; k < hi; ++k) {
-    const int j = scattered[k];
-    rank += ((static_cast<unsigned long long>(codes[j]) << 32) | unsigned(j)) < key;
-  }
+  for (int k = lo; k < hi; ++k) rank += scattered[k] < key;
   ids_out[lo + rank] = i;
 }
 
 // temp of a re-sort: cell counts (cleared by init_bounds_kernel, counted by
-// morton_kernel), cell cursors, the scan's workspace, the scattered order (n
-// ids). Each part starts on a 256-byte boundary: the scan's workspace holds
-// its look-back state, which hipCUB expects aligned.
+// morton_kernel), cell cursors, the scan's workspace, the scattered keys (n
+// of 8 bytes). Each part starts on a 256-byte boundary: the scan's workspace
+// holds its look-back state, which hipCUB expects aligned.
 struct CellBufs {
   int* count;
   int* cursor;
-  int* scattered;
+  unsigned long long* scattered;
   void* scan_temp;
   size_t scan_bytes;
 };
@@ -263,25 +272,25 @@ static hipError_t cell_buffers(DevBuf& temp, int n, CellBufs& c) {
   auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
   const size_t cell_bytes = sizeof(int) * kCells;  // a multiple of 256
   const size_t scan_at = 2 * cell_bytes, order_at = scan_at + up(c.scan_bytes);
-  if (e == hipSuccess) e = temp.ensure(order_at + sizeof(int) * size_t(n > 0 ? n : 1));
+  if (e == hipSuccess) e = temp.ensure(order_at + sizeof(unsigned long long) * size_t(n > 0 ? n : 1));
   if (e != hipSuccess) return e;
   c.count = temp.as<int>();
   c.cursor = c.count + kCells;
   c.scan_temp = static_cast<char*>(temp.p) + scan_at;
-  c.scattered = reinterpret_cast<int*>(static_cast<char*>(temp.p) + order_at);
+  c.scattered = reinterpret_cast<unsigned long long*>(static_cast<char*>(temp.p) + order_at);
   return hipSuccess;
 }
 
 // ids_out receives the n primitive ids in Morton order (cells by counting
-// sort, each cell ranked by (code, id)).
+// sort, each cell of up to kRankCap ranked by (code, id)).
 static hipError_t sort_codes(const CellBufs& c, const unsigned* codes, int n, int* ids_out, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   size_t scan_bytes = c.scan_bytes;
   hipError_t e = hipcub::DeviceScan::ExclusiveSum(c.scan_temp, scan_bytes, c.count, c.cursor, kCells, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(cell_scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, codes, c.cursor, c.scattered);
-  hipLaunchKernelGGL(cell_rank_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, codes, c.count, c.cursor,
-                     c.scattered, ids_out);
+  hipLaunchKernelGGL(cell_rank_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, c.count, c.cursor, c.scattered,
+                     ids_out);
   return hipGetLastError();
 }
 
