@@ -178,8 +178,11 @@ int ipc_ctx_destroy(ipc_ctx* ctx);
  * Replace edge_edge_collision / point_triangle_collision and the BVH
  * updates of detect_collision (contact.py:199-336,672-725). A BVH over the
  * current positions x is built, every edge (EE) or every vertex (PT) queries
- * it with the reference's inflation (edge boxes +- (thickness + buffer);
- * vertex boxes +- 2 (thickness + buffer) against triangle bounds), and pairs
+ * it (edge boxes +- (thickness + s); vertex boxes +- 2 (thickness + s)
+ * against triangle bounds, s = min(buffer, thickness 2^-10 + 2^-16 M), M
+ * the primitive's largest coordinate magnitude: the reference's inflation
+ * where s = buffer, and otherwise still wide enough for every pair its fp32
+ * distance test accepts, so the pairs found are the reference's), and pairs
  * passing the reference's filters (EE: i < j, no shared vertex; both: not
  * the same body when exclude_same_body) and its fp32 distance test
  * (< 2 thickness) are appended: contacts int32 [capacity][4] as

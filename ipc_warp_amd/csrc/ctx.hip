@@ -194,7 +194,8 @@ QueryLaunch dcd_pt_query(ipc_ctx* c, const float* x, float thickness, float buff
   q.tris = c->tris.as<int>();
   q.body = c->body.as<int>();
   q.x0 = x;
-  q.pad = (thickness + buffer) * 2.0f;
+  q.pad = thickness;  // boxes of 2 (thickness + min(buffer, slack)) in the query
+  q.cap = buffer;
   q.thickness = thickness;
   q.exclude_same_body = exclude_same_body;
   q.bvh = &c->tri_bvh;
@@ -264,7 +265,7 @@ BuildPrologue zero_prologue(int* a, int* b = nullptr) {
 }  // namespace
 
 // edge_edge_collision over a freshly built edge BVH (contact.py:672-704):
-// boxes +- (thickness + buffer), accepted pairs into contacts/e0e1 up to
+// boxes +- (thickness + min(buffer, slack)), accepted pairs into contacts/e0e1 up to
 // capacity, total count into *count (device int).
 int ipc_detect_collision_ee(ipc_ctx* c, const float* x, float thickness, float buffer, int exclude_same_body,
                             int* contacts, int* e0e1, int capacity, int* count, void* stream) {
@@ -273,7 +274,7 @@ int ipc_detect_collision_ee(ipc_ctx* c, const float* x, float thickness, float b
   if (!lists_aligned(contacts, e0e1))
     return ctx_fail(IPC_EINVAL, "detect_collision_ee: records must be 16-byte and ids 8-byte aligned");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  BuildJob job{&c->edge_bvh, BOX_EDGE, c->ne, c->edges.as<int>(), nullptr, thickness + buffer};
+  BuildJob job{&c->edge_bvh, BOX_EDGE, c->ne, c->edges.as<int>(), nullptr, thickness, nullptr, 0, buffer};
   int st = ctx_hip(timed(c, IPC_TIMER_BUILD_EDGE, s, [&] {
                      return build_bvhs(&job, 1, c->body.as<int>(), x, zero_prologue(count), s);
                    }),
@@ -321,7 +322,7 @@ int ipc_detect_collision(ipc_ctx* c, const float* x, float thickness, float buff
     return ctx_fail(IPC_EINVAL, "detect_collision: records must be 16-byte and ids 8-byte aligned");
   hipStream_t s = static_cast<hipStream_t>(stream);
   const int n_tri = c->nv > 0 ? c->nf : 0;
-  BuildJob jobs[2] = {{&c->edge_bvh, BOX_EDGE, c->ne, c->edges.as<int>(), nullptr, thickness + buffer},
+  BuildJob jobs[2] = {{&c->edge_bvh, BOX_EDGE, c->ne, c->edges.as<int>(), nullptr, thickness, nullptr, 0, buffer},
                       {&c->tri_bvh, BOX_TRI, n_tri, c->tris.as<int>(), nullptr, 0.0f}};
   QueryOrder qo;
   int st = ctx_hip(query_order(c->qcost_dcd, c->qperm_dcd, c->ne, n_tri > 0 ? c->nv : 0, s, qo),

@@ -92,9 +92,28 @@ __device__ __forceinline__ V3<float> end_pos(const float* __restrict__ x0, const
 
 // The box of a primitive with vertex ids v[0 .. k) (k = 2 for an edge, 3 for
 // a triangle) in box mode 0..3.
+// DCD box slack. The reference inflates each DCD edge box by thickness +
+// buffer (and each query point by twice that) but keeps only pairs whose fp32
+// distance is below 2 thickness; a box needs only thickness plus the
+// rounding of that distance test. With M the largest coordinate magnitude of
+// the primitive, the fp32 distance of two primitives within 2 thickness of
+// each other is at most about 12 ulp(M) below their exact distance, and
+// sigma = thickness 2^-10 + 2^-16 M (256 ulp(M) and more) covers it. A DCD
+// box is inflated by thickness + min(buffer, sigma): every pair the distance
+// test accepts still overlaps, and where sigma >= buffer the box is the
+// reference's, bit for bit, so the set found is the reference's either way.
+__device__ __forceinline__ float dcd_pad(float thickness, float buffer, float m) {
+  return thickness + fminf(buffer, thickness * 0x1p-10f + m * 0x1p-16f);
+}
+__device__ __forceinline__ float abs_max(V3<float> a, V3<float> b) {
+  return fmaxf(fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(a.z), fabsf(b.x))), fmaxf(fabsf(b.y), fabsf(b.z)));
+}
+
+// With mode BOX_EDGE (the DCD edge boxes), pad is the thickness and cap the
+// buffer; the other modes are inflated by pad.
 __device__ __forceinline__ void vertex_box(int mode, int k, const int v[3], const float* __restrict__ x0,
                                            const float* __restrict__ x1, const float* __restrict__ dx, float pad,
-                                           float4& lo4, float4& hi4) {
+                                           float cap, float4& lo4, float4& hi4) {
   V3<float> lo, hi;
   lo = hi = ldx(x0, v[0]);
   for (int j = 1; j < k; ++j) {
@@ -109,6 +128,7 @@ __device__ __forceinline__ void vertex_box(int mode, int k, const int v[3], cons
       hi = vmax(hi, q);
     }
   }
+  if (mode == BOX_EDGE) pad = dcd_pad(pad, cap, abs_max(lo, hi));
   V3<float> pv = mk(pad, pad, pad);
   lo = lo - pv;
   hi = hi + pv;
@@ -126,7 +146,7 @@ __device__ __forceinline__ void prim_box(int mode, int p, const int* __restrict_
   }
   const int k = (mode == 0 || mode == 2) ? 2 : 3;
   const int v[3] = {prims[k * p], prims[k * p + 1], k == 3 ? prims[k * p + 2] : 0};
-  vertex_box(mode, k, v, x0, x1, dx, pad, lo4, hi4);
+  vertex_box(mode, k, v, x0, x1, dx, pad, 0.f, lo4, hi4);
 }
 
 // Scene bounds of the primitive box centres (ordered-int atomics), for the
@@ -331,7 +351,7 @@ struct LeafJob {
   const int* prims;
   const float* x1;
   const float* dx;  // with dx, end positions are x0 - dx (x1 unused)
-  float pad;
+  float pad, cap;  // cap: the buffer of a BOX_EDGE job
   const int* ids;  // sorted primitive ids (leaf order)
   int* leaf_id;
   int4* leaf_vid;
@@ -377,7 +397,7 @@ __device__ __forceinline__ void leaf_block(const LeafJob& J, int bid, const int*
       vid = make_int4(v0, v1, v2, prim);
     }
     const int v[3] = {vid.x, vid.y, stride == 3 ? vid.z : 0};
-    vertex_box(J.mode, stride, v, x0, J.x1, J.dx, J.pad, lo, hi);
+    vertex_box(J.mode, stride, v, x0, J.x1, J.dx, J.pad, J.cap, lo, hi);
     lo.w = __int_as_float(bd);
     hi.w = __int_as_float(bd);
   }
@@ -582,7 +602,8 @@ struct QueryArgs {
   const float4* qlo;  // modes 0, 2: query box of slot k (the tree's leaf k)
   const float4* qhi;
   const int* order;   // optional query order (slot -> query primitive)
-  float pad;          // mode 1: 2 (th + buf); mode 3: CCD padding
+  float pad;          // mode 1: the thickness; mode 3: CCD padding
+  float cap;          // mode 1: the buffer
   float thickness;
   int exclude_same_body;
   // modes 0, 2: query slot k is leaf k of the same tree; each unordered pair
@@ -771,7 +792,10 @@ __device__ __forceinline__ void query_block(const QueryArgs& A, int bid, Smem& s
         lo = vmin(p, q);
         hi = vmax(p, q);
       }
-      V3<float> pv = mk(A.pad, A.pad, A.pad);
+      // DCD: 2 (thickness + min(buffer, sigma)); the reference's 2 (thickness
+      // + buffer) where sigma >= buffer
+      const float pe = MODE == 1 ? dcd_pad(A.pad, A.cap, abs_max(p, p)) * 2.0f : A.pad;
+      V3<float> pv = mk(pe, pe, pe);
       lo = lo - pv;
       hi = hi + pv;
       qlo[0] = lo.x; qlo[1] = lo.y; qlo[2] = lo.z;
@@ -1124,6 +1148,7 @@ LeafJob leaf_job(const BuildJob& j) {
   L.x1 = j.x1;
   L.dx = j.dx;
   L.pad = j.pad;
+  L.cap = j.cap;
   L.ids = B.leaf_order.as<int>();
   L.leaf_id = B.leaf_id.as<int>();
   L.leaf_vid = B.leaf_vid.as<int4>();
@@ -1236,6 +1261,7 @@ QueryArgs query_args(const QueryLaunch& q) {
   A.qhi = q.qhi;
   A.order = q.order;
   A.pad = q.pad;
+  A.cap = q.cap;
   A.thickness = q.thickness;
   A.exclude_same_body = q.exclude_same_body;
   A.self_query = q.self_query;

@@ -88,6 +88,7 @@ struct BuildJob {
   // that follow.
   const float* dx = nullptr;
   int n_end = 0;
+  float cap = 0.f;  // BOX_EDGE: the buffer (pad is the thickness)
 };
 
 // Device writes folded into the first workgroup of a call's first launch,
@@ -144,7 +145,8 @@ struct QueryLaunch {
   const float4* qlo;  // modes 0, 2: query box per slot (leaf boxes of the same tree)
   const float4* qhi;
   const int* order;   // optional slot -> query primitive (Morton order of the queries)
-  float pad;
+  float pad;          // DCD PT: the thickness; CCD PT: the padding
+  float cap = 0.f;    // DCD PT: the buffer
   float thickness;
   int exclude_same_body;
   int self_query;  // EE: slot k is leaf k of bvh (order = its leaf_id, qlo/qhi = its leaf boxes)
