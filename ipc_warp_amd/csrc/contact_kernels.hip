@@ -625,7 +625,10 @@ __global__ void __launch_bounds__(256) record_intersections_kernel(
 static inline int blocks_for(long long n, int bs) { return int((n + bs - 1) / bs); }
 
 // Contacts per wave: a list too short to fill the device several times over
-// is spread thinly (contact_slot); long lists run one contact per lane.
+// is spread thinly (contact_slot); long lists run one contact per lane. The
+// EE and PT halves of one launch share the device, so they are spread by
+// their total length (a PT list of 237k beside 697k EE contacts spread over
+// 16 lanes a wave issued four times the instructions it needed).
 constexpr int kSpreadBelow = 1 << 18;
 constexpr int kSpreadLanes = 16;        // Hessian (measured best of 4..64 at ~5e3 contacts)
 constexpr int kSpreadLanesEnergy = 32;  // energy (no Hessian, shorter chain)
@@ -633,22 +636,23 @@ static inline int lanes_per_wave(int n, int spread = kSpreadLanes) { return n < 
 static inline int blocks_spread(int n, int lpw) { return blocks_for(blocks_for(n, lpw), 4); }  // 4 waves per block
 
 static HessianArgs hessian_args(const float* x, const float* xcs, int nv, const int* contacts, int n, float l0,
-                                double kappa, int offset, int* rows, int* cols, float* vals, float* b) {
+                                double kappa, int offset, int* rows, int* cols, float* vals, float* b,
+                                int n_launch) {
   return HessianArgs{x, xcs, nv, reinterpret_cast<const int4*>(contacts), n, l0, kappa, offset, rows, cols, vals, b,
-                     lanes_per_wave(n)};
+                     lanes_per_wave(n_launch)};
 }
 
 static EnergyArgs energy_args(const float* x, const float* xcs, int nv, const int* contacts, int n, float l0,
-                              double kappa, double* energy) {
+                              double kappa, double* energy, int n_launch) {
   return EnergyArgs{x, xcs, nv, reinterpret_cast<const int4*>(contacts), n, l0, kappa, energy,
-                    lanes_per_wave(n, kSpreadLanesEnergy)};
+                    lanes_per_wave(n_launch, kSpreadLanesEnergy)};
 }
 
 hipError_t launch_contact_hessian(bool ee, const float* x, const float* xcs, int nv, const int* contacts, int n, float l0,
                                   double kappa, int offset, int* rows, int* cols, float* vals, float* b,
                                   hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const HessianArgs A = hessian_args(x, xcs, nv, contacts, n, l0, kappa, offset, rows, cols, vals, b);
+  const HessianArgs A = hessian_args(x, xcs, nv, contacts, n, l0, kappa, offset, rows, cols, vals, b, n);
   if (ee)
     hipLaunchKernelGGL(contact_hessian_kernel<true>, dim3(blocks_spread(n, A.lpw)), dim3(256), 0, s, A);
   else
@@ -664,8 +668,8 @@ hipError_t launch_collision_hessian(const float* x, const float* xcs, int nv, co
     if (e != hipSuccess) return e;
     return launch_contact_hessian(false, x, nullptr, nv, pt, n_pt, l0, kappa, n_ee, rows, cols, vals, b, s);
   }
-  const HessianArgs E = hessian_args(x, xcs, nv, ee, n_ee, l0, kappa, 0, rows, cols, vals, b);
-  const HessianArgs P = hessian_args(x, nullptr, nv, pt, n_pt, l0, kappa, n_ee, rows, cols, vals, b);
+  const HessianArgs E = hessian_args(x, xcs, nv, ee, n_ee, l0, kappa, 0, rows, cols, vals, b, n_ee + n_pt);
+  const HessianArgs P = hessian_args(x, nullptr, nv, pt, n_pt, l0, kappa, n_ee, rows, cols, vals, b, n_ee + n_pt);
   const int be = blocks_spread(n_ee, E.lpw), bp = blocks_spread(n_pt, P.lpw);
   hipLaunchKernelGGL(contact_hessian_pair_kernel<>, dim3(be + bp), dim3(256), 0, s, E, P, be);
   return hipGetLastError();
@@ -703,7 +707,7 @@ hipError_t launch_contact_energy_f64(bool ee, const double* x, const double* xcs
 hipError_t launch_contact_energy(bool ee, const float* x, const float* xcs, int nv, const int* contacts, int n, float l0,
                                  double kappa, double* energy, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const EnergyArgs A = energy_args(x, xcs, nv, contacts, n, l0, kappa, energy);
+  const EnergyArgs A = energy_args(x, xcs, nv, contacts, n, l0, kappa, energy, n);
   const int blocks = blocks_spread(n, A.lpw);
   if (ee)
     hipLaunchKernelGGL(contact_energy_kernel<true>, dim3(energy_grid(blocks)), dim3(256), 0, s, A, blocks);
@@ -724,8 +728,8 @@ hipError_t launch_collision_energy(const float* x, const float* xcs, int nv, con
     if (e != hipSuccess) return e;
     return launch_contact_energy(false, x, nullptr, nv, pt, n_pt, l0, kappa, energy, s);
   }
-  const EnergyArgs E = energy_args(x, xcs, nv, ee, n_ee, l0, kappa, energy);
-  const EnergyArgs P = energy_args(x, nullptr, nv, pt, n_pt, l0, kappa, energy);
+  const EnergyArgs E = energy_args(x, xcs, nv, ee, n_ee, l0, kappa, energy, n_ee + n_pt);
+  const EnergyArgs P = energy_args(x, nullptr, nv, pt, n_pt, l0, kappa, energy, n_ee + n_pt);
   const int be = blocks_spread(n_ee, E.lpw), bp = blocks_spread(n_pt, P.lpw);
   const int ge = energy_grid(be), gp = energy_grid(bp);
   hipLaunchKernelGGL(contact_energy_pair_kernel<>, dim3(ge + gp), dim3(256), 0, s, E, P, be, ge, bp);
