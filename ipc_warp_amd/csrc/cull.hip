@@ -187,6 +187,30 @@ __global__ void __launch_bounds__(256) centre_bounds_kernel(int mode, int n, con
   }
 }
 
+// Adds one for every lane with `want` to counters[cell], with one atomic per
+// distinct cell of the wave (the lanes of a wave hold neighbouring
+// primitives, and with cubic cells a few dozen of them share a cell: one
+// atomic per lane serialised on those addresses). Returns the lane's slot,
+// the counter's old value plus its rank among the wave's lanes of the same
+// cell, or -1 without `want`. Called by every lane of the wave.
+__device__ __forceinline__ int wave_cell_add(int* __restrict__ counters, int cell, bool want) {
+  const int lane = __lane_id();
+  unsigned long long left = __ballot(want);
+  int slot = -1;
+  while (left != 0ull) {
+    const int leader = __ffsll(static_cast<long long>(left)) - 1;
+    const int lc = __builtin_amdgcn_readlane(cell, leader);
+    const bool mine = want && cell == lc;
+    const unsigned long long same = __ballot(mine);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&counters[lc], __popcll(same));
+    base = __builtin_amdgcn_readlane(base, leader);
+    if (mine) slot = base + __popcll(same & ((1ull << lane) - 1ull));
+    left &= ~same;
+  }
+  return slot;
+}
+
 __global__ void __launch_bounds__(256) morton_kernel(int mode, int n, const int* __restrict__ prims,
                                                      const float* __restrict__ x0, const float* __restrict__ x1,
                                                      const float* __restrict__ dx, float pad,
@@ -201,7 +225,8 @@ __global__ void __launch_bounds__(256) morton_kernel(int mode, int n, const int*
   }
   __syncthreads();
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const bool live = i < n;
+  if (!live) i = n - 1;  // a valid primitive; its code is neither stored nor counted
   float4 lo, hi;
   prim_box(mode, i, prims, x0, x1, dx, pad, lo, hi);
   float c[3] = {0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z)};
@@ -221,8 +246,8 @@ __global__ void __launch_bounds__(256) morton_kernel(int mode, int n, const int*
     q[a] = unsigned(s);
   }
   const unsigned code = (expand_bits10(q[0]) << 2) | (expand_bits10(q[1]) << 1) | expand_bits10(q[2]);
-  codes[i] = code;
-  atomicAdd(&count[cell_of(code)], 1);
+  if (live) codes[i] = code;
+  wave_cell_add(count, cell_of(code), live);
 }
 
 // Leaf order by Morton code: primitives are bucketed by the top 18 of the 30
@@ -241,10 +266,10 @@ __global__ void __launch_bounds__(256) cell_scatter_kernel(int n, const unsigned
                                                            int* __restrict__ cursor,
                                                            unsigned long long* __restrict__ keys_out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    const unsigned code = codes[i];
-    keys_out[atomicAdd(&cursor[cell_of(code)], 1)] = (static_cast<unsigned long long>(code) << 32) | unsigned(i);
-  }
+  const bool live = i < n;
+  const unsigned code = live ? codes[i] : 0u;
+  const int at = wave_cell_add(cursor, cell_of(code), live);
+  if (live) keys_out[at] = (static_cast<unsigned long long>(code) << 32) | unsigned(i);
 }
 
 constexpr int kRankCap = 512;
