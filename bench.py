@@ -114,6 +114,22 @@ def measured_traffic(kernel: str, path: str = TRAFFIC_PROFILE, grid: int | None 
     return pick[0]["fetch_bytes"] + pick[0]["write_bytes"]
 
 
+ISSUE_PROFILE = os.path.join(ROOT, "profiles", "rod_twist_issue_counters.json")
+
+
+def measured_issue(kernel: str):
+    """VALU issue of `kernel` on the rod-twist step from the committed
+    rocprofv3 PMC summary (VALUBusy: share of the CUs' time with a VALU
+    instruction issuing; VALU instructions per launch), or None."""
+    try:
+        with open(ISSUE_PROFILE) as f:
+            e = json.load(f)["kernels"][kernel]
+    except (OSError, ValueError, KeyError):
+        return None
+    return {"valu_busy": e["valu_busy"], "valu_insts_per_launch": e["valu_insts"],
+            "valu_utilization": e["valu_utilization"], "source": os.path.relpath(ISSUE_PROFILE, ROOT)}
+
+
 FP64_PROFILE = os.path.join(ROOT, "profiles", "toi_stream_fp64.json")
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 vector
 
@@ -745,6 +761,8 @@ def main() -> int:
             "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
             "algorithmic_bytes_per_launch": alg[dominant],
             "avg_launch_ms": kernel_ms[dominant],
+            # the query launches are bound by VALU issue, not by HBM
+            "issue": measured_issue(dominant),
         },
         "hessian_on_workload": {"achieved": hess_gbs, "unit": "GB/s", "frac": hess_gbs / HBM_PEAK_GBS,
                                 "traffic": measured_traffic("hessian"), "avg_launch_ms": kernel_ms["hessian"],
