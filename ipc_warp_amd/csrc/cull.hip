@@ -102,8 +102,11 @@ __device__ __forceinline__ V3<float> end_pos(const float* __restrict__ x0, const
 // box is inflated by thickness + min(buffer, sigma): every pair the distance
 // test accepts still overlaps, and where sigma >= buffer the box is the
 // reference's, bit for bit, so the set found is the reference's either way.
+// (a NaN buffer, or a NaN or infinite coordinate, selects the buffer: the
+// reference's box exactly)
 __device__ __forceinline__ float dcd_pad(float thickness, float buffer, float m) {
-  return thickness + fminf(buffer, thickness * 0x1p-10f + m * 0x1p-16f);
+  const float sigma = thickness * 0x1p-10f + m * 0x1p-16f;
+  return thickness + (buffer >= sigma ? sigma : buffer);
 }
 __device__ __forceinline__ float abs_max(V3<float> a, V3<float> b) {
   return fmaxf(fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(a.z), fabsf(b.x))), fmaxf(fabsf(b.y), fabsf(b.z)));
