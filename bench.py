@@ -390,33 +390,37 @@ def bsr_assembly(S, x0, soup, dev, reps: int = 20):
 def newton_system_cg(S, x0, soup, dev, h: float = 0.025):
     """The linear solve of one rod-twist Newton iteration (time_integrator_base.py:
     500-507): A = M + h^2 (K + H_c) assembled with ipc_bsr_from_triplets (K the
-    PSD-projected Neo-Hookean tet Hessian of the harness, E = 1e5, nu = 0.4,
-    rho = 1000; H_c the contact Hessian) and solved by ipc_bsr_cg to the
+    stable Neo-Hookean tet Hessian of ipc_tet_hessian, fem/fem.py:138-203,
+    E = 1e5, nu = 0.4, rho = 1000; H_c the contact Hessian) and solved by
+    ipc_bsr_cg to the
     reference's tolerance 1e-4, unpreconditioned (as the reference calls cg)
     and with the Jacobi preconditioner, at most 5000 iterations (the barrier
     terms make this system far too ill-conditioned for CG to reach 1e-4; the
     reference's rod scenes use its direct solver)."""
     import torch
 
-    from ipc_warp_amd import sparse
-    from ipc_warp_amd.integrator import NeoHookeanTets, lame
+    from ipc_warp_amd import ops, sparse
+    from ipc_warp_amd.integrator import lame
 
     S.x.copy_(x0)
     S.detect_collision()
     trip = S.compute_collision_hessian(torch.zeros_like(S.x))
     nv = S.x.shape[0]
-    T = torch.from_numpy(soup.tets.astype(np.int64)).to(dev)
+    T = torch.from_numpy(soup.tets.astype(np.int32)).to(dev)
+    n_tet = int(T.shape[0])
     mu, lam = lame(1e5, 0.4)
-    el = NeoHookeanTets(torch.from_numpy(soup.xcs).to(dev), T, mu, lam)
-    _, He = el.gradient_hessian(S.x.double())
+    Bm, W = ops.tet_rest(torch.from_numpy(soup.xcs).to(dev), T)
+    k_rows = torch.empty(16 * n_tet, dtype=torch.int32, device=dev)
+    k_cols = torch.empty_like(k_rows)
+    k_vals = torch.empty((16 * n_tet, 3, 3), dtype=torch.float32, device=dev)
+    ops.tet_hessian(S.x, T, Bm, W, mu, lam, k_rows, k_cols, k_vals, torch.zeros_like(S.x))
     m = torch.zeros(nv, dtype=torch.float64, device=dev)
-    m.index_add_(0, T.reshape(-1), (1000.0 * el.W / 4.0).repeat_interleave(4))
+    m.index_add_(0, T.reshape(-1).long(), (1000.0 * W.double() / 4.0).repeat_interleave(4))
     m[m == 0] = 1.0
-    Kb = (h * h * He).reshape(-1, 4, 3, 4, 3).permute(0, 1, 3, 2, 4).reshape(-1, 3, 3).float()
-    idx = torch.arange(nv, device=dev)
-    rows = torch.cat([T[:, :, None].expand(-1, 4, 4).reshape(-1), trip.rows.long(), idx]).int().contiguous()
-    cols = torch.cat([T[:, None, :].expand(-1, 4, 4).reshape(-1), trip.cols.long(), idx]).int().contiguous()
-    vals = torch.cat([Kb, h * h * trip.vals, m.float()[:, None, None] * torch.eye(3, device=dev)]).contiguous()
+    idx = torch.arange(nv, device=dev, dtype=torch.int32)
+    rows = torch.cat([k_rows, trip.rows, idx]).contiguous()
+    cols = torch.cat([k_cols, trip.cols, idx]).contiguous()
+    vals = torch.cat([h * h * k_vals, h * h * trip.vals, m.float()[:, None, None] * torch.eye(3, device=dev)]).contiguous()
     A = sparse.bsr_from_triplets(nv, nv, rows, cols, vals)
     nnz = A.nnz_sync()
     g = torch.Generator(device=dev).manual_seed(0)
