@@ -940,14 +940,25 @@ __device__ __forceinline__ void query_block(const QueryArgs& A, int bid, Smem& s
     const int grand_off = level_off[internal ? L - 2 : 0];
     // Self-queries skip children whose leaves all sit at positions <= the
     // query's own (one past the child's last leaf is (rel + 1) 8^(L-1)).
-    const int shift = L >= 1 ? 3 * (L - 1) : 0;
-    auto beyond = [&](int rel) { return !self || ((long long)(rel + 1) << shift) > (long long)slot + 1; };
+    // With at most 2^27 leaves (build_bvhs refuses more) a tree has at most
+    // 10 levels above them and (rel + 1) 8^(L-1) stays below 2^31, so the
+    // test runs in 32-bit unsigned arithmetic (a lane without a node has a
+    // meaningless rel; its hit is false through cpos anyway).
+    const unsigned shift = L >= 1 ? 3u * unsigned(L - 1) : 0u;
+    auto beyond = [&](int rel) { return !self || (unsigned(rel + 1) << shift) > unsigned(slot) + 1u; };
     // (bitwise, so that no load is sunk behind a branch on another term)
     bool hit[kPopNodes];
 #pragma unroll
     for (int p = 0; p < kPopNodes; ++p)
       hit[p] = (cpos[p] >= 0) & (at[p] - child_off < child_cnt) & overlaps(qlo, qhi, clo[p], chi[p]) &
-               !((qbody >= 0) & (body_min(clo[p]) == qbody) & (body_max(chi[p]) == qbody)) & beyond(at[p] - child_off);
+               beyond(at[p] - child_off);
+    // same-body subtrees (a branch on the launch argument: the test costs
+    // nothing where bodies are not excluded)
+    if (A.exclude_same_body) {
+#pragma unroll
+      for (int p = 0; p < kPopNodes; ++p)
+        hit[p] &= !((qbody >= 0) & (body_min(clo[p]) == qbody) & (body_max(chi[p]) == qbody));
+    }
     // internal level: one entry per node with overlapping children, the
     // first node's pushed last so that it is visited first
 #pragma unroll
