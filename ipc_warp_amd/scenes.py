@@ -420,6 +420,46 @@ def ccd_stream(kind: str, n: int, seed: int = 0, hit_fraction: float = 0.3, coor
     return x0, x1, np.arange(4 * n, dtype=np.int32).reshape(n, 4), t
 
 
+def borderline_pairs(offset: float, thickness: float, seed: int = 0, n_case: int = 256) -> Soup:
+    """Crossing edge pairs and points over triangles at distances within a few
+    ulps of 2 thickness (and a spread around it), placed at coordinates
+    ~offset: the cases where a DCD box inflated by less than thickness +
+    buffer could drop a pair the fp32 distance test accepts. n_case cases
+    of 8 vertices each: two edges crossing at distance d, and a triangle
+    with a point at distance d over its interior or beyond an edge; one case
+    in four is axis-aligned, so that the boxes' gap equals the distance."""
+    rng = np.random.default_rng(seed)
+    L = 4 * thickness
+    ulp = float(np.spacing(np.float32(max(abs(offset), 1.0))))
+    xs, edges, tris = [], [], []
+    for k in range(n_case):
+        base = np.array([offset + (k % 16) * 20 * L, offset + (k // 16) * 20 * L, offset], np.float64)
+        step = rng.integers(-24, 25) * ulp if k % 2 else rng.uniform(-0.05, 0.05) * 2 * thickness
+        d = 2 * thickness + step
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        a = rng.normal(size=3)
+        a -= a.dot(u) * u
+        a /= np.linalg.norm(a)
+        b = np.cross(u, a)
+        if k % 8 >= 6:  # axis-aligned: the boxes' gap along u is the distance
+            u, a, b = np.eye(3)[2], np.eye(3)[0], np.eye(3)[1]
+        v0 = len(xs)
+        # edge (a0, a1) and edge (b0, b1) crossing at distance d along u
+        xs += [base - L * a, base + L * a, base + d * u - L * b, base + d * u + L * b]
+        edges += [(v0, v0 + 1), (v0 + 2, v0 + 3)]
+        # triangle in the plane through base + 6L u normal to u, and a point at
+        # distance d from it: over its interior (k mod 4 < 2) or beyond an edge
+        c = base + 6 * L * u
+        t0, t1, t2 = c - L * a - L * b, c + L * a - L * b, c + L * b
+        p = c + d * u if k % 4 < 2 else c - L * b - (d / np.sqrt(2)) * (b - u)
+        xs += [t0, t1, t2, p]
+        tris.append((v0 + 4, v0 + 5, v0 + 6))
+    x = np.asarray(xs, np.float32)
+    return Soup(x=x, xcs=x.copy(), edges=np.asarray(edges, np.int32), triangles=np.asarray(tris, np.int32),
+                body=np.zeros(len(x), np.int32))
+
+
 # ------------------------------------------------------- rod bundle (twist) ----
 def hex_disk(rings: int, radius: float):
     """Triangulated regular hexagon of circumradius `radius`: the triangular

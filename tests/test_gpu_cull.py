@@ -51,43 +51,7 @@ def test_dcd_two_cubes_pressed(dev, oracle):
     assert n_ee > 100 and n_pt > 100
 
 
-def _borderline_soup(offset, seed):
-    """Crossing edge pairs and points over triangles at distances within a few
-    ulps of 2 TH (and a spread around it), placed at coordinates ~offset:
-    the cases where a DCD box inflated by less than thickness + buffer could
-    drop a pair the fp32 distance test accepts."""
-    rng = np.random.default_rng(seed)
-    L = 4 * TH
-    ulp = float(np.spacing(np.float32(max(abs(offset), 1.0))))
-    xs, edges, tris = [], [], []
-    n_case = 256
-    for k in range(n_case):
-        base = np.array([offset + (k % 16) * 20 * L, offset + (k // 16) * 20 * L, offset], np.float64)
-        step = rng.integers(-24, 25) * ulp if k % 2 else rng.uniform(-0.05, 0.05) * 2 * TH
-        d = 2 * TH + step
-        u = rng.normal(size=3)
-        u /= np.linalg.norm(u)
-        a = rng.normal(size=3)
-        a -= a.dot(u) * u
-        a /= np.linalg.norm(a)
-        b = np.cross(u, a)
-        v0 = len(xs)
-        # edge (a0, a1) and edge (b0, b1) crossing at distance d along u
-        xs += [base - L * a, base + L * a, base + d * u - L * b, base + d * u + L * b]
-        edges += [(v0, v0 + 1), (v0 + 2, v0 + 3)]
-        # triangle in the plane through base + 6L u normal to u, and a point at
-        # distance d from it: over its interior (even k) or beyond an edge
-        c = base + 6 * L * u
-        t0, t1, t2 = c - L * a - L * b, c + L * a - L * b, c + L * b
-        p = c + d * u if k % 4 < 2 else c - L * b - (d / np.sqrt(2)) * (b - u)
-        xs += [t0, t1, t2, p]
-        tris.append((v0 + 4, v0 + 5, v0 + 6))
-    x = np.asarray(xs, np.float32)
-    return scenes.Soup(x=x, xcs=x.copy(), edges=np.asarray(edges, np.int32), triangles=np.asarray(tris, np.int32),
-                       body=np.zeros(len(x), np.int32))
-
-
-@pytest.mark.parametrize("offset", [0.0, 37.0, 1000.0])
+@pytest.mark.parametrize("offset", [0.0, 30.0, 37.0, 1000.0])
 @pytest.mark.parametrize("buffer", [TH, 0.0, 1e-3 * TH])
 def test_dcd_borderline_distances(dev, oracle, offset, buffer):
     """DCD boxes are inflated by thickness + min(buffer, slack) instead of the
@@ -95,7 +59,7 @@ def test_dcd_borderline_distances(dev, oracle, offset, buffer):
     few ulps of 2 thickness, at coordinates up to 1000 and with buffers
     smaller than the slack, the sets equal the oracle's, which applies the
     reference's own box test."""
-    soup = _borderline_soup(offset, seed=int(offset) + 3)
+    soup = scenes.borderline_pairs(offset, TH, seed=int(offset) + 3)
     x = torch.from_numpy(soup.x).to(dev)
     S = ContactSolverBase(x, x.clone(), torch.from_numpy(soup.edges), torch.from_numpy(soup.triangles),
                           torch.from_numpy(soup.body), thickness=TH, buffer_=buffer, kappa=1e8)
