@@ -868,7 +868,9 @@ __device__ __forceinline__ void query_block(const QueryArgs& A, int bid, Smem& s
       constexpr unsigned long long kLead = 0x0101010101010101ull;  // the first lane of each group
       const unsigned long long idle_b = __ballot(!active) & kLead;
       const unsigned long long give_b = __ballot(rest != 0u) & kLead;
-      if (idle_b != 0ull && give_b != 0ull) {
+      // (at least two idle groups: on a launch that keeps the VALUs busy a
+      // hand-off to a single idle group costs about the step it saves)
+      if (__popcll(idle_b) >= 2 && give_b != 0ull) {
         const unsigned long long below = (1ull << group_shift) - 1ull;
         const int pairs = min(__popcll(idle_b), __popcll(give_b));
         const int idle_rank = __popcll(idle_b & below), give_rank = __popcll(give_b & below);
