@@ -953,14 +953,9 @@ __device__ __forceinline__ void query_block(const QueryArgs& A, int bid, Smem& s
 #pragma unroll
     for (int p = 0; p < kPopNodes; ++p)
       hit[p] = (cpos[p] >= 0) & (at[p] - child_off < child_cnt) & overlaps(qlo, qhi, clo[p], chi[p]) &
-               beyond(at[p] - child_off);
-    // same-body subtrees (a branch on the launch argument: the test costs
-    // nothing where bodies are not excluded)
-    if (A.exclude_same_body) {
-#pragma unroll
-      for (int p = 0; p < kPopNodes; ++p)
-        hit[p] &= !((qbody >= 0) & (body_min(clo[p]) == qbody) & (body_max(chi[p]) == qbody));
-    }
+               !((qbody >= 0) & (body_min(clo[p]) == qbody) & (body_max(chi[p]) == qbody)) & beyond(at[p] - child_off);
+    // (A branch on exclude_same_body around the body test made the screw and
+    // nut's launches, which exclude same-body pairs, 15 % slower.)
     // internal level: one entry per node with overlapping children, the
     // first node's pushed last so that it is visited first
 #pragma unroll
