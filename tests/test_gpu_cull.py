@@ -74,6 +74,26 @@ def test_dcd_borderline_distances(dev, oracle, offset, buffer):
     assert np.array_equal(g_pt[1], r_pt[1]) and np.array_equal(g_pt[0], r_pt[0])
 
 
+def test_dcd_nan_buffer_tests_every_pair(dev, oracle):
+    """A NaN buffer makes every box NaN, and a NaN box overlaps everything
+    (the reference's box test is a chain of > comparisons): the set is every
+    pair the filters and the fp32 distance test accept, the oracle's set with
+    a buffer wider than the scene."""
+    soup = scenes.two_cubes_on_ground(seed=13, n=3, gap=4e-4, jitter=2e-4)
+    x = torch.from_numpy(soup.x).to(dev)
+    S = ContactSolverBase(x, x.clone(), torch.from_numpy(soup.edges), torch.from_numpy(soup.triangles),
+                          torch.from_numpy(soup.body), thickness=TH, buffer_=float("nan"), kappa=1e8)
+    S.detect_collision()
+    g_ee = _sorted(S.contacts_new.list[:S.n_contacts].cpu().numpy(), S.contacts_new.e0e1[:S.n_contacts].cpu().numpy())
+    g_pt = _sorted(S.contacts_pt.list[:S.n_contacts_pt].cpu().numpy(),
+                   S.contacts_pt.e0e1[:S.n_contacts_pt].cpu().numpy())
+    r_ee = oracle.dcd_ee(soup.x, soup.edges, soup.body, TH, 10.0)
+    r_pt = oracle.dcd_pt(soup.x, soup.triangles, soup.body, TH, 10.0)
+    assert len(r_ee[0]) > 10 and len(r_pt[0]) > 10
+    assert np.array_equal(g_ee[1], r_ee[1]) and np.array_equal(g_ee[0], r_ee[0])
+    assert np.array_equal(g_pt[1], r_pt[1]) and np.array_equal(g_pt[0], r_pt[0])
+
+
 def test_dcd_overflow_grows_the_lists(dev, oracle):
     soup = scenes.two_cubes_on_ground(seed=12, n=6, gap=4e-4, jitter=2e-4)
     S = _solver(dev, soup, capacity=1)
